@@ -1,0 +1,207 @@
+#!/usr/bin/env python3
+"""Closest-hit ray-casting throughput of one MI355X on the 1.31M-triangle, 1920x1080 workload.
+
+Workload (named in the JSON `config`): seeded displaced icosphere of frequency 256 (1,310,720
+triangles, per-vertex normals and UVs) plus a 2-triangle ground plane at y=-0.501 (extent +-4),
+camera at (0,0,3) looking at the origin, 45 deg vertical FOV, 1920x1080, one primary ray per pixel.
+
+A step = one launch of the fused ray-generation + closest-hit kernel over the whole frame
+(2,073,600 rays; hit records t,u,v,prim written per pixel) with the scene resident in HBM.
+value = rays / step time in Mrays/s. Also reported: LBVH build time, and the full render
+(primary + shadow ray + Lambert + bilinear RGBA8 texture) latency p50/p99 over the same count of
+frames.
+
+roofline: algorithmic bytes per launch = sum over rays of (64 B per node visit + 48 B per triangle
+test) + 16 B of hit output per ray; the visit/test counts are those of the CPU restatement's
+traversal of the identical BVH in the identical order (oracle/rt_twin.c), divided by the kernel's
+mean duration from HIP events on the launch stream.
+cpu_baseline: the same closest-hit traversal in the CPU restatement (C + OpenMP, all host
+threads) over full frames for about `--cpu-seconds`.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "Mrays/s closest-hit (1 GPU) + p50/p99 render latency, batched vs plain"
+
+
+def parse():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--freq", type=int, default=256, help="icosphere frequency (256 = 1.31M tris)")
+    ap.add_argument("--width", type=int, default=1920)
+    ap.add_argument("--height", type=int, default=1080)
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-roofline", action="store_true")
+    return ap.parse_args()
+
+
+def percentile(a, q):
+    return float(np.percentile(np.asarray(a, dtype=np.float64), q))
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (--gpus {args.gpus} requested): the ray-casting path "
+              "is measured on a single MI355X; multi-GPU sharding is a design note, not a run.",
+              file=sys.stderr)
+        sys.exit(2)
+    import torch
+    import rendertoy_amd as ren
+    from rendertoy_amd import scenes
+
+    W, H = args.width, args.height
+    n_rays = W * H
+    t0 = time.time()
+    sphere = scenes.icosphere(args.freq, 2000)
+    plane = scenes.ground_plane()
+    tex = scenes.value_noise_rgba8(1024, 2001)
+    meshes = [sphere, plane]
+    models = [ren.mesh_from_arrays(m["pos"], m["idx"], m["nrm"], m["uv"]) for m in meshes]
+    gen_s = time.time() - t0
+
+    # ---- build (timed separately) ----
+    builds = []
+    rc = None
+    for _ in range(3):
+        torch.cuda.synchronize()
+        b0 = time.perf_counter()
+        rc = ren.Raycaster(models)
+        torch.cuda.synchronize()
+        builds.append((time.perf_counter() - b0) * 1e3)
+    n_tris = rc.n_tris
+    cam = ren.Camera((0.0, 0.0, 3.0), (0.0, 0.0, 0.0), (0.0, 1.0, 0.0), math.pi / 4, W / H)
+    dev = torch.device("cuda")
+    out = {"t": torch.empty(n_rays, dtype=torch.float32, device=dev),
+           "u": torch.empty(n_rays, dtype=torch.float32, device=dev),
+           "v": torch.empty(n_rays, dtype=torch.float32, device=dev),
+           "prim": torch.empty(n_rays, dtype=torch.int32, device=dev)}
+
+    # ---- closest-hit steps ----
+    for _ in range(args.warmup):
+        rc.cast_camera(cam, W, H, out=out)
+    stream = torch.cuda.current_stream()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps + 1)]
+    torch.cuda.synchronize()
+    wall0 = time.perf_counter()
+    ev[0].record(stream)
+    for k in range(args.steps):
+        rc.cast_camera(cam, W, H, out=out)
+        ev[k + 1].record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - wall0
+    step_ms = [ev[k].elapsed_time(ev[k + 1]) for k in range(args.steps)]
+    total_ms = ev[0].elapsed_time(ev[-1])
+    ms_per_step = total_ms / args.steps
+    mrays = n_rays / (ms_per_step * 1e-3) / 1e6
+    prim = out["prim"].cpu().numpy()
+    hits = int((prim >= 0).sum())
+
+    # ---- full render latency (primary + shadow + shade) ----
+    texg = torch.from_numpy(tex).to(dev)
+    light = (np.array([1, 1, 1], np.float32) / np.float32(np.sqrt(3.0))).astype(np.float32)
+    img = torch.empty((H, W, 3), dtype=torch.float32, device=dev)
+    stats = torch.zeros(4, dtype=torch.int32, device=dev)
+    for _ in range(args.warmup):
+        rc.render(cam, W, H, light=light, textures=[texg], mesh_textures=[0, 0], out=img, stats=stats)
+    rev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+           for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    for a, b in rev:
+        a.record(stream)
+        rc.render(cam, W, H, light=light, textures=[texg], mesh_textures=[0, 0], out=img, stats=stats)
+        b.record(stream)
+    torch.cuda.synchronize()
+    render_ms = [a.elapsed_time(b) for a, b in rev]
+    st = stats.cpu().numpy()
+    render_rays = n_rays + int(st[1])
+
+    result = {
+        "metric": METRIC,
+        "value": round(mrays, 2),
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded displaced icosphere + ground plane, value-noise texture)",
+        "config": {
+            "workload": f"closest-hit primary rays, displaced icosphere f={args.freq} + ground plane, "
+                        f"{W}x{H}, 1 ray/pixel, BVH built on device",
+            "tris": n_tris, "width": W, "height": H, "rays_per_step": n_rays,
+        },
+        "step_ms_p50": round(percentile(step_ms, 50), 4),
+        "step_ms_p99": round(percentile(step_ms, 99), 4),
+        "primary_hits": hits,
+        "build_ms_median": round(float(np.median(builds)), 3),
+        "render": {
+            "what": "primary closest hit + 1 shadow ray per lit hit + Lambert + bilinear RGBA8",
+            "latency_ms_p50": round(percentile(render_ms, 50), 4),
+            "latency_ms_p99": round(percentile(render_ms, 99), 4),
+            "rays_per_frame": render_rays,
+            "shadow_rays": int(st[1]),
+            "mrays_per_s_p50": round(render_rays / (percentile(render_ms, 50) * 1e-3) / 1e6, 2),
+        },
+        "wall_s_timed": round(wall, 4),
+    }
+
+    # ---- roofline and CPU baseline from the CPU restatement (oracle) ----
+    if not args.no_roofline or not args.no_cpu_baseline:
+        import oracle
+        tw_t0 = time.time()
+        pos, nrm, uv, mo = scenes.triangle_arrays(meshes)
+        tw = oracle.TwinScene(pos, nrm, uv, mo, [1, 1], [1, 1])
+        o, d = oracle.twin_generate_rays(cam.as_array(), W, H)
+        res = tw.trace_closest(o, d, counters=True)
+        if not args.no_roofline:
+            visits = res["node_visits"].astype(np.int64).sum()
+            tests = res["tri_tests"].astype(np.int64).sum()
+            algo_bytes = 64 * visits + 48 * tests + 16 * n_rays
+            achieved = algo_bytes / (ms_per_step * 1e-3) / 1e9
+            result["roofline"] = {
+                "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "algorithmic_bytes_per_launch": int(algo_bytes),
+                "mean_node_visits": round(float(visits / n_rays), 3),
+                "mean_tri_tests": round(float(tests / n_rays), 3),
+                "kernel": "k_trace_camera",
+            }
+            result["parity_primary_ids_equal_twin"] = bool(np.array_equal(prim, res["prim"]))
+        if not args.no_cpu_baseline:
+            threads = len(os.sched_getaffinity(0))
+            if os.environ.get("OMP_NUM_THREADS", "").isdigit():
+                threads = min(threads, int(os.environ["OMP_NUM_THREADS"]))
+            oracle.set_threads(threads)
+            reps, c0 = 0, time.perf_counter()
+            while True:
+                tw.trace_closest(o, d)
+                reps += 1
+                if time.perf_counter() - c0 >= args.cpu_seconds:
+                    break
+            cpu_s = time.perf_counter() - c0
+            result["cpu_baseline"] = {
+                "value": round(reps * n_rays / cpu_s / 1e6, 3), "unit": "Mrays/s",
+                "cores": threads, "kind": "port",
+                "sample": f"{reps} full {W}x{H} frames of primary closest hit, same BVH, "
+                          f"{cpu_s:.1f} s on {threads} host threads",
+            }
+        result["oracle_s"] = round(time.time() - tw_t0, 2)
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

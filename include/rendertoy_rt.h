@@ -1,0 +1,194 @@
+/*
+ * rendertoy_rt.h — C ABI of librtx_hip.so, the MI355X (gfx950) ray-casting engine behind
+ * the rendertoy raycaster API.
+ *
+ * The reference raycaster is a Python class whose device work was meant to run as pyopencl
+ * kernels dispatched through `kernel_main` (rendering/_core.py:247-299). Its two entry points are
+ * stubs: `Raycaster._build_ads` (rendering/_raycaster.py:30-33) and `Raycaster.ray_cast`
+ * (rendering/_raycaster.py:35-36). This header declares the native entry points that the Python
+ * mirror of that class (rendertoy_amd/_raycaster.py) binds with ctypes. A maintainer wiring the
+ * reference itself would bind the same symbols from rendering/_raycaster.py.
+ *
+ * Conventions
+ *  - Every pointer named `*_dev` or documented as "device" is a device address (hipMalloc'd
+ *    memory, e.g. torch.Tensor.data_ptr()). The library never allocates or frees device memory:
+ *    the caller passes every buffer, including workspace.
+ *  - Every call is asynchronous on the caller's stream (`stream` is a hipStream_t; NULL means
+ *    the default stream) unless documented otherwise.
+ *  - Return value: RT_OK (0) or a negative RT_E* code; rt_last_error() returns a thread-local
+ *    message for the last failure on the calling thread.
+ *  - Triangle identity: global primitive id = offset of the mesh (sum of the triangle counts of
+ *    the meshes before it) + triangle index within the mesh; the hit reports (mesh, index) as
+ *    rendering/_raycaster.py:16-22 (BVH_Triangle.index / .mesh) defines them.
+ */
+#ifndef RENDERTOY_RT_H
+#define RENDERTOY_RT_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RT_ABI_VERSION 1
+
+enum {
+    RT_OK = 0,
+    RT_EINVAL = -1,    /* bad argument (null pointer, size mismatch, misaligned buffer) */
+    RT_EHIP = -2,      /* a HIP runtime call failed (message holds hipGetErrorString) */
+    RT_ECAPACITY = -3, /* a size exceeds what the kernels support */
+    RT_ENOTSUP = -4    /* unsupported option */
+};
+
+/* Byte offset marking an absent per-vertex attribute in rt_mesh_desc. */
+#define RT_ABSENT 0xFFFFFFFFu
+
+/* One input mesh. Mirrors rendering/_modeling.py:31-34 (Mesh(vertices, indices)) over the
+ * 80-byte MeshVertex record of rendering/_modeling.py:22-28 (P@0, N@16, C@32, T@48, B@64). */
+typedef struct rt_mesh_desc {
+    const void* vertices;     /* device: n_vertices records of vertex_stride bytes */
+    const int32_t* indices;   /* device: 3*n_tris vertex indices, or NULL: vertices 3t..3t+2 */
+    uint32_t n_vertices;
+    uint32_t n_tris;          /* indices ? (index count)/3 : n_vertices/3 (rendering/_raycaster.py:33) */
+    uint32_t vertex_stride;   /* bytes between vertex records (80 for MeshVertex) */
+    uint32_t pos_offset;      /* byte offset of float3 P */
+    uint32_t nrm_offset;      /* byte offset of float3 N, or RT_ABSENT (geometric normal used) */
+    uint32_t uv_offset;       /* byte offset of float2 C (texture coordinate), or RT_ABSENT */
+} rt_mesh_desc;
+
+/* A built scene: device pointers owned by the caller, sizes from rt_scene_sizes(). */
+typedef struct rt_scene {
+    uint32_t n_tris;
+    uint32_t n_meshes;
+    int32_t root;             /* >= 0: internal node id; < 0: leaf ~slot; RT_EMPTY_ROOT: no triangles */
+    uint32_t _reserved;
+    void* nodes;              /* device, max(n_tris-1,1) * 64 bytes (4 x float4 per node) */
+    void* tris;               /* device, n_tris * 48 bytes (v0|prim, e1, e2 as float4) */
+    void* shade;              /* device, n_tris * 64 bytes (N0,N1,N2,UV0,UV1,UV2,mesh) */
+    int32_t* mesh_offsets;    /* device, n_meshes+1 prefix sums of triangle counts */
+    float* bounds;            /* device, 8 floats: scene AABB lo.xyz, 0, hi.xyz, 0 */
+} rt_scene;
+
+#define RT_EMPTY_ROOT ((int32_t)0x7FFFFFFF)
+
+/* Per-call byte sizes of the scene arrays and of the build workspace for n_tris triangles. */
+typedef struct rt_scene_sizes {
+    uint64_t nodes_bytes, tris_bytes, shade_bytes, bounds_bytes, workspace_bytes;
+} rt_scene_sizes;
+
+/* Rays as field pointers plus one byte stride shared by all fields: SoA arrays use stride 4,
+ * the AoS Ray record {float3 origin @0; float3 direction @16} uses stride 32. tmin/tmax may be
+ * NULL, then tmin_default/tmax_default apply to every ray. */
+typedef struct rt_rays {
+    const float *ox, *oy, *oz, *dx, *dy, *dz;
+    const float *tmin, *tmax;
+    uint32_t stride;
+    float tmin_default, tmax_default;
+} rt_rays;
+
+/* Closest-hit output fields (device), same stride rule as rt_rays. Misses store t=+inf,
+ * u=v=0, prim=index=mesh=-1. Any pointer may be NULL (field not written). */
+typedef struct rt_hits {
+    float *t, *u, *v;
+    int32_t *prim, *index, *mesh;
+    uint32_t stride;
+} rt_hits;
+
+/* Pinhole camera of rendering/_core.py:528-548 (look_at + perspective, vertical fov) reduced to
+ * the four vectors the ray generator uses: the primary direction of pixel (col,row) of a W x H
+ * image is normalize(nx*right + ny*up + forward) with nx = 2(col+.5)/W - 1, ny = 1 - 2(row+.5)/H,
+ * right = xaxis * aspect*tan(fov/2), up = yaxis * tan(fov/2), forward = zaxis. */
+typedef struct rt_camera {
+    float eye[3];
+    float right[3];
+    float up[3];
+    float forward[3];
+} rt_camera;
+
+/* RGBA8 texture, row-major rows of `width` texels, row index = v (rendering/_core.py:94-96). */
+typedef struct rt_texture {
+    const uint8_t* texels;    /* device, width*height*4 bytes */
+    uint32_t width, height;
+} rt_texture;
+
+enum {
+    RT_RENDER_SHADOWS = 1,    /* trace one shadow ray toward the light per lit hit */
+    RT_RENDER_BILINEAR = 2    /* bilinear texture filter (else point sampling as sample2D) */
+};
+
+/* Shading of rendering/_core.py:94-96 and tutorials/lesson09_texture_mapping.py:73,93-94:
+ * rgb = albedo(uv) * (ambient + V * max(0, N.L)); misses store `background`. */
+typedef struct rt_render_params {
+    uint32_t width, height;
+    uint32_t flags;
+    float light[3];           /* unit direction toward the light */
+    float ambient;            /* 0.2 in the reference shaders */
+    float background[3];
+    float shadow_tmin;        /* shadow-ray start offset along the light direction */
+    const rt_texture* textures_dev;  /* device array of rt_texture, or NULL */
+    const int32_t* mesh_texture_dev; /* device, per mesh: texture slot or -1 (albedo 1) */
+    uint32_t n_textures;
+} rt_render_params;
+
+/* Output of rt_render. rgb is width*height*3 float32 (row-major, row 0 = top). The hit fields
+ * are optional (NULL = not written); `stats` (optional, device, 4 uint32) receives
+ * {primary hits, shadow rays traced, shadow rays occluded, 0}. */
+typedef struct rt_render_out {
+    float* rgb;
+    float* t;
+    int32_t* prim;
+    uint32_t* stats;
+} rt_render_out;
+
+/* ---- library ---- */
+int rt_abi_version(void);
+const char* rt_last_error(void);
+
+/* ---- scene build: rendering/_raycaster.py:26-33 (Raycaster.__init__ / _build_ads) ---- */
+
+/* Fills `out` with the scene-array and workspace byte sizes for n_tris triangles. */
+int rt_scene_sizes_for(uint32_t n_tris, rt_scene_sizes* out);
+
+/* Builds the LBVH of all meshes' triangles on the device: per-triangle AABB and centroid,
+ * 30-bit Morton codes over the centroid bounds, stable LSD radix sort of (code, primitive),
+ * Karras hierarchy, bottom-up AABB refit. `scene` must hold n_tris/n_meshes and the device
+ * arrays sized by rt_scene_sizes_for(); on return scene->root is set. `keys_out`/`order_out`
+ * (device, n_tris uint32 each, may be NULL) receive the sorted Morton codes and primitive ids. */
+int rt_scene_build(const rt_mesh_desc* meshes, uint32_t n_meshes, rt_scene* scene,
+                   void* workspace_dev, uint64_t workspace_bytes,
+                   uint32_t* keys_out, uint32_t* order_out, void* stream);
+
+/* ---- queries: rendering/_raycaster.py:35-36 (Raycaster.ray_cast) ---- */
+
+/* Closest hit of n rays (two-sided Moller-Trumbore, hit interval (tmin, tmax), ties in t go to
+ * the smaller global primitive id). */
+int rt_trace_closest(const rt_scene* scene, const rt_rays* rays, uint32_t n,
+                     const rt_hits* hits, void* stream);
+
+/* Any hit (occlusion) of n rays in (tmin, tmax); occluded[i] = 1 if some triangle is hit. */
+int rt_trace_any(const rt_scene* scene, const rt_rays* rays, uint32_t n,
+                 uint8_t* occluded_dev, void* stream);
+
+/* Camera rays of a W x H image written as rays (SoA, stride 4), for inspection and tests. */
+int rt_generate_rays(const rt_camera* cam, uint32_t width, uint32_t height,
+                     float* ox, float* oy, float* oz, float* dx, float* dy, float* dz,
+                     void* stream);
+
+/* Closest hit of the primary rays of a W x H image (ray generation fused into the traversal);
+ * hit record of pixel (col,row) at index row*W + col. */
+int rt_trace_camera(const rt_scene* scene, const rt_camera* cam, uint32_t width, uint32_t height,
+                    const rt_hits* hits, void* stream);
+
+/* Full render: fused ray generation + closest-hit trace, then the fused Lambert + shadow ray +
+ * texture shading kernel over the compacted list of hit pixels. `workspace_dev` needs
+ * rt_render_workspace_bytes(width*height) bytes. */
+uint64_t rt_render_workspace_bytes(uint32_t n_pixels);
+int rt_render(const rt_scene* scene, const rt_camera* cam, const rt_render_params* params,
+              const rt_render_out* out, void* workspace_dev, uint64_t workspace_bytes,
+              void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RENDERTOY_RT_H */

@@ -1,0 +1,271 @@
+"""ORACLE / TEST INFRASTRUCTURE ONLY — never imported by the product (rendertoy_amd).
+
+Python face of the CPU restatement in rt_twin.c (loaded through ctypes from oracle/librt_twin.so,
+built by oracle/build.py) plus small numpy restatements used to pin it:
+
+  morton_np          30-bit Morton codes, vectorised bit interleave (closed form)
+  camera_dirs_fp64   primary-ray directions in FP64 from the look_at/perspective definitions of
+                     rendering/_core.py:528-548 and the viewport of rendering/_raster.py:126-129
+  bilinear_fp64      OpenCL 1.2 linear filter with repeat addressing, FP64
+
+The reference ray caster is a stub (rendering/_raycaster.py:30-36) with no tests or golden data;
+the pins are the closed forms, invariants and brute-force checks in tests/test_oracle_*.py.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "librt_twin.so")
+EMPTY_ROOT = 0x7FFFFFFF
+
+_lib = None
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            from . import build as _b
+            _b.build_oracle()
+        L = ctypes.CDLL(LIB_PATH)
+        vp = ctypes.c_void_p
+        i64 = ctypes.c_int64
+        f32 = ctypes.c_float
+        f64 = ctypes.c_double
+        i32 = ctypes.c_int32
+        L.tw_expand10.restype = ctypes.c_uint32
+        L.tw_expand10.argtypes = [ctypes.c_uint32]
+        L.tw_morton3.restype = ctypes.c_uint32
+        L.tw_morton3.argtypes = [ctypes.c_uint32] * 3
+        L.tw_build.restype = ctypes.c_int
+        L.tw_build.argtypes = [i64] + [vp] * 6 + [vp] * 6 + [vp]
+        L.tw_trace_closest.restype = None
+        L.tw_trace_closest.argtypes = [vp, vp, i32, i64] + [vp] * 8 + [f32, f32] + [vp] * 6
+        L.tw_trace_any.restype = None
+        L.tw_trace_any.argtypes = [vp, vp, i32, i64] + [vp] * 8 + [f32, f32, vp]
+        L.tw_brute_closest32.restype = None
+        L.tw_brute_closest32.argtypes = [vp, i64, i64] + [vp] * 6 + [f32, f32, vp, vp]
+        L.tw_generate_rays.restype = None
+        L.tw_generate_rays.argtypes = [vp, ctypes.c_uint32, ctypes.c_uint32] + [vp] * 6
+        L.tw_render.restype = None
+        L.tw_render.argtypes = [vp, vp, vp, i32, vp, ctypes.c_uint32, ctypes.c_uint32, vp,
+                                ctypes.c_uint32] + [vp] * 9
+        L.tw_closest64_brute.restype = None
+        L.tw_closest64_brute.argtypes = [vp, i64, i64] + [vp] * 6 + [f64, f64] + [vp] * 5
+        L.tw_closest64_bvh.restype = None
+        L.tw_closest64_bvh.argtypes = [vp, i64, vp, vp, i32, i64] + [vp] * 6 + [f64, f64, f64] + [vp] * 5
+        L.tw_num_threads.restype = ctypes.c_int
+        L.tw_set_threads.argtypes = [ctypes.c_int]
+        _lib = L
+    return _lib
+
+
+def set_threads(n):
+    lib().tw_set_threads(int(n))
+
+
+def num_threads():
+    return lib().tw_num_threads()
+
+
+# ------------------------------------------------------------------------------------------
+# numpy restatements (closed forms)
+# ------------------------------------------------------------------------------------------
+
+def expand10_np(v):
+    """Spreads the low 10 bits of v to every third bit (bit k -> bit 3k), by definition."""
+    v = np.asarray(v, dtype=np.uint64) & 0x3FF
+    out = np.zeros_like(v)
+    for k in range(10):
+        out |= ((v >> np.uint64(k)) & np.uint64(1)) << np.uint64(3 * k)
+    return out.astype(np.uint32)
+
+
+def morton_np(x, y, z):
+    return ((expand10_np(x).astype(np.uint64) << 2) | (expand10_np(y).astype(np.uint64) << 1)
+            | expand10_np(z).astype(np.uint64)).astype(np.uint32)
+
+
+def camera_dirs_fp64(eye, target, up, fov, W, H):
+    """Primary directions [H*W, 3] in FP64: view-space (ndc.x/ws, ndc.y/hs, 1) with
+    hs = 1/tan(fov/2), ws = hs/aspect, rotated by the look_at basis (rendering/_core.py:528-548),
+    ndc from pixel centres with the viewport y flip (rendering/_raster.py:126-129, :298-299)."""
+    eye, target, up = (np.asarray(a, dtype=np.float64) for a in (eye, target, up))
+    z = target - eye
+    z /= np.linalg.norm(z)
+    x = np.cross(up, z)
+    x /= np.linalg.norm(x)
+    y = np.cross(z, x)
+    hs = 1.0 / np.tan(fov / 2)
+    ws = hs / (W / H)
+    col, row = np.meshgrid(np.arange(W), np.arange(H))
+    ndx = (col.ravel() + 0.5) / W * 2 - 1
+    ndy = 1 - (row.ravel() + 0.5) / H * 2
+    d = (ndx / ws)[:, None] * x + (ndy / hs)[:, None] * y + z
+    return d / np.linalg.norm(d, axis=1, keepdims=True)
+
+
+def bilinear_fp64(tex, u, v):
+    """OpenCL 1.2 sec. 8.2 linear filtering with CLK_ADDRESS_REPEAT on an RGBA8 image
+    [H, W, 4] (normalised coordinates, row = v); returns RGB in [0, 1], FP64."""
+    H, W = tex.shape[:2]
+    u = np.asarray(u, np.float64)
+    v = np.asarray(v, np.float64)
+    x = (u - np.floor(u)) * W - 0.5
+    y = (v - np.floor(v)) * H - 0.5
+    i0 = np.floor(x).astype(np.int64)
+    j0 = np.floor(y).astype(np.int64)
+    a = x - i0
+    b = y - j0
+    i1 = (i0 + 1) % W
+    j1 = (j0 + 1) % H
+    i0 %= W
+    j0 %= H
+    t = tex[..., :3].astype(np.float64) / 255.0
+    return (((1 - a) * (1 - b))[..., None] * t[j0, i0] + (a * (1 - b))[..., None] * t[j0, i1]
+            + ((1 - a) * b)[..., None] * t[j1, i0] + (a * b)[..., None] * t[j1, i1])
+
+
+# ------------------------------------------------------------------------------------------
+# twin (FP32 restatement of the device algorithm)
+# ------------------------------------------------------------------------------------------
+
+class TwinScene:
+    """The CPU twin's build of a scene from per-primitive arrays (scenes.triangle_arrays)."""
+
+    def __init__(self, pos, nrm, uv, mesh_of, has_nrm, has_uv):
+        n = pos.shape[0]
+        self.n = n
+        self.pos = np.ascontiguousarray(pos, np.float32)
+        self.keys = np.zeros(max(n, 1), np.uint32)
+        self.order = np.zeros(max(n, 1), np.uint32)
+        self.nodes = np.zeros((max(n - 1, 1), 16), np.float32)
+        self.tris = np.zeros((max(n, 1), 12), np.float32)
+        self.shade = np.zeros((max(n, 1), 16), np.float32)
+        self.bounds = np.zeros(8, np.float32)
+        root = ctypes.c_int32(0)
+        hn = np.ascontiguousarray(has_nrm, np.uint8)
+        hu = np.ascontiguousarray(has_uv, np.uint8)
+        rc = lib().tw_build(n, _p(self.pos), _p(np.ascontiguousarray(nrm, np.float32)),
+                            _p(np.ascontiguousarray(uv, np.float32)),
+                            _p(np.ascontiguousarray(mesh_of, np.int32)), _p(hn), _p(hu),
+                            _p(self.keys), _p(self.order), _p(self.nodes), _p(self.tris),
+                            _p(self.shade), _p(self.bounds), ctypes.byref(root))
+        if rc != 0:
+            raise MemoryError("tw_build failed")
+        self.root = root.value
+        self.keys, self.order = self.keys[:n], self.order[:n]
+        self.tris, self.shade = self.tris[:n], self.shade[:n]
+        self.nodes = self.nodes[:max(n - 1, 0)]
+
+    @property
+    def nodes_full(self):
+        return self.nodes if self.nodes.shape[0] else np.zeros((1, 16), np.float32)
+
+    def trace_closest(self, o, d, tmin=0.0, tmax=np.inf, counters=False):
+        o = np.ascontiguousarray(o, np.float32)
+        d = np.ascontiguousarray(d, np.float32)
+        n = o.shape[0]
+        cols = [np.ascontiguousarray(o[:, k]) for k in range(3)] + \
+               [np.ascontiguousarray(d[:, k]) for k in range(3)]
+        t = np.empty(n, np.float32)
+        u = np.empty(n, np.float32)
+        v = np.empty(n, np.float32)
+        prim = np.empty(n, np.int32)
+        slot = np.empty(n, np.int32)
+        cnt = np.empty((n, 2), np.int32) if counters else None
+        lib().tw_trace_closest(_p(self.nodes_full), _p(self.tris if self.n else np.zeros((1, 12), np.float32)),
+                               self.root, n, *[_p(c) for c in cols], None, None, tmin, tmax,
+                               _p(t), _p(u), _p(v), _p(prim), _p(slot), _p(cnt))
+        out = {"t": t, "u": u, "v": v, "prim": prim, "slot": slot}
+        if counters:
+            out["node_visits"] = cnt[:, 0]
+            out["tri_tests"] = cnt[:, 1]
+        return out
+
+    def trace_any(self, o, d, tmin=0.0, tmax=np.inf):
+        o = np.ascontiguousarray(o, np.float32)
+        d = np.ascontiguousarray(d, np.float32)
+        n = o.shape[0]
+        cols = [np.ascontiguousarray(o[:, k]) for k in range(3)] + \
+               [np.ascontiguousarray(d[:, k]) for k in range(3)]
+        occ = np.empty(n, np.uint8)
+        lib().tw_trace_any(_p(self.nodes_full), _p(self.tris if self.n else np.zeros((1, 12), np.float32)),
+                           self.root, n, *[_p(c) for c in cols], None, None, tmin, tmax, _p(occ))
+        return occ
+
+    def brute_closest32(self, o, d, tmin=0.0, tmax=np.inf):
+        o = np.ascontiguousarray(o, np.float32)
+        d = np.ascontiguousarray(d, np.float32)
+        n = o.shape[0]
+        cols = [np.ascontiguousarray(o[:, k]) for k in range(3)] + \
+               [np.ascontiguousarray(d[:, k]) for k in range(3)]
+        t = np.empty(n, np.float32)
+        prim = np.empty(n, np.int32)
+        lib().tw_brute_closest32(_p(self.tris), self.n, n, *[_p(c) for c in cols], tmin, tmax,
+                                 _p(t), _p(prim))
+        return t, prim
+
+    def render(self, cam12, W, H, light, ambient=0.2, background=(0, 0, 0), shadow_tmin=1e-4,
+               shadows=True, bilinear=True, textures=(), mesh_tex=None):
+        params = np.array([*light, ambient, *background, shadow_tmin], np.float32)
+        flags = (1 if shadows else 0) | (2 if bilinear else 0)
+        rgb = np.empty((H, W, 3), np.float32)
+        t = np.empty(H * W, np.float32)
+        prim = np.empty(H * W, np.int32)
+        stats = np.zeros(3, np.int64)
+        if textures:
+            blobs = [np.ascontiguousarray(x, np.uint8).reshape(-1) for x in textures]
+            offs = np.cumsum([0] + [b.size for b in blobs[:-1]]).astype(np.int64)
+            texcat = np.concatenate(blobs)
+            tw = np.array([x.shape[1] for x in textures], np.int32)
+            th = np.array([x.shape[0] for x in textures], np.int32)
+        else:
+            texcat, offs, tw, th = np.zeros(4, np.uint8), np.zeros(1, np.int64), np.ones(1, np.int32), np.ones(1, np.int32)
+        mt = None if mesh_tex is None else np.ascontiguousarray(mesh_tex, np.int32)
+        lib().tw_render(_p(self.nodes_full), _p(self.tris), _p(self.shade), self.root,
+                        _p(np.ascontiguousarray(cam12, np.float32)), W, H, _p(params), flags,
+                        _p(texcat), _p(offs), _p(tw), _p(th), _p(mt), _p(rgb), _p(t), _p(prim),
+                        _p(stats))
+        return rgb, t, prim, stats
+
+    def closest64(self, o, d, tmin=0.0, edge_eps=1e-6, margin=1e-5, brute=False):
+        return closest64(self.pos, o, d, tmin, edge_eps, margin, None if brute else self)
+
+
+def twin_generate_rays(cam12, W, H):
+    n = W * H
+    cols = [np.empty(n, np.float32) for _ in range(6)]
+    lib().tw_generate_rays(_p(np.ascontiguousarray(cam12, np.float32)), W, H, *[_p(c) for c in cols])
+    return np.stack(cols[:3], 1), np.stack(cols[3:], 1)
+
+
+def closest64(tri_pos, o, d, tmin=0.0, edge_eps=1e-6, margin=1e-5, scene=None):
+    """FP64 closest hit (brute force, or culled by `scene`'s hierarchy with FP64 boxes inflated
+    by `margin`): dict t (float64, inf on miss), prim (-1 on miss), u, v, edge (uint8)."""
+    tri_pos = np.ascontiguousarray(tri_pos, np.float32)
+    o = np.ascontiguousarray(o, np.float32)
+    d = np.ascontiguousarray(d, np.float32)
+    n = o.shape[0]
+    cols = [np.ascontiguousarray(o[:, k]) for k in range(3)] + \
+           [np.ascontiguousarray(d[:, k]) for k in range(3)]
+    t = np.empty(n, np.float64)
+    prim = np.empty(n, np.int32)
+    u = np.empty(n, np.float64)
+    v = np.empty(n, np.float64)
+    edge = np.empty(n, np.uint8)
+    if scene is None:
+        lib().tw_closest64_brute(_p(tri_pos), tri_pos.shape[0], n, *[_p(c) for c in cols], tmin,
+                                 edge_eps, _p(t), _p(prim), _p(u), _p(v), _p(edge))
+    else:
+        order = np.ascontiguousarray(scene.order if scene.n else np.zeros(1, np.uint32), np.uint32)
+        lib().tw_closest64_bvh(_p(tri_pos), tri_pos.shape[0], _p(scene.nodes_full), _p(order),
+                               scene.root, n, *[_p(c) for c in cols], tmin, edge_eps, margin,
+                               _p(t), _p(prim), _p(u), _p(v), _p(edge))
+    return {"t": t, "prim": prim, "u": u, "v": v, "edge": edge}

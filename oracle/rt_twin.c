@@ -1,0 +1,763 @@
+/*
+ * rt_twin.c — ORACLE / TEST INFRASTRUCTURE ONLY.
+ *
+ * A CPU (C11 + OpenMP) restatement of the ray-casting path that librtx_hip.so implements on the
+ * GPU, plus an independent FP64 closest-hit oracle. Only tests/, __graft_entry__.smoke() and the
+ * CPU-baseline leg of bench.py load this library; the product never does.
+ *
+ * The reference raycaster is a stub (rendering/_raycaster.py:30-36: `_build_ads` counts triangles,
+ * `ray_cast` is `pass`), so there is no reference algorithm to transcribe. What this file follows
+ * from the reference is its API and conventions:
+ *   - triangle identity (mesh, index) and per-mesh triangle count: rendering/_raycaster.py:16-22,33
+ *   - camera: look_at / perspective / viewport / pixel centre: rendering/_core.py:528-548,
+ *     rendering/_raster.py:126-129 and :298-299
+ *   - barycentric order v0*(1-u-v) + v1*u + v2*v: rendering/_raster.py:46
+ *   - Lambert with L = normalize(1,1,1), ambient 0.2: tutorials/lesson09_texture_mapping.py:73
+ *   - repeat texture addressing, row = v: rendering/_core.py:94-96
+ *
+ * Two layers:
+ *  (1) "twin": the FP32 algorithm of the device code, operation for operation (explicit fmaf,
+ *      no contraction: build with -ffp-contract=off), so device output can be compared
+ *      bit-for-bit: LBVH build (Morton codes, stable sort, Karras hierarchy, refit), ray
+ *      generation, closest/any-hit traversal with two-sided Moller-Trumbore, shading.
+ *  (2) "fp64": a brute-force / BVH-culled FP64 closest hit that shares no arithmetic with (1);
+ *      it classifies rays whose winning barycentric lies within `edge_eps` of an edge or whose
+ *      runner-up is within `edge_eps` relative distance ("edge rays", reported separately).
+ *
+ * Pinning (tests/test_oracle_*.py): Morton vs closed-form KATs; sort vs numpy stable argsort;
+ * LBVH invariants (2n-1 nodes, each leaf once, boxes are exact unions) and brute-force equality;
+ * FP32 traversal vs FP32 brute force; FP64 closest hit vs closed-form geometry (sphere/plane
+ * analytic intersections, axis-aligned triangles) and vs the FP32 twin within 1e-5 relative t.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#include <float.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define TW_API __attribute__((visibility("default")))
+
+/* ------------------------------------------------------------------------------------------ */
+/* Morton code (30 bits, x in the most significant interleaved position)                       */
+/* ------------------------------------------------------------------------------------------ */
+
+TW_API uint32_t tw_expand10(uint32_t v) {
+    v &= 0x3FFu;
+    v = (v | (v << 16)) & 0x030000FFu;
+    v = (v | (v << 8)) & 0x0300F00Fu;
+    v = (v | (v << 4)) & 0x030C30C3u;
+    v = (v | (v << 2)) & 0x09249249u;
+    return v;
+}
+
+TW_API uint32_t tw_morton3(uint32_t x, uint32_t y, uint32_t z) {
+    return (tw_expand10(x) << 2) | (tw_expand10(y) << 1) | tw_expand10(z);
+}
+
+static inline uint32_t quantize10(float c, float lo, float scale) {
+    float q = (c - lo) * scale;
+    q = fminf(fmaxf(q, 0.0f), 1023.0f);
+    return (uint32_t)q;
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* Per-triangle preparation (the device gather kernel)                                          */
+/* ------------------------------------------------------------------------------------------ */
+
+/* Pads a box by |coordinate| * 2^-18 + 2^-40 per side so that the box test can never cull a
+ * triangle that the FP32 Moller-Trumbore test accepts. */
+static inline float pad_of(float lo, float hi) {
+    float m = fmaxf(fabsf(lo), fabsf(hi));
+    return m * 0x1p-18f + 0x1p-40f;
+}
+
+/* Inputs are flattened per global primitive (mesh-major order):
+ *   pos[p*9 + k*3 + c]  vertex k of triangle p (fp32)
+ *   nrm[p*9 + ...]      per-vertex normals (ignored where has_nrm[mesh]==0)
+ *   uv [p*6 + k*2 + c]  per-vertex texture coordinates (0 where has_uv[mesh]==0)
+ *   mesh_of[p]          mesh id
+ * Outputs (leaf/slot order after the sort):
+ *   keys[n], order[n]   sorted Morton codes and primitive ids
+ *   nodes[(n-1)*16]     4 float4 per internal node (layout below)
+ *   tris[n*12]          (v0.xyz, prim bits), (e1.xyz, 0), (e2.xyz, 0)
+ *   shade[n*16]         (N0.xyz, N1.x), (N1.yz, N2.xy), (N2.z, uv0, uv1.x), (uv1.y, uv2, mesh bits)
+ *   bounds[8]           scene AABB lo.xyz,0,hi.xyz,0
+ * Node layout: [0]=(b0.lo.x,b0.hi.x,b0.lo.y,b0.hi.y) [1]=(b1.lo.x,b1.hi.x,b1.lo.y,b1.hi.y)
+ *              [2]=(b0.lo.z,b0.hi.z,b1.lo.z,b1.hi.z) [3]=(ref0,ref1,0,0) int bits; ref>=0 internal,
+ *              ref<0 leaf slot ~ref.
+ */
+
+typedef struct { float lo[3], hi[3]; } box3;
+
+static inline float canon(float x) { return x + 0.0f; } /* -0 -> +0 */
+
+static inline uint32_t fbits(float f) { uint32_t u; memcpy(&u, &f, 4); return u; }
+static inline float bitsf(uint32_t u) { float f; memcpy(&f, &u, 4); return f; }
+
+static int delta_k(const uint32_t* k, int64_t n, int64_t i, int64_t j) {
+    if (j < 0 || j >= n) return -1;
+    uint32_t a = k[i], b = k[j];
+    if (a != b) return __builtin_clz(a ^ b);
+    return 32 + __builtin_clz((uint32_t)i ^ (uint32_t)j);
+}
+
+static int cmp_u64(const void* a, const void* b) {
+    uint64_t x = *(const uint64_t*)a, y = *(const uint64_t*)b;
+    return (x > y) - (x < y);
+}
+
+TW_API int tw_build(int64_t n, const float* pos, const float* nrm, const float* uv,
+                    const int32_t* mesh_of, const uint8_t* has_nrm, const uint8_t* has_uv,
+                    uint32_t* keys, uint32_t* order, float* nodes, float* tris, float* shade,
+                    float* bounds, int32_t* root_out) {
+    if (n <= 0) {
+        for (int c = 0; c < 3; ++c) { bounds[c] = INFINITY; bounds[4 + c] = -INFINITY; }
+        bounds[3] = bounds[7] = 0.0f;
+        *root_out = 0x7FFFFFFF;
+        return 0;
+    }
+    box3* leafbox = (box3*)malloc(sizeof(box3) * n);
+    float* cen = (float*)malloc(sizeof(float) * 3 * n);
+    float* tri_u = (float*)malloc(sizeof(float) * 12 * n);
+    float* sh_u = (float*)malloc(sizeof(float) * 16 * n);
+    if (!leafbox || !cen || !tri_u || !sh_u) return -1;
+
+    /* gather */
+    #pragma omp parallel for schedule(static)
+    for (int64_t p = 0; p < n; ++p) {
+        float v[3][3];
+        for (int k = 0; k < 3; ++k)
+            for (int c = 0; c < 3; ++c) v[k][c] = canon(pos[p * 9 + k * 3 + c]);
+        box3 b;
+        for (int c = 0; c < 3; ++c) {
+            float lo = fminf(fminf(v[0][c], v[1][c]), v[2][c]);
+            float hi = fmaxf(fmaxf(v[0][c], v[1][c]), v[2][c]);
+            cen[p * 3 + c] = (lo + hi) * 0.5f;
+            float pd = pad_of(lo, hi);
+            b.lo[c] = lo - pd;
+            b.hi[c] = hi + pd;
+        }
+        leafbox[p] = b;
+        float e1[3], e2[3];
+        for (int c = 0; c < 3; ++c) { e1[c] = v[1][c] - v[0][c]; e2[c] = v[2][c] - v[0][c]; }
+        float* t = tri_u + p * 12;
+        t[0] = v[0][0]; t[1] = v[0][1]; t[2] = v[0][2]; t[3] = bitsf((uint32_t)p);
+        t[4] = e1[0]; t[5] = e1[1]; t[6] = e1[2]; t[7] = 0.0f;
+        t[8] = e2[0]; t[9] = e2[1]; t[10] = e2[2]; t[11] = 0.0f;
+        int m = mesh_of[p];
+        float N[3][3], UV[3][2];
+        if (has_nrm[m]) {
+            for (int k = 0; k < 3; ++k)
+                for (int c = 0; c < 3; ++c) N[k][c] = nrm[p * 9 + k * 3 + c];
+        } else {
+            /* geometric normal normalize(e1 x e2) */
+            float gx = fmaf(e1[1], e2[2], -(e1[2] * e2[1]));
+            float gy = fmaf(e1[2], e2[0], -(e1[0] * e2[2]));
+            float gz = fmaf(e1[0], e2[1], -(e1[1] * e2[0]));
+            float l2 = fmaf(gx, gx, fmaf(gy, gy, gz * gz));
+            float il = l2 > 0.0f ? 1.0f / sqrtf(l2) : 0.0f;
+            for (int k = 0; k < 3; ++k) { N[k][0] = gx * il; N[k][1] = gy * il; N[k][2] = gz * il; }
+        }
+        for (int k = 0; k < 3; ++k)
+            for (int c = 0; c < 2; ++c) UV[k][c] = has_uv[m] ? uv[p * 6 + k * 2 + c] : 0.0f;
+        float* s = sh_u + p * 16;
+        s[0] = N[0][0]; s[1] = N[0][1]; s[2] = N[0][2]; s[3] = N[1][0];
+        s[4] = N[1][1]; s[5] = N[1][2]; s[6] = N[2][0]; s[7] = N[2][1];
+        s[8] = N[2][2]; s[9] = UV[0][0]; s[10] = UV[0][1]; s[11] = UV[1][0];
+        s[12] = UV[1][1]; s[13] = UV[2][0]; s[14] = UV[2][1]; s[15] = bitsf((uint32_t)m);
+    }
+
+    /* centroid bounds and Morton codes */
+    float clo[3] = {INFINITY, INFINITY, INFINITY}, chi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int64_t p = 0; p < n; ++p)
+        for (int c = 0; c < 3; ++c) {
+            clo[c] = fminf(clo[c], cen[p * 3 + c]);
+            chi[c] = fmaxf(chi[c], cen[p * 3 + c]);
+        }
+    float sc[3];
+    for (int c = 0; c < 3; ++c) {
+        float ext = chi[c] - clo[c];
+        sc[c] = ext > 0.0f ? 1024.0f / ext : 0.0f;
+    }
+    uint64_t* kv = (uint64_t*)malloc(sizeof(uint64_t) * n);
+    if (!kv) return -1;
+    #pragma omp parallel for schedule(static)
+    for (int64_t p = 0; p < n; ++p) {
+        uint32_t qx = quantize10(cen[p * 3 + 0], clo[0], sc[0]);
+        uint32_t qy = quantize10(cen[p * 3 + 1], clo[1], sc[1]);
+        uint32_t qz = quantize10(cen[p * 3 + 2], clo[2], sc[2]);
+        kv[p] = ((uint64_t)tw_morton3(qx, qy, qz) << 32) | (uint64_t)p;
+    }
+    /* stable sort by key == sort by (key, primitive id) */
+    qsort(kv, (size_t)n, sizeof(uint64_t), cmp_u64);
+    for (int64_t i = 0; i < n; ++i) { keys[i] = (uint32_t)(kv[i] >> 32); order[i] = (uint32_t)kv[i]; }
+    free(kv);
+
+    /* leaf records in slot order */
+    #pragma omp parallel for schedule(static)
+    for (int64_t s = 0; s < n; ++s) {
+        uint32_t p = order[s];
+        memcpy(tris + s * 12, tri_u + (int64_t)p * 12, 48);
+        memcpy(shade + s * 16, sh_u + (int64_t)p * 16, 64);
+    }
+
+    if (n == 1) {
+        box3 b = leafbox[order[0]];
+        for (int c = 0; c < 3; ++c) { bounds[c] = b.lo[c]; bounds[4 + c] = b.hi[c]; }
+        bounds[3] = bounds[7] = 0.0f;
+        *root_out = ~0;
+        free(leafbox); free(cen); free(tri_u); free(sh_u);
+        return 0;
+    }
+
+    /* Karras 2012 binary radix tree over the 62-bit extended keys (code, slot) */
+    int32_t* left = (int32_t*)malloc(sizeof(int32_t) * (n - 1));
+    int32_t* right = (int32_t*)malloc(sizeof(int32_t) * (n - 1));
+    if (!left || !right) return -1;
+    #pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < n - 1; ++i) {
+        int d = (delta_k(keys, n, i, i + 1) - delta_k(keys, n, i, i - 1)) >= 0 ? 1 : -1;
+        int dmin = delta_k(keys, n, i, i - d);
+        int64_t lmax = 2;
+        while (delta_k(keys, n, i, i + lmax * d) > dmin) lmax *= 2;
+        int64_t l = 0;
+        for (int64_t t = lmax / 2; t >= 1; t /= 2)
+            if (delta_k(keys, n, i, i + (l + t) * d) > dmin) l += t;
+        int64_t j = i + l * d;
+        int dnode = delta_k(keys, n, i, j);
+        int64_t s = 0;
+        int64_t t = l;
+        for (;;) {
+            t = (t + 1) / 2;
+            if (delta_k(keys, n, i, i + (s + t) * d) > dnode) s += t;
+            if (t <= 1) break;
+        }
+        int64_t gamma = i + s * d + (d < 0 ? -1 : 0);
+        int64_t lo_ = i < j ? i : j, hi_ = i < j ? j : i;
+        left[i] = (lo_ == gamma) ? ~(int32_t)gamma : (int32_t)gamma;
+        right[i] = (hi_ == gamma + 1) ? ~(int32_t)(gamma + 1) : (int32_t)(gamma + 1);
+    }
+
+    /* refit: post-order over the tree, child boxes stored in the parent */
+    box3* nb = (box3*)malloc(sizeof(box3) * (n - 1));
+    int32_t* stack = (int32_t*)malloc(sizeof(int32_t) * 2 * (n + 1));
+    uint8_t* state = (uint8_t*)calloc((size_t)(n - 1), 1);
+    if (!nb || !stack || !state) return -1;
+    int64_t sp = 0;
+    stack[sp++] = 0;
+    while (sp) {
+        int32_t x = stack[sp - 1];
+        if (state[x] == 0) {
+            state[x] = 1;
+            if (right[x] >= 0) stack[sp++] = right[x];
+            if (left[x] >= 0) stack[sp++] = left[x];
+            continue;
+        }
+        --sp;
+        box3 b0 = left[x] >= 0 ? nb[left[x]] : leafbox[order[~left[x]]];
+        box3 b1 = right[x] >= 0 ? nb[right[x]] : leafbox[order[~right[x]]];
+        float* nd = nodes + (int64_t)x * 16;
+        nd[0] = b0.lo[0]; nd[1] = b0.hi[0]; nd[2] = b0.lo[1]; nd[3] = b0.hi[1];
+        nd[4] = b1.lo[0]; nd[5] = b1.hi[0]; nd[6] = b1.lo[1]; nd[7] = b1.hi[1];
+        nd[8] = b0.lo[2]; nd[9] = b0.hi[2]; nd[10] = b1.lo[2]; nd[11] = b1.hi[2];
+        nd[12] = bitsf((uint32_t)left[x]); nd[13] = bitsf((uint32_t)right[x]);
+        nd[14] = 0.0f; nd[15] = 0.0f;
+        box3 u;
+        for (int c = 0; c < 3; ++c) {
+            u.lo[c] = fminf(b0.lo[c], b1.lo[c]);
+            u.hi[c] = fmaxf(b0.hi[c], b1.hi[c]);
+        }
+        nb[x] = u;
+    }
+    for (int c = 0; c < 3; ++c) { bounds[c] = nb[0].lo[c]; bounds[4 + c] = nb[0].hi[c]; }
+    bounds[3] = bounds[7] = 0.0f;
+    *root_out = 0;
+    free(stack); free(state); free(nb); free(left); free(right);
+    free(leafbox); free(cen); free(tri_u); free(sh_u);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* FP32 traversal (the device traversal, operation for operation)                             */
+/* ------------------------------------------------------------------------------------------ */
+
+typedef struct {
+    float ox, oy, oz, dx, dy, dz, tmin, tmax;
+} ray32;
+
+static inline float safe_inv(float d) {
+    float dd = fabsf(d) < 0x1p-80f ? copysignf(0x1p-80f, d) : d;
+    return 1.0f / dd;
+}
+
+/* 1 + 2*gamma(3), gamma(n) = n*eps/(1-n*eps), eps = 2^-24, rounded up (Ize 2013). */
+#define TW_TFAR_SCALE 1.0000004f
+
+static inline int mt32(const float* tr, const ray32* r, float* t, float* u, float* v) {
+    float e1x = tr[4], e1y = tr[5], e1z = tr[6];
+    float e2x = tr[8], e2y = tr[9], e2z = tr[10];
+    float px = fmaf(r->dy, e2z, -(r->dz * e2y));
+    float py = fmaf(r->dz, e2x, -(r->dx * e2z));
+    float pz = fmaf(r->dx, e2y, -(r->dy * e2x));
+    float det = fmaf(e1x, px, fmaf(e1y, py, e1z * pz));
+    if (det == 0.0f) return 0;
+    float inv = 1.0f / det;
+    float sx = r->ox - tr[0], sy = r->oy - tr[1], sz = r->oz - tr[2];
+    float uu = fmaf(sx, px, fmaf(sy, py, sz * pz)) * inv;
+    if (uu < 0.0f || uu > 1.0f) return 0;
+    float qx = fmaf(sy, e1z, -(sz * e1y));
+    float qy = fmaf(sz, e1x, -(sx * e1z));
+    float qz = fmaf(sx, e1y, -(sy * e1x));
+    float vv = fmaf(r->dx, qx, fmaf(r->dy, qy, r->dz * qz)) * inv;
+    if (vv < 0.0f || uu + vv > 1.0f) return 0;
+    *t = fmaf(e2x, qx, fmaf(e2y, qy, e2z * qz)) * inv;
+    *u = uu;
+    *v = vv;
+    return 1;
+}
+
+typedef struct {
+    float t, u, v;
+    int32_t prim, slot;
+    int32_t node_visits, tri_tests;
+} hit32;
+
+/* any != 0: stop at the first accepted hit (occlusion); skip_slot is never tested. */
+static void traverse32(const float* nodes, const float* tris, int32_t root, const ray32* r,
+                       int any, int32_t skip_slot, hit32* h) {
+    h->t = r->tmax; h->u = 0.0f; h->v = 0.0f; h->prim = -1; h->slot = -1;
+    h->node_visits = 0; h->tri_tests = 0;
+    if (root == 0x7FFFFFFF) return;
+    float idx = safe_inv(r->dx), idy = safe_inv(r->dy), idz = safe_inv(r->dz);
+    float oxx = r->ox * idx, oyy = r->oy * idy, ozz = r->oz * idz;
+    int32_t stack[64];
+    int sp = 0;
+    int32_t cur = root;
+    for (;;) {
+        if (cur >= 0) {
+            const float* nd = nodes + (int64_t)cur * 16;
+            h->node_visits++;
+            float a0 = fmaf(nd[0], idx, -oxx), a1 = fmaf(nd[1], idx, -oxx);
+            float a2 = fmaf(nd[2], idy, -oyy), a3 = fmaf(nd[3], idy, -oyy);
+            float a4 = fmaf(nd[8], idz, -ozz), a5 = fmaf(nd[9], idz, -ozz);
+            float b0 = fmaf(nd[4], idx, -oxx), b1 = fmaf(nd[5], idx, -oxx);
+            float b2 = fmaf(nd[6], idy, -oyy), b3 = fmaf(nd[7], idy, -oyy);
+            float b4 = fmaf(nd[10], idz, -ozz), b5 = fmaf(nd[11], idz, -ozz);
+            float tn0 = fmaxf(fmaxf(fminf(a0, a1), fminf(a2, a3)), fmaxf(fminf(a4, a5), r->tmin));
+            float tf0 = fminf(fminf(fmaxf(a0, a1), fmaxf(a2, a3)), fmaxf(a4, a5)) * TW_TFAR_SCALE;
+            float tn1 = fmaxf(fmaxf(fminf(b0, b1), fminf(b2, b3)), fmaxf(fminf(b4, b5), r->tmin));
+            float tf1 = fminf(fminf(fmaxf(b0, b1), fmaxf(b2, b3)), fmaxf(b4, b5)) * TW_TFAR_SCALE;
+            int hit0 = tn0 <= fminf(tf0, h->t);
+            int hit1 = tn1 <= fminf(tf1, h->t);
+            int32_t r0 = (int32_t)fbits(nd[12]), r1 = (int32_t)fbits(nd[13]);
+            if (hit0 && hit1) {
+                int32_t nearc = r0, farc = r1;
+                if (tn1 < tn0) { nearc = r1; farc = r0; }
+                if (sp < 64) stack[sp++] = farc;
+                cur = nearc;
+                continue;
+            }
+            if (hit0) { cur = r0; continue; }
+            if (hit1) { cur = r1; continue; }
+        } else {
+            int32_t slot = ~cur;
+            if (slot != skip_slot) {
+                const float* tr = tris + (int64_t)slot * 12;
+                float t, u, v;
+                h->tri_tests++;
+                if (mt32(tr, r, &t, &u, &v) && t > r->tmin) {
+                    int32_t prim = (int32_t)fbits(tr[3]);
+                    if (t < h->t || (t == h->t && prim < h->prim)) {
+                        h->t = t; h->u = u; h->v = v; h->prim = prim; h->slot = slot;
+                        if (any) return;
+                    }
+                }
+            }
+        }
+        if (sp == 0) break;
+        cur = stack[--sp];
+    }
+}
+
+static inline void load_ray(const float* ox, const float* oy, const float* oz, const float* dx,
+                            const float* dy, const float* dz, const float* tmin,
+                            const float* tmax, float tmin_d, float tmax_d, int64_t i, ray32* r) {
+    r->ox = ox[i]; r->oy = oy[i]; r->oz = oz[i];
+    r->dx = dx[i]; r->dy = dy[i]; r->dz = dz[i];
+    r->tmin = tmin ? tmin[i] : tmin_d;
+    r->tmax = tmax ? tmax[i] : tmax_d;
+}
+
+/* Closest hit of n SoA rays. Outputs may be NULL. counters (may be NULL): per-ray node visits
+ * and triangle tests, 2 int32 per ray. */
+TW_API void tw_trace_closest(const float* nodes, const float* tris, int32_t root, int64_t n,
+                             const float* ox, const float* oy, const float* oz, const float* dx,
+                             const float* dy, const float* dz, const float* tmin,
+                             const float* tmax, float tmin_d, float tmax_d, float* out_t,
+                             float* out_u, float* out_v, int32_t* out_prim, int32_t* out_slot,
+                             int32_t* counters) {
+    #pragma omp parallel for schedule(dynamic, 256)
+    for (int64_t i = 0; i < n; ++i) {
+        ray32 r;
+        load_ray(ox, oy, oz, dx, dy, dz, tmin, tmax, tmin_d, tmax_d, i, &r);
+        hit32 h;
+        traverse32(nodes, tris, root, &r, 0, -1, &h);
+        if (h.prim < 0) { h.t = INFINITY; h.u = 0.0f; h.v = 0.0f; }
+        if (out_t) out_t[i] = h.t;
+        if (out_u) out_u[i] = h.u;
+        if (out_v) out_v[i] = h.v;
+        if (out_prim) out_prim[i] = h.prim;
+        if (out_slot) out_slot[i] = h.slot;
+        if (counters) { counters[2 * i] = h.node_visits; counters[2 * i + 1] = h.tri_tests; }
+    }
+}
+
+TW_API void tw_trace_any(const float* nodes, const float* tris, int32_t root, int64_t n,
+                         const float* ox, const float* oy, const float* oz, const float* dx,
+                         const float* dy, const float* dz, const float* tmin, const float* tmax,
+                         float tmin_d, float tmax_d, uint8_t* occluded) {
+    #pragma omp parallel for schedule(dynamic, 256)
+    for (int64_t i = 0; i < n; ++i) {
+        ray32 r;
+        load_ray(ox, oy, oz, dx, dy, dz, tmin, tmax, tmin_d, tmax_d, i, &r);
+        hit32 h;
+        traverse32(nodes, tris, root, &r, 1, -1, &h);
+        occluded[i] = h.prim >= 0;
+    }
+}
+
+/* FP32 brute force over every slot with the same Moller-Trumbore and tie rule. */
+TW_API void tw_brute_closest32(const float* tris, int64_t n_tris, int64_t n, const float* ox,
+                               const float* oy, const float* oz, const float* dx, const float* dy,
+                               const float* dz, float tmin_d, float tmax_d, float* out_t,
+                               int32_t* out_prim) {
+    #pragma omp parallel for schedule(dynamic, 16)
+    for (int64_t i = 0; i < n; ++i) {
+        ray32 r = {ox[i], oy[i], oz[i], dx[i], dy[i], dz[i], tmin_d, tmax_d};
+        float bt = tmax_d;
+        int32_t bp = -1;
+        for (int64_t s = 0; s < n_tris; ++s) {
+            const float* tr = tris + s * 12;
+            float t, u, v;
+            if (mt32(tr, &r, &t, &u, &v) && t > r.tmin) {
+                int32_t prim = (int32_t)fbits(tr[3]);
+                if (t < bt || (t == bt && prim < bp)) { bt = t; bp = prim; }
+            }
+        }
+        out_t[i] = bp >= 0 ? bt : INFINITY;
+        out_prim[i] = bp;
+    }
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* Ray generation and shading (FP32, the device formulas)                                       */
+/* ------------------------------------------------------------------------------------------ */
+
+/* cam: eye[3], right[3], up[3], forward[3] (see rt_camera in include/rendertoy_rt.h). */
+static inline void gen_ray(const float* cam, uint32_t W, uint32_t H, uint32_t col, uint32_t row,
+                           float* d) {
+    float nx = ((float)col + 0.5f) / (float)W * 2.0f - 1.0f;
+    float ny = 1.0f - ((float)row + 0.5f) / (float)H * 2.0f;
+    float x = fmaf(nx, cam[3], fmaf(ny, cam[6], cam[9]));
+    float y = fmaf(nx, cam[4], fmaf(ny, cam[7], cam[10]));
+    float z = fmaf(nx, cam[5], fmaf(ny, cam[8], cam[11]));
+    float il = 1.0f / sqrtf(fmaf(x, x, fmaf(y, y, z * z)));
+    d[0] = x * il; d[1] = y * il; d[2] = z * il;
+}
+
+TW_API void tw_generate_rays(const float* cam, uint32_t W, uint32_t H, float* ox, float* oy,
+                             float* oz, float* dx, float* dy, float* dz) {
+    #pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < (int64_t)W * H; ++i) {
+        float d[3];
+        gen_ray(cam, W, H, (uint32_t)(i % W), (uint32_t)(i / W), d);
+        ox[i] = cam[0]; oy[i] = cam[1]; oz[i] = cam[2];
+        dx[i] = d[0]; dy[i] = d[1]; dz[i] = d[2];
+    }
+}
+
+static inline int wrap_i(int i, int n) { int r = i % n; return r < 0 ? r + n : r; }
+
+/* RGBA8 texture fetch: bilinear (OpenCL 1.2 linear filter, repeat addressing) or point
+ * sampling as rendering/_core.py:94-96 (texel (int)(wrap(v)*H), (int)(wrap(u)*W)). */
+static inline void tex_sample(const uint8_t* tex, int W, int H, float u, float v, int bilinear,
+                              float* rgb) {
+    float su = u - floorf(u), sv = v - floorf(v);
+    if (!bilinear) {
+        int i = (int)(su * (float)W), j = (int)(sv * (float)H);
+        if (i > W - 1) i = W - 1;
+        if (j > H - 1) j = H - 1;
+        const uint8_t* t = tex + ((int64_t)j * W + i) * 4;
+        for (int c = 0; c < 3; ++c) rgb[c] = (float)t[c] * (1.0f / 255.0f);
+        return;
+    }
+    float x = su * (float)W - 0.5f, y = sv * (float)H - 0.5f;
+    float fx = floorf(x), fy = floorf(y);
+    float a = x - fx, b = y - fy;
+    int i0 = wrap_i((int)fx, W), j0 = wrap_i((int)fy, H);
+    int i1 = wrap_i((int)fx + 1, W), j1 = wrap_i((int)fy + 1, H);
+    const uint8_t* t00 = tex + ((int64_t)j0 * W + i0) * 4;
+    const uint8_t* t10 = tex + ((int64_t)j0 * W + i1) * 4;
+    const uint8_t* t01 = tex + ((int64_t)j1 * W + i0) * 4;
+    const uint8_t* t11 = tex + ((int64_t)j1 * W + i1) * 4;
+    for (int c = 0; c < 3; ++c) {
+        float top = fmaf(a, (float)t10[c] - (float)t00[c], (float)t00[c]);
+        float bot = fmaf(a, (float)t11[c] - (float)t01[c], (float)t01[c]);
+        rgb[c] = fmaf(b, bot - top, top) * (1.0f / 255.0f);
+    }
+}
+
+/* Full FP32 render of the device path: primary closest hit, Lambert with shadow ray, texture.
+ * textures: concatenated RGBA8 images; tex_off/tex_w/tex_h per texture; mesh_tex per mesh.
+ * params: light[3], ambient, background[3], shadow_tmin; flags bit0 shadows, bit1 bilinear. */
+TW_API void tw_render(const float* nodes, const float* tris, const float* shade, int32_t root,
+                      const float* cam, uint32_t W, uint32_t H, const float* params, uint32_t flags,
+                      const uint8_t* textures, const int64_t* tex_off, const int32_t* tex_w,
+                      const int32_t* tex_h, const int32_t* mesh_tex, float* rgb, float* out_t,
+                      int32_t* out_prim, int64_t* stats) {
+    int64_t n = (int64_t)W * H;
+    int64_t hits = 0, shadow_rays = 0, occl = 0;
+    const float* L = params;
+    #pragma omp parallel for schedule(dynamic, 64) reduction(+ : hits, shadow_rays, occl)
+    for (int64_t i = 0; i < n; ++i) {
+        float d[3];
+        gen_ray(cam, W, H, (uint32_t)(i % W), (uint32_t)(i / W), d);
+        ray32 r = {cam[0], cam[1], cam[2], d[0], d[1], d[2], 0.0f, INFINITY};
+        hit32 h;
+        traverse32(nodes, tris, root, &r, 0, -1, &h);
+        if (out_t) out_t[i] = h.prim >= 0 ? h.t : INFINITY;
+        if (out_prim) out_prim[i] = h.prim;
+        if (h.prim < 0) {
+            for (int c = 0; c < 3; ++c) rgb[i * 3 + c] = params[4 + c];
+            continue;
+        }
+        hits++;
+        const float* s = shade + (int64_t)h.slot * 16;
+        float w0 = 1.0f - h.u - h.v;
+        float N[3];
+        for (int c = 0; c < 3; ++c) {
+            float n0 = s[c], n1 = s[3 + c], n2 = s[6 + c];
+            N[c] = fmaf(w0, n0, fmaf(h.u, n1, h.v * n2));
+        }
+        float l2 = fmaf(N[0], N[0], fmaf(N[1], N[1], N[2] * N[2]));
+        float il = l2 > 0.0f ? 1.0f / sqrtf(l2) : 0.0f;
+        float ndl = fmaf(N[0] * il, L[0], fmaf(N[1] * il, L[1], (N[2] * il) * L[2]));
+        ndl = fmaxf(ndl, 0.0f);
+        float vis = 1.0f;
+        if ((flags & 1u) && ndl > 0.0f) {
+            ray32 sr;
+            sr.ox = fmaf(h.t, d[0], cam[0]);
+            sr.oy = fmaf(h.t, d[1], cam[1]);
+            sr.oz = fmaf(h.t, d[2], cam[2]);
+            sr.dx = L[0]; sr.dy = L[1]; sr.dz = L[2];
+            sr.tmin = params[7]; sr.tmax = INFINITY;
+            hit32 sh;
+            traverse32(nodes, tris, root, &sr, 1, h.slot, &sh);
+            shadow_rays++;
+            if (sh.prim >= 0) { vis = 0.0f; occl++; }
+        }
+        float alb[3] = {1.0f, 1.0f, 1.0f};
+        int32_t mesh = (int32_t)fbits(s[15]);
+        int32_t ti = mesh_tex ? mesh_tex[mesh] : -1;
+        if (ti >= 0) {
+            float uu = fmaf(w0, s[9], fmaf(h.u, s[11], h.v * s[13]));
+            float vv = fmaf(w0, s[10], fmaf(h.u, s[12], h.v * s[14]));
+            tex_sample(textures + tex_off[ti], tex_w[ti], tex_h[ti], uu, vv, (flags & 2u) != 0, alb);
+        }
+        float k = fmaf(vis, ndl, params[3]);
+        for (int c = 0; c < 3; ++c) rgb[i * 3 + c] = alb[c] * k;
+    }
+    if (stats) { stats[0] = hits; stats[1] = shadow_rays; stats[2] = occl; }
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* FP64 oracle (independent arithmetic)                                                         */
+/* ------------------------------------------------------------------------------------------ */
+
+/* Exact-as-possible FP64 Moller-Trumbore on FP32 vertex data (converted exactly to double). */
+static inline int mt64(const double* v0, const double* v1, const double* v2, const double* o,
+                       const double* d, double* t, double* u, double* v) {
+    double e1[3] = {v1[0] - v0[0], v1[1] - v0[1], v1[2] - v0[2]};
+    double e2[3] = {v2[0] - v0[0], v2[1] - v0[1], v2[2] - v0[2]};
+    double p[3] = {d[1] * e2[2] - d[2] * e2[1], d[2] * e2[0] - d[0] * e2[2],
+                   d[0] * e2[1] - d[1] * e2[0]};
+    double det = e1[0] * p[0] + e1[1] * p[1] + e1[2] * p[2];
+    if (det == 0.0) return 0;
+    double s[3] = {o[0] - v0[0], o[1] - v0[1], o[2] - v0[2]};
+    double uu = (s[0] * p[0] + s[1] * p[1] + s[2] * p[2]) / det;
+    double q[3] = {s[1] * e1[2] - s[2] * e1[1], s[2] * e1[0] - s[0] * e1[2],
+                   s[0] * e1[1] - s[1] * e1[0]};
+    double vv = (d[0] * q[0] + d[1] * q[1] + d[2] * q[2]) / det;
+    *t = (e2[0] * q[0] + e2[1] * q[1] + e2[2] * q[2]) / det;
+    *u = uu; *v = vv;
+    return 1;
+}
+
+/* tri_pos: n_tris*9 floats in global primitive order (vertex k of prim p at p*9+k*3).
+ * For each ray: out_t (+inf on miss), out_prim (-1 on miss), out_u/out_v, out_edge = 1 when the
+ * winner's minimum barycentric is < edge_eps, or a different triangle hits within edge_eps
+ * relative t of the winner, or a triangle missed by less than edge_eps barycentric lies closer. */
+static void closest64_ray(const float* tri_pos, int64_t n_tris, const double* o, const double* d,
+                          double tmin, double edge_eps, const int64_t* cand, int64_t n_cand,
+                          double* bt, int32_t* bp, double* bu, double* bv, uint8_t* edge) {
+    double best = INFINITY, near_miss = INFINITY, second = INFINITY;
+    int32_t prim = -1;
+    double wu = 0, wv = 0;
+    int64_t cnt = cand ? n_cand : n_tris;
+    for (int64_t k = 0; k < cnt; ++k) {
+        int64_t p = cand ? cand[k] : k;
+        const float* f = tri_pos + p * 9;
+        double v0[3] = {f[0], f[1], f[2]}, v1[3] = {f[3], f[4], f[5]}, v2[3] = {f[6], f[7], f[8]};
+        double t, u, v;
+        if (!mt64(v0, v1, v2, o, d, &t, &u, &v)) continue;
+        if (!(t > tmin)) continue;
+        double mb = fmin(fmin(u, v), 1.0 - u - v);
+        if (mb >= 0.0) {
+            if (t < best || (t == best && (int32_t)p < prim)) {
+                if (prim >= 0) second = fmin(second, best);
+                best = t; prim = (int32_t)p; wu = u; wv = v;
+            } else {
+                second = fmin(second, t);
+            }
+        } else if (mb > -edge_eps) {
+            near_miss = fmin(near_miss, t);
+        }
+    }
+    *bt = best; *bp = prim; *bu = wu; *bv = wv;
+    uint8_t e = 0;
+    if (prim >= 0) {
+        double mb = fmin(fmin(wu, wv), 1.0 - wu - wv);
+        if (mb < edge_eps) e = 1;
+        if (second <= best * (1.0 + edge_eps)) e = 1;
+        if (near_miss <= best * (1.0 + edge_eps)) e = 1;
+    } else if (near_miss < INFINITY) {
+        e = 1;
+    }
+    *edge = e;
+}
+
+TW_API void tw_closest64_brute(const float* tri_pos, int64_t n_tris, int64_t n, const float* ox,
+                               const float* oy, const float* oz, const float* dx,
+                               const float* dy, const float* dz, double tmin, double edge_eps,
+                               double* out_t, int32_t* out_prim, double* out_u, double* out_v,
+                               uint8_t* out_edge) {
+    #pragma omp parallel for schedule(dynamic, 16)
+    for (int64_t i = 0; i < n; ++i) {
+        double o[3] = {ox[i], oy[i], oz[i]}, d[3] = {dx[i], dy[i], dz[i]};
+        closest64_ray(tri_pos, n_tris, o, d, tmin, edge_eps, NULL, 0, &out_t[i], &out_prim[i],
+                      &out_u[i], &out_v[i], &out_edge[i]);
+    }
+}
+
+/* FP64 closest hit culled by an FP64 BVH over the same leaves: boxes are recomputed in FP64 from
+ * the vertices and inflated by `margin` (absolute), so culling is conservative by construction.
+ * nodes/order: the twin's hierarchy (only child refs are used). */
+TW_API void tw_closest64_bvh(const float* tri_pos, int64_t n_tris, const float* nodes,
+                             const uint32_t* order, int32_t root, int64_t n, const float* ox,
+                             const float* oy, const float* oz, const float* dx, const float* dy,
+                             const float* dz, double tmin, double edge_eps, double margin,
+                             double* out_t, int32_t* out_prim, double* out_u, double* out_v,
+                             uint8_t* out_edge) {
+    if (root == 0x7FFFFFFF || n_tris == 0) {
+        for (int64_t i = 0; i < n; ++i) {
+            out_t[i] = INFINITY; out_prim[i] = -1; out_u[i] = out_v[i] = 0; out_edge[i] = 0;
+        }
+        return;
+    }
+    int64_t nn = n_tris > 1 ? n_tris - 1 : 0;
+    double* nb = (double*)malloc(sizeof(double) * 6 * (nn + 1));
+    double* lb = (double*)malloc(sizeof(double) * 6 * n_tris);
+    /* leaf boxes by slot */
+    for (int64_t s = 0; s < n_tris; ++s) {
+        const float* f = tri_pos + (int64_t)order[s] * 9;
+        for (int c = 0; c < 3; ++c) {
+            double lo = fmin(fmin((double)f[c], (double)f[3 + c]), (double)f[6 + c]);
+            double hi = fmax(fmax((double)f[c], (double)f[3 + c]), (double)f[6 + c]);
+            lb[s * 6 + c] = lo - margin;
+            lb[s * 6 + 3 + c] = hi + margin;
+        }
+    }
+    /* internal boxes by post-order */
+    if (nn > 0) {
+        int32_t* st = (int32_t*)malloc(sizeof(int32_t) * 2 * (n_tris + 1));
+        uint8_t* state = (uint8_t*)calloc((size_t)nn, 1);
+        int64_t sp = 0;
+        st[sp++] = 0;
+        while (sp) {
+            int32_t x = st[sp - 1];
+            int32_t l = (int32_t)fbits(nodes[x * 16 + 12]), r = (int32_t)fbits(nodes[x * 16 + 13]);
+            if (!state[x]) {
+                state[x] = 1;
+                if (r >= 0) st[sp++] = r;
+                if (l >= 0) st[sp++] = l;
+                continue;
+            }
+            --sp;
+            const double* bl = l >= 0 ? nb + l * 6 : lb + (int64_t)(~l) * 6;
+            const double* br = r >= 0 ? nb + r * 6 : lb + (int64_t)(~r) * 6;
+            for (int c = 0; c < 3; ++c) {
+                nb[x * 6 + c] = fmin(bl[c], br[c]);
+                nb[x * 6 + 3 + c] = fmax(bl[3 + c], br[3 + c]);
+            }
+        }
+        free(st); free(state);
+    }
+    #pragma omp parallel
+    {
+        int64_t cap = 4096;
+        int64_t* cand = (int64_t*)malloc(sizeof(int64_t) * cap);
+        int32_t* stk = (int32_t*)malloc(sizeof(int32_t) * (n_tris + 64));
+        #pragma omp for schedule(dynamic, 64)
+        for (int64_t i = 0; i < n; ++i) {
+            double o[3] = {ox[i], oy[i], oz[i]}, d[3] = {dx[i], dy[i], dz[i]};
+            int64_t nc = 0, sp = 0;
+            stk[sp++] = root;
+            while (sp) {
+                int32_t x = stk[--sp];
+                const double* b = x >= 0 ? nb + (int64_t)x * 6 : lb + (int64_t)(~x) * 6;
+                double tn = -INFINITY, tf = INFINITY;
+                int ok = 1;
+                for (int c = 0; c < 3 && ok; ++c) {
+                    if (d[c] == 0.0) {
+                        if (o[c] < b[c] || o[c] > b[3 + c]) ok = 0;
+                        continue;
+                    }
+                    double t0 = (b[c] - o[c]) / d[c], t1 = (b[3 + c] - o[c]) / d[c];
+                    if (t0 > t1) { double tt = t0; t0 = t1; t1 = tt; }
+                    tn = fmax(tn, t0); tf = fmin(tf, t1);
+                }
+                if (!ok || tn > tf * (1.0 + 1e-12) + 1e-300 || tf < tmin - margin) continue;
+                if (x >= 0) {
+                    stk[sp++] = (int32_t)fbits(nodes[(int64_t)x * 16 + 12]);
+                    stk[sp++] = (int32_t)fbits(nodes[(int64_t)x * 16 + 13]);
+                } else {
+                    if (nc == cap) { cap *= 2; cand = (int64_t*)realloc(cand, sizeof(int64_t) * cap); }
+                    cand[nc++] = order[~x];
+                }
+            }
+            closest64_ray(tri_pos, n_tris, o, d, tmin, edge_eps, cand, nc, &out_t[i],
+                          &out_prim[i], &out_u[i], &out_v[i], &out_edge[i]);
+        }
+        free(cand); free(stk);
+    }
+    free(nb); free(lb);
+}
+
+TW_API int tw_num_threads(void) {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
+TW_API void tw_set_threads(int n) {
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+#else
+    (void)n;
+#endif
+}

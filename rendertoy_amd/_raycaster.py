@@ -1,0 +1,392 @@
+"""The ray caster of rendering/_raycaster.py on the MI355X kernels of librtx_hip.so.
+
+Reference API kept: `Raycaster(models: List[Mesh])` builds the acceleration structure
+(`_build_ads`, rendering/_raycaster.py:26-33) and `ray_cast(rays) -> array` answers closest-hit
+queries (:35-36); the `BVH_AABB` and `BVH_Triangle` structs (:8-22) are registered with the same
+layout. Additions: `Ray`/`Hit` structs, `occluded(rays)`, `render(...)` and `generate_rays(...)`.
+Triangle identity of a hit is (mesh, index) as BVH_Triangle defines it (:16-22).
+Every call runs on the GPU; there is no CPU path.
+"""
+import ctypes
+import math
+import typing
+
+import numpy as np
+import torch
+
+from . import _native as N
+from ._core import (kernel_struct, float3, float4, DeviceArray, create_buffer, Dispatcher,
+                    to_array, texture_pointer, RGBA)
+from ._modeling import Mesh, MeshVertex
+
+
+@kernel_struct
+class BVH_AABB:
+    min: float3
+    max: float3
+    count: int
+    elements: int
+
+
+@kernel_struct
+class BVH_Triangle:
+    v0: float3
+    v1: float3
+    v2: float3
+    index: int
+    mesh: int
+
+
+@kernel_struct
+class Ray:
+    """origin @0, direction @16 (32 bytes)."""
+    origin: float3
+    direction: float3
+
+
+@kernel_struct
+class Hit:
+    """t, barycentrics (u, v) of v1 and v2, triangle index within its mesh, mesh position in
+    `models`; a miss is t=+inf, index=mesh=-1 (20 bytes)."""
+    t: np.float32
+    u: np.float32
+    v: np.float32
+    index: np.int32
+    mesh: np.int32
+
+
+class Camera:
+    """Pinhole camera of look_at + perspective (rendering/_core.py:528-548) reduced to the vectors
+    the kernels use. Host math in float32 as the reference's make_float* values are."""
+
+    def __init__(self, eye, target=(0.0, 0.0, 0.0), up=(0.0, 1.0, 0.0), fov=math.pi / 4,
+                 aspect=1.0):
+        eye = np.asarray(to_array(eye) if np.asarray(eye).dtype.names else eye, dtype=np.float32)
+        target = np.asarray(to_array(target) if np.asarray(target).dtype.names else target,
+                            dtype=np.float32)
+        up = np.asarray(to_array(up) if np.asarray(up).dtype.names else up, dtype=np.float32)
+        z = target - eye
+        z = (z / np.float32(np.sqrt(np.dot(z, z)))).astype(np.float32)
+        x = np.cross(up, z).astype(np.float32)
+        x = (x / np.float32(np.sqrt(np.dot(x, x)))).astype(np.float32)
+        y = np.cross(z, x).astype(np.float32)
+        tan_half = math.tan(fov / 2.0)
+        self.eye = eye.astype(np.float32)
+        self.right = (x * np.float32(aspect * tan_half)).astype(np.float32)
+        self.up = (y * np.float32(tan_half)).astype(np.float32)
+        self.forward = z
+        self.fov = float(fov)
+        self.aspect = float(aspect)
+
+    def as_array(self):
+        """12 float32: eye, right, up, forward."""
+        return np.concatenate([self.eye, self.right, self.up, self.forward]).astype(np.float32)
+
+    def to_c(self):
+        c = N.rt_camera()
+        for name in ("eye", "right", "up", "forward"):
+            getattr(c, name)[:] = [float(v) for v in getattr(self, name)]
+        return c
+
+
+def _ptr(t) -> int:
+    if t is None:
+        return None
+    if isinstance(t, DeviceArray):
+        return t.data_ptr()
+    return t.data_ptr()
+
+
+def _check_device(t, what):
+    dev = t.device if not isinstance(t, DeviceArray) else t.storage.device
+    if dev.type != "cuda":
+        raise RuntimeError(f"{what} must live on the HIP device (got {dev})")
+
+
+class Raycaster:
+    """Closest-hit ray caster over a list of meshes (rendering/_raycaster.py:25-36)."""
+
+    def __init__(self, models: typing.List[Mesh], stream=None):
+        self.models = models
+        self._stream = stream
+        self._render_ws = None
+        self._build_ads()
+
+    # ------------------------------------------------------------------ build
+    def _build_ads(self):
+        lib = N.require_gpu()
+        descs = (N.rt_mesh_desc * max(1, len(self.models)))()
+        keep = []
+        n_tris = 0
+        for k, m in enumerate(self.models):
+            vb = m.vertices
+            _check_device(vb, "Mesh.vertices")
+            if vb.dtype != MeshVertex:
+                raise TypeError("Mesh.vertices must be a buffer of MeshVertex")
+            tris = m.vertices.shape[0] // 3 if m.indices is None else m.indices.size // 3
+            d = descs[k]
+            d.vertices = vb.data_ptr()
+            d.n_vertices = vb.shape[0]
+            d.n_tris = tris
+            d.vertex_stride = MeshVertex.itemsize
+            d.pos_offset = MeshVertex.fields["P"][1]
+            d.nrm_offset = MeshVertex.fields["N"][1] if getattr(m, "has_normals", True) else N.RT_ABSENT
+            d.uv_offset = MeshVertex.fields["C"][1] if getattr(m, "has_uvs", True) else N.RT_ABSENT
+            if m.indices is not None:
+                ib = m.indices
+                _check_device(ib, "Mesh.indices")
+                if ib.dtype != np.int32:
+                    # load_obj-style int64 index buffers (rendering/_loaders.py:17) are narrowed
+                    t = ib.tensor(torch.int64 if ib.dtype == np.int64 else None)
+                    ib32 = t.to(torch.int32).contiguous()
+                    keep.append(ib32)
+                    d.indices = ib32.data_ptr()
+                else:
+                    d.indices = ib.data_ptr()
+            else:
+                d.indices = None
+            n_tris += tris
+        self.n_tris = n_tris
+        self.mesh_tri_counts = [int(descs[k].n_tris) for k in range(len(self.models))]
+        sizes = N.rt_scene_sizes()
+        N.check(lib.rt_scene_sizes_for(n_tris, ctypes.byref(sizes)), "rt_scene_sizes_for")
+        dev = torch.device("cuda")
+        # allocate on the stream the build runs on: torch's allocator orders reuse of the freed
+        # workspace after the build kernels on that stream
+        with torch.cuda.stream(self._torch_stream()):
+            self._nodes = torch.empty(sizes.nodes_bytes, dtype=torch.uint8, device=dev)
+            self._tris = torch.empty(sizes.tris_bytes, dtype=torch.uint8, device=dev)
+            self._shade = torch.empty(sizes.shade_bytes, dtype=torch.uint8, device=dev)
+            self._bounds = torch.empty(8, dtype=torch.float32, device=dev)
+            self._mesh_offsets = torch.empty(len(self.models) + 1, dtype=torch.int32, device=dev)
+            self.keys = torch.empty(max(1, n_tris), dtype=torch.int32, device=dev)
+            self.order = torch.empty(max(1, n_tris), dtype=torch.int32, device=dev)
+            ws = torch.empty(sizes.workspace_bytes, dtype=torch.uint8, device=dev)
+        sc = N.rt_scene()
+        sc.n_tris = n_tris
+        sc.n_meshes = len(self.models)
+        sc.nodes = self._nodes.data_ptr()
+        sc.tris = self._tris.data_ptr()
+        sc.shade = self._shade.data_ptr()
+        sc.mesh_offsets = self._mesh_offsets.data_ptr()
+        sc.bounds = self._bounds.data_ptr()
+        code = lib.rt_scene_build(descs, len(self.models), ctypes.byref(sc), ws.data_ptr(),
+                                  sizes.workspace_bytes, self.keys.data_ptr(),
+                                  self.order.data_ptr(), N.stream_handle(self._stream))
+        N.check(code, "rt_scene_build")
+        self._scene = sc
+        self._keep = keep
+        del ws
+
+    def _torch_stream(self):
+        return self._stream if self._stream is not None else torch.cuda.current_stream()
+
+    @property
+    def root(self) -> int:
+        return int(self._scene.root)
+
+    def scene_arrays(self):
+        """Raw device arrays of the built scene: nodes [n-1,16] f32, tris [n,12] f32,
+        shade [n,16] f32, bounds [8], sorted Morton keys and primitive order."""
+        n = self.n_tris
+        ni = max(n - 1, 1)
+        return {
+            "nodes": self._nodes.view(torch.float32).view(ni, 16)[: max(n - 1, 0)],
+            "tris": self._tris.view(torch.float32).view(max(n, 1), 12)[:n],
+            "shade": self._shade.view(torch.float32).view(max(n, 1), 16)[:n],
+            "bounds": self._bounds,
+            "keys": self.keys[:n],
+            "order": self.order[:n],
+            "mesh_offsets": self._mesh_offsets,
+        }
+
+    # ------------------------------------------------------------------ queries
+    def _rays_desc(self, rays, tmin, tmax):
+        d = N.rt_rays()
+        if isinstance(rays, DeviceArray):
+            if rays.dtype != Ray:
+                raise TypeError("rays must be a buffer of Ray")
+            _check_device(rays, "rays")
+            base = rays.data_ptr()
+            n = rays.size
+            d.ox, d.oy, d.oz = base, base + 4, base + 8
+            d.dx, d.dy, d.dz = base + 16, base + 20, base + 24
+            d.stride = Ray.itemsize
+        elif isinstance(rays, dict):
+            for k in ("ox", "oy", "oz", "dx", "dy", "dz"):
+                _check_device(rays[k], "rays")
+                if rays[k].dtype != torch.float32 or not rays[k].is_contiguous():
+                    raise TypeError("SoA ray fields must be contiguous float32 tensors")
+            n = rays["ox"].numel()
+            d.ox, d.oy, d.oz = (rays[k].data_ptr() for k in ("ox", "oy", "oz"))
+            d.dx, d.dy, d.dz = (rays[k].data_ptr() for k in ("dx", "dy", "dz"))
+            d.stride = 4
+            if "tmin" in rays:
+                d.tmin = rays["tmin"].data_ptr()
+            if "tmax" in rays:
+                d.tmax = rays["tmax"].data_ptr()
+        else:
+            raise TypeError("rays must be a Ray buffer or a dict of SoA float32 tensors")
+        d.tmin_default = float(tmin)
+        d.tmax_default = float(tmax)
+        return d, n
+
+    def ray_cast(self, rays, tmin: float = 0.0, tmax: float = math.inf) -> DeviceArray:
+        """Closest hit of every ray: a buffer of Hit (rendering/_raycaster.py:35-36)."""
+        lib = N.require_gpu()
+        d, n = self._rays_desc(rays, tmin, tmax)
+        out = create_buffer(n, Hit)
+        base = out.data_ptr()
+        h = N.rt_hits()
+        h.t, h.u, h.v = base, base + 4, base + 8
+        h.index, h.mesh = base + 12, base + 16
+        h.prim = None
+        h.stride = Hit.itemsize
+        N.check(lib.rt_trace_closest(ctypes.byref(self._scene), ctypes.byref(d), n, ctypes.byref(h),
+                                     N.stream_handle(self._stream)), "rt_trace_closest")
+        return out
+
+    def trace_soa(self, rays: dict, tmin=0.0, tmax=math.inf, out=None):
+        """Closest hit of SoA rays into SoA torch tensors t, u, v, prim, index, mesh."""
+        lib = N.require_gpu()
+        d, n = self._rays_desc(rays, tmin, tmax)
+        dev = torch.device("cuda")
+        if out is None:
+            out = {k: torch.empty(n, dtype=torch.float32, device=dev) for k in ("t", "u", "v")}
+            out.update({k: torch.empty(n, dtype=torch.int32, device=dev)
+                        for k in ("prim", "index", "mesh")})
+        h = N.rt_hits()
+        for k in ("t", "u", "v", "prim", "index", "mesh"):
+            setattr(h, k, out[k].data_ptr() if out.get(k) is not None else None)
+        h.stride = 4
+        N.check(lib.rt_trace_closest(ctypes.byref(self._scene), ctypes.byref(d), n, ctypes.byref(h),
+                                     N.stream_handle(self._stream)), "rt_trace_closest")
+        return out
+
+    def occluded(self, rays, tmin: float = 0.0, tmax: float = math.inf) -> torch.Tensor:
+        """Any-hit: uint8 tensor, 1 where some triangle lies in (tmin, tmax) along the ray."""
+        lib = N.require_gpu()
+        d, n = self._rays_desc(rays, tmin, tmax)
+        out = torch.empty(max(n, 1), dtype=torch.uint8, device="cuda")
+        N.check(lib.rt_trace_any(ctypes.byref(self._scene), ctypes.byref(d), n, out.data_ptr(),
+                                 N.stream_handle(self._stream)), "rt_trace_any")
+        return out[:n]
+
+    @property
+    def trace(self):
+        """`rc.trace[n](rays, hits)` launch form of the reference's Dispatcher (rendering/_core.py:261)."""
+        def _fn(n, rays, hits_out):
+            res = self.trace_soa(rays, out=hits_out)
+            return res
+        return Dispatcher(_fn, "trace")
+
+    # ------------------------------------------------------------------ camera rays / render
+    @staticmethod
+    def generate_rays(camera: Camera, width: int, height: int, stream=None) -> dict:
+        """Primary rays of a width x height image as SoA float32 tensors (row-major pixels)."""
+        lib = N.require_gpu()
+        n = width * height
+        dev = torch.device("cuda")
+        out = {k: torch.empty(max(n, 1), dtype=torch.float32, device=dev)
+               for k in ("ox", "oy", "oz", "dx", "dy", "dz")}
+        cam = camera.to_c()
+        N.check(lib.rt_generate_rays(ctypes.byref(cam), width, height,
+                                     *[out[k].data_ptr() for k in ("ox", "oy", "oz", "dx", "dy", "dz")],
+                                     N.stream_handle(stream)), "rt_generate_rays")
+        return {k: v[:n] for k, v in out.items()}
+
+    def cast_camera(self, camera: Camera, width: int, height: int, out=None, fields=("t", "u", "v", "prim")):
+        """Closest hit of the primary rays of a width x height image (ray generation fused):
+        SoA tensors over pixels (row-major), misses t=+inf, prim/index/mesh=-1."""
+        lib = N.require_gpu()
+        n = width * height
+        dev = torch.device("cuda")
+        if out is None:
+            out = {}
+            for k in fields:
+                out[k] = torch.empty(max(n, 1), dtype=torch.float32 if k in ("t", "u", "v") else torch.int32,
+                                     device=dev)
+        h = N.rt_hits()
+        for k in ("t", "u", "v", "prim", "index", "mesh"):
+            setattr(h, k, out[k].data_ptr() if out.get(k) is not None else None)
+        h.stride = 4
+        cam = camera.to_c()
+        N.check(lib.rt_trace_camera(ctypes.byref(self._scene), ctypes.byref(cam), width, height,
+                                    ctypes.byref(h), N.stream_handle(self._stream)), "rt_trace_camera")
+        return out
+
+    def _render_workspace(self, n_pixels):
+        lib = N.load()
+        need = lib.rt_render_workspace_bytes(n_pixels)
+        if self._render_ws is None or self._render_ws.numel() < need:
+            self._render_ws = torch.empty(need, dtype=torch.uint8, device="cuda")
+        return self._render_ws
+
+    def render(self, camera: Camera, width: int, height: int, light=None, textures=None,
+               mesh_textures=None, shadows=True, bilinear=True, ambient=0.2,
+               background=(0.0, 0.0, 0.0), shadow_tmin=1e-4, out=None, return_hits=False,
+               stats=None):
+        """RGB float32 image [height, width, 3] of the scene: closest hit, Lambert with
+        L = normalize(1,1,1) and ambient 0.2 by default (tutorials/lesson09_texture_mapping.py:73),
+        shadow ray toward L, albedo from RGBA8 textures (texels as uint8 tensors [H, W, 4] or
+        (memory, Texture2D) pairs from create_texture2D(w, h, RGBA)); `mesh_textures[m]` picks
+        the texture of mesh m (-1: albedo 1)."""
+        lib = N.require_gpu()
+        n = width * height
+        dev = torch.device("cuda")
+        if light is None:
+            light = np.array([1.0, 1.0, 1.0], dtype=np.float32) / np.float32(np.sqrt(3.0))
+        light = np.asarray(light, dtype=np.float32)
+        p = N.rt_render_params()
+        p.width, p.height = width, height
+        p.flags = (N.RT_RENDER_SHADOWS if shadows else 0) | (N.RT_RENDER_BILINEAR if bilinear else 0)
+        p.light[:] = [float(v) for v in light]
+        p.ambient = float(ambient)
+        p.background[:] = [float(v) for v in background]
+        p.shadow_tmin = float(shadow_tmin)
+        tex_keep = []
+        if textures:
+            arr = (N.rt_texture * len(textures))()
+            for i, tx in enumerate(textures):
+                if isinstance(tx, tuple):
+                    mem, desc = tx
+                    if mem.dtype != RGBA:
+                        raise TypeError("render samples RGBA (uchar4) textures")
+                    arr[i].texels = texture_pointer(desc)
+                    arr[i].width, arr[i].height = mem.shape[1], mem.shape[0]
+                else:
+                    _check_device(tx, "texture")
+                    if tx.dtype != torch.uint8 or tx.dim() != 3 or tx.shape[2] != 4 or not tx.is_contiguous():
+                        raise TypeError("texture tensors must be contiguous uint8 [H, W, 4]")
+                    arr[i].texels = tx.data_ptr()
+                    arr[i].width, arr[i].height = tx.shape[1], tx.shape[0]
+            tdev = torch.frombuffer(bytearray(bytes(arr)), dtype=torch.uint8).to(dev)
+            tex_keep.append(tdev)
+            p.textures_dev = tdev.data_ptr()
+            p.n_textures = len(textures)
+            if mesh_textures is None:
+                mesh_textures = [0] * len(self.models)
+        if mesh_textures is not None:
+            mt = torch.tensor(list(mesh_textures), dtype=torch.int32).to(dev)
+            tex_keep.append(mt)
+            p.mesh_texture_dev = mt.data_ptr()
+        if out is None:
+            out = torch.empty((height, width, 3), dtype=torch.float32, device=dev)
+        o = N.rt_render_out()
+        o.rgb = out.data_ptr()
+        hits = None
+        if return_hits:
+            hits = {"t": torch.empty(n, dtype=torch.float32, device=dev),
+                    "prim": torch.empty(n, dtype=torch.int32, device=dev)}
+            o.t, o.prim = hits["t"].data_ptr(), hits["prim"].data_ptr()
+        if stats is not None:
+            o.stats = stats.data_ptr()
+        ws = self._render_workspace(n)
+        cam = camera.to_c()
+        N.check(lib.rt_render(ctypes.byref(self._scene), ctypes.byref(cam), ctypes.byref(p),
+                              ctypes.byref(o), ws.data_ptr(), ws.numel(),
+                              N.stream_handle(self._stream)), "rt_render")
+        self._tex_keep = tex_keep  # alive until the next render on this stream
+        if return_hits:
+            return out, hits
+        return out

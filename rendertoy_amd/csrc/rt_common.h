@@ -1,0 +1,186 @@
+// rt_common.h — device records, FP32 intersection routines and host error plumbing shared by the
+// gfx950 kernels of librtx_hip.so.
+//
+// Scene records (all 16-byte aligned, read with 128-bit loads):
+//   node  (64 B, 4 x float4)  [0] = (b0.lo.x, b0.hi.x, b0.lo.y, b0.hi.y)
+//                             [1] = (b1.lo.x, b1.hi.x, b1.lo.y, b1.hi.y)
+//                             [2] = (b0.lo.z, b0.hi.z, b1.lo.z, b1.hi.z)
+//                             [3] = (ref0, ref1, 0, 0) as int32 bits
+//         b0/b1 are the boxes of the two children; ref >= 0 is an internal node, ref < 0 is the
+//         leaf of triangle slot ~ref. A traversal step reads one node and tests both children.
+//   tri   (48 B, 3 x float4)  (v0.xyz, global primitive id bits), (e1.xyz, 0), (e2.xyz, 0)
+//         e1 = v1 - v0, e2 = v2 - v0 in FP32, stored in slot (Morton) order.
+//   shade (64 B, 4 x float4)  N0.xyz, N1.xyz, N2.xyz, uv0, uv1, uv2, mesh id bits (slot order)
+//
+// Arithmetic is written with explicit fmaf and the library is compiled with -ffp-contract=off
+// and correctly rounded division/sqrt, so every FP32 result is a fixed function of the inputs.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string>
+
+#include "../../include/rendertoy_rt.h"
+
+#define RT_DEV __device__ __forceinline__
+
+namespace rt {
+
+constexpr int kStackDepth = 64;        // >= 62: depth bound of a radix tree over 62-bit keys
+constexpr float kTfarScale = 1.0000004f;  // 1 + 2*gamma(3) rounded up (Ize 2013, robust slabs)
+
+struct Ray {
+    float ox, oy, oz, dx, dy, dz, tmin, tmax;
+};
+
+struct Hit {
+    float t, u, v;
+    int32_t prim, slot;
+};
+
+RT_DEV float safe_inv(float d) {
+    float dd = fabsf(d) < 0x1p-80f ? copysignf(0x1p-80f, d) : d;
+    return 1.0f / dd;
+}
+
+// Two-sided Moller-Trumbore (Moller & Trumbore 1997) in FP32 over a (v0, e1, e2) record.
+RT_DEV bool intersect_tri(const float4 a, const float4 b, const float4 c, const Ray& r, float& t,
+                          float& u, float& v) {
+    const float px = fmaf(r.dy, c.z, -(r.dz * c.y));
+    const float py = fmaf(r.dz, c.x, -(r.dx * c.z));
+    const float pz = fmaf(r.dx, c.y, -(r.dy * c.x));
+    const float det = fmaf(b.x, px, fmaf(b.y, py, b.z * pz));
+    if (det == 0.0f) return false;
+    const float inv = 1.0f / det;
+    const float sx = r.ox - a.x, sy = r.oy - a.y, sz = r.oz - a.z;
+    const float uu = fmaf(sx, px, fmaf(sy, py, sz * pz)) * inv;
+    if (uu < 0.0f || uu > 1.0f) return false;
+    const float qx = fmaf(sy, b.z, -(sz * b.y));
+    const float qy = fmaf(sz, b.x, -(sx * b.z));
+    const float qz = fmaf(sx, b.y, -(sy * b.x));
+    const float vv = fmaf(r.dx, qx, fmaf(r.dy, qy, r.dz * qz)) * inv;
+    if (vv < 0.0f || uu + vv > 1.0f) return false;
+    t = fmaf(c.x, qx, fmaf(c.y, qy, c.z * qz)) * inv;
+    u = uu;
+    v = vv;
+    return true;
+}
+
+// Per-lane traversal stack in private memory (64 entries, see kStackDepth).
+struct Stack {
+    int32_t e[kStackDepth];
+};
+
+// Closest-hit (ANY=false) or any-hit (ANY=true) traversal of the two-child node layout, nearer
+// child first. `skip_slot` is never tested (the shadow ray's own triangle). On return h.prim < 0
+// means no hit in (tmin, tmax).
+template <bool ANY>
+RT_DEV void traverse(const float4* __restrict__ nodes, const float4* __restrict__ tris,
+                     int32_t root, const Ray& r, int32_t skip_slot, Hit& h) {
+    h.t = r.tmax;
+    h.u = 0.0f;
+    h.v = 0.0f;
+    h.prim = -1;
+    h.slot = -1;
+    if (root == RT_EMPTY_ROOT) return;
+    const float idx = safe_inv(r.dx), idy = safe_inv(r.dy), idz = safe_inv(r.dz);
+    const float oxx = r.ox * idx, oyy = r.oy * idy, ozz = r.oz * idz;
+    int32_t stack[kStackDepth];
+    int sp = 0;
+    int32_t cur = root;
+    for (;;) {
+        if (cur >= 0) {
+            const float4* nd = nodes + (size_t)cur * 4;
+            const float4 n0 = nd[0], n1 = nd[1], n2 = nd[2], n3 = nd[3];
+            const float a0 = fmaf(n0.x, idx, -oxx), a1 = fmaf(n0.y, idx, -oxx);
+            const float a2 = fmaf(n0.z, idy, -oyy), a3 = fmaf(n0.w, idy, -oyy);
+            const float a4 = fmaf(n2.x, idz, -ozz), a5 = fmaf(n2.y, idz, -ozz);
+            const float b0 = fmaf(n1.x, idx, -oxx), b1 = fmaf(n1.y, idx, -oxx);
+            const float b2 = fmaf(n1.z, idy, -oyy), b3 = fmaf(n1.w, idy, -oyy);
+            const float b4 = fmaf(n2.z, idz, -ozz), b5 = fmaf(n2.w, idz, -ozz);
+            const float tn0 = fmaxf(fmaxf(fminf(a0, a1), fminf(a2, a3)), fmaxf(fminf(a4, a5), r.tmin));
+            const float tf0 = fminf(fminf(fmaxf(a0, a1), fmaxf(a2, a3)), fmaxf(a4, a5)) * kTfarScale;
+            const float tn1 = fmaxf(fmaxf(fminf(b0, b1), fminf(b2, b3)), fmaxf(fminf(b4, b5), r.tmin));
+            const float tf1 = fminf(fminf(fmaxf(b0, b1), fmaxf(b2, b3)), fmaxf(b4, b5)) * kTfarScale;
+            const bool hit0 = tn0 <= fminf(tf0, h.t);
+            const bool hit1 = tn1 <= fminf(tf1, h.t);
+            const int32_t r0 = __float_as_int(n3.x), r1 = __float_as_int(n3.y);
+            if (hit0 && hit1) {
+                int32_t nearc = r0, farc = r1;
+                if (tn1 < tn0) {
+                    nearc = r1;
+                    farc = r0;
+                }
+                if (sp < kStackDepth) stack[sp++] = farc;
+                cur = nearc;
+                continue;
+            }
+            if (hit0) {
+                cur = r0;
+                continue;
+            }
+            if (hit1) {
+                cur = r1;
+                continue;
+            }
+        } else {
+            const int32_t slot = ~cur;
+            if (slot != skip_slot) {
+                const float4* tr = tris + (size_t)slot * 3;
+                const float4 ta = tr[0], tb = tr[1], tc = tr[2];
+                float t, u, v;
+                if (intersect_tri(ta, tb, tc, r, t, u, v) && t > r.tmin) {
+                    const int32_t prim = __float_as_int(ta.w);
+                    if (t < h.t || (t == h.t && prim < h.prim)) {
+                        h.t = t;
+                        h.u = u;
+                        h.v = v;
+                        h.prim = prim;
+                        h.slot = slot;
+                        if (ANY) return;
+                    }
+                }
+            }
+        }
+        if (sp == 0) break;
+        cur = stack[--sp];
+    }
+}
+
+// Primary ray direction of pixel (col,row) (rendering/_core.py:528-548 look_at/perspective,
+// rendering/_raster.py:126-129 viewport with the y flip, :298-299 pixel centre).
+RT_DEV void camera_dir(const rt_camera& cam, uint32_t W, uint32_t H, uint32_t col, uint32_t row,
+                       float& dx, float& dy, float& dz) {
+    const float nx = ((float)col + 0.5f) / (float)W * 2.0f - 1.0f;
+    const float ny = 1.0f - ((float)row + 0.5f) / (float)H * 2.0f;
+    const float x = fmaf(nx, cam.right[0], fmaf(ny, cam.up[0], cam.forward[0]));
+    const float y = fmaf(nx, cam.right[1], fmaf(ny, cam.up[1], cam.forward[1]));
+    const float z = fmaf(nx, cam.right[2], fmaf(ny, cam.up[2], cam.forward[2]));
+    const float il = 1.0f / sqrtf(fmaf(x, x, fmaf(y, y, z * z)));
+    dx = x * il;
+    dy = y * il;
+    dz = z * il;
+}
+
+// Wave64 compaction: returns this lane's slot in a contiguous range appended to *counter by
+// the active lanes with `pred` set (one atomic per wave).
+RT_DEV uint32_t wave_append(uint32_t* counter, bool pred) {
+    const uint64_t mask = __ballot(pred);
+    const uint32_t lo = (uint32_t)mask, hi = (uint32_t)(mask >> 32);
+    const uint32_t rank = __builtin_amdgcn_mbcnt_hi(hi, __builtin_amdgcn_mbcnt_lo(lo, 0u));
+    const int leader = __ffsll((unsigned long long)mask) - 1;
+    uint32_t base = 0;
+    if (mask != 0 && (int)__lane_id() == leader) base = atomicAdd(counter, (uint32_t)__popcll(mask));
+    base = __shfl(base, leader < 0 ? 0 : leader);
+    return base + rank;
+}
+
+}  // namespace rt
+
+// ---- host-side error plumbing (rt_api.hip) ----
+namespace rt {
+int set_error(int code, const std::string& msg);
+int check_hip(hipError_t e, const char* what);
+int check_launch(const char* what);
+int device_cu_count();
+}  // namespace rt

@@ -1,0 +1,230 @@
+"""GPU parity: the HIP kernels of librtx_hip.so against the CPU twin (bit-exact) and the FP64
+oracle (IDs exact except counted edge rays, t within 1e-5 relative, RGB within 1/255)."""
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from conftest import c1_scene, c2_scene
+from helpers import twin_from_meshes, gpu_from_meshes, gpu_arrays, camera, random_rays, soa_tensors
+
+pytestmark = pytest.mark.gpu
+
+LIGHT = (np.array([1.0, 1.0, 1.0], np.float32) / np.float32(np.sqrt(3.0))).astype(np.float32)
+
+
+def _u32(a):
+    return np.ascontiguousarray(a).view(np.uint32)
+
+
+def assert_build_equal(rc, tw):
+    g = gpu_arrays(rc)
+    n = tw.n
+    assert rc.root == tw.root
+    np.testing.assert_array_equal(g["keys"].view(np.uint32), tw.keys)
+    np.testing.assert_array_equal(g["order"].view(np.uint32), tw.order)
+    np.testing.assert_array_equal(_u32(g["tris"]), _u32(tw.tris))
+    np.testing.assert_array_equal(_u32(g["shade"]), _u32(tw.shade))
+    np.testing.assert_array_equal(_u32(g["nodes"]), _u32(tw.nodes))
+    np.testing.assert_array_equal(_u32(g["bounds"]), _u32(tw.bounds))
+    assert g["mesh_offsets"][-1] == n
+
+
+def gpu_trace(rc, o, d, tmin=0.0, tmax=math.inf):
+    out = rc.trace_soa(soa_tensors(o, d), tmin=tmin, tmax=tmax)
+    return {k: v.cpu().numpy() for k, v in out.items()}
+
+
+def assert_trace_equal(g, t):
+    np.testing.assert_array_equal(g["prim"], t["prim"])
+    np.testing.assert_array_equal(_u32(g["t"]), _u32(np.where(t["prim"] >= 0, t["t"], np.float32(np.inf))))
+    np.testing.assert_array_equal(_u32(g["u"]), _u32(np.where(t["prim"] >= 0, t["u"], 0)))
+    np.testing.assert_array_equal(_u32(g["v"]), _u32(np.where(t["prim"] >= 0, t["v"], 0)))
+
+
+@pytest.fixture(scope="module")
+def c1(ren):
+    meshes = c1_scene()
+    tw = twin_from_meshes(meshes)
+    rc = gpu_from_meshes(meshes)
+    torch.cuda.synchronize()
+    return meshes, tw, rc
+
+
+def test_build_c1_matches_twin_bytes(c1):
+    meshes, tw, rc = c1
+    assert_build_equal(rc, tw)
+
+
+def test_camera_rays_match_twin(ren, twin):
+    cam = camera(aspect=640 / 480)
+    rays = ren.Raycaster.generate_rays(cam, 640, 480)
+    o, d = twin.twin_generate_rays(cam.as_array(), 640, 480)
+    for i, k in enumerate(("dx", "dy", "dz")):
+        np.testing.assert_array_equal(_u32(rays[k].cpu().numpy()), _u32(d[:, i]))
+    for i, k in enumerate(("ox", "oy", "oz")):
+        np.testing.assert_array_equal(rays[k].cpu().numpy(), o[:, i])
+
+
+def test_trace_c1_primary_matches_twin_and_fp64(c1, twin):
+    meshes, tw, rc = c1
+    cam = camera()
+    o, d = twin.twin_generate_rays(cam.as_array(), 512, 512)
+    g = gpu_trace(rc, o, d)
+    t = tw.trace_closest(o, d)
+    assert_trace_equal(g, t)
+    # FP64: identical IDs except counted edge rays, t within 1e-5 relative
+    r64 = tw.closest64(o, d)
+    mism = g["prim"] != r64["prim"]
+    n_edge_mism = int((mism & (r64["edge"] == 1)).sum())
+    assert int((mism & (r64["edge"] == 0)).sum()) == 0, "non-edge ID mismatches vs FP64"
+    hit = g["prim"] >= 0
+    rel = np.abs(g["t"][hit].astype(np.float64) - r64["t"][hit]) / r64["t"][hit]
+    print(f"C1 primary: hits={hit.sum()} edge_rays={int(r64['edge'].sum())} "
+          f"edge_mismatches={n_edge_mism} t_rel_max={rel.max():.3e} t_rel_mean={rel.mean():.3e}")
+    assert rel.max() <= 1e-5
+
+
+def test_trace_random_rays_match_twin(c1):
+    meshes, tw, rc = c1
+    o, d = random_rays(200_000, 7)
+    assert_trace_equal(gpu_trace(rc, o, d), tw.trace_closest(o, d))
+    # interval limits
+    assert_trace_equal(gpu_trace(rc, o, d, 2.5, 3.2), tw.trace_closest(o, d, 2.5, 3.2))
+
+
+def test_any_hit_matches_twin(c1):
+    meshes, tw, rc = c1
+    o, d = random_rays(100_000, 8)
+    occ = rc.occluded(soa_tensors(o, d)).cpu().numpy()
+    np.testing.assert_array_equal(occ, tw.trace_any(o, d))
+    occ = rc.occluded(soa_tensors(o, d), tmin=0.0, tmax=2.7).cpu().numpy()
+    np.testing.assert_array_equal(occ, tw.trace_any(o, d, 0.0, 2.7))
+
+
+def test_ray_cast_aos_hits(c1, ren):
+    meshes, tw, rc = c1
+    o, d = random_rays(5000, 9)
+    rays = np.zeros(len(o), dtype=ren.Ray)
+    rf = rays.view(np.float32).reshape(-1, 8)
+    rf[:, 0:3] = o
+    rf[:, 4:7] = d
+    hits = rc.ray_cast(ren.create_buffer_from(rays)).get()
+    t = tw.trace_closest(o, d)
+    np.testing.assert_array_equal(hits["index"], t["prim"])
+    np.testing.assert_array_equal(hits["mesh"], np.where(t["prim"] >= 0, 0, -1))
+    np.testing.assert_array_equal(_u32(hits["t"]), _u32(np.where(t["prim"] >= 0, t["t"], np.float32(np.inf))))
+
+
+def test_render_c1_matches_twin(c1, twin):
+    meshes, tw, rc = c1
+    cam = camera()
+    stats = torch.zeros(4, dtype=torch.int32, device="cuda")
+    img, hits = rc.render(cam, 512, 512, light=LIGHT, shadows=True, bilinear=True,
+                          return_hits=True, stats=stats)
+    rgb, t, prim, st = tw.render(cam.as_array(), 512, 512, LIGHT, shadows=True)
+    np.testing.assert_array_equal(hits["prim"].cpu().numpy(), prim)
+    np.testing.assert_array_equal(_u32(img.cpu().numpy()), _u32(rgb))
+    s = stats.cpu().numpy()
+    assert list(s[:3]) == list(st)
+
+
+def test_multi_mesh_nonindexed_int64_and_empty(ren, twin):
+    from rendertoy_amd import scenes
+    a = scenes.icosphere(8, 11)
+    b = scenes.icosphere(4, 12)
+    b["pos"] = (b["pos"] * 0.5 + 0.3).astype(np.float32)
+    empty = {"pos": np.zeros((3, 3), np.float32), "idx": np.zeros((0, 3), np.int32),
+             "nrm": None, "uv": None}
+    meshes = [a, empty, b]
+    tw = twin_from_meshes(meshes)
+    rc_i = gpu_from_meshes(meshes, indexed=True)
+    rc_n = gpu_from_meshes(meshes, indexed=False)
+    assert_build_equal(rc_i, tw)
+    o, d = random_rays(50_000, 13)
+    ti = tw.trace_closest(o, d)
+    gi = gpu_trace(rc_i, o, d)
+    gn = gpu_trace(rc_n, o, d)
+    assert_trace_equal(gi, ti)
+    np.testing.assert_array_equal(gn["prim"], gi["prim"])
+    # (mesh, index) from the global primitive id
+    offs = np.array([0, a["idx"].shape[0], a["idx"].shape[0], a["idx"].shape[0] + b["idx"].shape[0]])
+    hit = gi["prim"] >= 0
+    mesh = np.searchsorted(offs, gi["prim"], side="right") - 1
+    mesh = np.where(mesh == 1, 2, mesh)  # mesh 1 is empty: its offset equals mesh 2's
+    np.testing.assert_array_equal(gi["mesh"][hit], mesh[hit])
+    np.testing.assert_array_equal(gi["index"][hit], (gi["prim"] - offs[mesh])[hit])
+    # int64 index buffer (load_obj style) builds the same scene
+    m64 = ren.mesh_from_arrays(a["pos"], None, a["nrm"], a["uv"])
+    m64.indices = ren.create_buffer_from(a["idx"].reshape(-1).astype(np.int64))
+    rc64 = ren.Raycaster([m64])
+    tw_a = twin_from_meshes([a])
+    assert_build_equal(rc64, tw_a)
+
+
+def test_tiny_scenes(ren, twin):
+    # empty scene: every ray misses
+    rc0 = ren.Raycaster([])
+    o, d = random_rays(1000, 3)
+    g = gpu_trace(rc0, o, d)
+    assert (g["prim"] == -1).all() and np.isinf(g["t"]).all()
+    # one triangle: root is a leaf
+    tri = {"pos": np.array([[-1, -1, 0], [1, -1, 0], [0, 1, 0]], np.float32),
+           "idx": np.array([[0, 1, 2]], np.int32), "nrm": None, "uv": None}
+    tw = twin_from_meshes([tri])
+    rc1 = gpu_from_meshes([tri])
+    assert rc1.root == ~0
+    assert_build_equal(rc1, tw)
+    o = np.array([[0, 0, 3], [0, 0, -3], [5, 5, 3], [0, -1, 3]], np.float32)
+    d = np.array([[0, 0, -1], [0, 0, 1], [0, 0, -1], [0, 0, -1]], np.float32)
+    g = gpu_trace(rc1, o, d)
+    assert list(g["prim"]) == [0, 0, -1, 0]  # front, back face (two-sided), miss, on the edge
+    np.testing.assert_allclose(g["t"][:2], [3.0, 3.0])
+    # two triangles sharing an edge hit at the same t: the smaller primitive id wins
+    quad = {"pos": np.array([[-1, -1, 0], [1, -1, 0], [1, 1, 0], [-1, 1, 0]], np.float32),
+            "idx": np.array([[0, 1, 2], [0, 2, 3]], np.int32), "nrm": None, "uv": None}
+    rcq = gpu_from_meshes([quad])
+    g = gpu_trace(rcq, np.array([[0, 0, 2]], np.float32), np.array([[0, 0, -1]], np.float32))
+    assert g["prim"][0] == 0
+
+
+@pytest.fixture(scope="module")
+def c2(ren):
+    meshes, tex = c2_scene()
+    tw = twin_from_meshes(meshes)
+    rc = gpu_from_meshes(meshes)
+    torch.cuda.synchronize()
+    return meshes, tex, tw, rc
+
+
+def test_build_c2_matches_twin_bytes(c2):
+    meshes, tex, tw, rc = c2
+    assert_build_equal(rc, tw)
+
+
+def test_render_c2_1080p_matches_twin_and_fp64(c2, twin):
+    meshes, tex, tw, rc = c2
+    W, H = 1920, 1080
+    cam = camera(aspect=W / H)
+    texg = torch.from_numpy(tex).cuda()
+    stats = torch.zeros(4, dtype=torch.int32, device="cuda")
+    img, hits = rc.render(cam, W, H, light=LIGHT, textures=[texg], mesh_textures=[0, 0],
+                          shadows=True, bilinear=True, return_hits=True, stats=stats)
+    rgb, t, prim, st = tw.render(cam.as_array(), W, H, LIGHT, shadows=True, bilinear=True,
+                                 textures=[tex], mesh_tex=[0, 0])
+    np.testing.assert_array_equal(hits["prim"].cpu().numpy(), prim)
+    np.testing.assert_array_equal(_u32(hits["t"].cpu().numpy()), _u32(t))
+    np.testing.assert_array_equal(_u32(img.cpu().numpy()), _u32(rgb))
+    assert list(stats.cpu().numpy()[:3]) == list(st)
+    # primary IDs and t against FP64 on a 1/16 sample of the frame
+    o, d = twin.twin_generate_rays(cam.as_array(), W, H)
+    sel = np.arange(0, W * H, 16)
+    r64 = tw.closest64(o[sel], d[sel])
+    mism = prim[sel] != r64["prim"]
+    assert int((mism & (r64["edge"] == 0)).sum()) == 0
+    hit = prim[sel] >= 0
+    rel = np.abs(t[sel][hit].astype(np.float64) - r64["t"][hit]) / r64["t"][hit]
+    print(f"C2 primary (1/16 sample): edge_rays={int(r64['edge'].sum())} "
+          f"edge_mismatches={int(mism.sum())} t_rel_max={rel.max():.3e} mean={rel.mean():.3e}")
+    assert rel.max() <= 1e-5
