@@ -57,9 +57,9 @@ def lib():
         L.tw_render.argtypes = [vp, vp, vp, i32, vp, ctypes.c_uint32, ctypes.c_uint32, vp,
                                 ctypes.c_uint32] + [vp] * 9
         L.tw_closest64_brute.restype = None
-        L.tw_closest64_brute.argtypes = [vp, i64, i64] + [vp] * 6 + [f64, f64] + [vp] * 5
+        L.tw_closest64_brute.argtypes = [vp, i64, i64] + [vp] * 6 + [f64, f64, f64] + [vp] * 5
         L.tw_closest64_bvh.restype = None
-        L.tw_closest64_bvh.argtypes = [vp, i64, vp, vp, i32, i64] + [vp] * 6 + [f64, f64, f64] + [vp] * 5
+        L.tw_closest64_bvh.argtypes = [vp, i64, vp, vp, i32, i64] + [vp] * 6 + [f64, f64, f64, f64] + [vp] * 5
         L.tw_num_threads.restype = ctypes.c_int
         L.tw_set_threads.argtypes = [ctypes.c_int]
         _lib = L
@@ -235,8 +235,8 @@ class TwinScene:
                         _p(stats))
         return rgb, t, prim, stats
 
-    def closest64(self, o, d, tmin=0.0, edge_eps=1e-6, margin=1e-5, brute=False):
-        return closest64(self.pos, o, d, tmin, edge_eps, margin, None if brute else self)
+    def closest64(self, o, d, tmin=0.0, edge_eps=1e-6, tie_eps=1e-5, margin=1e-4, brute=False):
+        return closest64(self.pos, o, d, tmin, edge_eps, tie_eps, margin, None if brute else self)
 
 
 def twin_generate_rays(cam12, W, H):
@@ -246,9 +246,16 @@ def twin_generate_rays(cam12, W, H):
     return np.stack(cols[:3], 1), np.stack(cols[3:], 1)
 
 
-def closest64(tri_pos, o, d, tmin=0.0, edge_eps=1e-6, margin=1e-5, scene=None):
+EDGE, DEPTH_TIE, NEAR_MISS = 1, 2, 4
+
+
+def closest64(tri_pos, o, d, tmin=0.0, edge_eps=1e-6, tie_eps=1e-5, margin=1e-4, scene=None):
     """FP64 closest hit (brute force, or culled by `scene`'s hierarchy with FP64 boxes inflated
-    by `margin`): dict t (float64, inf on miss), prim (-1 on miss), u, v, edge (uint8)."""
+    by `margin`): dict t (float64, inf on miss), prim (-1 on miss), u, v, and `edge`, a bit set of
+    EDGE (winner within max(edge_eps, FP32 error bound) barycentric of an edge), DEPTH_TIE
+    (another hit within max(tie_eps relative, FP32 t error bounds) of the winner), NEAR_MISS (a
+    triangle missed by less than its FP32 barycentric error bound within that range): the rays on
+    which an FP32 Moller-Trumbore evaluation may legitimately differ; 0 elsewhere."""
     tri_pos = np.ascontiguousarray(tri_pos, np.float32)
     o = np.ascontiguousarray(o, np.float32)
     d = np.ascontiguousarray(d, np.float32)
@@ -262,10 +269,10 @@ def closest64(tri_pos, o, d, tmin=0.0, edge_eps=1e-6, margin=1e-5, scene=None):
     edge = np.empty(n, np.uint8)
     if scene is None:
         lib().tw_closest64_brute(_p(tri_pos), tri_pos.shape[0], n, *[_p(c) for c in cols], tmin,
-                                 edge_eps, _p(t), _p(prim), _p(u), _p(v), _p(edge))
+                                 edge_eps, tie_eps, _p(t), _p(prim), _p(u), _p(v), _p(edge))
     else:
         order = np.ascontiguousarray(scene.order if scene.n else np.zeros(1, np.uint32), np.uint32)
         lib().tw_closest64_bvh(_p(tri_pos), tri_pos.shape[0], _p(scene.nodes_full), _p(order),
-                               scene.root, n, *[_p(c) for c in cols], tmin, edge_eps, margin,
-                               _p(t), _p(prim), _p(u), _p(v), _p(edge))
+                               scene.root, n, *[_p(c) for c in cols], tmin, edge_eps, tie_eps,
+                               margin, _p(t), _p(prim), _p(u), _p(v), _p(edge))
     return {"t": t, "prim": prim, "u": u, "v": v, "edge": edge}

@@ -21,8 +21,10 @@
  *      bit-for-bit: LBVH build (Morton codes, stable sort, Karras hierarchy, refit), ray
  *      generation, closest/any-hit traversal with two-sided Moller-Trumbore, shading.
  *  (2) "fp64": a brute-force / BVH-culled FP64 closest hit that shares no arithmetic with (1);
- *      it classifies rays whose winning barycentric lies within `edge_eps` of an edge or whose
- *      runner-up is within `edge_eps` relative distance ("edge rays", reported separately).
+ *      it flags rays on which FP32 may legitimately pick another triangle, using a forward error
+ *      bound of the FP32 Moller-Trumbore operations: the winner lies within max(edge_eps, bound)
+ *      barycentric of an edge, a runner-up is hit within max(tie_eps relative, t bounds) of the
+ *      winner (depth tie), or a triangle is missed by less than its bound in that range.
  *
  * Pinning (tests/test_oracle_*.py): Morton vs closed-form KATs; sort vs numpy stable argsort;
  * LBVH invariants (2n-1 nodes, each leaf once, boxes are exact unions) and brute-force equality;
@@ -382,6 +384,29 @@ static void traverse32(const float* nodes, const float* tris, int32_t root, cons
     }
 }
 
+/* FP64 re-evaluation of the winner's (t, u, v) from its stored FP32 record (the device's
+ * refine_hit, operation for operation). */
+static void refine64(const float* tris, const ray32* r, hit32* h) {
+    const float* tr = tris + (int64_t)h->slot * 12;
+    double dx = r->dx, dy = r->dy, dz = r->dz;
+    double px = fma(dy, (double)tr[10], -(dz * (double)tr[9]));
+    double py = fma(dz, (double)tr[8], -(dx * (double)tr[10]));
+    double pz = fma(dx, (double)tr[9], -(dy * (double)tr[8]));
+    double det = fma((double)tr[4], px, fma((double)tr[5], py, (double)tr[6] * pz));
+    if (det == 0.0) return;
+    double sx = (double)r->ox - (double)tr[0], sy = (double)r->oy - (double)tr[1],
+           sz = (double)r->oz - (double)tr[2];
+    double qx = fma(sy, (double)tr[6], -(sz * (double)tr[5]));
+    double qy = fma(sz, (double)tr[4], -(sx * (double)tr[6]));
+    double qz = fma(sx, (double)tr[5], -(sy * (double)tr[4]));
+    double un = fma(sx, px, fma(sy, py, sz * pz));
+    double vn = fma(dx, qx, fma(dy, qy, dz * qz));
+    double tn = fma((double)tr[8], qx, fma((double)tr[9], qy, (double)tr[10] * qz));
+    h->t = (float)(tn / det);
+    h->u = (float)(un / det);
+    h->v = (float)(vn / det);
+}
+
 static inline void load_ray(const float* ox, const float* oy, const float* oz, const float* dx,
                             const float* dy, const float* dz, const float* tmin,
                             const float* tmax, float tmin_d, float tmax_d, int64_t i, ray32* r) {
@@ -405,6 +430,7 @@ TW_API void tw_trace_closest(const float* nodes, const float* tris, int32_t root
         load_ray(ox, oy, oz, dx, dy, dz, tmin, tmax, tmin_d, tmax_d, i, &r);
         hit32 h;
         traverse32(nodes, tris, root, &r, 0, -1, &h);
+        if (h.prim >= 0) refine64(tris, &r, &h);
         if (h.prim < 0) { h.t = INFINITY; h.u = 0.0f; h.v = 0.0f; }
         if (out_t) out_t[i] = h.t;
         if (out_u) out_u[i] = h.u;
@@ -528,6 +554,7 @@ TW_API void tw_render(const float* nodes, const float* tris, const float* shade,
         ray32 r = {cam[0], cam[1], cam[2], d[0], d[1], d[2], 0.0f, INFINITY};
         hit32 h;
         traverse32(nodes, tris, root, &r, 0, -1, &h);
+        if (h.prim >= 0) refine64(tris, &r, &h);
         if (out_t) out_t[i] = h.prim >= 0 ? h.t : INFINITY;
         if (out_prim) out_prim[i] = h.prim;
         if (h.prim < 0) {
@@ -577,9 +604,19 @@ TW_API void tw_render(const float* nodes, const float* tris, const float* shade,
 /* FP64 oracle (independent arithmetic)                                                         */
 /* ------------------------------------------------------------------------------------------ */
 
-/* Exact-as-possible FP64 Moller-Trumbore on FP32 vertex data (converted exactly to double). */
+/* FP64 Moller-Trumbore on FP32 data (converted exactly to double), with forward error bounds of
+ * the FP32 evaluation of the same formula (mt32): err_* bound |fp32 result - exact result| using
+ * the magnitudes of the intermediate products (unit roundoff 2^-24, safety factor 4). */
+typedef struct {
+    double t, u, v, det;
+    double err_u, err_v, err_w, err_t, err_det;
+} mt64_t;
+
+static inline double norm3(const double* a) { return sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]); }
+
 static inline int mt64(const double* v0, const double* v1, const double* v2, const double* o,
-                       const double* d, double* t, double* u, double* v) {
+                       const double* d, mt64_t* r) {
+    /* exact edges: differences of FP32 values are exact in FP64 */
     double e1[3] = {v1[0] - v0[0], v1[1] - v0[1], v1[2] - v0[2]};
     double e2[3] = {v2[0] - v0[0], v2[1] - v0[1], v2[2] - v0[2]};
     double p[3] = {d[1] * e2[2] - d[2] * e2[1], d[2] * e2[0] - d[0] * e2[2],
@@ -587,54 +624,88 @@ static inline int mt64(const double* v0, const double* v1, const double* v2, con
     double det = e1[0] * p[0] + e1[1] * p[1] + e1[2] * p[2];
     if (det == 0.0) return 0;
     double s[3] = {o[0] - v0[0], o[1] - v0[1], o[2] - v0[2]};
-    double uu = (s[0] * p[0] + s[1] * p[1] + s[2] * p[2]) / det;
     double q[3] = {s[1] * e1[2] - s[2] * e1[1], s[2] * e1[0] - s[0] * e1[2],
                    s[0] * e1[1] - s[1] * e1[0]};
-    double vv = (d[0] * q[0] + d[1] * q[1] + d[2] * q[2]) / det;
-    *t = (e2[0] * q[0] + e2[1] * q[1] + e2[2] * q[2]) / det;
-    *u = uu; *v = vv;
+    double un = s[0] * p[0] + s[1] * p[1] + s[2] * p[2];
+    double vn = d[0] * q[0] + d[1] * q[1] + d[2] * q[2];
+    double tn = e2[0] * q[0] + e2[1] * q[1] + e2[2] * q[2];
+    r->det = det;
+    r->u = un / det;
+    r->v = vn / det;
+    r->t = tn / det;
+    const double eps = 0x1p-24, K = 4.0;
+    double nd = norm3(d), ne1 = norm3(e1), ne2 = norm3(e2), ns = norm3(s);
+    double ad = fabs(det);
+    r->err_det = K * eps * 5.0 * ne1 * nd * ne2;
+    double e_un = K * eps * 6.0 * ns * nd * ne2;
+    double e_vn = K * eps * 6.0 * nd * ns * ne1;
+    double e_tn = K * eps * 6.0 * ne2 * ns * ne1;
+    if (ad <= r->err_det) {
+        r->err_u = r->err_v = r->err_w = r->err_t = INFINITY;
+        return 1;
+    }
+    double den = ad - r->err_det;
+    r->err_u = (e_un + fabs(r->u) * r->err_det) / den + K * eps * fabs(r->u);
+    r->err_v = (e_vn + fabs(r->v) * r->err_det) / den + K * eps * fabs(r->v);
+    r->err_w = r->err_u + r->err_v + K * eps;
+    r->err_t = (e_tn + fabs(r->t) * r->err_det) / den + K * eps * fabs(r->t);
     return 1;
 }
 
 /* tri_pos: n_tris*9 floats in global primitive order (vertex k of prim p at p*9+k*3).
- * For each ray: out_t (+inf on miss), out_prim (-1 on miss), out_u/out_v, out_edge = 1 when the
- * winner's minimum barycentric is < edge_eps, or a different triangle hits within edge_eps
- * relative t of the winner, or a triangle missed by less than edge_eps barycentric lies closer. */
+ * For each ray: out_t (+inf on miss), out_prim (-1 on miss), out_u/out_v, and out_edge, a bit set
+ * of the reasons an FP32 evaluation may legitimately pick a different triangle:
+ *   1  edge:      a barycentric of the winner is below max(edge_eps, its FP32 error bound)
+ *   2  depth tie: another triangle is hit within max(tie_eps * t, the FP32 t error bounds of
+ *                 both) of the winner's t (FP32 cannot order hits closer than its t accuracy)
+ *   4  near miss: a triangle missed by less than max(edge_eps, its FP32 barycentric error bound)
+ *                 lies at a t the FP32 evaluation could prefer (anywhere if nothing is hit) */
 static void closest64_ray(const float* tri_pos, int64_t n_tris, const double* o, const double* d,
-                          double tmin, double edge_eps, const int64_t* cand, int64_t n_cand,
-                          double* bt, int32_t* bp, double* bu, double* bv, uint8_t* edge) {
-    double best = INFINITY, near_miss = INFINITY, second = INFINITY;
+                          double tmin, double edge_eps, double tie_eps, const int64_t* cand,
+                          int64_t n_cand, double* bt, int32_t* bp, double* bu, double* bv,
+                          uint8_t* edge) {
+    double best = INFINITY, best_err = 0.0;
     int32_t prim = -1;
-    double wu = 0, wv = 0;
+    mt64_t w = {0};
     int64_t cnt = cand ? n_cand : n_tris;
+    /* pass 1: the FP64 winner (ties in t: smaller primitive id) */
     for (int64_t k = 0; k < cnt; ++k) {
         int64_t p = cand ? cand[k] : k;
         const float* f = tri_pos + p * 9;
         double v0[3] = {f[0], f[1], f[2]}, v1[3] = {f[3], f[4], f[5]}, v2[3] = {f[6], f[7], f[8]};
-        double t, u, v;
-        if (!mt64(v0, v1, v2, o, d, &t, &u, &v)) continue;
-        if (!(t > tmin)) continue;
-        double mb = fmin(fmin(u, v), 1.0 - u - v);
-        if (mb >= 0.0) {
-            if (t < best || (t == best && (int32_t)p < prim)) {
-                if (prim >= 0) second = fmin(second, best);
-                best = t; prim = (int32_t)p; wu = u; wv = v;
-            } else {
-                second = fmin(second, t);
-            }
-        } else if (mb > -edge_eps) {
-            near_miss = fmin(near_miss, t);
+        mt64_t r;
+        if (!mt64(v0, v1, v2, o, d, &r)) continue;
+        if (!(r.t > tmin)) continue;
+        if (r.u < 0.0 || r.v < 0.0 || r.u + r.v > 1.0) continue;
+        if (r.t < best || (r.t == best && (int32_t)p < prim)) {
+            best = r.t; prim = (int32_t)p; w = r;
         }
     }
-    *bt = best; *bp = prim; *bu = wu; *bv = wv;
+    *bt = best; *bp = prim; *bu = w.u; *bv = w.v;
     uint8_t e = 0;
     if (prim >= 0) {
-        double mb = fmin(fmin(wu, wv), 1.0 - wu - wv);
-        if (mb < edge_eps) e = 1;
-        if (second <= best * (1.0 + edge_eps)) e = 1;
-        if (near_miss <= best * (1.0 + edge_eps)) e = 1;
-    } else if (near_miss < INFINITY) {
-        e = 1;
+        best_err = w.err_t;
+        if (w.u < fmax(edge_eps, w.err_u) || w.v < fmax(edge_eps, w.err_v) ||
+            1.0 - w.u - w.v < fmax(edge_eps, w.err_w))
+            e |= 1;
+    }
+    /* pass 2: competitors the FP32 evaluation cannot separate from the winner */
+    for (int64_t k = 0; k < cnt; ++k) {
+        int64_t p = cand ? cand[k] : k;
+        if ((int32_t)p == prim) continue;
+        const float* f = tri_pos + p * 9;
+        double v0[3] = {f[0], f[1], f[2]}, v1[3] = {f[3], f[4], f[5]}, v2[3] = {f[6], f[7], f[8]};
+        mt64_t r;
+        if (!mt64(v0, v1, v2, o, d, &r)) continue;
+        double reach = prim >= 0 ? best + fmax(tie_eps * best, r.err_t + best_err) : INFINITY;
+        if (!(r.t + r.err_t > tmin) || !(r.t <= reach)) continue;
+        int inside = r.u >= 0.0 && r.v >= 0.0 && r.u + r.v <= 1.0;
+        if (inside) {
+            e |= 2;
+        } else if (r.u > -fmax(edge_eps, r.err_u) && r.v > -fmax(edge_eps, r.err_v) &&
+                   1.0 - r.u - r.v > -fmax(edge_eps, r.err_w)) {
+            e |= 4;
+        }
     }
     *edge = e;
 }
@@ -642,12 +713,12 @@ static void closest64_ray(const float* tri_pos, int64_t n_tris, const double* o,
 TW_API void tw_closest64_brute(const float* tri_pos, int64_t n_tris, int64_t n, const float* ox,
                                const float* oy, const float* oz, const float* dx,
                                const float* dy, const float* dz, double tmin, double edge_eps,
-                               double* out_t, int32_t* out_prim, double* out_u, double* out_v,
+                               double tie_eps, double* out_t, int32_t* out_prim, double* out_u, double* out_v,
                                uint8_t* out_edge) {
     #pragma omp parallel for schedule(dynamic, 16)
     for (int64_t i = 0; i < n; ++i) {
         double o[3] = {ox[i], oy[i], oz[i]}, d[3] = {dx[i], dy[i], dz[i]};
-        closest64_ray(tri_pos, n_tris, o, d, tmin, edge_eps, NULL, 0, &out_t[i], &out_prim[i],
+        closest64_ray(tri_pos, n_tris, o, d, tmin, edge_eps, tie_eps, NULL, 0, &out_t[i], &out_prim[i],
                       &out_u[i], &out_v[i], &out_edge[i]);
     }
 }
@@ -658,8 +729,8 @@ TW_API void tw_closest64_brute(const float* tri_pos, int64_t n_tris, int64_t n, 
 TW_API void tw_closest64_bvh(const float* tri_pos, int64_t n_tris, const float* nodes,
                              const uint32_t* order, int32_t root, int64_t n, const float* ox,
                              const float* oy, const float* oz, const float* dx, const float* dy,
-                             const float* dz, double tmin, double edge_eps, double margin,
-                             double* out_t, int32_t* out_prim, double* out_u, double* out_v,
+                             const float* dz, double tmin, double edge_eps, double tie_eps,
+                             double margin, double* out_t, int32_t* out_prim, double* out_u, double* out_v,
                              uint8_t* out_edge) {
     if (root == 0x7FFFFFFF || n_tris == 0) {
         for (int64_t i = 0; i < n; ++i) {
@@ -738,7 +809,7 @@ TW_API void tw_closest64_bvh(const float* tri_pos, int64_t n_tris, const float* 
                     cand[nc++] = order[~x];
                 }
             }
-            closest64_ray(tri_pos, n_tris, o, d, tmin, edge_eps, cand, nc, &out_t[i],
+            closest64_ray(tri_pos, n_tris, o, d, tmin, edge_eps, tie_eps, cand, nc, &out_t[i],
                           &out_prim[i], &out_u[i], &out_v[i], &out_edge[i]);
         }
         free(cand); free(stk);

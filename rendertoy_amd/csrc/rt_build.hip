@@ -334,8 +334,10 @@ __global__ void __launch_bounds__(kSortThreads) k_sort_scatter(
         if (valid) {
             uint32_t off = base[d] + run[d] + rank;
             for (int w = 0; w < wave; ++w) off += wcount[w][d];
-            kout[off] = key;
-            vout[off] = val;
+            if (off < n) {  // always true for a consistent histogram; keeps writes in bounds
+                kout[off] = key;
+                vout[off] = val;
+            }
         }
         __syncthreads();
         uint32_t add = 0;

@@ -66,10 +66,30 @@ RT_DEV bool intersect_tri(const float4 a, const float4 b, const float4 c, const 
     return true;
 }
 
-// Per-lane traversal stack in private memory (64 entries, see kStackDepth).
-struct Stack {
-    int32_t e[kStackDepth];
-};
+// FP64 re-evaluation of the winning triangle's (t, u, v): the selection stays FP32 (closest FP32
+// Moller-Trumbore t), the reported hit is accurate to FP64 of the stored record. FP32 t degrades
+// on sliver triangles far from the origin; this keeps the reported distance within 1e-5 relative.
+RT_DEV void refine_hit(const float4* __restrict__ tris, const Ray& r, Hit& h) {
+    const float4* tr = tris + (size_t)h.slot * 3;
+    const float4 a = tr[0], b = tr[1], c = tr[2];
+    const double dx = r.dx, dy = r.dy, dz = r.dz;
+    const double px = fma(dy, (double)c.z, -(dz * (double)c.y));
+    const double py = fma(dz, (double)c.x, -(dx * (double)c.z));
+    const double pz = fma(dx, (double)c.y, -(dy * (double)c.x));
+    const double det = fma((double)b.x, px, fma((double)b.y, py, (double)b.z * pz));
+    if (det == 0.0) return;
+    const double sx = (double)r.ox - (double)a.x, sy = (double)r.oy - (double)a.y,
+                 sz = (double)r.oz - (double)a.z;
+    const double qx = fma(sy, (double)b.z, -(sz * (double)b.y));
+    const double qy = fma(sz, (double)b.x, -(sx * (double)b.z));
+    const double qz = fma(sx, (double)b.y, -(sy * (double)b.x));
+    const double un = fma(sx, px, fma(sy, py, sz * pz));
+    const double vn = fma(dx, qx, fma(dy, qy, dz * qz));
+    const double tn = fma((double)c.x, qx, fma((double)c.y, qy, (double)c.z * qz));
+    h.t = (float)(tn / det);
+    h.u = (float)(un / det);
+    h.v = (float)(vn / det);
+}
 
 // Closest-hit (ANY=false) or any-hit (ANY=true) traversal of the two-child node layout, nearer
 // child first. `skip_slot` is never tested (the shadow ray's own triangle). On return h.prim < 0

@@ -60,6 +60,7 @@ __global__ void __launch_bounds__(256) k_trace_closest(SceneArgs sc, rt_rays ray
     r.tmax = rays.tmax ? ld_f(rays.tmax, i, rays.stride) : rays.tmax_default;
     Hit h;
     traverse<false>(sc.nodes, sc.tris, sc.root, r, -1, h);
+    if (h.prim >= 0) refine_hit(sc.tris, r, h);
     const bool hit = h.prim >= 0;
     int32_t mesh = -1, index = -1;
     if (hit) {
@@ -170,6 +171,7 @@ __global__ void __launch_bounds__(256) k_render_primary(SceneArgs sc, rt_camera 
         r.tmin = 0.0f;
         r.tmax = INFINITY;
         traverse<false>(sc.nodes, sc.tris, sc.root, r, -1, h);
+        if (h.prim >= 0) refine_hit(sc.tris, r, h);
     }
     const bool hit = valid && h.prim >= 0;
     const uint64_t pix = (uint64_t)row * W + col;
@@ -209,6 +211,7 @@ __global__ void __launch_bounds__(256) k_trace_camera(SceneArgs sc, rt_camera ca
     r.tmax = INFINITY;
     Hit h;
     traverse<false>(sc.nodes, sc.tris, sc.root, r, -1, h);
+    if (h.prim >= 0) refine_hit(sc.tris, r, h);
     const bool hit = h.prim >= 0;
     const uint64_t i = (uint64_t)row * W + col;
     int32_t mesh = -1, index = -1;
@@ -239,6 +242,7 @@ RT_DEV float4 texel(const uint8_t* __restrict__ tex, int W, int x, int y) {
 // reference's wrap rule (rendering/_core.py:94-96).
 RT_DEV void tex_sample(const rt_texture& tx, float u, float v, bool bilinear, float* rgb) {
     const int W = (int)tx.width, H = (int)tx.height;
+    if (W <= 0 || H <= 0 || !tx.texels) return;  // no texels: albedo stays 1
     const float su = u - floorf(u), sv = v - floorf(v);
     if (!bilinear) {
         int i = (int)(su * (float)W), j = (int)(sv * (float)H);
