@@ -11,10 +11,11 @@ value = rays / step time in Mrays/s. Also reported: LBVH build time, and the ful
 (primary + shadow ray + Lambert + bilinear RGBA8 texture) latency p50/p99 over the same count of
 frames.
 
-roofline: algorithmic bytes per launch = sum over rays of (64 B per node visit + 48 B per triangle
-test) + 16 B of hit output per ray; the visit/test counts are those of the CPU restatement's
-traversal of the identical BVH in the identical order (oracle/rt_twin.c), divided by the kernel's
-mean duration from HIP events on the launch stream.
+roofline: algorithmic bytes per launch = 128 B per 4-wide node record + 48 B per triangle record
+fetched per wave (the kernel's wave-coherent traversal fetches each record once per 64-ray wave), 48 B per
+hit for the FP64 refinement re-read, 16 B of hit output per ray; the fetch counts come from the CPU
+restatement's lane-exact emulation of the kernel on the identical BVH (oracle/rt_twin.c), divided
+by the kernel's mean duration from HIP events on the launch stream.
 cpu_baseline: the same closest-hit traversal in the CPU restatement (C + OpenMP, all host
 threads) over full frames for about `--cpu-seconds`.
 """
@@ -42,6 +43,9 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-roofline", action="store_true")
+    ap.add_argument("--copy-calib-mib", type=int, default=0,
+                    help="first run 3 device-to-device copies of this many MiB (a known-traffic "
+                         "kernel that scales the FETCH_SIZE/WRITE_SIZE counters of a profile)")
     return ap.parse_args()
 
 
@@ -62,6 +66,14 @@ def main():
 
     W, H = args.width, args.height
     n_rays = W * H
+    if args.copy_calib_mib:
+        nb = args.copy_calib_mib << 20
+        src = torch.ones(nb, dtype=torch.uint8, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     t0 = time.time()
     sphere = scenes.icosphere(args.freq, 2000)
     plane = scenes.ground_plane()
@@ -165,23 +177,27 @@ def main():
         tw_t0 = time.time()
         pos, nrm, uv, mo = scenes.triangle_arrays(meshes)
         tw = oracle.TwinScene(pos, nrm, uv, mo, [1, 1], [1, 1])
-        o, d = oracle.twin_generate_rays(cam.as_array(), W, H)
-        res = tw.trace_closest(o, d, counters=True)
         if not args.no_roofline:
-            visits = res["node_visits"].astype(np.int64).sum()
-            tests = res["tri_tests"].astype(np.int64).sum()
-            algo_bytes = 64 * visits + 48 * tests + 16 * n_rays
+            # records fetched per wave by the device kernel, from its lane-exact emulation
+            wv = tw.trace_camera(cam.as_array(), W, H)
+            nodes_f = int(wv["wave_nodes"].sum())
+            tris_f = int(wv["wave_tris"].sum())
+            twin_hits = int((wv["prim"] >= 0).sum())
+            algo_bytes = 128 * nodes_f + 48 * tris_f + 48 * twin_hits + 16 * n_rays
             achieved = algo_bytes / (ms_per_step * 1e-3) / 1e9
             result["roofline"] = {
-                "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
                 "algorithmic_bytes_per_launch": int(algo_bytes),
-                "mean_node_visits": round(float(visits / n_rays), 3),
-                "mean_tri_tests": round(float(tests / n_rays), 3),
+                "bytes_model": "128 B per 4-wide node record + 48 B per triangle record fetched "
+                               "per wave, 48 B per hit (FP64 refinement re-read), 16 B hit "
+                               "output per ray",
+                "wave_node_fetches": nodes_f, "wave_triangle_fetches": tris_f,
                 "kernel": "k_trace_camera",
             }
-            result["parity_primary_ids_equal_twin"] = bool(np.array_equal(prim, res["prim"]))
+            result["parity_primary_ids_equal_twin"] = bool(np.array_equal(prim, wv["prim"]))
         if not args.no_cpu_baseline:
+            o, d = oracle.twin_generate_rays(cam.as_array(), W, H)
             threads = len(os.sched_getaffinity(0))
             if os.environ.get("OMP_NUM_THREADS", "").isdigit():
                 threads = min(threads, int(os.environ["OMP_NUM_THREADS"]))
@@ -196,8 +212,8 @@ def main():
             result["cpu_baseline"] = {
                 "value": round(reps * n_rays / cpu_s / 1e6, 3), "unit": "Mrays/s",
                 "cores": threads, "kind": "port",
-                "sample": f"{reps} full {W}x{H} frames of primary closest hit, same BVH, "
-                          f"{cpu_s:.1f} s on {threads} host threads",
+                "sample": f"{reps} full {W}x{H} frames of primary closest hit (per-ray stack "
+                          f"traversal of the same BVH), {cpu_s:.1f} s on {threads} host threads",
             }
         result["oracle_s"] = round(time.time() - tw_t0, 2)
     print(json.dumps(result), flush=True)

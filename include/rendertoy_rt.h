@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define RT_ABI_VERSION 1
+#define RT_ABI_VERSION 2
 
 enum {
     RT_OK = 0,
@@ -56,13 +56,17 @@ typedef struct rt_mesh_desc {
     uint32_t uv_offset;       /* byte offset of float2 C (texture coordinate), or RT_ABSENT */
 } rt_mesh_desc;
 
-/* A built scene: device pointers owned by the caller, sizes from rt_scene_sizes(). */
+/* A built scene: device pointers owned by the caller, sizes from rt_scene_sizes_for().
+ * Node references: >= 0 an internal node id; < 0 a leaf of count = ((~ref) >> 29) + 1 consecutive
+ * triangle slots starting at (~ref) & 0x1FFFFFFF (subtrees of <= 4 triangles are leaves);
+ * RT_EMPTY_ROOT: no triangles. */
 typedef struct rt_scene {
-    uint32_t n_tris;
+    uint32_t n_tris;          /* at most 2^29 - 1 */
     uint32_t n_meshes;
-    int32_t root;             /* >= 0: internal node id; < 0: leaf ~slot; RT_EMPTY_ROOT: no triangles */
+    int32_t root;
     uint32_t _reserved;
-    void* nodes;              /* device, max(n_tris-1,1) * 64 bytes (4 x float4 per node) */
+    void* nodes;              /* device, max(n_tris-1,1) * 64 bytes: binary nodes (4 x float4) */
+    void* nodes4;             /* device, max(n_tris-1,1) * 128 bytes: 4-wide nodes (8 x float4) */
     void* tris;               /* device, n_tris * 48 bytes (v0|prim, e1, e2 as float4) */
     void* shade;              /* device, n_tris * 64 bytes (N0,N1,N2,UV0,UV1,UV2,mesh) */
     int32_t* mesh_offsets;    /* device, n_meshes+1 prefix sums of triangle counts */
@@ -73,7 +77,7 @@ typedef struct rt_scene {
 
 /* Per-call byte sizes of the scene arrays and of the build workspace for n_tris triangles. */
 typedef struct rt_scene_sizes {
-    uint64_t nodes_bytes, tris_bytes, shade_bytes, bounds_bytes, workspace_bytes;
+    uint64_t nodes_bytes, nodes4_bytes, tris_bytes, shade_bytes, bounds_bytes, workspace_bytes;
 } rt_scene_sizes;
 
 /* Rays as field pointers plus one byte stride shared by all fields: SoA arrays use stride 4,

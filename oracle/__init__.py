@@ -44,13 +44,15 @@ def lib():
         L.tw_morton3.restype = ctypes.c_uint32
         L.tw_morton3.argtypes = [ctypes.c_uint32] * 3
         L.tw_build.restype = ctypes.c_int
-        L.tw_build.argtypes = [i64] + [vp] * 6 + [vp] * 6 + [vp]
+        L.tw_build.argtypes = [i64] + [vp] * 6 + [vp] * 7 + [vp]
         L.tw_trace_closest.restype = None
         L.tw_trace_closest.argtypes = [vp, vp, i32, i64] + [vp] * 8 + [f32, f32] + [vp] * 6
         L.tw_trace_any.restype = None
         L.tw_trace_any.argtypes = [vp, vp, i32, i64] + [vp] * 8 + [f32, f32, vp]
         L.tw_brute_closest32.restype = None
         L.tw_brute_closest32.argtypes = [vp, i64, i64] + [vp] * 6 + [f32, f32, vp, vp]
+        L.tw_trace_camera.restype = None
+        L.tw_trace_camera.argtypes = [vp, vp, i32, vp, ctypes.c_uint32, ctypes.c_uint32] + [vp] * 8
         L.tw_generate_rays.restype = None
         L.tw_generate_rays.argtypes = [vp, ctypes.c_uint32, ctypes.c_uint32] + [vp] * 6
         L.tw_render.restype = None
@@ -146,6 +148,7 @@ class TwinScene:
         self.keys = np.zeros(max(n, 1), np.uint32)
         self.order = np.zeros(max(n, 1), np.uint32)
         self.nodes = np.zeros((max(n - 1, 1), 16), np.float32)
+        self.nodes4 = np.zeros((max(n - 1, 1), 32), np.float32)
         self.tris = np.zeros((max(n, 1), 12), np.float32)
         self.shade = np.zeros((max(n, 1), 16), np.float32)
         self.bounds = np.zeros(8, np.float32)
@@ -155,7 +158,7 @@ class TwinScene:
         rc = lib().tw_build(n, _p(self.pos), _p(np.ascontiguousarray(nrm, np.float32)),
                             _p(np.ascontiguousarray(uv, np.float32)),
                             _p(np.ascontiguousarray(mesh_of, np.int32)), _p(hn), _p(hu),
-                            _p(self.keys), _p(self.order), _p(self.nodes), _p(self.tris),
+                            _p(self.keys), _p(self.order), _p(self.nodes), _p(self.nodes4), _p(self.tris),
                             _p(self.shade), _p(self.bounds), ctypes.byref(root))
         if rc != 0:
             raise MemoryError("tw_build failed")
@@ -163,10 +166,15 @@ class TwinScene:
         self.keys, self.order = self.keys[:n], self.order[:n]
         self.tris, self.shade = self.tris[:n], self.shade[:n]
         self.nodes = self.nodes[:max(n - 1, 0)]
+        self.nodes4 = self.nodes4[:max(n - 1, 0)]
 
     @property
     def nodes_full(self):
         return self.nodes if self.nodes.shape[0] else np.zeros((1, 16), np.float32)
+
+    @property
+    def nodes4_full(self):
+        return self.nodes4 if self.nodes4.shape[0] else np.zeros((1, 32), np.float32)
 
     def trace_closest(self, o, d, tmin=0.0, tmax=np.inf, counters=False):
         o = np.ascontiguousarray(o, np.float32)
@@ -187,6 +195,25 @@ class TwinScene:
         if counters:
             out["node_visits"] = cnt[:, 0]
             out["tri_tests"] = cnt[:, 1]
+        return out
+
+    def trace_camera(self, cam12, W, H):
+        """Camera closest hit with the device's wave-coherent traversal over 8x8 tiles: dict of
+        per-pixel t, u, v, prim, slot plus per-tile wave_nodes / wave_leaves / wave_tris (4-wide
+        node records, leaves and triangle records fetched by the wave)."""
+        n = W * H
+        out = {"t": np.empty(n, np.float32), "u": np.empty(n, np.float32),
+               "v": np.empty(n, np.float32), "prim": np.empty(n, np.int32),
+               "slot": np.empty(n, np.int32)}
+        ntiles = ((W + 7) // 8) * ((H + 7) // 8)
+        out["wave_nodes"] = np.empty(ntiles, np.int64)
+        out["wave_leaves"] = np.empty(ntiles, np.int64)
+        out["wave_tris"] = np.empty(ntiles, np.int64)
+        lib().tw_trace_camera(_p(self.nodes4_full), _p(self.tris if self.n else np.zeros((1, 12), np.float32)),
+                              self.root, _p(np.ascontiguousarray(cam12, np.float32)), W, H,
+                              _p(out["t"]), _p(out["u"]), _p(out["v"]), _p(out["prim"]),
+                              _p(out["slot"]), _p(out["wave_nodes"]), _p(out["wave_leaves"]),
+                              _p(out["wave_tris"]))
         return out
 
     def trace_any(self, o, d, tmin=0.0, tmax=np.inf):
@@ -229,7 +256,8 @@ class TwinScene:
         else:
             texcat, offs, tw, th = np.zeros(4, np.uint8), np.zeros(1, np.int64), np.ones(1, np.int32), np.ones(1, np.int32)
         mt = None if mesh_tex is None else np.ascontiguousarray(mesh_tex, np.int32)
-        lib().tw_render(_p(self.nodes_full), _p(self.tris), _p(self.shade), self.root,
+        lib().tw_render(_p(self.nodes4_full), _p(self.tris if self.n else np.zeros((1, 12), np.float32)),
+                        _p(self.shade if self.n else np.zeros((1, 16), np.float32)), self.root,
                         _p(np.ascontiguousarray(cam12, np.float32)), W, H, _p(params), flags,
                         _p(texcat), _p(offs), _p(tw), _p(th), _p(mt), _p(rgb), _p(t), _p(prim),
                         _p(stats))

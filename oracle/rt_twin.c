@@ -69,11 +69,12 @@ static inline uint32_t quantize10(float c, float lo, float scale) {
 /* Per-triangle preparation (the device gather kernel)                                          */
 /* ------------------------------------------------------------------------------------------ */
 
-/* Pads a box by |coordinate| * 2^-18 + 2^-40 per side so that the box test can never cull a
- * triangle that the FP32 Moller-Trumbore test accepts. */
+/* Pads a box by |coordinate| * 2^-14 + 2^-40 per side: wider than the world-space error of an
+ * FP32 Moller-Trumbore acceptance, so the box test does not cull a triangle that the FP32 test
+ * would accept and the closest hit does not depend on the traversal order. */
 static inline float pad_of(float lo, float hi) {
     float m = fmaxf(fabsf(lo), fabsf(hi));
-    return m * 0x1p-18f + 0x1p-40f;
+    return m * 0x1p-14f + 0x1p-40f;
 }
 
 /* Inputs are flattened per global primitive (mesh-major order):
@@ -99,6 +100,14 @@ static inline float canon(float x) { return x + 0.0f; } /* -0 -> +0 */
 static inline uint32_t fbits(float f) { uint32_t u; memcpy(&u, &f, 4); return u; }
 static inline float bitsf(uint32_t u) { float f; memcpy(&f, &u, 4); return f; }
 
+/* Leaves: subtrees of at most LEAF_MAX consecutive slots, referenced as ~(first | (count-1) << 29). */
+#define LEAF_MAX 4
+static inline int32_t leaf_ref(uint32_t first, uint32_t count) {
+    return ~(int32_t)(first | ((count - 1u) << 29));
+}
+static inline uint32_t leaf_first(int32_t ref) { return (uint32_t)(~ref) & 0x1FFFFFFFu; }
+static inline uint32_t leaf_count(int32_t ref) { return ((uint32_t)(~ref) >> 29) + 1u; }
+
 static int delta_k(const uint32_t* k, int64_t n, int64_t i, int64_t j) {
     if (j < 0 || j >= n) return -1;
     uint32_t a = k[i], b = k[j];
@@ -113,8 +122,8 @@ static int cmp_u64(const void* a, const void* b) {
 
 TW_API int tw_build(int64_t n, const float* pos, const float* nrm, const float* uv,
                     const int32_t* mesh_of, const uint8_t* has_nrm, const uint8_t* has_uv,
-                    uint32_t* keys, uint32_t* order, float* nodes, float* tris, float* shade,
-                    float* bounds, int32_t* root_out) {
+                    uint32_t* keys, uint32_t* order, float* nodes, float* nodes4, float* tris,
+                    float* shade, float* bounds, int32_t* root_out) {
     if (n <= 0) {
         for (int c = 0; c < 3; ++c) { bounds[c] = INFINITY; bounds[4 + c] = -INFINITY; }
         bounds[3] = bounds[7] = 0.0f;
@@ -218,7 +227,9 @@ TW_API int tw_build(int64_t n, const float* pos, const float* nrm, const float* 
     /* Karras 2012 binary radix tree over the 62-bit extended keys (code, slot) */
     int32_t* left = (int32_t*)malloc(sizeof(int32_t) * (n - 1));
     int32_t* right = (int32_t*)malloc(sizeof(int32_t) * (n - 1));
-    if (!left || !right) return -1;
+    int32_t* tleft = (int32_t*)malloc(sizeof(int32_t) * (n - 1));
+    int32_t* tright = (int32_t*)malloc(sizeof(int32_t) * (n - 1));
+    if (!left || !right || !tleft || !tright) return -1;
     #pragma omp parallel for schedule(static)
     for (int64_t i = 0; i < n - 1; ++i) {
         int d = (delta_k(keys, n, i, i + 1) - delta_k(keys, n, i, i - 1)) >= 0 ? 1 : -1;
@@ -241,6 +252,9 @@ TW_API int tw_build(int64_t n, const float* pos, const float* nrm, const float* 
         int64_t lo_ = i < j ? i : j, hi_ = i < j ? j : i;
         left[i] = (lo_ == gamma) ? ~(int32_t)gamma : (int32_t)gamma;
         right[i] = (hi_ == gamma + 1) ? ~(int32_t)(gamma + 1) : (int32_t)(gamma + 1);
+        int64_t nl = gamma - lo_ + 1, nr = hi_ - gamma;
+        tleft[i] = nl <= LEAF_MAX ? leaf_ref((uint32_t)lo_, (uint32_t)nl) : (int32_t)gamma;
+        tright[i] = nr <= LEAF_MAX ? leaf_ref((uint32_t)(gamma + 1), (uint32_t)nr) : (int32_t)(gamma + 1);
     }
 
     /* refit: post-order over the tree, child boxes stored in the parent */
@@ -265,7 +279,7 @@ TW_API int tw_build(int64_t n, const float* pos, const float* nrm, const float* 
         nd[0] = b0.lo[0]; nd[1] = b0.hi[0]; nd[2] = b0.lo[1]; nd[3] = b0.hi[1];
         nd[4] = b1.lo[0]; nd[5] = b1.hi[0]; nd[6] = b1.lo[1]; nd[7] = b1.hi[1];
         nd[8] = b0.lo[2]; nd[9] = b0.hi[2]; nd[10] = b1.lo[2]; nd[11] = b1.hi[2];
-        nd[12] = bitsf((uint32_t)left[x]); nd[13] = bitsf((uint32_t)right[x]);
+        nd[12] = bitsf((uint32_t)tleft[x]); nd[13] = bitsf((uint32_t)tright[x]);
         nd[14] = 0.0f; nd[15] = 0.0f;
         box3 u;
         for (int c = 0; c < 3; ++c) {
@@ -276,8 +290,40 @@ TW_API int tw_build(int64_t n, const float* pos, const float* nrm, const float* 
     }
     for (int c = 0; c < 3; ++c) { bounds[c] = nb[0].lo[c]; bounds[4 + c] = nb[0].hi[c]; }
     bounds[3] = bounds[7] = 0.0f;
-    *root_out = 0;
-    free(stack); free(state); free(nb); free(left); free(right);
+    *root_out = n <= LEAF_MAX ? leaf_ref(0, (uint32_t)n) : 0;
+    /* 4-wide records: grandchildren through internal children, leaf children directly */
+    #pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < n - 1; ++i) {
+        float lo[3][4], hi[3][4];
+        int32_t ref[4];
+        int k = 0;
+        const float* nd = nodes + i * 16;
+        int32_t c[2] = {(int32_t)fbits(nd[12]), (int32_t)fbits(nd[13])};
+        float cb[2][6] = {{nd[0], nd[2], nd[8], nd[1], nd[3], nd[9]},
+                          {nd[4], nd[6], nd[10], nd[5], nd[7], nd[11]}};
+        for (int e = 0; e < 2; ++e) {
+            if (c[e] >= 0) {
+                const float* cd = nodes + (int64_t)c[e] * 16;
+                lo[0][k] = cd[0]; hi[0][k] = cd[1]; lo[1][k] = cd[2]; hi[1][k] = cd[3];
+                lo[2][k] = cd[8]; hi[2][k] = cd[9]; ref[k] = (int32_t)fbits(cd[12]); ++k;
+                lo[0][k] = cd[4]; hi[0][k] = cd[5]; lo[1][k] = cd[6]; hi[1][k] = cd[7];
+                lo[2][k] = cd[10]; hi[2][k] = cd[11]; ref[k] = (int32_t)fbits(cd[13]); ++k;
+            } else {
+                for (int a = 0; a < 3; ++a) { lo[a][k] = cb[e][a]; hi[a][k] = cb[e][3 + a]; }
+                ref[k] = c[e]; ++k;
+            }
+        }
+        for (; k < 4; ++k) {
+            for (int a = 0; a < 3; ++a) { lo[a][k] = INFINITY; hi[a][k] = -INFINITY; }
+            ref[k] = 0x7FFFFFFF;
+        }
+        float* o = nodes4 + i * 32;
+        for (int a = 0; a < 3; ++a)
+            for (int q = 0; q < 4; ++q) { o[8 * a + q] = lo[a][q]; o[8 * a + 4 + q] = hi[a][q]; }
+        for (int q = 0; q < 4; ++q) o[24 + q] = bitsf((uint32_t)ref[q]);
+        for (int q = 0; q < 4; ++q) o[28 + q] = 0.0f;
+    }
+    free(stack); free(state); free(nb); free(left); free(right); free(tleft); free(tright);
     free(leafbox); free(cen); free(tri_u); free(sh_u);
     return 0;
 }
@@ -365,8 +411,10 @@ static void traverse32(const float* nodes, const float* tris, int32_t root, cons
             if (hit0) { cur = r0; continue; }
             if (hit1) { cur = r1; continue; }
         } else {
-            int32_t slot = ~cur;
-            if (slot != skip_slot) {
+            uint32_t first = leaf_first(cur), count = leaf_count(cur);
+            for (uint32_t k = 0; k < count; ++k) {
+                int32_t slot = (int32_t)(first + k);
+                if (slot == skip_slot) continue;
                 const float* tr = tris + (int64_t)slot * 12;
                 float t, u, v;
                 h->tri_tests++;
@@ -536,66 +584,244 @@ static inline void tex_sample(const uint8_t* tex, int W, int H, float u, float v
     }
 }
 
-/* Full FP32 render of the device path: primary closest hit, Lambert with shadow ray, texture.
+/* ------------------------------------------------------------------------------------------ */
+/* Wave-coherent traversal (the device traverse_packet), one 64-lane wave emulated lane by lane */
+/* ------------------------------------------------------------------------------------------ */
+
+typedef struct {
+    int64_t node_visits, leaf_visits, tri_fetches;  /* per wave: 4-wide node records, leaves and
+                                                        triangle records fetched */
+} wavecount;
+
+static void packet32(const float* nodes4, const float* tris, int32_t root, const ray32* r,
+                     const int* want, const int32_t* skip, int any, hit32* h, wavecount* wc) {
+    uint64_t m = 0;
+    float idx[64], idy[64], idz[64], oxx[64], oyy[64], ozz[64];
+    for (int l = 0; l < 64; ++l) {
+        h[l].t = r[l].tmax; h[l].u = 0.0f; h[l].v = 0.0f; h[l].prim = -1; h[l].slot = -1;
+        h[l].node_visits = 0; h[l].tri_tests = 0;
+        if (want[l]) m |= 1ull << l;
+        idx[l] = safe_inv(r[l].dx); idy[l] = safe_inv(r[l].dy); idz[l] = safe_inv(r[l].dz);
+        oxx[l] = r[l].ox * idx[l]; oyy[l] = r[l].oy * idy[l]; ozz[l] = r[l].oz * idz[l];
+    }
+    if (root == 0x7FFFFFFF || m == 0) return;
+    uint64_t alive = m;
+    int32_t stk_node[128];
+    uint64_t stk_m[128];
+    int sp = 0;
+    int32_t cur = root;
+    for (;;) {
+        if (cur >= 0) {
+            const float* nd = nodes4 + (int64_t)cur * 32;
+            if (wc) wc->node_visits++;
+            int rep = __builtin_ctzll(m);
+            float key[4];
+            uint64_t mk[4];
+            int32_t ref[4];
+            for (int k = 0; k < 4; ++k) {
+                ref[k] = (int32_t)fbits(nd[24 + k]);
+                mk[k] = 0;
+                key[k] = INFINITY;
+            }
+            for (int l = 0; l < 64; ++l) {
+                if (!((m >> l) & 1)) continue;
+                h[l].node_visits++;
+                for (int k = 0; k < 4; ++k) {
+                    float x0 = fmaf(nd[k], idx[l], -oxx[l]), x1 = fmaf(nd[4 + k], idx[l], -oxx[l]);
+                    float y0 = fmaf(nd[8 + k], idy[l], -oyy[l]), y1 = fmaf(nd[12 + k], idy[l], -oyy[l]);
+                    float z0 = fmaf(nd[16 + k], idz[l], -ozz[l]), z1 = fmaf(nd[20 + k], idz[l], -ozz[l]);
+                    float tn = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), r[l].tmin));
+                    float tf = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fmaxf(z0, z1)) * TW_TFAR_SCALE;
+                    int hk = ref[k] != 0x7FFFFFFF && tn <= fminf(tf, h[l].t);
+                    if (hk) mk[k] |= 1ull << l;
+                    if (l == rep) key[k] = hk ? tn : INFINITY;
+                }
+            }
+            /* order by (key, k) with the device's sorting network */
+            int ord[4] = {0, 1, 2, 3};
+            const int net[5][2] = {{0, 1}, {2, 3}, {0, 2}, {1, 3}, {1, 2}};
+            for (int e = 0; e < 5; ++e) {
+                int a = net[e][0], b = net[e][1];
+                int ka = ord[a], kb = ord[b];
+                if (key[kb] < key[ka] || (key[kb] == key[ka] && kb < ka)) { ord[a] = kb; ord[b] = ka; }
+            }
+            int32_t next = -1;
+            uint64_t next_m = 0;
+            for (int q = 3; q >= 0; --q) {
+                int k = ord[q];
+                if (!mk[k]) continue;
+                if (next_m && sp < 128) { stk_node[sp] = next; stk_m[sp] = next_m; ++sp; }
+                next = ref[k];
+                next_m = mk[k];
+            }
+            if (next_m) { cur = next; m = next_m; continue; }
+        } else {
+            uint32_t first = leaf_first(cur), count = leaf_count(cur);
+            if (wc) { wc->leaf_visits++; wc->tri_fetches += count; }
+            for (uint32_t k = 0; k < count; ++k) {
+                int32_t slot = (int32_t)(first + k);
+                const float* tr = tris + (int64_t)slot * 12;
+                for (int l = 0; l < 64; ++l) {
+                    if (!((m >> l) & 1) || slot == skip[l]) continue;
+                    float t, u, v;
+                    h[l].tri_tests++;
+                    if (mt32(tr, &r[l], &t, &u, &v) && t > r[l].tmin) {
+                        int32_t prim = (int32_t)fbits(tr[3]);
+                        if (t < h[l].t || (t == h[l].t && prim < h[l].prim)) {
+                            h[l].t = t; h[l].u = u; h[l].v = v; h[l].prim = prim; h[l].slot = slot;
+                        }
+                    }
+                }
+            }
+            if (any) {
+                for (int l = 0; l < 64; ++l)
+                    if (h[l].prim >= 0) alive &= ~(1ull << l);
+                if (!alive) return;
+            }
+        }
+        for (;;) {
+            if (sp == 0) return;
+            --sp;
+            cur = stk_node[sp];
+            m = stk_m[sp];
+            if (any) m &= alive;
+            if (m) break;
+        }
+    }
+}
+
+/* The device's wave tiling: a 256-thread block covers 16x16 pixels, wave w of the block the 8x8
+ * sub-tile ((w&1)*8, (w>>1)*8), lane l the pixel (l&7, l>>3) of it; every 8x8-aligned tile is one
+ * wave. */
+static inline void tile_lane(int64_t tile, uint32_t W, int64_t* col, int64_t* row, int l) {
+    int64_t tiles_x8 = ((int64_t)W + 7) / 8;
+    *col = (tile % tiles_x8) * 8 + (l & 7);
+    *row = (tile / tiles_x8) * 8 + (l >> 3);
+}
+
+/* Camera closest hit with the device's wave-coherent traversal (rt_trace_camera). Per-pixel
+ * outputs (may be NULL); wave_nodes / wave_leaves / wave_tris (may be NULL): 4-wide node records,
+ * leaves and triangle records fetched per 8x8 tile (the device kernel's traffic and step count). */
+TW_API void tw_trace_camera(const float* nodes4, const float* tris, int32_t root, const float* cam,
+                            uint32_t W, uint32_t H, float* out_t, float* out_u, float* out_v,
+                            int32_t* out_prim, int32_t* out_slot, int64_t* wave_nodes,
+                            int64_t* wave_leaves, int64_t* wave_tris) {
+    int64_t ntiles = (((int64_t)W + 7) / 8) * (((int64_t)H + 7) / 8);
+    #pragma omp parallel for schedule(dynamic, 16)
+    for (int64_t tile = 0; tile < ntiles; ++tile) {
+        ray32 r[64];
+        int want[64];
+        int32_t skip[64];
+        hit32 h[64];
+        int64_t col[64], row[64];
+        for (int l = 0; l < 64; ++l) {
+            tile_lane(tile, W, &col[l], &row[l], l);
+            want[l] = col[l] < W && row[l] < H;
+            skip[l] = -1;
+            float d[3];
+            gen_ray(cam, W, H, want[l] ? (uint32_t)col[l] : 0, want[l] ? (uint32_t)row[l] : 0, d);
+            ray32 rr = {cam[0], cam[1], cam[2], d[0], d[1], d[2], 0.0f, INFINITY};
+            r[l] = rr;
+        }
+        wavecount wc = {0, 0, 0};
+        packet32(nodes4, tris, root, r, want, skip, 0, h, &wc);
+        if (wave_nodes) wave_nodes[tile] = wc.node_visits;
+        if (wave_leaves) wave_leaves[tile] = wc.leaf_visits;
+        if (wave_tris) wave_tris[tile] = wc.tri_fetches;
+        for (int l = 0; l < 64; ++l) {
+            if (!want[l]) continue;
+            if (h[l].prim >= 0) refine64(tris, &r[l], &h[l]);
+            int64_t i = row[l] * W + col[l];
+            int hit = h[l].prim >= 0;
+            if (out_t) out_t[i] = hit ? h[l].t : INFINITY;
+            if (out_u) out_u[i] = hit ? h[l].u : 0.0f;
+            if (out_v) out_v[i] = hit ? h[l].v : 0.0f;
+            if (out_prim) out_prim[i] = h[l].prim;
+            if (out_slot) out_slot[i] = h[l].slot;
+        }
+    }
+}
+
+/* Full FP32 render of the device path (rt_render): per 8x8 tile, wave-coherent primary closest
+ * hit, FP64 hit refinement, Lambert, wave-coherent shadow any-hit of the lit hits, texture.
  * textures: concatenated RGBA8 images; tex_off/tex_w/tex_h per texture; mesh_tex per mesh.
  * params: light[3], ambient, background[3], shadow_tmin; flags bit0 shadows, bit1 bilinear. */
-TW_API void tw_render(const float* nodes, const float* tris, const float* shade, int32_t root,
+TW_API void tw_render(const float* nodes4, const float* tris, const float* shade, int32_t root,
                       const float* cam, uint32_t W, uint32_t H, const float* params, uint32_t flags,
                       const uint8_t* textures, const int64_t* tex_off, const int32_t* tex_w,
                       const int32_t* tex_h, const int32_t* mesh_tex, float* rgb, float* out_t,
                       int32_t* out_prim, int64_t* stats) {
-    int64_t n = (int64_t)W * H;
+    int64_t ntiles = (((int64_t)W + 7) / 8) * (((int64_t)H + 7) / 8);
     int64_t hits = 0, shadow_rays = 0, occl = 0;
     const float* L = params;
-    #pragma omp parallel for schedule(dynamic, 64) reduction(+ : hits, shadow_rays, occl)
-    for (int64_t i = 0; i < n; ++i) {
-        float d[3];
-        gen_ray(cam, W, H, (uint32_t)(i % W), (uint32_t)(i / W), d);
-        ray32 r = {cam[0], cam[1], cam[2], d[0], d[1], d[2], 0.0f, INFINITY};
-        hit32 h;
-        traverse32(nodes, tris, root, &r, 0, -1, &h);
-        if (h.prim >= 0) refine64(tris, &r, &h);
-        if (out_t) out_t[i] = h.prim >= 0 ? h.t : INFINITY;
-        if (out_prim) out_prim[i] = h.prim;
-        if (h.prim < 0) {
-            for (int c = 0; c < 3; ++c) rgb[i * 3 + c] = params[4 + c];
-            continue;
+    #pragma omp parallel for schedule(dynamic, 16) reduction(+ : hits, shadow_rays, occl)
+    for (int64_t tile = 0; tile < ntiles; ++tile) {
+        ray32 r[64], sr[64];
+        int want[64], wsh[64];
+        int32_t skip[64], sskip[64];
+        hit32 h[64], sh[64];
+        int64_t col[64], row[64];
+        float ndl[64], tu[64], tv[64];
+        int32_t mesh[64];
+        for (int l = 0; l < 64; ++l) {
+            tile_lane(tile, W, &col[l], &row[l], l);
+            want[l] = col[l] < W && row[l] < H;
+            skip[l] = -1;
+            float d[3];
+            gen_ray(cam, W, H, want[l] ? (uint32_t)col[l] : 0, want[l] ? (uint32_t)row[l] : 0, d);
+            ray32 rr = {cam[0], cam[1], cam[2], d[0], d[1], d[2], 0.0f, INFINITY};
+            r[l] = rr;
         }
-        hits++;
-        const float* s = shade + (int64_t)h.slot * 16;
-        float w0 = 1.0f - h.u - h.v;
-        float N[3];
-        for (int c = 0; c < 3; ++c) {
-            float n0 = s[c], n1 = s[3 + c], n2 = s[6 + c];
-            N[c] = fmaf(w0, n0, fmaf(h.u, n1, h.v * n2));
+        packet32(nodes4, tris, root, r, want, skip, 0, h, NULL);
+        for (int l = 0; l < 64; ++l) {
+            int hit = want[l] && h[l].prim >= 0;
+            ndl[l] = 0.0f; tu[l] = 0.0f; tv[l] = 0.0f; mesh[l] = 0;
+            if (hit) {
+                refine64(tris, &r[l], &h[l]);
+                const float* s = shade + (int64_t)h[l].slot * 16;
+                float w0 = 1.0f - h[l].u - h[l].v;
+                float nx = fmaf(w0, s[0], fmaf(h[l].u, s[3], h[l].v * s[6]));
+                float ny = fmaf(w0, s[1], fmaf(h[l].u, s[4], h[l].v * s[7]));
+                float nz = fmaf(w0, s[2], fmaf(h[l].u, s[5], h[l].v * s[8]));
+                float l2 = fmaf(nx, nx, fmaf(ny, ny, nz * nz));
+                float il = l2 > 0.0f ? 1.0f / sqrtf(l2) : 0.0f;
+                float nl = fmaf(nx * il, L[0], fmaf(ny * il, L[1], (nz * il) * L[2]));
+                ndl[l] = fmaxf(nl, 0.0f);
+                tu[l] = fmaf(w0, s[9], fmaf(h[l].u, s[11], h[l].v * s[13]));
+                tv[l] = fmaf(w0, s[10], fmaf(h[l].u, s[12], h[l].v * s[14]));
+                mesh[l] = (int32_t)fbits(s[15]);
+            }
+            wsh[l] = hit && (flags & 1u) && ndl[l] > 0.0f;
+            sskip[l] = hit ? h[l].slot : -1;
+            sr[l].ox = fmaf(h[l].t, r[l].dx, r[l].ox);
+            sr[l].oy = fmaf(h[l].t, r[l].dy, r[l].oy);
+            sr[l].oz = fmaf(h[l].t, r[l].dz, r[l].oz);
+            sr[l].dx = L[0]; sr[l].dy = L[1]; sr[l].dz = L[2];
+            sr[l].tmin = params[7]; sr[l].tmax = INFINITY;
         }
-        float l2 = fmaf(N[0], N[0], fmaf(N[1], N[1], N[2] * N[2]));
-        float il = l2 > 0.0f ? 1.0f / sqrtf(l2) : 0.0f;
-        float ndl = fmaf(N[0] * il, L[0], fmaf(N[1] * il, L[1], (N[2] * il) * L[2]));
-        ndl = fmaxf(ndl, 0.0f);
-        float vis = 1.0f;
-        if ((flags & 1u) && ndl > 0.0f) {
-            ray32 sr;
-            sr.ox = fmaf(h.t, d[0], cam[0]);
-            sr.oy = fmaf(h.t, d[1], cam[1]);
-            sr.oz = fmaf(h.t, d[2], cam[2]);
-            sr.dx = L[0]; sr.dy = L[1]; sr.dz = L[2];
-            sr.tmin = params[7]; sr.tmax = INFINITY;
-            hit32 sh;
-            traverse32(nodes, tris, root, &sr, 1, h.slot, &sh);
-            shadow_rays++;
-            if (sh.prim >= 0) { vis = 0.0f; occl++; }
+        packet32(nodes4, tris, root, sr, wsh, sskip, 1, sh, NULL);
+        for (int l = 0; l < 64; ++l) {
+            if (!want[l]) continue;
+            int64_t i = row[l] * W + col[l];
+            int hit = h[l].prim >= 0;
+            int occ = wsh[l] && sh[l].prim >= 0;
+            if (hit) hits++;
+            if (wsh[l]) shadow_rays++;
+            if (occ) occl++;
+            if (out_t) out_t[i] = hit ? h[l].t : INFINITY;
+            if (out_prim) out_prim[i] = hit ? h[l].prim : -1;
+            if (!hit) {
+                for (int c = 0; c < 3; ++c) rgb[i * 3 + c] = params[4 + c];
+                continue;
+            }
+            float alb[3] = {1.0f, 1.0f, 1.0f};
+            int32_t ti = mesh_tex ? mesh_tex[mesh[l]] : -1;
+            if (ti >= 0)
+                tex_sample(textures + tex_off[ti], tex_w[ti], tex_h[ti], tu[l], tv[l],
+                           (flags & 2u) != 0, alb);
+            float k = fmaf(occ ? 0.0f : 1.0f, ndl[l], params[3]);
+            for (int c = 0; c < 3; ++c) rgb[i * 3 + c] = alb[c] * k;
         }
-        float alb[3] = {1.0f, 1.0f, 1.0f};
-        int32_t mesh = (int32_t)fbits(s[15]);
-        int32_t ti = mesh_tex ? mesh_tex[mesh] : -1;
-        if (ti >= 0) {
-            float uu = fmaf(w0, s[9], fmaf(h.u, s[11], h.v * s[13]));
-            float vv = fmaf(w0, s[10], fmaf(h.u, s[12], h.v * s[14]));
-            tex_sample(textures + tex_off[ti], tex_w[ti], tex_h[ti], uu, vv, (flags & 2u) != 0, alb);
-        }
-        float k = fmaf(vis, ndl, params[3]);
-        for (int c = 0; c < 3; ++c) rgb[i * 3 + c] = alb[c] * k;
     }
     if (stats) { stats[0] = hits; stats[1] = shadow_rays; stats[2] = occl; }
 }
@@ -723,6 +949,36 @@ TW_API void tw_closest64_brute(const float* tri_pos, int64_t n_tris, int64_t n, 
     }
 }
 
+/* FP64 box of a traversal ref: internal node box, or the union of a leaf's slot boxes. */
+static void ref_box64(const double* nb, const double* lb, int32_t ref, double* b) {
+    if (ref >= 0) {
+        for (int c = 0; c < 6; ++c) b[c] = nb[(int64_t)ref * 6 + c];
+        return;
+    }
+    uint32_t first = leaf_first(ref), count = leaf_count(ref);
+    for (int c = 0; c < 3; ++c) { b[c] = INFINITY; b[3 + c] = -INFINITY; }
+    for (uint32_t k = 0; k < count; ++k)
+        for (int c = 0; c < 3; ++c) {
+            b[c] = fmin(b[c], lb[(int64_t)(first + k) * 6 + c]);
+            b[3 + c] = fmax(b[3 + c], lb[(int64_t)(first + k) * 6 + 3 + c]);
+        }
+}
+
+/* FP64 slab test of a box already inflated by the margin. */
+static int box_hit64(const double* b, const double* o, const double* d, double tmin, double margin) {
+    double tn = -INFINITY, tf = INFINITY;
+    for (int c = 0; c < 3; ++c) {
+        if (d[c] == 0.0) {
+            if (o[c] < b[c] || o[c] > b[3 + c]) return 0;
+            continue;
+        }
+        double t0 = (b[c] - o[c]) / d[c], t1 = (b[3 + c] - o[c]) / d[c];
+        if (t0 > t1) { double tt = t0; t0 = t1; t1 = tt; }
+        tn = fmax(tn, t0); tf = fmin(tf, t1);
+    }
+    return !(tn > tf * (1.0 + 1e-12) + 1e-300 || tf < tmin - margin);
+}
+
 /* FP64 closest hit culled by an FP64 BVH over the same leaves: boxes are recomputed in FP64 from
  * the vertices and inflated by `margin` (absolute), so culling is conservative by construction.
  * nodes/order: the twin's hierarchy (only child refs are used). */
@@ -767,8 +1023,9 @@ TW_API void tw_closest64_bvh(const float* tri_pos, int64_t n_tris, const float* 
                 continue;
             }
             --sp;
-            const double* bl = l >= 0 ? nb + l * 6 : lb + (int64_t)(~l) * 6;
-            const double* br = r >= 0 ? nb + r * 6 : lb + (int64_t)(~r) * 6;
+            double bl[6], br[6];
+            ref_box64(nb, lb, l, bl);
+            ref_box64(nb, lb, r, br);
             for (int c = 0; c < 3; ++c) {
                 nb[x * 6 + c] = fmin(bl[c], br[c]);
                 nb[x * 6 + 3 + c] = fmax(bl[3 + c], br[3 + c]);
@@ -788,25 +1045,17 @@ TW_API void tw_closest64_bvh(const float* tri_pos, int64_t n_tris, const float* 
             stk[sp++] = root;
             while (sp) {
                 int32_t x = stk[--sp];
-                const double* b = x >= 0 ? nb + (int64_t)x * 6 : lb + (int64_t)(~x) * 6;
-                double tn = -INFINITY, tf = INFINITY;
-                int ok = 1;
-                for (int c = 0; c < 3 && ok; ++c) {
-                    if (d[c] == 0.0) {
-                        if (o[c] < b[c] || o[c] > b[3 + c]) ok = 0;
-                        continue;
-                    }
-                    double t0 = (b[c] - o[c]) / d[c], t1 = (b[3 + c] - o[c]) / d[c];
-                    if (t0 > t1) { double tt = t0; t0 = t1; t1 = tt; }
-                    tn = fmax(tn, t0); tf = fmin(tf, t1);
-                }
-                if (!ok || tn > tf * (1.0 + 1e-12) + 1e-300 || tf < tmin - margin) continue;
                 if (x >= 0) {
+                    if (!box_hit64(nb + (int64_t)x * 6, o, d, tmin, margin)) continue;
                     stk[sp++] = (int32_t)fbits(nodes[(int64_t)x * 16 + 12]);
                     stk[sp++] = (int32_t)fbits(nodes[(int64_t)x * 16 + 13]);
                 } else {
-                    if (nc == cap) { cap *= 2; cand = (int64_t*)realloc(cand, sizeof(int64_t) * cap); }
-                    cand[nc++] = order[~x];
+                    uint32_t first = leaf_first(x), count = leaf_count(x);
+                    for (uint32_t k = 0; k < count; ++k) {
+                        if (!box_hit64(lb + (int64_t)(first + k) * 6, o, d, tmin, margin)) continue;
+                        if (nc == cap) { cap *= 2; cand = (int64_t*)realloc(cand, sizeof(int64_t) * cap); }
+                        cand[nc++] = order[first + k];
+                    }
                 }
             }
             closest64_ray(tri_pos, n_tris, o, d, tmin, edge_eps, tie_eps, cand, nc, &out_t[i],

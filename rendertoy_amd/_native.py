@@ -12,7 +12,7 @@ import torch  # noqa: F401  (must be imported before the HIP library; see module
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "librtx_hip.so")
 
-RT_ABI_VERSION = 1
+RT_ABI_VERSION = 2
 RT_ABSENT = 0xFFFFFFFF
 RT_EMPTY_ROOT = 0x7FFFFFFF
 RT_RENDER_SHADOWS = 1
@@ -43,6 +43,7 @@ class rt_scene(ctypes.Structure):
         ("root", ctypes.c_int32),
         ("_reserved", ctypes.c_uint32),
         ("nodes", ctypes.c_void_p),
+        ("nodes4", ctypes.c_void_p),
         ("tris", ctypes.c_void_p),
         ("shade", ctypes.c_void_p),
         ("mesh_offsets", ctypes.c_void_p),
@@ -53,6 +54,7 @@ class rt_scene(ctypes.Structure):
 class rt_scene_sizes(ctypes.Structure):
     _fields_ = [
         ("nodes_bytes", ctypes.c_uint64),
+        ("nodes4_bytes", ctypes.c_uint64),
         ("tris_bytes", ctypes.c_uint64),
         ("shade_bytes", ctypes.c_uint64),
         ("bounds_bytes", ctypes.c_uint64),

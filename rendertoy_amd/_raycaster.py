@@ -155,6 +155,7 @@ class Raycaster:
         # workspace after the build kernels on that stream
         with torch.cuda.stream(self._torch_stream()):
             self._nodes = torch.empty(sizes.nodes_bytes, dtype=torch.uint8, device=dev)
+            self._nodes4 = torch.empty(sizes.nodes4_bytes, dtype=torch.uint8, device=dev)
             self._tris = torch.empty(sizes.tris_bytes, dtype=torch.uint8, device=dev)
             self._shade = torch.empty(sizes.shade_bytes, dtype=torch.uint8, device=dev)
             self._bounds = torch.empty(8, dtype=torch.float32, device=dev)
@@ -166,6 +167,7 @@ class Raycaster:
         sc.n_tris = n_tris
         sc.n_meshes = len(self.models)
         sc.nodes = self._nodes.data_ptr()
+        sc.nodes4 = self._nodes4.data_ptr()
         sc.tris = self._tris.data_ptr()
         sc.shade = self._shade.data_ptr()
         sc.mesh_offsets = self._mesh_offsets.data_ptr()
@@ -192,6 +194,7 @@ class Raycaster:
         ni = max(n - 1, 1)
         return {
             "nodes": self._nodes.view(torch.float32).view(ni, 16)[: max(n - 1, 0)],
+            "nodes4": self._nodes4.view(torch.float32).view(ni, 32)[: max(n - 1, 0)],
             "tris": self._tris.view(torch.float32).view(max(n, 1), 12)[:n],
             "shade": self._shade.view(torch.float32).view(max(n, 1), 16)[:n],
             "bounds": self._bounds,

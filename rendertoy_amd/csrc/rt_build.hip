@@ -8,12 +8,16 @@
 //   2. bounds      two-stage min/max reduction of the centroids.
 //   3. morton      30-bit Morton code of each centroid quantised to 1024^3 over those bounds.
 //   4. sort        stable LSD radix sort of (code, primitive) in 4 passes of 8 bits; each pass is
-//                  histogram -> digit-major exclusive scan -> scatter, the scatter ranking keys
-//                  in LDS with wave64 ballot/mbcnt multi-split so equal digits keep input order.
+//                  per-tile histogram -> per-digit row scan -> scatter; the scatter block scans
+//                  the 256 digit totals itself and ranks keys in LDS with wave64 ballot/mbcnt
+//                  multi-split so equal digits keep input order.
 //   5. hierarchy   Karras 2012 binary radix tree over the extended keys (code, slot).
 //   6. emit        permute tri/shade records into slot order.
-//   7. refit       bottom-up from every leaf; the second thread to reach a node (agent-scope
-//                  acq_rel counter) unions its children's boxes and continues upward.
+//   7. refit       bottom-up from every leaf; the second thread to reach a node unions its
+//                  children's boxes and continues upward. Nodes whose leaf range lies inside one
+//                  2048-leaf block are refit by that block's workgroup through LDS counters; the
+//                  ~0.6% spanning blocks by a second pass with device-scope acq_rel counters
+//                  (each device-scope release writes back L2, so it is kept off the bulk).
 // Every result is an integer or an exact FP32 min/max/sub, so the output is byte-identical to the
 // CPU restatement under the same inputs.
 #include <algorithm>
@@ -41,9 +45,12 @@ struct Workspace {
     uint32_t* keys1;
     uint32_t* vals1;
     uint32_t* hist;    // kRadix * nblk
+    uint32_t* totals;  // kRadix
     float* partial;    // kReduceBlocks * 8
     float* cbounds;    // 8
     int32_t* parent;   // (n-1) internal then n leaves
+    int32_t* rangej;   // n-1: the other end j of internal node i's leaf range [min(i,j), max(i,j)]
+    int32_t* split;    // n-1: split position g of internal node i (children ranges [lo,g], [g+1,hi])
     uint32_t* flags;   // n-1
     float4* nodebox;   // 2(n-1)
     uint64_t bytes;
@@ -70,9 +77,12 @@ Workspace carve(char* base, uint64_t n) {
     w.keys1 = (uint32_t*)take(4 * n);
     w.vals1 = (uint32_t*)take(4 * n);
     w.hist = (uint32_t*)take(4 * kRadix * (nblk ? nblk : 1));
+    w.totals = (uint32_t*)take(4 * kRadix);
     w.partial = (float*)take(4 * 8 * kReduceBlocks);
     w.cbounds = (float*)take(4 * 8);
     w.parent = (int32_t*)take(4 * (ni + n));
+    w.rangej = (int32_t*)take(4 * ni);
+    w.split = (int32_t*)take(4 * ni);
     w.flags = (uint32_t*)take(4 * ni);
     w.nodebox = (float4*)take(32 * ni);
     w.bytes = off;
@@ -81,9 +91,12 @@ Workspace carve(char* base, uint64_t n) {
 
 RT_DEV float canon(float x) { return x + 0.0f; }  // -0 -> +0
 
+// Leaf boxes are padded by |coordinate| * 2^-14 + 2^-40 per side, wider than the world-space error
+// of an FP32 Moller-Trumbore acceptance, so a box test never culls a triangle the FP32 test would
+// accept: the closest hit is then independent of the traversal order (per-ray or wave-coherent).
 RT_DEV float pad_of(float lo, float hi) {
     const float m = fmaxf(fabsf(lo), fabsf(hi));
-    return m * 0x1p-18f + 0x1p-40f;
+    return m * 0x1p-14f + 0x1p-40f;
 }
 
 RT_DEV float3 load_f3(const char* base, uint64_t i, uint32_t stride, uint32_t off) {
@@ -273,42 +286,55 @@ __global__ void __launch_bounds__(kSortThreads) k_sort_hist(const uint32_t* __re
     hist[(uint64_t)threadIdx.x * nblk + blockIdx.x] = h[threadIdx.x];
 }
 
-// Exclusive scan of the digit-major histogram (kRadix * nblk entries) by one block.
-__global__ void __launch_bounds__(1024) k_sort_scan(uint32_t* __restrict__ hist, uint32_t total) {
-    __shared__ uint32_t s[1024];
-    const uint32_t per = (total + 1023) / 1024;
-    const uint32_t beg = threadIdx.x * per;
-    const uint32_t end = min(total, beg + per);
-    uint32_t sum = 0;
-    for (uint32_t i = beg; i < end; ++i) sum += hist[i];
-    s[threadIdx.x] = sum;
-    __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
-        const uint32_t v = threadIdx.x >= (uint32_t)off ? s[threadIdx.x - off] : 0u;
+// Row scan: block d turns row d of the digit-major histogram (counts of digit d in tiles
+// 0..nblk-1) into its exclusive prefix sums, in place, and writes the row total to totals[d].
+__global__ void __launch_bounds__(kSortThreads) k_sort_rowscan(uint32_t* __restrict__ hist,
+                                                               uint32_t nblk,
+                                                               uint32_t* __restrict__ totals) {
+    __shared__ uint32_t s[kSortThreads];
+    uint32_t* row = hist + (uint64_t)blockIdx.x * nblk;
+    uint32_t run = 0;
+    for (uint32_t c0 = 0; c0 < nblk; c0 += kSortThreads) {
+        const uint32_t i = c0 + threadIdx.x;
+        const uint32_t v = i < nblk ? row[i] : 0u;
+        s[threadIdx.x] = v;
         __syncthreads();
-        s[threadIdx.x] += v;
+        for (int off = 1; off < kSortThreads; off <<= 1) {
+            const uint32_t x = threadIdx.x >= (uint32_t)off ? s[threadIdx.x - off] : 0u;
+            __syncthreads();
+            s[threadIdx.x] += x;
+            __syncthreads();
+        }
+        if (i < nblk) row[i] = run + s[threadIdx.x] - v;
+        run += s[kSortThreads - 1];
         __syncthreads();
     }
-    uint32_t run = s[threadIdx.x] - sum;  // exclusive prefix of this thread's segment
-    for (uint32_t i = beg; i < end; ++i) {
-        const uint32_t c = hist[i];
-        hist[i] = run;
-        run += c;
-    }
+    if (threadIdx.x == 0) totals[blockIdx.x] = run;
 }
 
 // Stable scatter: items of a tile are visited in rounds of 256 (item order = (round, thread)),
 // each round ranked by wave64 multi-split (8 ballots give the lanes sharing this lane's digit).
 __global__ void __launch_bounds__(kSortThreads) k_sort_scatter(
     const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin, uint32_t n, int shift,
-    const uint32_t* __restrict__ hist, uint32_t nblk, uint32_t* __restrict__ kout,
+    const uint32_t* __restrict__ hist, const uint32_t* __restrict__ totals, uint32_t nblk,
+    uint32_t* __restrict__ kout,
     uint32_t* __restrict__ vout) {
     constexpr int kWaves = kSortThreads / 64;
     __shared__ uint32_t base[kRadix];
     __shared__ uint32_t run[kRadix];
     __shared__ uint32_t wcount[kWaves][kRadix];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    base[tid] = hist[(uint64_t)tid * nblk + blockIdx.x];
+    // exclusive scan of the digit totals gives each digit's first output position
+    run[tid] = totals[tid];
+    __syncthreads();
+    for (int off = 1; off < kRadix; off <<= 1) {
+        const uint32_t x = tid >= off ? run[tid - off] : 0u;
+        __syncthreads();
+        run[tid] += x;
+        __syncthreads();
+    }
+    base[tid] = run[tid] - totals[tid] + hist[(uint64_t)tid * nblk + blockIdx.x];
+    __syncthreads();
     run[tid] = 0;
     for (int w = 0; w < kWaves; ++w) wcount[w][tid] = 0;
     __syncthreads();
@@ -361,7 +387,9 @@ RT_DEV int delta(const uint32_t* __restrict__ k, int64_t n, int64_t i, int64_t j
 
 __global__ void __launch_bounds__(256) k_hierarchy(const uint32_t* __restrict__ keys, uint32_t n,
                                                     float4* __restrict__ nodes,
-                                                    int32_t* __restrict__ parent) {
+                                                    int32_t* __restrict__ parent,
+                                                    int32_t* __restrict__ rangej,
+                                                    int32_t* __restrict__ split) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= (int64_t)n - 1) return;
     const int d = (delta(keys, n, i, i + 1) - delta(keys, n, i, i - 1)) >= 0 ? 1 : -1;
@@ -381,12 +409,20 @@ __global__ void __launch_bounds__(256) k_hierarchy(const uint32_t* __restrict__ 
     }
     const int64_t gamma = i + s * d + (d < 0 ? -1 : 0);
     const int64_t lo = i < j ? i : j, hi = i < j ? j : i;
+    // binary radix tree (used by the refit): children gamma / gamma+1, leaves when their range is
+    // a single slot
     const int32_t left = (lo == gamma) ? ~(int32_t)gamma : (int32_t)gamma;
     const int32_t right = (hi == gamma + 1) ? ~(int32_t)(gamma + 1) : (int32_t)(gamma + 1);
-    nodes[(uint64_t)i * 4 + 3] = make_float4(__int_as_float(left), __int_as_float(right), 0.0f, 0.0f);
+    // traversal refs: a child whose range holds <= kLeafMax slots becomes one leaf of that range
+    const int64_t nl = gamma - lo + 1, nr = hi - gamma;
+    const int32_t tl = nl <= kLeafMax ? leaf_ref((uint32_t)lo, (uint32_t)nl) : (int32_t)gamma;
+    const int32_t tr = nr <= kLeafMax ? leaf_ref((uint32_t)(gamma + 1), (uint32_t)nr) : (int32_t)(gamma + 1);
+    nodes[(uint64_t)i * 4 + 3] = make_float4(__int_as_float(tl), __int_as_float(tr), 0.0f, 0.0f);
     const uint64_t ni = (uint64_t)n - 1;
     parent[left >= 0 ? (uint64_t)left : ni + (uint64_t)(~left)] = (int32_t)i;
     parent[right >= 0 ? (uint64_t)right : ni + (uint64_t)(~right)] = (int32_t)i;
+    rangej[i] = (int32_t)j;
+    split[i] = (int32_t)gamma;
     if (i == 0) parent[0] = -1;
 }
 
@@ -407,67 +443,209 @@ __global__ void __launch_bounds__(256) k_emit(const uint32_t* __restrict__ order
     shade[4 * (uint64_t)s + 3] = shade_u[4 * p + 3];
 }
 
-__global__ void __launch_bounds__(256) k_fill_u32(uint32_t* __restrict__ p, uint64_t n, uint32_t v) {
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (uint64_t)gridDim.x * blockDim.x)
-        p[i] = v;
+// ---- refit ----
+// Internal node i covers the sorted leaves [min(i,j), max(i,j)] (j = rangej[i]). With blocks of
+// kRefitBlock consecutive leaves, a node whose range lies in one block is "local": its whole
+// subtree is refit by that block's workgroup, which synchronises only through LDS. The few nodes
+// whose ranges span blocks ("top" nodes) are refit afterwards with device-scope counters.
+constexpr int kRefitBlock = 2048;
+
+// Children of internal node i in the binary radix tree: internal ids, or ~slot for single leaves.
+RT_DEV void binary_children(const int32_t* __restrict__ rangej, const int32_t* __restrict__ split,
+                            int32_t i, int32_t& l, int32_t& r) {
+    const int32_t j = rangej[i], g = split[i];
+    const int32_t lo = i < j ? i : j, hi = i < j ? j : i;
+    l = lo == g ? ~g : g;
+    r = hi == g + 1 ? ~(g + 1) : g + 1;
 }
 
-RT_DEV float load_coherent(const float* p) {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+RT_DEV void leaf_box(const float4* __restrict__ leafbox, const uint32_t* __restrict__ order,
+                     int32_t ref, float* lo, float* hi) {
+    const uint64_t p = order[~ref];
+    const float4 a = leafbox[2 * p], b = leafbox[2 * p + 1];
+    lo[0] = a.x; lo[1] = a.y; lo[2] = a.z;
+    hi[0] = b.x; hi[1] = b.y; hi[2] = b.z;
 }
 
-RT_DEV void load_box(const float4* __restrict__ leafbox, const uint32_t* __restrict__ order,
+// Writes the child boxes into node record `node`, its own box (union) to nodebox, and returns it.
+RT_DEV void store_node(float4* __restrict__ nodes, float4* __restrict__ nodebox,
+                       float* __restrict__ bounds, int32_t node, const float* l_lo, const float* l_hi,
+                       const float* r_lo, const float* r_hi, float* u_lo, float* u_hi) {
+    float4* nd = nodes + (uint64_t)node * 4;
+    nd[0] = make_float4(l_lo[0], l_hi[0], l_lo[1], l_hi[1]);
+    nd[1] = make_float4(r_lo[0], r_hi[0], r_lo[1], r_hi[1]);
+    nd[2] = make_float4(l_lo[2], l_hi[2], r_lo[2], r_hi[2]);
+    for (int c = 0; c < 3; ++c) {
+        u_lo[c] = fminf(l_lo[c], r_lo[c]);
+        u_hi[c] = fmaxf(l_hi[c], r_hi[c]);
+    }
+    nodebox[2 * (uint64_t)node] = make_float4(u_lo[0], u_lo[1], u_lo[2], 0.0f);
+    nodebox[2 * (uint64_t)node + 1] = make_float4(u_hi[0], u_hi[1], u_hi[2], 0.0f);
+    if (node == 0) {
+        bounds[0] = u_lo[0]; bounds[1] = u_lo[1]; bounds[2] = u_lo[2]; bounds[3] = 0.0f;
+        bounds[4] = u_hi[0]; bounds[5] = u_hi[1]; bounds[6] = u_hi[2]; bounds[7] = 0.0f;
+    }
+}
+
+// Phase 1: block b walks up from each of its leaves through the local nodes. The first thread to
+// reach a node stops; the second (LDS counter, workgroup acq_rel) unions the children's boxes,
+// which sit in LDS (local internal children) or in leafbox (leaves). Also clears the device-scope
+// counters of this block's node ids for phase 2.
+__global__ void __launch_bounds__(256) k_refit_local(const uint32_t* __restrict__ order, uint32_t n,
+                                                      const float4* __restrict__ leafbox,
+                                                      const int32_t* __restrict__ parent,
+                                                      const int32_t* __restrict__ rangej,
+                                                      const int32_t* __restrict__ split,
+                                                      uint32_t* __restrict__ flags,
+                                                      float4* __restrict__ nodes,
+                                                      float4* __restrict__ nodebox,
+                                                      float* __restrict__ bounds) {
+    __shared__ uint32_t lflag[kRefitBlock];
+    __shared__ float lbox[kRefitBlock][6];
+    const int64_t blo = (int64_t)blockIdx.x * kRefitBlock;
+    const int64_t bhi = std::min<int64_t>(blo + kRefitBlock, n);  // exclusive
+    const int64_t ni = (int64_t)n - 1;
+    for (int k = threadIdx.x; k < kRefitBlock; k += blockDim.x) {
+        lflag[k] = 0;
+        if (blo + k < ni) flags[blo + k] = 0;
+    }
+    __syncthreads();
+    for (int64_t s = blo + threadIdx.x; s < bhi; s += blockDim.x) {
+        int32_t node = parent[ni + s];
+        while (node >= 0) {
+            const int64_t j = rangej[node];
+            const int64_t lo = std::min<int64_t>(node, j), hi = std::max<int64_t>(node, j);
+            if (lo < blo || hi >= bhi) break;  // a top node: phase 2
+            const uint32_t prev = __hip_atomic_fetch_add(&lflag[node - blo], 1u, __ATOMIC_ACQ_REL,
+                                                         __HIP_MEMORY_SCOPE_WORKGROUP);
+            if (prev == 0) break;
+            int32_t l, r;
+            binary_children(rangej, split, node, l, r);
+            float l_lo[3], l_hi[3], r_lo[3], r_hi[3], u_lo[3], u_hi[3];
+            if (l < 0) {
+                leaf_box(leafbox, order, l, l_lo, l_hi);
+            } else {
+                for (int c = 0; c < 3; ++c) { l_lo[c] = lbox[l - blo][c]; l_hi[c] = lbox[l - blo][3 + c]; }
+            }
+            if (r < 0) {
+                leaf_box(leafbox, order, r, r_lo, r_hi);
+            } else {
+                for (int c = 0; c < 3; ++c) { r_lo[c] = lbox[r - blo][c]; r_hi[c] = lbox[r - blo][3 + c]; }
+            }
+            store_node(nodes, nodebox, bounds, node, l_lo, l_hi, r_lo, r_hi, u_lo, u_hi);
+            for (int c = 0; c < 3; ++c) {
+                lbox[node - blo][c] = u_lo[c];
+                lbox[node - blo][3 + c] = u_hi[c];
+            }
+            node = parent[node];
+        }
+    }
+}
+
+RT_DEV bool is_local(const int32_t* __restrict__ rangej, int32_t ref) {
+    if (ref < 0) return true;
+    const int32_t j = rangej[ref];
+    const int32_t lo = ref < j ? ref : j, hi = ref < j ? j : ref;
+    return lo / kRefitBlock == hi / kRefitBlock;
+}
+
+RT_DEV void node_box(const float4* __restrict__ leafbox, const uint32_t* __restrict__ order,
                      const float4* nodebox, int32_t ref, float* lo, float* hi) {
     if (ref < 0) {
-        const uint64_t p = order[~ref];
-        const float4 a = leafbox[2 * p], b = leafbox[2 * p + 1];
+        leaf_box(leafbox, order, ref, lo, hi);
+    } else {
+        const float4 a = nodebox[2 * (uint64_t)ref], b = nodebox[2 * (uint64_t)ref + 1];
         lo[0] = a.x; lo[1] = a.y; lo[2] = a.z;
         hi[0] = b.x; hi[1] = b.y; hi[2] = b.z;
-    } else {
-        // written by another thread of this launch: read past the non-coherent L1
-        const float* q = (const float*)(nodebox + 2 * (uint64_t)ref);
-        lo[0] = load_coherent(q + 0); lo[1] = load_coherent(q + 1); lo[2] = load_coherent(q + 2);
-        hi[0] = load_coherent(q + 4); hi[1] = load_coherent(q + 5); hi[2] = load_coherent(q + 6);
     }
 }
 
-__global__ void __launch_bounds__(256) k_refit(const uint32_t* __restrict__ order, uint32_t n,
-                                                const float4* __restrict__ leafbox,
-                                                const int32_t* __restrict__ parent,
-                                                uint32_t* __restrict__ flags,
-                                                float4* __restrict__ nodes, float4* nodebox,
-                                                float* __restrict__ bounds) {
-    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= n) return;
-    const uint64_t ni = (uint64_t)n - 1;
-    int32_t node = parent[ni + s];
-    while (node >= 0) {
-        const uint32_t prev =
-            __hip_atomic_fetch_add(&flags[node], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        if (prev == 0) return;  // the sibling subtree is not finished; its last thread continues
-        const float4 refs = nodes[(uint64_t)node * 4 + 3];
-        const int32_t l = __float_as_int(refs.x), r = __float_as_int(refs.y);
-        float l_lo[3], l_hi[3], r_lo[3], r_hi[3];
-        load_box(leafbox, order, nodebox, l, l_lo, l_hi);
-        load_box(leafbox, order, nodebox, r, r_lo, r_hi);
-        float4* nd = nodes + (uint64_t)node * 4;
-        nd[0] = make_float4(l_lo[0], l_hi[0], l_lo[1], l_hi[1]);
-        nd[1] = make_float4(r_lo[0], r_hi[0], r_lo[1], r_hi[1]);
-        nd[2] = make_float4(l_lo[2], l_hi[2], r_lo[2], r_hi[2]);
-        const float4 ulo = make_float4(fminf(l_lo[0], r_lo[0]), fminf(l_lo[1], r_lo[1]),
-                                       fminf(l_lo[2], r_lo[2]), 0.0f);
-        const float4 uhi = make_float4(fmaxf(l_hi[0], r_hi[0]), fmaxf(l_hi[1], r_hi[1]),
-                                       fmaxf(l_hi[2], r_hi[2]), 0.0f);
-        nodebox[2 * (uint64_t)node] = ulo;
-        nodebox[2 * (uint64_t)node + 1] = uhi;
-        if (node == 0) {
-            bounds[0] = ulo.x; bounds[1] = ulo.y; bounds[2] = ulo.z; bounds[3] = 0.0f;
-            bounds[4] = uhi.x; bounds[5] = uhi.y; bounds[6] = uhi.z; bounds[7] = 0.0f;
+// Phase 2: one thread per top node. Its local children are complete (phase 1); a node with two
+// local children is computed at once, one with a single local child counts that child's arrival
+// on the device-scope counter. The second arrival at a node computes it and continues upward
+// (every ancestor of a top node is a top node).
+__global__ void __launch_bounds__(256) k_refit_top(const uint32_t* __restrict__ order, uint32_t n,
+                                                    const float4* __restrict__ leafbox,
+                                                    const int32_t* __restrict__ parent,
+                                                    const int32_t* __restrict__ rangej,
+                                                    const int32_t* __restrict__ split,
+                                                    uint32_t* __restrict__ flags,
+                                                    float4* __restrict__ nodes, float4* nodebox,
+                                                    float* __restrict__ bounds) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (int64_t)n - 1) return;
+    int32_t node = (int32_t)i;
+    if (is_local(rangej, node)) return;
+    {
+        int32_t l, r;
+        binary_children(rangej, split, node, l, r);
+        const int n_local = (int)is_local(rangej, l) + (int)is_local(rangej, r);
+        if (n_local == 0) return;
+        if (n_local == 1) {
+            const uint32_t prev = __hip_atomic_fetch_add(&flags[node], 1u, __ATOMIC_ACQ_REL,
+                                                         __HIP_MEMORY_SCOPE_AGENT);
+            if (prev == 0) return;
         }
-        __threadfence();
-        node = parent[node];
     }
+    for (;;) {
+        int32_t l, r;
+        binary_children(rangej, split, node, l, r);
+        float l_lo[3], l_hi[3], r_lo[3], r_hi[3], u_lo[3], u_hi[3];
+        node_box(leafbox, order, nodebox, l, l_lo, l_hi);
+        node_box(leafbox, order, nodebox, r, r_lo, r_hi);
+        store_node(nodes, nodebox, bounds, node, l_lo, l_hi, r_lo, r_hi, u_lo, u_hi);
+        const int32_t p = parent[node];
+        if (p < 0) return;
+        const uint32_t prev =
+            __hip_atomic_fetch_add(&flags[p], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        if (prev == 0) return;
+        node = p;
+    }
+}
+
+// ---- 4-wide nodes ----
+// nodes4[i] (8 x float4, 128 B) lists the grandchildren of binary node i reached through its
+// internal children, and its leaf children directly, in the order (left..., right...):
+//   [0]=lo.x[4] [1]=hi.x[4] [2]=lo.y[4] [3]=hi.y[4] [4]=lo.z[4] [5]=hi.z[4] [6]=refs[4] [7]=0
+// Unused slots hold ref RT_EMPTY_ROOT and an empty box (lo=+inf, hi=-inf). Every binary node gets
+// a record (traversal from the root only reaches every other level), so refs stay binary ids.
+__global__ void __launch_bounds__(256) k_wide(uint32_t n, const float4* __restrict__ nodes,
+                                               float4* __restrict__ nodes4) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (int64_t)n - 1) return;
+    float lo[3][4], hi[3][4];
+    int32_t ref[4];
+    int k = 0;
+    const float4* nd = nodes + (uint64_t)i * 4;
+    const float4 n0 = nd[0], n1 = nd[1], n2 = nd[2], n3 = nd[3];
+    const int32_t c[2] = {__float_as_int(n3.x), __float_as_int(n3.y)};
+    const float cb[2][6] = {{n0.x, n0.z, n2.x, n0.y, n0.w, n2.y}, {n1.x, n1.z, n2.z, n1.y, n1.w, n2.w}};
+    for (int e = 0; e < 2; ++e) {
+        if (c[e] >= 0) {
+            const float4* cd = nodes + (uint64_t)c[e] * 4;
+            const float4 m0 = cd[0], m1 = cd[1], m2 = cd[2], m3 = cd[3];
+            lo[0][k] = m0.x; hi[0][k] = m0.y; lo[1][k] = m0.z; hi[1][k] = m0.w;
+            lo[2][k] = m2.x; hi[2][k] = m2.y; ref[k] = __float_as_int(m3.x); ++k;
+            lo[0][k] = m1.x; hi[0][k] = m1.y; lo[1][k] = m1.z; hi[1][k] = m1.w;
+            lo[2][k] = m2.z; hi[2][k] = m2.w; ref[k] = __float_as_int(m3.y); ++k;
+        } else {
+            lo[0][k] = cb[e][0]; lo[1][k] = cb[e][1]; lo[2][k] = cb[e][2];
+            hi[0][k] = cb[e][3]; hi[1][k] = cb[e][4]; hi[2][k] = cb[e][5];
+            ref[k] = c[e]; ++k;
+        }
+    }
+    for (; k < 4; ++k) {
+        for (int a = 0; a < 3; ++a) { lo[a][k] = INFINITY; hi[a][k] = -INFINITY; }
+        ref[k] = RT_EMPTY_ROOT;
+    }
+    float4* o = nodes4 + (uint64_t)i * 8;
+    for (int a = 0; a < 3; ++a) {
+        o[2 * a] = make_float4(lo[a][0], lo[a][1], lo[a][2], lo[a][3]);
+        o[2 * a + 1] = make_float4(hi[a][0], hi[a][1], hi[a][2], hi[a][3]);
+    }
+    o[6] = make_float4(__int_as_float(ref[0]), __int_as_float(ref[1]), __int_as_float(ref[2]),
+                       __int_as_float(ref[3]));
+    o[7] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 }
 
 // Scene of one triangle: no internal node; the root is leaf slot 0.
@@ -500,9 +678,10 @@ using namespace rt;
 
 extern "C" int rt_scene_sizes_for(uint32_t n_tris, rt_scene_sizes* out) {
     if (!out) return set_error(RT_EINVAL, "rt_scene_sizes_for: out is NULL");
-    if (n_tris > 0x7FFFFFF0u) return set_error(RT_ECAPACITY, "rt_scene_sizes_for: too many triangles");
+    if (n_tris > kMaxTris) return set_error(RT_ECAPACITY, "rt_scene_sizes_for: too many triangles");
     const uint64_t n = n_tris;
     out->nodes_bytes = 64 * (n > 1 ? n - 1 : 1);
+    out->nodes4_bytes = 128 * (n > 1 ? n - 1 : 1);
     out->tris_bytes = 48 * (n ? n : 1);
     out->shade_bytes = 64 * (n ? n : 1);
     out->bounds_bytes = 32;
@@ -531,13 +710,13 @@ extern "C" int rt_scene_build(const rt_mesh_desc* meshes, uint32_t n_meshes, rt_
             return set_error(RT_EINVAL, "rt_scene_build: bad uv offset");
         n64 += d.n_tris;
     }
-    if (n64 > 0x7FFFFFF0ull) return set_error(RT_ECAPACITY, "rt_scene_build: too many triangles");
+    if (n64 > kMaxTris) return set_error(RT_ECAPACITY, "rt_scene_build: too many triangles");
     const uint32_t n = (uint32_t)n64;
     if (sc->n_tris != n || sc->n_meshes != n_meshes)
         return set_error(RT_EINVAL, "rt_scene_build: scene n_tris/n_meshes do not match the meshes");
-    if (!sc->nodes || !sc->tris || !sc->shade || !sc->mesh_offsets || !sc->bounds)
+    if (!sc->nodes || !sc->nodes4 || !sc->tris || !sc->shade || !sc->mesh_offsets || !sc->bounds)
         return set_error(RT_EINVAL, "rt_scene_build: scene arrays must be allocated");
-    if (((uintptr_t)sc->nodes | (uintptr_t)sc->tris | (uintptr_t)sc->shade) & 15)
+    if (((uintptr_t)sc->nodes | (uintptr_t)sc->nodes4 | (uintptr_t)sc->tris | (uintptr_t)sc->shade) & 15)
         return set_error(RT_EINVAL, "rt_scene_build: scene arrays must be 16-byte aligned");
     hipStream_t s = (hipStream_t)stream;
 
@@ -579,9 +758,10 @@ extern "C" int rt_scene_build(const rt_mesh_desc* meshes, uint32_t n_meshes, rt_
     uint32_t *ki = w.keys0, *vi = w.vals0, *ko = w.keys1, *vo = w.vals1;
     for (int shift = 0; shift < 30; shift += kRadixBits) {
         hipLaunchKernelGGL(k_sort_hist, dim3(nblk), dim3(kSortThreads), 0, s, ki, n, shift, w.hist, nblk);
-        hipLaunchKernelGGL(k_sort_scan, dim3(1), dim3(1024), 0, s, w.hist, (uint32_t)(kRadix * nblk));
+        hipLaunchKernelGGL(k_sort_rowscan, dim3(kRadix), dim3(kSortThreads), 0, s, w.hist, nblk,
+                           w.totals);
         hipLaunchKernelGGL(k_sort_scatter, dim3(nblk), dim3(kSortThreads), 0, s, ki, vi, n, shift,
-                           w.hist, nblk, ko, vo);
+                           w.hist, w.totals, nblk, ko, vo);
         std::swap(ki, ko);
         std::swap(vi, vo);
     }
@@ -590,15 +770,19 @@ extern "C" int rt_scene_build(const rt_mesh_desc* meshes, uint32_t n_meshes, rt_
     float4* nodes = (float4*)sc->nodes;
     if (n > 1)
         hipLaunchKernelGGL(k_hierarchy, dim3(blocks_for(n - 1, 256)), dim3(256), 0, s, ki, n, nodes,
-                           w.parent);
+                           w.parent, w.rangej, w.split);
     hipLaunchKernelGGL(k_emit, dim3(blocks_for(n, 256)), dim3(256), 0, s, vi, n, w.tri_u, w.shade_u,
                        (float4*)sc->tris, (float4*)sc->shade);
     if (n > 1) {
-        hipLaunchKernelGGL(k_fill_u32, dim3(std::min<uint32_t>(1024, blocks_for(n - 1, 256))), dim3(256),
-                           0, s, w.flags, (uint64_t)(n - 1), 0u);
-        hipLaunchKernelGGL(k_refit, dim3(blocks_for(n, 256)), dim3(256), 0, s, vi, n, w.leafbox,
-                           w.parent, w.flags, nodes, w.nodebox, sc->bounds);
-        sc->root = 0;
+        hipLaunchKernelGGL(k_refit_local, dim3(blocks_for(n, kRefitBlock)), dim3(256), 0, s, vi, n,
+                           w.leafbox, w.parent, w.rangej, w.split, w.flags, nodes, w.nodebox,
+                           sc->bounds);
+        hipLaunchKernelGGL(k_refit_top, dim3(blocks_for(n - 1, 256)), dim3(256), 0, s, vi, n,
+                           w.leafbox, w.parent, w.rangej, w.split, w.flags, nodes, w.nodebox,
+                           sc->bounds);
+        hipLaunchKernelGGL(k_wide, dim3(blocks_for(n - 1, 256)), dim3(256), 0, s, n,
+                           (const float4*)nodes, (float4*)sc->nodes4);
+        sc->root = n <= (uint32_t)kLeafMax ? leaf_ref(0, n) : 0;
     } else {
         hipLaunchKernelGGL(k_single_leaf_bounds, dim3(1), dim3(1), 0, s, w.leafbox, sc->bounds);
         sc->root = ~0;

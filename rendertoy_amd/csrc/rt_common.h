@@ -27,7 +27,19 @@
 namespace rt {
 
 constexpr int kStackDepth = 64;        // >= 62: depth bound of a radix tree over 62-bit keys
+constexpr int kWideStack = 128;        // >= 3 * 31: pushes per level x levels of the 4-wide tree
 constexpr float kTfarScale = 1.0000004f;  // 1 + 2*gamma(3) rounded up (Ize 2013, robust slabs)
+
+// Leaves: a subtree holding at most kLeafMax consecutive slots is one leaf, referenced as
+// ~(first | (count - 1) << 29), so traversal refs are < 0 for leaves and node ids otherwise.
+constexpr int kLeafMax = 4;
+constexpr uint32_t kMaxTris = (1u << 29) - 1;
+
+__host__ __device__ inline int32_t leaf_ref(uint32_t first, uint32_t count) {
+    return ~(int32_t)(first | ((count - 1u) << 29));
+}
+__host__ __device__ inline uint32_t leaf_first(int32_t ref) { return (uint32_t)(~ref) & 0x1FFFFFFFu; }
+__host__ __device__ inline uint32_t leaf_count(int32_t ref) { return ((uint32_t)(~ref) >> 29) + 1u; }
 
 struct Ray {
     float ox, oy, oz, dx, dy, dz, tmin, tmax;
@@ -144,8 +156,10 @@ RT_DEV void traverse(const float4* __restrict__ nodes, const float4* __restrict_
                 continue;
             }
         } else {
-            const int32_t slot = ~cur;
-            if (slot != skip_slot) {
+            const uint32_t first = leaf_first(cur), count = leaf_count(cur);
+            for (uint32_t k = 0; k < count; ++k) {
+                const int32_t slot = (int32_t)(first + k);
+                if (slot == skip_slot) continue;
                 const float4* tr = tris + (size_t)slot * 3;
                 const float4 ta = tr[0], tb = tr[1], tc = tr[2];
                 float t, u, v;
@@ -164,6 +178,144 @@ RT_DEV void traverse(const float4* __restrict__ nodes, const float4* __restrict_
         }
         if (sp == 0) break;
         cur = stack[--sp];
+    }
+}
+
+// Wave-coherent ("packet") traversal of the 4-wide nodes for coherent rays (camera tiles, shadow
+// rays toward one light from a tile's hit points). The 64 lanes walk one node sequence together:
+//  - the current node, the stack pointer and the 64-bit mask of lanes that passed the current
+//    node's box are wave-uniform, so node and triangle records are fetched with scalar loads;
+//  - each lane tests only nodes whose box it passed itself (the mask), with its own t culling,
+//    so a lane's result is that of its own ray;
+//  - children are visited in the order of the entry distances of the lowest lane of the mask
+//    (children that lane misses after those it hits, ties by slot);
+//  - the stack is a pair of VGPRs per field, lane k holding entries k and 64+k (push = per-lane
+//    select, pop = v_readlane): at most 3 pushes on each of at most 31 levels.
+// Must be called by all 64 lanes of the wave with uniform control flow (no lane may have exited).
+template <bool ANY>
+RT_DEV void traverse_packet(const float4* __restrict__ nodes4, const float4* __restrict__ tris,
+                            int32_t root, const Ray& r, bool want, int32_t skip_slot, Hit& h) {
+    h.t = r.tmax;
+    h.u = 0.0f;
+    h.v = 0.0f;
+    h.prim = -1;
+    h.slot = -1;
+    uint64_t m = __ballot(want);
+    if (root == RT_EMPTY_ROOT || m == 0) return;
+    uint64_t alive = m;
+    const uint32_t lane = __lane_id();
+    const float idx = safe_inv(r.dx), idy = safe_inv(r.dy), idz = safe_inv(r.dz);
+    const float oxx = r.ox * idx, oyy = r.oy * idy, ozz = r.oz * idz;
+    int32_t stk_node0 = 0, stk_node1 = 0;
+    uint32_t stk_mlo0 = 0, stk_mlo1 = 0, stk_mhi0 = 0, stk_mhi1 = 0;
+    int32_t sp = 0;
+    int32_t cur = root;
+    for (;;) {
+        if (cur >= 0) {
+            const float4* nd = nodes4 + (size_t)cur * 8;
+            const float4 lx = nd[0], hx = nd[1], ly = nd[2], hy = nd[3], lz = nd[4], hz = nd[5];
+            const float4 rf = nd[6];
+            const bool mine = (m >> lane) & 1u;
+            const float clx[4] = {lx.x, lx.y, lx.z, lx.w}, chx[4] = {hx.x, hx.y, hx.z, hx.w};
+            const float cly[4] = {ly.x, ly.y, ly.z, ly.w}, chy[4] = {hy.x, hy.y, hy.z, hy.w};
+            const float clz[4] = {lz.x, lz.y, lz.z, lz.w}, chz[4] = {hz.x, hz.y, hz.z, hz.w};
+            const int32_t ref[4] = {__float_as_int(rf.x), __float_as_int(rf.y), __float_as_int(rf.z),
+                                    __float_as_int(rf.w)};
+            struct Child {
+                float key;
+                int32_t k, ref;
+                uint64_t mask;
+            };
+            Child c[4];
+            const int rep = __ffsll((unsigned long long)m) - 1;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const float x0 = fmaf(clx[k], idx, -oxx), x1 = fmaf(chx[k], idx, -oxx);
+                const float y0 = fmaf(cly[k], idy, -oyy), y1 = fmaf(chy[k], idy, -oyy);
+                const float z0 = fmaf(clz[k], idz, -ozz), z1 = fmaf(chz[k], idz, -ozz);
+                const float tn = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), r.tmin));
+                const float tf = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fmaxf(z0, z1)) * kTfarScale;
+                const bool hk = mine && ref[k] != RT_EMPTY_ROOT && tn <= fminf(tf, h.t);
+                c[k].mask = __ballot(hk);
+                c[k].key = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(hk ? tn : INFINITY), rep));
+                c[k].k = k;
+                c[k].ref = ref[k];
+            }
+            // order by (key, k): 4-element sorting network over whole records (no indexing)
+            auto cswap = [](Child& a, Child& b) {
+                if (b.key < a.key || (b.key == a.key && b.k < a.k)) {
+                    const Child t = a;
+                    a = b;
+                    b = t;
+                }
+            };
+            cswap(c[0], c[1]);
+            cswap(c[2], c[3]);
+            cswap(c[0], c[2]);
+            cswap(c[1], c[3]);
+            cswap(c[1], c[2]);
+            int32_t next = -1;
+            uint64_t next_m = 0;
+            // push in reverse order so the nearest child after the first pops first
+#pragma unroll
+            for (int q = 3; q >= 0; --q) {
+                if (c[q].mask == 0) continue;
+                if (next_m != 0 && sp < kWideStack) {
+                    const bool at0 = (int32_t)lane == sp, at1 = (int32_t)lane + 64 == sp;
+                    stk_node0 = at0 ? next : stk_node0;
+                    stk_mlo0 = at0 ? (uint32_t)next_m : stk_mlo0;
+                    stk_mhi0 = at0 ? (uint32_t)(next_m >> 32) : stk_mhi0;
+                    stk_node1 = at1 ? next : stk_node1;
+                    stk_mlo1 = at1 ? (uint32_t)next_m : stk_mlo1;
+                    stk_mhi1 = at1 ? (uint32_t)(next_m >> 32) : stk_mhi1;
+                    ++sp;
+                }
+                next = c[q].ref;
+                next_m = c[q].mask;
+            }
+            if (next_m != 0) {
+                cur = next;
+                m = next_m;
+                continue;
+            }
+        } else {
+            const uint32_t first = leaf_first(cur), count = leaf_count(cur);
+#pragma unroll
+            for (uint32_t k = 0; k < (uint32_t)kLeafMax; ++k) {
+                if (k >= count) break;
+                const int32_t slot = (int32_t)(first + k);
+                const float4* tr = tris + (size_t)slot * 3;
+                const float4 ta = tr[0], tb = tr[1], tc = tr[2];
+                const bool mine = ((m >> lane) & 1u) && slot != skip_slot;
+                float t, u, v;
+                if (mine && intersect_tri(ta, tb, tc, r, t, u, v) && t > r.tmin) {
+                    const int32_t prim = __float_as_int(ta.w);
+                    if (t < h.t || (t == h.t && prim < h.prim)) {
+                        h.t = t;
+                        h.u = u;
+                        h.v = v;
+                        h.prim = prim;
+                        h.slot = slot;
+                    }
+                }
+            }
+            if (ANY) {
+                alive &= ~__ballot(h.prim >= 0);
+                if (alive == 0) return;
+            }
+        }
+        for (;;) {
+            if (sp == 0) return;
+            --sp;
+            const bool hi = sp >= 64;
+            const int32_t lsp = hi ? sp - 64 : sp;
+            cur = __builtin_amdgcn_readlane(hi ? stk_node1 : stk_node0, lsp);
+            const uint32_t mlo = (uint32_t)__builtin_amdgcn_readlane((int)(hi ? stk_mlo1 : stk_mlo0), lsp);
+            const uint32_t mhi = (uint32_t)__builtin_amdgcn_readlane((int)(hi ? stk_mhi1 : stk_mhi0), lsp);
+            m = ((uint64_t)mhi << 32) | mlo;
+            if (ANY) m &= alive;
+            if (m != 0) break;
+        }
     }
 }
 

@@ -1,15 +1,16 @@
-// rt_trace.hip — closest-hit / any-hit queries and the two-kernel render path.
+// rt_trace.hip — closest-hit / any-hit queries and the render path.
 //
 // Queries (rendering/_raycaster.py:35-36, Raycaster.ray_cast — a stub in the reference):
-//   k_trace_closest / k_trace_any: one lane per ray, rays and hits read/written through
-//   (pointer, byte stride) fields so SoA (stride 4) and the AoS Ray/Hit records both coalesce.
-// Render:
-//   k_render_primary: each wave64 owns an 8x8 pixel tile (a block = 16x16 pixels) so the lanes of
-//     a wave start with nearly identical traversals; rays are generated in registers, traced, and
-//     the hit lanes are appended to a compacted hit queue (ballot + mbcnt, one atomic per wave).
-//     Miss pixels get the background colour directly.
-//   k_shade: persistent grid-stride pass over the compacted queue — fused Lambert, shadow ray
-//     (any-hit, skipping the hit's own triangle) and RGBA8 texture fetch; writes RGB float32.
+//   k_trace_closest / k_trace_any: one lane per ray with its own stack (rays may be incoherent);
+//   rays and hits are read/written through (pointer, byte stride) fields so SoA (stride 4) and
+//   the AoS Ray/Hit records both coalesce.
+// Camera rays:
+//   each wave64 owns an 8x8 pixel tile (a 256-thread block = 16x16 pixels); rays are generated in
+//   registers and traced with the wave-coherent traversal (scalar node fetches, one VGPR stack).
+//   k_trace_camera writes per-pixel hit records.
+//   k_render fuses the whole frame: primary closest hit, FP64 hit refinement, Lambert, one shadow
+//   ray toward the light per lit hit (wave-coherent any-hit over the tile's hit points, skipping
+//   each hit's own triangle), RGBA8 texture fetch, RGB float32 store.
 #include <algorithm>
 
 #include "rt_common.h"
@@ -38,6 +39,7 @@ RT_DEV int32_t mesh_of_prim(const int32_t* __restrict__ offs, uint32_t n_meshes,
 
 struct SceneArgs {
     const float4* nodes;
+    const float4* nodes4;
     const float4* tris;
     const float4* shade;
     const int32_t* mesh_offsets;
@@ -45,10 +47,7 @@ struct SceneArgs {
     uint32_t n_meshes;
 };
 
-__global__ void __launch_bounds__(256) k_trace_closest(SceneArgs sc, rt_rays rays, uint32_t n,
-                                                        rt_hits hits) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+RT_DEV Ray load_ray(const rt_rays& rays, uint64_t i) {
     Ray r;
     r.ox = ld_f(rays.ox, i, rays.stride);
     r.oy = ld_f(rays.oy, i, rays.stride);
@@ -58,12 +57,13 @@ __global__ void __launch_bounds__(256) k_trace_closest(SceneArgs sc, rt_rays ray
     r.dz = ld_f(rays.dz, i, rays.stride);
     r.tmin = rays.tmin ? ld_f(rays.tmin, i, rays.stride) : rays.tmin_default;
     r.tmax = rays.tmax ? ld_f(rays.tmax, i, rays.stride) : rays.tmax_default;
-    Hit h;
-    traverse<false>(sc.nodes, sc.tris, sc.root, r, -1, h);
-    if (h.prim >= 0) refine_hit(sc.tris, r, h);
+    return r;
+}
+
+RT_DEV void store_hit(const SceneArgs& sc, const rt_hits& hits, uint64_t i, const Hit& h) {
     const bool hit = h.prim >= 0;
     int32_t mesh = -1, index = -1;
-    if (hit) {
+    if (hit && (hits.mesh || hits.index)) {
         mesh = mesh_of_prim(sc.mesh_offsets, sc.n_meshes, h.prim);
         index = h.prim - sc.mesh_offsets[mesh];
     }
@@ -75,19 +75,22 @@ __global__ void __launch_bounds__(256) k_trace_closest(SceneArgs sc, rt_rays ray
     st(hits.mesh, i, hits.stride, mesh);
 }
 
+__global__ void __launch_bounds__(256) k_trace_closest(SceneArgs sc, rt_rays rays, uint32_t n,
+                                                        rt_hits hits) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const Ray r = load_ray(rays, i);
+    Hit h;
+    traverse<false>(sc.nodes, sc.tris, sc.root, r, -1, h);
+    if (h.prim >= 0) refine_hit(sc.tris, r, h);
+    store_hit(sc, hits, i, h);
+}
+
 __global__ void __launch_bounds__(256) k_trace_any(SceneArgs sc, rt_rays rays, uint32_t n,
                                                     uint8_t* __restrict__ occluded) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    Ray r;
-    r.ox = ld_f(rays.ox, i, rays.stride);
-    r.oy = ld_f(rays.oy, i, rays.stride);
-    r.oz = ld_f(rays.oz, i, rays.stride);
-    r.dx = ld_f(rays.dx, i, rays.stride);
-    r.dy = ld_f(rays.dy, i, rays.stride);
-    r.dz = ld_f(rays.dz, i, rays.stride);
-    r.tmin = rays.tmin ? ld_f(rays.tmin, i, rays.stride) : rays.tmin_default;
-    r.tmax = rays.tmax ? ld_f(rays.tmax, i, rays.stride) : rays.tmax_default;
+    const Ray r = load_ray(rays, i);
     Hit h;
     traverse<true>(sc.nodes, sc.tris, sc.root, r, -1, h);
     occluded[i] = h.prim >= 0 ? 1 : 0;
@@ -108,123 +111,44 @@ __global__ void __launch_bounds__(256) k_generate_rays(rt_camera cam, uint32_t W
     dz[i] = z;
 }
 
-// Render workspace: counters (256 B) then the SoA hit queue.
-struct Queue {
-    uint32_t* counter;  // [0] = queue length
-    uint32_t* pix;
-    uint32_t* slot;
-    float* u;
-    float* v;
-    float* t;
-};
-
-inline uint64_t a256(uint64_t x) { return (x + 255) & ~uint64_t(255); }
-
-Queue carve_queue(char* base, uint64_t n) {
-    Queue q{};
-    uint64_t off = 0;
-    auto take = [&](uint64_t bytes) {
-        char* p = base ? base + off : nullptr;
-        off = a256(off + bytes);
-        return p;
-    };
-    q.counter = (uint32_t*)take(256);
-    q.pix = (uint32_t*)take(4 * n);
-    q.slot = (uint32_t*)take(4 * n);
-    q.u = (float*)take(4 * n);
-    q.v = (float*)take(4 * n);
-    q.t = (float*)take(4 * n);
-    return q;
-}
-
-uint64_t queue_bytes(uint64_t n) {
-    uint64_t off = a256(256);
-    off += 5 * a256(4 * n);
-    return off;
-}
-
-__global__ void k_zero_u32(uint32_t* c, uint32_t n) {
-    if (threadIdx.x < n) c[threadIdx.x] = 0;
-}
-
 constexpr int kTile = 16;  // block = 16x16 pixels, wave = 8x8
 
-__global__ void __launch_bounds__(256) k_render_primary(SceneArgs sc, rt_camera cam, uint32_t W,
-                                                         uint32_t H, uint32_t tiles_x,
-                                                         float bg0, float bg1, float bg2,
-                                                         float* __restrict__ rgb,
-                                                         float* __restrict__ out_t,
-                                                         int32_t* __restrict__ out_prim, Queue q) {
+struct TilePixel {
+    uint32_t col, row;
+    bool valid;
+};
+
+RT_DEV TilePixel tile_pixel(uint32_t W, uint32_t H, uint32_t tiles_x) {
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const uint32_t tx = blockIdx.x % tiles_x, ty = blockIdx.x / tiles_x;
-    const uint32_t col = tx * kTile + (wave & 1) * 8 + (lane & 7);
-    const uint32_t row = ty * kTile + (wave >> 1) * 8 + (lane >> 3);
-    const bool valid = col < W && row < H;
-    Hit h;
-    h.prim = -1;
-    if (valid) {
-        Ray r;
-        r.ox = cam.eye[0];
-        r.oy = cam.eye[1];
-        r.oz = cam.eye[2];
-        camera_dir(cam, W, H, col, row, r.dx, r.dy, r.dz);
-        r.tmin = 0.0f;
-        r.tmax = INFINITY;
-        traverse<false>(sc.nodes, sc.tris, sc.root, r, -1, h);
-        if (h.prim >= 0) refine_hit(sc.tris, r, h);
-    }
-    const bool hit = valid && h.prim >= 0;
-    const uint64_t pix = (uint64_t)row * W + col;
-    if (valid) {
-        if (out_t) out_t[pix] = hit ? h.t : INFINITY;
-        if (out_prim) out_prim[pix] = h.prim;
-        if (!hit) {
-            rgb[3 * pix] = bg0;
-            rgb[3 * pix + 1] = bg1;
-            rgb[3 * pix + 2] = bg2;
-        }
-    }
-    const uint32_t k = wave_append(q.counter, hit);
-    if (hit) {
-        q.pix[k] = (uint32_t)pix;
-        q.slot[k] = (uint32_t)h.slot;
-        q.u[k] = h.u;
-        q.v[k] = h.v;
-        q.t[k] = h.t;
-    }
+    TilePixel p;
+    p.col = tx * kTile + (wave & 1) * 8 + (lane & 7);
+    p.row = ty * kTile + (wave >> 1) * 8 + (lane >> 3);
+    p.valid = p.col < W && p.row < H;
+    return p;
+}
+
+RT_DEV Ray camera_ray(const rt_camera& cam, uint32_t W, uint32_t H, const TilePixel& p) {
+    Ray r;
+    r.ox = cam.eye[0];
+    r.oy = cam.eye[1];
+    r.oz = cam.eye[2];
+    const uint32_t col = p.valid ? p.col : 0, row = p.valid ? p.row : 0;
+    camera_dir(cam, W, H, col, row, r.dx, r.dy, r.dz);
+    r.tmin = 0.0f;
+    r.tmax = INFINITY;
+    return r;
 }
 
 // Camera closest hit: ray generation fused into the traversal, one hit record per pixel.
 __global__ void __launch_bounds__(256) k_trace_camera(SceneArgs sc, rt_camera cam, uint32_t W,
                                                        uint32_t H, uint32_t tiles_x, rt_hits hits) {
-    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const uint32_t tx = blockIdx.x % tiles_x, ty = blockIdx.x / tiles_x;
-    const uint32_t col = tx * kTile + (wave & 1) * 8 + (lane & 7);
-    const uint32_t row = ty * kTile + (wave >> 1) * 8 + (lane >> 3);
-    if (col >= W || row >= H) return;
-    Ray r;
-    r.ox = cam.eye[0];
-    r.oy = cam.eye[1];
-    r.oz = cam.eye[2];
-    camera_dir(cam, W, H, col, row, r.dx, r.dy, r.dz);
-    r.tmin = 0.0f;
-    r.tmax = INFINITY;
+    const TilePixel p = tile_pixel(W, H, tiles_x);
+    const Ray r = camera_ray(cam, W, H, p);
     Hit h;
-    traverse<false>(sc.nodes, sc.tris, sc.root, r, -1, h);
+    traverse_packet<false>(sc.nodes4, sc.tris, sc.root, r, p.valid, -1, h);
     if (h.prim >= 0) refine_hit(sc.tris, r, h);
-    const bool hit = h.prim >= 0;
-    const uint64_t i = (uint64_t)row * W + col;
-    int32_t mesh = -1, index = -1;
-    if (hit && (hits.mesh || hits.index)) {
-        mesh = mesh_of_prim(sc.mesh_offsets, sc.n_meshes, h.prim);
-        index = h.prim - sc.mesh_offsets[mesh];
-    }
-    st(hits.t, i, hits.stride, hit ? h.t : INFINITY);
-    st(hits.u, i, hits.stride, hit ? h.u : 0.0f);
-    st(hits.v, i, hits.stride, hit ? h.v : 0.0f);
-    st(hits.prim, i, hits.stride, h.prim);
-    st(hits.index, i, hits.stride, index);
-    st(hits.mesh, i, hits.stride, mesh);
+    if (p.valid) store_hit(sc, hits, (uint64_t)p.row * W + p.col, h);
 }
 
 RT_DEV int wrap_i(int i, int n) {
@@ -271,81 +195,94 @@ RT_DEV void tex_sample(const rt_texture& tx, float u, float v, bool bilinear, fl
     }
 }
 
-RT_DEV uint32_t wave_sum(uint32_t x) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off);
-    return x;
-}
+// Fused render of one frame (see the file comment).
+__global__ void __launch_bounds__(256) k_render(SceneArgs sc, rt_camera cam, rt_render_params p,
+                                                 uint32_t tiles_x, float* __restrict__ rgb,
+                                                 float* __restrict__ out_t,
+                                                 int32_t* __restrict__ out_prim,
+                                                 uint32_t* __restrict__ stats) {
+    const uint32_t W = p.width, H = p.height;
+    const TilePixel px = tile_pixel(W, H, tiles_x);
+    const Ray r = camera_ray(cam, W, H, px);
+    Hit h;
+    traverse_packet<false>(sc.nodes4, sc.tris, sc.root, r, px.valid, -1, h);
+    const bool hit = px.valid && h.prim >= 0;
+    if (hit) refine_hit(sc.tris, r, h);
 
-__global__ void __launch_bounds__(256) k_shade(SceneArgs sc, rt_camera cam, uint32_t W,
-                                                uint32_t H, rt_render_params p,
-                                                float* __restrict__ rgb, Queue q,
-                                                uint32_t* __restrict__ stats) {
-    const uint32_t count = q.counter[0];
-    uint32_t n_shadow = 0, n_occl = 0;
-    const bool shadows = (p.flags & RT_RENDER_SHADOWS) != 0;
-    const bool bilinear = (p.flags & RT_RENDER_BILINEAR) != 0;
-    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < count; k += gridDim.x * blockDim.x) {
-        const uint32_t pix = q.pix[k];
-        const int32_t slot = (int32_t)q.slot[k];
-        const float u = q.u[k], v = q.v[k], t = q.t[k];
-        const float4* s = sc.shade + (uint64_t)slot * 4;
+    float ndl = 0.0f, tu = 0.0f, tv = 0.0f;
+    int32_t mesh = 0;
+    if (hit) {
+        const float4* s = sc.shade + (uint64_t)h.slot * 4;
         const float4 s0 = s[0], s1 = s[1], s2 = s[2], s3 = s[3];
-        const float w0 = 1.0f - u - v;
-        const float nx = fmaf(w0, s0.x, fmaf(u, s0.w, v * s1.z));
-        const float ny = fmaf(w0, s0.y, fmaf(u, s1.x, v * s1.w));
-        const float nz = fmaf(w0, s0.z, fmaf(u, s1.y, v * s2.x));
+        const float w0 = 1.0f - h.u - h.v;
+        const float nx = fmaf(w0, s0.x, fmaf(h.u, s0.w, h.v * s1.z));
+        const float ny = fmaf(w0, s0.y, fmaf(h.u, s1.x, h.v * s1.w));
+        const float nz = fmaf(w0, s0.z, fmaf(h.u, s1.y, h.v * s2.x));
         const float l2 = fmaf(nx, nx, fmaf(ny, ny, nz * nz));
         const float il = l2 > 0.0f ? 1.0f / sqrtf(l2) : 0.0f;
-        float ndl = fmaf(nx * il, p.light[0], fmaf(ny * il, p.light[1], (nz * il) * p.light[2]));
+        ndl = fmaf(nx * il, p.light[0], fmaf(ny * il, p.light[1], (nz * il) * p.light[2]));
         ndl = fmaxf(ndl, 0.0f);
-        float vis = 1.0f;
-        if (shadows && ndl > 0.0f) {
-            float dx, dy, dz;
-            camera_dir(cam, W, H, pix % W, pix / W, dx, dy, dz);
-            Ray r;
-            r.ox = fmaf(t, dx, cam.eye[0]);
-            r.oy = fmaf(t, dy, cam.eye[1]);
-            r.oz = fmaf(t, dz, cam.eye[2]);
-            r.dx = p.light[0];
-            r.dy = p.light[1];
-            r.dz = p.light[2];
-            r.tmin = p.shadow_tmin;
-            r.tmax = INFINITY;
-            Hit sh;
-            traverse<true>(sc.nodes, sc.tris, sc.root, r, slot, sh);
-            ++n_shadow;
-            if (sh.prim >= 0) {
-                vis = 0.0f;
-                ++n_occl;
-            }
+        tu = fmaf(w0, s2.y, fmaf(h.u, s2.w, h.v * s3.y));
+        tv = fmaf(w0, s2.z, fmaf(h.u, s3.x, h.v * s3.z));
+        mesh = __float_as_int(s3.w);
+    }
+
+    // shadow ray toward the light from every lit hit of the tile
+    const bool want_shadow = hit && (p.flags & RT_RENDER_SHADOWS) && ndl > 0.0f;
+    Ray sr;
+    sr.ox = fmaf(h.t, r.dx, r.ox);
+    sr.oy = fmaf(h.t, r.dy, r.oy);
+    sr.oz = fmaf(h.t, r.dz, r.oz);
+    sr.dx = p.light[0];
+    sr.dy = p.light[1];
+    sr.dz = p.light[2];
+    sr.tmin = p.shadow_tmin;
+    sr.tmax = INFINITY;
+    Hit sh;
+    traverse_packet<true>(sc.nodes4, sc.tris, sc.root, sr, want_shadow, hit ? h.slot : -1, sh);
+    const bool occluded = want_shadow && sh.prim >= 0;
+
+    float alb[3] = {1.0f, 1.0f, 1.0f};
+    if (hit && p.mesh_texture_dev) {
+        const int32_t ti = p.mesh_texture_dev[mesh];
+        if (ti >= 0 && (uint32_t)ti < p.n_textures)
+            tex_sample(p.textures_dev[ti], tu, tv, (p.flags & RT_RENDER_BILINEAR) != 0, alb);
+    }
+    if (px.valid) {
+        const uint64_t pix = (uint64_t)px.row * W + px.col;
+        if (hit) {
+            const float kk = fmaf(occluded ? 0.0f : 1.0f, ndl, p.ambient);
+            rgb[3 * pix] = alb[0] * kk;
+            rgb[3 * pix + 1] = alb[1] * kk;
+            rgb[3 * pix + 2] = alb[2] * kk;
+        } else {
+            rgb[3 * pix] = p.background[0];
+            rgb[3 * pix + 1] = p.background[1];
+            rgb[3 * pix + 2] = p.background[2];
         }
-        float alb[3] = {1.0f, 1.0f, 1.0f};
-        const int32_t mesh = __float_as_int(s3.w);
-        const int32_t ti = p.mesh_texture_dev ? p.mesh_texture_dev[mesh] : -1;
-        if (ti >= 0 && (uint32_t)ti < p.n_textures) {
-            const float tu = fmaf(w0, s2.y, fmaf(u, s2.w, v * s3.y));
-            const float tv = fmaf(w0, s2.z, fmaf(u, s3.x, v * s3.z));
-            tex_sample(p.textures_dev[ti], tu, tv, bilinear, alb);
-        }
-        const float kk = fmaf(vis, ndl, p.ambient);
-        rgb[3 * (uint64_t)pix] = alb[0] * kk;
-        rgb[3 * (uint64_t)pix + 1] = alb[1] * kk;
-        rgb[3 * (uint64_t)pix + 2] = alb[2] * kk;
+        if (out_t) out_t[pix] = hit ? h.t : INFINITY;
+        if (out_prim) out_prim[pix] = hit ? h.prim : -1;
     }
     if (stats) {
-        const uint32_t a = wave_sum(n_shadow), b = wave_sum(n_occl);
+        const uint32_t n_hit = (uint32_t)__popcll(__ballot(hit));
+        const uint32_t n_sh = (uint32_t)__popcll(__ballot(want_shadow));
+        const uint32_t n_oc = (uint32_t)__popcll(__ballot(occluded));
         if ((threadIdx.x & 63) == 0) {
-            if (a) atomicAdd(&stats[1], a);
-            if (b) atomicAdd(&stats[2], b);
+            if (n_hit) atomicAdd(&stats[0], n_hit);
+            if (n_sh) atomicAdd(&stats[1], n_sh);
+            if (n_oc) atomicAdd(&stats[2], n_oc);
         }
-        if (blockIdx.x == 0 && threadIdx.x == 0) stats[0] = count;
     }
+}
+
+__global__ void k_zero_u32(uint32_t* c, uint32_t n) {
+    if (threadIdx.x < n) c[threadIdx.x] = 0;
 }
 
 SceneArgs scene_args(const rt_scene* s) {
     SceneArgs a;
     a.nodes = (const float4*)s->nodes;
+    a.nodes4 = (const float4*)s->nodes4;
     a.tris = (const float4*)s->tris;
     a.shade = (const float4*)s->shade;
     a.mesh_offsets = s->mesh_offsets;
@@ -356,7 +293,7 @@ SceneArgs scene_args(const rt_scene* s) {
 
 int check_scene(const rt_scene* s, const char* what) {
     if (!s) return set_error(RT_EINVAL, std::string(what) + ": scene is NULL");
-    if (s->root != RT_EMPTY_ROOT && (!s->nodes || !s->tris || !s->mesh_offsets))
+    if (s->root != RT_EMPTY_ROOT && (!s->nodes || !s->nodes4 || !s->tris || !s->shade || !s->mesh_offsets))
         return set_error(RT_EINVAL, std::string(what) + ": scene arrays are NULL");
     if (s->root != RT_EMPTY_ROOT && s->n_tris == 0)
         return set_error(RT_EINVAL, std::string(what) + ": scene root set but n_tris is 0");
@@ -430,11 +367,16 @@ extern "C" int rt_trace_camera(const rt_scene* scene, const rt_camera* cam, uint
     return check_launch("rt_trace_camera");
 }
 
-extern "C" uint64_t rt_render_workspace_bytes(uint32_t n_pixels) { return queue_bytes(n_pixels); }
+extern "C" uint64_t rt_render_workspace_bytes(uint32_t n_pixels) {
+    (void)n_pixels;
+    return 0;
+}
 
 extern "C" int rt_render(const rt_scene* scene, const rt_camera* cam, const rt_render_params* p,
                          const rt_render_out* out, void* workspace, uint64_t ws_bytes,
                          void* stream) {
+    (void)workspace;
+    (void)ws_bytes;
     int e = check_scene(scene, "rt_render");
     if (e) return e;
     if (!cam || !p || !out || !out->rgb) return set_error(RT_EINVAL, "rt_render: NULL argument");
@@ -443,21 +385,10 @@ extern "C" int rt_render(const rt_scene* scene, const rt_camera* cam, const rt_r
     if (n > 0x7FFFFFFFull) return set_error(RT_ECAPACITY, "rt_render: image too large");
     if (p->n_textures && !p->textures_dev)
         return set_error(RT_EINVAL, "rt_render: n_textures > 0 but textures_dev is NULL");
-    if (!workspace || ws_bytes < queue_bytes(n))
-        return set_error(RT_EINVAL, "rt_render: workspace too small (need " +
-                                        std::to_string(queue_bytes(n)) + " bytes)");
-    if ((uintptr_t)workspace & 255) return set_error(RT_EINVAL, "rt_render: workspace must be 256-byte aligned");
     hipStream_t s = (hipStream_t)stream;
-    Queue q = carve_queue((char*)workspace, n);
-    const SceneArgs sa = scene_args(scene);
-    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, q.counter, 64u);
     if (out->stats) hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, out->stats, 4u);
     const uint32_t tiles_x = (p->width + kTile - 1) / kTile, tiles_y = (p->height + kTile - 1) / kTile;
-    hipLaunchKernelGGL(k_render_primary, dim3(tiles_x * tiles_y), dim3(256), 0, s, sa, *cam, p->width,
-                       p->height, tiles_x, p->background[0], p->background[1], p->background[2],
-                       out->rgb, out->t, out->prim, q);
-    const uint32_t grid = std::min<uint32_t>(blocks_for(n, 256), (uint32_t)device_cu_count() * 8);
-    hipLaunchKernelGGL(k_shade, dim3(grid), dim3(256), 0, s, sa, *cam, p->width, p->height, *p,
-                       out->rgb, q, out->stats);
+    hipLaunchKernelGGL(k_render, dim3(tiles_x * tiles_y), dim3(256), 0, s, scene_args(scene), *cam, *p,
+                       tiles_x, out->rgb, out->t, out->prim, out->stats);
     return check_launch("rt_render");
 }

@@ -46,6 +46,7 @@ def assert_build_equal(rc, tw):
     np.testing.assert_array_equal(_u32(g["tris"]), _u32(tw.tris))
     np.testing.assert_array_equal(_u32(g["shade"]), _u32(tw.shade))
     np.testing.assert_array_equal(_u32(g["nodes"]), _u32(tw.nodes))
+    np.testing.assert_array_equal(_u32(g["nodes4"]), _u32(tw.nodes4))
     np.testing.assert_array_equal(_u32(g["bounds"]), _u32(tw.bounds))
     assert g["mesh_offsets"][-1] == n
 
@@ -93,8 +94,10 @@ def test_trace_c1_primary_matches_twin_and_fp64(c1, twin):
     g = gpu_trace(rc, o, d)
     t = tw.trace_closest(o, d)
     assert_trace_equal(g, t)
-    # the fused camera kernel gives the same hits
+    # the camera kernel (wave-coherent traversal) matches its twin emulation and the per-ray twin
     gc = {k: v.cpu().numpy() for k, v in rc.cast_camera(cam, 512, 512).items()}
+    tc = tw.trace_camera(cam.as_array(), 512, 512)
+    assert_trace_equal(gc, tc)
     assert_trace_equal(gc, t)
     # FP64: identical IDs except flagged rays, t within 1e-5 relative
     fp64_report("C1 primary 512x512", g["prim"], g["t"], tw.closest64(o, d))
@@ -231,9 +234,10 @@ def test_render_c2_1080p_matches_twin_and_fp64(c2, twin):
     np.testing.assert_array_equal(_u32(hits["t"].cpu().numpy()), _u32(t))
     np.testing.assert_array_equal(_u32(img.cpu().numpy()), _u32(rgb))
     assert list(stats.cpu().numpy()[:3]) == list(st)
-    # the fused camera kernel gives the same hits
-    gc = rc.cast_camera(cam, W, H)
-    np.testing.assert_array_equal(gc["prim"].cpu().numpy(), prim)
+    # the camera kernel gives the same hits as the render's primary pass and the twin
+    gc = {k: v.cpu().numpy() for k, v in rc.cast_camera(cam, W, H).items()}
+    np.testing.assert_array_equal(gc["prim"], prim)
+    assert_trace_equal(gc, tw.trace_camera(cam.as_array(), W, H))
     # primary IDs and t against FP64 on the full frame
     o, d = twin.twin_generate_rays(cam.as_array(), W, H)
     fp64_report("C2 primary 1920x1080", prim, t, tw.closest64(o, d))
