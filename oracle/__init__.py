@@ -62,6 +62,8 @@ def lib():
         L.tw_closest64_brute.argtypes = [vp, i64, i64] + [vp] * 6 + [f64, f64, f64] + [vp] * 5
         L.tw_closest64_bvh.restype = None
         L.tw_closest64_bvh.argtypes = [vp, i64, vp, vp, i32, i64] + [vp] * 6 + [f64, f64, f64, f64] + [vp] * 5
+        L.tw_tex_sample.restype = None
+        L.tw_tex_sample.argtypes = [vp, i32, i32, i64, vp, vp, i32, vp]
         L.tw_num_threads.restype = ctypes.c_int
         L.tw_set_threads.argtypes = [ctypes.c_int]
         _lib = L
@@ -267,6 +269,16 @@ class TwinScene:
         return closest64(self.pos, o, d, tmin, edge_eps, tie_eps, margin, None if brute else self)
 
 
+def twin_tex_sample(tex, u, v, bilinear=True):
+    """The device's RGBA8 texture fetch at (u, v) arrays: RGB float32 [n, 3]."""
+    tex = np.ascontiguousarray(tex, np.uint8)
+    u = np.ascontiguousarray(u, np.float32)
+    v = np.ascontiguousarray(v, np.float32)
+    out = np.empty((u.size, 3), np.float32)
+    lib().tw_tex_sample(_p(tex), tex.shape[1], tex.shape[0], u.size, _p(u), _p(v), int(bilinear), _p(out))
+    return out
+
+
 def twin_generate_rays(cam12, W, H):
     n = W * H
     cols = [np.empty(n, np.float32) for _ in range(6)]
@@ -304,3 +316,63 @@ def closest64(tri_pos, o, d, tmin=0.0, edge_eps=1e-6, tie_eps=1e-5, margin=1e-4,
                                scene.root, n, *[_p(c) for c in cols], tmin, edge_eps, tie_eps,
                                margin, _p(t), _p(prim), _p(u), _p(v), _p(edge))
     return {"t": t, "prim": prim, "u": u, "v": v, "edge": edge}
+
+
+def shade64(tri_pos, tri_nrm, tri_uv, mesh_of, has_nrm, o, d, light, ambient=0.2,
+            background=(0.0, 0.0, 0.0), shadow_tmin=1e-4, shadows=True, textures=(),
+            mesh_tex=None, scene=None):
+    """FP64 shading of rays o + t d (the device's shading formula evaluated in FP64 on the FP64
+    closest hit): rgb [n, 3] float64 and flags [n] (bit set of the primary ray's FP64 flags,
+    plus 8 when the shadow ray is ambiguous). rgb = albedo * (ambient + V * max(0, N.L)), N the
+    normalised barycentric interpolation (w0, u, v) of the vertex normals (geometric normal for
+    meshes without normals), albedo the OpenCL linear filter of the mesh's RGBA8 texture (1 when
+    none), V = 0 when the shadow ray from the hit point toward L hits anything beyond shadow_tmin."""
+    tri_pos = np.ascontiguousarray(tri_pos, np.float32)
+    prim = closest64(tri_pos, o, d, scene=scene)
+    n = len(o)
+    rgb = np.tile(np.asarray(background, np.float64), (n, 1))
+    flags = prim["edge"].astype(np.int32)
+    hit = prim["prim"] >= 0
+    p = prim["prim"][hit]
+    u = prim["u"][hit]
+    v = prim["v"][hit]
+    w0 = 1.0 - u - v
+    P = tri_pos[p].astype(np.float64).reshape(-1, 3, 3)
+    Nv = np.asarray(tri_nrm, np.float32)[p].astype(np.float64).reshape(-1, 3, 3)
+    geo = np.cross(P[:, 1] - P[:, 0], P[:, 2] - P[:, 0])
+    geo /= np.maximum(np.linalg.norm(geo, axis=1, keepdims=True), 1e-300)
+    mh = np.asarray(has_nrm, bool)[np.asarray(mesh_of)[p]]
+    N = w0[:, None] * Nv[:, 0] + u[:, None] * Nv[:, 1] + v[:, None] * Nv[:, 2]
+    N = np.where(mh[:, None], N, geo)
+    ln = np.linalg.norm(N, axis=1, keepdims=True)
+    N = np.where(ln > 0, N / np.where(ln > 0, ln, 1), 0)
+    L = np.asarray(light, np.float64)
+    ndl = np.maximum(N @ L, 0.0)
+    vis = np.ones(len(p))
+    if shadows:
+        lit = ndl > 0
+        hp = o[hit].astype(np.float64) + prim["t"][hit][:, None] * d[hit].astype(np.float64)
+        so = hp[lit].astype(np.float32)
+        sd = np.tile(np.asarray(light, np.float32), (int(lit.sum()), 1))
+        sh = closest64(tri_pos, so, sd, tmin=shadow_tmin, scene=scene)
+        occ = sh["prim"] >= 0
+        vis_l = np.where(occ, 0.0, 1.0)
+        vis[lit] = vis_l
+        sflag = np.zeros(len(p), np.int32)
+        sflag[lit] = np.where(sh["edge"] != 0, 8, 0)
+        # a shadow ray starting within the start offset of a surface is ambiguous too
+        near = np.zeros(len(p), bool)
+        near[lit] = np.abs(sh["t"] - shadow_tmin) < 1e-5
+        flags[np.nonzero(hit)[0]] |= sflag | np.where(near, 8, 0)
+    alb = np.ones((len(p), 3))
+    if textures and mesh_tex is not None:
+        UV = np.asarray(tri_uv, np.float32)[p].astype(np.float64).reshape(-1, 3, 2)
+        tu = w0 * UV[:, 0, 0] + u * UV[:, 1, 0] + v * UV[:, 2, 0]
+        tv = w0 * UV[:, 0, 1] + u * UV[:, 1, 1] + v * UV[:, 2, 1]
+        ti = np.asarray(mesh_tex)[np.asarray(mesh_of)[p]]
+        for k, tex in enumerate(textures):
+            sel = ti == k
+            if sel.any():
+                alb[sel] = bilinear_fp64(np.asarray(tex), tu[sel], tv[sel])
+    rgb[hit] = alb * (ambient + vis * ndl)[:, None]
+    return rgb, flags

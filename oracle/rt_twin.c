@@ -584,6 +584,12 @@ static inline void tex_sample(const uint8_t* tex, int W, int H, float u, float v
     }
 }
 
+/* Texture fetch of the device shading kernel, exposed for known-answer tests. */
+TW_API void tw_tex_sample(const uint8_t* tex, int32_t W, int32_t H, int64_t n, const float* u,
+                          const float* v, int32_t bilinear, float* rgb) {
+    for (int64_t i = 0; i < n; ++i) tex_sample(tex, W, H, u[i], v[i], bilinear, rgb + 3 * i);
+}
+
 /* ------------------------------------------------------------------------------------------ */
 /* Wave-coherent traversal (the device traverse_packet), one 64-lane wave emulated lane by lane */
 /* ------------------------------------------------------------------------------------------ */

@@ -273,7 +273,7 @@ def clear(b: DeviceArray, value=np.float32(0)):
         value = np.float32(value)
     if isinstance(value, int):
         value = np.int32(value)
-    pat = np.asarray(value).view(np.uint8).reshape(-1)
+    pat = np.frombuffer(np.ascontiguousarray(value).tobytes(), dtype=np.uint8)
     nb = b.nbytes
     if nb % pat.size:
         raise ValueError("buffer size is not a multiple of the fill pattern")

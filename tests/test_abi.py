@@ -1,0 +1,102 @@
+"""The C ABI boundary: librtx_hip.so loads (no GPU needed to load it) and exports every function
+that include/rendertoy_rt.h declares; the ctypes mirrors in rendertoy_amd/_native.py have the
+sizes and field offsets the C compiler gives the header's structs."""
+import ctypes
+import os
+import re
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "rendertoy_rt.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    names = re.findall(r"\b(rt_[a-z0-9_]+)\s*\(", text)
+    return sorted(set(names))
+
+
+def test_header_declares_the_entry_points():
+    names = declared_functions()
+    for n in ("rt_abi_version", "rt_last_error", "rt_scene_sizes_for", "rt_scene_build",
+              "rt_trace_closest", "rt_trace_any", "rt_trace_camera", "rt_generate_rays",
+              "rt_render", "rt_render_workspace_bytes"):
+        assert n in names
+
+
+def test_library_loads_and_exports_every_declared_symbol():
+    from rendertoy_amd import _native
+    lib = _native.load()
+    for name in declared_functions():
+        assert hasattr(lib, name), f"librtx_hip.so does not export {name}"
+    assert lib.rt_abi_version() == _native.RT_ABI_VERSION
+    # every declared function has a ctypes signature
+    sig = {n for n, _, _ in _native.SIGNATURES}
+    assert set(declared_functions()) <= sig
+
+
+def test_sizes_query_and_error_path_without_gpu():
+    from rendertoy_amd import _native
+    lib = _native.load()
+    s = _native.rt_scene_sizes()
+    assert lib.rt_scene_sizes_for(1000, ctypes.byref(s)) == 0
+    assert s.nodes_bytes == 64 * 999 and s.nodes4_bytes == 128 * 999
+    assert s.tris_bytes == 48 * 1000 and s.shade_bytes == 64 * 1000
+    assert s.workspace_bytes > 0
+    # too many triangles: RT_ECAPACITY and a message, no device work
+    assert lib.rt_scene_sizes_for(1 << 30, ctypes.byref(s)) == -3
+    assert b"too many" in lib.rt_last_error()
+    # NULL scene pointer is rejected before any launch
+    assert lib.rt_trace_closest(None, None, 1, None, None) == -1
+
+
+PROBE = r"""
+#include <stddef.h>
+#include <stdio.h>
+#include "rendertoy_rt.h"
+#define S(T) printf(#T " %zu\n", sizeof(T));
+#define O(T, f) printf(#T "." #f " %zu\n", offsetof(T, f));
+int main(void) {
+    S(rt_mesh_desc) O(rt_mesh_desc, vertices) O(rt_mesh_desc, indices) O(rt_mesh_desc, n_vertices)
+    O(rt_mesh_desc, n_tris) O(rt_mesh_desc, vertex_stride) O(rt_mesh_desc, pos_offset)
+    O(rt_mesh_desc, nrm_offset) O(rt_mesh_desc, uv_offset)
+    S(rt_scene) O(rt_scene, root) O(rt_scene, nodes) O(rt_scene, nodes4) O(rt_scene, tris)
+    O(rt_scene, shade) O(rt_scene, mesh_offsets) O(rt_scene, bounds)
+    S(rt_scene_sizes) O(rt_scene_sizes, nodes4_bytes) O(rt_scene_sizes, workspace_bytes)
+    S(rt_rays) O(rt_rays, tmin) O(rt_rays, stride) O(rt_rays, tmin_default) O(rt_rays, tmax_default)
+    S(rt_hits) O(rt_hits, mesh) O(rt_hits, stride)
+    S(rt_camera) O(rt_camera, up) O(rt_camera, forward)
+    S(rt_texture) O(rt_texture, width)
+    S(rt_render_params) O(rt_render_params, light) O(rt_render_params, ambient)
+    O(rt_render_params, shadow_tmin) O(rt_render_params, textures_dev)
+    O(rt_render_params, mesh_texture_dev) O(rt_render_params, n_textures)
+    S(rt_render_out) O(rt_render_out, stats)
+    return 0;
+}
+"""
+
+
+def test_ctypes_layouts_match_the_c_header(tmp_path):
+    from rendertoy_amd import _native as N
+    src = tmp_path / "probe.c"
+    src.write_text(PROBE)
+    exe = tmp_path / "probe"
+    r = subprocess.run(["gcc", "-std=c11", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout
+    c = {}
+    for line in out.splitlines():
+        k, v = line.split()
+        c[k] = int(v)
+    for name in ("rt_mesh_desc", "rt_scene", "rt_scene_sizes", "rt_rays", "rt_hits", "rt_camera",
+                 "rt_texture", "rt_render_params", "rt_render_out"):
+        cls = getattr(N, name)
+        assert ctypes.sizeof(cls) == c[name], name
+        for key, off in c.items():
+            if key.startswith(name + "."):
+                field = key.split(".", 1)[1]
+                assert getattr(cls, field).offset == off, key

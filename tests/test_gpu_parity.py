@@ -241,3 +241,13 @@ def test_render_c2_1080p_matches_twin_and_fp64(c2, twin):
     # primary IDs and t against FP64 on the full frame
     o, d = twin.twin_generate_rays(cam.as_array(), W, H)
     fp64_report("C2 primary 1920x1080", prim, t, tw.closest64(o, d))
+    # shaded RGB against the FP64 shading of the FP64 hits: within 1/255 off flagged pixels
+    from rendertoy_amd import scenes
+    pos, nrm, uv, mo = scenes.triangle_arrays(meshes)
+    rgb64, flags = twin.shade64(pos, nrm, uv, mo, [1, 1], o, d, LIGHT, textures=[tex],
+                                mesh_tex=[0, 0], scene=tw)
+    err = np.abs(img.cpu().numpy().reshape(-1, 3) - rgb64).max(1)
+    print(f"C2 render 1920x1080 vs FP64: rgb_err_max_unflagged={err[flags == 0].max():.3e} "
+          f"rgb_err_mean={err[flags == 0].mean():.3e} flagged_pixels={int((flags != 0).sum())} "
+          f"over_1/255_flagged={int(((err > 1 / 255) & (flags != 0)).sum())}")
+    assert (err[flags == 0] <= 1 / 255).all()

@@ -1,0 +1,170 @@
+"""Pins of the oracle's LBVH build (CPU only): Morton keys from a numpy restatement of the
+quantiser, the order from numpy's stable argsort, the tree from an independent top-down binary
+radix tree construction, node boxes as exact unions of padded leaf boxes, the 4-wide records from
+the binary ones, and leaves that partition the slots. Also: traversal of that tree (per-ray and the
+wave-coherent emulation) equals FP32 brute force."""
+import numpy as np
+import pytest
+
+import oracle
+from helpers import twin_from_meshes, random_rays
+from rendertoy_amd import scenes
+
+LEAF_MAX = 4
+EMPTY = 0x7FFFFFFF
+
+
+def leaf_decode(ref):
+    u = (~np.int64(ref)) & 0xFFFFFFFF
+    return int(u & 0x1FFFFFFF), int(u >> 29) + 1
+
+
+def keys_numpy(pos):
+    """Morton codes of the AABB centres quantised to 1024^3 over the centre bounds (float32)."""
+    v = pos.reshape(-1, 3, 3).astype(np.float32) + np.float32(0.0)
+    lo, hi = v.min(1), v.max(1)
+    c = ((lo + hi) * np.float32(0.5)).astype(np.float32)
+    clo, chi = c.min(0), c.max(0)
+    ext = (chi - clo).astype(np.float32)
+    sc = np.where(ext > 0, np.float32(1024.0) / np.where(ext > 0, ext, 1), 0).astype(np.float32)
+    q = ((c - clo).astype(np.float32) * sc).astype(np.float32)
+    q = np.clip(q, np.float32(0), np.float32(1023)).astype(np.uint32)
+    return oracle.morton_np(q[:, 0], q[:, 1], q[:, 2])
+
+
+def leaf_boxes_numpy(pos, order):
+    v = pos.reshape(-1, 3, 3).astype(np.float32)[order]
+    lo, hi = v.min(1), v.max(1)
+    pad = (np.maximum(np.abs(lo), np.abs(hi)) * np.float32(2.0 ** -14) + np.float32(2.0 ** -40)).astype(np.float32)
+    return (lo - pad).astype(np.float32), (hi + pad).astype(np.float32)
+
+
+def radix_splits(keys):
+    """Top-down binary radix tree over the extended keys (key, slot): {(lo, hi): split}."""
+    n = len(keys)
+    ext = [(int(k) << 32) | i for i, k in enumerate(keys)]
+    out = {}
+    todo = [(0, n - 1)]
+    while todo:
+        lo, hi = todo.pop()
+        if lo == hi:
+            continue
+        x = ext[lo] ^ ext[hi]
+        top = x.bit_length() - 1          # highest differing bit within the range
+        g = lo
+        while g + 1 <= hi and not ((ext[g + 1] ^ ext[lo]) >> top) & 1:
+            g += 1
+        out[(lo, hi)] = g
+        todo += [(lo, g), (g + 1, hi)]
+    return out
+
+
+@pytest.mark.parametrize("n_tris_f", [1, 2, 3, 5, 8, 13])
+def test_build_invariants_on_icospheres(n_tris_f):
+    d = scenes.icosphere(n_tris_f, 100 + n_tris_f)
+    tw = twin_from_meshes([{"pos": d["pos"], "idx": d["idx"], "nrm": d["nrm"], "uv": d["uv"]}])
+    check_build(tw)
+
+
+def test_build_invariants_with_duplicate_keys():
+    # many triangles with the same centre: equal Morton keys, order by primitive id
+    rng = np.random.default_rng(9)
+    base = rng.uniform(-1, 1, (40, 3, 3)).astype(np.float32)
+    pos = np.concatenate([base, base, base[:7]]).reshape(-1, 3)
+    idx = np.arange(len(pos), dtype=np.int32).reshape(-1, 3)
+    tw = twin_from_meshes([{"pos": pos, "idx": idx, "nrm": None, "uv": None}])
+    check_build(tw)
+    assert len(np.unique(tw.keys)) < tw.n
+
+
+def check_build(tw):
+    n = tw.n
+    pos = tw.pos
+    keys = keys_numpy(pos)
+    order = np.argsort(keys, kind="stable")
+    np.testing.assert_array_equal(tw.order, order)
+    np.testing.assert_array_equal(tw.keys, keys[order])
+    lb_lo, lb_hi = leaf_boxes_numpy(pos, order)
+    # triangles in slot order with the primitive id in v0.w
+    np.testing.assert_array_equal(tw.tris[:, 3].view(np.int32), order)
+    if n <= LEAF_MAX:
+        assert tw.root == ~np.int32(0 | ((n - 1) << 29))
+        return
+    assert tw.root == 0
+    splits = radix_splits(tw.keys)
+    refs = tw.nodes[:, 12:14].view(np.int32)
+    seen = np.zeros(n, np.int32)
+    stack = [(0, 0, n - 1)]
+    internal = 0
+    while stack:
+        node, lo, hi = stack.pop()
+        internal += 1
+        g = splits[(lo, hi)]
+        kids = [(lo, g), (g + 1, hi)]
+        rec = tw.nodes[node]
+        boxes = [(rec[[0, 2, 8]], rec[[1, 3, 9]]), (rec[[4, 6, 10]], rec[[5, 7, 11]])]
+        for (clo, chi), ref, (blo, bhi) in zip(kids, refs[node], boxes):
+            # child box = exact union of its slots' padded leaf boxes
+            np.testing.assert_array_equal(blo, lb_lo[clo:chi + 1].min(0))
+            np.testing.assert_array_equal(bhi, lb_hi[clo:chi + 1].max(0))
+            size = chi - clo + 1
+            if size <= LEAF_MAX:
+                assert ref < 0 and leaf_decode(ref) == (clo, size)
+                seen[clo:chi + 1] += 1
+            else:
+                assert ref >= 0 and ref in (clo, chi)  # Karras: a node id is an end of its range
+                stack.append((ref, clo, chi))
+        # the 4-wide record lists the grandchildren / leaf children in order
+        w = tw.nodes4[node]
+        ent = []
+        for ref, (blo, bhi) in zip(refs[node], boxes):
+            if ref >= 0:
+                c = tw.nodes[ref]
+                ent += [(c[12:13].view(np.int32)[0], c[[0, 2, 8]], c[[1, 3, 9]]),
+                        (c[13:14].view(np.int32)[0], c[[4, 6, 10]], c[[5, 7, 11]])]
+            else:
+                ent.append((ref, blo, bhi))
+        wrefs = w[24:28].view(np.int32)
+        for k in range(4):
+            if k < len(ent):
+                assert wrefs[k] == ent[k][0]
+                np.testing.assert_array_equal(w[[k, 8 + k, 16 + k]], ent[k][1])
+                np.testing.assert_array_equal(w[[4 + k, 12 + k, 20 + k]], ent[k][2])
+            else:
+                assert wrefs[k] == EMPTY
+    assert (seen == 1).all(), "leaves must partition the slots"
+    # scene bounds = union of all leaf boxes
+    np.testing.assert_array_equal(tw.bounds[[0, 1, 2]], lb_lo.min(0))
+    np.testing.assert_array_equal(tw.bounds[[4, 5, 6]], lb_hi.max(0))
+
+
+def test_traversals_equal_fp32_brute_force():
+    a = scenes.icosphere(12, 21)
+    b = scenes.icosphere(6, 22)
+    b["pos"] = (b["pos"] * 0.4 + 0.25).astype(np.float32)
+    tw = twin_from_meshes([a, b])
+    o, d = random_rays(3000, 23)
+    per_ray = tw.trace_closest(o, d)
+    bt, bp = tw.brute_closest32(o, d)
+    np.testing.assert_array_equal(per_ray["prim"], bp)
+    hit = bp >= 0
+    # brute force reports the FP32 t, traversal the FP64-refined t of the same triangle
+    np.testing.assert_allclose(per_ray["t"][hit], bt[hit], rtol=2e-5)
+    occ = tw.trace_any(o, d)
+    np.testing.assert_array_equal(occ, (bp >= 0).astype(np.uint8))
+
+
+def test_wave_emulation_equals_per_ray_on_camera_tiles():
+    from helpers import camera
+    d = scenes.icosphere(24, 31)
+    g = scenes.ground_plane()
+    tw = twin_from_meshes([d, g])
+    W, H = 96, 72
+    cam = camera((0.2, 0.4, 2.2), (0, 0, 0), (0, 1, 0), 0.9, W / H)
+    packet = tw.trace_camera(cam.as_array(), W, H)
+    o, dd = oracle.twin_generate_rays(cam.as_array(), W, H)
+    per_ray = tw.trace_closest(o, dd)
+    np.testing.assert_array_equal(packet["prim"], per_ray["prim"])
+    np.testing.assert_array_equal(packet["t"].view(np.uint32), per_ray["t"].view(np.uint32))
+    bt, bp = tw.brute_closest32(o, dd)
+    np.testing.assert_array_equal(per_ray["prim"], bp)
