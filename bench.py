@@ -44,8 +44,9 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-roofline", action="store_true")
     ap.add_argument("--copy-calib-mib", type=int, default=0,
-                    help="first run 3 device-to-device copies of this many MiB (a known-traffic "
-                         "kernel that scales the FETCH_SIZE/WRITE_SIZE counters of a profile)")
+                    help="first run 3 elementwise int32 adds over this many MiB (a kernel of "
+                         "known read/write traffic that scales the FETCH_SIZE/WRITE_SIZE "
+                         "counters of a profile)")
     return ap.parse_args()
 
 
@@ -67,11 +68,12 @@ def main():
     W, H = args.width, args.height
     n_rays = W * H
     if args.copy_calib_mib:
+        # one elementwise kernel reading and writing exactly `nb` bytes each (int32 add)
         nb = args.copy_calib_mib << 20
-        src = torch.ones(nb, dtype=torch.uint8, device="cuda")
+        src = torch.ones(nb // 4, dtype=torch.int32, device="cuda")
         dst = torch.empty_like(src)
         for _ in range(3):
-            dst.copy_(src)
+            torch.add(src, 1, out=dst)
         torch.cuda.synchronize()
         del src, dst
     t0 = time.time()
