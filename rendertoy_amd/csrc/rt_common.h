@@ -103,6 +103,43 @@ RT_DEV void refine_hit(const float4* __restrict__ tris, const Ray& r, Hit& h) {
     h.v = (float)(vn / det);
 }
 
+// Tests the triangles of leaf `ref` in slot order against r, updating h (closest hit; ties to the
+// smaller primitive id). Every record of the leaf is fetched before the first test (slots past the
+// leaf's count re-read its first record), so a leaf costs one memory round trip instead of one per
+// triangle. `active` = this lane takes part; `skip_slot` = a slot never reported (a shadow ray's
+// own triangle). Returns whether h changed.
+RT_DEV bool test_leaf(const float4* __restrict__ tris, int32_t ref, const Ray& r, bool active,
+                      int32_t skip_slot, Hit& h) {
+    const uint32_t first = leaf_first(ref), count = leaf_count(ref);
+    float4 a[kLeafMax], b[kLeafMax], c[kLeafMax];
+#pragma unroll
+    for (int k = 0; k < kLeafMax; ++k) {
+        const float4* tr = tris + (size_t)(first + ((uint32_t)k < count ? (uint32_t)k : 0u)) * 3;
+        a[k] = tr[0];
+        b[k] = tr[1];
+        c[k] = tr[2];
+    }
+    bool changed = false;
+#pragma unroll
+    for (int k = 0; k < kLeafMax; ++k) {
+        const int32_t slot = (int32_t)(first + (uint32_t)k);
+        float t, u, v;
+        if ((uint32_t)k < count && active && slot != skip_slot &&
+            intersect_tri(a[k], b[k], c[k], r, t, u, v) && t > r.tmin) {
+            const int32_t prim = __float_as_int(a[k].w);
+            if (t < h.t || (t == h.t && prim < h.prim)) {
+                h.t = t;
+                h.u = u;
+                h.v = v;
+                h.prim = prim;
+                h.slot = slot;
+                changed = true;
+            }
+        }
+    }
+    return changed;
+}
+
 // Closest-hit (ANY=false) or any-hit (ANY=true) traversal of the two-child node layout, nearer
 // child first. `skip_slot` is never tested (the shadow ray's own triangle). On return h.prim < 0
 // means no hit in (tmin, tmax).
@@ -156,25 +193,7 @@ RT_DEV void traverse(const float4* __restrict__ nodes, const float4* __restrict_
                 continue;
             }
         } else {
-            const uint32_t first = leaf_first(cur), count = leaf_count(cur);
-            for (uint32_t k = 0; k < count; ++k) {
-                const int32_t slot = (int32_t)(first + k);
-                if (slot == skip_slot) continue;
-                const float4* tr = tris + (size_t)slot * 3;
-                const float4 ta = tr[0], tb = tr[1], tc = tr[2];
-                float t, u, v;
-                if (intersect_tri(ta, tb, tc, r, t, u, v) && t > r.tmin) {
-                    const int32_t prim = __float_as_int(ta.w);
-                    if (t < h.t || (t == h.t && prim < h.prim)) {
-                        h.t = t;
-                        h.u = u;
-                        h.v = v;
-                        h.prim = prim;
-                        h.slot = slot;
-                        if (ANY) return;
-                    }
-                }
-            }
+            if (test_leaf(tris, cur, r, true, skip_slot, h) && ANY) return;
         }
         if (sp == 0) break;
         cur = stack[--sp];
@@ -235,7 +254,7 @@ RT_DEV void traverse_packet(const float4* __restrict__ nodes4, const float4* __r
                 const float z0 = fmaf(clz[k], idz, -ozz), z1 = fmaf(chz[k], idz, -ozz);
                 const float tn = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), r.tmin));
                 const float tf = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fmaxf(z0, z1)) * kTfarScale;
-                const bool hk = mine && ref[k] != RT_EMPTY_ROOT && tn <= fminf(tf, h.t);
+                const bool hk = mine & (ref[k] != RT_EMPTY_ROOT) & (tn <= fminf(tf, h.t));  // no short circuit: branch free
                 c[k].mask = __ballot(hk);
                 c[k].key = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(hk ? tn : INFINITY), rep));
                 c[k].k = k;
@@ -279,26 +298,7 @@ RT_DEV void traverse_packet(const float4* __restrict__ nodes4, const float4* __r
                 continue;
             }
         } else {
-            const uint32_t first = leaf_first(cur), count = leaf_count(cur);
-#pragma unroll
-            for (uint32_t k = 0; k < (uint32_t)kLeafMax; ++k) {
-                if (k >= count) break;
-                const int32_t slot = (int32_t)(first + k);
-                const float4* tr = tris + (size_t)slot * 3;
-                const float4 ta = tr[0], tb = tr[1], tc = tr[2];
-                const bool mine = ((m >> lane) & 1u) && slot != skip_slot;
-                float t, u, v;
-                if (mine && intersect_tri(ta, tb, tc, r, t, u, v) && t > r.tmin) {
-                    const int32_t prim = __float_as_int(ta.w);
-                    if (t < h.t || (t == h.t && prim < h.prim)) {
-                        h.t = t;
-                        h.u = u;
-                        h.v = v;
-                        h.prim = prim;
-                        h.slot = slot;
-                    }
-                }
-            }
+            test_leaf(tris, cur, r, ((m >> lane) & 1u) != 0, skip_slot, h);
             if (ANY) {
                 alive &= ~__ballot(h.prim >= 0);
                 if (alive == 0) return;
@@ -316,6 +316,84 @@ RT_DEV void traverse_packet(const float4* __restrict__ nodes4, const float4* __r
             if (ANY) m &= alive;
             if (m != 0) break;
         }
+    }
+}
+
+// Per-lane traversal of the 4-wide nodes: every lane walks its own ray (no lane waits on another's
+// subtrees, so the longest lane bounds the wave), nearer children first by the lane's own entry
+// distances, pending children on a per-lane stack in LDS. `stk` points at this thread's first
+// entry; entry k lives at stk[k * stride] (stride = threads per block, so the lanes of a wave touch
+// consecutive words at equal depth). Returns false if the ray needed more than kLaneStack pending
+// entries (the bound of the tree is 93): the hit is then invalid and the caller re-traces the ray
+// with the binary traversal, whose 64-entry stack always suffices.
+constexpr int kLaneStack = 32;
+
+template <bool ANY>
+RT_DEV bool traverse_lane4(const float4* __restrict__ nodes4, const float4* __restrict__ tris,
+                           int32_t root, const Ray& r, int32_t skip_slot, Hit& h, int32_t* stk,
+                           int stride) {
+    h.t = r.tmax;
+    h.u = 0.0f;
+    h.v = 0.0f;
+    h.prim = -1;
+    h.slot = -1;
+    if (root == RT_EMPTY_ROOT) return true;
+    const float idx = safe_inv(r.dx), idy = safe_inv(r.dy), idz = safe_inv(r.dz);
+    const float oxx = r.ox * idx, oyy = r.oy * idy, ozz = r.oz * idz;
+    int sp = 0;
+    int32_t cur = root;
+    for (;;) {
+        if (cur >= 0) {
+            const float4* nd = nodes4 + (size_t)cur * 8;
+            const float4 lx = nd[0], hx = nd[1], ly = nd[2], hy = nd[3], lz = nd[4], hz = nd[5];
+            const float4 rf = nd[6];
+            const float clx[4] = {lx.x, lx.y, lx.z, lx.w}, chx[4] = {hx.x, hx.y, hx.z, hx.w};
+            const float cly[4] = {ly.x, ly.y, ly.z, ly.w}, chy[4] = {hy.x, hy.y, hy.z, hy.w};
+            const float clz[4] = {lz.x, lz.y, lz.z, lz.w}, chz[4] = {hz.x, hz.y, hz.z, hz.w};
+            const int32_t ref[4] = {__float_as_int(rf.x), __float_as_int(rf.y), __float_as_int(rf.z),
+                                    __float_as_int(rf.w)};
+            float key[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const float x0 = fmaf(clx[k], idx, -oxx), x1 = fmaf(chx[k], idx, -oxx);
+                const float y0 = fmaf(cly[k], idy, -oyy), y1 = fmaf(chy[k], idy, -oyy);
+                const float z0 = fmaf(clz[k], idz, -ozz), z1 = fmaf(chz[k], idz, -ozz);
+                const float tn = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), r.tmin));
+                const float tf = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fmaxf(z0, z1)) * kTfarScale;
+                const bool hk = (ref[k] != RT_EMPTY_ROOT) & (tn <= fminf(tf, h.t));
+                key[k] = hk ? tn : INFINITY;
+            }
+            // (key, k) order by a 4-element sorting network over (key, ref) pairs
+            float k0 = key[0], k1 = key[1], k2 = key[2], k3 = key[3];
+            int32_t r0 = ref[0], r1 = ref[1], r2 = ref[2], r3 = ref[3];
+            int i0 = 0, i1 = 1, i2 = 2, i3 = 3;
+#define RT_LSWAP(ka, ra, ia, kb, rb, ib)                                     \
+    if (kb < ka || (kb == ka && ib < ia)) {                                  \
+        const float tk = ka; ka = kb; kb = tk;                               \
+        const int32_t tr = ra; ra = rb; rb = tr;                             \
+        const int ti = ia; ia = ib; ib = ti;                                 \
+    }
+            RT_LSWAP(k0, r0, i0, k1, r1, i1)
+            RT_LSWAP(k2, r2, i2, k3, r3, i3)
+            RT_LSWAP(k0, r0, i0, k2, r2, i2)
+            RT_LSWAP(k1, r1, i1, k3, r3, i3)
+            RT_LSWAP(k1, r1, i1, k2, r2, i2)
+#undef RT_LSWAP
+            const int nhit = (k0 < INFINITY) + (k1 < INFINITY) + (k2 < INFINITY) + (k3 < INFINITY);
+            if (nhit > 0) {
+                if (sp + nhit - 1 > kLaneStack) return false;
+                // push the farther hit children (farthest first), continue with the nearest
+                if (nhit >= 4) stk[(sp++) * stride] = r3;
+                if (nhit >= 3) stk[(sp++) * stride] = r2;
+                if (nhit >= 2) stk[(sp++) * stride] = r1;
+                cur = r0;
+                continue;
+            }
+        } else {
+            if (test_leaf(tris, cur, r, true, skip_slot, h) && ANY) return true;
+        }
+        if (sp == 0) return true;
+        cur = stk[(--sp) * stride];
     }
 }
 

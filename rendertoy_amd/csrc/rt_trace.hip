@@ -12,6 +12,7 @@
 //   ray toward the light per lit hit (wave-coherent any-hit over the tile's hit points, skipping
 //   each hit's own triangle), RGBA8 texture fetch, RGB float32 store.
 #include <algorithm>
+#include <cstdlib>
 
 #include "rt_common.h"
 
@@ -118,9 +119,50 @@ struct TilePixel {
     bool valid;
 };
 
-RT_DEV TilePixel tile_pixel(uint32_t W, uint32_t H, uint32_t tiles_x) {
+// Block -> tile mapping. Workgroups are handed to the 8 XCDs round robin (block b runs on XCD
+// b % 8) and each XCD has its own L2, so consecutive row-major tiles would spread every screen
+// region over all 8 L2s. Instead the image is cut into super-tiles of S x S blocks, super-tile s
+// belongs to XCD s % 8, and XCD x walks its super-tiles in order: the rays one L2 serves come from
+// a few compact screen regions and share their upper tree and surface nodes. S = 0 keeps the
+// row-major order.
+struct TileMap {
+    uint32_t tiles_x, tiles_y;  // 16x16 blocks across / down
+    uint32_t S, super_x, n_super;
+};
+
+TileMap make_tile_map(uint32_t W, uint32_t H, uint32_t S) {
+    TileMap m;
+    m.tiles_x = (W + kTile - 1) / kTile;
+    m.tiles_y = (H + kTile - 1) / kTile;
+    m.S = S;
+    m.super_x = S ? (m.tiles_x + S - 1) / S : 0;
+    m.n_super = S ? m.super_x * ((m.tiles_y + S - 1) / S) : 0;
+    return m;
+}
+
+uint32_t tile_map_blocks(const TileMap& m) {
+    if (!m.S) return m.tiles_x * m.tiles_y;
+    return 8u * m.S * m.S * ((m.n_super + 7u) / 8u);
+}
+
+// Tile (tx, ty) of this block; false if the block maps outside the image (it then does nothing).
+RT_DEV bool block_tile(const TileMap& m, uint32_t& tx, uint32_t& ty) {
+    const uint32_t b = blockIdx.x;
+    if (!m.S) {
+        tx = b % m.tiles_x;
+        ty = b / m.tiles_x;
+        return true;
+    }
+    const uint32_t x = b & 7u, j = b >> 3, SS = m.S * m.S;
+    const uint32_t s = (j / SS) * 8u + x, w = j % SS;
+    if (s >= m.n_super) return false;
+    tx = (s % m.super_x) * m.S + w % m.S;
+    ty = (s / m.super_x) * m.S + w / m.S;
+    return tx < m.tiles_x && ty < m.tiles_y;
+}
+
+RT_DEV TilePixel tile_pixel(uint32_t W, uint32_t H, uint32_t tx, uint32_t ty) {
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const uint32_t tx = blockIdx.x % tiles_x, ty = blockIdx.x / tiles_x;
     TilePixel p;
     p.col = tx * kTile + (wave & 1) * 8 + (lane & 7);
     p.row = ty * kTile + (wave >> 1) * 8 + (lane >> 3);
@@ -140,13 +182,43 @@ RT_DEV Ray camera_ray(const rt_camera& cam, uint32_t W, uint32_t H, const TilePi
     return r;
 }
 
+// Traversal of a block's 256 rays, one per lane. LANE: per-lane 4-wide traversal with the
+// per-thread stacks in `stk` (kLaneStack x 256 words); a ray that overflows its stack is traced
+// again by its wave with the packet traversal (128-entry stack, enough for any tree), which finds
+// the same closest hit. Otherwise the packet traversal throughout. Called by all lanes.
+template <bool ANY, bool LANE>
+RT_DEV void trace_block(const SceneArgs& sc, const Ray& r, bool want, int32_t skip_slot, Hit& h,
+                        int32_t* stk) {
+    if (!LANE) {
+        traverse_packet<ANY>(sc.nodes4, sc.tris, sc.root, r, want, skip_slot, h);
+        return;
+    }
+    bool ok = true;
+    h.t = r.tmax;
+    h.u = 0.0f;
+    h.v = 0.0f;
+    h.prim = -1;
+    h.slot = -1;
+    if (want) ok = traverse_lane4<ANY>(sc.nodes4, sc.tris, sc.root, r, skip_slot, h, stk + threadIdx.x, 256);
+    const bool redo = !ok;
+    if (__ballot(redo)) {
+        Hit h2;
+        traverse_packet<ANY>(sc.nodes4, sc.tris, sc.root, r, redo, skip_slot, h2);
+        if (redo) h = h2;
+    }
+}
+
 // Camera closest hit: ray generation fused into the traversal, one hit record per pixel.
+template <bool LANE>
 __global__ void __launch_bounds__(256) k_trace_camera(SceneArgs sc, rt_camera cam, uint32_t W,
-                                                       uint32_t H, uint32_t tiles_x, rt_hits hits) {
-    const TilePixel p = tile_pixel(W, H, tiles_x);
+                                                       uint32_t H, TileMap tm, rt_hits hits) {
+    __shared__ int32_t stk[LANE ? kLaneStack * 256 : 1];
+    uint32_t tx, ty;
+    if (!block_tile(tm, tx, ty)) return;
+    const TilePixel p = tile_pixel(W, H, tx, ty);
     const Ray r = camera_ray(cam, W, H, p);
     Hit h;
-    traverse_packet<false>(sc.nodes4, sc.tris, sc.root, r, p.valid, -1, h);
+    trace_block<false, LANE>(sc, r, p.valid, -1, h, stk);
     if (h.prim >= 0) refine_hit(sc.tris, r, h);
     if (p.valid) store_hit(sc, hits, (uint64_t)p.row * W + p.col, h);
 }
@@ -195,17 +267,22 @@ RT_DEV void tex_sample(const rt_texture& tx, float u, float v, bool bilinear, fl
     }
 }
 
-// Fused render of one frame (see the file comment).
+// Fused render of one frame (see the file comment). PLANE / SLANE: per-lane traversal for the
+// primary / shadow rays (else the packet traversal).
+template <bool PLANE, bool SLANE>
 __global__ void __launch_bounds__(256) k_render(SceneArgs sc, rt_camera cam, rt_render_params p,
-                                                 uint32_t tiles_x, float* __restrict__ rgb,
+                                                 TileMap tm, float* __restrict__ rgb,
                                                  float* __restrict__ out_t,
                                                  int32_t* __restrict__ out_prim,
                                                  uint32_t* __restrict__ stats) {
     const uint32_t W = p.width, H = p.height;
-    const TilePixel px = tile_pixel(W, H, tiles_x);
+    __shared__ int32_t stk[(PLANE || SLANE) ? kLaneStack * 256 : 1];
+    uint32_t tx, ty;
+    if (!block_tile(tm, tx, ty)) return;
+    const TilePixel px = tile_pixel(W, H, tx, ty);
     const Ray r = camera_ray(cam, W, H, px);
     Hit h;
-    traverse_packet<false>(sc.nodes4, sc.tris, sc.root, r, px.valid, -1, h);
+    trace_block<false, PLANE>(sc, r, px.valid, -1, h, stk);
     const bool hit = px.valid && h.prim >= 0;
     if (hit) refine_hit(sc.tris, r, h);
 
@@ -239,7 +316,7 @@ __global__ void __launch_bounds__(256) k_render(SceneArgs sc, rt_camera cam, rt_
     sr.tmin = p.shadow_tmin;
     sr.tmax = INFINITY;
     Hit sh;
-    traverse_packet<true>(sc.nodes4, sc.tris, sc.root, sr, want_shadow, hit ? h.slot : -1, sh);
+    trace_block<true, SLANE>(sc, sr, want_shadow, hit ? h.slot : -1, sh, stk);
     const bool occluded = want_shadow && sh.prim >= 0;
 
     float alb[3] = {1.0f, 1.0f, 1.0f};
@@ -308,6 +385,30 @@ int check_rays(const rt_rays* r, const char* what) {
     return RT_OK;
 }
 
+bool lane_traversal() {
+    static const bool v = [] {
+        const char* e = getenv("RT_TRAVERSAL");
+        return e && std::string(e) == "lane";
+    }();
+    return v;
+}
+
+int render_mode() {
+    static const int v = [] {
+        const char* e = getenv("RT_RENDER_MODE");
+        return e ? atoi(e) : 1;
+    }();
+    return v;
+}
+
+uint32_t super_tile() {
+    static const uint32_t v = [] {
+        const char* e = getenv("RT_SUPER");
+        return e ? (uint32_t)atoi(e) : 0u;
+    }();
+    return v;
+}
+
 inline uint32_t blocks_for(uint64_t n, uint32_t per) { return (uint32_t)((n + per - 1) / per); }
 
 }  // namespace
@@ -361,9 +462,13 @@ extern "C" int rt_trace_camera(const rt_scene* scene, const rt_camera* cam, uint
     const uint64_t n = (uint64_t)W * H;
     if (n == 0) return RT_OK;
     if (n > 0x7FFFFFFFull) return set_error(RT_ECAPACITY, "rt_trace_camera: image too large");
-    const uint32_t tiles_x = (W + kTile - 1) / kTile, tiles_y = (H + kTile - 1) / kTile;
-    hipLaunchKernelGGL(k_trace_camera, dim3(tiles_x * tiles_y), dim3(256), 0, (hipStream_t)stream,
-                       scene_args(scene), *cam, W, H, tiles_x, *hits);
+    const TileMap tm = make_tile_map(W, H, super_tile());
+    if (lane_traversal())
+        hipLaunchKernelGGL(k_trace_camera<true>, dim3(tile_map_blocks(tm)), dim3(256), 0,
+                           (hipStream_t)stream, scene_args(scene), *cam, W, H, tm, *hits);
+    else
+        hipLaunchKernelGGL(k_trace_camera<false>, dim3(tile_map_blocks(tm)), dim3(256), 0,
+                           (hipStream_t)stream, scene_args(scene), *cam, W, H, tm, *hits);
     return check_launch("rt_trace_camera");
 }
 
@@ -387,8 +492,11 @@ extern "C" int rt_render(const rt_scene* scene, const rt_camera* cam, const rt_r
         return set_error(RT_EINVAL, "rt_render: n_textures > 0 but textures_dev is NULL");
     hipStream_t s = (hipStream_t)stream;
     if (out->stats) hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, out->stats, 4u);
-    const uint32_t tiles_x = (p->width + kTile - 1) / kTile, tiles_y = (p->height + kTile - 1) / kTile;
-    hipLaunchKernelGGL(k_render, dim3(tiles_x * tiles_y), dim3(256), 0, s, scene_args(scene), *cam, *p,
-                       tiles_x, out->rgb, out->t, out->prim, out->stats);
+    const TileMap tm = make_tile_map(p->width, p->height, super_tile());
+    const int mode = render_mode();
+    auto kern = mode == 3 ? k_render<true, true> : mode == 2 ? k_render<true, false>
+              : mode == 1 ? k_render<false, true> : k_render<false, false>;
+    hipLaunchKernelGGL(kern, dim3(tile_map_blocks(tm)), dim3(256), 0, s, scene_args(scene), *cam, *p,
+                       tm, out->rgb, out->t, out->prim, out->stats);
     return check_launch("rt_render");
 }
