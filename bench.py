@@ -11,11 +11,14 @@ value = rays / step time in Mrays/s. Also reported: LBVH build time, and the ful
 (primary + shadow ray + Lambert + bilinear RGBA8 texture) latency p50/p99 over the same count of
 frames.
 
-roofline: algorithmic bytes per launch = 128 B per 4-wide node record + 48 B per triangle record
-fetched per wave (the kernel's wave-coherent traversal fetches each record once per 64-ray wave), 48 B per
-hit for the FP64 refinement re-read, 16 B of hit output per ray; the fetch counts come from the CPU
-restatement's lane-exact emulation of the kernel on the identical BVH (oracle/rt_twin.c), divided
-by the kernel's mean duration from HIP events on the launch stream.
+roofline: algorithmic bytes per launch = per ray 16 B of hit output + 128 B per 4-wide node
+record + 48 B per triangle record its ray tests, + 48 B per hit (the winner's record re-read for
+the FP64 refinement); the per-ray visit counts come from the CPU restatement's lane-exact
+emulation of the kernel's traversal on the identical BVH (oracle/rt_twin.c), divided by the
+kernel's mean duration from HIP events on the launch stream. The packet traversal fetches a record
+once per 64-ray wave, so the bytes it actually fetches are also reported (fetched_bytes), and
+`traffic` is the HBM traffic per launch measured by rocprofv3 (FETCH_SIZE/WRITE_SIZE, scaled by
+the same counters on a copy kernel of known size), read from profiles/k_trace_camera_c2.txt.
 cpu_baseline: the same closest-hit traversal in the CPU restatement (C + OpenMP, all host
 threads) over full frames for about `--cpu-seconds`.
 """
@@ -48,6 +51,28 @@ def parse():
                          "known read/write traffic that scales the FETCH_SIZE/WRITE_SIZE "
                          "counters of a profile)")
     return ap.parse_args()
+
+
+PROFILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "k_trace_camera_c2.txt")
+
+
+def measured_traffic(workload):
+    """HBM bytes per launch of k_trace_camera from the committed rocprofv3 counter summary, if it
+    was taken on this workload; None otherwise."""
+    if not os.path.exists(PROFILE):
+        return None
+    vals = {}
+    with open(PROFILE) as f:
+        for line in f:
+            if ":" in line and not line.startswith("#"):
+                k, v = line.split(":", 1)
+                vals[k.strip()] = v.strip()
+    try:
+        if vals.get("workload") != workload:
+            return None
+        return int(float(vals["hbm_bytes_per_launch"]))
+    except (KeyError, ValueError):
+        return None
 
 
 def percentile(a, q):
@@ -180,21 +205,24 @@ def main():
         pos, nrm, uv, mo = scenes.triangle_arrays(meshes)
         tw = oracle.TwinScene(pos, nrm, uv, mo, [1, 1], [1, 1])
         if not args.no_roofline:
-            # records fetched per wave by the device kernel, from its lane-exact emulation
             wv = tw.trace_camera(cam.as_array(), W, H)
-            nodes_f = int(wv["wave_nodes"].sum())
-            tris_f = int(wv["wave_tris"].sum())
             twin_hits = int((wv["prim"] >= 0).sum())
-            algo_bytes = 128 * nodes_f + 48 * tris_f + 48 * twin_hits + 16 * n_rays
+            lane_nodes = int(wv["lane_nodes"].sum())
+            lane_tris = int(wv["lane_tris"].sum())
+            algo_bytes = 16 * n_rays + 128 * lane_nodes + 48 * lane_tris + 48 * twin_hits
+            fetched = 16 * n_rays + 128 * int(wv["wave_nodes"].sum()) + 48 * int(wv["wave_tris"].sum()) \
+                + 48 * twin_hits
             achieved = algo_bytes / (ms_per_step * 1e-3) / 1e9
             result["roofline"] = {
                 "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                "frac": round(achieved / HBM_PEAK_GBS, 5),
+                "traffic": measured_traffic(result["config"]["workload"]),
                 "algorithmic_bytes_per_launch": int(algo_bytes),
-                "bytes_model": "128 B per 4-wide node record + 48 B per triangle record fetched "
-                               "per wave, 48 B per hit (FP64 refinement re-read), 16 B hit "
-                               "output per ray",
-                "wave_node_fetches": nodes_f, "wave_triangle_fetches": tris_f,
+                "bytes_model": "per ray: 16 B hit + 128 B per 4-wide node record + 48 B per "
+                               "triangle record its ray tests; + 48 B per hit (FP64 refinement)",
+                "mean_node_records_per_ray": round(lane_nodes / n_rays, 4),
+                "mean_triangles_per_ray": round(lane_tris / n_rays, 4),
+                "fetched_bytes_per_launch": int(fetched),
                 "kernel": "k_trace_camera",
             }
             result["parity_primary_ids_equal_twin"] = bool(np.array_equal(prim, wv["prim"]))

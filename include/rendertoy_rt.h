@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define RT_ABI_VERSION 2
+#define RT_ABI_VERSION 3
 
 enum {
     RT_OK = 0,
@@ -144,9 +144,30 @@ typedef struct rt_render_out {
     uint32_t* stats;
 } rt_render_out;
 
+/* Traversal algorithms of the camera and render kernels. Both return the same hits (the closest
+ * hit does not depend on the visiting order); they differ in speed only. */
+enum {
+    RT_TRAVERSAL_PACKET = 0, /* a wave's 64 rays walk one node sequence (scalar node fetches) */
+    RT_TRAVERSAL_LANE = 1    /* each lane walks its own ray (per-lane stack in LDS) */
+};
+#define RT_LANE_STACK_MAX 32
+
+/* Launch choices of rt_trace_camera / rt_render (process-wide, read at every launch; no
+ * counterpart in the reference, whose kernels take a fixed 32-thread group, _core.py:286). */
+typedef struct rt_tuning {
+    int32_t primary;          /* RT_TRAVERSAL_* for camera rays (default PACKET) */
+    int32_t shadow;           /* RT_TRAVERSAL_* for rt_render's shadow rays (default LANE) */
+    uint32_t super_tile;      /* blocks are mapped to XCDs in super-tiles of this many 16x16-pixel
+                                 blocks per side; 0 = row-major block order (default) */
+    uint32_t lane_stack;      /* per-lane stack entries, 1..RT_LANE_STACK_MAX (default 32); a ray
+                                 needing more is traced again by its wave with the packet traversal */
+} rt_tuning;
+
 /* ---- library ---- */
 int rt_abi_version(void);
 const char* rt_last_error(void);
+int rt_get_tuning(rt_tuning* out);
+int rt_set_tuning(const rt_tuning* tuning);   /* RT_EINVAL if a field is out of range */
 
 /* ---- scene build: rendering/_raycaster.py:26-33 (Raycaster.__init__ / _build_ads) ---- */
 
@@ -183,9 +204,10 @@ int rt_generate_rays(const rt_camera* cam, uint32_t width, uint32_t height,
 int rt_trace_camera(const rt_scene* scene, const rt_camera* cam, uint32_t width, uint32_t height,
                     const rt_hits* hits, void* stream);
 
-/* Full render: fused ray generation + closest-hit trace, then the fused Lambert + shadow ray +
- * texture shading kernel over the compacted list of hit pixels. `workspace_dev` needs
- * rt_render_workspace_bytes(width*height) bytes. */
+/* Full render in one kernel: per 8x8 pixel tile, ray generation, closest-hit trace, FP64 hit
+ * refinement, Lambert with interpolated normals, one shadow ray toward the light per lit hit,
+ * RGBA8 texture fetch, RGB store. `workspace_dev` needs rt_render_workspace_bytes(width*height)
+ * bytes (currently 0: NULL is accepted). */
 uint64_t rt_render_workspace_bytes(uint32_t n_pixels);
 int rt_render(const rt_scene* scene, const rt_camera* cam, const rt_render_params* params,
               const rt_render_out* out, void* workspace_dev, uint64_t workspace_bytes,

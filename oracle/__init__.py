@@ -52,7 +52,7 @@ def lib():
         L.tw_brute_closest32.restype = None
         L.tw_brute_closest32.argtypes = [vp, i64, i64] + [vp] * 6 + [f32, f32, vp, vp]
         L.tw_trace_camera.restype = None
-        L.tw_trace_camera.argtypes = [vp, vp, i32, vp, ctypes.c_uint32, ctypes.c_uint32] + [vp] * 8
+        L.tw_trace_camera.argtypes = [vp, vp, i32, vp, ctypes.c_uint32, ctypes.c_uint32] + [vp] * 9
         L.tw_generate_rays.restype = None
         L.tw_generate_rays.argtypes = [vp, ctypes.c_uint32, ctypes.c_uint32] + [vp] * 6
         L.tw_render.restype = None
@@ -202,7 +202,8 @@ class TwinScene:
     def trace_camera(self, cam12, W, H):
         """Camera closest hit with the device's wave-coherent traversal over 8x8 tiles: dict of
         per-pixel t, u, v, prim, slot plus per-tile wave_nodes / wave_leaves / wave_tris (4-wide
-        node records, leaves and triangle records fetched by the wave)."""
+        node records, leaves and triangle records fetched by the wave) and per-pixel lane_nodes /
+        lane_tris (node records and triangles that pixel's own ray tested)."""
         n = W * H
         out = {"t": np.empty(n, np.float32), "u": np.empty(n, np.float32),
                "v": np.empty(n, np.float32), "prim": np.empty(n, np.int32),
@@ -211,11 +212,14 @@ class TwinScene:
         out["wave_nodes"] = np.empty(ntiles, np.int64)
         out["wave_leaves"] = np.empty(ntiles, np.int64)
         out["wave_tris"] = np.empty(ntiles, np.int64)
+        lane = np.empty((n, 2), np.int32)
         lib().tw_trace_camera(_p(self.nodes4_full), _p(self.tris if self.n else np.zeros((1, 12), np.float32)),
                               self.root, _p(np.ascontiguousarray(cam12, np.float32)), W, H,
                               _p(out["t"]), _p(out["u"]), _p(out["v"]), _p(out["prim"]),
                               _p(out["slot"]), _p(out["wave_nodes"]), _p(out["wave_leaves"]),
-                              _p(out["wave_tris"]))
+                              _p(out["wave_tris"]), _p(lane))
+        out["lane_nodes"] = lane[:, 0]
+        out["lane_tris"] = lane[:, 1]
         return out
 
     def trace_any(self, o, d, tmin=0.0, tmax=np.inf):

@@ -707,11 +707,13 @@ static inline void tile_lane(int64_t tile, uint32_t W, int64_t* col, int64_t* ro
 
 /* Camera closest hit with the device's wave-coherent traversal (rt_trace_camera). Per-pixel
  * outputs (may be NULL); wave_nodes / wave_leaves / wave_tris (may be NULL): 4-wide node records,
- * leaves and triangle records fetched per 8x8 tile (the device kernel's traffic and step count). */
+ * leaves and triangle records fetched per 8x8 tile (the device kernel's traffic and step count);
+ * lane_counts (may be NULL): per pixel, the node records and triangles its own ray tested (the
+ * nodes and leaves where its lane was in the wave's mask). */
 TW_API void tw_trace_camera(const float* nodes4, const float* tris, int32_t root, const float* cam,
                             uint32_t W, uint32_t H, float* out_t, float* out_u, float* out_v,
                             int32_t* out_prim, int32_t* out_slot, int64_t* wave_nodes,
-                            int64_t* wave_leaves, int64_t* wave_tris) {
+                            int64_t* wave_leaves, int64_t* wave_tris, int32_t* lane_counts) {
     int64_t ntiles = (((int64_t)W + 7) / 8) * (((int64_t)H + 7) / 8);
     #pragma omp parallel for schedule(dynamic, 16)
     for (int64_t tile = 0; tile < ntiles; ++tile) {
@@ -743,6 +745,7 @@ TW_API void tw_trace_camera(const float* nodes4, const float* tris, int32_t root
             if (out_u) out_u[i] = hit ? h[l].u : 0.0f;
             if (out_v) out_v[i] = hit ? h[l].v : 0.0f;
             if (out_prim) out_prim[i] = h[l].prim;
+            if (lane_counts) { lane_counts[2 * i] = h[l].node_visits; lane_counts[2 * i + 1] = h[l].tri_tests; }
             if (out_slot) out_slot[i] = h[l].slot;
         }
     }

@@ -12,11 +12,14 @@ import torch  # noqa: F401  (must be imported before the HIP library; see module
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "librtx_hip.so")
 
-RT_ABI_VERSION = 2
+RT_ABI_VERSION = 3
 RT_ABSENT = 0xFFFFFFFF
 RT_EMPTY_ROOT = 0x7FFFFFFF
 RT_RENDER_SHADOWS = 1
 RT_RENDER_BILINEAR = 2
+RT_TRAVERSAL_PACKET = 0
+RT_TRAVERSAL_LANE = 1
+RT_LANE_STACK_MAX = 32
 
 _c_float_p = ctypes.POINTER(ctypes.c_float)
 _c_i32_p = ctypes.POINTER(ctypes.c_int32)
@@ -122,10 +125,21 @@ class rt_render_out(ctypes.Structure):
     ]
 
 
+class rt_tuning(ctypes.Structure):
+    _fields_ = [
+        ("primary", ctypes.c_int32),
+        ("shadow", ctypes.c_int32),
+        ("super_tile", ctypes.c_uint32),
+        ("lane_stack", ctypes.c_uint32),
+    ]
+
+
 # (name, restype, argtypes) of every exported entry point
 SIGNATURES = [
     ("rt_abi_version", ctypes.c_int, []),
     ("rt_last_error", ctypes.c_char_p, []),
+    ("rt_get_tuning", ctypes.c_int, [ctypes.POINTER(rt_tuning)]),
+    ("rt_set_tuning", ctypes.c_int, [ctypes.POINTER(rt_tuning)]),
     ("rt_scene_sizes_for", ctypes.c_int, [ctypes.c_uint32, ctypes.POINTER(rt_scene_sizes)]),
     ("rt_scene_build", ctypes.c_int,
      [ctypes.POINTER(rt_mesh_desc), ctypes.c_uint32, ctypes.POINTER(rt_scene), ctypes.c_void_p,

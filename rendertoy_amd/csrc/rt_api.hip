@@ -10,7 +10,14 @@ namespace {
 thread_local std::string g_last_error;
 std::mutex g_cu_mutex;
 int g_cu_count[64] = {0};
+std::mutex g_tuning_mutex;
+rt_tuning g_tuning = {RT_TRAVERSAL_PACKET, RT_TRAVERSAL_LANE, 0u, (uint32_t)RT_LANE_STACK_MAX};
 }  // namespace
+
+rt_tuning current_tuning() {
+    std::lock_guard<std::mutex> lock(g_tuning_mutex);
+    return g_tuning;
+}
 
 int set_error(int code, const std::string& msg) {
     g_last_error = msg;
@@ -43,3 +50,22 @@ int device_cu_count() {
 extern "C" int rt_abi_version(void) { return RT_ABI_VERSION; }
 
 extern "C" const char* rt_last_error(void) { return rt::g_last_error.c_str(); }
+
+extern "C" int rt_get_tuning(rt_tuning* out) {
+    if (!out) return rt::set_error(RT_EINVAL, "rt_get_tuning: out is NULL");
+    *out = rt::current_tuning();
+    return RT_OK;
+}
+
+extern "C" int rt_set_tuning(const rt_tuning* t) {
+    if (!t) return rt::set_error(RT_EINVAL, "rt_set_tuning: tuning is NULL");
+    auto alg_ok = [](int32_t a) { return a == RT_TRAVERSAL_PACKET || a == RT_TRAVERSAL_LANE; };
+    if (!alg_ok(t->primary) || !alg_ok(t->shadow))
+        return rt::set_error(RT_EINVAL, "rt_set_tuning: traversal must be RT_TRAVERSAL_PACKET or RT_TRAVERSAL_LANE");
+    if (t->super_tile > 64) return rt::set_error(RT_EINVAL, "rt_set_tuning: super_tile must be 0..64");
+    if (t->lane_stack < 1 || t->lane_stack > RT_LANE_STACK_MAX)
+        return rt::set_error(RT_EINVAL, "rt_set_tuning: lane_stack must be 1..RT_LANE_STACK_MAX");
+    std::lock_guard<std::mutex> lock(rt::g_tuning_mutex);
+    rt::g_tuning = *t;
+    return RT_OK;
+}

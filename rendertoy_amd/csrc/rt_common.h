@@ -323,15 +323,15 @@ RT_DEV void traverse_packet(const float4* __restrict__ nodes4, const float4* __r
 // subtrees, so the longest lane bounds the wave), nearer children first by the lane's own entry
 // distances, pending children on a per-lane stack in LDS. `stk` points at this thread's first
 // entry; entry k lives at stk[k * stride] (stride = threads per block, so the lanes of a wave touch
-// consecutive words at equal depth). Returns false if the ray needed more than kLaneStack pending
-// entries (the bound of the tree is 93): the hit is then invalid and the caller re-traces the ray
-// with the binary traversal, whose 64-entry stack always suffices.
-constexpr int kLaneStack = 32;
+// consecutive words at equal depth). Returns false if the ray needed more than `limit` (at most
+// kLaneStack) pending entries (3 per level of the 4-wide tree: up to 93): the hit is then invalid
+// and the caller traces the ray again with a traversal whose stack always suffices.
+constexpr int kLaneStack = RT_LANE_STACK_MAX;
 
 template <bool ANY>
 RT_DEV bool traverse_lane4(const float4* __restrict__ nodes4, const float4* __restrict__ tris,
                            int32_t root, const Ray& r, int32_t skip_slot, Hit& h, int32_t* stk,
-                           int stride) {
+                           int stride, int limit) {
     h.t = r.tmax;
     h.u = 0.0f;
     h.v = 0.0f;
@@ -381,7 +381,7 @@ RT_DEV bool traverse_lane4(const float4* __restrict__ nodes4, const float4* __re
 #undef RT_LSWAP
             const int nhit = (k0 < INFINITY) + (k1 < INFINITY) + (k2 < INFINITY) + (k3 < INFINITY);
             if (nhit > 0) {
-                if (sp + nhit - 1 > kLaneStack) return false;
+                if (sp + nhit - 1 > limit) return false;
                 // push the farther hit children (farthest first), continue with the nearest
                 if (nhit >= 4) stk[(sp++) * stride] = r3;
                 if (nhit >= 3) stk[(sp++) * stride] = r2;
@@ -433,4 +433,5 @@ int set_error(int code, const std::string& msg);
 int check_hip(hipError_t e, const char* what);
 int check_launch(const char* what);
 int device_cu_count();
+rt_tuning current_tuning();
 }  // namespace rt

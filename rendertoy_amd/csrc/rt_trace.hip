@@ -12,7 +12,6 @@
 //   ray toward the light per lit hit (wave-coherent any-hit over the tile's hit points, skipping
 //   each hit's own triangle), RGBA8 texture fetch, RGB float32 store.
 #include <algorithm>
-#include <cstdlib>
 
 #include "rt_common.h"
 
@@ -188,7 +187,7 @@ RT_DEV Ray camera_ray(const rt_camera& cam, uint32_t W, uint32_t H, const TilePi
 // the same closest hit. Otherwise the packet traversal throughout. Called by all lanes.
 template <bool ANY, bool LANE>
 RT_DEV void trace_block(const SceneArgs& sc, const Ray& r, bool want, int32_t skip_slot, Hit& h,
-                        int32_t* stk) {
+                        int32_t* stk, int lane_stack) {
     if (!LANE) {
         traverse_packet<ANY>(sc.nodes4, sc.tris, sc.root, r, want, skip_slot, h);
         return;
@@ -199,7 +198,9 @@ RT_DEV void trace_block(const SceneArgs& sc, const Ray& r, bool want, int32_t sk
     h.v = 0.0f;
     h.prim = -1;
     h.slot = -1;
-    if (want) ok = traverse_lane4<ANY>(sc.nodes4, sc.tris, sc.root, r, skip_slot, h, stk + threadIdx.x, 256);
+    if (want)
+        ok = traverse_lane4<ANY>(sc.nodes4, sc.tris, sc.root, r, skip_slot, h, stk + threadIdx.x, 256,
+                                 lane_stack);
     const bool redo = !ok;
     if (__ballot(redo)) {
         Hit h2;
@@ -211,14 +212,15 @@ RT_DEV void trace_block(const SceneArgs& sc, const Ray& r, bool want, int32_t sk
 // Camera closest hit: ray generation fused into the traversal, one hit record per pixel.
 template <bool LANE>
 __global__ void __launch_bounds__(256) k_trace_camera(SceneArgs sc, rt_camera cam, uint32_t W,
-                                                       uint32_t H, TileMap tm, rt_hits hits) {
+                                                       uint32_t H, TileMap tm, int lane_stack,
+                                                       rt_hits hits) {
     __shared__ int32_t stk[LANE ? kLaneStack * 256 : 1];
     uint32_t tx, ty;
     if (!block_tile(tm, tx, ty)) return;
     const TilePixel p = tile_pixel(W, H, tx, ty);
     const Ray r = camera_ray(cam, W, H, p);
     Hit h;
-    trace_block<false, LANE>(sc, r, p.valid, -1, h, stk);
+    trace_block<false, LANE>(sc, r, p.valid, -1, h, stk, lane_stack);
     if (h.prim >= 0) refine_hit(sc.tris, r, h);
     if (p.valid) store_hit(sc, hits, (uint64_t)p.row * W + p.col, h);
 }
@@ -271,7 +273,7 @@ RT_DEV void tex_sample(const rt_texture& tx, float u, float v, bool bilinear, fl
 // primary / shadow rays (else the packet traversal).
 template <bool PLANE, bool SLANE>
 __global__ void __launch_bounds__(256) k_render(SceneArgs sc, rt_camera cam, rt_render_params p,
-                                                 TileMap tm, float* __restrict__ rgb,
+                                                 TileMap tm, int lane_stack, float* __restrict__ rgb,
                                                  float* __restrict__ out_t,
                                                  int32_t* __restrict__ out_prim,
                                                  uint32_t* __restrict__ stats) {
@@ -282,7 +284,7 @@ __global__ void __launch_bounds__(256) k_render(SceneArgs sc, rt_camera cam, rt_
     const TilePixel px = tile_pixel(W, H, tx, ty);
     const Ray r = camera_ray(cam, W, H, px);
     Hit h;
-    trace_block<false, PLANE>(sc, r, px.valid, -1, h, stk);
+    trace_block<false, PLANE>(sc, r, px.valid, -1, h, stk, lane_stack);
     const bool hit = px.valid && h.prim >= 0;
     if (hit) refine_hit(sc.tris, r, h);
 
@@ -316,7 +318,7 @@ __global__ void __launch_bounds__(256) k_render(SceneArgs sc, rt_camera cam, rt_
     sr.tmin = p.shadow_tmin;
     sr.tmax = INFINITY;
     Hit sh;
-    trace_block<true, SLANE>(sc, sr, want_shadow, hit ? h.slot : -1, sh, stk);
+    trace_block<true, SLANE>(sc, sr, want_shadow, hit ? h.slot : -1, sh, stk, lane_stack);
     const bool occluded = want_shadow && sh.prim >= 0;
 
     float alb[3] = {1.0f, 1.0f, 1.0f};
@@ -385,30 +387,6 @@ int check_rays(const rt_rays* r, const char* what) {
     return RT_OK;
 }
 
-bool lane_traversal() {
-    static const bool v = [] {
-        const char* e = getenv("RT_TRAVERSAL");
-        return e && std::string(e) == "lane";
-    }();
-    return v;
-}
-
-int render_mode() {
-    static const int v = [] {
-        const char* e = getenv("RT_RENDER_MODE");
-        return e ? atoi(e) : 1;
-    }();
-    return v;
-}
-
-uint32_t super_tile() {
-    static const uint32_t v = [] {
-        const char* e = getenv("RT_SUPER");
-        return e ? (uint32_t)atoi(e) : 0u;
-    }();
-    return v;
-}
-
 inline uint32_t blocks_for(uint64_t n, uint32_t per) { return (uint32_t)((n + per - 1) / per); }
 
 }  // namespace
@@ -462,13 +440,11 @@ extern "C" int rt_trace_camera(const rt_scene* scene, const rt_camera* cam, uint
     const uint64_t n = (uint64_t)W * H;
     if (n == 0) return RT_OK;
     if (n > 0x7FFFFFFFull) return set_error(RT_ECAPACITY, "rt_trace_camera: image too large");
-    const TileMap tm = make_tile_map(W, H, super_tile());
-    if (lane_traversal())
-        hipLaunchKernelGGL(k_trace_camera<true>, dim3(tile_map_blocks(tm)), dim3(256), 0,
-                           (hipStream_t)stream, scene_args(scene), *cam, W, H, tm, *hits);
-    else
-        hipLaunchKernelGGL(k_trace_camera<false>, dim3(tile_map_blocks(tm)), dim3(256), 0,
-                           (hipStream_t)stream, scene_args(scene), *cam, W, H, tm, *hits);
+    const rt_tuning tu = current_tuning();
+    const TileMap tm = make_tile_map(W, H, tu.super_tile);
+    auto kern = tu.primary == RT_TRAVERSAL_LANE ? k_trace_camera<true> : k_trace_camera<false>;
+    hipLaunchKernelGGL(kern, dim3(tile_map_blocks(tm)), dim3(256), 0, (hipStream_t)stream,
+                       scene_args(scene), *cam, W, H, tm, (int)tu.lane_stack, *hits);
     return check_launch("rt_trace_camera");
 }
 
@@ -492,11 +468,12 @@ extern "C" int rt_render(const rt_scene* scene, const rt_camera* cam, const rt_r
         return set_error(RT_EINVAL, "rt_render: n_textures > 0 but textures_dev is NULL");
     hipStream_t s = (hipStream_t)stream;
     if (out->stats) hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, out->stats, 4u);
-    const TileMap tm = make_tile_map(p->width, p->height, super_tile());
-    const int mode = render_mode();
-    auto kern = mode == 3 ? k_render<true, true> : mode == 2 ? k_render<true, false>
-              : mode == 1 ? k_render<false, true> : k_render<false, false>;
+    const rt_tuning tu = current_tuning();
+    const TileMap tm = make_tile_map(p->width, p->height, tu.super_tile);
+    const bool pl = tu.primary == RT_TRAVERSAL_LANE, sl = tu.shadow == RT_TRAVERSAL_LANE;
+    auto kern = pl ? (sl ? k_render<true, true> : k_render<true, false>)
+                   : (sl ? k_render<false, true> : k_render<false, false>);
     hipLaunchKernelGGL(kern, dim3(tile_map_blocks(tm)), dim3(256), 0, s, scene_args(scene), *cam, *p,
-                       tm, out->rgb, out->t, out->prim, out->stats);
+                       tm, (int)tu.lane_stack, out->rgb, out->t, out->prim, out->stats);
     return check_launch("rt_render");
 }

@@ -1,0 +1,45 @@
+"""Launch choices of the camera and render kernels (process-wide): which traversal walks the
+primary and the shadow rays, the block -> XCD mapping, and the per-lane stack size. Every choice
+returns the same hits; they trade speed only. The reference has no counterpart (its kernels run a
+fixed 32-work-item group, rendering/_core.py:286)."""
+import contextlib
+
+from . import _native
+
+PACKET = _native.RT_TRAVERSAL_PACKET
+LANE = _native.RT_TRAVERSAL_LANE
+_NAMES = {"packet": PACKET, "lane": LANE}
+
+
+def get_tuning():
+    """Current settings as a dict {primary, shadow, super_tile, lane_stack}."""
+    lib = _native.load()
+    t = _native.rt_tuning()
+    _native.check(lib.rt_get_tuning(t), "rt_get_tuning")
+    return {"primary": t.primary, "shadow": t.shadow, "super_tile": t.super_tile,
+            "lane_stack": t.lane_stack}
+
+
+def set_tuning(**kw):
+    """Updates the given fields; traversals may be given as "packet"/"lane" or the constants.
+    Raises RuntimeError (the library's message) for out-of-range values."""
+    lib = _native.load()
+    cur = get_tuning()
+    for k, v in kw.items():
+        if k not in cur:
+            raise TypeError(f"unknown tuning field {k!r}")
+        cur[k] = _NAMES.get(v, v) if k in ("primary", "shadow") else v
+    t = _native.rt_tuning(int(cur["primary"]), int(cur["shadow"]), int(cur["super_tile"]),
+                          int(cur["lane_stack"]))
+    _native.check(lib.rt_set_tuning(t), "rt_set_tuning")
+
+
+@contextlib.contextmanager
+def tuning(**kw):
+    """Context manager: set_tuning(**kw) inside the block, previous settings restored after."""
+    old = get_tuning()
+    set_tuning(**kw)
+    try:
+        yield
+    finally:
+        set_tuning(**old)

@@ -74,6 +74,7 @@ int main(void) {
     O(rt_render_params, shadow_tmin) O(rt_render_params, textures_dev)
     O(rt_render_params, mesh_texture_dev) O(rt_render_params, n_textures)
     S(rt_render_out) O(rt_render_out, stats)
+    S(rt_tuning) O(rt_tuning, shadow) O(rt_tuning, super_tile) O(rt_tuning, lane_stack)
     return 0;
 }
 """
@@ -93,10 +94,27 @@ def test_ctypes_layouts_match_the_c_header(tmp_path):
         k, v = line.split()
         c[k] = int(v)
     for name in ("rt_mesh_desc", "rt_scene", "rt_scene_sizes", "rt_rays", "rt_hits", "rt_camera",
-                 "rt_texture", "rt_render_params", "rt_render_out"):
+                 "rt_texture", "rt_render_params", "rt_render_out", "rt_tuning"):
         cls = getattr(N, name)
         assert ctypes.sizeof(cls) == c[name], name
         for key, off in c.items():
             if key.startswith(name + "."):
                 field = key.split(".", 1)[1]
                 assert getattr(cls, field).offset == off, key
+
+
+def test_tuning_get_set_and_range_checks_without_gpu():
+    import rendertoy_amd as ren
+    from rendertoy_amd import _native
+    d = ren.get_tuning()
+    assert d == {"primary": _native.RT_TRAVERSAL_PACKET, "shadow": _native.RT_TRAVERSAL_LANE,
+                 "super_tile": 0, "lane_stack": _native.RT_LANE_STACK_MAX}
+    with ren.tuning(primary="lane", lane_stack=3, super_tile=2):
+        assert ren.get_tuning() == {"primary": 1, "shadow": 1, "super_tile": 2, "lane_stack": 3}
+    assert ren.get_tuning() == d
+    for bad in ({"lane_stack": 0}, {"lane_stack": 33}, {"primary": 2}, {"super_tile": 65}):
+        with pytest.raises(RuntimeError, match="rt_set_tuning"):
+            ren.set_tuning(**bad)
+    assert ren.get_tuning() == d
+    with pytest.raises(TypeError):
+        ren.set_tuning(tile=4)

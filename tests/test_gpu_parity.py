@@ -147,6 +147,27 @@ def test_render_c1_matches_twin(c1, twin):
     assert list(s[:3]) == list(st)
 
 
+@pytest.mark.parametrize("primary,shadow,lane_stack,super_tile", [
+    ("lane", "lane", 32, 0), ("lane", "packet", 2, 3), ("packet", "lane", 1, 1), ("lane", "lane", 4, 0)])
+def test_tuning_choices_give_identical_results(c1, twin, primary, shadow, lane_stack, super_tile):
+    """Every traversal / stack / block-mapping choice returns the twin's hits and RGB. Stacks of 1-4
+    entries overflow on most rays, which the waves then trace again with the packet traversal."""
+    import rendertoy_amd as ren
+    meshes, tw, rc = c1
+    cam = camera()
+    rgb, t, prim, st = tw.render(cam.as_array(), 512, 512, LIGHT, shadows=True)
+    ref = tw.trace_camera(cam.as_array(), 512, 512)
+    with ren.tuning(primary=primary, shadow=shadow, lane_stack=lane_stack, super_tile=super_tile):
+        stats = torch.zeros(4, dtype=torch.int32, device="cuda")
+        img, hits = rc.render(cam, 512, 512, light=LIGHT, shadows=True, bilinear=True,
+                              return_hits=True, stats=stats)
+        gc = {k: v.cpu().numpy() for k, v in rc.cast_camera(cam, 512, 512).items()}
+    np.testing.assert_array_equal(hits["prim"].cpu().numpy(), prim)
+    np.testing.assert_array_equal(_u32(img.cpu().numpy()), _u32(rgb))
+    assert list(stats.cpu().numpy()[:3]) == list(st)
+    assert_trace_equal(gc, ref)
+
+
 def test_multi_mesh_nonindexed_int64_and_empty(ren, twin):
     from rendertoy_amd import scenes
     a = scenes.icosphere(8, 11)
