@@ -46,6 +46,10 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-roofline", action="store_true")
+    ap.add_argument("--no-batched", action="store_true",
+                    help="skip the batched-vs-plain measurement (256 scenes)")
+    ap.add_argument("--batch-scenes", type=int, default=256)
+    ap.add_argument("--batch-reps", type=int, default=5)
     ap.add_argument("--copy-calib-mib", type=int, default=0,
                     help="first run 3 elementwise int32 adds over this many MiB (a kernel of "
                          "known read/write traffic that scales the FETCH_SIZE/WRITE_SIZE "
@@ -73,6 +77,75 @@ def measured_traffic(workload):
         return int(float(vals["hbm_bytes_per_launch"]))
     except (KeyError, ValueError):
         return None
+
+
+def batched_workload(n_scenes, ren, scenes, torch):
+    """The batched workload: scene i has T_i = exp(U[ln 1e3, ln 5e5]) triangles (icosphere
+    frequency round(sqrt(T_i / 20))), a random rotation about a uniform axis and a scale in
+    [0.8, 1.2], and an image of 128^2, 256^2, 512^2 or 1024^2 (uniform), all from the seed
+    3000 + i; camera (0,0,3) -> origin, 45 deg; Lambert + one shadow ray per lit hit, albedo 1."""
+    from rendertoy_amd.batch import RenderRequest
+    reqs, tris = [], 0
+    for i in range(n_scenes):
+        rng = np.random.default_rng(3000 + i)
+        T = math.exp(rng.uniform(math.log(1e3), math.log(5e5)))
+        f = max(1, int(round(math.sqrt(T / 20.0))))
+        res = int(rng.choice([128, 256, 512, 1024]))
+        M = scenes.random_transform(rng)
+        d = scenes.icosphere(f, 3000 + i)
+        pos = (d["pos"].astype(np.float64) @ M).astype(np.float32)
+        nrm = d["nrm"].astype(np.float64) @ M
+        nrm = (nrm / np.linalg.norm(nrm, axis=1, keepdims=True)).astype(np.float32)
+        rc = ren.Raycaster([ren.mesh_from_arrays(pos, d["idx"], nrm, d["uv"])])
+        cam = ren.Camera((0.0, 0.0, 3.0), (0.0, 0.0, 0.0), (0.0, 1.0, 0.0), math.pi / 4, 1.0)
+        reqs.append(RenderRequest(rc, cam, res, res))
+        tris += rc.n_tris
+    torch.cuda.synchronize()
+    return reqs, tris
+
+
+def measure_batched(args, ren, scenes, torch):
+    """256 plain rt_render launches back to back vs the same jobs in one graph-captured
+    rt_render_batch launch (autotuned launch shape). Latency of a set of requests submitted
+    together = time until the last image is done."""
+    from rendertoy_amd.batch import BatchRender, autotune, time_launches
+    g0 = time.time()
+    reqs, tris = batched_workload(args.batch_scenes, ren, scenes, torch)
+    gen_s = time.time() - g0
+    dev = torch.device("cuda")
+    # plain: one rt_render per job, each into its own image, all on one stream
+    plain_imgs = [torch.empty((r.height, r.width, 3), dtype=torch.float32, device=dev) for r in reqs]
+    plain_stats = torch.zeros((len(reqs), 4), dtype=torch.int32, device=dev)
+
+    def plain():
+        for k, r in enumerate(reqs):
+            r.raycaster.render(r.camera, r.width, r.height, out=plain_imgs[k], stats=plain_stats[k])
+    plain_ms = time_launches(plain, args.batch_reps)
+    b = BatchRender(reqs)
+    best, table = autotune(b, reps=2)
+    b.capture()
+    batch_ms = time_launches(b.replay, args.batch_reps)
+    torch.cuda.synchronize()
+    same = all(torch.equal(x.view(torch.int32), y.view(torch.int32)) for x, y in zip(b.images, plain_imgs))
+    same = same and torch.equal(b.stats, plain_stats)
+    st = b.stats.cpu().numpy()
+    rays = int(b.rays_primary + st[:, 1].sum())
+    p_med, b_med = float(np.median(plain_ms)), float(np.median(batch_ms))
+    return {
+        "what": f"{len(reqs)} scenes ({tris} triangles), 128^2..1024^2 images, primary + shadow rays",
+        "scenes": len(reqs), "triangles": int(tris), "primary_rays": int(b.rays_primary),
+        "rays": rays,
+        "plain_ms_p50": round(p_med, 3), "plain_ms_p99": round(percentile(plain_ms, 99), 3),
+        "batched_ms_p50": round(b_med, 3), "batched_ms_p99": round(percentile(batch_ms, 99), 3),
+        "plain_mrays_per_s": round(rays / (p_med * 1e-3) / 1e6, 2),
+        "batched_mrays_per_s": round(rays / (b_med * 1e-3) / 1e6, 2),
+        "speedup": round(p_med / b_med, 3),
+        "batched_equals_plain": bool(same),
+        "autotuned": {"waves_per_cu": best.waves_per_cu, "tiles_per_grab": best.tiles_per_grab,
+                      "shadow": best.shadow},
+        "autotune_ms": {f"w{c.waves_per_cu}g{c.tiles_per_grab}{c.shadow[0]}": round(ms, 3) for c, ms in table},
+        "setup_s": round(gen_s, 1),
+    }
 
 
 def percentile(a, q):
@@ -197,6 +270,9 @@ def main():
         },
         "wall_s_timed": round(wall, 4),
     }
+
+    if not args.no_batched:
+        result["batched"] = measure_batched(args, ren, scenes, torch)
 
     # ---- roofline and CPU baseline from the CPU restatement (oracle) ----
     if not args.no_roofline or not args.no_cpu_baseline:

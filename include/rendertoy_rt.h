@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define RT_ABI_VERSION 3
+#define RT_ABI_VERSION 4
 
 enum {
     RT_OK = 0,
@@ -212,6 +212,44 @@ uint64_t rt_render_workspace_bytes(uint32_t n_pixels);
 int rt_render(const rt_scene* scene, const rt_camera* cam, const rt_render_params* params,
               const rt_render_out* out, void* workspace_dev, uint64_t workspace_bytes,
               void* stream);
+
+/* ---- batched render: many independent (scene, camera, image) jobs in one persistent launch ----
+ * No reference counterpart (the reference renders one image per call). One job = one rt_render
+ * call's arguments; the array lives in device memory so that a captured graph can be replayed
+ * after the caller rewrites it. */
+typedef struct rt_render_job {
+    rt_scene scene;
+    rt_camera cam;
+    rt_render_params params;
+    rt_render_out out;
+} rt_render_job;
+
+typedef struct rt_batch_config {
+    uint32_t waves_per_cu;    /* persistent waves per compute unit, 1..32 */
+    uint32_t tiles_per_grab;  /* consecutive 8x8 pixel tiles a wave takes per queue pop, 1..64 */
+    int32_t shadow;           /* RT_TRAVERSAL_* for the shadow rays (camera rays: packet) */
+    uint32_t lane_stack;      /* per-lane stack entries for RT_TRAVERSAL_LANE, 1..RT_LANE_STACK_MAX */
+} rt_batch_config;
+
+#define RT_BATCH_MAX_JOBS 4096u
+#define RT_BATCH_MAX_JOB_PIXELS (1u << 24)
+
+/* Workspace of rt_render_batch for n_jobs jobs (work counter, per-job tile prefix, status). */
+uint64_t rt_render_batch_workspace_bytes(uint32_t n_jobs);
+
+/* Renders every job exactly as rt_render would (bit-identical output), in two kernels on `stream`:
+ * a prepare kernel (per-job 8x8-tile counts and their prefix sum, work counter and every job's
+ * stats zeroed) and one persistent kernel whose waves pull tiles of any job from a global queue.
+ * Graph-capturable (no host synchronisation, no memset nodes). `jobs_dev`: device array of n_jobs
+ * records (1..RT_BATCH_MAX_JOBS). A job that cannot be rendered is skipped and sets a bit of the
+ * status word (rt_render_batch_status): 1 = more than RT_BATCH_MAX_JOB_PIXELS pixels, 2 = a NULL
+ * rgb output, texture table (with n_textures > 0) or scene array (non-empty scene). Jobs may share
+ * scenes and textures; their output buffers must not overlap. */
+int rt_render_batch(const rt_render_job* jobs_dev, uint32_t n_jobs, const rt_batch_config* cfg,
+                    void* workspace_dev, uint64_t workspace_bytes, void* stream);
+
+/* Device address of the uint32 status word inside a batch workspace (0 = every job rendered). */
+const uint32_t* rt_render_batch_status(const void* workspace_dev);
 
 #ifdef __cplusplus
 }

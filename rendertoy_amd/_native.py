@@ -12,7 +12,7 @@ import torch  # noqa: F401  (must be imported before the HIP library; see module
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "librtx_hip.so")
 
-RT_ABI_VERSION = 3
+RT_ABI_VERSION = 4
 RT_ABSENT = 0xFFFFFFFF
 RT_EMPTY_ROOT = 0x7FFFFFFF
 RT_RENDER_SHADOWS = 1
@@ -20,6 +20,8 @@ RT_RENDER_BILINEAR = 2
 RT_TRAVERSAL_PACKET = 0
 RT_TRAVERSAL_LANE = 1
 RT_LANE_STACK_MAX = 32
+RT_BATCH_MAX_JOBS = 4096
+RT_BATCH_MAX_JOB_PIXELS = 1 << 24
 
 _c_float_p = ctypes.POINTER(ctypes.c_float)
 _c_i32_p = ctypes.POINTER(ctypes.c_int32)
@@ -134,6 +136,24 @@ class rt_tuning(ctypes.Structure):
     ]
 
 
+class rt_render_job(ctypes.Structure):
+    _fields_ = [
+        ("scene", rt_scene),
+        ("cam", rt_camera),
+        ("params", rt_render_params),
+        ("out", rt_render_out),
+    ]
+
+
+class rt_batch_config(ctypes.Structure):
+    _fields_ = [
+        ("waves_per_cu", ctypes.c_uint32),
+        ("tiles_per_grab", ctypes.c_uint32),
+        ("shadow", ctypes.c_int32),
+        ("lane_stack", ctypes.c_uint32),
+    ]
+
+
 # (name, restype, argtypes) of every exported entry point
 SIGNATURES = [
     ("rt_abi_version", ctypes.c_int, []),
@@ -160,6 +180,11 @@ SIGNATURES = [
     ("rt_render", ctypes.c_int,
      [ctypes.POINTER(rt_scene), ctypes.POINTER(rt_camera), ctypes.POINTER(rt_render_params),
       ctypes.POINTER(rt_render_out), ctypes.c_void_p, ctypes.c_uint64, ctypes.c_void_p]),
+    ("rt_render_batch_workspace_bytes", ctypes.c_uint64, [ctypes.c_uint32]),
+    ("rt_render_batch", ctypes.c_int,
+     [ctypes.c_void_p, ctypes.c_uint32, ctypes.POINTER(rt_batch_config), ctypes.c_void_p,
+      ctypes.c_uint64, ctypes.c_void_p]),
+    ("rt_render_batch_status", ctypes.c_void_p, [ctypes.c_void_p]),
 ]
 
 _lib = None

@@ -325,16 +325,9 @@ class Raycaster:
             self._render_ws = torch.empty(need, dtype=torch.uint8, device="cuda")
         return self._render_ws
 
-    def render(self, camera: Camera, width: int, height: int, light=None, textures=None,
-               mesh_textures=None, shadows=True, bilinear=True, ambient=0.2,
-               background=(0.0, 0.0, 0.0), shadow_tmin=1e-4, out=None, return_hits=False,
-               stats=None):
-        """RGB float32 image [height, width, 3] of the scene: closest hit, Lambert with
-        L = normalize(1,1,1) and ambient 0.2 by default (tutorials/lesson09_texture_mapping.py:73),
-        shadow ray toward L, albedo from RGBA8 textures (texels as uint8 tensors [H, W, 4] or
-        (memory, Texture2D) pairs from create_texture2D(w, h, RGBA)); `mesh_textures[m]` picks
-        the texture of mesh m (-1: albedo 1)."""
-        lib = N.require_gpu()
+    def _render_args(self, width, height, light, textures, mesh_textures, shadows, bilinear,
+                     ambient, background, shadow_tmin, out, return_hits, stats):
+        """rt_render_params / rt_render_out of one render plus the device tensors they point to."""
         n = width * height
         dev = torch.device("cuda")
         if light is None:
@@ -375,15 +368,43 @@ class Raycaster:
             p.mesh_texture_dev = mt.data_ptr()
         if out is None:
             out = torch.empty((height, width, 3), dtype=torch.float32, device=dev)
+        elif out.dtype != torch.float32 or out.numel() < 3 * n or not out.is_contiguous():
+            raise TypeError("out must be a contiguous float32 tensor of height*width*3 elements")
         o = N.rt_render_out()
         o.rgb = out.data_ptr()
         hits = None
         if return_hits:
-            hits = {"t": torch.empty(n, dtype=torch.float32, device=dev),
-                    "prim": torch.empty(n, dtype=torch.int32, device=dev)}
+            hits = {"t": torch.empty(max(n, 1), dtype=torch.float32, device=dev),
+                    "prim": torch.empty(max(n, 1), dtype=torch.int32, device=dev)}
             o.t, o.prim = hits["t"].data_ptr(), hits["prim"].data_ptr()
         if stats is not None:
             o.stats = stats.data_ptr()
+        return p, o, out, hits, tex_keep
+
+    def render_many(self, requests, config=None):
+        """Renders several images of this scene in one persistent launch (rt_render_batch):
+        `requests` = list of dicts of render() keyword arguments (camera, width, height, ...).
+        Returns the list of RGB images; see rendertoy_amd.batch for batches over many scenes."""
+        from .batch import RenderRequest, BatchRender
+        reqs = [RenderRequest(self, **r) for r in requests]
+        b = BatchRender(reqs, config=config)
+        b.launch()
+        return b.images
+
+    def render(self, camera: Camera, width: int, height: int, light=None, textures=None,
+               mesh_textures=None, shadows=True, bilinear=True, ambient=0.2,
+               background=(0.0, 0.0, 0.0), shadow_tmin=1e-4, out=None, return_hits=False,
+               stats=None):
+        """RGB float32 image [height, width, 3] of the scene: closest hit, Lambert with
+        L = normalize(1,1,1) and ambient 0.2 by default (tutorials/lesson09_texture_mapping.py:73),
+        shadow ray toward L, albedo from RGBA8 textures (texels as uint8 tensors [H, W, 4] or
+        (memory, Texture2D) pairs from create_texture2D(w, h, RGBA)); `mesh_textures[m]` picks
+        the texture of mesh m (-1: albedo 1)."""
+        lib = N.require_gpu()
+        n = width * height
+        p, o, out, hits, tex_keep = self._render_args(width, height, light, textures, mesh_textures,
+                                                      shadows, bilinear, ambient, background,
+                                                      shadow_tmin, out, return_hits, stats)
         ws = self._render_workspace(n)
         cam = camera.to_c()
         N.check(lib.rt_render(ctypes.byref(self._scene), ctypes.byref(cam), ctypes.byref(p),

@@ -1,0 +1,161 @@
+"""Batched rendering: many independent (scene, camera, image) jobs in one persistent launch.
+
+The reference renders one image per call (its kernels are dispatched one at a time through
+rendering/_core.py:247-299). Here a batch is a device array of job records (rt_render_job: the
+arguments of one rt_render call) rendered by rt_render_batch: a prepare kernel and one persistent
+kernel whose waves pull 8x8 tiles of any job from a global queue, so that the long dependent
+chains of the hardest tiles of all images overlap instead of running one frame after another.
+Output is bit-identical to rendering each job with Raycaster.render.
+
+  reqs = [RenderRequest(rc_i, camera_i, w_i, h_i) for ...]
+  b = BatchRender(reqs)             # device job table, outputs, workspace
+  b.launch()                        # or: b.capture(); b.replay()  (one hipGraph launch)
+  b.images[i]                       # [h_i, w_i, 3] float32
+  cfg, table = autotune(b)          # waves per CU x tiles per grab x shadow traversal
+"""
+import ctypes
+import dataclasses
+import time
+import typing
+
+import numpy as np
+import torch
+
+from . import _native as N
+
+
+@dataclasses.dataclass
+class RenderRequest:
+    """Arguments of one Raycaster.render call."""
+    raycaster: typing.Any
+    camera: typing.Any
+    width: int
+    height: int
+    light: typing.Any = None
+    textures: typing.Any = None
+    mesh_textures: typing.Any = None
+    shadows: bool = True
+    bilinear: bool = True
+    ambient: float = 0.2
+    background: typing.Any = (0.0, 0.0, 0.0)
+    shadow_tmin: float = 1e-4
+    out: typing.Any = None
+    return_hits: bool = False
+
+
+@dataclasses.dataclass
+class BatchConfig:
+    """Launch shape of rt_render_batch (rt_batch_config)."""
+    waves_per_cu: int = 8
+    tiles_per_grab: int = 1
+    shadow: str = "lane"
+    lane_stack: int = N.RT_LANE_STACK_MAX
+
+    def to_c(self):
+        c = N.rt_batch_config()
+        c.waves_per_cu = int(self.waves_per_cu)
+        c.tiles_per_grab = int(self.tiles_per_grab)
+        c.shadow = {"packet": N.RT_TRAVERSAL_PACKET, "lane": N.RT_TRAVERSAL_LANE}[self.shadow]
+        c.lane_stack = int(self.lane_stack)
+        return c
+
+
+class BatchRender:
+    """A batch of render jobs resident on the device (job table, outputs, workspace).
+
+    `images[i]` / `hits[i]` / `stats[i]` are the outputs of request i. Relaunching (launch() or a
+    captured graph's replay()) re-renders every job with the current contents of the inputs."""
+
+    def __init__(self, requests: typing.List[RenderRequest], config: BatchConfig = None,
+                 stats: bool = True, stream=None):
+        lib = N.require_gpu()
+        if not requests:
+            raise ValueError("BatchRender needs at least one request")
+        if len(requests) > N.RT_BATCH_MAX_JOBS:
+            raise ValueError(f"at most {N.RT_BATCH_MAX_JOBS} jobs per batch")
+        self.config = config or BatchConfig()
+        self._stream = stream
+        dev = torch.device("cuda")
+        n = len(requests)
+        self.stats = torch.zeros((n, 4), dtype=torch.int32, device=dev) if stats else None
+        jobs = (N.rt_render_job * n)()
+        self.images, self.hits, self._keep = [], [], []
+        self.rays_primary = 0
+        for i, r in enumerate(requests):
+            if r.width * r.height > N.RT_BATCH_MAX_JOB_PIXELS:
+                raise ValueError(f"request {i}: more than {N.RT_BATCH_MAX_JOB_PIXELS} pixels")
+            p, o, out, hits, keep = r.raycaster._render_args(
+                r.width, r.height, r.light, r.textures, r.mesh_textures, r.shadows, r.bilinear,
+                r.ambient, r.background, r.shadow_tmin, r.out, r.return_hits,
+                self.stats[i] if stats else None)
+            jobs[i].scene = r.raycaster._scene
+            jobs[i].cam = r.camera.to_c()
+            jobs[i].params = p
+            jobs[i].out = o
+            self.images.append(out)
+            self.hits.append(hits)
+            self._keep += keep + [r.raycaster]
+            self.rays_primary += r.width * r.height
+        self.n_jobs = n
+        self.jobs_dev = torch.frombuffer(bytearray(bytes(jobs)), dtype=torch.uint8).to(dev)
+        wsb = lib.rt_render_batch_workspace_bytes(n)
+        self.workspace = torch.zeros(wsb, dtype=torch.uint8, device=dev)
+        self._graph = None
+        torch.cuda.current_stream().synchronize()
+
+    def launch(self, config: BatchConfig = None):
+        """Renders every job (asynchronous on the stream)."""
+        lib = N.load()
+        cfg = (config or self.config).to_c()
+        N.check(lib.rt_render_batch(self.jobs_dev.data_ptr(), self.n_jobs, ctypes.byref(cfg),
+                                    self.workspace.data_ptr(), self.workspace.numel(),
+                                    N.stream_handle(self._stream)), "rt_render_batch")
+
+    def capture(self):
+        """Captures one launch (prepare + persistent kernel) into a hipGraph; replay() runs it."""
+        self.launch()  # warm up outside the capture
+        torch.cuda.synchronize()
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            self.launch()
+        self._graph = g
+        return g
+
+    def replay(self):
+        if self._graph is None:
+            raise RuntimeError("capture() first")
+        self._graph.replay()
+
+    def status(self) -> int:
+        """0 if every job was rendered, bit 1 if a job exceeded the pixel limit (synchronises)."""
+        return int(self.workspace[4:8].view(torch.int32).item())
+
+
+def time_launches(fn, reps, warmup=1):
+    """Per-call durations (ms) of `fn` from HIP events on the current stream."""
+    for _ in range(warmup):
+        fn()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(reps + 1)]
+    torch.cuda.synchronize()
+    ev[0].record()
+    for k in range(reps):
+        fn()
+        ev[k + 1].record()
+    torch.cuda.synchronize()
+    return [ev[k].elapsed_time(ev[k + 1]) for k in range(reps)]
+
+
+DEFAULT_CANDIDATES = [BatchConfig(w, g, s) for s in ("lane", "packet") for w in (4, 8, 16)
+                      for g in (1, 4)]
+
+
+def autotune(batch: BatchRender, candidates=None, reps=3):
+    """Times every candidate launch shape on this batch (median of `reps` launches after one
+    warm-up) and sets batch.config to the fastest. Returns (best, [(config, median_ms), ...])."""
+    table = []
+    for c in candidates or DEFAULT_CANDIDATES:
+        ms = time_launches(lambda: batch.launch(c), reps)
+        table.append((c, float(np.median(ms))))
+    best = min(table, key=lambda x: x[1])[0]
+    batch.config = best
+    return best, table

@@ -187,7 +187,7 @@ RT_DEV Ray camera_ray(const rt_camera& cam, uint32_t W, uint32_t H, const TilePi
 // the same closest hit. Otherwise the packet traversal throughout. Called by all lanes.
 template <bool ANY, bool LANE>
 RT_DEV void trace_block(const SceneArgs& sc, const Ray& r, bool want, int32_t skip_slot, Hit& h,
-                        int32_t* stk, int lane_stack) {
+                        int32_t* stk, int stk_stride, int lane_stack) {
     if (!LANE) {
         traverse_packet<ANY>(sc.nodes4, sc.tris, sc.root, r, want, skip_slot, h);
         return;
@@ -199,7 +199,7 @@ RT_DEV void trace_block(const SceneArgs& sc, const Ray& r, bool want, int32_t sk
     h.prim = -1;
     h.slot = -1;
     if (want)
-        ok = traverse_lane4<ANY>(sc.nodes4, sc.tris, sc.root, r, skip_slot, h, stk + threadIdx.x, 256,
+        ok = traverse_lane4<ANY>(sc.nodes4, sc.tris, sc.root, r, skip_slot, h, stk, stk_stride,
                                  lane_stack);
     const bool redo = !ok;
     if (__ballot(redo)) {
@@ -220,7 +220,7 @@ __global__ void __launch_bounds__(256) k_trace_camera(SceneArgs sc, rt_camera ca
     const TilePixel p = tile_pixel(W, H, tx, ty);
     const Ray r = camera_ray(cam, W, H, p);
     Hit h;
-    trace_block<false, LANE>(sc, r, p.valid, -1, h, stk, lane_stack);
+    trace_block<false, LANE>(sc, r, p.valid, -1, h, stk + threadIdx.x, 256, lane_stack);
     if (h.prim >= 0) refine_hit(sc.tris, r, h);
     if (p.valid) store_hit(sc, hits, (uint64_t)p.row * W + p.col, h);
 }
@@ -269,22 +269,20 @@ RT_DEV void tex_sample(const rt_texture& tx, float u, float v, bool bilinear, fl
     }
 }
 
-// Fused render of one frame (see the file comment). PLANE / SLANE: per-lane traversal for the
-// primary / shadow rays (else the packet traversal).
+// Shading of one pixel per lane: primary closest hit, FP64 refinement, Lambert with interpolated
+// normals, one shadow ray toward the light per lit hit (skipping the hit's own triangle), RGBA8
+// albedo, RGB store; per-wave counts into stats. PLANE / SLANE: per-lane traversal for the primary /
+// shadow rays (else the packet traversal); `stk` = this thread's lane stack (entries `stk_stride`
+// words apart). Called by all 64 lanes of a wave.
 template <bool PLANE, bool SLANE>
-__global__ void __launch_bounds__(256) k_render(SceneArgs sc, rt_camera cam, rt_render_params p,
-                                                 TileMap tm, int lane_stack, float* __restrict__ rgb,
-                                                 float* __restrict__ out_t,
-                                                 int32_t* __restrict__ out_prim,
-                                                 uint32_t* __restrict__ stats) {
+RT_DEV void render_pixel(const SceneArgs& sc, const rt_camera& cam, const rt_render_params& p,
+                         const TilePixel& px, float* __restrict__ rgb, float* __restrict__ out_t,
+                         int32_t* __restrict__ out_prim, uint32_t* __restrict__ stats, int32_t* stk,
+                         int stk_stride, int lane_stack) {
     const uint32_t W = p.width, H = p.height;
-    __shared__ int32_t stk[(PLANE || SLANE) ? kLaneStack * 256 : 1];
-    uint32_t tx, ty;
-    if (!block_tile(tm, tx, ty)) return;
-    const TilePixel px = tile_pixel(W, H, tx, ty);
     const Ray r = camera_ray(cam, W, H, px);
     Hit h;
-    trace_block<false, PLANE>(sc, r, px.valid, -1, h, stk, lane_stack);
+    trace_block<false, PLANE>(sc, r, px.valid, -1, h, stk, stk_stride, lane_stack);
     const bool hit = px.valid && h.prim >= 0;
     if (hit) refine_hit(sc.tris, r, h);
 
@@ -318,7 +316,7 @@ __global__ void __launch_bounds__(256) k_render(SceneArgs sc, rt_camera cam, rt_
     sr.tmin = p.shadow_tmin;
     sr.tmax = INFINITY;
     Hit sh;
-    trace_block<true, SLANE>(sc, sr, want_shadow, hit ? h.slot : -1, sh, stk, lane_stack);
+    trace_block<true, SLANE>(sc, sr, want_shadow, hit ? h.slot : -1, sh, stk, stk_stride, lane_stack);
     const bool occluded = want_shadow && sh.prim >= 0;
 
     float alb[3] = {1.0f, 1.0f, 1.0f};
@@ -350,6 +348,146 @@ __global__ void __launch_bounds__(256) k_render(SceneArgs sc, rt_camera cam, rt_
             if (n_hit) atomicAdd(&stats[0], n_hit);
             if (n_sh) atomicAdd(&stats[1], n_sh);
             if (n_oc) atomicAdd(&stats[2], n_oc);
+        }
+    }
+}
+
+// Fused render of one frame (see the file comment), one 16x16-pixel block per workgroup.
+template <bool PLANE, bool SLANE>
+__global__ void __launch_bounds__(256) k_render(SceneArgs sc, rt_camera cam, rt_render_params p,
+                                                 TileMap tm, int lane_stack, float* __restrict__ rgb,
+                                                 float* __restrict__ out_t,
+                                                 int32_t* __restrict__ out_prim,
+                                                 uint32_t* __restrict__ stats) {
+    __shared__ int32_t stk[(PLANE || SLANE) ? kLaneStack * 256 : 1];
+    uint32_t tx, ty;
+    if (!block_tile(tm, tx, ty)) return;
+    const TilePixel px = tile_pixel(p.width, p.height, tx, ty);
+    render_pixel<PLANE, SLANE>(sc, cam, p, px, rgb, out_t, out_prim, stats, stk + threadIdx.x, 256,
+                               lane_stack);
+}
+
+// ---- batched render ----
+// Workspace: [0] uint32 work counter, [1] status, then (n_jobs + 1) uint32 tile prefix at word 64.
+constexpr uint32_t kBatchPrefixWord = 64;
+
+// 8x8 tiles of a job, or 0 with a status bit if the job cannot be rendered: 1 = more than
+// RT_BATCH_MAX_JOB_PIXELS pixels, 2 = a required pointer is NULL (rgb output, texture table with
+// n_textures > 0, scene arrays of a non-empty scene).
+RT_DEV uint32_t job_tiles(const rt_render_job& j, uint32_t& status) {
+    const rt_render_params& p = j.params;
+    const uint64_t px = (uint64_t)p.width * p.height;
+    if (px == 0) return 0;
+    if (px > RT_BATCH_MAX_JOB_PIXELS) {
+        status |= 1u;
+        return 0;
+    }
+    const rt_scene& s = j.scene;
+    if (!j.out.rgb || (p.n_textures && !p.textures_dev) ||
+        (s.root != RT_EMPTY_ROOT && (!s.nodes4 || !s.tris || !s.shade))) {
+        status |= 2u;
+        return 0;
+    }
+    return ((p.width + 7u) / 8u) * ((p.height + 7u) / 8u);
+}
+
+// One block of 1024 threads: per-job tile counts, exclusive prefix (n_jobs <= 4096, at most 4 jobs
+// per thread), work counter and status cleared, every job's stats zeroed.
+__global__ void __launch_bounds__(1024) k_batch_prepare(const rt_render_job* __restrict__ jobs,
+                                                         uint32_t n_jobs, uint32_t* __restrict__ ws) {
+    __shared__ uint32_t part[1024];
+    __shared__ uint32_t status;
+    const uint32_t t = threadIdx.x;
+    if (t == 0) status = 0;
+    __syncthreads();
+    uint32_t cnt[4], sum = 0, bad = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const uint32_t j = t * 4 + k;
+        cnt[k] = 0;
+        if (j < n_jobs) {
+            cnt[k] = job_tiles(jobs[j], bad);
+            uint32_t* st = jobs[j].out.stats;
+            if (st) {
+                st[0] = 0;
+                st[1] = 0;
+                st[2] = 0;
+                st[3] = 0;
+            }
+        }
+        sum += cnt[k];
+    }
+    part[t] = sum;
+    __syncthreads();
+    // Hillis-Steele inclusive scan over the 1024 partial sums
+    for (uint32_t off = 1; off < 1024; off <<= 1) {
+        const uint32_t v = t >= off ? part[t - off] : 0u;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    uint32_t run = part[t] - sum;
+    uint32_t* pref = ws + kBatchPrefixWord;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const uint32_t j = t * 4 + k;
+        if (j < n_jobs) pref[j] = run;
+        run += cnt[k];
+    }
+    if (t == 1023) pref[n_jobs] = part[1023];
+    if (bad) atomicOr(&status, bad);
+    __syncthreads();
+    if (t == 0) {
+        ws[0] = 0;
+        ws[1] = status;
+    }
+}
+
+// Persistent batched render: one wave per block; each wave pops `grab` consecutive 8x8 tiles of the
+// concatenated job list from the work counter until the list is exhausted (every wave reaches the
+// exit: the counter only grows). Tile -> job by binary search of the prefix, copied into LDS.
+template <bool SLANE>
+__global__ void __launch_bounds__(64) k_render_batch(const rt_render_job* __restrict__ jobs,
+                                                      uint32_t n_jobs, uint32_t* __restrict__ ws,
+                                                      uint32_t grab, int lane_stack) {
+    extern __shared__ uint32_t smem[];
+    uint32_t* pref = smem;
+    int32_t* stk = (int32_t*)(smem + ((n_jobs + 1 + 3) & ~3u));
+    const uint32_t lane = threadIdx.x;
+    for (uint32_t j = lane; j <= n_jobs; j += 64) pref[j] = ws[kBatchPrefixWord + j];
+    __syncthreads();
+    const uint32_t total = pref[n_jobs];
+    for (;;) {
+        uint32_t u0 = 0;
+        if (lane == 0) u0 = atomicAdd(&ws[0], grab);
+        u0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)u0);
+        if (u0 >= total) break;
+        const uint32_t u1 = u0 + grab < total ? u0 + grab : total;
+        for (uint32_t u = u0; u < u1; ++u) {
+            // largest j with pref[j] <= u (jobs with no tiles share a prefix value: take the last)
+            uint32_t lo = 0, hi = n_jobs;  // answer in [lo, hi)
+            while (hi - lo > 1) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (pref[mid] <= u) lo = mid; else hi = mid;
+            }
+            const uint32_t j = (uint32_t)__builtin_amdgcn_readfirstlane((int)lo);
+            const rt_render_job& job = jobs[j];
+            const rt_render_params p = job.params;
+            const uint32_t tiles_x = (p.width + 7u) / 8u, local = u - pref[j];
+            TilePixel px;
+            px.col = (local % tiles_x) * 8u + (lane & 7u);
+            px.row = (local / tiles_x) * 8u + (lane >> 3);
+            px.valid = px.col < p.width && px.row < p.height;
+            SceneArgs sc;
+            sc.nodes = (const float4*)job.scene.nodes;
+            sc.nodes4 = (const float4*)job.scene.nodes4;
+            sc.tris = (const float4*)job.scene.tris;
+            sc.shade = (const float4*)job.scene.shade;
+            sc.mesh_offsets = job.scene.mesh_offsets;
+            sc.root = job.scene.root;
+            sc.n_meshes = job.scene.n_meshes;
+            render_pixel<false, SLANE>(sc, job.cam, p, px, job.out.rgb, job.out.t, job.out.prim,
+                                       job.out.stats, stk + lane, 64, lane_stack);
         }
     }
 }
@@ -460,9 +598,10 @@ extern "C" int rt_render(const rt_scene* scene, const rt_camera* cam, const rt_r
     (void)ws_bytes;
     int e = check_scene(scene, "rt_render");
     if (e) return e;
-    if (!cam || !p || !out || !out->rgb) return set_error(RT_EINVAL, "rt_render: NULL argument");
+    if (!cam || !p || !out) return set_error(RT_EINVAL, "rt_render: NULL argument");
     const uint64_t n = (uint64_t)p->width * p->height;
     if (n == 0) return RT_OK;
+    if (!out->rgb) return set_error(RT_EINVAL, "rt_render: rgb output is NULL");
     if (n > 0x7FFFFFFFull) return set_error(RT_ECAPACITY, "rt_render: image too large");
     if (p->n_textures && !p->textures_dev)
         return set_error(RT_EINVAL, "rt_render: n_textures > 0 but textures_dev is NULL");
@@ -476,4 +615,40 @@ extern "C" int rt_render(const rt_scene* scene, const rt_camera* cam, const rt_r
     hipLaunchKernelGGL(kern, dim3(tile_map_blocks(tm)), dim3(256), 0, s, scene_args(scene), *cam, *p,
                        tm, (int)tu.lane_stack, out->rgb, out->t, out->prim, out->stats);
     return check_launch("rt_render");
+}
+
+extern "C" uint64_t rt_render_batch_workspace_bytes(uint32_t n_jobs) {
+    return 4ull * (kBatchPrefixWord + (uint64_t)n_jobs + 1);
+}
+
+extern "C" const uint32_t* rt_render_batch_status(const void* workspace_dev) {
+    return workspace_dev ? (const uint32_t*)workspace_dev + 1 : nullptr;
+}
+
+extern "C" int rt_render_batch(const rt_render_job* jobs_dev, uint32_t n_jobs, const rt_batch_config* cfg,
+                               void* workspace, uint64_t ws_bytes, void* stream) {
+    if (!jobs_dev || !cfg) return set_error(RT_EINVAL, "rt_render_batch: NULL argument");
+    if (n_jobs == 0) return RT_OK;
+    if (n_jobs > RT_BATCH_MAX_JOBS)
+        return set_error(RT_ECAPACITY, "rt_render_batch: more than RT_BATCH_MAX_JOBS jobs");
+    if (cfg->waves_per_cu < 1 || cfg->waves_per_cu > 32 || cfg->tiles_per_grab < 1 ||
+        cfg->tiles_per_grab > 64 || cfg->lane_stack < 1 || cfg->lane_stack > RT_LANE_STACK_MAX ||
+        (cfg->shadow != RT_TRAVERSAL_PACKET && cfg->shadow != RT_TRAVERSAL_LANE))
+        return set_error(RT_EINVAL, "rt_render_batch: rt_batch_config field out of range");
+    if (!workspace || ws_bytes < rt_render_batch_workspace_bytes(n_jobs))
+        return set_error(RT_EINVAL, "rt_render_batch: workspace too small");
+    if ((uintptr_t)workspace & 15) return set_error(RT_EINVAL, "rt_render_batch: workspace must be 16-byte aligned");
+    hipStream_t s = (hipStream_t)stream;
+    uint32_t* ws = (uint32_t*)workspace;
+    hipLaunchKernelGGL(k_batch_prepare, dim3(1), dim3(1024), 0, s, jobs_dev, n_jobs, ws);
+    const bool sl = cfg->shadow == RT_TRAVERSAL_LANE;
+    const size_t lds = 4u * ((n_jobs + 1 + 3) & ~3u) + (sl ? 4u * 64u * kLaneStack : 0u);
+    const uint32_t grid = (uint32_t)device_cu_count() * cfg->waves_per_cu;
+    if (sl)
+        hipLaunchKernelGGL(k_render_batch<true>, dim3(grid), dim3(64), lds, s, jobs_dev, n_jobs, ws,
+                           cfg->tiles_per_grab, (int)cfg->lane_stack);
+    else
+        hipLaunchKernelGGL(k_render_batch<false>, dim3(grid), dim3(64), lds, s, jobs_dev, n_jobs, ws,
+                           cfg->tiles_per_grab, (int)cfg->lane_stack);
+    return check_launch("rt_render_batch");
 }

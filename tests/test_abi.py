@@ -75,6 +75,8 @@ int main(void) {
     O(rt_render_params, mesh_texture_dev) O(rt_render_params, n_textures)
     S(rt_render_out) O(rt_render_out, stats)
     S(rt_tuning) O(rt_tuning, shadow) O(rt_tuning, super_tile) O(rt_tuning, lane_stack)
+    S(rt_render_job) O(rt_render_job, cam) O(rt_render_job, params) O(rt_render_job, out)
+    S(rt_batch_config) O(rt_batch_config, tiles_per_grab) O(rt_batch_config, shadow) O(rt_batch_config, lane_stack)
     return 0;
 }
 """
@@ -94,7 +96,8 @@ def test_ctypes_layouts_match_the_c_header(tmp_path):
         k, v = line.split()
         c[k] = int(v)
     for name in ("rt_mesh_desc", "rt_scene", "rt_scene_sizes", "rt_rays", "rt_hits", "rt_camera",
-                 "rt_texture", "rt_render_params", "rt_render_out", "rt_tuning"):
+                 "rt_texture", "rt_render_params", "rt_render_out", "rt_tuning", "rt_render_job",
+                 "rt_batch_config"):
         cls = getattr(N, name)
         assert ctypes.sizeof(cls) == c[name], name
         for key, off in c.items():
@@ -118,3 +121,24 @@ def test_tuning_get_set_and_range_checks_without_gpu():
     assert ren.get_tuning() == d
     with pytest.raises(TypeError):
         ren.set_tuning(tile=4)
+
+
+def test_batch_entry_rejects_bad_arguments_without_gpu():
+    from rendertoy_amd import _native as N
+    lib = N.load()
+    cfg = N.rt_batch_config(8, 1, N.RT_TRAVERSAL_LANE, 32)
+    ws = ctypes.create_string_buffer(1 << 16)
+    wsp = ctypes.cast(ws, ctypes.c_void_p).value
+    fake_jobs = 0x1000
+    assert lib.rt_render_batch(None, 1, ctypes.byref(cfg), wsp, 1 << 16, None) == -1
+    assert lib.rt_render_batch(fake_jobs, 1, None, wsp, 1 << 16, None) == -1
+    assert lib.rt_render_batch(fake_jobs, N.RT_BATCH_MAX_JOBS + 1, ctypes.byref(cfg), wsp, 1 << 20, None) == -3
+    for bad in (N.rt_batch_config(0, 1, 1, 32), N.rt_batch_config(33, 1, 1, 32),
+                N.rt_batch_config(8, 0, 1, 32), N.rt_batch_config(8, 65, 1, 32),
+                N.rt_batch_config(8, 1, 2, 32), N.rt_batch_config(8, 1, 1, 0)):
+        assert lib.rt_render_batch(fake_jobs, 1, ctypes.byref(bad), wsp, 1 << 16, None) == -1
+    need = lib.rt_render_batch_workspace_bytes(10)
+    assert need == 4 * (64 + 11)
+    assert lib.rt_render_batch(fake_jobs, 10, ctypes.byref(cfg), wsp, need - 1, None) == -1
+    assert b"workspace" in lib.rt_last_error()
+    assert lib.rt_render_batch(fake_jobs, 0, ctypes.byref(cfg), None, 0, None) == 0  # nothing to do
