@@ -50,6 +50,11 @@ def parse():
                     help="skip the batched-vs-plain measurement (256 scenes)")
     ap.add_argument("--batch-scenes", type=int, default=256)
     ap.add_argument("--batch-reps", type=int, default=5)
+    ap.add_argument("--no-serving", action="store_true",
+                    help="skip the serving measurement (64 resident scenes, Poisson arrivals)")
+    ap.add_argument("--serving-requests", type=int, default=300, help="requests per arrival rate")
+    ap.add_argument("--serving-rates", default="100,500,2000",
+                    help="comma-separated arrival rates (requests/s)")
     ap.add_argument("--copy-calib-mib", type=int, default=0,
                     help="first run 3 elementwise int32 adds over this many MiB (a kernel of "
                          "known read/write traffic that scales the FETCH_SIZE/WRITE_SIZE "
@@ -146,6 +151,57 @@ def measure_batched(args, ren, scenes, torch):
         "autotune_ms": {f"w{c.waves_per_cu}g{c.tiles_per_grab}{c.shadow[0]}": round(ms, 3) for c, ms in table},
         "setup_s": round(gen_s, 1),
     }
+
+
+def serving_pool(n_scenes, scenes):
+    """The serving pool: scene k has T = exp(U[ln 1e4, ln 2e6]) triangles (icosphere frequency
+    round(sqrt(T / 20))) and a 512^2 RGBA8 value-noise texture, from the seeds 4000 + k / 4100 + k.
+    Host data is generated when the scene is first made resident."""
+    from rendertoy_amd.serving import SceneSpec
+
+    def make(k):
+        rng = np.random.default_rng(4000 + k)
+        T = math.exp(rng.uniform(math.log(1e4), math.log(2e6)))
+        f = max(1, int(round(math.sqrt(T / 20.0))))
+        return SceneSpec([scenes.icosphere(f, 4000 + k)], scenes.value_noise_rgba8(512, 4100 + k))
+    return {k: (lambda k=k: make(k)) for k in range(n_scenes)}
+
+
+def measure_serving(args, ren, scenes, torch):
+    """Requests (scene uniform over 64 resident scenes, eye 3d with d uniform on the sphere, image
+    256^2 / 512^2 / 1024^2) arriving as a Poisson process at each rate, served by the batching
+    loop; latency = GPU completion of the request's batch - arrival."""
+    from rendertoy_amd import serving
+    from rendertoy_amd.batch import time_launches
+    g0 = time.time()
+    store = serving.SceneStore(serving_pool(64, scenes), budget_bytes=200 << 30)
+    for k in range(64):
+        store.release(store.acquire(k))
+    torch.cuda.synchronize()
+    setup_s = time.time() - g0
+    tris = sum(r.raycaster.n_tris for r in store.resident.values())
+    # one request alone (no queueing) per image size, scene 0
+    res = store.acquire(0)
+    alone = {}
+    for size in (256, 512, 1024):
+        cam = serving.camera_for((0.0, 0.0, 3.0))
+        ms = time_launches(lambda: res.raycaster.render(cam, size, size, textures=res.binding), 5)
+        alone[str(size)] = round(float(np.median(ms)), 3)
+    store.release(res)
+    srv = serving.RenderServer(store)
+    rates = {}
+    for rate in [float(x) for x in args.serving_rates.split(",") if x]:
+        reqs = serving.make_requests(args.serving_requests, 64, rate, 4000 + int(rate))
+        srv.batches = []
+        srv.run(reqs)
+        sm = serving.latency_summary(reqs)
+        sm["mean_batch"] = round(float(np.mean(srv.batches)), 2)
+        rates[str(int(rate))] = sm
+    return {"what": "64 resident scenes, 512^2 RGBA8 textures, Poisson arrivals, images 256^2..1024^2",
+            "triangles_resident": int(tris), "resident_bytes": int(store.bytes),
+            "alone_render_ms": alone, "rates_rps": rates,
+            "store": {"hits": store.hits, "misses": store.misses, "evictions": store.evictions},
+            "setup_s": round(setup_s, 1)}
 
 
 def percentile(a, q):
@@ -273,6 +329,8 @@ def main():
 
     if not args.no_batched:
         result["batched"] = measure_batched(args, ren, scenes, torch)
+    if not args.no_serving:
+        result["serving"] = measure_serving(args, ren, scenes, torch)
 
     # ---- roofline and CPU baseline from the CPU restatement (oracle) ----
     if not args.no_roofline or not args.no_cpu_baseline:

@@ -103,6 +103,25 @@ def _check_device(t, what):
         raise RuntimeError(f"{what} must live on the HIP device (got {dev})")
 
 
+class TextureBinding:
+    """Textures and the mesh -> texture map of a scene uploaded once (device table of rt_texture
+    and int32 map), for callers that render the same scene many times; pass it as `textures`."""
+
+    def __init__(self, textures, mesh_textures):
+        dev = torch.device("cuda")
+        arr = (N.rt_texture * max(1, len(textures)))()
+        for i, tx in enumerate(textures):
+            _check_device(tx, "texture")
+            if tx.dtype != torch.uint8 or tx.dim() != 3 or tx.shape[2] != 4 or not tx.is_contiguous():
+                raise TypeError("texture tensors must be contiguous uint8 [H, W, 4]")
+            arr[i].texels = tx.data_ptr()
+            arr[i].width, arr[i].height = tx.shape[1], tx.shape[0]
+        self.textures = list(textures)
+        self.n_textures = len(textures)
+        self.table = torch.frombuffer(bytearray(bytes(arr)), dtype=torch.uint8).to(dev)
+        self.mesh_map = torch.tensor(list(mesh_textures), dtype=torch.int32).to(dev)
+
+
 class Raycaster:
     """Closest-hit ray caster over a list of meshes (rendering/_raycaster.py:25-36)."""
 
@@ -179,6 +198,19 @@ class Raycaster:
         self._scene = sc
         self._keep = keep
         del ws
+
+    def device_bytes(self) -> int:
+        """Device memory held by the built scene (BVH, triangle and shading records, keys)."""
+        ts = [self._nodes, self._nodes4, self._tris, self._shade, self._bounds, self._mesh_offsets,
+              self.keys, self.order]
+        return int(sum(t.numel() * t.element_size() for t in ts))
+
+    def release_inputs(self):
+        """Drops the references to the input meshes' device buffers; queries and renders only read
+        the built scene, so a resident scene need not keep its vertices."""
+        self.n_models = len(self.models)
+        self.models = [None] * self.n_models
+        self._keep = []
 
     def _torch_stream(self):
         return self._stream if self._stream is not None else torch.cuda.current_stream()
@@ -341,6 +373,12 @@ class Raycaster:
         p.background[:] = [float(v) for v in background]
         p.shadow_tmin = float(shadow_tmin)
         tex_keep = []
+        if isinstance(textures, TextureBinding):
+            p.textures_dev = textures.table.data_ptr()
+            p.n_textures = textures.n_textures
+            p.mesh_texture_dev = textures.mesh_map.data_ptr()
+            tex_keep.append(textures)
+            textures, mesh_textures = None, None
         if textures:
             arr = (N.rt_texture * len(textures))()
             for i, tx in enumerate(textures):

@@ -101,7 +101,6 @@ class BatchRender:
         wsb = lib.rt_render_batch_workspace_bytes(n)
         self.workspace = torch.zeros(wsb, dtype=torch.uint8, device=dev)
         self._graph = None
-        torch.cuda.current_stream().synchronize()
 
     def launch(self, config: BatchConfig = None):
         """Renders every job (asynchronous on the stream)."""
@@ -127,7 +126,8 @@ class BatchRender:
         self._graph.replay()
 
     def status(self) -> int:
-        """0 if every job was rendered, bit 1 if a job exceeded the pixel limit (synchronises)."""
+        """0 if every job was rendered; bit 1: a job exceeded RT_BATCH_MAX_JOB_PIXELS, bit 2: a job
+        lacked a required buffer (synchronises)."""
         return int(self.workspace[4:8].view(torch.int32).item())
 
 
@@ -145,8 +145,8 @@ def time_launches(fn, reps, warmup=1):
     return [ev[k].elapsed_time(ev[k + 1]) for k in range(reps)]
 
 
-DEFAULT_CANDIDATES = [BatchConfig(w, g, s) for s in ("lane", "packet") for w in (4, 8, 16)
-                      for g in (1, 4)]
+DEFAULT_CANDIDATES = [BatchConfig(w, g, s) for s in ("lane", "packet") for w in (8, 16, 32)
+                      for g in (1, 4, 16)]
 
 
 def autotune(batch: BatchRender, candidates=None, reps=3):

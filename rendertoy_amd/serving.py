@@ -1,0 +1,259 @@
+"""Render serving: resident scenes under a device-memory budget (LRU) and a request loop that
+batches every request admitted since the previous launch into one rt_render_batch launch.
+
+The reference has no serving layer (it renders one frame per call from a tutorial script); this
+module is the caller the batched kernel is for. Scheduling is continuous batching at launch
+granularity: while one batch runs on the GPU the loop admits arrivals, makes their scenes
+resident (building the BVH on a miss, evicting least-recently-used scenes that no queued batch
+uses) and enqueues the next batch behind it on the same stream, so the GPU goes from one batch
+to the next without waiting for the host. A request's latency is the GPU completion time of its
+batch (HIP event) minus its arrival time.
+"""
+import collections
+import dataclasses
+import math
+import time
+import typing
+
+import numpy as np
+import torch
+
+from . import _native as N
+from .batch import BatchConfig, BatchRender, RenderRequest
+
+
+@dataclasses.dataclass
+class SceneSpec:
+    """Host-side description of a scene: meshes as dicts of arrays (pos, idx, nrm, uv) and an
+    optional RGBA8 texture [H, W, 4] applied to every mesh."""
+    meshes: list
+    texture: typing.Any = None
+
+    def tris(self) -> int:
+        return int(sum(len(m["idx"]) for m in self.meshes))
+
+
+class Resident:
+    """A scene held on the device: its Raycaster, texture and their device bytes."""
+
+    def __init__(self, raycaster, texture, nbytes):
+        self.raycaster = raycaster
+        self.texture = texture
+        self.binding = None
+        if texture is not None:
+            from ._raycaster import TextureBinding
+            self.binding = TextureBinding([texture], [0] * raycaster.n_models)
+        self.nbytes = nbytes
+        self.pins = 0
+
+
+def scene_bytes_estimate(n_tris: int, texture=None):
+    """(resident, transient) device bytes of a scene of n_tris triangles: its arrays and texture,
+    and the build workspace plus input vertices it needs while it is built."""
+    lib = N.load()
+    import ctypes
+    s = N.rt_scene_sizes()
+    N.check(lib.rt_scene_sizes_for(max(1, n_tris), ctypes.byref(s)), "rt_scene_sizes_for")
+    resident = s.nodes_bytes + s.nodes4_bytes + s.tris_bytes + s.shade_bytes + 8 * n_tris + 64
+    tex = 0 if texture is None else int(np.asarray(texture).nbytes)
+    return int(resident + tex), int(s.workspace_bytes + 3 * 80 * n_tris)
+
+
+def build_resident(spec: SceneSpec) -> Resident:
+    """Uploads a scene's meshes and texture and builds its BVH on the device."""
+    from ._modeling import mesh_from_arrays
+    from ._raycaster import Raycaster
+    models = [mesh_from_arrays(m["pos"], m["idx"], m.get("nrm"), m.get("uv")) for m in spec.meshes]
+    rc = Raycaster(models)
+    rc.release_inputs()
+    tex = None
+    if spec.texture is not None:
+        tex = torch.from_numpy(np.ascontiguousarray(spec.texture, np.uint8)).cuda()
+    return Resident(rc, tex, rc.device_bytes() + (0 if tex is None else tex.numel()))
+
+
+class SceneStore:
+    """Resident scenes keyed by id, LRU-evicted to keep their device bytes within `budget_bytes`.
+    `specs` maps id -> SceneSpec (or a callable returning one). Scenes pinned by an in-flight
+    batch are never evicted."""
+
+    def __init__(self, specs, budget_bytes: int, build=None, estimate=None):
+        self.specs = specs
+        self.budget = int(budget_bytes)
+        self.resident = collections.OrderedDict()
+        self.bytes = 0
+        self.hits = self.misses = self.evictions = 0
+        self._build = build or build_resident
+        self._estimate = estimate or (lambda spec: scene_bytes_estimate(spec.tris(), spec.texture)[0])
+
+    def _spec(self, sid):
+        s = self.specs[sid]
+        return s() if callable(s) else s
+
+    def _evict_for(self, need):
+        for sid in list(self.resident.keys()):
+            if self.bytes + need <= self.budget:
+                break
+            r = self.resident[sid]
+            if r.pins:
+                continue
+            del self.resident[sid]
+            self.bytes -= r.nbytes
+            self.evictions += 1
+        if self.bytes + need > self.budget:
+            raise MemoryError(f"scene of {need} bytes does not fit the budget {self.budget} "
+                              f"({self.bytes} bytes pinned by queued batches)")
+
+    def acquire(self, sid) -> Resident:
+        """The resident scene `sid` (built on a miss), marked most recently used and pinned."""
+        r = self.resident.get(sid)
+        if r is not None:
+            self.resident.move_to_end(sid)
+            self.hits += 1
+        else:
+            spec = self._spec(sid)
+            self._evict_for(self._estimate(spec))
+            r = self._build(spec)
+            self.resident[sid] = r
+            self.bytes += r.nbytes
+            self.misses += 1
+        r.pins += 1
+        return r
+
+    def release(self, r: Resident):
+        r.pins -= 1
+
+
+@dataclasses.dataclass
+class Request:
+    """One render request: scene id, camera eye (looking at the origin), square image size,
+    arrival time in seconds from the start of the run."""
+    scene: typing.Any
+    eye: tuple
+    size: int
+    arrival: float = 0.0
+    keep_image: bool = False
+    # filled in by the server
+    latency: float = math.nan
+    batch: int = -1
+    image: typing.Any = None
+
+
+def camera_for(eye):
+    """Camera at `eye` looking at the origin, 45 deg, up +y (+z when the view is near-vertical)."""
+    from ._raycaster import Camera
+    e = np.asarray(eye, np.float64)
+    d = -e / np.linalg.norm(e)
+    up = (0.0, 0.0, 1.0) if abs(d[1]) > 0.99 else (0.0, 1.0, 0.0)
+    return Camera(tuple(float(x) for x in e), (0.0, 0.0, 0.0), up, math.pi / 4, 1.0)
+
+
+class RenderServer:
+    """Serves Requests in arrival order (see the module docstring)."""
+
+    def __init__(self, store: SceneStore, config: BatchConfig = None, max_batch: int = 512,
+                 max_inflight: int = 2):
+        self.store = store
+        self.config = config or BatchConfig(waves_per_cu=16, tiles_per_grab=4, shadow="lane")
+        self.max_batch = max_batch
+        self.max_inflight = max_inflight
+        self.batches = []
+
+    def _launch(self, reqs, t0_event):
+        """Launches the longest prefix of `reqs` whose scenes fit the budget next to the pinned
+        ones; returns (record or None, requests left over)."""
+        residents = []
+        for k, r in enumerate(reqs):
+            try:
+                residents.append(self.store.acquire(r.scene))
+            except MemoryError:
+                if not residents:
+                    return None, reqs
+                reqs, rest = reqs[:k], reqs[k:]
+                break
+        else:
+            rest = []
+        rr = [RenderRequest(res.raycaster, camera_for(r.eye), r.size, r.size, textures=res.binding)
+              for r, res in zip(reqs, residents)]
+        b = BatchRender(rr, config=self.config, stats=False)
+        b.launch()
+        ev = torch.cuda.Event(enable_timing=True)
+        ev.record()
+        return {"reqs": reqs, "batch": b, "event": ev, "residents": residents, "t0": t0_event}, rest
+
+    def _finish(self, f):
+        done_s = f["t0"].elapsed_time(f["event"]) * 1e-3
+        k = len(self.batches)
+        for i, r in enumerate(f["reqs"]):
+            r.latency = done_s - r.arrival
+            r.batch = k
+            if r.keep_image:
+                r.image = f["batch"].images[i]
+        for res in f["residents"]:
+            self.store.release(res)
+        self.batches.append(len(f["reqs"]))
+
+    def run(self, requests: typing.List[Request], realtime: bool = True):
+        """Serves `requests` (sorted by arrival). realtime: arrivals are released at their wall-clock
+        times from the start of the run; otherwise all are present at t = 0."""
+        pending = collections.deque(sorted(requests, key=lambda r: r.arrival))
+        queue = collections.deque()
+        inflight = collections.deque()
+        torch.cuda.synchronize()
+        t0_event = torch.cuda.Event(enable_timing=True)
+        t0_event.record()
+        t0 = time.perf_counter()
+        while pending or queue or inflight:
+            now = time.perf_counter() - t0
+            while pending and (not realtime or pending[0].arrival <= now):
+                queue.append(pending.popleft())
+            while inflight and inflight[0]["event"].query():
+                self._finish(inflight.popleft())
+            if queue and len(inflight) < self.max_inflight:
+                n = min(len(queue), self.max_batch)
+                rec, rest = self._launch([queue.popleft() for _ in range(n)], t0_event)
+                queue.extendleft(reversed(rest))
+                if rec is not None:
+                    inflight.append(rec)
+                    continue
+                if not inflight:  # a single scene larger than the whole budget
+                    raise MemoryError(f"scene {rest[0].scene} does not fit the budget")
+                inflight[0]["event"].synchronize()  # the oldest batch unpins its scenes
+                self._finish(inflight.popleft())
+                continue
+            if inflight and (len(inflight) >= self.max_inflight or not pending):
+                inflight[0]["event"].synchronize()  # no slot, or nothing left to admit
+                continue
+            if pending:  # a slot is free: wait for the next arrival
+                dt = pending[0].arrival - (time.perf_counter() - t0)
+                if dt > 0:
+                    time.sleep(min(dt, 2e-4 if inflight else 1e-3))
+        return requests
+
+
+def poisson_arrivals(rate: float, n: int, seed: int) -> np.ndarray:
+    """Arrival times (s) of a Poisson process of `rate` requests/s: cumulative exponential gaps."""
+    rng = np.random.default_rng(seed)
+    return np.cumsum(rng.exponential(1.0 / rate, n))
+
+
+def make_requests(n: int, n_scenes: int, rate: float, seed: int, sizes=(256, 512, 1024)):
+    """The serving workload: scene uniform over the pool, eye = 3 d with d uniform on the unit
+    sphere (looking at the origin), image side uniform over `sizes`, Poisson arrivals."""
+    rng = np.random.default_rng(seed)
+    arr = poisson_arrivals(rate, n, seed + 1)
+    out = []
+    for i in range(n):
+        d = rng.standard_normal(3)
+        d /= np.linalg.norm(d)
+        out.append(Request(int(rng.integers(n_scenes)), tuple(3.0 * d), int(rng.choice(sizes)),
+                           float(arr[i])))
+    return out
+
+
+def latency_summary(reqs):
+    lat = np.array([r.latency for r in reqs]) * 1e3
+    span = max(r.arrival + r.latency for r in reqs) - min(r.arrival for r in reqs)
+    return {"requests": len(reqs), "p50_ms": round(float(np.percentile(lat, 50)), 3),
+            "p99_ms": round(float(np.percentile(lat, 99)), 3),
+            "max_ms": round(float(lat.max()), 3), "throughput_rps": round(len(reqs) / span, 1)}
