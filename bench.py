@@ -147,8 +147,8 @@ def measure_batched(args, ren, scenes, torch):
         "speedup": round(p_med / b_med, 3),
         "batched_equals_plain": bool(same),
         "autotuned": {"waves_per_cu": best.waves_per_cu, "tiles_per_grab": best.tiles_per_grab,
-                      "shadow": best.shadow},
-        "autotune_ms": {f"w{c.waves_per_cu}g{c.tiles_per_grab}{c.shadow[0]}": round(ms, 3) for c, ms in table},
+                      "primary": best.primary, "shadow": best.shadow},
+        "autotune_ms": {c.key(): round(ms, 3) for c, ms in table},
         "setup_s": round(gen_s, 1),
     }
 

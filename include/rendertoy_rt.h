@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define RT_ABI_VERSION 4
+#define RT_ABI_VERSION 6
 
 enum {
     RT_OK = 0,
@@ -148,19 +148,25 @@ typedef struct rt_render_out {
  * hit does not depend on the visiting order); they differ in speed only. */
 enum {
     RT_TRAVERSAL_PACKET = 0, /* a wave's 64 rays walk one node sequence (scalar node fetches) */
-    RT_TRAVERSAL_LANE = 1    /* each lane walks its own ray (per-lane stack in LDS) */
+    RT_TRAVERSAL_LANE = 1,   /* each lane walks its own ray (per-lane stack in LDS) */
+    RT_TRAVERSAL_HYBRID = 2, /* packet for at most packet_budget steps, then per lane from the root
+                                bounded by the best hit so far (for rays that diverge) */
+    RT_TRAVERSAL_AUTO = 3    /* camera rays only: PACKET when an 8x8 pixel tile spans at most 0.009
+                                of the image plane at unit distance (16 tan(fov/2) / height, e.g.
+                                45 deg at >= 768 rows), else LANE; decided per image */
 };
 #define RT_LANE_STACK_MAX 32
 
 /* Launch choices of rt_trace_camera / rt_render (process-wide, read at every launch; no
  * counterpart in the reference, whose kernels take a fixed 32-thread group, _core.py:286). */
 typedef struct rt_tuning {
-    int32_t primary;          /* RT_TRAVERSAL_* for camera rays (default PACKET) */
+    int32_t primary;          /* RT_TRAVERSAL_* for camera rays (default AUTO) */
     int32_t shadow;           /* RT_TRAVERSAL_* for rt_render's shadow rays (default LANE) */
     uint32_t super_tile;      /* blocks are mapped to XCDs in super-tiles of this many 16x16-pixel
                                  blocks per side; 0 = row-major block order (default) */
     uint32_t lane_stack;      /* per-lane stack entries, 1..RT_LANE_STACK_MAX (default 32); a ray
                                  needing more is traced again by its wave with the packet traversal */
+    uint32_t packet_budget;   /* RT_TRAVERSAL_HYBRID: packet steps before the per-lane walk (256) */
 } rt_tuning;
 
 /* ---- library ---- */
@@ -227,8 +233,10 @@ typedef struct rt_render_job {
 typedef struct rt_batch_config {
     uint32_t waves_per_cu;    /* persistent waves per compute unit, 1..32 */
     uint32_t tiles_per_grab;  /* consecutive 8x8 pixel tiles a wave takes per queue pop, 1..64 */
-    int32_t shadow;           /* RT_TRAVERSAL_* for the shadow rays (camera rays: packet) */
-    uint32_t lane_stack;      /* per-lane stack entries for RT_TRAVERSAL_LANE, 1..RT_LANE_STACK_MAX */
+    int32_t primary;          /* RT_TRAVERSAL_* for the camera rays (AUTO: decided per job) */
+    int32_t shadow;           /* RT_TRAVERSAL_* for the shadow rays */
+    uint32_t lane_stack;      /* per-lane stack entries, 1..RT_LANE_STACK_MAX */
+    uint32_t packet_budget;   /* RT_TRAVERSAL_HYBRID packet steps, >= 1 */
 } rt_batch_config;
 
 #define RT_BATCH_MAX_JOBS 4096u

@@ -12,13 +12,15 @@ import torch  # noqa: F401  (must be imported before the HIP library; see module
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "librtx_hip.so")
 
-RT_ABI_VERSION = 4
+RT_ABI_VERSION = 6
 RT_ABSENT = 0xFFFFFFFF
 RT_EMPTY_ROOT = 0x7FFFFFFF
 RT_RENDER_SHADOWS = 1
 RT_RENDER_BILINEAR = 2
 RT_TRAVERSAL_PACKET = 0
 RT_TRAVERSAL_LANE = 1
+RT_TRAVERSAL_HYBRID = 2
+RT_TRAVERSAL_AUTO = 3
 RT_LANE_STACK_MAX = 32
 RT_BATCH_MAX_JOBS = 4096
 RT_BATCH_MAX_JOB_PIXELS = 1 << 24
@@ -133,6 +135,7 @@ class rt_tuning(ctypes.Structure):
         ("shadow", ctypes.c_int32),
         ("super_tile", ctypes.c_uint32),
         ("lane_stack", ctypes.c_uint32),
+        ("packet_budget", ctypes.c_uint32),
     ]
 
 
@@ -149,8 +152,10 @@ class rt_batch_config(ctypes.Structure):
     _fields_ = [
         ("waves_per_cu", ctypes.c_uint32),
         ("tiles_per_grab", ctypes.c_uint32),
+        ("primary", ctypes.c_int32),
         ("shadow", ctypes.c_int32),
         ("lane_stack", ctypes.c_uint32),
+        ("packet_budget", ctypes.c_uint32),
     ]
 
 

@@ -11,7 +11,7 @@ Output is bit-identical to rendering each job with Raycaster.render.
   b = BatchRender(reqs)             # device job table, outputs, workspace
   b.launch()                        # or: b.capture(); b.replay()  (one hipGraph launch)
   b.images[i]                       # [h_i, w_i, 3] float32
-  cfg, table = autotune(b)          # waves per CU x tiles per grab x shadow traversal
+  cfg, table = autotune(b)          # waves per CU x tiles per grab x traversals
 """
 import ctypes
 import dataclasses
@@ -43,21 +43,32 @@ class RenderRequest:
     return_hits: bool = False
 
 
+_ALG = {"packet": N.RT_TRAVERSAL_PACKET, "lane": N.RT_TRAVERSAL_LANE, "hybrid": N.RT_TRAVERSAL_HYBRID,
+        "auto": N.RT_TRAVERSAL_AUTO}
+
+
 @dataclasses.dataclass
 class BatchConfig:
     """Launch shape of rt_render_batch (rt_batch_config)."""
-    waves_per_cu: int = 8
-    tiles_per_grab: int = 1
+    waves_per_cu: int = 16
+    tiles_per_grab: int = 4
     shadow: str = "lane"
+    primary: str = "auto"
     lane_stack: int = N.RT_LANE_STACK_MAX
+    packet_budget: int = 256
 
     def to_c(self):
         c = N.rt_batch_config()
         c.waves_per_cu = int(self.waves_per_cu)
         c.tiles_per_grab = int(self.tiles_per_grab)
-        c.shadow = {"packet": N.RT_TRAVERSAL_PACKET, "lane": N.RT_TRAVERSAL_LANE}[self.shadow]
+        c.primary = _ALG[self.primary]
+        c.shadow = _ALG[self.shadow]
         c.lane_stack = int(self.lane_stack)
+        c.packet_budget = int(self.packet_budget)
         return c
+
+    def key(self):
+        return f"w{self.waves_per_cu}g{self.tiles_per_grab}{self.primary[0]}{self.shadow[0]}"
 
 
 class BatchRender:
@@ -97,7 +108,10 @@ class BatchRender:
             self._keep += keep + [r.raycaster]
             self.rays_primary += r.width * r.height
         self.n_jobs = n
-        self.jobs_dev = torch.frombuffer(bytearray(bytes(jobs)), dtype=torch.uint8).to(dev)
+        # pinned staging + asynchronous copy: a pageable copy would wait for every batch already
+        # queued on the stream, and a serving loop enqueues the next batch behind a running one
+        self._jobs_host = torch.frombuffer(bytearray(bytes(jobs)), dtype=torch.uint8).pin_memory()
+        self.jobs_dev = self._jobs_host.to(dev, non_blocking=True)
         wsb = lib.rt_render_batch_workspace_bytes(n)
         self.workspace = torch.zeros(wsb, dtype=torch.uint8, device=dev)
         self._graph = None
@@ -145,8 +159,8 @@ def time_launches(fn, reps, warmup=1):
     return [ev[k].elapsed_time(ev[k + 1]) for k in range(reps)]
 
 
-DEFAULT_CANDIDATES = [BatchConfig(w, g, s) for s in ("lane", "packet") for w in (8, 16, 32)
-                      for g in (1, 4, 16)]
+DEFAULT_CANDIDATES = [BatchConfig(w, g, s, p) for p in ("auto", "packet", "lane")
+                      for s in ("lane", "packet") for w in (8, 16, 32) for g in (1, 4)]
 
 
 def autotune(batch: BatchRender, candidates=None, reps=3):

@@ -11,7 +11,7 @@ thread_local std::string g_last_error;
 std::mutex g_cu_mutex;
 int g_cu_count[64] = {0};
 std::mutex g_tuning_mutex;
-rt_tuning g_tuning = {RT_TRAVERSAL_PACKET, RT_TRAVERSAL_LANE, 0u, (uint32_t)RT_LANE_STACK_MAX};
+rt_tuning g_tuning = {RT_TRAVERSAL_AUTO, RT_TRAVERSAL_LANE, 0u, (uint32_t)RT_LANE_STACK_MAX, 256u};
 }  // namespace
 
 rt_tuning current_tuning() {
@@ -59,9 +59,11 @@ extern "C" int rt_get_tuning(rt_tuning* out) {
 
 extern "C" int rt_set_tuning(const rt_tuning* t) {
     if (!t) return rt::set_error(RT_EINVAL, "rt_set_tuning: tuning is NULL");
-    auto alg_ok = [](int32_t a) { return a == RT_TRAVERSAL_PACKET || a == RT_TRAVERSAL_LANE; };
-    if (!alg_ok(t->primary) || !alg_ok(t->shadow))
-        return rt::set_error(RT_EINVAL, "rt_set_tuning: traversal must be RT_TRAVERSAL_PACKET or RT_TRAVERSAL_LANE");
+    auto alg_ok = [](int32_t a) { return a >= RT_TRAVERSAL_PACKET && a <= RT_TRAVERSAL_HYBRID; };
+    if (!(alg_ok(t->primary) || t->primary == RT_TRAVERSAL_AUTO) || !alg_ok(t->shadow))
+        return rt::set_error(RT_EINVAL, "rt_set_tuning: traversal must be one of RT_TRAVERSAL_* "
+                                        "(RT_TRAVERSAL_AUTO for camera rays only)");
+    if (t->packet_budget < 1) return rt::set_error(RT_EINVAL, "rt_set_tuning: packet_budget must be >= 1");
     if (t->super_tile > 64) return rt::set_error(RT_EINVAL, "rt_set_tuning: super_tile must be 0..64");
     if (t->lane_stack < 1 || t->lane_stack > RT_LANE_STACK_MAX)
         return rt::set_error(RT_EINVAL, "rt_set_tuning: lane_stack must be 1..RT_LANE_STACK_MAX");

@@ -210,17 +210,22 @@ RT_DEV void traverse(const float4* __restrict__ nodes, const float4* __restrict_
 //    (children that lane misses after those it hits, ties by slot);
 //  - the stack is a pair of VGPRs per field, lane k holding entries k and 64+k (push = per-lane
 //    select, pop = v_readlane): at most 3 pushes on each of at most 31 levels.
-// Must be called by all 64 lanes of the wave with uniform control flow (no lane may have exited).
+// `budget` > 0 caps the node and leaf steps of the wave: the walk stops after that many and returns
+// false (h then holds the best hit found so far, a valid bound for a restart); otherwise it runs to
+// the end and returns true. Must be called by all 64 lanes of the wave with uniform control flow
+// (no lane may have exited).
 template <bool ANY>
-RT_DEV void traverse_packet(const float4* __restrict__ nodes4, const float4* __restrict__ tris,
-                            int32_t root, const Ray& r, bool want, int32_t skip_slot, Hit& h) {
+RT_DEV bool traverse_packet(const float4* __restrict__ nodes4, const float4* __restrict__ tris,
+                            int32_t root, const Ray& r, bool want, int32_t skip_slot, Hit& h,
+                            int budget = 0) {
     h.t = r.tmax;
     h.u = 0.0f;
     h.v = 0.0f;
     h.prim = -1;
     h.slot = -1;
     uint64_t m = __ballot(want);
-    if (root == RT_EMPTY_ROOT || m == 0) return;
+    if (root == RT_EMPTY_ROOT || m == 0) return true;
+    int steps = 0;
     uint64_t alive = m;
     const uint32_t lane = __lane_id();
     const float idx = safe_inv(r.dx), idy = safe_inv(r.dy), idz = safe_inv(r.dz);
@@ -230,6 +235,7 @@ RT_DEV void traverse_packet(const float4* __restrict__ nodes4, const float4* __r
     int32_t sp = 0;
     int32_t cur = root;
     for (;;) {
+        if (budget > 0 && ++steps > budget) return false;
         if (cur >= 0) {
             const float4* nd = nodes4 + (size_t)cur * 8;
             const float4 lx = nd[0], hx = nd[1], ly = nd[2], hy = nd[3], lz = nd[4], hz = nd[5];
@@ -301,11 +307,11 @@ RT_DEV void traverse_packet(const float4* __restrict__ nodes4, const float4* __r
             test_leaf(tris, cur, r, ((m >> lane) & 1u) != 0, skip_slot, h);
             if (ANY) {
                 alive &= ~__ballot(h.prim >= 0);
-                if (alive == 0) return;
+                if (alive == 0) return true;
             }
         }
         for (;;) {
-            if (sp == 0) return;
+            if (sp == 0) return true;
             --sp;
             const bool hi = sp >= 64;
             const int32_t lsp = hi ? sp - 64 : sp;
@@ -331,12 +337,14 @@ constexpr int kLaneStack = RT_LANE_STACK_MAX;
 template <bool ANY>
 RT_DEV bool traverse_lane4(const float4* __restrict__ nodes4, const float4* __restrict__ tris,
                            int32_t root, const Ray& r, int32_t skip_slot, Hit& h, int32_t* stk,
-                           int stride, int limit) {
-    h.t = r.tmax;
-    h.u = 0.0f;
-    h.v = 0.0f;
-    h.prim = -1;
-    h.slot = -1;
+                           int stride, int limit, bool fresh = true) {
+    if (fresh) {  // else h holds a hit of this ray found earlier: it bounds the walk
+        h.t = r.tmax;
+        h.u = 0.0f;
+        h.v = 0.0f;
+        h.prim = -1;
+        h.slot = -1;
+    }
     if (root == RT_EMPTY_ROOT) return true;
     const float idx = safe_inv(r.dx), idy = safe_inv(r.dy), idz = safe_inv(r.dz);
     const float oxx = r.ox * idx, oyy = r.oy * idy, ozz = r.oz * idz;

@@ -181,26 +181,40 @@ RT_DEV Ray camera_ray(const rt_camera& cam, uint32_t W, uint32_t H, const TilePi
     return r;
 }
 
-// Traversal of a block's 256 rays, one per lane. LANE: per-lane 4-wide traversal with the
-// per-thread stacks in `stk` (kLaneStack x 256 words); a ray that overflows its stack is traced
-// again by its wave with the packet traversal (128-entry stack, enough for any tree), which finds
-// the same closest hit. Otherwise the packet traversal throughout. Called by all lanes.
-template <bool ANY, bool LANE>
+// Traversal of one ray per lane by one of three algorithms (MODE = RT_TRAVERSAL_*):
+//  PACKET  the wave's rays walk one node sequence together (traverse_packet);
+//  LANE    each lane walks its own ray with a per-lane stack of `lane_stack` entries in LDS at `stk`
+//          (entries `stk_stride` words apart); a ray that overflows it is traced again by its wave
+//          with the packet traversal (128-entry stack, enough for any tree);
+//  HYBRID  packet traversal for at most `budget` steps; if the wave is not done by then, each lane
+//          continues alone from the root with its best hit so far as the bound (as LANE).
+// All three return the same hit: the closest hit does not depend on the visiting order, and a
+// restart bounded by a hit of the same ray visits every box that could hold a closer one.
+// Called by all 64 lanes of a wave with uniform control flow.
+template <bool ANY, int MODE>
 RT_DEV void trace_block(const SceneArgs& sc, const Ray& r, bool want, int32_t skip_slot, Hit& h,
-                        int32_t* stk, int stk_stride, int lane_stack) {
-    if (!LANE) {
+                        int32_t* stk, int stk_stride, int lane_stack, int budget) {
+    static_assert(MODE >= RT_TRAVERSAL_PACKET && MODE <= RT_TRAVERSAL_HYBRID, "resolved mode");
+    if (MODE == RT_TRAVERSAL_PACKET) {
         traverse_packet<ANY>(sc.nodes4, sc.tris, sc.root, r, want, skip_slot, h);
         return;
     }
+    bool walk = want, fresh = true;
+    if (MODE == RT_TRAVERSAL_HYBRID) {
+        if (traverse_packet<ANY>(sc.nodes4, sc.tris, sc.root, r, want, skip_slot, h, budget)) return;
+        walk = want && !(ANY && h.prim >= 0);  // an occluded shadow ray is done
+        fresh = false;
+    } else {
+        h.t = r.tmax;
+        h.u = 0.0f;
+        h.v = 0.0f;
+        h.prim = -1;
+        h.slot = -1;
+    }
     bool ok = true;
-    h.t = r.tmax;
-    h.u = 0.0f;
-    h.v = 0.0f;
-    h.prim = -1;
-    h.slot = -1;
-    if (want)
+    if (walk)
         ok = traverse_lane4<ANY>(sc.nodes4, sc.tris, sc.root, r, skip_slot, h, stk, stk_stride,
-                                 lane_stack);
+                                 lane_stack, fresh);
     const bool redo = !ok;
     if (__ballot(redo)) {
         Hit h2;
@@ -209,18 +223,46 @@ RT_DEV void trace_block(const SceneArgs& sc, const Ray& r, bool want, int32_t sk
     }
 }
 
+constexpr bool uses_lds(int mode) { return mode != RT_TRAVERSAL_PACKET; }
+
+// RT_TRAVERSAL_AUTO for camera rays: the packet traversal when the 8x8 tile of a wave spans at most
+// kCoherentTile in the image plane at unit distance (8 pixels x 2 tan(fov/2) / height, or the
+// width's counterpart), where its rays share most of their nodes; per-lane beyond.
+constexpr float kCoherentTile = 0.009f;
+
+__host__ __device__ inline bool coherent_tiles(const rt_camera& c, uint32_t W, uint32_t H) {
+    const float up = sqrtf(c.up[0] * c.up[0] + c.up[1] * c.up[1] + c.up[2] * c.up[2]);
+    const float rt = sqrtf(c.right[0] * c.right[0] + c.right[1] * c.right[1] + c.right[2] * c.right[2]);
+    const float fy = H ? 16.0f * up / (float)H : 0.0f, fx = W ? 16.0f * rt / (float)W : 0.0f;
+    return (fx > fy ? fx : fy) <= kCoherentTile;
+}
+
+// Camera-ray traversal with AUTO resolved per wave from `coherent` (uniform).
+template <int PM>
+RT_DEV void trace_primary(const SceneArgs& sc, const Ray& r, bool want, Hit& h, int32_t* stk,
+                          int stk_stride, int lane_stack, int budget, bool coherent) {
+    if constexpr (PM == RT_TRAVERSAL_AUTO) {
+        if (coherent)
+            trace_block<false, RT_TRAVERSAL_PACKET>(sc, r, want, -1, h, stk, stk_stride, lane_stack, budget);
+        else
+            trace_block<false, RT_TRAVERSAL_LANE>(sc, r, want, -1, h, stk, stk_stride, lane_stack, budget);
+    } else {
+        trace_block<false, PM>(sc, r, want, -1, h, stk, stk_stride, lane_stack, budget);
+    }
+}
+
 // Camera closest hit: ray generation fused into the traversal, one hit record per pixel.
-template <bool LANE>
+template <int MODE>
 __global__ void __launch_bounds__(256) k_trace_camera(SceneArgs sc, rt_camera cam, uint32_t W,
                                                        uint32_t H, TileMap tm, int lane_stack,
-                                                       rt_hits hits) {
-    __shared__ int32_t stk[LANE ? kLaneStack * 256 : 1];
+                                                       int budget, rt_hits hits) {
+    __shared__ int32_t stk[uses_lds(MODE) ? kLaneStack * 256 : 1];
     uint32_t tx, ty;
     if (!block_tile(tm, tx, ty)) return;
     const TilePixel p = tile_pixel(W, H, tx, ty);
     const Ray r = camera_ray(cam, W, H, p);
     Hit h;
-    trace_block<false, LANE>(sc, r, p.valid, -1, h, stk + threadIdx.x, 256, lane_stack);
+    trace_block<false, MODE>(sc, r, p.valid, -1, h, stk + threadIdx.x, 256, lane_stack, budget);
     if (h.prim >= 0) refine_hit(sc.tris, r, h);
     if (p.valid) store_hit(sc, hits, (uint64_t)p.row * W + p.col, h);
 }
@@ -271,18 +313,18 @@ RT_DEV void tex_sample(const rt_texture& tx, float u, float v, bool bilinear, fl
 
 // Shading of one pixel per lane: primary closest hit, FP64 refinement, Lambert with interpolated
 // normals, one shadow ray toward the light per lit hit (skipping the hit's own triangle), RGBA8
-// albedo, RGB store; per-wave counts into stats. PLANE / SLANE: per-lane traversal for the primary /
-// shadow rays (else the packet traversal); `stk` = this thread's lane stack (entries `stk_stride`
-// words apart). Called by all 64 lanes of a wave.
-template <bool PLANE, bool SLANE>
+// albedo, RGB store; per-wave counts into stats. PM / SM: traversal (RT_TRAVERSAL_*) of the primary
+// / shadow rays; `stk` = this thread's lane stack (entries `stk_stride` words apart). Called by all
+// 64 lanes of a wave.
+template <int PM, int SM>
 RT_DEV void render_pixel(const SceneArgs& sc, const rt_camera& cam, const rt_render_params& p,
                          const TilePixel& px, float* __restrict__ rgb, float* __restrict__ out_t,
                          int32_t* __restrict__ out_prim, uint32_t* __restrict__ stats, int32_t* stk,
-                         int stk_stride, int lane_stack) {
+                         int stk_stride, int lane_stack, int budget, bool coherent) {
     const uint32_t W = p.width, H = p.height;
     const Ray r = camera_ray(cam, W, H, px);
     Hit h;
-    trace_block<false, PLANE>(sc, r, px.valid, -1, h, stk, stk_stride, lane_stack);
+    trace_primary<PM>(sc, r, px.valid, h, stk, stk_stride, lane_stack, budget, coherent);
     const bool hit = px.valid && h.prim >= 0;
     if (hit) refine_hit(sc.tris, r, h);
 
@@ -316,7 +358,8 @@ RT_DEV void render_pixel(const SceneArgs& sc, const rt_camera& cam, const rt_ren
     sr.tmin = p.shadow_tmin;
     sr.tmax = INFINITY;
     Hit sh;
-    trace_block<true, SLANE>(sc, sr, want_shadow, hit ? h.slot : -1, sh, stk, stk_stride, lane_stack);
+    trace_block<true, SM>(sc, sr, want_shadow, hit ? h.slot : -1, sh, stk, stk_stride, lane_stack,
+                          budget);
     const bool occluded = want_shadow && sh.prim >= 0;
 
     float alb[3] = {1.0f, 1.0f, 1.0f};
@@ -353,18 +396,18 @@ RT_DEV void render_pixel(const SceneArgs& sc, const rt_camera& cam, const rt_ren
 }
 
 // Fused render of one frame (see the file comment), one 16x16-pixel block per workgroup.
-template <bool PLANE, bool SLANE>
+template <int PM, int SM>
 __global__ void __launch_bounds__(256) k_render(SceneArgs sc, rt_camera cam, rt_render_params p,
-                                                 TileMap tm, int lane_stack, float* __restrict__ rgb,
-                                                 float* __restrict__ out_t,
+                                                 TileMap tm, int lane_stack, int budget,
+                                                 float* __restrict__ rgb, float* __restrict__ out_t,
                                                  int32_t* __restrict__ out_prim,
                                                  uint32_t* __restrict__ stats) {
-    __shared__ int32_t stk[(PLANE || SLANE) ? kLaneStack * 256 : 1];
+    __shared__ int32_t stk[(uses_lds(PM) || uses_lds(SM)) ? kLaneStack * 256 : 1];
     uint32_t tx, ty;
     if (!block_tile(tm, tx, ty)) return;
     const TilePixel px = tile_pixel(p.width, p.height, tx, ty);
-    render_pixel<PLANE, SLANE>(sc, cam, p, px, rgb, out_t, out_prim, stats, stk + threadIdx.x, 256,
-                               lane_stack);
+    render_pixel<PM, SM>(sc, cam, p, px, rgb, out_t, out_prim, stats, stk + threadIdx.x, 256,
+                         lane_stack, budget, true);
 }
 
 // ---- batched render ----
@@ -446,10 +489,10 @@ __global__ void __launch_bounds__(1024) k_batch_prepare(const rt_render_job* __r
 // Persistent batched render: one wave per block; each wave pops `grab` consecutive 8x8 tiles of the
 // concatenated job list from the work counter until the list is exhausted (every wave reaches the
 // exit: the counter only grows). Tile -> job by binary search of the prefix, copied into LDS.
-template <bool SLANE>
+template <int PM, int SM>
 __global__ void __launch_bounds__(64) k_render_batch(const rt_render_job* __restrict__ jobs,
                                                       uint32_t n_jobs, uint32_t* __restrict__ ws,
-                                                      uint32_t grab, int lane_stack) {
+                                                      uint32_t grab, int lane_stack, int budget) {
     extern __shared__ uint32_t smem[];
     uint32_t* pref = smem;
     int32_t* stk = (int32_t*)(smem + ((n_jobs + 1 + 3) & ~3u));
@@ -486,8 +529,9 @@ __global__ void __launch_bounds__(64) k_render_batch(const rt_render_job* __rest
             sc.mesh_offsets = job.scene.mesh_offsets;
             sc.root = job.scene.root;
             sc.n_meshes = job.scene.n_meshes;
-            render_pixel<false, SLANE>(sc, job.cam, p, px, job.out.rgb, job.out.t, job.out.prim,
-                                       job.out.stats, stk + lane, 64, lane_stack);
+            render_pixel<PM, SM>(sc, job.cam, p, px, job.out.rgb, job.out.t, job.out.prim,
+                                 job.out.stats, stk + lane, 64, lane_stack, budget,
+                                 coherent_tiles(job.cam, p.width, p.height));
         }
     }
 }
@@ -523,6 +567,11 @@ int check_rays(const rt_rays* r, const char* what) {
     if (r->stride < 4 || (r->stride & 3))
         return set_error(RT_EINVAL, std::string(what) + ": ray stride must be a multiple of 4");
     return RT_OK;
+}
+
+int resolve_primary(int mode, const rt_camera& cam, uint32_t W, uint32_t H) {
+    if (mode != RT_TRAVERSAL_AUTO) return mode;
+    return coherent_tiles(cam, W, H) ? RT_TRAVERSAL_PACKET : RT_TRAVERSAL_LANE;
 }
 
 inline uint32_t blocks_for(uint64_t n, uint32_t per) { return (uint32_t)((n + per - 1) / per); }
@@ -580,9 +629,12 @@ extern "C" int rt_trace_camera(const rt_scene* scene, const rt_camera* cam, uint
     if (n > 0x7FFFFFFFull) return set_error(RT_ECAPACITY, "rt_trace_camera: image too large");
     const rt_tuning tu = current_tuning();
     const TileMap tm = make_tile_map(W, H, tu.super_tile);
-    auto kern = tu.primary == RT_TRAVERSAL_LANE ? k_trace_camera<true> : k_trace_camera<false>;
-    hipLaunchKernelGGL(kern, dim3(tile_map_blocks(tm)), dim3(256), 0, (hipStream_t)stream,
-                       scene_args(scene), *cam, W, H, tm, (int)tu.lane_stack, *hits);
+    using K = decltype(&k_trace_camera<0>);
+    static const K kerns[3] = {k_trace_camera<0>, k_trace_camera<1>, k_trace_camera<2>};
+    const int pm = resolve_primary(tu.primary, *cam, W, H);
+    hipLaunchKernelGGL(kerns[pm], dim3(tile_map_blocks(tm)), dim3(256), 0,
+                       (hipStream_t)stream, scene_args(scene), *cam, W, H, tm, (int)tu.lane_stack,
+                       (int)tu.packet_budget, *hits);
     return check_launch("rt_trace_camera");
 }
 
@@ -609,11 +661,14 @@ extern "C" int rt_render(const rt_scene* scene, const rt_camera* cam, const rt_r
     if (out->stats) hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, out->stats, 4u);
     const rt_tuning tu = current_tuning();
     const TileMap tm = make_tile_map(p->width, p->height, tu.super_tile);
-    const bool pl = tu.primary == RT_TRAVERSAL_LANE, sl = tu.shadow == RT_TRAVERSAL_LANE;
-    auto kern = pl ? (sl ? k_render<true, true> : k_render<true, false>)
-                   : (sl ? k_render<false, true> : k_render<false, false>);
-    hipLaunchKernelGGL(kern, dim3(tile_map_blocks(tm)), dim3(256), 0, s, scene_args(scene), *cam, *p,
-                       tm, (int)tu.lane_stack, out->rgb, out->t, out->prim, out->stats);
+    using K = decltype(&k_render<0, 0>);
+    static const K kerns[3][3] = {{k_render<0, 0>, k_render<0, 1>, k_render<0, 2>},
+                                  {k_render<1, 0>, k_render<1, 1>, k_render<1, 2>},
+                                  {k_render<2, 0>, k_render<2, 1>, k_render<2, 2>}};
+    const int pm = resolve_primary(tu.primary, *cam, p->width, p->height);
+    hipLaunchKernelGGL(kerns[pm][tu.shadow], dim3(tile_map_blocks(tm)), dim3(256), 0, s,
+                       scene_args(scene), *cam, *p, tm, (int)tu.lane_stack, (int)tu.packet_budget,
+                       out->rgb, out->t, out->prim, out->stats);
     return check_launch("rt_render");
 }
 
@@ -631,9 +686,11 @@ extern "C" int rt_render_batch(const rt_render_job* jobs_dev, uint32_t n_jobs, c
     if (n_jobs == 0) return RT_OK;
     if (n_jobs > RT_BATCH_MAX_JOBS)
         return set_error(RT_ECAPACITY, "rt_render_batch: more than RT_BATCH_MAX_JOBS jobs");
+    auto alg_ok = [](int32_t a) { return a >= RT_TRAVERSAL_PACKET && a <= RT_TRAVERSAL_HYBRID; };
     if (cfg->waves_per_cu < 1 || cfg->waves_per_cu > 32 || cfg->tiles_per_grab < 1 ||
         cfg->tiles_per_grab > 64 || cfg->lane_stack < 1 || cfg->lane_stack > RT_LANE_STACK_MAX ||
-        (cfg->shadow != RT_TRAVERSAL_PACKET && cfg->shadow != RT_TRAVERSAL_LANE))
+        !(alg_ok(cfg->primary) || cfg->primary == RT_TRAVERSAL_AUTO) || !alg_ok(cfg->shadow) ||
+        cfg->packet_budget < 1)
         return set_error(RT_EINVAL, "rt_render_batch: rt_batch_config field out of range");
     if (!workspace || ws_bytes < rt_render_batch_workspace_bytes(n_jobs))
         return set_error(RT_EINVAL, "rt_render_batch: workspace too small");
@@ -641,14 +698,17 @@ extern "C" int rt_render_batch(const rt_render_job* jobs_dev, uint32_t n_jobs, c
     hipStream_t s = (hipStream_t)stream;
     uint32_t* ws = (uint32_t*)workspace;
     hipLaunchKernelGGL(k_batch_prepare, dim3(1), dim3(1024), 0, s, jobs_dev, n_jobs, ws);
-    const bool sl = cfg->shadow == RT_TRAVERSAL_LANE;
-    const size_t lds = 4u * ((n_jobs + 1 + 3) & ~3u) + (sl ? 4u * 64u * kLaneStack : 0u);
+    const bool lds_stack = cfg->primary != RT_TRAVERSAL_PACKET || cfg->shadow != RT_TRAVERSAL_PACKET;
+    const size_t lds = 4u * ((n_jobs + 1 + 3) & ~3u) + (lds_stack ? 4u * 64u * kLaneStack : 0u);
     const uint32_t grid = (uint32_t)device_cu_count() * cfg->waves_per_cu;
-    if (sl)
-        hipLaunchKernelGGL(k_render_batch<true>, dim3(grid), dim3(64), lds, s, jobs_dev, n_jobs, ws,
-                           cfg->tiles_per_grab, (int)cfg->lane_stack);
-    else
-        hipLaunchKernelGGL(k_render_batch<false>, dim3(grid), dim3(64), lds, s, jobs_dev, n_jobs, ws,
-                           cfg->tiles_per_grab, (int)cfg->lane_stack);
+    using K = decltype(&k_render_batch<0, 0>);
+    static const K kerns[4][3] = {
+        {k_render_batch<0, 0>, k_render_batch<0, 1>, k_render_batch<0, 2>},
+        {k_render_batch<1, 0>, k_render_batch<1, 1>, k_render_batch<1, 2>},
+        {k_render_batch<2, 0>, k_render_batch<2, 1>, k_render_batch<2, 2>},
+        {k_render_batch<3, 0>, k_render_batch<3, 1>, k_render_batch<3, 2>}};
+    hipLaunchKernelGGL(kerns[cfg->primary][cfg->shadow], dim3(grid), dim3(64), lds, s, jobs_dev,
+                       n_jobs, ws, cfg->tiles_per_grab, (int)cfg->lane_stack,
+                       (int)cfg->packet_budget);
     return check_launch("rt_render_batch");
 }

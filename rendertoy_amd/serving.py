@@ -135,6 +135,7 @@ class Request:
     keep_image: bool = False
     # filled in by the server
     latency: float = math.nan
+    service: float = math.nan   # GPU time of the request's batch
     batch: int = -1
     image: typing.Any = None
 
@@ -157,7 +158,8 @@ class RenderServer:
         self.config = config or BatchConfig(waves_per_cu=16, tiles_per_grab=4, shadow="lane")
         self.max_batch = max_batch
         self.max_inflight = max_inflight
-        self.batches = []
+        self.batches = []      # requests per launched batch
+        self.batch_log = []    # (requests, GPU start s, GPU end s) per batch, from the run's start
 
     def _launch(self, reqs, t0_event):
         """Launches the longest prefix of `reqs` whose scenes fit the budget next to the pinned
@@ -176,16 +178,22 @@ class RenderServer:
         rr = [RenderRequest(res.raycaster, camera_for(r.eye), r.size, r.size, textures=res.binding)
               for r, res in zip(reqs, residents)]
         b = BatchRender(rr, config=self.config, stats=False)
+        start = torch.cuda.Event(enable_timing=True)
+        start.record()
         b.launch()
         ev = torch.cuda.Event(enable_timing=True)
         ev.record()
-        return {"reqs": reqs, "batch": b, "event": ev, "residents": residents, "t0": t0_event}, rest
+        return {"reqs": reqs, "batch": b, "event": ev, "start": start, "residents": residents,
+                "t0": t0_event}, rest
 
     def _finish(self, f):
         done_s = f["t0"].elapsed_time(f["event"]) * 1e-3
+        start_s = f["t0"].elapsed_time(f["start"]) * 1e-3
         k = len(self.batches)
+        self.batch_log.append((len(f["reqs"]), start_s, done_s))
         for i, r in enumerate(f["reqs"]):
-            r.latency = done_s - r.arrival
+            r.latency = done_s - (r.arrival if self._realtime else 0.0)
+            r.service = done_s - start_s
             r.batch = k
             if r.keep_image:
                 r.image = f["batch"].images[i]
@@ -196,6 +204,7 @@ class RenderServer:
     def run(self, requests: typing.List[Request], realtime: bool = True):
         """Serves `requests` (sorted by arrival). realtime: arrivals are released at their wall-clock
         times from the start of the run; otherwise all are present at t = 0."""
+        self._realtime = realtime  # else every request is present at t = 0: latency from there
         pending = collections.deque(sorted(requests, key=lambda r: r.arrival))
         queue = collections.deque()
         inflight = collections.deque()
@@ -253,7 +262,11 @@ def make_requests(n: int, n_scenes: int, rate: float, seed: int, sizes=(256, 512
 
 def latency_summary(reqs):
     lat = np.array([r.latency for r in reqs]) * 1e3
+    svc = np.array([r.service for r in reqs]) * 1e3
     span = max(r.arrival + r.latency for r in reqs) - min(r.arrival for r in reqs)
     return {"requests": len(reqs), "p50_ms": round(float(np.percentile(lat, 50)), 3),
             "p99_ms": round(float(np.percentile(lat, 99)), 3),
-            "max_ms": round(float(lat.max()), 3), "throughput_rps": round(len(reqs) / span, 1)}
+            "max_ms": round(float(lat.max()), 3),
+            "batch_gpu_ms_p50": round(float(np.percentile(svc, 50)), 3),
+            "batch_gpu_ms_p99": round(float(np.percentile(svc, 99)), 3),
+            "throughput_rps": round(len(reqs) / span, 1)}

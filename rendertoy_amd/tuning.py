@@ -1,5 +1,6 @@
 """Launch choices of the camera and render kernels (process-wide): which traversal walks the
-primary and the shadow rays, the block -> XCD mapping, and the per-lane stack size. Every choice
+primary and the shadow rays (packet, per-lane, or packet then per-lane after a step budget), the
+block -> XCD mapping, the per-lane stack size and the hybrid's packet budget. Every choice
 returns the same hits; they trade speed only. The reference has no counterpart (its kernels run a
 fixed 32-work-item group, rendering/_core.py:286)."""
 import contextlib
@@ -8,20 +9,23 @@ from . import _native
 
 PACKET = _native.RT_TRAVERSAL_PACKET
 LANE = _native.RT_TRAVERSAL_LANE
-_NAMES = {"packet": PACKET, "lane": LANE}
+HYBRID = _native.RT_TRAVERSAL_HYBRID
+AUTO = _native.RT_TRAVERSAL_AUTO
+_NAMES = {"packet": PACKET, "lane": LANE, "hybrid": HYBRID, "auto": AUTO}
 
 
 def get_tuning():
-    """Current settings as a dict {primary, shadow, super_tile, lane_stack}."""
+    """Current settings as a dict {primary, shadow, super_tile, lane_stack, packet_budget}."""
     lib = _native.load()
     t = _native.rt_tuning()
     _native.check(lib.rt_get_tuning(t), "rt_get_tuning")
     return {"primary": t.primary, "shadow": t.shadow, "super_tile": t.super_tile,
-            "lane_stack": t.lane_stack}
+            "lane_stack": t.lane_stack, "packet_budget": t.packet_budget}
 
 
 def set_tuning(**kw):
-    """Updates the given fields; traversals may be given as "packet"/"lane" or the constants.
+    """Updates the given fields; traversals may be given as "packet"/"lane"/"hybrid"/"auto" (camera
+    rays only) or the constants.
     Raises RuntimeError (the library's message) for out-of-range values."""
     lib = _native.load()
     cur = get_tuning()
@@ -30,7 +34,7 @@ def set_tuning(**kw):
             raise TypeError(f"unknown tuning field {k!r}")
         cur[k] = _NAMES.get(v, v) if k in ("primary", "shadow") else v
     t = _native.rt_tuning(int(cur["primary"]), int(cur["shadow"]), int(cur["super_tile"]),
-                          int(cur["lane_stack"]))
+                          int(cur["lane_stack"]), int(cur["packet_budget"]))
     _native.check(lib.rt_set_tuning(t), "rt_set_tuning")
 
 

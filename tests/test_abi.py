@@ -76,7 +76,9 @@ int main(void) {
     S(rt_render_out) O(rt_render_out, stats)
     S(rt_tuning) O(rt_tuning, shadow) O(rt_tuning, super_tile) O(rt_tuning, lane_stack)
     S(rt_render_job) O(rt_render_job, cam) O(rt_render_job, params) O(rt_render_job, out)
-    S(rt_batch_config) O(rt_batch_config, tiles_per_grab) O(rt_batch_config, shadow) O(rt_batch_config, lane_stack)
+    S(rt_batch_config) O(rt_batch_config, tiles_per_grab) O(rt_batch_config, primary)
+    O(rt_batch_config, shadow) O(rt_batch_config, lane_stack) O(rt_batch_config, packet_budget)
+    O(rt_tuning, packet_budget)
     return 0;
 }
 """
@@ -110,12 +112,14 @@ def test_tuning_get_set_and_range_checks_without_gpu():
     import rendertoy_amd as ren
     from rendertoy_amd import _native
     d = ren.get_tuning()
-    assert d == {"primary": _native.RT_TRAVERSAL_PACKET, "shadow": _native.RT_TRAVERSAL_LANE,
-                 "super_tile": 0, "lane_stack": _native.RT_LANE_STACK_MAX}
-    with ren.tuning(primary="lane", lane_stack=3, super_tile=2):
-        assert ren.get_tuning() == {"primary": 1, "shadow": 1, "super_tile": 2, "lane_stack": 3}
+    assert d == {"primary": _native.RT_TRAVERSAL_AUTO, "shadow": _native.RT_TRAVERSAL_LANE,
+                 "super_tile": 0, "lane_stack": _native.RT_LANE_STACK_MAX, "packet_budget": 256}
+    with ren.tuning(primary="hybrid", lane_stack=3, super_tile=2, packet_budget=7):
+        assert ren.get_tuning() == {"primary": 2, "shadow": 1, "super_tile": 2, "lane_stack": 3,
+                                    "packet_budget": 7}
     assert ren.get_tuning() == d
-    for bad in ({"lane_stack": 0}, {"lane_stack": 33}, {"primary": 2}, {"super_tile": 65}):
+    for bad in ({"lane_stack": 0}, {"lane_stack": 33}, {"primary": 4}, {"shadow": 3},
+                {"super_tile": 65}, {"packet_budget": 0}):
         with pytest.raises(RuntimeError, match="rt_set_tuning"):
             ren.set_tuning(**bad)
     assert ren.get_tuning() == d
@@ -126,16 +130,17 @@ def test_tuning_get_set_and_range_checks_without_gpu():
 def test_batch_entry_rejects_bad_arguments_without_gpu():
     from rendertoy_amd import _native as N
     lib = N.load()
-    cfg = N.rt_batch_config(8, 1, N.RT_TRAVERSAL_LANE, 32)
+    cfg = N.rt_batch_config(8, 1, N.RT_TRAVERSAL_HYBRID, N.RT_TRAVERSAL_LANE, 32, 256)
     ws = ctypes.create_string_buffer(1 << 16)
     wsp = ctypes.cast(ws, ctypes.c_void_p).value
     fake_jobs = 0x1000
     assert lib.rt_render_batch(None, 1, ctypes.byref(cfg), wsp, 1 << 16, None) == -1
     assert lib.rt_render_batch(fake_jobs, 1, None, wsp, 1 << 16, None) == -1
     assert lib.rt_render_batch(fake_jobs, N.RT_BATCH_MAX_JOBS + 1, ctypes.byref(cfg), wsp, 1 << 20, None) == -3
-    for bad in (N.rt_batch_config(0, 1, 1, 32), N.rt_batch_config(33, 1, 1, 32),
-                N.rt_batch_config(8, 0, 1, 32), N.rt_batch_config(8, 65, 1, 32),
-                N.rt_batch_config(8, 1, 2, 32), N.rt_batch_config(8, 1, 1, 0)):
+    for bad in (N.rt_batch_config(0, 1, 0, 1, 32, 256), N.rt_batch_config(33, 1, 0, 1, 32, 256),
+                N.rt_batch_config(8, 0, 0, 1, 32, 256), N.rt_batch_config(8, 65, 0, 1, 32, 256),
+                N.rt_batch_config(8, 1, 4, 1, 32, 256), N.rt_batch_config(8, 1, 0, 3, 32, 256),
+                N.rt_batch_config(8, 1, 0, 1, 0, 256), N.rt_batch_config(8, 1, 0, 1, 32, 0)):
         assert lib.rt_render_batch(fake_jobs, 1, ctypes.byref(bad), wsp, 1 << 16, None) == -1
     need = lib.rt_render_batch_workspace_bytes(10)
     assert need == 4 * (64 + 11)

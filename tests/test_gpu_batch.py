@@ -44,6 +44,7 @@ def requests(ren, scenes_, with_hits=True):
         (0, 0, 16, (0.0, 0.0, 3.0), False),       # no pixels
         (2, 128, 96, (0.2, -0.4, 3.2), True),
         (0, 8, 8, (0.1, 0.1, 2.0), False),
+        (1, 1024, 768, (0.3, 0.6, 2.6), True),   # coherent tiles: AUTO takes the packet walk
     ]
     reqs = []
     for s, W, H, eye, textured in specs:
@@ -74,8 +75,11 @@ def assert_batch_equals_alone(b, reqs):
         np.testing.assert_array_equal(b.stats[i].cpu().numpy(), st)
 
 
-@pytest.mark.parametrize("cfg", [dict(), dict(shadow="packet"), dict(tiles_per_grab=3, waves_per_cu=1),
-                                 dict(tiles_per_grab=64, waves_per_cu=32, lane_stack=2)])
+@pytest.mark.parametrize("cfg", [dict(), dict(shadow="packet", primary="packet"),
+                                 dict(tiles_per_grab=3, waves_per_cu=1, primary="lane"),
+                                 dict(tiles_per_grab=64, waves_per_cu=32, lane_stack=2),
+                                 dict(primary="hybrid", shadow="hybrid", packet_budget=2),
+                                 dict(primary="packet", shadow="lane"), dict(primary="auto")])
 def test_batch_equals_individual_renders(ren, scenes_, cfg):
     from rendertoy_amd.batch import BatchRender, BatchConfig
     reqs = requests(ren, scenes_)

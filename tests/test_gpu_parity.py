@@ -147,17 +147,22 @@ def test_render_c1_matches_twin(c1, twin):
     assert list(s[:3]) == list(st)
 
 
-@pytest.mark.parametrize("primary,shadow,lane_stack,super_tile", [
-    ("lane", "lane", 32, 0), ("lane", "packet", 2, 3), ("packet", "lane", 1, 1), ("lane", "lane", 4, 0)])
-def test_tuning_choices_give_identical_results(c1, twin, primary, shadow, lane_stack, super_tile):
-    """Every traversal / stack / block-mapping choice returns the twin's hits and RGB. Stacks of 1-4
-    entries overflow on most rays, which the waves then trace again with the packet traversal."""
+@pytest.mark.parametrize("primary,shadow,lane_stack,super_tile,budget", [
+    ("lane", "lane", 32, 0, 256), ("lane", "packet", 2, 3, 256), ("packet", "lane", 1, 1, 256),
+    ("lane", "lane", 4, 0, 256), ("hybrid", "hybrid", 32, 0, 3), ("hybrid", "lane", 2, 0, 1),
+    ("hybrid", "hybrid", 32, 0, 40), ("auto", "lane", 32, 0, 256)])
+def test_tuning_choices_give_identical_results(c1, twin, primary, shadow, lane_stack, super_tile,
+                                               budget):
+    """Every traversal / stack / block-mapping / budget choice returns the twin's hits and RGB.
+    Stacks of 1-4 entries overflow on most rays, which the waves then trace again with the packet
+    traversal; budgets of 1-40 packet steps send most waves on to the per-lane walk."""
     import rendertoy_amd as ren
     meshes, tw, rc = c1
     cam = camera()
     rgb, t, prim, st = tw.render(cam.as_array(), 512, 512, LIGHT, shadows=True)
     ref = tw.trace_camera(cam.as_array(), 512, 512)
-    with ren.tuning(primary=primary, shadow=shadow, lane_stack=lane_stack, super_tile=super_tile):
+    with ren.tuning(primary=primary, shadow=shadow, lane_stack=lane_stack, super_tile=super_tile,
+                    packet_budget=budget):
         stats = torch.zeros(4, dtype=torch.int32, device="cuda")
         img, hits = rc.render(cam, 512, 512, light=LIGHT, shadows=True, bilinear=True,
                               return_hits=True, stats=stats)

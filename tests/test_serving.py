@@ -79,7 +79,8 @@ def test_served_images_equal_alone_renders_under_eviction(ren):
     srv.run(reqs, realtime=False)
     torch.cuda.synchronize()
     assert store.evictions > 0 and store.hits > 0
-    assert all(r.latency > 0 for r in reqs) and sum(srv.batches) == len(reqs)
+    assert all(r.latency > 0 for r in reqs), [r.latency for r in reqs]
+    assert sum(srv.batches) == len(reqs)
     for r in reqs:
         res = store.acquire(r.scene)
         img = res.raycaster.render(serving.camera_for(r.eye), r.size, r.size, textures=res.binding)
