@@ -55,6 +55,8 @@ def parse():
     ap.add_argument("--serving-requests", type=int, default=300, help="requests per arrival rate")
     ap.add_argument("--serving-rates", default="100,500,2000",
                     help="comma-separated arrival rates (requests/s)")
+    ap.add_argument("--no-incoherent", action="store_true",
+                    help="skip the incoherent-ray measurement (diffuse paths, 4 spp, 2 bounces)")
     ap.add_argument("--copy-calib-mib", type=int, default=0,
                     help="first run 3 elementwise int32 adds over this many MiB (a kernel of "
                          "known read/write traffic that scales the FETCH_SIZE/WRITE_SIZE "
@@ -151,6 +153,47 @@ def measure_batched(args, ren, scenes, torch):
         "autotune_ms": {c.key(): round(ms, 3) for c, ms in table},
         "setup_s": round(gen_s, 1),
     }
+
+
+def incoherent_scene(scenes):
+    """8 displaced icospheres of frequency 112 (250,880 triangles each), sphere k scaled by 0.35
+    and centred at a point uniform in [-1, 1]^3, from the seed 5000 + k."""
+    meshes = []
+    for k in range(8):
+        rng = np.random.default_rng(5000 + k)
+        c = rng.uniform(-1.0, 1.0, 3)
+        d = dict(scenes.icosphere(112, 5000 + k))
+        d["pos"] = (d["pos"].astype(np.float64) * 0.35 + c).astype(np.float32)
+        meshes.append(d)
+    return meshes
+
+
+def measure_incoherent(args, ren, scenes, torch):
+    """Diffuse paths at 1920x1080, 4 samples per pixel, primary + 2 cosine-weighted bounces
+    (Philox4x32-10 key (0x5EED, frame)); rays/s counts the rays actually traced (a path stops
+    when it escapes)."""
+    from rendertoy_amd.batch import time_launches
+    meshes = incoherent_scene(scenes)
+    rc = ren.Raycaster([ren.mesh_from_arrays(m["pos"], m["idx"], m["nrm"], m["uv"]) for m in meshes])
+    W, H, spp, bounces = 1920, 1080, 4, 2
+    cam = ren.Camera((0.0, 0.0, 3.0), (0.0, 0.0, 0.0), (0.0, 1.0, 0.0), math.pi / 4, W / H)
+    out = torch.empty((H, W), dtype=torch.float32, device="cuda")
+    counts = torch.zeros(bounces + 1, dtype=torch.int32, device="cuda")
+    frame = [0]
+
+    def step():
+        frame[0] += 1
+        rc.trace_paths(cam, W, H, spp=spp, bounces=bounces, frame=frame[0], out=out, counts=counts)
+    ms = time_launches(step, max(5, args.steps // 5), warmup=2)
+    cnt = counts.cpu().numpy().astype(np.int64)
+    rays = int(cnt.sum())
+    med = float(np.median(ms))
+    return {"what": "8 spheres x 250,880 triangles, 1920x1080 x 4 spp, primary + 2 diffuse bounces",
+            "triangles": int(rc.n_tris), "ms_per_frame_p50": round(med, 3),
+            "ms_per_frame_p99": round(percentile(ms, 99), 3),
+            "rays_per_frame": rays, "rays_by_depth": [int(x) for x in cnt],
+            "nominal_rays_per_frame": W * H * spp * (bounces + 1),
+            "mrays_per_s": round(rays / (med * 1e-3) / 1e6, 2)}
 
 
 def serving_pool(n_scenes, scenes):
@@ -331,6 +374,8 @@ def main():
         result["batched"] = measure_batched(args, ren, scenes, torch)
     if not args.no_serving:
         result["serving"] = measure_serving(args, ren, scenes, torch)
+    if not args.no_incoherent:
+        result["incoherent"] = measure_incoherent(args, ren, scenes, torch)
 
     # ---- roofline and CPU baseline from the CPU restatement (oracle) ----
     if not args.no_roofline or not args.no_cpu_baseline:

@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define RT_ABI_VERSION 6
+#define RT_ABI_VERSION 7
 
 enum {
     RT_OK = 0,
@@ -258,6 +258,38 @@ int rt_render_batch(const rt_render_job* jobs_dev, uint32_t n_jobs, const rt_bat
 
 /* Device address of the uint32 status word inside a batch workspace (0 = every job rendered). */
 const uint32_t* rt_render_batch_status(const void* workspace_dev);
+
+/* ---- incoherent rays: diffuse paths (no reference counterpart) ----
+ * spp jittered camera rays per pixel, each followed by up to `bounces` cosine-weighted diffuse
+ * bounces off the geometric normal (facing the incoming ray), lit by a uniform sky: a sample's
+ * radiance is sky * albedo^k if its path escapes after k bounces, 0 if it is still on a surface
+ * after the last one; the pixel is the mean of its samples. Random numbers: Philox4x32-10 with
+ * key (seed, frame) and counter (pixel, sample, depth, 0); depth 0 jitters the camera ray, depth k
+ * draws bounce k. Rays that hit are compacted into a queue per depth (wave ballot, one atomic per
+ * wave). Deterministic: the output does not depend on queue order. */
+#define RT_PATHS_MAX_BOUNCES 16u
+
+typedef struct rt_paths_params {
+    uint32_t width, height;
+    uint32_t spp;             /* samples per pixel, 1..1024 */
+    uint32_t bounces;         /* 0..RT_PATHS_MAX_BOUNCES */
+    uint32_t seed;            /* Philox key word 0 */
+    uint32_t frame;           /* Philox key word 1 */
+    float albedo;             /* diffuse reflectance of every surface */
+    float sky;                /* radiance of the sky */
+    float ray_tmin;           /* bounce rays start this far along (and skip their own triangle) */
+} rt_paths_params;
+
+typedef struct rt_paths_out {
+    float* radiance;          /* device, width*height float32, row-major */
+    uint32_t* counts;         /* device, optional: bounces+1 uint32, rays traced at each depth */
+} rt_paths_out;
+
+/* Workspace: counters, per-sample radiance, two ray queues of width*height*spp 48-byte records. */
+uint64_t rt_paths_workspace_bytes(uint32_t width, uint32_t height, uint32_t spp);
+int rt_trace_paths(const rt_scene* scene, const rt_camera* cam, const rt_paths_params* params,
+                   const rt_paths_out* out, void* workspace_dev, uint64_t workspace_bytes,
+                   void* stream);
 
 #ifdef __cplusplus
 }

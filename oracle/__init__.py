@@ -58,6 +58,14 @@ def lib():
         L.tw_render.restype = None
         L.tw_render.argtypes = [vp, vp, vp, i32, vp, ctypes.c_uint32, ctypes.c_uint32, vp,
                                 ctypes.c_uint32] + [vp] * 9
+        L.tw_philox4x32_10.restype = None
+        L.tw_philox4x32_10.argtypes = [i64, vp, vp, vp]
+        L.tw_cos_sin_2pi.restype = None
+        L.tw_cos_sin_2pi.argtypes = [i64, vp, vp, vp]
+        L.tw_cosine_dir.restype = None
+        L.tw_cosine_dir.argtypes = [i64, vp, vp, vp, vp]
+        L.tw_paths.restype = None
+        L.tw_paths.argtypes = [vp, vp, i32, vp] + [ctypes.c_uint32] * 6 + [f32, f32, f32, vp, vp, vp]
         L.tw_closest64_brute.restype = None
         L.tw_closest64_brute.argtypes = [vp, i64, i64] + [vp] * 6 + [f64, f64, f64] + [vp] * 5
         L.tw_closest64_bvh.restype = None
@@ -269,6 +277,18 @@ class TwinScene:
                         _p(stats))
         return rgb, t, prim, stats
 
+    def paths(self, cam12, W, H, spp=4, bounces=2, seed=0x5EED, frame=0, albedo=0.7, sky=1.0,
+              tmin=1e-4):
+        """Diffuse paths of rt_trace_paths: (radiance [H*W], samples [H*W*spp], counts per depth)."""
+        n = W * H
+        rad = np.empty(n, np.float32)
+        smp = np.empty(n * spp, np.float32)
+        cnt = np.zeros(17, np.int64)
+        lib().tw_paths(_p(self.nodes_full), _p(self.tris if self.n else np.zeros((1, 12), np.float32)),
+                       self.root, _p(np.ascontiguousarray(cam12, np.float32)), W, H, spp, bounces,
+                       seed, frame, albedo, sky, tmin, _p(smp), _p(rad), _p(cnt))
+        return rad, smp, cnt[:bounces + 1]
+
     def closest64(self, o, d, tmin=0.0, edge_eps=1e-6, tie_eps=1e-5, margin=1e-4, brute=False):
         return closest64(self.pos, o, d, tmin, edge_eps, tie_eps, margin, None if brute else self)
 
@@ -380,3 +400,49 @@ def shade64(tri_pos, tri_nrm, tri_uv, mesh_of, has_nrm, o, d, light, ambient=0.2
                 alb[sel] = bilinear_fp64(np.asarray(tex), tu[sel], tv[sel])
     rgb[hit] = alb * (ambient + vis * ndl)[:, None]
     return rgb, flags
+
+
+# ---- Philox4x32-10 and the path sampling (rt_trace_paths) ----
+
+def twin_philox(ctr, key):
+    """Philox4x32-10 of the C restatement: ctr [n, 4] uint32, key (k0, k1) -> [n, 4] uint32."""
+    ctr = np.ascontiguousarray(ctr, np.uint32).reshape(-1, 4)
+    out = np.empty_like(ctr)
+    k = np.ascontiguousarray(key, np.uint32)
+    lib().tw_philox4x32_10(len(ctr), _p(ctr), _p(k), _p(out))
+    return out
+
+
+def philox_np(ctr, key):
+    """Philox4x32-10 vectorised in numpy (uint64 products), from the SC'11 definition: round
+    (c0..c3) -> (hi(M1 c2) ^ c1 ^ k0, lo(M1 c2), hi(M0 c0) ^ c3 ^ k1, lo(M0 c0)), M0 = 0xD2511F53,
+    M1 = 0xCD9E8D57; key += (0x9E3779B9, 0xBB67AE85) between rounds; 10 rounds."""
+    c = np.array(ctr, dtype=np.uint64).reshape(-1, 4)
+    k0 = np.uint64(key[0])
+    k1 = np.uint64(key[1])
+    m32 = np.uint64(0xFFFFFFFF)
+    for _ in range(10):
+        p0 = np.uint64(0xD2511F53) * c[:, 0]
+        p1 = np.uint64(0xCD9E8D57) * c[:, 2]
+        c = np.stack([(p1 >> np.uint64(32)) ^ c[:, 1] ^ k0, p1 & m32,
+                      (p0 >> np.uint64(32)) ^ c[:, 3] ^ k1, p0 & m32], axis=1)
+        k0 = (k0 + np.uint64(0x9E3779B9)) & m32
+        k1 = (k1 + np.uint64(0xBB67AE85)) & m32
+    return c.astype(np.uint32)
+
+
+def twin_cos_sin_2pi(x):
+    x = np.ascontiguousarray(x, np.float32)
+    c = np.empty_like(x)
+    s = np.empty_like(x)
+    lib().tw_cos_sin_2pi(len(x), _p(x), _p(c), _p(s))
+    return c, s
+
+
+def twin_cosine_dir(n, u0, u1):
+    n = np.ascontiguousarray(n, np.float32).reshape(-1, 3)
+    u0 = np.ascontiguousarray(u0, np.float32)
+    u1 = np.ascontiguousarray(u1, np.float32)
+    d = np.empty_like(n)
+    lib().tw_cosine_dir(len(n), _p(n), _p(u0), _p(u1), _p(d))
+    return d

@@ -1090,3 +1090,147 @@ TW_API void tw_set_threads(int n) {
     (void)n;
 #endif
 }
+
+/* ------------------------------------------------------------------------------------------ */
+/* Diffuse paths (rt_trace_paths), operation for operation                                   */
+/* ------------------------------------------------------------------------------------------ */
+
+/* Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11): 10 rounds of two 32x32->64 multiplies,
+ * key bumped by the Weyl constants between rounds. */
+static void philox4x32_10(uint32_t c[4], uint32_t k0, uint32_t k1) {
+    for (int i = 0; i < 10; ++i) {
+        uint64_t p0 = (uint64_t)0xD2511F53u * c[0];
+        uint64_t p1 = (uint64_t)0xCD9E8D57u * c[2];
+        uint32_t n0 = (uint32_t)(p1 >> 32) ^ c[1] ^ k0, n1 = (uint32_t)p1;
+        uint32_t n2 = (uint32_t)(p0 >> 32) ^ c[3] ^ k1, n3 = (uint32_t)p0;
+        c[0] = n0; c[1] = n1; c[2] = n2; c[3] = n3;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+}
+
+TW_API void tw_philox4x32_10(int64_t n, const uint32_t* ctr, const uint32_t* key, uint32_t* out) {
+    for (int64_t i = 0; i < n; ++i) {
+        uint32_t c[4] = {ctr[4 * i], ctr[4 * i + 1], ctr[4 * i + 2], ctr[4 * i + 3]};
+        philox4x32_10(c, key[0], key[1]);
+        for (int k = 0; k < 4; ++k) out[4 * i + k] = c[k];
+    }
+}
+
+static inline float unit_f(uint32_t x) { return (float)(x >> 8) * 0x1p-24f; }
+
+/* cos / sin of 2 pi x, x in [0,1): quadrant + Taylor polynomials on [0, pi/2) (Horner, fmaf) */
+static void cos_sin_2pi(float x, float* c, float* s) {
+    float x4 = x * 4.0f;
+    int q = (int)x4;
+    float a = (x4 - (float)q) * 1.57079632679489662f;
+    float a2 = a * a;
+    float sp = fmaf(a2, -2.50521083854417188e-8f, 2.75573192239858907e-6f);
+    sp = fmaf(a2, sp, -1.98412698412698413e-4f);
+    sp = fmaf(a2, sp, 8.33333333333333333e-3f);
+    sp = fmaf(a2, sp, -1.66666666666666667e-1f);
+    sp = fmaf(a2, sp, 1.0f);
+    float sa = a * sp;
+    float cp = fmaf(a2, 2.08767569878680990e-9f, -2.75573192239858907e-7f);
+    cp = fmaf(a2, cp, 2.48015873015873016e-5f);
+    cp = fmaf(a2, cp, -1.38888888888888889e-3f);
+    cp = fmaf(a2, cp, 4.16666666666666667e-2f);
+    cp = fmaf(a2, cp, -0.5f);
+    float ca = fmaf(a2, cp, 1.0f);
+    *c = q == 0 ? ca : q == 1 ? -sa : q == 2 ? -ca : sa;
+    *s = q == 0 ? sa : q == 1 ? ca : q == 2 ? -sa : -ca;
+}
+
+TW_API void tw_cos_sin_2pi(int64_t n, const float* x, float* c, float* s) {
+    for (int64_t i = 0; i < n; ++i) cos_sin_2pi(x[i], &c[i], &s[i]);
+}
+
+/* cosine-weighted direction about unit n: Malley's method in the Duff et al. 2017 basis */
+static void cosine_dir(const float n[3], float u0, float u1, float d[3]) {
+    float rad = sqrtf(u0), cs, sn;
+    cos_sin_2pi(u1, &cs, &sn);
+    float lx = rad * cs, ly = rad * sn, lz = sqrtf(fmaxf(0.0f, 1.0f - u0));
+    float sg = copysignf(1.0f, n[2]);
+    float a = -1.0f / (sg + n[2]);
+    float b = n[0] * n[1] * a;
+    float tx = fmaf(sg * n[0] * n[0], a, 1.0f), ty = sg * b, tz = -sg * n[0];
+    float bx = b, by = fmaf(n[1] * n[1], a, sg), bz = -n[1];
+    float x = fmaf(lx, tx, fmaf(ly, bx, lz * n[0]));
+    float y = fmaf(lx, ty, fmaf(ly, by, lz * n[1]));
+    float z = fmaf(lx, tz, fmaf(ly, bz, lz * n[2]));
+    float il = 1.0f / sqrtf(fmaf(x, x, fmaf(y, y, z * z)));
+    d[0] = x * il; d[1] = y * il; d[2] = z * il;
+}
+
+TW_API void tw_cosine_dir(int64_t n, const float* nrm, const float* u0, const float* u1, float* d) {
+    for (int64_t i = 0; i < n; ++i) cosine_dir(nrm + 3 * i, u0[i], u1[i], d + 3 * i);
+}
+
+/* unit geometric normal e1 x e2 of a triangle record, facing against direction d */
+static void facing_normal(const float* tr, const float d[3], float n[3]) {
+    float x = fmaf(tr[5], tr[10], -(tr[6] * tr[9]));
+    float y = fmaf(tr[6], tr[8], -(tr[4] * tr[10]));
+    float z = fmaf(tr[4], tr[9], -(tr[5] * tr[8]));
+    float il = 1.0f / sqrtf(fmaf(x, x, fmaf(y, y, z * z)));
+    x *= il; y *= il; z *= il;
+    int flip = fmaf(x, d[0], fmaf(y, d[1], z * d[2])) > 0.0f;
+    n[0] = flip ? -x : x; n[1] = flip ? -y : y; n[2] = flip ? -z : z;
+}
+
+/* rt_trace_paths: per pixel and sample, the jittered camera ray and up to `bounces` diffuse
+ * bounces (closest hits by the per-ray traversal, FP64-refined); samples[pixel*spp+s] = sky *
+ * albedo^k for a path escaping after k bounces, else 0; radiance = mean over samples in order;
+ * counts[k] = rays traced at depth k (k = 0..bounces). */
+TW_API void tw_paths(const float* nodes, const float* tris, int32_t root, const float* cam,
+                     uint32_t W, uint32_t H, uint32_t spp, uint32_t bounces, uint32_t key0,
+                     uint32_t key1, float albedo, float sky, float tmin, float* samples,
+                     float* radiance, int64_t* counts) {
+    int64_t npx = (int64_t)W * H;
+    int64_t cnt[17] = {0};
+    #pragma omp parallel for schedule(dynamic, 64) reduction(+:cnt[0:17])
+    for (int64_t p = 0; p < npx; ++p) {
+        uint32_t col = (uint32_t)(p % W), row = (uint32_t)(p / W);
+        float sum = 0.0f;
+        for (uint32_t s = 0; s < spp; ++s) {
+            uint32_t c[4] = {(uint32_t)p, s, 0u, 0u};
+            philox4x32_10(c, key0, key1);
+            float fx = (float)col + unit_f(c[0]), fy = (float)row + unit_f(c[1]);
+            float nx = fx / (float)W * 2.0f - 1.0f, ny = 1.0f - fy / (float)H * 2.0f;
+            float x = fmaf(nx, cam[3], fmaf(ny, cam[6], cam[9]));
+            float y = fmaf(nx, cam[4], fmaf(ny, cam[7], cam[10]));
+            float z = fmaf(nx, cam[5], fmaf(ny, cam[8], cam[11]));
+            float il = 1.0f / sqrtf(fmaf(x, x, fmaf(y, y, z * z)));
+            ray32 r = {cam[0], cam[1], cam[2], x * il, y * il, z * il, 0.0f, INFINITY};
+            hit32 h;
+            traverse32(nodes, tris, root, &r, 0, -1, &h);
+            cnt[0]++;
+            float val = 0.0f, thr = 1.0f;
+            if (h.prim < 0) {
+                val = sky;
+            } else {
+                refine64(tris, &r, &h);
+                for (uint32_t k = 1; k <= bounces; ++k) {
+                    const float* tr = tris + (int64_t)h.slot * 12;
+                    float d[3] = {r.dx, r.dy, r.dz}, n[3], nd[3];
+                    facing_normal(tr, d, n);
+                    uint32_t c2[4] = {(uint32_t)p, s, k, 0u};
+                    philox4x32_10(c2, key0, key1);
+                    cosine_dir(n, unit_f(c2[0]), unit_f(c2[1]), nd);
+                    float ox = fmaf(h.t, r.dx, r.ox), oy = fmaf(h.t, r.dy, r.oy), oz = fmaf(h.t, r.dz, r.oz);
+                    thr = thr * albedo;
+                    int32_t skip = h.slot;
+                    ray32 rb = {ox, oy, oz, nd[0], nd[1], nd[2], tmin, INFINITY};
+                    r = rb;
+                    traverse32(nodes, tris, root, &r, 0, skip, &h);
+                    cnt[k]++;
+                    if (h.prim < 0) { val = thr * sky; break; }
+                    refine64(tris, &r, &h);
+                }
+            }
+            if (samples) samples[p * spp + s] = val;
+            sum += val;
+        }
+        radiance[p] = sum / (float)spp;
+    }
+    for (uint32_t k = 0; k <= bounces && k < 17; ++k) counts[k] = cnt[k];
+}

@@ -12,7 +12,7 @@ import torch  # noqa: F401  (must be imported before the HIP library; see module
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "librtx_hip.so")
 
-RT_ABI_VERSION = 6
+RT_ABI_VERSION = 7
 RT_ABSENT = 0xFFFFFFFF
 RT_EMPTY_ROOT = 0x7FFFFFFF
 RT_RENDER_SHADOWS = 1
@@ -24,6 +24,7 @@ RT_TRAVERSAL_AUTO = 3
 RT_LANE_STACK_MAX = 32
 RT_BATCH_MAX_JOBS = 4096
 RT_BATCH_MAX_JOB_PIXELS = 1 << 24
+RT_PATHS_MAX_BOUNCES = 16
 
 _c_float_p = ctypes.POINTER(ctypes.c_float)
 _c_i32_p = ctypes.POINTER(ctypes.c_int32)
@@ -159,6 +160,27 @@ class rt_batch_config(ctypes.Structure):
     ]
 
 
+class rt_paths_params(ctypes.Structure):
+    _fields_ = [
+        ("width", ctypes.c_uint32),
+        ("height", ctypes.c_uint32),
+        ("spp", ctypes.c_uint32),
+        ("bounces", ctypes.c_uint32),
+        ("seed", ctypes.c_uint32),
+        ("frame", ctypes.c_uint32),
+        ("albedo", ctypes.c_float),
+        ("sky", ctypes.c_float),
+        ("ray_tmin", ctypes.c_float),
+    ]
+
+
+class rt_paths_out(ctypes.Structure):
+    _fields_ = [
+        ("radiance", ctypes.c_void_p),
+        ("counts", ctypes.c_void_p),
+    ]
+
+
 # (name, restype, argtypes) of every exported entry point
 SIGNATURES = [
     ("rt_abi_version", ctypes.c_int, []),
@@ -190,6 +212,10 @@ SIGNATURES = [
      [ctypes.c_void_p, ctypes.c_uint32, ctypes.POINTER(rt_batch_config), ctypes.c_void_p,
       ctypes.c_uint64, ctypes.c_void_p]),
     ("rt_render_batch_status", ctypes.c_void_p, [ctypes.c_void_p]),
+    ("rt_paths_workspace_bytes", ctypes.c_uint64, [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32]),
+    ("rt_trace_paths", ctypes.c_int,
+     [ctypes.POINTER(rt_scene), ctypes.POINTER(rt_camera), ctypes.POINTER(rt_paths_params),
+      ctypes.POINTER(rt_paths_out), ctypes.c_void_p, ctypes.c_uint64, ctypes.c_void_p]),
 ]
 
 _lib = None

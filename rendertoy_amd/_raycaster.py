@@ -350,6 +350,31 @@ class Raycaster:
                                     ctypes.byref(h), N.stream_handle(self._stream)), "rt_trace_camera")
         return out
 
+    def trace_paths(self, camera: Camera, width: int, height: int, spp: int = 4, bounces: int = 2,
+                    seed: int = 0x5EED, frame: int = 0, albedo: float = 0.7, sky: float = 1.0,
+                    ray_tmin: float = 1e-4, out=None, counts=None):
+        """Diffuse path tracing under a uniform sky (rt_trace_paths): `spp` jittered camera rays
+        per pixel, each followed by up to `bounces` cosine-weighted bounces (Philox4x32-10 with key
+        (seed, frame)); returns the radiance image [height, width] float32. `counts` (optional
+        int32 tensor of bounces+1) receives the rays traced at each depth."""
+        lib = N.require_gpu()
+        dev = torch.device("cuda")
+        p = N.rt_paths_params(width, height, spp, bounces, seed & 0xFFFFFFFF, frame & 0xFFFFFFFF,
+                              float(albedo), float(sky), float(ray_tmin))
+        if out is None:
+            out = torch.empty((height, width), dtype=torch.float32, device=dev)
+        elif out.dtype != torch.float32 or out.numel() < width * height or not out.is_contiguous():
+            raise TypeError("out must be a contiguous float32 tensor of height*width elements")
+        need = lib.rt_paths_workspace_bytes(width, height, spp)
+        if getattr(self, "_paths_ws", None) is None or self._paths_ws.numel() < need:
+            self._paths_ws = torch.empty(need, dtype=torch.uint8, device=dev)
+        o = N.rt_paths_out(out.data_ptr(), counts.data_ptr() if counts is not None else None)
+        cam = camera.to_c()
+        N.check(lib.rt_trace_paths(ctypes.byref(self._scene), ctypes.byref(cam), ctypes.byref(p),
+                                   ctypes.byref(o), self._paths_ws.data_ptr(), self._paths_ws.numel(),
+                                   N.stream_handle(self._stream)), "rt_trace_paths")
+        return out
+
     def _render_workspace(self, n_pixels):
         lib = N.load()
         need = lib.rt_render_workspace_bytes(n_pixels)

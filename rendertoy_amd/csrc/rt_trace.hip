@@ -13,7 +13,7 @@
 //   each hit's own triangle), RGBA8 texture fetch, RGB float32 store.
 #include <algorithm>
 
-#include "rt_common.h"
+#include "rt_trace_shared.h"
 
 namespace rt {
 namespace {
@@ -36,16 +36,6 @@ RT_DEV int32_t mesh_of_prim(const int32_t* __restrict__ offs, uint32_t n_meshes,
     }
     return (int32_t)lo;
 }
-
-struct SceneArgs {
-    const float4* nodes;
-    const float4* nodes4;
-    const float4* tris;
-    const float4* shade;
-    const int32_t* mesh_offsets;
-    int32_t root;
-    uint32_t n_meshes;
-};
 
 RT_DEV Ray load_ray(const rt_rays& rays, uint64_t i) {
     Ray r;
@@ -179,76 +169,6 @@ RT_DEV Ray camera_ray(const rt_camera& cam, uint32_t W, uint32_t H, const TilePi
     r.tmin = 0.0f;
     r.tmax = INFINITY;
     return r;
-}
-
-// Traversal of one ray per lane by one of three algorithms (MODE = RT_TRAVERSAL_*):
-//  PACKET  the wave's rays walk one node sequence together (traverse_packet);
-//  LANE    each lane walks its own ray with a per-lane stack of `lane_stack` entries in LDS at `stk`
-//          (entries `stk_stride` words apart); a ray that overflows it is traced again by its wave
-//          with the packet traversal (128-entry stack, enough for any tree);
-//  HYBRID  packet traversal for at most `budget` steps; if the wave is not done by then, each lane
-//          continues alone from the root with its best hit so far as the bound (as LANE).
-// All three return the same hit: the closest hit does not depend on the visiting order, and a
-// restart bounded by a hit of the same ray visits every box that could hold a closer one.
-// Called by all 64 lanes of a wave with uniform control flow.
-template <bool ANY, int MODE>
-RT_DEV void trace_block(const SceneArgs& sc, const Ray& r, bool want, int32_t skip_slot, Hit& h,
-                        int32_t* stk, int stk_stride, int lane_stack, int budget) {
-    static_assert(MODE >= RT_TRAVERSAL_PACKET && MODE <= RT_TRAVERSAL_HYBRID, "resolved mode");
-    if (MODE == RT_TRAVERSAL_PACKET) {
-        traverse_packet<ANY>(sc.nodes4, sc.tris, sc.root, r, want, skip_slot, h);
-        return;
-    }
-    bool walk = want, fresh = true;
-    if (MODE == RT_TRAVERSAL_HYBRID) {
-        if (traverse_packet<ANY>(sc.nodes4, sc.tris, sc.root, r, want, skip_slot, h, budget)) return;
-        walk = want && !(ANY && h.prim >= 0);  // an occluded shadow ray is done
-        fresh = false;
-    } else {
-        h.t = r.tmax;
-        h.u = 0.0f;
-        h.v = 0.0f;
-        h.prim = -1;
-        h.slot = -1;
-    }
-    bool ok = true;
-    if (walk)
-        ok = traverse_lane4<ANY>(sc.nodes4, sc.tris, sc.root, r, skip_slot, h, stk, stk_stride,
-                                 lane_stack, fresh);
-    const bool redo = !ok;
-    if (__ballot(redo)) {
-        Hit h2;
-        traverse_packet<ANY>(sc.nodes4, sc.tris, sc.root, r, redo, skip_slot, h2);
-        if (redo) h = h2;
-    }
-}
-
-constexpr bool uses_lds(int mode) { return mode != RT_TRAVERSAL_PACKET; }
-
-// RT_TRAVERSAL_AUTO for camera rays: the packet traversal when the 8x8 tile of a wave spans at most
-// kCoherentTile in the image plane at unit distance (8 pixels x 2 tan(fov/2) / height, or the
-// width's counterpart), where its rays share most of their nodes; per-lane beyond.
-constexpr float kCoherentTile = 0.009f;
-
-__host__ __device__ inline bool coherent_tiles(const rt_camera& c, uint32_t W, uint32_t H) {
-    const float up = sqrtf(c.up[0] * c.up[0] + c.up[1] * c.up[1] + c.up[2] * c.up[2]);
-    const float rt = sqrtf(c.right[0] * c.right[0] + c.right[1] * c.right[1] + c.right[2] * c.right[2]);
-    const float fy = H ? 16.0f * up / (float)H : 0.0f, fx = W ? 16.0f * rt / (float)W : 0.0f;
-    return (fx > fy ? fx : fy) <= kCoherentTile;
-}
-
-// Camera-ray traversal with AUTO resolved per wave from `coherent` (uniform).
-template <int PM>
-RT_DEV void trace_primary(const SceneArgs& sc, const Ray& r, bool want, Hit& h, int32_t* stk,
-                          int stk_stride, int lane_stack, int budget, bool coherent) {
-    if constexpr (PM == RT_TRAVERSAL_AUTO) {
-        if (coherent)
-            trace_block<false, RT_TRAVERSAL_PACKET>(sc, r, want, -1, h, stk, stk_stride, lane_stack, budget);
-        else
-            trace_block<false, RT_TRAVERSAL_LANE>(sc, r, want, -1, h, stk, stk_stride, lane_stack, budget);
-    } else {
-        trace_block<false, PM>(sc, r, want, -1, h, stk, stk_stride, lane_stack, budget);
-    }
 }
 
 // Camera closest hit: ray generation fused into the traversal, one hit record per pixel.
@@ -540,38 +460,12 @@ __global__ void k_zero_u32(uint32_t* c, uint32_t n) {
     if (threadIdx.x < n) c[threadIdx.x] = 0;
 }
 
-SceneArgs scene_args(const rt_scene* s) {
-    SceneArgs a;
-    a.nodes = (const float4*)s->nodes;
-    a.nodes4 = (const float4*)s->nodes4;
-    a.tris = (const float4*)s->tris;
-    a.shade = (const float4*)s->shade;
-    a.mesh_offsets = s->mesh_offsets;
-    a.root = s->root;
-    a.n_meshes = s->n_meshes;
-    return a;
-}
-
-int check_scene(const rt_scene* s, const char* what) {
-    if (!s) return set_error(RT_EINVAL, std::string(what) + ": scene is NULL");
-    if (s->root != RT_EMPTY_ROOT && (!s->nodes || !s->nodes4 || !s->tris || !s->shade || !s->mesh_offsets))
-        return set_error(RT_EINVAL, std::string(what) + ": scene arrays are NULL");
-    if (s->root != RT_EMPTY_ROOT && s->n_tris == 0)
-        return set_error(RT_EINVAL, std::string(what) + ": scene root set but n_tris is 0");
-    return RT_OK;
-}
-
 int check_rays(const rt_rays* r, const char* what) {
     if (!r || !r->ox || !r->oy || !r->oz || !r->dx || !r->dy || !r->dz)
         return set_error(RT_EINVAL, std::string(what) + ": ray field pointer is NULL");
     if (r->stride < 4 || (r->stride & 3))
         return set_error(RT_EINVAL, std::string(what) + ": ray stride must be a multiple of 4");
     return RT_OK;
-}
-
-int resolve_primary(int mode, const rt_camera& cam, uint32_t W, uint32_t H) {
-    if (mode != RT_TRAVERSAL_AUTO) return mode;
-    return coherent_tiles(cam, W, H) ? RT_TRAVERSAL_PACKET : RT_TRAVERSAL_LANE;
 }
 
 inline uint32_t blocks_for(uint64_t n, uint32_t per) { return (uint32_t)((n + per - 1) / per); }

@@ -79,6 +79,8 @@ int main(void) {
     S(rt_batch_config) O(rt_batch_config, tiles_per_grab) O(rt_batch_config, primary)
     O(rt_batch_config, shadow) O(rt_batch_config, lane_stack) O(rt_batch_config, packet_budget)
     O(rt_tuning, packet_budget)
+    S(rt_paths_params) O(rt_paths_params, spp) O(rt_paths_params, seed) O(rt_paths_params, albedo)
+    O(rt_paths_params, ray_tmin) S(rt_paths_out) O(rt_paths_out, counts)
     return 0;
 }
 """
@@ -99,7 +101,7 @@ def test_ctypes_layouts_match_the_c_header(tmp_path):
         c[k] = int(v)
     for name in ("rt_mesh_desc", "rt_scene", "rt_scene_sizes", "rt_rays", "rt_hits", "rt_camera",
                  "rt_texture", "rt_render_params", "rt_render_out", "rt_tuning", "rt_render_job",
-                 "rt_batch_config"):
+                 "rt_batch_config", "rt_paths_params", "rt_paths_out"):
         cls = getattr(N, name)
         assert ctypes.sizeof(cls) == c[name], name
         for key, off in c.items():
@@ -147,3 +149,29 @@ def test_batch_entry_rejects_bad_arguments_without_gpu():
     assert lib.rt_render_batch(fake_jobs, 10, ctypes.byref(cfg), wsp, need - 1, None) == -1
     assert b"workspace" in lib.rt_last_error()
     assert lib.rt_render_batch(fake_jobs, 0, ctypes.byref(cfg), None, 0, None) == 0  # nothing to do
+
+
+def test_paths_entry_rejects_bad_arguments_without_gpu():
+    from rendertoy_amd import _native as N
+    lib = N.load()
+    sc = N.rt_scene()
+    sc.root = N.RT_EMPTY_ROOT
+    cam = N.rt_camera()
+    out = N.rt_paths_out(0x1000, None)
+    ws = 0x10000
+    ok = N.rt_paths_params(8, 8, 4, 2, 1, 0, 0.7, 1.0, 1e-4)
+    need = lib.rt_paths_workspace_bytes(8, 8, 4)
+    assert need == 256 + 4 * 256 + 2 * 48 * 256  # counters, samples, two queues
+    assert lib.rt_trace_paths(None, ctypes.byref(cam), ctypes.byref(ok), ctypes.byref(out), ws, need, None) == -1
+    for bad in (N.rt_paths_params(8, 8, 0, 2, 1, 0, 0.7, 1.0, 1e-4),
+                N.rt_paths_params(8, 8, 4, 17, 1, 0, 0.7, 1.0, 1e-4)):
+        assert lib.rt_trace_paths(ctypes.byref(sc), ctypes.byref(cam), ctypes.byref(bad),
+                                  ctypes.byref(out), ws, need, None) == -1
+    assert lib.rt_trace_paths(ctypes.byref(sc), ctypes.byref(cam), ctypes.byref(ok), ctypes.byref(out),
+                              ws, need - 1, None) == -1
+    assert lib.rt_trace_paths(ctypes.byref(sc), ctypes.byref(cam), ctypes.byref(ok), ctypes.byref(out),
+                              ws + 16, need, None) == -1
+    assert b"256-byte" in lib.rt_last_error()
+    big = N.rt_paths_params(65536, 65536, 1, 2, 1, 0, 0.7, 1.0, 1e-4)
+    assert lib.rt_trace_paths(ctypes.byref(sc), ctypes.byref(cam), ctypes.byref(big), ctypes.byref(out),
+                              ws, need, None) == -3
