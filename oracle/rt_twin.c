@@ -344,6 +344,8 @@ static inline float safe_inv(float d) {
 /* 1 + 2*gamma(3), gamma(n) = n*eps/(1-n*eps), eps = 2^-24, rounded up (Ize 2013). */
 #define TW_TFAR_SCALE 1.0000004f
 
+/* Two-sided Moller-Trumbore; the barycentric tests compare the numerators with the determinant
+ * after multiplying both by sign(det) (exact), the division happens only for a hit. */
 static inline int mt32(const float* tr, const ray32* r, float* t, float* u, float* v) {
     float e1x = tr[4], e1y = tr[5], e1z = tr[6];
     float e2x = tr[8], e2y = tr[9], e2z = tr[10];
@@ -352,18 +354,21 @@ static inline int mt32(const float* tr, const ray32* r, float* t, float* u, floa
     float pz = fmaf(r->dx, e2y, -(r->dy * e2x));
     float det = fmaf(e1x, px, fmaf(e1y, py, e1z * pz));
     if (det == 0.0f) return 0;
-    float inv = 1.0f / det;
+    float sg = det > 0.0f ? 1.0f : -1.0f, ad = det * sg;
     float sx = r->ox - tr[0], sy = r->oy - tr[1], sz = r->oz - tr[2];
-    float uu = fmaf(sx, px, fmaf(sy, py, sz * pz)) * inv;
-    if (uu < 0.0f || uu > 1.0f) return 0;
+    float un = fmaf(sx, px, fmaf(sy, py, sz * pz));
+    float us = un * sg;
+    if (us < 0.0f || us > ad) return 0;
     float qx = fmaf(sy, e1z, -(sz * e1y));
     float qy = fmaf(sz, e1x, -(sx * e1z));
     float qz = fmaf(sx, e1y, -(sy * e1x));
-    float vv = fmaf(r->dx, qx, fmaf(r->dy, qy, r->dz * qz)) * inv;
-    if (vv < 0.0f || uu + vv > 1.0f) return 0;
+    float vn = fmaf(r->dx, qx, fmaf(r->dy, qy, r->dz * qz));
+    float vs = vn * sg;
+    if (vs < 0.0f || us + vs > ad) return 0;
+    float inv = 1.0f / det;
     *t = fmaf(e2x, qx, fmaf(e2y, qy, e2z * qz)) * inv;
-    *u = uu;
-    *v = vv;
+    *u = un * inv;
+    *v = vn * inv;
     return 1;
 }
 
@@ -599,6 +604,12 @@ typedef struct {
                                                         triangle records fetched */
 } wavecount;
 
+/* The device's packet traversal (traverse_packet), lane-exact: node steps test the 4 children of
+ * the current node for the lanes in the mask; hit leaf children are tested in the same step (per
+ * lane in the child's mask, in child order); internal children are ordered by the lowest active
+ * lane's entry distances (sorting network on (key, k)), the nearest is visited next, the others are
+ * pushed; with no internal child hit the next node is popped before the leaf tests (any-hit:
+ * entries whose lanes have all found an occluder are skipped, re-checked at the next step). */
 static void packet32(const float* nodes4, const float* tris, int32_t root, const ray32* r,
                      const int* want, const int32_t* skip, int any, hit32* h, wavecount* wc) {
     uint64_t m = 0;
@@ -611,88 +622,123 @@ static void packet32(const float* nodes4, const float* tris, int32_t root, const
         oxx[l] = r[l].ox * idx[l]; oyy[l] = r[l].oy * idy[l]; ozz[l] = r[l].oz * idz[l];
     }
     if (root == 0x7FFFFFFF || m == 0) return;
+    if (root < 0) {  /* single-leaf tree */
+        uint32_t first = leaf_first(root), count = leaf_count(root);
+        if (wc) { wc->leaf_visits++; wc->tri_fetches += count; }
+        for (uint32_t k = 0; k < count; ++k) {
+            int32_t slot = (int32_t)(first + k);
+            const float* tr = tris + (int64_t)slot * 12;
+            for (int l = 0; l < 64; ++l) {
+                if (!((m >> l) & 1) || slot == skip[l]) continue;
+                float t, u, v;
+                h[l].tri_tests++;
+                if (mt32(tr, &r[l], &t, &u, &v) && t > r[l].tmin) {
+                    int32_t prim = (int32_t)fbits(tr[3]);
+                    if (t < h[l].t || (t == h[l].t && prim < h[l].prim)) {
+                        h[l].t = t; h[l].u = u; h[l].v = v; h[l].prim = prim; h[l].slot = slot;
+                    }
+                }
+            }
+        }
+        return;
+    }
     uint64_t alive = m;
     int32_t stk_node[128];
     uint64_t stk_m[128];
     int sp = 0;
     int32_t cur = root;
     for (;;) {
-        if (cur >= 0) {
-            const float* nd = nodes4 + (int64_t)cur * 32;
-            if (wc) wc->node_visits++;
-            int rep = __builtin_ctzll(m);
-            float key[4];
-            uint64_t mk[4];
-            int32_t ref[4];
+        if (any) {
+            m &= alive;
+            while (m == 0) {
+                if (sp == 0) return;
+                --sp;
+                cur = stk_node[sp];
+                m = stk_m[sp] & alive;
+            }
+        }
+        const float* nd = nodes4 + (int64_t)cur * 32;
+        if (wc) wc->node_visits++;
+        int rep = __builtin_ctzll(m);
+        float key[4];
+        uint64_t mk[4];
+        int32_t ref[4];
+        for (int k = 0; k < 4; ++k) {
+            ref[k] = (int32_t)fbits(nd[24 + k]);
+            mk[k] = 0;
+            key[k] = INFINITY;
+        }
+        for (int l = 0; l < 64; ++l) {
+            if (!((m >> l) & 1)) continue;
+            h[l].node_visits++;
             for (int k = 0; k < 4; ++k) {
-                ref[k] = (int32_t)fbits(nd[24 + k]);
-                mk[k] = 0;
-                key[k] = INFINITY;
+                float x0 = fmaf(nd[k], idx[l], -oxx[l]), x1 = fmaf(nd[4 + k], idx[l], -oxx[l]);
+                float y0 = fmaf(nd[8 + k], idy[l], -oyy[l]), y1 = fmaf(nd[12 + k], idy[l], -oyy[l]);
+                float z0 = fmaf(nd[16 + k], idz[l], -ozz[l]), z1 = fmaf(nd[20 + k], idz[l], -ozz[l]);
+                float tn = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), r[l].tmin));
+                float tf = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fmaxf(z0, z1)) * TW_TFAR_SCALE;
+                int hk = ref[k] != 0x7FFFFFFF && tn <= fminf(tf, h[l].t);
+                if (hk) mk[k] |= 1ull << l;
+                if (l == rep) key[k] = hk ? tn : INFINITY;
             }
-            for (int l = 0; l < 64; ++l) {
-                if (!((m >> l) & 1)) continue;
-                h[l].node_visits++;
-                for (int k = 0; k < 4; ++k) {
-                    float x0 = fmaf(nd[k], idx[l], -oxx[l]), x1 = fmaf(nd[4 + k], idx[l], -oxx[l]);
-                    float y0 = fmaf(nd[8 + k], idy[l], -oyy[l]), y1 = fmaf(nd[12 + k], idy[l], -oyy[l]);
-                    float z0 = fmaf(nd[16 + k], idz[l], -ozz[l]), z1 = fmaf(nd[20 + k], idz[l], -ozz[l]);
-                    float tn = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), r[l].tmin));
-                    float tf = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fmaxf(z0, z1)) * TW_TFAR_SCALE;
-                    int hk = ref[k] != 0x7FFFFFFF && tn <= fminf(tf, h[l].t);
-                    if (hk) mk[k] |= 1ull << l;
-                    if (l == rep) key[k] = hk ? tn : INFINITY;
-                }
-            }
-            /* order by (key, k) with the device's sorting network */
-            int ord[4] = {0, 1, 2, 3};
-            const int net[5][2] = {{0, 1}, {2, 3}, {0, 2}, {1, 3}, {1, 2}};
-            for (int e = 0; e < 5; ++e) {
-                int a = net[e][0], b = net[e][1];
-                int ka = ord[a], kb = ord[b];
-                if (key[kb] < key[ka] || (key[kb] == key[ka] && kb < ka)) { ord[a] = kb; ord[b] = ka; }
-            }
-            int32_t next = -1;
-            uint64_t next_m = 0;
-            for (int q = 3; q >= 0; --q) {
-                int k = ord[q];
-                if (!mk[k]) continue;
-                if (next_m && sp < 128) { stk_node[sp] = next; stk_m[sp] = next_m; ++sp; }
-                next = ref[k];
-                next_m = mk[k];
-            }
-            if (next_m) { cur = next; m = next_m; continue; }
-        } else {
-            uint32_t first = leaf_first(cur), count = leaf_count(cur);
+        }
+        /* internal children in (key, k) order by the device's sorting network */
+        int ord[4] = {0, 1, 2, 3};
+        const int net[5][2] = {{0, 1}, {2, 3}, {0, 2}, {1, 3}, {1, 2}};
+        for (int e = 0; e < 5; ++e) {
+            int a = net[e][0], b = net[e][1];
+            int ka = ord[a], kb = ord[b];
+            if (key[kb] < key[ka] || (key[kb] == key[ka] && kb < ka)) { ord[a] = kb; ord[b] = ka; }
+        }
+        int32_t next = -1;
+        uint64_t next_m = 0;
+        for (int q = 3; q >= 0; --q) {
+            int k = ord[q];
+            uint64_t im = ref[k] >= 0 ? mk[k] : 0;
+            if (!im) continue;
+            if (next_m && sp < 128) { stk_node[sp] = next; stk_m[sp] = next_m; ++sp; }
+            next = ref[k];
+            next_m = im;
+        }
+        int more = next_m != 0;
+        while (!more && sp > 0) {  /* pop before the leaf tests, as the device does */
+            --sp;
+            next = stk_node[sp];
+            next_m = stk_m[sp];
+            if (any) next_m &= alive;
+            more = next_m != 0;
+        }
+        /* the hit leaf children, in child order */
+        int tested = 0;
+        for (int k = 0; k < 4; ++k) {
+            if (ref[k] >= 0 || !mk[k]) continue;
+            tested = 1;
+            uint32_t first = leaf_first(ref[k]), count = leaf_count(ref[k]);
             if (wc) { wc->leaf_visits++; wc->tri_fetches += count; }
-            for (uint32_t k = 0; k < count; ++k) {
-                int32_t slot = (int32_t)(first + k);
+            for (uint32_t t = 0; t < count; ++t) {
+                int32_t slot = (int32_t)(first + t);
                 const float* tr = tris + (int64_t)slot * 12;
                 for (int l = 0; l < 64; ++l) {
-                    if (!((m >> l) & 1) || slot == skip[l]) continue;
-                    float t, u, v;
+                    if (!((mk[k] >> l) & 1) || slot == skip[l]) continue;
+                    float tt, u, v;
                     h[l].tri_tests++;
-                    if (mt32(tr, &r[l], &t, &u, &v) && t > r[l].tmin) {
+                    if (mt32(tr, &r[l], &tt, &u, &v) && tt > r[l].tmin) {
                         int32_t prim = (int32_t)fbits(tr[3]);
-                        if (t < h[l].t || (t == h[l].t && prim < h[l].prim)) {
-                            h[l].t = t; h[l].u = u; h[l].v = v; h[l].prim = prim; h[l].slot = slot;
+                        if (tt < h[l].t || (tt == h[l].t && prim < h[l].prim)) {
+                            h[l].t = tt; h[l].u = u; h[l].v = v; h[l].prim = prim; h[l].slot = slot;
                         }
                     }
                 }
             }
-            if (any) {
-                for (int l = 0; l < 64; ++l)
-                    if (h[l].prim >= 0) alive &= ~(1ull << l);
-                if (!alive) return;
-            }
         }
-        for (;;) {
-            if (sp == 0) return;
-            --sp;
-            cur = stk_node[sp];
-            m = stk_m[sp];
-            if (any) m &= alive;
-            if (m) break;
+        if (any && tested) {
+            for (int l = 0; l < 64; ++l)
+                if (h[l].prim >= 0) alive &= ~(1ull << l);
+            if (!alive) return;
         }
+        if (!more) return;
+        cur = next;
+        m = next_m;
     }
 }
 

@@ -28,6 +28,7 @@ namespace rt {
 
 constexpr int kStackDepth = 64;        // >= 62: depth bound of a radix tree over 62-bit keys
 constexpr int kWideStack = 128;        // >= 3 * 31: pushes per level x levels of the 4-wide tree
+constexpr int kPacketLds = 48;         // float4 per wave: triangles of 4 leaves staged by a node step
 constexpr float kTfarScale = 1.0000004f;  // 1 + 2*gamma(3) rounded up (Ize 2013, robust slabs)
 
 // Leaves: a subtree holding at most kLeafMax consecutive slots is one leaf, referenced as
@@ -55,7 +56,10 @@ RT_DEV float safe_inv(float d) {
     return 1.0f / dd;
 }
 
-// Two-sided Moller-Trumbore (Moller & Trumbore 1997) in FP32 over a (v0, e1, e2) record.
+// Two-sided Moller-Trumbore (Moller & Trumbore 1997) in FP32 over a (v0, e1, e2) record. The
+// barycentric tests compare the numerators with the determinant, sign-normalised (u = un/det in
+// [0, 1] <=> 0 <= sg*un <= |det| with sg = sign(det); multiplying by +-1 is exact), so a rejection
+// costs no division; t, u, v are divided out only for a hit.
 RT_DEV bool intersect_tri(const float4 a, const float4 b, const float4 c, const Ray& r, float& t,
                           float& u, float& v) {
     const float px = fmaf(r.dy, c.z, -(r.dz * c.y));
@@ -63,18 +67,21 @@ RT_DEV bool intersect_tri(const float4 a, const float4 b, const float4 c, const 
     const float pz = fmaf(r.dx, c.y, -(r.dy * c.x));
     const float det = fmaf(b.x, px, fmaf(b.y, py, b.z * pz));
     if (det == 0.0f) return false;
-    const float inv = 1.0f / det;
+    const float sg = det > 0.0f ? 1.0f : -1.0f, ad = det * sg;
     const float sx = r.ox - a.x, sy = r.oy - a.y, sz = r.oz - a.z;
-    const float uu = fmaf(sx, px, fmaf(sy, py, sz * pz)) * inv;
-    if (uu < 0.0f || uu > 1.0f) return false;
+    const float un = fmaf(sx, px, fmaf(sy, py, sz * pz));
+    const float us = un * sg;
+    if (us < 0.0f || us > ad) return false;
     const float qx = fmaf(sy, b.z, -(sz * b.y));
     const float qy = fmaf(sz, b.x, -(sx * b.z));
     const float qz = fmaf(sx, b.y, -(sy * b.x));
-    const float vv = fmaf(r.dx, qx, fmaf(r.dy, qy, r.dz * qz)) * inv;
-    if (vv < 0.0f || uu + vv > 1.0f) return false;
+    const float vn = fmaf(r.dx, qx, fmaf(r.dy, qy, r.dz * qz));
+    const float vs = vn * sg;
+    if (vs < 0.0f || us + vs > ad) return false;
+    const float inv = 1.0f / det;
     t = fmaf(c.x, qx, fmaf(c.y, qy, c.z * qz)) * inv;
-    u = uu;
-    v = vv;
+    u = un * inv;
+    v = vn * inv;
     return true;
 }
 
@@ -203,21 +210,27 @@ RT_DEV void traverse(const float4* __restrict__ nodes, const float4* __restrict_
 // Wave-coherent ("packet") traversal of the 4-wide nodes for coherent rays (camera tiles, shadow
 // rays toward one light from a tile's hit points). The 64 lanes walk one node sequence together:
 //  - the current node, the stack pointer and the 64-bit mask of lanes that passed the current
-//    node's box are wave-uniform, so node and triangle records are fetched with scalar loads;
-//  - each lane tests only nodes whose box it passed itself (the mask), with its own t culling,
-//    so a lane's result is that of its own ray;
-//  - children are visited in the order of the entry distances of the lowest lane of the mask
-//    (children that lane misses after those it hits, ties by slot);
+//    node's box are wave-uniform, so node records are fetched with scalar loads;
+//  - each lane tests only boxes and triangles of nodes it entered itself (the mask), with its own
+//    t culling, so a lane's result is that of its own ray;
+//  - the leaf children a node step finds hit are tested in that same step: one vector load stages
+//    the triangles of all of them (at most 4 leaves x 4 triangles x 3 float4 = 48 lanes) in the
+//    wave's 48-float4 LDS area `lds`, and every lane reads them from there. Only internal nodes are
+//    pushed, and the record of the node visited next (the nearest internal child, else the stack
+//    top) is requested before the leaf tests, so its fetch overlaps them: a step costs one memory
+//    round trip;
+//  - internal children are visited in the order of the entry distances of the lowest lane of the
+//    mask (children that lane misses after those it hits, ties by child index);
 //  - the stack is a pair of VGPRs per field, lane k holding entries k and 64+k (push = per-lane
 //    select, pop = v_readlane): at most 3 pushes on each of at most 31 levels.
-// `budget` > 0 caps the node and leaf steps of the wave: the walk stops after that many and returns
-// false (h then holds the best hit found so far, a valid bound for a restart); otherwise it runs to
-// the end and returns true. Must be called by all 64 lanes of the wave with uniform control flow
-// (no lane may have exited).
+// `budget` > 0 caps the node steps of the wave: the walk stops after that many and returns false
+// (h then holds the best hit found so far, a valid bound for a restart); otherwise it runs to the
+// end and returns true. Must be called by all 64 lanes of the wave with uniform control flow (no
+// lane may have exited).
 template <bool ANY>
 RT_DEV bool traverse_packet(const float4* __restrict__ nodes4, const float4* __restrict__ tris,
                             int32_t root, const Ray& r, bool want, int32_t skip_slot, Hit& h,
-                            int budget = 0) {
+                            float4* lds, int budget = 0) {
     h.t = r.tmax;
     h.u = 0.0f;
     h.v = 0.0f;
@@ -225,103 +238,161 @@ RT_DEV bool traverse_packet(const float4* __restrict__ nodes4, const float4* __r
     h.slot = -1;
     uint64_t m = __ballot(want);
     if (root == RT_EMPTY_ROOT || m == 0) return true;
-    int steps = 0;
-    uint64_t alive = m;
     const uint32_t lane = __lane_id();
+    if (root < 0) {  // the whole tree is one leaf
+        test_leaf(tris, root, r, ((m >> lane) & 1u) != 0, skip_slot, h);
+        return true;
+    }
+    uint64_t alive = m;
     const float idx = safe_inv(r.dx), idy = safe_inv(r.dy), idz = safe_inv(r.dz);
     const float oxx = r.ox * idx, oyy = r.oy * idy, ozz = r.oz * idz;
     int32_t stk_node0 = 0, stk_node1 = 0;
     uint32_t stk_mlo0 = 0, stk_mlo1 = 0, stk_mhi0 = 0, stk_mhi1 = 0;
     int32_t sp = 0;
-    int32_t cur = root;
+    // pops entries until one has lanes left (after `alive` for any-hit); false if the stack is empty
+    auto pop = [&](int32_t& node, uint64_t& mask) -> bool {
+        for (;;) {
+            if (sp == 0) return false;
+            --sp;
+            const bool hi = sp >= 64;
+            const int32_t lsp = hi ? sp - 64 : sp;
+            node = __builtin_amdgcn_readlane(hi ? stk_node1 : stk_node0, lsp);
+            const uint32_t mlo = (uint32_t)__builtin_amdgcn_readlane((int)(hi ? stk_mlo1 : stk_mlo0), lsp);
+            const uint32_t mhi = (uint32_t)__builtin_amdgcn_readlane((int)(hi ? stk_mhi1 : stk_mhi0), lsp);
+            mask = ((uint64_t)mhi << 32) | mlo;
+            if (ANY) mask &= alive;
+            if (mask != 0) return true;
+        }
+    };
+    const float4* nd = nodes4 + (size_t)root * 8;
+    float4 lx = nd[0], hx = nd[1], ly = nd[2], hy = nd[3], lz = nd[4], hz = nd[5], rf = nd[6];
+    int steps = 0;
     for (;;) {
+        if (ANY) {
+            m &= alive;
+            if (m == 0) {  // every lane of this entry found its occluder meanwhile
+                int32_t node;
+                if (!pop(node, m)) return true;
+                nd = nodes4 + (size_t)node * 8;
+                lx = nd[0]; hx = nd[1]; ly = nd[2]; hy = nd[3]; lz = nd[4]; hz = nd[5]; rf = nd[6];
+            }
+        }
         if (budget > 0 && ++steps > budget) return false;
-        if (cur >= 0) {
-            const float4* nd = nodes4 + (size_t)cur * 8;
-            const float4 lx = nd[0], hx = nd[1], ly = nd[2], hy = nd[3], lz = nd[4], hz = nd[5];
-            const float4 rf = nd[6];
-            const bool mine = (m >> lane) & 1u;
-            const float clx[4] = {lx.x, lx.y, lx.z, lx.w}, chx[4] = {hx.x, hx.y, hx.z, hx.w};
-            const float cly[4] = {ly.x, ly.y, ly.z, ly.w}, chy[4] = {hy.x, hy.y, hy.z, hy.w};
-            const float clz[4] = {lz.x, lz.y, lz.z, lz.w}, chz[4] = {hz.x, hz.y, hz.z, hz.w};
-            const int32_t ref[4] = {__float_as_int(rf.x), __float_as_int(rf.y), __float_as_int(rf.z),
-                                    __float_as_int(rf.w)};
-            struct Child {
-                float key;
-                int32_t k, ref;
-                uint64_t mask;
-            };
-            Child c[4];
-            const int rep = __ffsll((unsigned long long)m) - 1;
+        const bool mine = (m >> lane) & 1u;
+        const float clx[4] = {lx.x, lx.y, lx.z, lx.w}, chx[4] = {hx.x, hx.y, hx.z, hx.w};
+        const float cly[4] = {ly.x, ly.y, ly.z, ly.w}, chy[4] = {hy.x, hy.y, hy.z, hy.w};
+        const float clz[4] = {lz.x, lz.y, lz.z, lz.w}, chz[4] = {hz.x, hz.y, hz.z, hz.w};
+        const int32_t ref[4] = {__float_as_int(rf.x), __float_as_int(rf.y), __float_as_int(rf.z),
+                                __float_as_int(rf.w)};
+        struct Child {
+            float key;
+            int32_t k, ref;
+            uint64_t mask;
+        };
+        Child c[4];
+        uint64_t lmask[4];
+        const int rep = __ffsll((unsigned long long)m) - 1;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const float x0 = fmaf(clx[k], idx, -oxx), x1 = fmaf(chx[k], idx, -oxx);
+            const float y0 = fmaf(cly[k], idy, -oyy), y1 = fmaf(chy[k], idy, -oyy);
+            const float z0 = fmaf(clz[k], idz, -ozz), z1 = fmaf(chz[k], idz, -ozz);
+            const float tn = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), r.tmin));
+            const float tf = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fmaxf(z0, z1)) * kTfarScale;
+            const bool hk = mine & (ref[k] != RT_EMPTY_ROOT) & (tn <= fminf(tf, h.t));  // no short circuit
+            const uint64_t mk = __ballot(hk);
+            const bool leaf = ref[k] < 0;
+            lmask[k] = leaf ? mk : 0;
+            c[k].mask = leaf ? 0 : mk;
+            c[k].key = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(hk ? tn : INFINITY), rep));
+            c[k].k = k;
+            c[k].ref = ref[k];
+        }
+        // internal children by (key, k): 4-element sorting network over whole records
+        auto cswap = [](Child& a, Child& b) {
+            if (b.key < a.key || (b.key == a.key && b.k < a.k)) {
+                const Child t = a;
+                a = b;
+                b = t;
+            }
+        };
+        cswap(c[0], c[1]);
+        cswap(c[2], c[3]);
+        cswap(c[0], c[2]);
+        cswap(c[1], c[3]);
+        cswap(c[1], c[2]);
+        int32_t next = -1;
+        uint64_t next_m = 0;
+#pragma unroll
+        for (int q = 3; q >= 0; --q) {  // push the farther ones, the nearest is visited next
+            if (c[q].mask == 0) continue;
+            if (next_m != 0 && sp < kWideStack) {
+                const bool at0 = (int32_t)lane == sp, at1 = (int32_t)lane + 64 == sp;
+                stk_node0 = at0 ? next : stk_node0;
+                stk_mlo0 = at0 ? (uint32_t)next_m : stk_mlo0;
+                stk_mhi0 = at0 ? (uint32_t)(next_m >> 32) : stk_mhi0;
+                stk_node1 = at1 ? next : stk_node1;
+                stk_mlo1 = at1 ? (uint32_t)next_m : stk_mlo1;
+                stk_mhi1 = at1 ? (uint32_t)(next_m >> 32) : stk_mhi1;
+                ++sp;
+            }
+            next = c[q].ref;
+            next_m = c[q].mask;
+        }
+        bool more = next_m != 0;
+        if (!more) more = pop(next, next_m);
+        // request the next node's record now: its fetch overlaps the leaf tests below
+        if (more) {
+            nd = nodes4 + (size_t)next * 8;
+            lx = nd[0]; hx = nd[1]; ly = nd[2]; hy = nd[3]; lz = nd[4]; hz = nd[5]; rf = nd[6];
+        }
+        if ((lmask[0] | lmask[1] | lmask[2] | lmask[3]) != 0) {
+            // stage: lane j < 48 fetches float4 (j % 12) of leaf child j / 12
+            const uint32_t kj = lane / 12u, part = lane - kj * 12u;
+            const uint64_t mj = kj == 0 ? lmask[0] : kj == 1 ? lmask[1] : kj == 2 ? lmask[2] : lmask[3];
+            const int32_t rj = kj == 0 ? ref[0] : kj == 1 ? ref[1] : kj == 2 ? ref[2] : ref[3];
+            if (lane < 48u && mj != 0) {
+                const uint32_t t = part / 3u, comp = part - t * 3u, cnt = leaf_count(rj);
+                const uint32_t slot = leaf_first(rj) + (t < cnt ? t : 0u);
+                lds[lane] = tris[(size_t)slot * 3 + comp];
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
-                const float x0 = fmaf(clx[k], idx, -oxx), x1 = fmaf(chx[k], idx, -oxx);
-                const float y0 = fmaf(cly[k], idy, -oyy), y1 = fmaf(chy[k], idy, -oyy);
-                const float z0 = fmaf(clz[k], idz, -ozz), z1 = fmaf(chz[k], idz, -ozz);
-                const float tn = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), r.tmin));
-                const float tf = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fmaxf(z0, z1)) * kTfarScale;
-                const bool hk = mine & (ref[k] != RT_EMPTY_ROOT) & (tn <= fminf(tf, h.t));  // no short circuit: branch free
-                c[k].mask = __ballot(hk);
-                c[k].key = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(hk ? tn : INFINITY), rep));
-                c[k].k = k;
-                c[k].ref = ref[k];
-            }
-            // order by (key, k): 4-element sorting network over whole records (no indexing)
-            auto cswap = [](Child& a, Child& b) {
-                if (b.key < a.key || (b.key == a.key && b.k < a.k)) {
-                    const Child t = a;
-                    a = b;
-                    b = t;
-                }
-            };
-            cswap(c[0], c[1]);
-            cswap(c[2], c[3]);
-            cswap(c[0], c[2]);
-            cswap(c[1], c[3]);
-            cswap(c[1], c[2]);
-            int32_t next = -1;
-            uint64_t next_m = 0;
-            // push in reverse order so the nearest child after the first pops first
+                if (lmask[k] == 0) continue;
+                const bool act = (lmask[k] >> lane) & 1u;
+                const uint32_t first = leaf_first(ref[k]), cnt = leaf_count(ref[k]);
 #pragma unroll
-            for (int q = 3; q >= 0; --q) {
-                if (c[q].mask == 0) continue;
-                if (next_m != 0 && sp < kWideStack) {
-                    const bool at0 = (int32_t)lane == sp, at1 = (int32_t)lane + 64 == sp;
-                    stk_node0 = at0 ? next : stk_node0;
-                    stk_mlo0 = at0 ? (uint32_t)next_m : stk_mlo0;
-                    stk_mhi0 = at0 ? (uint32_t)(next_m >> 32) : stk_mhi0;
-                    stk_node1 = at1 ? next : stk_node1;
-                    stk_mlo1 = at1 ? (uint32_t)next_m : stk_mlo1;
-                    stk_mhi1 = at1 ? (uint32_t)(next_m >> 32) : stk_mhi1;
-                    ++sp;
+                for (uint32_t t = 0; t < (uint32_t)kLeafMax; ++t) {
+                    if (t >= cnt) break;
+                    const float4 ta = lds[k * 12 + t * 3], tb = lds[k * 12 + t * 3 + 1],
+                                 tc = lds[k * 12 + t * 3 + 2];
+                    const int32_t slot = (int32_t)(first + t);
+                    float tt, u, v;
+                    if (act && slot != skip_slot && intersect_tri(ta, tb, tc, r, tt, u, v) && tt > r.tmin) {
+                        const int32_t prim = __float_as_int(ta.w);
+                        if (tt < h.t || (tt == h.t && prim < h.prim)) {
+                            h.t = tt;
+                            h.u = u;
+                            h.v = v;
+                            h.prim = prim;
+                            h.slot = slot;
+                        }
+                    }
                 }
-                next = c[q].ref;
-                next_m = c[q].mask;
             }
-            if (next_m != 0) {
-                cur = next;
-                m = next_m;
-                continue;
-            }
-        } else {
-            test_leaf(tris, cur, r, ((m >> lane) & 1u) != 0, skip_slot, h);
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
             if (ANY) {
                 alive &= ~__ballot(h.prim >= 0);
                 if (alive == 0) return true;
             }
         }
-        for (;;) {
-            if (sp == 0) return true;
-            --sp;
-            const bool hi = sp >= 64;
-            const int32_t lsp = hi ? sp - 64 : sp;
-            cur = __builtin_amdgcn_readlane(hi ? stk_node1 : stk_node0, lsp);
-            const uint32_t mlo = (uint32_t)__builtin_amdgcn_readlane((int)(hi ? stk_mlo1 : stk_mlo0), lsp);
-            const uint32_t mhi = (uint32_t)__builtin_amdgcn_readlane((int)(hi ? stk_mhi1 : stk_mhi0), lsp);
-            m = ((uint64_t)mhi << 32) | mlo;
-            if (ANY) m &= alive;
-            if (m != 0) break;
-        }
+        if (!more) return true;
+        m = next_m;
     }
 }
 

@@ -143,6 +143,7 @@ __global__ void __launch_bounds__(256) k_paths_primary(SceneArgs sc, rt_camera c
                                                         uint32_t tiles_x, int lane_stack, int budget,
                                                         bool coherent) {
     __shared__ int32_t stk[uses_lds(PM) ? kLaneStack * 256 : 1];
+    __shared__ float4 pk[4][kPacketLds];
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const uint32_t col = (blockIdx.x % tiles_x) * 16 + (wave & 1) * 8 + (lane & 7);
     const uint32_t row = (blockIdx.x / tiles_x) * 16 + (wave >> 1) * 8 + (lane >> 3);
@@ -166,7 +167,8 @@ __global__ void __launch_bounds__(256) k_paths_primary(SceneArgs sc, rt_camera c
         r.tmin = 0.0f;
         r.tmax = INFINITY;
         Hit h;
-        trace_primary<PM>(sc, r, valid, h, stk + threadIdx.x, 256, lane_stack, budget, coherent);
+        trace_primary<PM>(sc, r, valid, h, pk[wave], stk + threadIdx.x, 256, lane_stack, budget,
+                          coherent);
         const bool hit = valid && h.prim >= 0;
         if (hit) refine_hit(sc.tris, r, h);
         const uint32_t id = pixel * a.spp + s;
@@ -180,6 +182,7 @@ __global__ void __launch_bounds__(256) k_paths_primary(SceneArgs sc, rt_camera c
 __global__ void __launch_bounds__(64) k_paths_bounce(SceneArgs sc, PathArgs a, uint32_t k,
                                                       int lane_stack) {
     __shared__ int32_t stk[kLaneStack * 64];
+    __shared__ float4 pk[kPacketLds];
     const uint32_t n = a.counts[k];
     const float4* qi = a.q[(k - 1) & 1];
     float4* qo = a.q[k & 1];
@@ -211,7 +214,7 @@ __global__ void __launch_bounds__(64) k_paths_bounce(SceneArgs sc, PathArgs a, u
         r.tmin = a.tmin;
         r.tmax = INFINITY;
         Hit h;
-        trace_block<false, RT_TRAVERSAL_LANE>(sc, r, valid, skip, h, stk + lane, 64, lane_stack, 0);
+        trace_block<false, RT_TRAVERSAL_LANE>(sc, r, valid, skip, h, pk, stk + lane, 64, lane_stack, 0);
         const bool hit = valid && h.prim >= 0;
         if (valid && !hit) a.samples[id] = thr * a.sky;
         if (hit) refine_hit(sc.tris, r, h);

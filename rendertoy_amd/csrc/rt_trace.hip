@@ -177,12 +177,14 @@ __global__ void __launch_bounds__(256) k_trace_camera(SceneArgs sc, rt_camera ca
                                                        uint32_t H, TileMap tm, int lane_stack,
                                                        int budget, rt_hits hits) {
     __shared__ int32_t stk[uses_lds(MODE) ? kLaneStack * 256 : 1];
+    __shared__ float4 pk[4][kPacketLds];
     uint32_t tx, ty;
     if (!block_tile(tm, tx, ty)) return;
     const TilePixel p = tile_pixel(W, H, tx, ty);
     const Ray r = camera_ray(cam, W, H, p);
     Hit h;
-    trace_block<false, MODE>(sc, r, p.valid, -1, h, stk + threadIdx.x, 256, lane_stack, budget);
+    trace_block<false, MODE>(sc, r, p.valid, -1, h, pk[threadIdx.x >> 6], stk + threadIdx.x, 256,
+                             lane_stack, budget);
     if (h.prim >= 0) refine_hit(sc.tris, r, h);
     if (p.valid) store_hit(sc, hits, (uint64_t)p.row * W + p.col, h);
 }
@@ -239,12 +241,12 @@ RT_DEV void tex_sample(const rt_texture& tx, float u, float v, bool bilinear, fl
 template <int PM, int SM>
 RT_DEV void render_pixel(const SceneArgs& sc, const rt_camera& cam, const rt_render_params& p,
                          const TilePixel& px, float* __restrict__ rgb, float* __restrict__ out_t,
-                         int32_t* __restrict__ out_prim, uint32_t* __restrict__ stats, int32_t* stk,
-                         int stk_stride, int lane_stack, int budget, bool coherent) {
+                         int32_t* __restrict__ out_prim, uint32_t* __restrict__ stats, float4* pk,
+                         int32_t* stk, int stk_stride, int lane_stack, int budget, bool coherent) {
     const uint32_t W = p.width, H = p.height;
     const Ray r = camera_ray(cam, W, H, px);
     Hit h;
-    trace_primary<PM>(sc, r, px.valid, h, stk, stk_stride, lane_stack, budget, coherent);
+    trace_primary<PM>(sc, r, px.valid, h, pk, stk, stk_stride, lane_stack, budget, coherent);
     const bool hit = px.valid && h.prim >= 0;
     if (hit) refine_hit(sc.tris, r, h);
 
@@ -278,7 +280,7 @@ RT_DEV void render_pixel(const SceneArgs& sc, const rt_camera& cam, const rt_ren
     sr.tmin = p.shadow_tmin;
     sr.tmax = INFINITY;
     Hit sh;
-    trace_block<true, SM>(sc, sr, want_shadow, hit ? h.slot : -1, sh, stk, stk_stride, lane_stack,
+    trace_block<true, SM>(sc, sr, want_shadow, hit ? h.slot : -1, sh, pk, stk, stk_stride, lane_stack,
                           budget);
     const bool occluded = want_shadow && sh.prim >= 0;
 
@@ -323,11 +325,12 @@ __global__ void __launch_bounds__(256) k_render(SceneArgs sc, rt_camera cam, rt_
                                                  int32_t* __restrict__ out_prim,
                                                  uint32_t* __restrict__ stats) {
     __shared__ int32_t stk[(uses_lds(PM) || uses_lds(SM)) ? kLaneStack * 256 : 1];
+    __shared__ float4 pk[4][kPacketLds];
     uint32_t tx, ty;
     if (!block_tile(tm, tx, ty)) return;
     const TilePixel px = tile_pixel(p.width, p.height, tx, ty);
-    render_pixel<PM, SM>(sc, cam, p, px, rgb, out_t, out_prim, stats, stk + threadIdx.x, 256,
-                         lane_stack, budget, true);
+    render_pixel<PM, SM>(sc, cam, p, px, rgb, out_t, out_prim, stats, pk[threadIdx.x >> 6],
+                         stk + threadIdx.x, 256, lane_stack, budget, true);
 }
 
 // ---- batched render ----
@@ -414,6 +417,7 @@ __global__ void __launch_bounds__(64) k_render_batch(const rt_render_job* __rest
                                                       uint32_t n_jobs, uint32_t* __restrict__ ws,
                                                       uint32_t grab, int lane_stack, int budget) {
     extern __shared__ uint32_t smem[];
+    __shared__ float4 pk[kPacketLds];
     uint32_t* pref = smem;
     int32_t* stk = (int32_t*)(smem + ((n_jobs + 1 + 3) & ~3u));
     const uint32_t lane = threadIdx.x;
@@ -450,7 +454,7 @@ __global__ void __launch_bounds__(64) k_render_batch(const rt_render_job* __rest
             sc.root = job.scene.root;
             sc.n_meshes = job.scene.n_meshes;
             render_pixel<PM, SM>(sc, job.cam, p, px, job.out.rgb, job.out.t, job.out.prim,
-                                 job.out.stats, stk + lane, 64, lane_stack, budget,
+                                 job.out.stats, pk, stk + lane, 64, lane_stack, budget,
                                  coherent_tiles(job.cam, p.width, p.height));
         }
     }

@@ -17,7 +17,8 @@ struct SceneArgs {
     uint32_t n_meshes;
 };
 
-// Traversal of one ray per lane by one of three algorithms (MODE = RT_TRAVERSAL_*):
+// Traversal of one ray per lane by one of three algorithms (MODE = RT_TRAVERSAL_*); `pk` = the
+// wave's kPacketLds-float4 LDS area of the packet traversal:
 //  PACKET  the wave's rays walk one node sequence together (traverse_packet);
 //  LANE    each lane walks its own ray with a per-lane stack of `lane_stack` entries in LDS at `stk`
 //          (entries `stk_stride` words apart); a ray that overflows it is traced again by its wave
@@ -29,15 +30,15 @@ struct SceneArgs {
 // Called by all 64 lanes of a wave with uniform control flow.
 template <bool ANY, int MODE>
 RT_DEV void trace_block(const SceneArgs& sc, const Ray& r, bool want, int32_t skip_slot, Hit& h,
-                        int32_t* stk, int stk_stride, int lane_stack, int budget) {
+                        float4* pk, int32_t* stk, int stk_stride, int lane_stack, int budget) {
     static_assert(MODE >= RT_TRAVERSAL_PACKET && MODE <= RT_TRAVERSAL_HYBRID, "resolved mode");
     if (MODE == RT_TRAVERSAL_PACKET) {
-        traverse_packet<ANY>(sc.nodes4, sc.tris, sc.root, r, want, skip_slot, h);
+        traverse_packet<ANY>(sc.nodes4, sc.tris, sc.root, r, want, skip_slot, h, pk);
         return;
     }
     bool walk = want, fresh = true;
     if (MODE == RT_TRAVERSAL_HYBRID) {
-        if (traverse_packet<ANY>(sc.nodes4, sc.tris, sc.root, r, want, skip_slot, h, budget)) return;
+        if (traverse_packet<ANY>(sc.nodes4, sc.tris, sc.root, r, want, skip_slot, h, pk, budget)) return;
         walk = want && !(ANY && h.prim >= 0);  // an occluded shadow ray is done
         fresh = false;
     } else {
@@ -54,7 +55,7 @@ RT_DEV void trace_block(const SceneArgs& sc, const Ray& r, bool want, int32_t sk
     const bool redo = !ok;
     if (__ballot(redo)) {
         Hit h2;
-        traverse_packet<ANY>(sc.nodes4, sc.tris, sc.root, r, redo, skip_slot, h2);
+        traverse_packet<ANY>(sc.nodes4, sc.tris, sc.root, r, redo, skip_slot, h2, pk);
         if (redo) h = h2;
     }
 }
@@ -75,15 +76,17 @@ __host__ __device__ inline bool coherent_tiles(const rt_camera& c, uint32_t W, u
 
 // Camera-ray traversal with AUTO resolved per wave from `coherent` (uniform).
 template <int PM>
-RT_DEV void trace_primary(const SceneArgs& sc, const Ray& r, bool want, Hit& h, int32_t* stk,
-                          int stk_stride, int lane_stack, int budget, bool coherent) {
+RT_DEV void trace_primary(const SceneArgs& sc, const Ray& r, bool want, Hit& h, float4* pk,
+                          int32_t* stk, int stk_stride, int lane_stack, int budget, bool coherent) {
     if constexpr (PM == RT_TRAVERSAL_AUTO) {
         if (coherent)
-            trace_block<false, RT_TRAVERSAL_PACKET>(sc, r, want, -1, h, stk, stk_stride, lane_stack, budget);
+            trace_block<false, RT_TRAVERSAL_PACKET>(sc, r, want, -1, h, pk, stk, stk_stride, lane_stack,
+                                                    budget);
         else
-            trace_block<false, RT_TRAVERSAL_LANE>(sc, r, want, -1, h, stk, stk_stride, lane_stack, budget);
+            trace_block<false, RT_TRAVERSAL_LANE>(sc, r, want, -1, h, pk, stk, stk_stride, lane_stack,
+                                                  budget);
     } else {
-        trace_block<false, PM>(sc, r, want, -1, h, stk, stk_stride, lane_stack, budget);
+        trace_block<false, PM>(sc, r, want, -1, h, pk, stk, stk_stride, lane_stack, budget);
     }
 }
 
