@@ -53,6 +53,8 @@ struct Workspace {
     int32_t* split;    // n-1: split position g of internal node i (children ranges [lo,g], [g+1,hi])
     uint32_t* flags;   // n-1
     float4* nodebox;   // 2(n-1)
+    int32_t* top_list; // n-1: internal nodes whose leaf range spans several refit blocks
+    uint32_t* top_count;
     uint64_t bytes;
 };
 
@@ -85,6 +87,8 @@ Workspace carve(char* base, uint64_t n) {
     w.split = (int32_t*)take(4 * ni);
     w.flags = (uint32_t*)take(4 * ni);
     w.nodebox = (float4*)take(32 * ni);
+    w.top_list = (int32_t*)take(4 * ni);
+    w.top_count = (uint32_t*)take(4);
     w.bytes = off;
     return w;
 }
@@ -378,6 +382,10 @@ __global__ void __launch_bounds__(kSortThreads) k_sort_scatter(
 
 // ---- hierarchy ----
 
+// Refit blocks: bottom-up refit inside each range of kRefitBlock leaves runs in one workgroup
+// (k_refit_local); the nodes whose range spans several blocks ("top" nodes) are refit afterwards.
+constexpr int kRefitBlock = 2048;
+
 RT_DEV int delta(const uint32_t* __restrict__ k, int64_t n, int64_t i, int64_t j) {
     if (j < 0 || j >= n) return -1;
     const uint32_t a = k[i], b = k[j];
@@ -389,7 +397,9 @@ __global__ void __launch_bounds__(256) k_hierarchy(const uint32_t* __restrict__ 
                                                     float4* __restrict__ nodes,
                                                     int32_t* __restrict__ parent,
                                                     int32_t* __restrict__ rangej,
-                                                    int32_t* __restrict__ split) {
+                                                    int32_t* __restrict__ split,
+                                                    uint32_t* __restrict__ top_count,
+                                                    int32_t* __restrict__ top_list) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= (int64_t)n - 1) return;
     const int d = (delta(keys, n, i, i + 1) - delta(keys, n, i, i - 1)) >= 0 ? 1 : -1;
@@ -424,6 +434,7 @@ __global__ void __launch_bounds__(256) k_hierarchy(const uint32_t* __restrict__ 
     rangej[i] = (int32_t)j;
     split[i] = (int32_t)gamma;
     if (i == 0) parent[0] = -1;
+    if (lo / kRefitBlock != hi / kRefitBlock) top_list[atomicAdd(top_count, 1u)] = (int32_t)i;
 }
 
 __global__ void __launch_bounds__(256) k_emit(const uint32_t* __restrict__ order, uint32_t n,
@@ -448,7 +459,6 @@ __global__ void __launch_bounds__(256) k_emit(const uint32_t* __restrict__ order
 // kRefitBlock consecutive leaves, a node whose range lies in one block is "local": its whole
 // subtree is refit by that block's workgroup, which synchronises only through LDS. The few nodes
 // whose ranges span blocks ("top" nodes) are refit afterwards with device-scope counters.
-constexpr int kRefitBlock = 2048;
 
 // Children of internal node i in the binary radix tree: internal ids, or ~slot for single leaves.
 RT_DEV void binary_children(const int32_t* __restrict__ rangej, const int32_t* __restrict__ split,
@@ -487,10 +497,12 @@ RT_DEV void store_node(float4* __restrict__ nodes, float4* __restrict__ nodebox,
     }
 }
 
-// Phase 1: block b walks up from each of its leaves through the local nodes. The first thread to
-// reach a node stops; the second (LDS counter, workgroup acq_rel) unions the children's boxes,
-// which sit in LDS (local internal children) or in leafbox (leaves). Also clears the device-scope
-// counters of this block's node ids for phase 2.
+// Phase 1: block b walks up from each of its leaves through the local nodes (those whose leaf range
+// lies inside the block; a node's id is an end of its range, so local nodes have ids in the block
+// too). The block first copies what the walks read into LDS: the rangej / split / parent entries of
+// its node ids and the boxes of its leaves (gathered through `order`). The first thread to reach a
+// node stops; the second (LDS counter, workgroup acq_rel) unions the children's boxes from LDS and
+// writes the node. Also clears the counters of this block's node ids for phase 2.
 __global__ void __launch_bounds__(256) k_refit_local(const uint32_t* __restrict__ order, uint32_t n,
                                                       const float4* __restrict__ leafbox,
                                                       const int32_t* __restrict__ parent,
@@ -501,43 +513,55 @@ __global__ void __launch_bounds__(256) k_refit_local(const uint32_t* __restrict_
                                                       float4* __restrict__ nodebox,
                                                       float* __restrict__ bounds) {
     __shared__ uint32_t lflag[kRefitBlock];
-    __shared__ float lbox[kRefitBlock][6];
+    __shared__ float lbox[kRefitBlock][6];   // boxes of the block's internal nodes
+    __shared__ float lleaf[kRefitBlock][6];  // boxes of the block's leaves (slots)
+    __shared__ int32_t lj[kRefitBlock], lsplit[kRefitBlock], lpar[kRefitBlock];
     const int64_t blo = (int64_t)blockIdx.x * kRefitBlock;
     const int64_t bhi = std::min<int64_t>(blo + kRefitBlock, n);  // exclusive
     const int64_t ni = (int64_t)n - 1;
     for (int k = threadIdx.x; k < kRefitBlock; k += blockDim.x) {
         lflag[k] = 0;
-        if (blo + k < ni) flags[blo + k] = 0;
+        if (blo + k < ni) {
+            flags[blo + k] = 0;
+            lj[k] = rangej[blo + k];
+            lsplit[k] = split[blo + k];
+            lpar[k] = parent[blo + k];
+        }
+        if (blo + k < bhi) {
+            const uint64_t p = order[blo + k];
+            const float4 a = leafbox[2 * p], b = leafbox[2 * p + 1];
+            lleaf[k][0] = a.x; lleaf[k][1] = a.y; lleaf[k][2] = a.z;
+            lleaf[k][3] = b.x; lleaf[k][4] = b.y; lleaf[k][5] = b.z;
+        }
     }
     __syncthreads();
     for (int64_t s = blo + threadIdx.x; s < bhi; s += blockDim.x) {
         int32_t node = parent[ni + s];
-        while (node >= 0) {
-            const int64_t j = rangej[node];
+        while (node >= blo && node < bhi && node < ni) {
+            const int32_t li = (int32_t)(node - blo);
+            const int64_t j = lj[li];
             const int64_t lo = std::min<int64_t>(node, j), hi = std::max<int64_t>(node, j);
             if (lo < blo || hi >= bhi) break;  // a top node: phase 2
-            const uint32_t prev = __hip_atomic_fetch_add(&lflag[node - blo], 1u, __ATOMIC_ACQ_REL,
+            const uint32_t prev = __hip_atomic_fetch_add(&lflag[li], 1u, __ATOMIC_ACQ_REL,
                                                          __HIP_MEMORY_SCOPE_WORKGROUP);
             if (prev == 0) break;
-            int32_t l, r;
-            binary_children(rangej, split, node, l, r);
+            const int32_t g = lsplit[li];
+            const int32_t l = lo == g ? ~g : g, r = hi == g + 1 ? ~(g + 1) : g + 1;
+            const float* lb = l < 0 ? lleaf[~l - blo] : lbox[l - blo];
+            const float* rb = r < 0 ? lleaf[~r - blo] : lbox[r - blo];
             float l_lo[3], l_hi[3], r_lo[3], r_hi[3], u_lo[3], u_hi[3];
-            if (l < 0) {
-                leaf_box(leafbox, order, l, l_lo, l_hi);
-            } else {
-                for (int c = 0; c < 3; ++c) { l_lo[c] = lbox[l - blo][c]; l_hi[c] = lbox[l - blo][3 + c]; }
-            }
-            if (r < 0) {
-                leaf_box(leafbox, order, r, r_lo, r_hi);
-            } else {
-                for (int c = 0; c < 3; ++c) { r_lo[c] = lbox[r - blo][c]; r_hi[c] = lbox[r - blo][3 + c]; }
+            for (int c = 0; c < 3; ++c) {
+                l_lo[c] = lb[c];
+                l_hi[c] = lb[3 + c];
+                r_lo[c] = rb[c];
+                r_hi[c] = rb[3 + c];
             }
             store_node(nodes, nodebox, bounds, node, l_lo, l_hi, r_lo, r_hi, u_lo, u_hi);
             for (int c = 0; c < 3; ++c) {
-                lbox[node - blo][c] = u_lo[c];
-                lbox[node - blo][3 + c] = u_hi[c];
+                lbox[li][c] = u_lo[c];
+                lbox[li][3 + c] = u_hi[c];
             }
-            node = parent[node];
+            node = lpar[li];
         }
     }
 }
@@ -560,48 +584,55 @@ RT_DEV void node_box(const float4* __restrict__ leafbox, const uint32_t* __restr
     }
 }
 
-// Phase 2: one thread per top node. Its local children are complete (phase 1); a node with two
-// local children is computed at once, one with a single local child counts that child's arrival
-// on the device-scope counter. The second arrival at a node computes it and continues upward
-// (every ancestor of a top node is a top node).
-__global__ void __launch_bounds__(256) k_refit_top(const uint32_t* __restrict__ order, uint32_t n,
-                                                    const float4* __restrict__ leafbox,
-                                                    const int32_t* __restrict__ parent,
-                                                    const int32_t* __restrict__ rangej,
-                                                    const int32_t* __restrict__ split,
-                                                    uint32_t* __restrict__ flags,
-                                                    float4* __restrict__ nodes, float4* nodebox,
-                                                    float* __restrict__ bounds) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= (int64_t)n - 1) return;
-    int32_t node = (int32_t)i;
-    if (is_local(rangej, node)) return;
-    {
-        int32_t l, r;
-        binary_children(rangej, split, node, l, r);
-        const int n_local = (int)is_local(rangej, l) + (int)is_local(rangej, r);
-        if (n_local == 0) return;
-        if (n_local == 1) {
-            const uint32_t prev = __hip_atomic_fetch_add(&flags[node], 1u, __ATOMIC_ACQ_REL,
-                                                         __HIP_MEMORY_SCOPE_AGENT);
-            if (prev == 0) return;
+// Phase 2, in one workgroup, over the list of top nodes written by k_hierarchy: their local
+// children are complete (phase 1); a node with two local children is computed at once, one with a
+// single local child counts that child's arrival on the node's counter. The second arrival at a
+// node computes it and continues upward (every ancestor of a top node is a top node). All counters
+// are touched by the waves of this one workgroup on one CU, so workgroup-scope atomics order the
+// box stores; an agent-scope release would write the L2 back on every step of the chain, the L2s
+// of the 8 XCDs not being coherent with each other.
+__global__ void __launch_bounds__(1024) k_refit_top(const uint32_t* __restrict__ order, uint32_t n,
+                                                      const float4* __restrict__ leafbox,
+                                                      const int32_t* __restrict__ parent,
+                                                      const int32_t* __restrict__ rangej,
+                                                      const int32_t* __restrict__ split,
+                                                      uint32_t* __restrict__ flags,
+                                                      float4* __restrict__ nodes, float4* nodebox,
+                                                      float* __restrict__ bounds,
+                                                      const int32_t* __restrict__ top_list,
+                                                      const uint32_t* __restrict__ top_count) {
+    const uint32_t cnt = *top_count;
+    for (uint32_t t = threadIdx.x; t < cnt; t += blockDim.x) {
+        int32_t node = top_list[t];
+        {
+            int32_t l, r;
+            binary_children(rangej, split, node, l, r);
+            const int n_local = (int)is_local(rangej, l) + (int)is_local(rangej, r);
+            if (n_local == 0) continue;
+            if (n_local == 1) {
+                const uint32_t prev = __hip_atomic_fetch_add(&flags[node], 1u, __ATOMIC_ACQ_REL,
+                                                             __HIP_MEMORY_SCOPE_WORKGROUP);
+                if (prev == 0) continue;
+            }
+        }
+        for (;;) {
+            int32_t l, r;
+            binary_children(rangej, split, node, l, r);
+            float l_lo[3], l_hi[3], r_lo[3], r_hi[3], u_lo[3], u_hi[3];
+            node_box(leafbox, order, nodebox, l, l_lo, l_hi);
+            node_box(leafbox, order, nodebox, r, r_lo, r_hi);
+            store_node(nodes, nodebox, bounds, node, l_lo, l_hi, r_lo, r_hi, u_lo, u_hi);
+            const int32_t p = parent[node];
+            if (p < 0) break;
+            const uint32_t prev = __hip_atomic_fetch_add(&flags[p], 1u, __ATOMIC_ACQ_REL,
+                                                         __HIP_MEMORY_SCOPE_WORKGROUP);
+            if (prev == 0) break;
+            node = p;
         }
     }
-    for (;;) {
-        int32_t l, r;
-        binary_children(rangej, split, node, l, r);
-        float l_lo[3], l_hi[3], r_lo[3], r_hi[3], u_lo[3], u_hi[3];
-        node_box(leafbox, order, nodebox, l, l_lo, l_hi);
-        node_box(leafbox, order, nodebox, r, r_lo, r_hi);
-        store_node(nodes, nodebox, bounds, node, l_lo, l_hi, r_lo, r_hi, u_lo, u_hi);
-        const int32_t p = parent[node];
-        if (p < 0) return;
-        const uint32_t prev =
-            __hip_atomic_fetch_add(&flags[p], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        if (prev == 0) return;
-        node = p;
-    }
 }
+
+__global__ void k_zero_word(uint32_t* w) { *w = 0u; }
 
 // ---- 4-wide nodes ----
 // nodes4[i] (8 x float4, 128 B) lists the grandchildren of binary node i reached through its
@@ -768,18 +799,20 @@ extern "C" int rt_scene_build(const rt_mesh_desc* meshes, uint32_t n_meshes, rt_
     // after 4 passes the sorted arrays are back in keys0/vals0
     // 5. hierarchy, 6. emit, 7. refit
     float4* nodes = (float4*)sc->nodes;
-    if (n > 1)
+    if (n > 1) {
+        hipLaunchKernelGGL(k_zero_word, dim3(1), dim3(1), 0, s, w.top_count);
         hipLaunchKernelGGL(k_hierarchy, dim3(blocks_for(n - 1, 256)), dim3(256), 0, s, ki, n, nodes,
-                           w.parent, w.rangej, w.split);
+                           w.parent, w.rangej, w.split, w.top_count, w.top_list);
+    }
     hipLaunchKernelGGL(k_emit, dim3(blocks_for(n, 256)), dim3(256), 0, s, vi, n, w.tri_u, w.shade_u,
                        (float4*)sc->tris, (float4*)sc->shade);
     if (n > 1) {
         hipLaunchKernelGGL(k_refit_local, dim3(blocks_for(n, kRefitBlock)), dim3(256), 0, s, vi, n,
                            w.leafbox, w.parent, w.rangej, w.split, w.flags, nodes, w.nodebox,
                            sc->bounds);
-        hipLaunchKernelGGL(k_refit_top, dim3(blocks_for(n - 1, 256)), dim3(256), 0, s, vi, n,
-                           w.leafbox, w.parent, w.rangej, w.split, w.flags, nodes, w.nodebox,
-                           sc->bounds);
+        hipLaunchKernelGGL(k_refit_top, dim3(1), dim3(1024), 0, s, vi, n, w.leafbox, w.parent,
+                           w.rangej, w.split, w.flags, nodes, w.nodebox, sc->bounds, w.top_list,
+                           w.top_count);
         hipLaunchKernelGGL(k_wide, dim3(blocks_for(n - 1, 256)), dim3(256), 0, s, n,
                            (const float4*)nodes, (float4*)sc->nodes4);
         sc->root = n <= (uint32_t)kLeafMax ? leaf_ref(0, n) : 0;
