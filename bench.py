@@ -18,7 +18,8 @@ emulation of the kernel's traversal on the identical BVH (oracle/rt_twin.c), div
 kernel's mean duration from HIP events on the launch stream. The packet traversal fetches a record
 once per 64-ray wave, so the bytes it actually fetches are also reported (fetched_bytes), and
 `traffic` is the HBM traffic per launch measured by rocprofv3 (FETCH_SIZE/WRITE_SIZE, scaled by
-the same counters on a copy kernel of known size), read from profiles/k_trace_camera_c2.txt.
+the same counters on a copy kernel of known size), read from profiles/k_trace_camera_c2.txt when
+that summary was taken of the same kernel on the same workload.
 cpu_baseline: the same closest-hit traversal in the CPU restatement (C + OpenMP, all host
 threads) over full frames for about `--cpu-seconds`.
 """
@@ -67,9 +68,9 @@ def parse():
 PROFILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "k_trace_camera_c2.txt")
 
 
-def measured_traffic(workload):
-    """HBM bytes per launch of k_trace_camera from the committed rocprofv3 counter summary, if it
-    was taken on this workload; None otherwise."""
+def measured_traffic(workload, kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter summary, if it was
+    taken on this workload and kernel; None otherwise."""
     if not os.path.exists(PROFILE):
         return None
     vals = {}
@@ -79,7 +80,7 @@ def measured_traffic(workload):
                 k, v = line.split(":", 1)
                 vals[k.strip()] = v.strip()
     try:
-        if vals.get("workload") != workload:
+        if vals.get("workload") != workload or vals.get("kernel") != kernel:
             return None
         return int(float(vals["hbm_bytes_per_launch"]))
     except (KeyError, ValueError):
@@ -384,7 +385,9 @@ def main():
         pos, nrm, uv, mo = scenes.triangle_arrays(meshes)
         tw = oracle.TwinScene(pos, nrm, uv, mo, [1, 1], [1, 1])
         if not args.no_roofline:
-            wv = tw.trace_camera(cam.as_array(), W, H)
+            from rendertoy_amd import _native
+            quad = ren.get_tuning()["primary"] == _native.RT_TRAVERSAL_QUAD  # the kernel that ran
+            wv = tw.trace_camera(cam.as_array(), W, H, quad=quad)
             twin_hits = int((wv["prim"] >= 0).sum())
             lane_nodes = int(wv["lane_nodes"].sum())
             lane_tris = int(wv["lane_tris"].sum())
@@ -395,14 +398,15 @@ def main():
             result["roofline"] = {
                 "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 5),
-                "traffic": measured_traffic(result["config"]["workload"]),
+                "traffic": measured_traffic(result["config"]["workload"],
+                                            "k_trace_camera_quad" if quad else "k_trace_camera"),
                 "algorithmic_bytes_per_launch": int(algo_bytes),
                 "bytes_model": "per ray: 16 B hit + 128 B per 4-wide node record + 48 B per "
                                "triangle record its ray tests; + 48 B per hit (FP64 refinement)",
                 "mean_node_records_per_ray": round(lane_nodes / n_rays, 4),
                 "mean_triangles_per_ray": round(lane_tris / n_rays, 4),
                 "fetched_bytes_per_launch": int(fetched),
-                "kernel": "k_trace_camera",
+                "kernel": "k_trace_camera_quad" if quad else "k_trace_camera",
             }
             result["parity_primary_ids_equal_twin"] = bool(np.array_equal(prim, wv["prim"]))
         if not args.no_cpu_baseline:

@@ -151,17 +151,20 @@ enum {
     RT_TRAVERSAL_LANE = 1,   /* each lane walks its own ray (per-lane stack in LDS) */
     RT_TRAVERSAL_HYBRID = 2, /* packet for at most packet_budget steps, then per lane from the root
                                 bounded by the best hit so far (for rays that diverge) */
-    RT_TRAVERSAL_AUTO = 3    /* camera rays only: PACKET when an 8x8 pixel tile spans at most 0.009
+    RT_TRAVERSAL_AUTO = 3,   /* camera rays only: PACKET when an 8x8 pixel tile spans at most 0.009
                                 of the image plane at unit distance (16 tan(fov/2) / height, e.g.
                                 45 deg at >= 768 rows), else LANE; decided per image */
+    RT_TRAVERSAL_QUAD = 4    /* camera rays: packets of 16 rays (4x4 pixels), 4 lanes per ray, each
+                                lane testing one child box / one leaf triangle (the default) */
 };
 #define RT_LANE_STACK_MAX 32
 
 /* Launch choices of rt_trace_camera / rt_render (process-wide, read at every launch; no
  * counterpart in the reference, whose kernels take a fixed 32-thread group, _core.py:286). */
 typedef struct rt_tuning {
-    int32_t primary;          /* RT_TRAVERSAL_* for camera rays (default AUTO) */
-    int32_t shadow;           /* RT_TRAVERSAL_* for rt_render's shadow rays (default LANE) */
+    int32_t primary;          /* RT_TRAVERSAL_* for camera rays (default QUAD) */
+    int32_t shadow;           /* RT_TRAVERSAL_* for rt_render's shadow rays (default LANE); not
+                                 QUAD, which is for camera rays */
     uint32_t super_tile;      /* blocks are mapped to XCDs in super-tiles of this many 16x16-pixel
                                  blocks per side; 0 = row-major block order (default) */
     uint32_t lane_stack;      /* per-lane stack entries, 1..RT_LANE_STACK_MAX (default 32); a ray
@@ -213,7 +216,8 @@ int rt_trace_camera(const rt_scene* scene, const rt_camera* cam, uint32_t width,
 /* Full render in one kernel: per 8x8 pixel tile, ray generation, closest-hit trace, FP64 hit
  * refinement, Lambert with interpolated normals, one shadow ray toward the light per lit hit,
  * RGBA8 texture fetch, RGB store. `workspace_dev` needs rt_render_workspace_bytes(width*height)
- * bytes (currently 0: NULL is accepted). */
+ * bytes (currently 0: NULL is accepted). The QUAD camera-ray traversal walks the primary rays with
+ * the packet traversal here (faster inside this kernel). */
 uint64_t rt_render_workspace_bytes(uint32_t n_pixels);
 int rt_render(const rt_scene* scene, const rt_camera* cam, const rt_render_params* params,
               const rt_render_out* out, void* workspace_dev, uint64_t workspace_bytes,
@@ -232,9 +236,11 @@ typedef struct rt_render_job {
 
 typedef struct rt_batch_config {
     uint32_t waves_per_cu;    /* persistent waves per compute unit, 1..32 */
-    uint32_t tiles_per_grab;  /* consecutive 8x8 pixel tiles a wave takes per queue pop, 1..64 */
-    int32_t primary;          /* RT_TRAVERSAL_* for the camera rays (AUTO: decided per job) */
-    int32_t shadow;           /* RT_TRAVERSAL_* for the shadow rays */
+    uint32_t tiles_per_grab;  /* consecutive pixel tiles (8x8; 4x4 with a QUAD primary) a wave
+                                 takes per queue pop, 1..64 */
+    int32_t primary;          /* RT_TRAVERSAL_* for the camera rays (AUTO: decided per job; QUAD:
+                                 4x4-pixel tiles, shadow rays by the quad-lane traversal too) */
+    int32_t shadow;           /* RT_TRAVERSAL_* for the shadow rays (ignored with a QUAD primary) */
     uint32_t lane_stack;      /* per-lane stack entries, 1..RT_LANE_STACK_MAX */
     uint32_t packet_budget;   /* RT_TRAVERSAL_HYBRID packet steps, >= 1 */
 } rt_batch_config;

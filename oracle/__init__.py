@@ -52,7 +52,7 @@ def lib():
         L.tw_brute_closest32.restype = None
         L.tw_brute_closest32.argtypes = [vp, i64, i64] + [vp] * 6 + [f32, f32, vp, vp]
         L.tw_trace_camera.restype = None
-        L.tw_trace_camera.argtypes = [vp, vp, i32, vp, ctypes.c_uint32, ctypes.c_uint32] + [vp] * 9
+        L.tw_trace_camera.argtypes = [vp, vp, i32, vp, ctypes.c_uint32, ctypes.c_uint32] + [vp] * 9 + [ctypes.c_int]
         L.tw_generate_rays.restype = None
         L.tw_generate_rays.argtypes = [vp, ctypes.c_uint32, ctypes.c_uint32] + [vp] * 6
         L.tw_render.restype = None
@@ -207,16 +207,18 @@ class TwinScene:
             out["tri_tests"] = cnt[:, 1]
         return out
 
-    def trace_camera(self, cam12, W, H):
-        """Camera closest hit with the device's wave-coherent traversal over 8x8 tiles: dict of
-        per-pixel t, u, v, prim, slot plus per-tile wave_nodes / wave_leaves / wave_tris (4-wide
-        node records, leaves and triangle records fetched by the wave) and per-pixel lane_nodes /
-        lane_tris (node records and triangles that pixel's own ray tested)."""
+    def trace_camera(self, cam12, W, H, quad=False):
+        """Camera closest hit with the device's wave-coherent traversal over 8x8 tiles (quad: the
+        quad-lane kernel's 16-ray packets over 4x4 tiles): dict of per-pixel t, u, v, prim, slot
+        plus per-tile wave_nodes / wave_leaves / wave_tris (4-wide node records, leaves and
+        triangle records fetched by the wave) and per-pixel lane_nodes / lane_tris (node records
+        and triangles that pixel's own ray tested)."""
         n = W * H
         out = {"t": np.empty(n, np.float32), "u": np.empty(n, np.float32),
                "v": np.empty(n, np.float32), "prim": np.empty(n, np.int32),
                "slot": np.empty(n, np.int32)}
-        ntiles = ((W + 7) // 8) * ((H + 7) // 8)
+        ts = 4 if quad else 8
+        ntiles = ((W + ts - 1) // ts) * ((H + ts - 1) // ts)
         out["wave_nodes"] = np.empty(ntiles, np.int64)
         out["wave_leaves"] = np.empty(ntiles, np.int64)
         out["wave_tris"] = np.empty(ntiles, np.int64)
@@ -225,7 +227,7 @@ class TwinScene:
                               self.root, _p(np.ascontiguousarray(cam12, np.float32)), W, H,
                               _p(out["t"]), _p(out["u"]), _p(out["v"]), _p(out["prim"]),
                               _p(out["slot"]), _p(out["wave_nodes"]), _p(out["wave_leaves"]),
-                              _p(out["wave_tris"]), _p(lane))
+                              _p(out["wave_tris"]), _p(lane), 1 if quad else 0)
         out["lane_nodes"] = lane[:, 0]
         out["lane_tris"] = lane[:, 1]
         return out

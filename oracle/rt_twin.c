@@ -759,8 +759,12 @@ static inline void tile_lane(int64_t tile, uint32_t W, int64_t* col, int64_t* ro
 TW_API void tw_trace_camera(const float* nodes4, const float* tris, int32_t root, const float* cam,
                             uint32_t W, uint32_t H, float* out_t, float* out_u, float* out_v,
                             int32_t* out_prim, int32_t* out_slot, int64_t* wave_nodes,
-                            int64_t* wave_leaves, int64_t* wave_tris, int32_t* lane_counts) {
-    int64_t ntiles = (((int64_t)W + 7) / 8) * (((int64_t)H + 7) / 8);
+                            int64_t* wave_leaves, int64_t* wave_tris, int32_t* lane_counts,
+                            int quad) {
+    /* quad != 0: the quad-lane kernel's 16-ray packets over 4x4 tiles (ray q -> (q & 3, q >> 2)) */
+    int64_t ts = quad ? 4 : 8, nr = quad ? 16 : 64;
+    int64_t tiles_x = ((int64_t)W + ts - 1) / ts;
+    int64_t ntiles = tiles_x * (((int64_t)H + ts - 1) / ts);
     #pragma omp parallel for schedule(dynamic, 16)
     for (int64_t tile = 0; tile < ntiles; ++tile) {
         ray32 r[64];
@@ -769,8 +773,13 @@ TW_API void tw_trace_camera(const float* nodes4, const float* tris, int32_t root
         hit32 h[64];
         int64_t col[64], row[64];
         for (int l = 0; l < 64; ++l) {
-            tile_lane(tile, W, &col[l], &row[l], l);
-            want[l] = col[l] < W && row[l] < H;
+            if (quad) {
+                col[l] = (tile % tiles_x) * 4 + (l & 3);
+                row[l] = (tile / tiles_x) * 4 + (l >> 2);
+            } else {
+                tile_lane(tile, W, &col[l], &row[l], l);
+            }
+            want[l] = l < nr && col[l] < W && row[l] < H;
             skip[l] = -1;
             float d[3];
             gen_ray(cam, W, H, want[l] ? (uint32_t)col[l] : 0, want[l] ? (uint32_t)row[l] : 0, d);

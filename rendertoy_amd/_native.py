@@ -21,6 +21,7 @@ RT_TRAVERSAL_PACKET = 0
 RT_TRAVERSAL_LANE = 1
 RT_TRAVERSAL_HYBRID = 2
 RT_TRAVERSAL_AUTO = 3
+RT_TRAVERSAL_QUAD = 4
 RT_LANE_STACK_MAX = 32
 RT_BATCH_MAX_JOBS = 4096
 RT_BATCH_MAX_JOB_PIXELS = 1 << 24

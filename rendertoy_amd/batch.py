@@ -44,7 +44,7 @@ class RenderRequest:
 
 
 _ALG = {"packet": N.RT_TRAVERSAL_PACKET, "lane": N.RT_TRAVERSAL_LANE, "hybrid": N.RT_TRAVERSAL_HYBRID,
-        "auto": N.RT_TRAVERSAL_AUTO}
+        "auto": N.RT_TRAVERSAL_AUTO, "quad": N.RT_TRAVERSAL_QUAD}
 
 
 @dataclasses.dataclass
@@ -53,7 +53,7 @@ class BatchConfig:
     waves_per_cu: int = 16
     tiles_per_grab: int = 4
     shadow: str = "lane"
-    primary: str = "auto"
+    primary: str = "quad"
     lane_stack: int = N.RT_LANE_STACK_MAX
     packet_budget: int = 256
 
@@ -159,8 +159,9 @@ def time_launches(fn, reps, warmup=1):
     return [ev[k].elapsed_time(ev[k + 1]) for k in range(reps)]
 
 
-DEFAULT_CANDIDATES = [BatchConfig(w, g, s, p) for p in ("auto", "packet", "lane")
-                      for s in ("lane", "packet") for w in (8, 16, 32) for g in (1, 4)]
+DEFAULT_CANDIDATES = ([BatchConfig(w, g, "lane", "quad") for w in (8, 16, 32) for g in (1, 4, 16)] +
+                      [BatchConfig(w, g, s, p) for p in ("auto", "packet") for s in ("lane", "packet")
+                       for w in (16, 32) for g in (1, 4)])
 
 
 def autotune(batch: BatchRender, candidates=None, reps=3):

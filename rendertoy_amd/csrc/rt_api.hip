@@ -11,7 +11,7 @@ thread_local std::string g_last_error;
 std::mutex g_cu_mutex;
 int g_cu_count[64] = {0};
 std::mutex g_tuning_mutex;
-rt_tuning g_tuning = {RT_TRAVERSAL_AUTO, RT_TRAVERSAL_LANE, 0u, (uint32_t)RT_LANE_STACK_MAX, 256u};
+rt_tuning g_tuning = {RT_TRAVERSAL_QUAD, RT_TRAVERSAL_LANE, 0u, (uint32_t)RT_LANE_STACK_MAX, 256u};
 }  // namespace
 
 rt_tuning current_tuning() {
@@ -60,7 +60,8 @@ extern "C" int rt_get_tuning(rt_tuning* out) {
 extern "C" int rt_set_tuning(const rt_tuning* t) {
     if (!t) return rt::set_error(RT_EINVAL, "rt_set_tuning: tuning is NULL");
     auto alg_ok = [](int32_t a) { return a >= RT_TRAVERSAL_PACKET && a <= RT_TRAVERSAL_HYBRID; };
-    if (!(alg_ok(t->primary) || t->primary == RT_TRAVERSAL_AUTO) || !alg_ok(t->shadow))
+    if (!(alg_ok(t->primary) || t->primary == RT_TRAVERSAL_AUTO || t->primary == RT_TRAVERSAL_QUAD) ||
+        !alg_ok(t->shadow))
         return rt::set_error(RT_EINVAL, "rt_set_tuning: traversal must be one of RT_TRAVERSAL_* "
                                         "(RT_TRAVERSAL_AUTO for camera rays only)");
     if (t->packet_budget < 1) return rt::set_error(RT_EINVAL, "rt_set_tuning: packet_budget must be >= 1");

@@ -396,6 +396,215 @@ RT_DEV bool traverse_packet(const float4* __restrict__ nodes4, const float4* __r
     }
 }
 
+// ---- quad-lane packets: 16 rays per wave, 4 lanes per ray ----
+// Lane 4q+c holds ray q (the 4 lanes of a quad hold the same ray and the same `want`); at each node
+// lane c tests child c's box and at each leaf triangle c, so a step costs one box test and at most
+// four triangle tests per lane; the quad's closest candidate is combined across its 4 lanes with
+// DPP quad permutations. The visiting order is the packet traversal's (traverse_packet) over the
+// 16 rays: masks hold one bit per ray (lane 4q), internal children by the entry distances of the
+// lowest active ray, hit leaf children tested in the node step, the next node fetched before the
+// leaf tests. Results are replicated in the 4 lanes of a quad.
+constexpr uint64_t kQuadBits = 0x1111111111111111ull;  // lane 4q of every quad
+
+RT_DEV int dpp_xor1(int v) { return __builtin_amdgcn_update_dpp(0, v, 0xB1, 0xF, 0xF, false); }  // [1,0,3,2]
+RT_DEV int dpp_xor2(int v) { return __builtin_amdgcn_update_dpp(0, v, 0x4E, 0xF, 0xF, false); }  // [2,3,0,1]
+
+// closest of the candidates (t, prim) of the 4 lanes of each quad, ties to the smaller prim;
+// prim < 0 marks no candidate (t = +inf). Called with all lanes active.
+RT_DEV void quad_closest(float& t, float& u, float& v, int32_t& prim, int32_t& slot) {
+#pragma unroll
+    for (int step = 0; step < 2; ++step) {
+        const float ot = __int_as_float(step ? dpp_xor2(__float_as_int(t)) : dpp_xor1(__float_as_int(t)));
+        const float ou = __int_as_float(step ? dpp_xor2(__float_as_int(u)) : dpp_xor1(__float_as_int(u)));
+        const float ov = __int_as_float(step ? dpp_xor2(__float_as_int(v)) : dpp_xor1(__float_as_int(v)));
+        const int32_t op = step ? dpp_xor2(prim) : dpp_xor1(prim);
+        const int32_t os = step ? dpp_xor2(slot) : dpp_xor1(slot);
+        const bool take = op >= 0 && (prim < 0 || ot < t || (ot == t && op < prim));
+        t = take ? ot : t;
+        u = take ? ou : u;
+        v = take ? ov : v;
+        prim = take ? op : prim;
+        slot = take ? os : slot;
+    }
+}
+
+template <bool ANY>
+RT_DEV bool traverse_quad(const float4* __restrict__ nodes4, const float4* __restrict__ tris,
+                          int32_t root, const Ray& r, bool want, int32_t skip_slot, Hit& h,
+                          int budget = 0) {
+    h.t = r.tmax;
+    h.u = 0.0f;
+    h.v = 0.0f;
+    h.prim = -1;
+    h.slot = -1;
+    const uint32_t lane = __lane_id(), c = lane & 3u, qb = lane & ~3u;
+    uint64_t m = __ballot(want) & kQuadBits;
+    if (root == RT_EMPTY_ROOT || m == 0) return true;
+    uint64_t alive = m;
+    const float idx = safe_inv(r.dx), idy = safe_inv(r.dy), idz = safe_inv(r.dz);
+    const float oxx = r.ox * idx, oyy = r.oy * idy, ozz = r.oz * idz;
+    // tests triangle c of each leaf in `lref` (ray in the leaf's mask) and merges the quad's
+    // closest candidate into h
+    auto leaves = [&](const int32_t* lref, const uint64_t* lmsk) {
+        float bt = INFINITY, bu = 0.0f, bv = 0.0f;
+        int32_t bp = -1, bs = -1;
+        float4 ta[4], tb[4], tc[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            if (lmsk[k] == 0) continue;
+            const uint32_t first = leaf_first(lref[k]), cnt = leaf_count(lref[k]);
+            const float4* tr = tris + (size_t)(first + (c < cnt ? c : 0u)) * 3;
+            ta[k] = tr[0];
+            tb[k] = tr[1];
+            tc[k] = tr[2];
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            if (lmsk[k] == 0) continue;
+            const uint32_t first = leaf_first(lref[k]), cnt = leaf_count(lref[k]);
+            const int32_t slot = (int32_t)(first + c);
+            float tt, uu, vv;
+            if (c < cnt && ((lmsk[k] >> qb) & 1u) && slot != skip_slot &&
+                intersect_tri(ta[k], tb[k], tc[k], r, tt, uu, vv) && tt > r.tmin) {
+                const int32_t prim = __float_as_int(ta[k].w);
+                if (bp < 0 || tt < bt || (tt == bt && prim < bp)) {
+                    bt = tt;
+                    bu = uu;
+                    bv = vv;
+                    bp = prim;
+                    bs = slot;
+                }
+            }
+        }
+        quad_closest(bt, bu, bv, bp, bs);
+        if (bp >= 0 && (bt < h.t || (bt == h.t && bp < h.prim))) {
+            h.t = bt;
+            h.u = bu;
+            h.v = bv;
+            h.prim = bp;
+            h.slot = bs;
+        }
+    };
+    if (root < 0) {  // the whole tree is one leaf
+        const int32_t lref[4] = {root, 0, 0, 0};
+        const uint64_t lmsk[4] = {m, 0, 0, 0};
+        leaves(lref, lmsk);
+        return true;
+    }
+    int32_t stk_node0 = 0, stk_node1 = 0;
+    uint32_t stk_mlo0 = 0, stk_mlo1 = 0, stk_mhi0 = 0, stk_mhi1 = 0;
+    int32_t sp = 0;
+    auto pop = [&](int32_t& node, uint64_t& mask) -> bool {
+        for (;;) {
+            if (sp == 0) return false;
+            --sp;
+            const bool hi = sp >= 64;
+            const int32_t lsp = hi ? sp - 64 : sp;
+            node = __builtin_amdgcn_readlane(hi ? stk_node1 : stk_node0, lsp);
+            const uint32_t mlo = (uint32_t)__builtin_amdgcn_readlane((int)(hi ? stk_mlo1 : stk_mlo0), lsp);
+            const uint32_t mhi = (uint32_t)__builtin_amdgcn_readlane((int)(hi ? stk_mhi1 : stk_mhi0), lsp);
+            mask = ((uint64_t)mhi << 32) | mlo;
+            if (ANY) mask &= alive;
+            if (mask != 0) return true;
+        }
+    };
+    // lane c's child of node `cur`: its box and ref (fields c of the SoA record)
+    const float* nf = (const float*)(nodes4 + (size_t)root * 8);
+    float clx = nf[c], chx = nf[4 + c], cly = nf[8 + c], chy = nf[12 + c], clz = nf[16 + c],
+          chz = nf[20 + c];
+    int32_t cref = __float_as_int(nf[24 + c]);
+    int steps = 0;
+    for (;;) {
+        if (ANY) {
+            m &= alive;
+            if (m == 0) {
+                int32_t node;
+                if (!pop(node, m)) return true;
+                nf = (const float*)(nodes4 + (size_t)node * 8);
+                clx = nf[c]; chx = nf[4 + c]; cly = nf[8 + c]; chy = nf[12 + c]; clz = nf[16 + c];
+                chz = nf[20 + c]; cref = __float_as_int(nf[24 + c]);
+            }
+        }
+        if (budget > 0 && ++steps > budget) return false;
+        const bool mine = (m >> qb) & 1u;
+        const float x0 = fmaf(clx, idx, -oxx), x1 = fmaf(chx, idx, -oxx);
+        const float y0 = fmaf(cly, idy, -oyy), y1 = fmaf(chy, idy, -oyy);
+        const float z0 = fmaf(clz, idz, -ozz), z1 = fmaf(chz, idz, -ozz);
+        const float tn = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), r.tmin));
+        const float tf = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fmaxf(z0, z1)) * kTfarScale;
+        const bool hk = mine & (cref != RT_EMPTY_ROOT) & (tn <= fminf(tf, h.t));  // no short circuit
+        const uint64_t mk = __ballot(hk);
+        const int rep = __ffsll((unsigned long long)m) - 1;  // lane 4q of the lowest active ray
+        const float keyv = hk ? tn : INFINITY;
+        struct Child {
+            float key;
+            int32_t k, ref;
+            uint64_t mask;
+        };
+        Child ch[4];
+        int32_t lref[4];
+        uint64_t lmsk[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int32_t ref = __builtin_amdgcn_readlane(cref, k);
+            const uint64_t mr = (mk >> k) & kQuadBits;
+            const bool leaf = ref < 0;
+            lref[k] = ref;
+            lmsk[k] = leaf ? mr : 0;
+            ch[k].mask = leaf ? 0 : mr;
+            ch[k].key = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(keyv), rep + k));
+            ch[k].k = k;
+            ch[k].ref = ref;
+        }
+        auto cswap = [](Child& a, Child& b) {
+            if (b.key < a.key || (b.key == a.key && b.k < a.k)) {
+                const Child t = a;
+                a = b;
+                b = t;
+            }
+        };
+        cswap(ch[0], ch[1]);
+        cswap(ch[2], ch[3]);
+        cswap(ch[0], ch[2]);
+        cswap(ch[1], ch[3]);
+        cswap(ch[1], ch[2]);
+        int32_t next = -1;
+        uint64_t next_m = 0;
+#pragma unroll
+        for (int q = 3; q >= 0; --q) {
+            if (ch[q].mask == 0) continue;
+            if (next_m != 0 && sp < kWideStack) {
+                const bool at0 = (int32_t)lane == sp, at1 = (int32_t)lane + 64 == sp;
+                stk_node0 = at0 ? next : stk_node0;
+                stk_mlo0 = at0 ? (uint32_t)next_m : stk_mlo0;
+                stk_mhi0 = at0 ? (uint32_t)(next_m >> 32) : stk_mhi0;
+                stk_node1 = at1 ? next : stk_node1;
+                stk_mlo1 = at1 ? (uint32_t)next_m : stk_mlo1;
+                stk_mhi1 = at1 ? (uint32_t)(next_m >> 32) : stk_mhi1;
+                ++sp;
+            }
+            next = ch[q].ref;
+            next_m = ch[q].mask;
+        }
+        bool more = next_m != 0;
+        if (!more) more = pop(next, next_m);
+        if (more) {  // the next node's fetch overlaps the leaf tests
+            nf = (const float*)(nodes4 + (size_t)next * 8);
+            clx = nf[c]; chx = nf[4 + c]; cly = nf[8 + c]; chy = nf[12 + c]; clz = nf[16 + c];
+            chz = nf[20 + c]; cref = __float_as_int(nf[24 + c]);
+        }
+        if ((lmsk[0] | lmsk[1] | lmsk[2] | lmsk[3]) != 0) {
+            leaves(lref, lmsk);
+            if (ANY) {
+                alive &= ~(__ballot(h.prim >= 0) & kQuadBits);
+                if (alive == 0) return true;
+            }
+        }
+        if (!more) return true;
+        m = next_m;
+    }
+}
+
 // Per-lane traversal of the 4-wide nodes: every lane walks its own ray (no lane waits on another's
 // subtrees, so the longest lane bounds the wave), nearer children first by the lane's own entry
 // distances, pending children on a per-lane stack in LDS. `stk` points at this thread's first

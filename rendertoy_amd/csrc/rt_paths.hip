@@ -142,11 +142,17 @@ template <int PM>
 __global__ void __launch_bounds__(256) k_paths_primary(SceneArgs sc, rt_camera cam, PathArgs a,
                                                         uint32_t tiles_x, int lane_stack, int budget,
                                                         bool coherent) {
-    __shared__ int32_t stk[uses_lds(PM) ? kLaneStack * 256 : 1];
+    // PM == RT_TRAVERSAL_QUAD: quad layout, a block covers 8x8 pixels (wave w the 4x4 sub-tile
+    // ((w & 1) * 4, (w >> 1) * 4), lanes 4q..4q+3 its pixel (q & 3, q >> 2)); else 16x16 pixels
+    constexpr bool QL = PM == RT_TRAVERSAL_QUAD;
+    __shared__ int32_t stk[(uses_lds(PM) && !QL) ? kLaneStack * 256 : 1];
     __shared__ float4 pk[4][kPacketLds];
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const uint32_t col = (blockIdx.x % tiles_x) * 16 + (wave & 1) * 8 + (lane & 7);
-    const uint32_t row = (blockIdx.x / tiles_x) * 16 + (wave >> 1) * 8 + (lane >> 3);
+    const uint32_t col = QL ? (blockIdx.x % tiles_x) * 8 + (wave & 1) * 4 + ((lane >> 2) & 3)
+                            : (blockIdx.x % tiles_x) * 16 + (wave & 1) * 8 + (lane & 7);
+    const uint32_t row = QL ? (blockIdx.x / tiles_x) * 8 + (wave >> 1) * 4 + (lane >> 4)
+                            : (blockIdx.x / tiles_x) * 16 + (wave >> 1) * 8 + (lane >> 3);
+    const bool lead = !QL || (lane & 3) == 0;  // the lane that writes for its pixel
     const bool valid = col < a.W && row < a.H;
     const uint32_t pixel = valid ? row * a.W + col : 0u;
     for (uint32_t s = 0; s < a.spp; ++s) {
@@ -167,13 +173,16 @@ __global__ void __launch_bounds__(256) k_paths_primary(SceneArgs sc, rt_camera c
         r.tmin = 0.0f;
         r.tmax = INFINITY;
         Hit h;
-        trace_primary<PM>(sc, r, valid, h, pk[wave], stk + threadIdx.x, 256, lane_stack, budget,
-                          coherent);
+        if constexpr (QL)
+            traverse_quad<false>(sc.nodes4, sc.tris, sc.root, r, valid, -1, h);
+        else
+            trace_primary<PM>(sc, r, valid, h, pk[wave], stk + threadIdx.x, 256, lane_stack, budget,
+                              coherent);
         const bool hit = valid && h.prim >= 0;
         if (hit) refine_hit(sc.tris, r, h);
         const uint32_t id = pixel * a.spp + s;
-        if (valid) a.samples[id] = hit ? 0.0f : a.sky;
-        if (a.bounces > 0) emit_bounce(sc, a, r, h, hit, id, 1u, 1.0f, a.q[0], &a.counts[1]);
+        if (valid && lead) a.samples[id] = hit ? 0.0f : a.sky;
+        if (a.bounces > 0) emit_bounce(sc, a, r, h, hit && lead, id, 1u, 1.0f, a.q[0], &a.counts[1]);
     }
 }
 
@@ -285,8 +294,14 @@ extern "C" int rt_trace_paths(const rt_scene* scene, const rt_camera* cam, const
     using K = decltype(&k_paths_primary<0>);
     static const K prim[4] = {k_paths_primary<0>, k_paths_primary<1>, k_paths_primary<2>,
                               k_paths_primary<3>};
-    hipLaunchKernelGGL(prim[tu.primary], dim3(tiles_x * tiles_y), dim3(256), 0, s, sc, *cam, a,
-                       tiles_x, (int)tu.lane_stack, (int)tu.packet_budget, coherent);
+    if (tu.primary == RT_TRAVERSAL_QUAD) {
+        const uint32_t tx8 = (p->width + 7) / 8, ty8 = (p->height + 7) / 8;
+        hipLaunchKernelGGL(k_paths_primary<RT_TRAVERSAL_QUAD>, dim3(tx8 * ty8), dim3(256), 0, s, sc,
+                           *cam, a, tx8, (int)tu.lane_stack, (int)tu.packet_budget, coherent);
+    } else {
+        hipLaunchKernelGGL(prim[tu.primary], dim3(tiles_x * tiles_y), dim3(256), 0, s, sc, *cam, a,
+                           tiles_x, (int)tu.lane_stack, (int)tu.packet_budget, coherent);
+    }
     const uint32_t grid = (uint32_t)device_cu_count() * 16u;
     for (uint32_t k = 1; k <= p->bounces; ++k)
         hipLaunchKernelGGL(k_paths_bounce, dim3(grid), dim3(64), 0, s, sc, a, k, (int)tu.lane_stack);

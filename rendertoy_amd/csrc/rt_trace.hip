@@ -189,6 +189,25 @@ __global__ void __launch_bounds__(256) k_trace_camera(SceneArgs sc, rt_camera ca
     if (p.valid) store_hit(sc, hits, (uint64_t)p.row * W + p.col, h);
 }
 
+// Camera closest hit with quad-lane packets: a 256-thread block covers 8x8 pixels, wave w the 4x4
+// sub-tile ((w & 1) * 4, (w >> 1) * 4), lanes 4q..4q+3 the pixel (q & 3, q >> 2) of it.
+__global__ void __launch_bounds__(256) k_trace_camera_quad(SceneArgs sc, rt_camera cam, uint32_t W,
+                                                            uint32_t H, uint32_t tiles_x8,
+                                                            int budget, rt_hits hits) {
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6, q = lane >> 2;
+    TilePixel p;
+    p.col = (blockIdx.x % tiles_x8) * 8 + (wave & 1) * 4 + (q & 3);
+    p.row = (blockIdx.x / tiles_x8) * 8 + (wave >> 1) * 4 + (q >> 2);
+    p.valid = p.col < W && p.row < H;
+    const Ray r = camera_ray(cam, W, H, p);
+    Hit h;
+    traverse_quad<false>(sc.nodes4, sc.tris, sc.root, r, p.valid, -1, h, budget);
+    if (p.valid && (lane & 3) == 0) {
+        if (h.prim >= 0) refine_hit(sc.tris, r, h);
+        store_hit(sc, hits, (uint64_t)p.row * W + p.col, h);
+    }
+}
+
 RT_DEV int wrap_i(int i, int n) {
     const int r = i % n;
     return r < 0 ? r + n : r;
@@ -243,10 +262,17 @@ RT_DEV void render_pixel(const SceneArgs& sc, const rt_camera& cam, const rt_ren
                          const TilePixel& px, float* __restrict__ rgb, float* __restrict__ out_t,
                          int32_t* __restrict__ out_prim, uint32_t* __restrict__ stats, float4* pk,
                          int32_t* stk, int stk_stride, int lane_stack, int budget, bool coherent) {
+    // PM == RT_TRAVERSAL_QUAD: quad layout (the 4 lanes of a quad hold the same pixel), both rays by
+    // the quad-lane traversal, outputs and counts from lane 0 of each quad
+    constexpr bool QL = PM == RT_TRAVERSAL_QUAD;
     const uint32_t W = p.width, H = p.height;
     const Ray r = camera_ray(cam, W, H, px);
     Hit h;
-    trace_primary<PM>(sc, r, px.valid, h, pk, stk, stk_stride, lane_stack, budget, coherent);
+    if constexpr (QL) {
+        traverse_quad<false>(sc.nodes4, sc.tris, sc.root, r, px.valid, -1, h);
+    } else {
+        trace_primary<PM>(sc, r, px.valid, h, pk, stk, stk_stride, lane_stack, budget, coherent);
+    }
     const bool hit = px.valid && h.prim >= 0;
     if (hit) refine_hit(sc.tris, r, h);
 
@@ -280,8 +306,11 @@ RT_DEV void render_pixel(const SceneArgs& sc, const rt_camera& cam, const rt_ren
     sr.tmin = p.shadow_tmin;
     sr.tmax = INFINITY;
     Hit sh;
-    trace_block<true, SM>(sc, sr, want_shadow, hit ? h.slot : -1, sh, pk, stk, stk_stride, lane_stack,
-                          budget);
+    if constexpr (QL)
+        traverse_quad<true>(sc.nodes4, sc.tris, sc.root, sr, want_shadow, hit ? h.slot : -1, sh);
+    else
+        trace_block<true, SM>(sc, sr, want_shadow, hit ? h.slot : -1, sh, pk, stk, stk_stride, lane_stack,
+                              budget);
     const bool occluded = want_shadow && sh.prim >= 0;
 
     float alb[3] = {1.0f, 1.0f, 1.0f};
@@ -290,7 +319,7 @@ RT_DEV void render_pixel(const SceneArgs& sc, const rt_camera& cam, const rt_ren
         if (ti >= 0 && (uint32_t)ti < p.n_textures)
             tex_sample(p.textures_dev[ti], tu, tv, (p.flags & RT_RENDER_BILINEAR) != 0, alb);
     }
-    if (px.valid) {
+    if (px.valid && (!QL || (threadIdx.x & 3) == 0)) {
         const uint64_t pix = (uint64_t)px.row * W + px.col;
         if (hit) {
             const float kk = fmaf(occluded ? 0.0f : 1.0f, ndl, p.ambient);
@@ -306,9 +335,10 @@ RT_DEV void render_pixel(const SceneArgs& sc, const rt_camera& cam, const rt_ren
         if (out_prim) out_prim[pix] = hit ? h.prim : -1;
     }
     if (stats) {
-        const uint32_t n_hit = (uint32_t)__popcll(__ballot(hit));
-        const uint32_t n_sh = (uint32_t)__popcll(__ballot(want_shadow));
-        const uint32_t n_oc = (uint32_t)__popcll(__ballot(occluded));
+        const uint64_t one = QL ? kQuadBits : ~0ull;  // one lane per pixel
+        const uint32_t n_hit = (uint32_t)__popcll(__ballot(hit) & one);
+        const uint32_t n_sh = (uint32_t)__popcll(__ballot(want_shadow) & one);
+        const uint32_t n_oc = (uint32_t)__popcll(__ballot(occluded) & one);
         if ((threadIdx.x & 63) == 0) {
             if (n_hit) atomicAdd(&stats[0], n_hit);
             if (n_sh) atomicAdd(&stats[1], n_sh);
@@ -333,6 +363,22 @@ __global__ void __launch_bounds__(256) k_render(SceneArgs sc, rt_camera cam, rt_
                          stk + threadIdx.x, 256, lane_stack, budget, true);
 }
 
+// Fused render with quad-lane packets for the primary and the shadow rays: one 8x8-pixel tile per
+// block, wave w = 4x4 quarter (w & 1, w >> 1), lanes 4q..4q+3 = pixel (q & 3, q >> 2).
+__global__ void __launch_bounds__(256) k_render_quad(SceneArgs sc, rt_camera cam, rt_render_params p,
+                                                      uint32_t tiles_x8, float* __restrict__ rgb,
+                                                      float* __restrict__ out_t,
+                                                      int32_t* __restrict__ out_prim,
+                                                      uint32_t* __restrict__ stats) {
+    const uint32_t wave = threadIdx.x >> 6, q = (threadIdx.x & 63) >> 2;
+    TilePixel px;
+    px.col = (blockIdx.x % tiles_x8) * 8 + (wave & 1) * 4 + (q & 3);
+    px.row = (blockIdx.x / tiles_x8) * 8 + (wave >> 1) * 4 + (q >> 2);
+    px.valid = px.col < p.width && px.row < p.height;
+    render_pixel<RT_TRAVERSAL_QUAD, RT_TRAVERSAL_QUAD>(sc, cam, p, px, rgb, out_t, out_prim, stats,
+                                                       nullptr, nullptr, 0, 0, 0, true);
+}
+
 // ---- batched render ----
 // Workspace: [0] uint32 work counter, [1] status, then (n_jobs + 1) uint32 tile prefix at word 64.
 constexpr uint32_t kBatchPrefixWord = 64;
@@ -340,7 +386,7 @@ constexpr uint32_t kBatchPrefixWord = 64;
 // 8x8 tiles of a job, or 0 with a status bit if the job cannot be rendered: 1 = more than
 // RT_BATCH_MAX_JOB_PIXELS pixels, 2 = a required pointer is NULL (rgb output, texture table with
 // n_textures > 0, scene arrays of a non-empty scene).
-RT_DEV uint32_t job_tiles(const rt_render_job& j, uint32_t& status) {
+RT_DEV uint32_t job_tiles(const rt_render_job& j, uint32_t unit, uint32_t& status) {
     const rt_render_params& p = j.params;
     const uint64_t px = (uint64_t)p.width * p.height;
     if (px == 0) return 0;
@@ -354,13 +400,14 @@ RT_DEV uint32_t job_tiles(const rt_render_job& j, uint32_t& status) {
         status |= 2u;
         return 0;
     }
-    return ((p.width + 7u) / 8u) * ((p.height + 7u) / 8u);
+    return ((p.width + unit - 1u) / unit) * ((p.height + unit - 1u) / unit);
 }
 
 // One block of 1024 threads: per-job tile counts, exclusive prefix (n_jobs <= 4096, at most 4 jobs
 // per thread), work counter and status cleared, every job's stats zeroed.
 __global__ void __launch_bounds__(1024) k_batch_prepare(const rt_render_job* __restrict__ jobs,
-                                                         uint32_t n_jobs, uint32_t* __restrict__ ws) {
+                                                         uint32_t n_jobs, uint32_t unit,
+                                                         uint32_t* __restrict__ ws) {
     __shared__ uint32_t part[1024];
     __shared__ uint32_t status;
     const uint32_t t = threadIdx.x;
@@ -372,7 +419,7 @@ __global__ void __launch_bounds__(1024) k_batch_prepare(const rt_render_job* __r
         const uint32_t j = t * 4 + k;
         cnt[k] = 0;
         if (j < n_jobs) {
-            cnt[k] = job_tiles(jobs[j], bad);
+            cnt[k] = job_tiles(jobs[j], unit, bad);
             uint32_t* st = jobs[j].out.stats;
             if (st) {
                 st[0] = 0;
@@ -440,10 +487,14 @@ __global__ void __launch_bounds__(64) k_render_batch(const rt_render_job* __rest
             const uint32_t j = (uint32_t)__builtin_amdgcn_readfirstlane((int)lo);
             const rt_render_job& job = jobs[j];
             const rt_render_params p = job.params;
-            const uint32_t tiles_x = (p.width + 7u) / 8u, local = u - pref[j];
+            constexpr uint32_t unit = PM == RT_TRAVERSAL_QUAD ? 4u : 8u;
+            const uint32_t tiles_x = (p.width + unit - 1u) / unit, local = u - pref[j];
+            // 8x8 tile: lane l -> (l & 7, l >> 3); quad layout 4x4 tile: lanes 4q..4q+3 -> (q & 3, q >> 2)
+            const uint32_t dx = PM == RT_TRAVERSAL_QUAD ? (lane >> 2) & 3u : lane & 7u;
+            const uint32_t dy = PM == RT_TRAVERSAL_QUAD ? lane >> 4 : lane >> 3;
             TilePixel px;
-            px.col = (local % tiles_x) * 8u + (lane & 7u);
-            px.row = (local / tiles_x) * 8u + (lane >> 3);
+            px.col = (local % tiles_x) * unit + dx;
+            px.row = (local / tiles_x) * unit + dy;
             px.valid = px.col < p.width && px.row < p.height;
             SceneArgs sc;
             sc.nodes = (const float4*)job.scene.nodes;
@@ -527,6 +578,12 @@ extern "C" int rt_trace_camera(const rt_scene* scene, const rt_camera* cam, uint
     if (n > 0x7FFFFFFFull) return set_error(RT_ECAPACITY, "rt_trace_camera: image too large");
     const rt_tuning tu = current_tuning();
     const TileMap tm = make_tile_map(W, H, tu.super_tile);
+    if (tu.primary == RT_TRAVERSAL_QUAD) {
+        const uint32_t tx8 = (W + 7) / 8, ty8 = (H + 7) / 8;
+        hipLaunchKernelGGL(k_trace_camera_quad, dim3(tx8 * ty8), dim3(256), 0, (hipStream_t)stream,
+                           scene_args(scene), *cam, W, H, tx8, 0, *hits);
+        return check_launch("rt_trace_camera");
+    }
     using K = decltype(&k_trace_camera<0>);
     static const K kerns[3] = {k_trace_camera<0>, k_trace_camera<1>, k_trace_camera<2>};
     const int pm = resolve_primary(tu.primary, *cam, W, H);
@@ -563,6 +620,15 @@ extern "C" int rt_render(const rt_scene* scene, const rt_camera* cam, const rt_r
     static const K kerns[3][3] = {{k_render<0, 0>, k_render<0, 1>, k_render<0, 2>},
                                   {k_render<1, 0>, k_render<1, 1>, k_render<1, 2>},
                                   {k_render<2, 0>, k_render<2, 1>, k_render<2, 2>}};
+    // QUAD: the quad-lane traversal for both rays where the 8x8 tiles are not coherent (the AUTO
+    // rule); on coherent frames the packet traversal with the tuning's shadow traversal, faster
+    // there than quad-lane shadow rays
+    if (tu.primary == RT_TRAVERSAL_QUAD && !coherent_tiles(*cam, p->width, p->height)) {
+        const uint32_t tx8 = (p->width + 7) / 8, ty8 = (p->height + 7) / 8;
+        hipLaunchKernelGGL(k_render_quad, dim3(tx8 * ty8), dim3(256), 0, s, scene_args(scene), *cam, *p,
+                           tx8, out->rgb, out->t, out->prim, out->stats);
+        return check_launch("rt_render");
+    }
     const int pm = resolve_primary(tu.primary, *cam, p->width, p->height);
     hipLaunchKernelGGL(kerns[pm][tu.shadow], dim3(tile_map_blocks(tm)), dim3(256), 0, s,
                        scene_args(scene), *cam, *p, tm, (int)tu.lane_stack, (int)tu.packet_budget,
@@ -587,7 +653,8 @@ extern "C" int rt_render_batch(const rt_render_job* jobs_dev, uint32_t n_jobs, c
     auto alg_ok = [](int32_t a) { return a >= RT_TRAVERSAL_PACKET && a <= RT_TRAVERSAL_HYBRID; };
     if (cfg->waves_per_cu < 1 || cfg->waves_per_cu > 32 || cfg->tiles_per_grab < 1 ||
         cfg->tiles_per_grab > 64 || cfg->lane_stack < 1 || cfg->lane_stack > RT_LANE_STACK_MAX ||
-        !(alg_ok(cfg->primary) || cfg->primary == RT_TRAVERSAL_AUTO) || !alg_ok(cfg->shadow) ||
+        !(alg_ok(cfg->primary) || cfg->primary == RT_TRAVERSAL_AUTO || cfg->primary == RT_TRAVERSAL_QUAD) ||
+        !alg_ok(cfg->shadow) ||
         cfg->packet_budget < 1)
         return set_error(RT_EINVAL, "rt_render_batch: rt_batch_config field out of range");
     if (!workspace || ws_bytes < rt_render_batch_workspace_bytes(n_jobs))
@@ -595,8 +662,9 @@ extern "C" int rt_render_batch(const rt_render_job* jobs_dev, uint32_t n_jobs, c
     if ((uintptr_t)workspace & 15) return set_error(RT_EINVAL, "rt_render_batch: workspace must be 16-byte aligned");
     hipStream_t s = (hipStream_t)stream;
     uint32_t* ws = (uint32_t*)workspace;
-    hipLaunchKernelGGL(k_batch_prepare, dim3(1), dim3(1024), 0, s, jobs_dev, n_jobs, ws);
-    const bool lds_stack = cfg->primary != RT_TRAVERSAL_PACKET || cfg->shadow != RT_TRAVERSAL_PACKET;
+    const bool quad = cfg->primary == RT_TRAVERSAL_QUAD;
+    hipLaunchKernelGGL(k_batch_prepare, dim3(1), dim3(1024), 0, s, jobs_dev, n_jobs, quad ? 4u : 8u, ws);
+    const bool lds_stack = !quad && (cfg->primary != RT_TRAVERSAL_PACKET || cfg->shadow != RT_TRAVERSAL_PACKET);
     const size_t lds = 4u * ((n_jobs + 1 + 3) & ~3u) + (lds_stack ? 4u * 64u * kLaneStack : 0u);
     const uint32_t grid = (uint32_t)device_cu_count() * cfg->waves_per_cu;
     using K = decltype(&k_render_batch<0, 0>);
@@ -605,8 +673,8 @@ extern "C" int rt_render_batch(const rt_render_job* jobs_dev, uint32_t n_jobs, c
         {k_render_batch<1, 0>, k_render_batch<1, 1>, k_render_batch<1, 2>},
         {k_render_batch<2, 0>, k_render_batch<2, 1>, k_render_batch<2, 2>},
         {k_render_batch<3, 0>, k_render_batch<3, 1>, k_render_batch<3, 2>}};
-    hipLaunchKernelGGL(kerns[cfg->primary][cfg->shadow], dim3(grid), dim3(64), lds, s, jobs_dev,
-                       n_jobs, ws, cfg->tiles_per_grab, (int)cfg->lane_stack,
-                       (int)cfg->packet_budget);
+    const K kern = quad ? k_render_batch<RT_TRAVERSAL_QUAD, RT_TRAVERSAL_QUAD> : kerns[cfg->primary][cfg->shadow];
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(64), lds, s, jobs_dev, n_jobs, ws, cfg->tiles_per_grab,
+                       (int)cfg->lane_stack, (int)cfg->packet_budget);
     return check_launch("rt_render_batch");
 }

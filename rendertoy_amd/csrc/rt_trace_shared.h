@@ -111,7 +111,10 @@ inline int check_scene(const rt_scene* s, const char* what) {
     return RT_OK;
 }
 
+// The camera-ray traversal a kernel with PACKET / LANE / HYBRID variants runs for `mode`: AUTO
+// decided by the coherence rule, QUAD (implemented by rt_trace_camera only) as PACKET.
 inline int resolve_primary(int mode, const rt_camera& cam, uint32_t W, uint32_t H) {
+    if (mode == RT_TRAVERSAL_QUAD) return RT_TRAVERSAL_PACKET;
     if (mode != RT_TRAVERSAL_AUTO) return mode;
     return coherent_tiles(cam, W, H) ? RT_TRAVERSAL_PACKET : RT_TRAVERSAL_LANE;
 }

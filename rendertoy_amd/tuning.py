@@ -11,7 +11,8 @@ PACKET = _native.RT_TRAVERSAL_PACKET
 LANE = _native.RT_TRAVERSAL_LANE
 HYBRID = _native.RT_TRAVERSAL_HYBRID
 AUTO = _native.RT_TRAVERSAL_AUTO
-_NAMES = {"packet": PACKET, "lane": LANE, "hybrid": HYBRID, "auto": AUTO}
+QUAD = _native.RT_TRAVERSAL_QUAD
+_NAMES = {"packet": PACKET, "lane": LANE, "hybrid": HYBRID, "auto": AUTO, "quad": QUAD}
 
 
 def get_tuning():

@@ -114,13 +114,13 @@ def test_tuning_get_set_and_range_checks_without_gpu():
     import rendertoy_amd as ren
     from rendertoy_amd import _native
     d = ren.get_tuning()
-    assert d == {"primary": _native.RT_TRAVERSAL_AUTO, "shadow": _native.RT_TRAVERSAL_LANE,
+    assert d == {"primary": _native.RT_TRAVERSAL_QUAD, "shadow": _native.RT_TRAVERSAL_LANE,
                  "super_tile": 0, "lane_stack": _native.RT_LANE_STACK_MAX, "packet_budget": 256}
     with ren.tuning(primary="hybrid", lane_stack=3, super_tile=2, packet_budget=7):
         assert ren.get_tuning() == {"primary": 2, "shadow": 1, "super_tile": 2, "lane_stack": 3,
                                     "packet_budget": 7}
     assert ren.get_tuning() == d
-    for bad in ({"lane_stack": 0}, {"lane_stack": 33}, {"primary": 4}, {"shadow": 3},
+    for bad in ({"lane_stack": 0}, {"lane_stack": 33}, {"primary": 5}, {"shadow": 3},
                 {"super_tile": 65}, {"packet_budget": 0}):
         with pytest.raises(RuntimeError, match="rt_set_tuning"):
             ren.set_tuning(**bad)
@@ -141,7 +141,7 @@ def test_batch_entry_rejects_bad_arguments_without_gpu():
     assert lib.rt_render_batch(fake_jobs, N.RT_BATCH_MAX_JOBS + 1, ctypes.byref(cfg), wsp, 1 << 20, None) == -3
     for bad in (N.rt_batch_config(0, 1, 0, 1, 32, 256), N.rt_batch_config(33, 1, 0, 1, 32, 256),
                 N.rt_batch_config(8, 0, 0, 1, 32, 256), N.rt_batch_config(8, 65, 0, 1, 32, 256),
-                N.rt_batch_config(8, 1, 4, 1, 32, 256), N.rt_batch_config(8, 1, 0, 3, 32, 256),
+                N.rt_batch_config(8, 1, 5, 1, 32, 256), N.rt_batch_config(8, 1, 0, 3, 32, 256),
                 N.rt_batch_config(8, 1, 0, 1, 0, 256), N.rt_batch_config(8, 1, 0, 1, 32, 0)):
         assert lib.rt_render_batch(fake_jobs, 1, ctypes.byref(bad), wsp, 1 << 16, None) == -1
     need = lib.rt_render_batch_workspace_bytes(10)

@@ -150,7 +150,7 @@ def test_render_c1_matches_twin(c1, twin):
 @pytest.mark.parametrize("primary,shadow,lane_stack,super_tile,budget", [
     ("lane", "lane", 32, 0, 256), ("lane", "packet", 2, 3, 256), ("packet", "lane", 1, 1, 256),
     ("lane", "lane", 4, 0, 256), ("hybrid", "hybrid", 32, 0, 3), ("hybrid", "lane", 2, 0, 1),
-    ("hybrid", "hybrid", 32, 0, 40), ("auto", "lane", 32, 0, 256)])
+    ("hybrid", "hybrid", 32, 0, 40), ("auto", "lane", 32, 0, 256), ("quad", "lane", 32, 0, 256)])
 def test_tuning_choices_give_identical_results(c1, twin, primary, shadow, lane_stack, super_tile,
                                                budget):
     """Every traversal / stack / block-mapping / budget choice returns the twin's hits and RGB.
@@ -260,10 +260,15 @@ def test_render_c2_1080p_matches_twin_and_fp64(c2, twin):
     np.testing.assert_array_equal(_u32(hits["t"].cpu().numpy()), _u32(t))
     np.testing.assert_array_equal(_u32(img.cpu().numpy()), _u32(rgb))
     assert list(stats.cpu().numpy()[:3]) == list(st)
-    # the camera kernel gives the same hits as the render's primary pass and the twin
-    gc = {k: v.cpu().numpy() for k, v in rc.cast_camera(cam, W, H).items()}
-    np.testing.assert_array_equal(gc["prim"], prim)
-    assert_trace_equal(gc, tw.trace_camera(cam.as_array(), W, H))
+    # the camera kernel gives the same hits as the render's primary pass and the twin, with the
+    # 64-ray packets and with the quad-lane 16-ray packets
+    import rendertoy_amd as ren
+    tref = tw.trace_camera(cam.as_array(), W, H)
+    for mode in ("packet", "quad"):
+        with ren.tuning(primary=mode):
+            gc = {k: v.cpu().numpy() for k, v in rc.cast_camera(cam, W, H).items()}
+        np.testing.assert_array_equal(gc["prim"], prim)
+        assert_trace_equal(gc, tref)
     # primary IDs and t against FP64 on the full frame
     o, d = twin.twin_generate_rays(cam.as_array(), W, H)
     fp64_report("C2 primary 1920x1080", prim, t, tw.closest64(o, d))

@@ -32,7 +32,7 @@ def run(rc, cam, W, H, **kw):
     return img.cpu().numpy().reshape(-1), counts.cpu().numpy()
 
 
-@pytest.mark.parametrize("primary", ["auto", "packet", "lane"])
+@pytest.mark.parametrize("primary", ["quad", "auto", "packet", "lane"])
 @pytest.mark.parametrize("bounces,spp", [(2, 4), (0, 1), (3, 2)])
 def test_paths_equal_twin(ren, cluster, primary, bounces, spp):
     meshes, tw, rc = cluster
