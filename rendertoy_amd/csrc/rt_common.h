@@ -399,11 +399,12 @@ RT_DEV bool traverse_packet(const float4* __restrict__ nodes4, const float4* __r
 // ---- quad-lane packets: 16 rays per wave, 4 lanes per ray ----
 // Lane 4q+c holds ray q (the 4 lanes of a quad hold the same ray and the same `want`); at each node
 // lane c tests child c's box and at each leaf triangle c, so a step costs one box test and at most
-// four triangle tests per lane; the quad's closest candidate is combined across its 4 lanes with
-// DPP quad permutations. The visiting order is the packet traversal's (traverse_packet) over the
-// 16 rays: masks hold one bit per ray (lane 4q), internal children by the entry distances of the
-// lowest active ray, hit leaf children tested in the node step, the next node fetched before the
-// leaf tests. Results are replicated in the 4 lanes of a quad.
+// four triangle tests per lane. Each lane keeps its own closest candidate; boxes are culled with the
+// quad's best t (a DPP minimum after each leaf step), and the quad's closest hit is combined across
+// its 4 lanes with DPP quad permutations once, at the end. The visiting order is the packet
+// traversal's (traverse_packet) over the 16 rays: masks hold one bit per ray (lane 4q), internal
+// children by the entry distances of the lowest active ray, hit leaf children tested in the node
+// step, the next node fetched before the leaf tests. Results are replicated in the 4 lanes of a quad.
 constexpr uint64_t kQuadBits = 0x1111111111111111ull;  // lane 4q of every quad
 
 RT_DEV int dpp_xor1(int v) { return __builtin_amdgcn_update_dpp(0, v, 0xB1, 0xF, 0xF, false); }  // [1,0,3,2]
@@ -428,10 +429,20 @@ RT_DEV void quad_closest(float& t, float& u, float& v, int32_t& prim, int32_t& s
     }
 }
 
+// minimum of t over the 4 lanes of each quad (all lanes active)
+RT_DEV float quad_min(float t) {
+    t = fminf(t, __int_as_float(dpp_xor1(__float_as_int(t))));
+    return fminf(t, __int_as_float(dpp_xor2(__float_as_int(t))));
+}
+
+// bit 4q set if any lane of quad q is set in the ballot b
+RT_DEV uint64_t quad_any(uint64_t b) { return (b | (b >> 1) | (b >> 2) | (b >> 3)) & kQuadBits; }
+
+// The traversal proper: on return each lane's h is its own closest candidate (see traverse_quad).
 template <bool ANY>
-RT_DEV bool traverse_quad(const float4* __restrict__ nodes4, const float4* __restrict__ tris,
-                          int32_t root, const Ray& r, bool want, int32_t skip_slot, Hit& h,
-                          int budget = 0) {
+RT_DEV bool traverse_quad_lanes(const float4* __restrict__ nodes4, const float4* __restrict__ tris,
+                                int32_t root, const Ray& r, bool want, int32_t skip_slot, Hit& h,
+                                int budget) {
     h.t = r.tmax;
     h.u = 0.0f;
     h.v = 0.0f;
@@ -443,8 +454,9 @@ RT_DEV bool traverse_quad(const float4* __restrict__ nodes4, const float4* __res
     uint64_t alive = m;
     const float idx = safe_inv(r.dx), idy = safe_inv(r.dy), idz = safe_inv(r.dz);
     const float oxx = r.ox * idx, oyy = r.oy * idy, ozz = r.oz * idz;
-    // tests triangle c of each leaf in `lref` (ray in the leaf's mask) and merges the quad's
-    // closest candidate into h
+    float qt = h.t;  // the quad's best t so far (culling); h holds each lane's own best until the end
+    // tests triangle c of each leaf in `lref` (ray in the leaf's mask), keeps this lane's closest
+    // candidate in h and refreshes the quad's best t
     auto leaves = [&](const int32_t* lref, const uint64_t* lmsk) {
         float bt = INFINITY, bu = 0.0f, bv = 0.0f;
         int32_t bp = -1, bs = -1;
@@ -476,7 +488,6 @@ RT_DEV bool traverse_quad(const float4* __restrict__ nodes4, const float4* __res
                 }
             }
         }
-        quad_closest(bt, bu, bv, bp, bs);
         if (bp >= 0 && (bt < h.t || (bt == h.t && bp < h.prim))) {
             h.t = bt;
             h.u = bu;
@@ -484,6 +495,7 @@ RT_DEV bool traverse_quad(const float4* __restrict__ nodes4, const float4* __res
             h.prim = bp;
             h.slot = bs;
         }
+        qt = quad_min(h.t);
     };
     if (root < 0) {  // the whole tree is one leaf
         const int32_t lref[4] = {root, 0, 0, 0};
@@ -492,7 +504,7 @@ RT_DEV bool traverse_quad(const float4* __restrict__ nodes4, const float4* __res
         return true;
     }
     int32_t stk_node0 = 0, stk_node1 = 0;
-    uint32_t stk_mlo0 = 0, stk_mlo1 = 0, stk_mhi0 = 0, stk_mhi1 = 0;
+    uint32_t stk_m0 = 0, stk_m1 = 0;  // ray masks, bit 4q -> bit 4q (q < 8) / 4(q-8)+1
     int32_t sp = 0;
     auto pop = [&](int32_t& node, uint64_t& mask) -> bool {
         for (;;) {
@@ -501,9 +513,8 @@ RT_DEV bool traverse_quad(const float4* __restrict__ nodes4, const float4* __res
             const bool hi = sp >= 64;
             const int32_t lsp = hi ? sp - 64 : sp;
             node = __builtin_amdgcn_readlane(hi ? stk_node1 : stk_node0, lsp);
-            const uint32_t mlo = (uint32_t)__builtin_amdgcn_readlane((int)(hi ? stk_mlo1 : stk_mlo0), lsp);
-            const uint32_t mhi = (uint32_t)__builtin_amdgcn_readlane((int)(hi ? stk_mhi1 : stk_mhi0), lsp);
-            mask = ((uint64_t)mhi << 32) | mlo;
+            const uint32_t cm = (uint32_t)__builtin_amdgcn_readlane((int)(hi ? stk_m1 : stk_m0), lsp);
+            mask = (uint64_t)(cm & 0x11111111u) | ((uint64_t)((cm >> 1) & 0x11111111u) << 32);
             if (ANY) mask &= alive;
             if (mask != 0) return true;
         }
@@ -532,59 +543,92 @@ RT_DEV bool traverse_quad(const float4* __restrict__ nodes4, const float4* __res
         const float z0 = fmaf(clz, idz, -ozz), z1 = fmaf(chz, idz, -ozz);
         const float tn = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), r.tmin));
         const float tf = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fmaxf(z0, z1)) * kTfarScale;
-        const bool hk = mine & (cref != RT_EMPTY_ROOT) & (tn <= fminf(tf, h.t));  // no short circuit
+        const bool hk = mine & (cref != RT_EMPTY_ROOT) & (tn <= fminf(tf, qt));  // no short circuit
         const uint64_t mk = __ballot(hk);
-        const int rep = __ffsll((unsigned long long)m) - 1;  // lane 4q of the lowest active ray
-        const float keyv = hk ? tn : INFINITY;
-        struct Child {
-            float key;
-            int32_t k, ref;
-            uint64_t mask;
-        };
-        Child ch[4];
-        int32_t lref[4];
-        uint64_t lmsk[4];
+        // children some ray entered (bit c of the OR of the 16 quads' nibbles): internal ones (live)
+        // and leaves (tested below, in this step)
+        uint64_t fold = mk | (mk >> 32);
+        fold |= fold >> 16;
+        fold |= fold >> 8;
+        fold |= fold >> 4;
+        const uint32_t lbits = (uint32_t)__ballot(cref < 0);  // bit c: child c is a leaf
+        const uint32_t live = (uint32_t)fold & ~lbits & 0xFu, leafs = (uint32_t)fold & lbits & 0xFu;
+        int32_t lref[4] = {0, 0, 0, 0};
+        uint64_t lmsk[4] = {0, 0, 0, 0};
+        if (leafs != 0) {
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            const int32_t ref = __builtin_amdgcn_readlane(cref, k);
-            const uint64_t mr = (mk >> k) & kQuadBits;
-            const bool leaf = ref < 0;
-            lref[k] = ref;
-            lmsk[k] = leaf ? mr : 0;
-            ch[k].mask = leaf ? 0 : mr;
-            ch[k].key = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(keyv), rep + k));
-            ch[k].k = k;
-            ch[k].ref = ref;
-        }
-        auto cswap = [](Child& a, Child& b) {
-            if (b.key < a.key || (b.key == a.key && b.k < a.k)) {
-                const Child t = a;
-                a = b;
-                b = t;
+            for (int k = 0; k < 4; ++k) {
+                if ((leafs >> k) & 1u) {
+                    lref[k] = __builtin_amdgcn_readlane(cref, k);
+                    lmsk[k] = (mk >> k) & kQuadBits;
+                }
             }
-        };
-        cswap(ch[0], ch[1]);
-        cswap(ch[2], ch[3]);
-        cswap(ch[0], ch[2]);
-        cswap(ch[1], ch[3]);
-        cswap(ch[1], ch[2]);
-        int32_t next = -1;
-        uint64_t next_m = 0;
-#pragma unroll
-        for (int q = 3; q >= 0; --q) {
-            if (ch[q].mask == 0) continue;
-            if (next_m != 0 && sp < kWideStack) {
+        }
+        auto push = [&](int32_t node, uint64_t mask) {
+            if (sp < kWideStack) {
                 const bool at0 = (int32_t)lane == sp, at1 = (int32_t)lane + 64 == sp;
-                stk_node0 = at0 ? next : stk_node0;
-                stk_mlo0 = at0 ? (uint32_t)next_m : stk_mlo0;
-                stk_mhi0 = at0 ? (uint32_t)(next_m >> 32) : stk_mhi0;
-                stk_node1 = at1 ? next : stk_node1;
-                stk_mlo1 = at1 ? (uint32_t)next_m : stk_mlo1;
-                stk_mhi1 = at1 ? (uint32_t)(next_m >> 32) : stk_mhi1;
+                const uint32_t cm = (uint32_t)mask | ((uint32_t)(mask >> 32) << 1);
+                stk_node0 = at0 ? node : stk_node0;
+                stk_m0 = at0 ? cm : stk_m0;
+                stk_node1 = at1 ? node : stk_node1;
+                stk_m1 = at1 ? cm : stk_m1;
                 ++sp;
             }
-            next = ch[q].ref;
-            next_m = ch[q].mask;
+        };
+        // Internal children in the order of (entry distance of the lowest active ray, k): the
+        // nearest is visited next, the others are pushed farthest first. One or two live children
+        // (the common case) take a direct path; three or four a 5-comparator network.
+        int32_t next = -1;
+        uint64_t next_m = 0;
+        const int rep = __ffsll((unsigned long long)m) - 1;  // lane 4q of the lowest active ray
+        const float keyv = hk ? tn : INFINITY;
+        const int nlive = __builtin_popcount(live);
+        if (nlive == 1) {
+            const int k = __builtin_ctz(live);
+            next = __builtin_amdgcn_readlane(cref, k);
+            next_m = (mk >> k) & kQuadBits;
+        } else if (nlive == 2) {
+            const int k0 = __builtin_ctz(live), k1 = 31 - __builtin_clz(live);  // k0 < k1
+            const float key0 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(keyv), rep + k0));
+            const float key1 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(keyv), rep + k1));
+            const bool swap = key1 < key0;
+            const int kn = swap ? k1 : k0, kf = swap ? k0 : k1;
+            push(__builtin_amdgcn_readlane(cref, kf), (mk >> kf) & kQuadBits);
+            next = __builtin_amdgcn_readlane(cref, kn);
+            next_m = (mk >> kn) & kQuadBits;
+        } else if (nlive > 2) {
+            struct Child {
+                float key;
+                int32_t k, ref;
+                uint64_t mask;
+            };
+            Child ch[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                ch[k].key = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(keyv), rep + k));
+                ch[k].k = k;
+                ch[k].ref = __builtin_amdgcn_readlane(cref, k);
+                ch[k].mask = (live >> k) & 1u ? (mk >> k) & kQuadBits : 0;
+            }
+            auto cswap = [](Child& a, Child& b) {
+                if (b.key < a.key || (b.key == a.key && b.k < a.k)) {
+                    const Child t = a;
+                    a = b;
+                    b = t;
+                }
+            };
+            cswap(ch[0], ch[1]);
+            cswap(ch[2], ch[3]);
+            cswap(ch[0], ch[2]);
+            cswap(ch[1], ch[3]);
+            cswap(ch[1], ch[2]);
+#pragma unroll
+            for (int q = 3; q >= 0; --q) {
+                if (ch[q].mask == 0) continue;
+                if (next_m != 0) push(next, next_m);
+                next = ch[q].ref;
+                next_m = ch[q].mask;
+            }
         }
         bool more = next_m != 0;
         if (!more) more = pop(next, next_m);
@@ -593,16 +637,27 @@ RT_DEV bool traverse_quad(const float4* __restrict__ nodes4, const float4* __res
             clx = nf[c]; chx = nf[4 + c]; cly = nf[8 + c]; chy = nf[12 + c]; clz = nf[16 + c];
             chz = nf[20 + c]; cref = __float_as_int(nf[24 + c]);
         }
-        if ((lmsk[0] | lmsk[1] | lmsk[2] | lmsk[3]) != 0) {
+        if (leafs != 0) {
             leaves(lref, lmsk);
             if (ANY) {
-                alive &= ~(__ballot(h.prim >= 0) & kQuadBits);
+                alive &= ~quad_any(__ballot(h.prim >= 0));
                 if (alive == 0) return true;
             }
         }
         if (!more) return true;
         m = next_m;
     }
+}
+
+// Quad-lane traversal of the 16 rays of a wave (lane 4q+c holds ray q); h = the ray's closest hit
+// (ANY: a hit) in all 4 lanes of its quad. Called by all 64 lanes.
+template <bool ANY>
+RT_DEV bool traverse_quad(const float4* __restrict__ nodes4, const float4* __restrict__ tris,
+                          int32_t root, const Ray& r, bool want, int32_t skip_slot, Hit& h,
+                          int budget = 0) {
+    const bool done = traverse_quad_lanes<ANY>(nodes4, tris, root, r, want, skip_slot, h, budget);
+    quad_closest(h.t, h.u, h.v, h.prim, h.slot);
+    return done;
 }
 
 // Per-lane traversal of the 4-wide nodes: every lane walks its own ray (no lane waits on another's
