@@ -440,9 +440,8 @@ RT_DEV uint64_t quad_any(uint64_t b) { return (b | (b >> 1) | (b >> 2) | (b >> 3
 
 // The traversal proper: on return each lane's h is its own closest candidate (see traverse_quad).
 template <bool ANY>
-RT_DEV bool traverse_quad_lanes(const float4* __restrict__ nodes4, const float4* __restrict__ tris,
-                                int32_t root, const Ray& r, bool want, int32_t skip_slot, Hit& h,
-                                int budget) {
+RT_DEV void traverse_quad_lanes(const float4* __restrict__ nodes4, const float4* __restrict__ tris,
+                                int32_t root, const Ray& r, bool want, int32_t skip_slot, Hit& h) {
     h.t = r.tmax;
     h.u = 0.0f;
     h.v = 0.0f;
@@ -450,20 +449,20 @@ RT_DEV bool traverse_quad_lanes(const float4* __restrict__ nodes4, const float4*
     h.slot = -1;
     const uint32_t lane = __lane_id(), c = lane & 3u, qb = lane & ~3u;
     uint64_t m = __ballot(want) & kQuadBits;
-    if (root == RT_EMPTY_ROOT || m == 0) return true;
+    if (root == RT_EMPTY_ROOT || m == 0) return;
     uint64_t alive = m;
     const float idx = safe_inv(r.dx), idy = safe_inv(r.dy), idz = safe_inv(r.dz);
     const float oxx = r.ox * idx, oyy = r.oy * idy, ozz = r.oz * idz;
     float qt = h.t;  // the quad's best t so far (culling); h holds each lane's own best until the end
     // tests triangle c of each leaf in `lref` (ray in the leaf's mask), keeps this lane's closest
     // candidate in h and refreshes the quad's best t
-    auto leaves = [&](const int32_t* lref, const uint64_t* lmsk) {
+    auto leaves = [&](uint32_t lset, const int32_t* lref, const uint64_t* lmsk) {
         float bt = INFINITY, bu = 0.0f, bv = 0.0f;
         int32_t bp = -1, bs = -1;
         float4 ta[4], tb[4], tc[4];
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
-            if (lmsk[k] == 0) continue;
+            if (!((lset >> k) & 1u)) continue;
             const uint32_t first = leaf_first(lref[k]), cnt = leaf_count(lref[k]);
             const float4* tr = tris + (size_t)(first + (c < cnt ? c : 0u)) * 3;
             ta[k] = tr[0];
@@ -472,7 +471,7 @@ RT_DEV bool traverse_quad_lanes(const float4* __restrict__ nodes4, const float4*
         }
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
-            if (lmsk[k] == 0) continue;
+            if (!((lset >> k) & 1u)) continue;
             const uint32_t first = leaf_first(lref[k]), cnt = leaf_count(lref[k]);
             const int32_t slot = (int32_t)(first + c);
             float tt, uu, vv;
@@ -500,8 +499,8 @@ RT_DEV bool traverse_quad_lanes(const float4* __restrict__ nodes4, const float4*
     if (root < 0) {  // the whole tree is one leaf
         const int32_t lref[4] = {root, 0, 0, 0};
         const uint64_t lmsk[4] = {m, 0, 0, 0};
-        leaves(lref, lmsk);
-        return true;
+        leaves(1u, lref, lmsk);
+        return;
     }
     int32_t stk_node0 = 0, stk_node1 = 0;
     uint32_t stk_m0 = 0, stk_m1 = 0;  // ray masks, bit 4q -> bit 4q (q < 8) / 4(q-8)+1
@@ -524,19 +523,17 @@ RT_DEV bool traverse_quad_lanes(const float4* __restrict__ nodes4, const float4*
     float clx = nf[c], chx = nf[4 + c], cly = nf[8 + c], chy = nf[12 + c], clz = nf[16 + c],
           chz = nf[20 + c];
     int32_t cref = __float_as_int(nf[24 + c]);
-    int steps = 0;
     for (;;) {
         if (ANY) {
             m &= alive;
             if (m == 0) {
                 int32_t node;
-                if (!pop(node, m)) return true;
+                if (!pop(node, m)) return;
                 nf = (const float*)(nodes4 + (size_t)node * 8);
                 clx = nf[c]; chx = nf[4 + c]; cly = nf[8 + c]; chy = nf[12 + c]; clz = nf[16 + c];
                 chz = nf[20 + c]; cref = __float_as_int(nf[24 + c]);
             }
         }
-        if (budget > 0 && ++steps > budget) return false;
         const bool mine = (m >> qb) & 1u;
         const float x0 = fmaf(clx, idx, -oxx), x1 = fmaf(chx, idx, -oxx);
         const float y0 = fmaf(cly, idy, -oyy), y1 = fmaf(chy, idy, -oyy);
@@ -553,8 +550,8 @@ RT_DEV bool traverse_quad_lanes(const float4* __restrict__ nodes4, const float4*
         fold |= fold >> 4;
         const uint32_t lbits = (uint32_t)__ballot(cref < 0);  // bit c: child c is a leaf
         const uint32_t live = (uint32_t)fold & ~lbits & 0xFu, leafs = (uint32_t)fold & lbits & 0xFu;
-        int32_t lref[4] = {0, 0, 0, 0};
-        uint64_t lmsk[4] = {0, 0, 0, 0};
+        int32_t lref[4];
+        uint64_t lmsk[4];
         if (leafs != 0) {
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
@@ -638,13 +635,13 @@ RT_DEV bool traverse_quad_lanes(const float4* __restrict__ nodes4, const float4*
             chz = nf[20 + c]; cref = __float_as_int(nf[24 + c]);
         }
         if (leafs != 0) {
-            leaves(lref, lmsk);
+            leaves(leafs, lref, lmsk);
             if (ANY) {
                 alive &= ~quad_any(__ballot(h.prim >= 0));
-                if (alive == 0) return true;
+                if (alive == 0) return;
             }
         }
-        if (!more) return true;
+        if (!more) return;
         m = next_m;
     }
 }
@@ -652,12 +649,10 @@ RT_DEV bool traverse_quad_lanes(const float4* __restrict__ nodes4, const float4*
 // Quad-lane traversal of the 16 rays of a wave (lane 4q+c holds ray q); h = the ray's closest hit
 // (ANY: a hit) in all 4 lanes of its quad. Called by all 64 lanes.
 template <bool ANY>
-RT_DEV bool traverse_quad(const float4* __restrict__ nodes4, const float4* __restrict__ tris,
-                          int32_t root, const Ray& r, bool want, int32_t skip_slot, Hit& h,
-                          int budget = 0) {
-    const bool done = traverse_quad_lanes<ANY>(nodes4, tris, root, r, want, skip_slot, h, budget);
+RT_DEV void traverse_quad(const float4* __restrict__ nodes4, const float4* __restrict__ tris,
+                          int32_t root, const Ray& r, bool want, int32_t skip_slot, Hit& h) {
+    traverse_quad_lanes<ANY>(nodes4, tris, root, r, want, skip_slot, h);
     quad_closest(h.t, h.u, h.v, h.prim, h.slot);
-    return done;
 }
 
 // Per-lane traversal of the 4-wide nodes: every lane walks its own ray (no lane waits on another's

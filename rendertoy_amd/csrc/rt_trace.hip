@@ -193,7 +193,7 @@ __global__ void __launch_bounds__(256) k_trace_camera(SceneArgs sc, rt_camera ca
 // sub-tile ((w & 1) * 4, (w >> 1) * 4), lanes 4q..4q+3 the pixel (q & 3, q >> 2) of it.
 __global__ void __launch_bounds__(256) k_trace_camera_quad(SceneArgs sc, rt_camera cam, uint32_t W,
                                                             uint32_t H, uint32_t tiles_x8,
-                                                            int budget, rt_hits hits) {
+                                                            rt_hits hits) {
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6, q = lane >> 2;
     TilePixel p;
     p.col = (blockIdx.x % tiles_x8) * 8 + (wave & 1) * 4 + (q & 3);
@@ -201,7 +201,7 @@ __global__ void __launch_bounds__(256) k_trace_camera_quad(SceneArgs sc, rt_came
     p.valid = p.col < W && p.row < H;
     const Ray r = camera_ray(cam, W, H, p);
     Hit h;
-    traverse_quad<false>(sc.nodes4, sc.tris, sc.root, r, p.valid, -1, h, budget);
+    traverse_quad<false>(sc.nodes4, sc.tris, sc.root, r, p.valid, -1, h);
     if (p.valid && (lane & 3) == 0) {
         if (h.prim >= 0) refine_hit(sc.tris, r, h);
         store_hit(sc, hits, (uint64_t)p.row * W + p.col, h);
@@ -581,7 +581,7 @@ extern "C" int rt_trace_camera(const rt_scene* scene, const rt_camera* cam, uint
     if (tu.primary == RT_TRAVERSAL_QUAD) {
         const uint32_t tx8 = (W + 7) / 8, ty8 = (H + 7) / 8;
         hipLaunchKernelGGL(k_trace_camera_quad, dim3(tx8 * ty8), dim3(256), 0, (hipStream_t)stream,
-                           scene_args(scene), *cam, W, H, tx8, 0, *hits);
+                           scene_args(scene), *cam, W, H, tx8, *hits);
         return check_launch("rt_trace_camera");
     }
     using K = decltype(&k_trace_camera<0>);
