@@ -66,18 +66,17 @@ RT_DEV bool intersect_tri(const float4 a, const float4 b, const float4 c, const 
     const float py = fmaf(r.dz, c.x, -(r.dx * c.z));
     const float pz = fmaf(r.dx, c.y, -(r.dy * c.x));
     const float det = fmaf(b.x, px, fmaf(b.y, py, b.z * pz));
-    if (det == 0.0f) return false;
     const float sg = det > 0.0f ? 1.0f : -1.0f, ad = det * sg;
     const float sx = r.ox - a.x, sy = r.oy - a.y, sz = r.oz - a.z;
     const float un = fmaf(sx, px, fmaf(sy, py, sz * pz));
     const float us = un * sg;
-    if (us < 0.0f || us > ad) return false;
+    if ((det == 0.0f) | (us < 0.0f) | (us > ad)) return false;  // one branch for the three tests
     const float qx = fmaf(sy, b.z, -(sz * b.y));
     const float qy = fmaf(sz, b.x, -(sx * b.z));
     const float qz = fmaf(sx, b.y, -(sy * b.x));
     const float vn = fmaf(r.dx, qx, fmaf(r.dy, qy, r.dz * qz));
     const float vs = vn * sg;
-    if (vs < 0.0f || us + vs > ad) return false;
+    if ((vs < 0.0f) | (us + vs > ad)) return false;
     const float inv = 1.0f / det;
     t = fmaf(c.x, qx, fmaf(c.y, qy, c.z * qz)) * inv;
     u = un * inv;
