@@ -44,7 +44,7 @@ def lib():
         L.tw_morton3.restype = ctypes.c_uint32
         L.tw_morton3.argtypes = [ctypes.c_uint32] * 3
         L.tw_build.restype = ctypes.c_int
-        L.tw_build.argtypes = [i64] + [vp] * 6 + [vp] * 7 + [vp]
+        L.tw_build.argtypes = [i64] + [vp] * 6 + [vp] * 7 + [vp, ctypes.c_int]
         L.tw_trace_closest.restype = None
         L.tw_trace_closest.argtypes = [vp, vp, i32, i64] + [vp] * 8 + [f32, f32] + [vp] * 6
         L.tw_trace_any.restype = None
@@ -151,7 +151,9 @@ def bilinear_fp64(tex, u, v):
 class TwinScene:
     """The CPU twin's build of a scene from per-primitive arrays (scenes.triangle_arrays)."""
 
-    def __init__(self, pos, nrm, uv, mesh_of, has_nrm, has_uv):
+    BUILDERS = {"lbvh": 0, "ploc": 1}  # the device's RT_BUILD_* values
+
+    def __init__(self, pos, nrm, uv, mesh_of, has_nrm, has_uv, builder="lbvh"):
         n = pos.shape[0]
         self.n = n
         self.pos = np.ascontiguousarray(pos, np.float32)
@@ -169,7 +171,8 @@ class TwinScene:
                             _p(np.ascontiguousarray(uv, np.float32)),
                             _p(np.ascontiguousarray(mesh_of, np.int32)), _p(hn), _p(hu),
                             _p(self.keys), _p(self.order), _p(self.nodes), _p(self.nodes4), _p(self.tris),
-                            _p(self.shade), _p(self.bounds), ctypes.byref(root))
+                            _p(self.shade), _p(self.bounds), ctypes.byref(root),
+                            self.BUILDERS[builder])
         if rc != 0:
             raise MemoryError("tw_build failed")
         self.root = root.value

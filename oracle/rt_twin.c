@@ -120,10 +120,16 @@ static int cmp_u64(const void* a, const void* b) {
     return (x > y) - (x < y);
 }
 
+/* Builders (the device's rt_tuning.builder): 0 = LBVH (Karras radix tree over the Morton order),
+ * 1 = PLOC (locally ordered clustering over the Morton order, leaves collapsed to <= LEAF_MAX). */
+#define TW_BUILD_LBVH 0
+#define TW_BUILD_PLOC 1
+static int ploc_hierarchy(int64_t n, const box3* leafbox, uint32_t* order, float* nodes, float* bounds);
+
 TW_API int tw_build(int64_t n, const float* pos, const float* nrm, const float* uv,
                     const int32_t* mesh_of, const uint8_t* has_nrm, const uint8_t* has_uv,
                     uint32_t* keys, uint32_t* order, float* nodes, float* nodes4, float* tris,
-                    float* shade, float* bounds, int32_t* root_out) {
+                    float* shade, float* bounds, int32_t* root_out, int builder) {
     if (n <= 0) {
         for (int c = 0; c < 3; ++c) { bounds[c] = INFINITY; bounds[4 + c] = -INFINITY; }
         bounds[3] = bounds[7] = 0.0f;
@@ -207,15 +213,9 @@ TW_API int tw_build(int64_t n, const float* pos, const float* nrm, const float* 
     for (int64_t i = 0; i < n; ++i) { keys[i] = (uint32_t)(kv[i] >> 32); order[i] = (uint32_t)kv[i]; }
     free(kv);
 
-    /* leaf records in slot order */
-    #pragma omp parallel for schedule(static)
-    for (int64_t s = 0; s < n; ++s) {
-        uint32_t p = order[s];
-        memcpy(tris + s * 12, tri_u + (int64_t)p * 12, 48);
-        memcpy(shade + s * 16, sh_u + (int64_t)p * 16, 64);
-    }
-
     if (n == 1) {
+        memcpy(tris, tri_u + (int64_t)order[0] * 12, 48);
+        memcpy(shade, sh_u + (int64_t)order[0] * 16, 64);
         box3 b = leafbox[order[0]];
         for (int c = 0; c < 3; ++c) { bounds[c] = b.lo[c]; bounds[4 + c] = b.hi[c]; }
         bounds[3] = bounds[7] = 0.0f;
@@ -224,6 +224,9 @@ TW_API int tw_build(int64_t n, const float* pos, const float* nrm, const float* 
         return 0;
     }
 
+    if (builder == TW_BUILD_PLOC) {
+        if (ploc_hierarchy(n, leafbox, order, nodes, bounds) != 0) return -1;
+    } else {
     /* Karras 2012 binary radix tree over the 62-bit extended keys (code, slot) */
     int32_t* left = (int32_t*)malloc(sizeof(int32_t) * (n - 1));
     int32_t* right = (int32_t*)malloc(sizeof(int32_t) * (n - 1));
@@ -290,6 +293,15 @@ TW_API int tw_build(int64_t n, const float* pos, const float* nrm, const float* 
     }
     for (int c = 0; c < 3; ++c) { bounds[c] = nb[0].lo[c]; bounds[4 + c] = nb[0].hi[c]; }
     bounds[3] = bounds[7] = 0.0f;
+    free(stack); free(state); free(nb); free(left); free(right); free(tleft); free(tright);
+    }
+    /* leaf records in slot order */
+    #pragma omp parallel for schedule(static)
+    for (int64_t s = 0; s < n; ++s) {
+        uint32_t p = order[s];
+        memcpy(tris + s * 12, tri_u + (int64_t)p * 12, 48);
+        memcpy(shade + s * 16, sh_u + (int64_t)p * 16, 64);
+    }
     *root_out = n <= LEAF_MAX ? leaf_ref(0, (uint32_t)n) : 0;
     /* 4-wide records: grandchildren through internal children, leaf children directly */
     #pragma omp parallel for schedule(static)
@@ -323,8 +335,163 @@ TW_API int tw_build(int64_t n, const float* pos, const float* nrm, const float* 
         for (int q = 0; q < 4; ++q) o[24 + q] = bitsf((uint32_t)ref[q]);
         for (int q = 0; q < 4; ++q) o[28 + q] = 0.0f;
     }
-    free(stack); free(state); free(nb); free(left); free(right); free(tleft); free(tright);
     free(leafbox); free(cen); free(tri_u); free(sh_u);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* PLOC: parallel locally-ordered clustering (Meister & Bittner 2018), the device's iteration   */
+/* ------------------------------------------------------------------------------------------ */
+/* Clusters start as the triangles in Morton order. Each iteration: every cluster i finds its
+ * nearest neighbour NN(i) among positions [i-R, i+R] (smallest surface area of the union box; ties
+ * to the smaller position); mutual pairs (NN(NN(i)) == i) merge into a new node at the lower
+ * position, the other position drops out; the survivors keep their order. An iteration without a
+ * merge (only possible with NaN boxes) merges positions 0 and 1. The k-th node created (counting
+ * over iterations in position order) gets id n-2-k, so the root is 0 and children have larger
+ * ids than their parents. Then subtrees of <= LEAF_MAX triangles whose parent is larger become
+ * leaves ("leaf roots"): the leaf roots take consecutive slot ranges in the order of the smallest
+ * Morton position of their triangles, and the triangles of a leaf root take its range in
+ * left-to-right order. */
+#define PLOC_RADIUS 16
+
+static inline float union_area(const box3* a, const box3* b) {
+    float dx = fmaxf(a->hi[0], b->hi[0]) - fminf(a->lo[0], b->lo[0]);
+    float dy = fmaxf(a->hi[1], b->hi[1]) - fminf(a->lo[1], b->lo[1]);
+    float dz = fmaxf(a->hi[2], b->hi[2]) - fminf(a->lo[2], b->lo[2]);
+    return dx * dy + dy * dz + dz * dx;
+}
+
+typedef struct {
+    int64_t n;
+    const int32_t *left, *right, *size;
+    const uint32_t* pos_of;
+} ploc_tree;
+
+static inline int32_t ploc_size(const ploc_tree* t, int32_t c) { return c >= 0 ? t->size[c] : 1; }
+
+static uint32_t ploc_minpos(const ploc_tree* t, int32_t c) {
+    if (c < 0) return t->pos_of[~c];
+    uint32_t a = ploc_minpos(t, t->left[c]), b = ploc_minpos(t, t->right[c]);
+    return a < b ? a : b;
+}
+
+static void ploc_slots(const ploc_tree* t, int32_t c, uint32_t* next, uint32_t* order,
+                       uint32_t* slot_of, uint32_t* fslot) {
+    if (c < 0) {
+        order[*next] = (uint32_t)~c;
+        slot_of[~c] = *next;
+        ++*next;
+        return;
+    }
+    fslot[c] = *next;
+    ploc_slots(t, t->left[c], next, order, slot_of, fslot);
+    ploc_slots(t, t->right[c], next, order, slot_of, fslot);
+}
+
+static int ploc_hierarchy(int64_t n, const box3* leafbox, uint32_t* order, float* nodes, float* bounds) {
+    int32_t* C = (int32_t*)malloc(sizeof(int32_t) * n);
+    int32_t* C2 = (int32_t*)malloc(sizeof(int32_t) * n);
+    int64_t* NN = (int64_t*)malloc(sizeof(int64_t) * n);
+    box3* B = (box3*)malloc(sizeof(box3) * n);
+    box3* B2 = (box3*)malloc(sizeof(box3) * n);
+    int32_t* left = (int32_t*)malloc(sizeof(int32_t) * (n - 1));
+    int32_t* right = (int32_t*)malloc(sizeof(int32_t) * (n - 1));
+    int32_t* size = (int32_t*)malloc(sizeof(int32_t) * (n - 1));
+    int32_t* parent = (int32_t*)malloc(sizeof(int32_t) * (2 * n - 1)); /* internal ids, then n-1+p */
+    uint32_t* pos_of = (uint32_t*)malloc(sizeof(uint32_t) * n);
+    uint32_t* morder = (uint32_t*)malloc(sizeof(uint32_t) * n);
+    uint32_t* slot_of = (uint32_t*)malloc(sizeof(uint32_t) * n);
+    uint32_t* fslot = (uint32_t*)malloc(sizeof(uint32_t) * (n - 1));
+    if (!C || !C2 || !NN || !B || !B2 || !left || !right || !size || !parent || !pos_of || !morder ||
+        !slot_of || !fslot)
+        return -1;
+    for (int64_t i = 0; i < n; ++i) {
+        morder[i] = order[i];
+        pos_of[order[i]] = (uint32_t)i;
+        C[i] = ~(int32_t)order[i];
+        B[i] = leafbox[order[i]];
+    }
+    int64_t m = n, created = 0;
+    while (m > 1) {
+        #pragma omp parallel for schedule(static)
+        for (int64_t i = 0; i < m; ++i) {
+            int64_t lo = i - PLOC_RADIUS < 0 ? 0 : i - PLOC_RADIUS;
+            int64_t hi = i + PLOC_RADIUS > m - 1 ? m - 1 : i + PLOC_RADIUS;
+            int64_t bj = lo == i ? i + 1 : lo;
+            float best = union_area(&B[i], &B[bj]);
+            for (int64_t j = bj + 1; j <= hi; ++j) {
+                if (j == i) continue;
+                float a = union_area(&B[i], &B[j]);
+                if (a < best) { best = a; bj = j; }
+            }
+            NN[i] = bj;
+        }
+        int any = 0;
+        for (int64_t i = 0; i < m && !any; ++i) any = NN[NN[i]] == i;
+        if (!any) { NN[0] = 1; NN[1] = 0; }
+        int64_t k = 0;
+        for (int64_t i = 0; i < m; ++i) {
+            int64_t j = NN[i];
+            if (NN[j] == i) {
+                if (j < i) continue;
+                int32_t id = (int32_t)(n - 2 - created);
+                ++created;
+                left[id] = C[i];
+                right[id] = C[j];
+                size[id] = (C[i] >= 0 ? size[C[i]] : 1) + (C[j] >= 0 ? size[C[j]] : 1);
+                parent[C[i] >= 0 ? C[i] : n - 1 + ~C[i]] = id;
+                parent[C[j] >= 0 ? C[j] : n - 1 + ~C[j]] = id;
+                const box3* a = &B[i];
+                const box3* b = &B[j];
+                float* nd = nodes + (int64_t)id * 16;
+                nd[0] = a->lo[0]; nd[1] = a->hi[0]; nd[2] = a->lo[1]; nd[3] = a->hi[1];
+                nd[4] = b->lo[0]; nd[5] = b->hi[0]; nd[6] = b->lo[1]; nd[7] = b->hi[1];
+                nd[8] = a->lo[2]; nd[9] = a->hi[2]; nd[10] = b->lo[2]; nd[11] = b->hi[2];
+                nd[14] = 0.0f; nd[15] = 0.0f;
+                box3 u;
+                for (int c = 0; c < 3; ++c) { u.lo[c] = fminf(a->lo[c], b->lo[c]); u.hi[c] = fmaxf(a->hi[c], b->hi[c]); }
+                C2[k] = id;
+                B2[k] = u;
+                ++k;
+            } else {
+                C2[k] = C[i];
+                B2[k] = B[i];
+                ++k;
+            }
+        }
+        int32_t* tc = C; C = C2; C2 = tc;
+        box3* tb = B; B = B2; B2 = tb;
+        m = k;
+    }
+    parent[0] = -1;
+    for (int c = 0; c < 3; ++c) { bounds[c] = B[0].lo[c]; bounds[4 + c] = B[0].hi[c]; }
+    bounds[3] = bounds[7] = 0.0f;
+    /* leaf roots in the order of their smallest Morton position; slots left to right */
+    ploc_tree t = {n, left, right, size, pos_of};
+    uint32_t next = 0;
+    for (int64_t i = 0; i < n; ++i) {
+        int32_t x = ~(int32_t)morder[i];
+        for (;;) {
+            int32_t p = parent[x >= 0 ? x : n - 1 + ~x];
+            if (p < 0 || size[p] > LEAF_MAX) break;
+            x = p;
+        }
+        if (ploc_minpos(&t, x) != (uint32_t)i) continue;
+        ploc_slots(&t, x, &next, order, slot_of, fslot);
+    }
+    /* traversal refs: a child of more than LEAF_MAX triangles by id, else the leaf of its slots */
+    for (int64_t x = 0; x < n - 1; ++x) {
+        int32_t ch[2] = {left[x], right[x]};
+        for (int e = 0; e < 2; ++e) {
+            int32_t c = ch[e], r;
+            if (c < 0) r = leaf_ref(slot_of[~c], 1);
+            else if (size[c] > LEAF_MAX) r = c;
+            else r = leaf_ref(fslot[c], (uint32_t)size[c]);
+            nodes[x * 16 + 12 + e] = bitsf((uint32_t)r);
+        }
+    }
+    free(C); free(C2); free(NN); free(B); free(B2); free(left); free(right); free(size);
+    free(parent); free(pos_of); free(morder); free(slot_of); free(fslot);
     return 0;
 }
 

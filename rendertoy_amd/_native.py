@@ -12,7 +12,7 @@ import torch  # noqa: F401  (must be imported before the HIP library; see module
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "librtx_hip.so")
 
-RT_ABI_VERSION = 7
+RT_ABI_VERSION = 8
 RT_ABSENT = 0xFFFFFFFF
 RT_EMPTY_ROOT = 0x7FFFFFFF
 RT_RENDER_SHADOWS = 1
@@ -23,6 +23,8 @@ RT_TRAVERSAL_HYBRID = 2
 RT_TRAVERSAL_AUTO = 3
 RT_TRAVERSAL_QUAD = 4
 RT_LANE_STACK_MAX = 32
+RT_BUILD_LBVH = 0
+RT_BUILD_PLOC = 1
 RT_BATCH_MAX_JOBS = 4096
 RT_BATCH_MAX_JOB_PIXELS = 1 << 24
 RT_PATHS_MAX_BOUNCES = 16
@@ -138,6 +140,7 @@ class rt_tuning(ctypes.Structure):
         ("super_tile", ctypes.c_uint32),
         ("lane_stack", ctypes.c_uint32),
         ("packet_budget", ctypes.c_uint32),
+        ("builder", ctypes.c_uint32),
     ]
 
 

@@ -11,7 +11,7 @@ thread_local std::string g_last_error;
 std::mutex g_cu_mutex;
 int g_cu_count[64] = {0};
 std::mutex g_tuning_mutex;
-rt_tuning g_tuning = {RT_TRAVERSAL_QUAD, RT_TRAVERSAL_LANE, 0u, (uint32_t)RT_LANE_STACK_MAX, 256u};
+rt_tuning g_tuning = {RT_TRAVERSAL_QUAD, RT_TRAVERSAL_LANE, 0u, (uint32_t)RT_LANE_STACK_MAX, 256u, RT_BUILD_LBVH};
 }  // namespace
 
 rt_tuning current_tuning() {
@@ -68,6 +68,8 @@ extern "C" int rt_set_tuning(const rt_tuning* t) {
     if (t->super_tile > 64) return rt::set_error(RT_EINVAL, "rt_set_tuning: super_tile must be 0..64");
     if (t->lane_stack < 1 || t->lane_stack > RT_LANE_STACK_MAX)
         return rt::set_error(RT_EINVAL, "rt_set_tuning: lane_stack must be 1..RT_LANE_STACK_MAX");
+    if (t->builder != RT_BUILD_LBVH && t->builder != RT_BUILD_PLOC)
+        return rt::set_error(RT_EINVAL, "rt_set_tuning: builder must be RT_BUILD_LBVH or RT_BUILD_PLOC");
     std::lock_guard<std::mutex> lock(rt::g_tuning_mutex);
     rt::g_tuning = *t;
     return RT_OK;

@@ -21,6 +21,7 @@
 // Every result is an integer or an exact FP32 min/max/sub, so the output is byte-identical to the
 // CPU restatement under the same inputs.
 #include <algorithm>
+#include <cmath>
 #include <utility>
 
 #include "rt_common.h"
@@ -55,6 +56,20 @@ struct Workspace {
     float4* nodebox;   // 2(n-1)
     int32_t* top_list; // n-1: internal nodes whose leaf range spans several refit blocks
     uint32_t* top_count;
+    // PLOC builder
+    int32_t* cid[2];   // n: cluster refs by position (internal id, or ~prim), ping-pong
+    float4* cbox[2];   // 2n: cluster boxes by position, ping-pong
+    int32_t* nn;       // n: nearest neighbour position
+    int32_t* nleft;    // n-1: children and triangle count of each internal node
+    int32_t* nright;
+    int32_t* nsize;
+    uint32_t* pos_of;  // n: Morton position of each primitive
+    uint32_t* slot_of; // n: slot of each primitive
+    uint32_t* fslot;   // n-1: first slot of each internal node inside a leaf
+    uint32_t* scan;    // n: leaf-root sizes at their first Morton position, then their first slot
+    uint32_t* part;    // 2 * chunks: per-chunk counts, then exclusive offsets
+    int32_t* state;    // 8: PlocState per iteration parity
+    uint32_t* slot_order;  // n: primitive of each slot
     uint64_t bytes;
 };
 
@@ -89,6 +104,22 @@ Workspace carve(char* base, uint64_t n) {
     w.nodebox = (float4*)take(32 * ni);
     w.top_list = (int32_t*)take(4 * ni);
     w.top_count = (uint32_t*)take(4);
+    const uint64_t nchunk = (n + 1023) / 1024 + 1;
+    for (int k = 0; k < 2; ++k) {
+        w.cid[k] = (int32_t*)take(4 * n);
+        w.cbox[k] = (float4*)take(32 * n);
+    }
+    w.nn = (int32_t*)take(4 * n);
+    w.nleft = (int32_t*)take(4 * ni);
+    w.nright = (int32_t*)take(4 * ni);
+    w.nsize = (int32_t*)take(4 * ni);
+    w.pos_of = (uint32_t*)take(4 * n);
+    w.slot_of = (uint32_t*)take(4 * n);
+    w.fslot = (uint32_t*)take(4 * ni);
+    w.scan = (uint32_t*)take(4 * n);
+    w.part = (uint32_t*)take(8 * nchunk);
+    w.state = (int32_t*)take(4 * 8);
+    w.slot_order = (uint32_t*)take(4 * n);
     w.bytes = off;
     return w;
 }
@@ -634,6 +665,540 @@ __global__ void __launch_bounds__(1024) k_refit_top(const uint32_t* __restrict__
 
 __global__ void k_zero_word(uint32_t* w) { *w = 0u; }
 
+// ---- PLOC builder (parallel locally-ordered clustering, Meister & Bittner 2018) ----
+// Clusters start as the triangles in Morton order (positions 0..n-1, refs ~prim, padded leaf
+// boxes). Each iteration: every cluster i picks its nearest neighbour NN(i) among positions
+// [i-R, i+R] (smallest surface area of the union box, ties to the smaller position); mutual pairs
+// merge into a new node kept at the lower position, the higher one drops out, survivors keep their
+// order. An iteration with no mutual pair (possible only with NaN boxes) merges positions 0 and 1.
+// The k-th node created (over iterations, in position order) gets id n-2-k: the root is 0 and
+// children have larger ids than their parents. Large iterations run as four grid launches
+// (neighbours; per-chunk counts; chunk offsets and the next cluster count; compaction); once at
+// most kPlocTail clusters remain, one workgroup finishes in a single launch. Subtrees of at most
+// kLeafMax triangles whose parent is larger become leaves: leaf roots take consecutive slot ranges
+// in the order of their smallest Morton position, their triangles left to right.
+constexpr int kPlocRadius = 16;
+constexpr int kPlocTile = 256;        // clusters per neighbour-search tile (one per thread)
+constexpr int kPlocChunk = 1024;      // clusters per count / compaction chunk (4 per thread)
+constexpr int kPlocGrid = 1024;       // blocks of the grid-stride iteration kernels
+constexpr int32_t kPlocTail = 4096;   // cluster count handed to the one-workgroup tail
+
+// state[4 * parity]: clusters m, nodes created, forced merge, buffer parity of the clusters
+struct PlocState {
+    int32_t m, created, forced, buf;
+};
+
+RT_DEV float union_area(const float4& alo, const float4& ahi, const float4& blo, const float4& bhi) {
+    const float dx = fmaxf(ahi.x, bhi.x) - fminf(alo.x, blo.x);
+    const float dy = fmaxf(ahi.y, bhi.y) - fminf(alo.y, blo.y);
+    const float dz = fmaxf(ahi.z, bhi.z) - fminf(alo.z, blo.z);
+    return dx * dy + dy * dz + dz * dx;
+}
+
+RT_DEV int32_t ploc_size(const int32_t* __restrict__ nsize, int32_t c) { return c >= 0 ? nsize[c] : 1; }
+
+__global__ void __launch_bounds__(256) k_ploc_init(const uint32_t* __restrict__ morder, uint32_t n,
+                                                    const float4* __restrict__ leafbox,
+                                                    int32_t* __restrict__ cid, float4* __restrict__ cbox,
+                                                    uint32_t* __restrict__ pos_of,
+                                                    int32_t* __restrict__ parent, int32_t* __restrict__ state) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t p = morder[i];
+    pos_of[p] = i;
+    cid[i] = ~(int32_t)p;
+    cbox[2 * (uint64_t)i] = leafbox[2 * (uint64_t)p];
+    cbox[2 * (uint64_t)i + 1] = leafbox[2 * (uint64_t)p + 1];
+    if (i == 0) {
+        parent[0] = -1;
+        state[0] = (int32_t)n;
+        state[1] = 0;
+        state[2] = 0;
+        state[3] = 0;
+    }
+}
+
+// nearest neighbour of position i among [i-R, i+R] (m >= 2), reading boxes through `box(j)`
+template <typename BoxAt>
+RT_DEV int32_t ploc_nn(int32_t i, int32_t m, BoxAt box) {
+    const int32_t lo = i - kPlocRadius < 0 ? 0 : i - kPlocRadius;
+    const int32_t hi = i + kPlocRadius > m - 1 ? m - 1 : i + kPlocRadius;
+    float4 ilo, ihi, jlo, jhi;
+    box(i, ilo, ihi);
+    int32_t bj = lo == i ? i + 1 : lo;
+    box(bj, jlo, jhi);
+    float best = union_area(ilo, ihi, jlo, jhi);
+    for (int32_t j = bj + 1; j <= hi; ++j) {
+        if (j == i) continue;
+        box(j, jlo, jhi);
+        const float a = union_area(ilo, ihi, jlo, jhi);
+        if (a < best) {
+            best = a;
+            bj = j;
+        }
+    }
+    return bj;
+}
+
+__global__ void __launch_bounds__(kPlocTile) k_ploc_nn(int it, const int32_t* __restrict__ state,
+                                                        const float4* __restrict__ cbox0,
+                                                        const float4* __restrict__ cbox1,
+                                                        int32_t* __restrict__ nn) {
+    const int32_t m = state[4 * (it & 1)];
+    if (m <= kPlocTail) return;
+    const float4* __restrict__ B = (it & 1) ? cbox1 : cbox0;
+    __shared__ float4 wlo[kPlocTile + 2 * kPlocRadius], whi[kPlocTile + 2 * kPlocRadius];
+    for (int32_t base = blockIdx.x * kPlocTile; base < m; base += gridDim.x * kPlocTile) {
+        const int32_t wb = base - kPlocRadius;
+        for (int k = threadIdx.x; k < kPlocTile + 2 * kPlocRadius; k += kPlocTile) {
+            const int32_t g = wb + k;
+            if (g >= 0 && g < m) {
+                wlo[k] = B[2 * (uint64_t)g];
+                whi[k] = B[2 * (uint64_t)g + 1];
+            }
+        }
+        __syncthreads();
+        const int32_t i = base + (int32_t)threadIdx.x;
+        if (i < m)
+            nn[i] = ploc_nn(i, m, [&](int32_t j, float4& lo, float4& hi) {
+                lo = wlo[j - wb];
+                hi = whi[j - wb];
+            });
+        __syncthreads();
+    }
+}
+
+// keep / merge flags of position i (forced: the iteration's fallback merge of 0 and 1)
+RT_DEV void ploc_flags(const int32_t* __restrict__ nn, int32_t i, bool forced, uint32_t& keep,
+                       uint32_t& merge) {
+    if (forced) {
+        keep = i != 1;
+        merge = i == 0;
+        return;
+    }
+    const int32_t j = nn[i];
+    const bool mutual = nn[j] == i;
+    keep = !mutual || i < j;
+    merge = mutual && i < j;
+}
+
+// exclusive scan of (keep, merge) pairs packed as keep | merge << 32 over a 1024-thread... block
+template <int T>
+RT_DEV uint64_t block_exclusive_scan(uint64_t v, uint64_t* lds, uint64_t& total) {
+    const int t = threadIdx.x;
+    lds[t] = v;
+    __syncthreads();
+    for (int d = 1; d < T; d <<= 1) {
+        const uint64_t x = t >= d ? lds[t - d] : 0;
+        __syncthreads();
+        lds[t] += x;
+        __syncthreads();
+    }
+    total = lds[T - 1];
+    const uint64_t incl = lds[t];
+    __syncthreads();
+    return incl - v;
+}
+
+// per chunk of kPlocChunk positions: number kept and number merged
+__global__ void __launch_bounds__(256) k_ploc_count(int it, const int32_t* __restrict__ state,
+                                                     const int32_t* __restrict__ nn,
+                                                     uint32_t* __restrict__ part) {
+    const int32_t m = state[4 * (it & 1)];
+    if (m <= kPlocTail) return;
+    __shared__ uint64_t lds[256];
+    const int32_t nchunk = (m + kPlocChunk - 1) / kPlocChunk;
+    for (int32_t c = blockIdx.x; c < nchunk; c += gridDim.x) {
+        uint64_t v = 0;
+        for (int q = 0; q < 4; ++q) {
+            const int32_t i = c * kPlocChunk + (int32_t)threadIdx.x * 4 + q;
+            if (i < m) {
+                uint32_t keep, merge;
+                ploc_flags(nn, i, false, keep, merge);
+                v += (uint64_t)keep | ((uint64_t)merge << 32);
+            }
+        }
+        uint64_t total;
+        block_exclusive_scan<256>(v, lds, total);
+        if (threadIdx.x == 0) {
+            part[2 * (uint64_t)c] = (uint32_t)total;
+            part[2 * (uint64_t)c + 1] = (uint32_t)(total >> 32);
+        }
+    }
+}
+
+// one workgroup: chunk counts -> exclusive offsets (in place); the next iteration's state
+__global__ void __launch_bounds__(1024) k_ploc_offsets(int it, int32_t* __restrict__ state,
+                                                        uint32_t* __restrict__ part) {
+    const int32_t* cur = state + 4 * (it & 1);
+    int32_t* nxt = state + 4 * ((it + 1) & 1);
+    const int32_t m = cur[0], created = cur[1];
+    if (m <= kPlocTail) {
+        if (threadIdx.x == 0) {
+            nxt[0] = m;
+            nxt[1] = created;
+            nxt[2] = 0;
+            nxt[3] = cur[3];
+        }
+        return;
+    }
+    __shared__ uint64_t lds[1024];
+    __shared__ uint64_t carry;
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    const int32_t nchunk = (m + kPlocChunk - 1) / kPlocChunk;
+    for (int32_t b = 0; b < nchunk; b += 1024) {
+        const int32_t c = b + (int32_t)threadIdx.x;
+        const uint64_t v = c < nchunk ? (uint64_t)part[2 * (uint64_t)c] | ((uint64_t)part[2 * (uint64_t)c + 1] << 32) : 0;
+        uint64_t total;
+        const uint64_t ex = block_exclusive_scan<1024>(v, lds, total) + carry;
+        if (c < nchunk) {
+            part[2 * (uint64_t)c] = (uint32_t)ex;
+            part[2 * (uint64_t)c + 1] = (uint32_t)(ex >> 32);
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) carry += total;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const uint32_t kept = (uint32_t)carry, merged = (uint32_t)(carry >> 32);
+        const bool forced = merged == 0;
+        nxt[0] = forced ? m - 1 : (int32_t)kept;
+        nxt[1] = created + (forced ? 1 : (int32_t)merged);
+        nxt[2] = forced ? 1 : 0;
+        nxt[3] = (it + 1) & 1;
+    }
+}
+
+// writes the merged node `id` from the clusters at positions i < j
+RT_DEV void ploc_merge(uint32_t n, int32_t id, int32_t ci, int32_t cj, float4 ilo, float4 ihi,
+                       float4 jlo, float4 jhi, int32_t* __restrict__ nleft, int32_t* __restrict__ nright,
+                       int32_t* __restrict__ nsize, int32_t* __restrict__ parent, float4* __restrict__ nodes,
+                       float* __restrict__ bounds, float4& ulo, float4& uhi) {
+    const uint64_t ni = (uint64_t)n - 1;
+    nleft[id] = ci;
+    nright[id] = cj;
+    nsize[id] = ploc_size(nsize, ci) + ploc_size(nsize, cj);
+    parent[ci >= 0 ? (uint64_t)ci : ni + (uint64_t)(~ci)] = id;
+    parent[cj >= 0 ? (uint64_t)cj : ni + (uint64_t)(~cj)] = id;
+    float4* nd = nodes + (uint64_t)id * 4;
+    nd[0] = make_float4(ilo.x, ihi.x, ilo.y, ihi.y);
+    nd[1] = make_float4(jlo.x, jhi.x, jlo.y, jhi.y);
+    nd[2] = make_float4(ilo.z, ihi.z, jlo.z, jhi.z);
+    ulo = make_float4(fminf(ilo.x, jlo.x), fminf(ilo.y, jlo.y), fminf(ilo.z, jlo.z), 0.0f);
+    uhi = make_float4(fmaxf(ihi.x, jhi.x), fmaxf(ihi.y, jhi.y), fmaxf(ihi.z, jhi.z), 0.0f);
+    if (id == 0) {
+        bounds[0] = ulo.x; bounds[1] = ulo.y; bounds[2] = ulo.z; bounds[3] = 0.0f;
+        bounds[4] = uhi.x; bounds[5] = uhi.y; bounds[6] = uhi.z; bounds[7] = 0.0f;
+    }
+}
+
+// compaction of one chunk: survivors to their new positions, merged pairs as new nodes
+__global__ void __launch_bounds__(256) k_ploc_compact(int it, uint32_t n, const int32_t* __restrict__ state,
+                                                       const int32_t* __restrict__ nn,
+                                                       const uint32_t* __restrict__ part,
+                                                       int32_t* __restrict__ cid0, int32_t* __restrict__ cid1,
+                                                       float4* __restrict__ cbox0, float4* __restrict__ cbox1,
+                                                       int32_t* __restrict__ nleft, int32_t* __restrict__ nright,
+                                                       int32_t* __restrict__ nsize, int32_t* __restrict__ parent,
+                                                       float4* __restrict__ nodes, float* __restrict__ bounds) {
+    const int32_t* cur = state + 4 * (it & 1);
+    const int32_t m = cur[0], created = cur[1];
+    if (m <= kPlocTail) return;
+    const bool forced = state[4 * ((it + 1) & 1) + 2] != 0;
+    const int32_t* __restrict__ C = (it & 1) ? cid1 : cid0;
+    const float4* __restrict__ B = (it & 1) ? cbox1 : cbox0;
+    int32_t* __restrict__ C2 = (it & 1) ? cid0 : cid1;
+    float4* __restrict__ B2 = (it & 1) ? cbox0 : cbox1;
+    __shared__ uint64_t lds[256];
+    const int32_t nchunk = (m + kPlocChunk - 1) / kPlocChunk;
+    for (int32_t c = blockIdx.x; c < nchunk; c += gridDim.x) {
+        uint32_t keep[4], merge[4];
+        uint64_t v = 0;
+        for (int q = 0; q < 4; ++q) {
+            const int32_t i = c * kPlocChunk + (int32_t)threadIdx.x * 4 + q;
+            keep[q] = merge[q] = 0;
+            if (i < m) ploc_flags(nn, i, forced, keep[q], merge[q]);
+            v += (uint64_t)keep[q] | ((uint64_t)merge[q] << 32);
+        }
+        uint64_t total;
+        uint64_t ex = block_exclusive_scan<256>(v, lds, total);
+        uint32_t dst = part[2 * (uint64_t)c] + (uint32_t)ex;
+        uint32_t rank = (forced ? 0u : part[2 * (uint64_t)c + 1]) + (uint32_t)(ex >> 32);
+        for (int q = 0; q < 4; ++q) {
+            const int32_t i = c * kPlocChunk + (int32_t)threadIdx.x * 4 + q;
+            if (!keep[q]) continue;
+            if (forced) dst = i == 0 ? 0u : (uint32_t)(i - 1);
+            const float4 ilo = B[2 * (uint64_t)i], ihi = B[2 * (uint64_t)i + 1];
+            if (merge[q]) {
+                const int32_t j = forced ? 1 : nn[i];
+                const int32_t id = (int32_t)n - 2 - (created + (int32_t)rank);
+                float4 ulo, uhi;
+                ploc_merge(n, id, C[i], C[j], ilo, ihi, B[2 * (uint64_t)j], B[2 * (uint64_t)j + 1], nleft,
+                           nright, nsize, parent, nodes, bounds, ulo, uhi);
+                C2[dst] = id;
+                B2[2 * (uint64_t)dst] = ulo;
+                B2[2 * (uint64_t)dst + 1] = uhi;
+                ++rank;
+            } else {
+                C2[dst] = C[i];
+                B2[2 * (uint64_t)dst] = ilo;
+                B2[2 * (uint64_t)dst + 1] = ihi;
+            }
+            ++dst;
+        }
+    }
+}
+
+// The remaining iterations (m <= kPlocTail normally; any m is handled) in one workgroup, from the
+// state left in state[4 * slot].
+__global__ void __launch_bounds__(1024) k_ploc_tail(int slot, uint32_t n, int32_t* __restrict__ state,
+                                                     int32_t* __restrict__ nn, int32_t* __restrict__ cid0,
+                                                     int32_t* __restrict__ cid1, float4* __restrict__ cbox0,
+                                                     float4* __restrict__ cbox1, int32_t* __restrict__ nleft,
+                                                     int32_t* __restrict__ nright, int32_t* __restrict__ nsize,
+                                                     int32_t* __restrict__ parent, float4* __restrict__ nodes,
+                                                     float* __restrict__ bounds) {
+    __shared__ uint64_t lds[1024];
+    __shared__ int32_t sm, screated;
+    __shared__ uint32_t smerged;
+    __shared__ uint64_t carry;
+    if (threadIdx.x == 0) {
+        sm = state[4 * slot];
+        screated = state[4 * slot + 1];
+    }
+    int par = state[4 * slot + 3];
+    __syncthreads();
+    for (;;) {
+        const int32_t m = sm, created = screated;
+        if (m <= 1) break;
+        const int32_t* __restrict__ C = par ? cid1 : cid0;
+        const float4* __restrict__ B = par ? cbox1 : cbox0;
+        int32_t* __restrict__ C2 = par ? cid0 : cid1;
+        float4* __restrict__ B2 = par ? cbox0 : cbox1;
+        for (int32_t i = threadIdx.x; i < m; i += blockDim.x)
+            nn[i] = ploc_nn(i, m, [&](int32_t j, float4& lo, float4& hi) {
+                lo = B[2 * (uint64_t)j];
+                hi = B[2 * (uint64_t)j + 1];
+            });
+        if (threadIdx.x == 0) smerged = 0;
+        __syncthreads();
+        uint32_t mine = 0;
+        for (int32_t i = threadIdx.x; i < m; i += blockDim.x) {
+            uint32_t keep, merge;
+            ploc_flags(nn, i, false, keep, merge);
+            mine += merge;
+        }
+        if (mine) atomicAdd(&smerged, mine);
+        __syncthreads();
+        const bool forced = smerged == 0;
+        if (threadIdx.x == 0) carry = 0;
+        __syncthreads();
+        for (int32_t b = 0; b < m; b += 4 * (int32_t)blockDim.x) {
+            uint32_t keep[4], merge[4];
+            uint64_t v = 0;
+            for (int q = 0; q < 4; ++q) {
+                const int32_t i = b + (int32_t)threadIdx.x * 4 + q;
+                keep[q] = merge[q] = 0;
+                if (i < m) ploc_flags(nn, i, forced, keep[q], merge[q]);
+                v += (uint64_t)keep[q] | ((uint64_t)merge[q] << 32);
+            }
+            uint64_t total;
+            const uint64_t ex = block_exclusive_scan<1024>(v, lds, total) + carry;
+            uint32_t dst = (uint32_t)ex, rank = (uint32_t)(ex >> 32);
+            for (int q = 0; q < 4; ++q) {
+                const int32_t i = b + (int32_t)threadIdx.x * 4 + q;
+                if (!keep[q]) continue;
+                if (forced) dst = i == 0 ? 0u : (uint32_t)(i - 1);
+                const float4 ilo = B[2 * (uint64_t)i], ihi = B[2 * (uint64_t)i + 1];
+                if (merge[q]) {
+                    const int32_t j = forced ? 1 : nn[i];
+                    const int32_t id = (int32_t)n - 2 - (created + (int32_t)rank);
+                    float4 ulo, uhi;
+                    ploc_merge(n, id, C[i], C[j], ilo, ihi, B[2 * (uint64_t)j], B[2 * (uint64_t)j + 1],
+                               nleft, nright, nsize, parent, nodes, bounds, ulo, uhi);
+                    C2[dst] = id;
+                    B2[2 * (uint64_t)dst] = ulo;
+                    B2[2 * (uint64_t)dst + 1] = uhi;
+                    ++rank;
+                } else {
+                    C2[dst] = C[i];
+                    B2[2 * (uint64_t)dst] = ilo;
+                    B2[2 * (uint64_t)dst + 1] = ihi;
+                }
+                ++dst;
+            }
+            __syncthreads();
+            if (threadIdx.x == 0) carry += total;
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) {
+            sm = forced ? m - 1 : (int32_t)(uint32_t)carry;
+            screated = created + (forced ? 1 : (int32_t)(carry >> 32));
+        }
+        par ^= 1;
+        __syncthreads();
+    }
+}
+
+// leaf root of primitive p: walk up while the parent holds at most kLeafMax triangles
+RT_DEV int32_t ploc_leaf_root(uint32_t n, const int32_t* __restrict__ parent,
+                              const int32_t* __restrict__ nsize, uint32_t p) {
+    const uint64_t ni = (uint64_t)n - 1;
+    int32_t x = ~(int32_t)p;
+    for (;;) {
+        const int32_t q = parent[x >= 0 ? (uint64_t)x : ni + (uint64_t)(~x)];
+        if (q < 0 || nsize[q] > kLeafMax) return x;
+        x = q;
+    }
+}
+
+// smallest Morton position among the (at most kLeafMax) triangles under x
+RT_DEV uint32_t ploc_minpos(int32_t x, const int32_t* __restrict__ nleft, const int32_t* __restrict__ nright,
+                            const uint32_t* __restrict__ pos_of) {
+    int32_t stk[2 * kLeafMax];
+    int sp = 0;
+    stk[sp++] = x;
+    uint32_t best = 0xFFFFFFFFu;
+    while (sp) {
+        const int32_t c = stk[--sp];
+        if (c < 0) {
+            const uint32_t q = pos_of[~c];
+            best = q < best ? q : best;
+        } else if (sp + 2 <= 2 * kLeafMax) {  // always true for a subtree of <= kLeafMax triangles
+            stk[sp++] = nright[c];
+            stk[sp++] = nleft[c];
+        }
+    }
+    return best;
+}
+
+__global__ void __launch_bounds__(256) k_ploc_leafroots(uint32_t n, const uint32_t* __restrict__ morder,
+                                                         const int32_t* __restrict__ parent,
+                                                         const int32_t* __restrict__ nsize,
+                                                         const int32_t* __restrict__ nleft,
+                                                         const int32_t* __restrict__ nright,
+                                                         const uint32_t* __restrict__ pos_of,
+                                                         uint32_t* __restrict__ scan) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int32_t x = ploc_leaf_root(n, parent, nsize, morder[i]);
+    scan[i] = ploc_minpos(x, nleft, nright, pos_of) == i ? (uint32_t)ploc_size(nsize, x) : 0u;
+}
+
+// exclusive scan of scan[0..n) in place: per-chunk totals, chunk offsets, chunk scans
+__global__ void __launch_bounds__(256) k_scan_chunks(uint32_t n, const uint32_t* __restrict__ a,
+                                                      uint32_t* __restrict__ part) {
+    __shared__ uint64_t lds[256];
+    const uint32_t c = blockIdx.x;
+    uint64_t v = 0;
+    for (int q = 0; q < 4; ++q) {
+        const uint64_t i = (uint64_t)c * kPlocChunk + threadIdx.x * 4 + q;
+        if (i < n) v += a[i];
+    }
+    uint64_t total;
+    block_exclusive_scan<256>(v, lds, total);
+    if (threadIdx.x == 0) part[c] = (uint32_t)total;
+}
+
+__global__ void __launch_bounds__(1024) k_scan_top(uint32_t nchunk, uint32_t* __restrict__ part) {
+    __shared__ uint64_t lds[1024];
+    __shared__ uint64_t carry;
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    for (uint32_t b = 0; b < nchunk; b += 1024) {
+        const uint32_t c = b + threadIdx.x;
+        const uint64_t v = c < nchunk ? part[c] : 0;
+        uint64_t total;
+        const uint64_t ex = block_exclusive_scan<1024>(v, lds, total) + carry;
+        if (c < nchunk) part[c] = (uint32_t)ex;
+        __syncthreads();
+        if (threadIdx.x == 0) carry += total;
+        __syncthreads();
+    }
+}
+
+__global__ void __launch_bounds__(256) k_scan_apply(uint32_t n, uint32_t* __restrict__ a,
+                                                     const uint32_t* __restrict__ part) {
+    __shared__ uint64_t lds[256];
+    const uint32_t c = blockIdx.x;
+    uint32_t x[4];
+    uint64_t v = 0;
+    for (int q = 0; q < 4; ++q) {
+        const uint64_t i = (uint64_t)c * kPlocChunk + threadIdx.x * 4 + q;
+        x[q] = i < n ? a[i] : 0u;
+        v += x[q];
+    }
+    uint64_t total;
+    uint64_t run = block_exclusive_scan<256>(v, lds, total) + part[c];
+    for (int q = 0; q < 4; ++q) {
+        const uint64_t i = (uint64_t)c * kPlocChunk + threadIdx.x * 4 + q;
+        if (i < n) a[i] = (uint32_t)run;
+        run += x[q];
+    }
+}
+
+// slots of the leaf root whose smallest Morton position is i: its triangles left to right from
+// the root's first slot; first slots of its internal nodes
+__global__ void __launch_bounds__(256) k_ploc_slots(uint32_t n, const uint32_t* __restrict__ morder,
+                                                     const int32_t* __restrict__ parent,
+                                                     const int32_t* __restrict__ nsize,
+                                                     const int32_t* __restrict__ nleft,
+                                                     const int32_t* __restrict__ nright,
+                                                     const uint32_t* __restrict__ pos_of,
+                                                     const uint32_t* __restrict__ scan,
+                                                     uint32_t* __restrict__ slot_order,
+                                                     uint32_t* __restrict__ slot_of,
+                                                     uint32_t* __restrict__ fslot) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int32_t x = ploc_leaf_root(n, parent, nsize, morder[i]);
+    if (ploc_minpos(x, nleft, nright, pos_of) != i) return;
+    uint32_t next = scan[i];
+    int32_t stk[2 * kLeafMax];
+    int sp = 0;
+    stk[sp++] = x;
+    while (sp) {
+        const int32_t c = stk[--sp];
+        if (c < 0) {
+            slot_order[next] = (uint32_t)~c;
+            slot_of[~c] = next;
+            ++next;
+        } else if (sp + 2 <= 2 * kLeafMax) {  // always true for a subtree of <= kLeafMax triangles
+            fslot[c] = next;
+            stk[sp++] = nright[c];
+            stk[sp++] = nleft[c];
+        }
+    }
+}
+
+// traversal refs of every internal node: a child of more than kLeafMax triangles by id, else the
+// leaf of its slots
+__global__ void __launch_bounds__(256) k_ploc_refs(uint32_t n, const int32_t* __restrict__ nleft,
+                                                    const int32_t* __restrict__ nright,
+                                                    const int32_t* __restrict__ nsize,
+                                                    const uint32_t* __restrict__ slot_of,
+                                                    const uint32_t* __restrict__ fslot,
+                                                    float4* __restrict__ nodes) {
+    const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= n - 1) return;
+    auto tref = [&](int32_t c) -> int32_t {
+        if (c < 0) return leaf_ref(slot_of[~c], 1u);
+        if (nsize[c] > kLeafMax) return c;
+        return leaf_ref(fslot[c], (uint32_t)nsize[c]);
+    };
+    nodes[(uint64_t)x * 4 + 3] = make_float4(__int_as_float(tref(nleft[x])), __int_as_float(tref(nright[x])),
+                                             0.0f, 0.0f);
+}
+
+// big PLOC iterations to launch for n clusters: enough for m to fall to kPlocTail at 0.88 per
+// iteration (measured ~0.83-0.90), plus margin; the tail finishes whatever remains
+inline int ploc_iterations(uint32_t n) {
+    if (n <= (uint32_t)kPlocTail) return 0;
+    return (int)std::ceil(std::log((double)n / kPlocTail) / std::log(1.0 / 0.88)) + 4;
+}
+
+
 // ---- 4-wide nodes ----
 // nodes4[i] (8 x float4, 128 B) lists the grandchildren of binary node i reached through its
 // internal children, and its leaf children directly, in the order (left..., right...):
@@ -797,8 +1362,45 @@ extern "C" int rt_scene_build(const rt_mesh_desc* meshes, uint32_t n_meshes, rt_
         std::swap(vi, vo);
     }
     // after 4 passes the sorted arrays are back in keys0/vals0
-    // 5. hierarchy, 6. emit, 7. refit
     float4* nodes = (float4*)sc->nodes;
+    if (n > 1 && current_tuning().builder == RT_BUILD_PLOC) {
+        // 5. PLOC hierarchy (node boxes as it merges, scene bounds at the root), 6. leaf slots and
+        // traversal refs, 7. leaf records in slot order, 8. 4-wide records
+        hipLaunchKernelGGL(k_ploc_init, dim3(blocks_for(n, 256)), dim3(256), 0, s, vi, n, w.leafbox,
+                           w.cid[0], w.cbox[0], w.pos_of, w.parent, w.state);
+        const int iters = ploc_iterations(n);
+        const uint32_t nchunk = blocks_for(n, kPlocChunk);
+        const uint32_t gnn = std::min<uint32_t>(kPlocGrid, blocks_for(n, kPlocTile));
+        const uint32_t gch = std::min<uint32_t>(kPlocGrid, nchunk);
+        for (int it = 0; it < iters; ++it) {
+            hipLaunchKernelGGL(k_ploc_nn, dim3(gnn), dim3(kPlocTile), 0, s, it, w.state, w.cbox[0],
+                               w.cbox[1], w.nn);
+            hipLaunchKernelGGL(k_ploc_count, dim3(gch), dim3(256), 0, s, it, w.state, w.nn, w.part);
+            hipLaunchKernelGGL(k_ploc_offsets, dim3(1), dim3(1024), 0, s, it, w.state, w.part);
+            hipLaunchKernelGGL(k_ploc_compact, dim3(gch), dim3(256), 0, s, it, n, w.state, w.nn, w.part,
+                               w.cid[0], w.cid[1], w.cbox[0], w.cbox[1], w.nleft, w.nright, w.nsize,
+                               w.parent, nodes, sc->bounds);
+        }
+        hipLaunchKernelGGL(k_ploc_tail, dim3(1), dim3(1024), 0, s, iters & 1, n, w.state, w.nn, w.cid[0],
+                           w.cid[1], w.cbox[0], w.cbox[1], w.nleft, w.nright, w.nsize, w.parent, nodes,
+                           sc->bounds);
+        hipLaunchKernelGGL(k_ploc_leafroots, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, vi, w.parent,
+                           w.nsize, w.nleft, w.nright, w.pos_of, w.scan);
+        hipLaunchKernelGGL(k_scan_chunks, dim3(nchunk), dim3(256), 0, s, n, w.scan, w.part);
+        hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, s, nchunk, w.part);
+        hipLaunchKernelGGL(k_scan_apply, dim3(nchunk), dim3(256), 0, s, n, w.scan, w.part);
+        hipLaunchKernelGGL(k_ploc_slots, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, vi, w.parent,
+                           w.nsize, w.nleft, w.nright, w.pos_of, w.scan, w.slot_order, w.slot_of, w.fslot);
+        hipLaunchKernelGGL(k_ploc_refs, dim3(blocks_for(n - 1, 256)), dim3(256), 0, s, n, w.nleft,
+                           w.nright, w.nsize, w.slot_of, w.fslot, nodes);
+        hipLaunchKernelGGL(k_emit, dim3(blocks_for(n, 256)), dim3(256), 0, s, w.slot_order, n, w.tri_u,
+                           w.shade_u, (float4*)sc->tris, (float4*)sc->shade);
+        hipLaunchKernelGGL(k_wide, dim3(blocks_for(n - 1, 256)), dim3(256), 0, s, n,
+                           (const float4*)nodes, (float4*)sc->nodes4);
+        sc->root = n <= (uint32_t)kLeafMax ? leaf_ref(0, n) : 0;
+        vi = w.slot_order;
+    } else {
+    // 5. hierarchy, 6. emit, 7. refit
     if (n > 1) {
         hipLaunchKernelGGL(k_zero_word, dim3(1), dim3(1), 0, s, w.top_count);
         hipLaunchKernelGGL(k_hierarchy, dim3(blocks_for(n - 1, 256)), dim3(256), 0, s, ki, n, nodes,
@@ -819,6 +1421,7 @@ extern "C" int rt_scene_build(const rt_mesh_desc* meshes, uint32_t n_meshes, rt_
     } else {
         hipLaunchKernelGGL(k_single_leaf_bounds, dim3(1), dim3(1), 0, s, w.leafbox, sc->bounds);
         sc->root = ~0;
+    }
     }
     if (keys_out)
         hipLaunchKernelGGL(k_copy_u32, dim3(blocks_for(n, 256)), dim3(256), 0, s, ki, keys_out, n);

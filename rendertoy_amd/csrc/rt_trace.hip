@@ -119,10 +119,10 @@ struct TileMap {
     uint32_t S, super_x, n_super;
 };
 
-TileMap make_tile_map(uint32_t W, uint32_t H, uint32_t S) {
+TileMap make_tile_map(uint32_t W, uint32_t H, uint32_t S, uint32_t tile = kTile) {
     TileMap m;
-    m.tiles_x = (W + kTile - 1) / kTile;
-    m.tiles_y = (H + kTile - 1) / kTile;
+    m.tiles_x = (W + tile - 1) / tile;
+    m.tiles_y = (H + tile - 1) / tile;
     m.S = S;
     m.super_x = S ? (m.tiles_x + S - 1) / S : 0;
     m.n_super = S ? m.super_x * ((m.tiles_y + S - 1) / S) : 0;
@@ -192,12 +192,13 @@ __global__ void __launch_bounds__(256) k_trace_camera(SceneArgs sc, rt_camera ca
 // Camera closest hit with quad-lane packets: a 256-thread block covers 8x8 pixels, wave w the 4x4
 // sub-tile ((w & 1) * 4, (w >> 1) * 4), lanes 4q..4q+3 the pixel (q & 3, q >> 2) of it.
 __global__ void __launch_bounds__(256) k_trace_camera_quad(SceneArgs sc, rt_camera cam, uint32_t W,
-                                                            uint32_t H, uint32_t tiles_x8,
-                                                            rt_hits hits) {
+                                                            uint32_t H, TileMap tm, rt_hits hits) {
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6, q = lane >> 2;
+    uint32_t tx, ty;
+    if (!block_tile(tm, tx, ty)) return;
     TilePixel p;
-    p.col = (blockIdx.x % tiles_x8) * 8 + (wave & 1) * 4 + (q & 3);
-    p.row = (blockIdx.x / tiles_x8) * 8 + (wave >> 1) * 4 + (q >> 2);
+    p.col = tx * 8 + (wave & 1) * 4 + (q & 3);
+    p.row = ty * 8 + (wave >> 1) * 4 + (q >> 2);
     p.valid = p.col < W && p.row < H;
     const Ray r = camera_ray(cam, W, H, p);
     Hit h;
@@ -578,10 +579,10 @@ extern "C" int rt_trace_camera(const rt_scene* scene, const rt_camera* cam, uint
     if (n > 0x7FFFFFFFull) return set_error(RT_ECAPACITY, "rt_trace_camera: image too large");
     const rt_tuning tu = current_tuning();
     const TileMap tm = make_tile_map(W, H, tu.super_tile);
-    if (tu.primary == RT_TRAVERSAL_QUAD) {
-        const uint32_t tx8 = (W + 7) / 8, ty8 = (H + 7) / 8;
-        hipLaunchKernelGGL(k_trace_camera_quad, dim3(tx8 * ty8), dim3(256), 0, (hipStream_t)stream,
-                           scene_args(scene), *cam, W, H, tx8, *hits);
+    if (tu.primary == RT_TRAVERSAL_QUAD) {  // 8x8-pixel blocks
+        const TileMap tm8 = make_tile_map(W, H, tu.super_tile, 8);
+        hipLaunchKernelGGL(k_trace_camera_quad, dim3(tile_map_blocks(tm8)), dim3(256), 0,
+                           (hipStream_t)stream, scene_args(scene), *cam, W, H, tm8, *hits);
         return check_launch("rt_trace_camera");
     }
     using K = decltype(&k_trace_camera<0>);

@@ -8,6 +8,7 @@ change of the generator is detected instead of silently compared against.
 
   lbvh_980.npz   f=7 icosphere (980 triangles): Morton keys, order, binary and 4-wide node
                  records, triangle records, bounds, root of the device LBVH build
+  ploc_980.npz   the same scene built by the device's PLOC builder
   c1_tile.npz    C1 scene (f=64, 81,920 triangles), 512x512 camera, the 64x64 tile at the image
                  centre: FP64 closest hit (prim, t, flags) and the FP32 device-path result
   shade_c2s.npz  C2-shaped scene at f=32 (20,480 triangles + ground plane) with a 64x64 texture,
@@ -35,14 +36,22 @@ def geometry_hash(*arrays):
     return h.hexdigest()
 
 
-def lbvh_case():
+def build_case(builder):
     from rendertoy_amd import scenes
     from helpers import twin_from_meshes
     d = scenes.icosphere(7, 4242)
-    tw = twin_from_meshes([d])
+    tw = twin_from_meshes([d], builder=builder)
     return {"geometry": geometry_hash(d["pos"], d["idx"]), "keys": tw.keys, "order": tw.order,
             "nodes": tw.nodes, "nodes4": tw.nodes4, "tris": tw.tris, "shade": tw.shade,
             "bounds": tw.bounds, "root": np.int32(tw.root)}
+
+
+def lbvh_case():
+    return build_case("lbvh")
+
+
+def ploc_case():
+    return build_case("ploc")
 
 
 def c1_tile_case():
@@ -83,7 +92,8 @@ def shade_case():
             "rgb32": rgb32.reshape(-1, 3), "prim32": prim32, "stats": stats}
 
 
-CASES = {"lbvh_980.npz": lbvh_case, "c1_tile.npz": c1_tile_case, "shade_c2s.npz": shade_case}
+CASES = {"lbvh_980.npz": lbvh_case, "ploc_980.npz": ploc_case, "c1_tile.npz": c1_tile_case,
+         "shade_c2s.npz": shade_case}
 
 
 def main():

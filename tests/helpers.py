@@ -5,13 +5,13 @@ import math
 import numpy as np
 
 
-def twin_from_meshes(meshes):
+def twin_from_meshes(meshes, builder="lbvh"):
     import oracle
     from rendertoy_amd import scenes
     pos, nrm, uv, mo = scenes.triangle_arrays(meshes)
     has_n = [m.get("nrm") is not None for m in meshes]
     has_uv = [m.get("uv") is not None for m in meshes]
-    return oracle.TwinScene(pos, nrm, uv, mo, has_n, has_uv)
+    return oracle.TwinScene(pos, nrm, uv, mo, has_n, has_uv, builder=builder)
 
 
 def gpu_from_meshes(meshes, indexed=True):

@@ -78,7 +78,7 @@ int main(void) {
     S(rt_render_job) O(rt_render_job, cam) O(rt_render_job, params) O(rt_render_job, out)
     S(rt_batch_config) O(rt_batch_config, tiles_per_grab) O(rt_batch_config, primary)
     O(rt_batch_config, shadow) O(rt_batch_config, lane_stack) O(rt_batch_config, packet_budget)
-    O(rt_tuning, packet_budget)
+    O(rt_tuning, packet_budget) O(rt_tuning, builder)
     S(rt_paths_params) O(rt_paths_params, spp) O(rt_paths_params, seed) O(rt_paths_params, albedo)
     O(rt_paths_params, ray_tmin) S(rt_paths_out) O(rt_paths_out, counts)
     return 0;
@@ -115,13 +115,14 @@ def test_tuning_get_set_and_range_checks_without_gpu():
     from rendertoy_amd import _native
     d = ren.get_tuning()
     assert d == {"primary": _native.RT_TRAVERSAL_QUAD, "shadow": _native.RT_TRAVERSAL_LANE,
-                 "super_tile": 0, "lane_stack": _native.RT_LANE_STACK_MAX, "packet_budget": 256}
-    with ren.tuning(primary="hybrid", lane_stack=3, super_tile=2, packet_budget=7):
+                 "super_tile": 0, "lane_stack": _native.RT_LANE_STACK_MAX, "packet_budget": 256,
+                 "builder": _native.RT_BUILD_LBVH}
+    with ren.tuning(primary="hybrid", lane_stack=3, super_tile=2, packet_budget=7, builder="ploc"):
         assert ren.get_tuning() == {"primary": 2, "shadow": 1, "super_tile": 2, "lane_stack": 3,
-                                    "packet_budget": 7}
+                                    "packet_budget": 7, "builder": _native.RT_BUILD_PLOC}
     assert ren.get_tuning() == d
     for bad in ({"lane_stack": 0}, {"lane_stack": 33}, {"primary": 5}, {"shadow": 3},
-                {"super_tile": 65}, {"packet_budget": 0}):
+                {"super_tile": 65}, {"packet_budget": 0}, {"builder": 2}):
         with pytest.raises(RuntimeError, match="rt_set_tuning"):
             ren.set_tuning(**bad)
     assert ren.get_tuning() == d

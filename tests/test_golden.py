@@ -25,9 +25,10 @@ def cases():
     return make_golden
 
 
-def test_oracle_reproduces_lbvh_fixture():
-    g = load("lbvh_980.npz")
-    c = cases().lbvh_case()
+@pytest.mark.parametrize("builder", ["lbvh", "ploc"])
+def test_oracle_reproduces_build_fixture(builder):
+    g = load(f"{builder}_980.npz")
+    c = cases().build_case(builder)
     assert c["geometry"] == str(g["geometry"]), "scene generator changed"
     for k in ("keys", "order", "nodes", "nodes4", "tris", "shade", "bounds", "root"):
         np.testing.assert_array_equal(u32(np.asarray(c[k])), u32(g[k]), err_msg=k)
@@ -61,11 +62,13 @@ def test_oracle_reproduces_shading_fixture():
 
 
 @pytest.mark.gpu
-def test_device_matches_lbvh_fixture(ren):
+@pytest.mark.parametrize("builder", ["lbvh", "ploc"])
+def test_device_matches_build_fixture(ren, builder):
     from rendertoy_amd import scenes
     from helpers import gpu_from_meshes, gpu_arrays
-    g = load("lbvh_980.npz")
-    rc = gpu_from_meshes([scenes.icosphere(7, 4242)])
+    g = load(f"{builder}_980.npz")
+    with ren.tuning(builder=builder):
+        rc = gpu_from_meshes([scenes.icosphere(7, 4242)])
     a = gpu_arrays(rc)
     assert rc.root == int(g["root"])
     for k in ("keys", "order", "nodes", "nodes4", "tris", "shade", "bounds"):

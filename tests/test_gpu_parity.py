@@ -282,3 +282,26 @@ def test_render_c2_1080p_matches_twin_and_fp64(c2, twin):
           f"rgb_err_mean={err[flags == 0].mean():.3e} flagged_pixels={int((flags != 0).sum())} "
           f"over_1/255_flagged={int(((err > 1 / 255) & (flags != 0)).sum())}")
     assert (err[flags == 0] <= 1 / 255).all()
+
+
+@pytest.mark.parametrize("scene", ["c1", "c2"])
+def test_ploc_build_matches_twin_bytes_and_lbvh_hits(ren, scene):
+    """The PLOC builder (grid iterations while more than 4096 clusters remain, then one
+    workgroup) against the twin's PLOC, byte for byte; its tree gives the LBVH tree's hits."""
+    meshes = c1_scene() if scene == "c1" else c2_scene()[0]
+    tw = twin_from_meshes(meshes, builder="ploc")
+    with ren.tuning(builder="ploc"):
+        rc = gpu_from_meshes(meshes)
+    torch.cuda.synchronize()
+    assert_build_equal(rc, tw)
+    lb = gpu_from_meshes(meshes)
+    W, H = (512, 512) if scene == "c1" else (960, 540)
+    cam = camera(aspect=W / H)
+    for mode in ("quad", "packet", "lane"):
+        with ren.tuning(primary=mode):
+            a = {k: v.cpu().numpy() for k, v in rc.cast_camera(cam, W, H).items()}
+            b = {k: v.cpu().numpy() for k, v in lb.cast_camera(cam, W, H).items()}
+        np.testing.assert_array_equal(a["prim"], b["prim"])
+        np.testing.assert_array_equal(_u32(a["t"]), _u32(b["t"]))
+    o, d = random_rays(20000, seed=11)
+    np.testing.assert_array_equal(gpu_trace(rc, o, d)["prim"], gpu_trace(lb, o, d)["prim"])

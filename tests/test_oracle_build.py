@@ -1,4 +1,4 @@
-"""Pins of the oracle's LBVH build (CPU only): Morton keys from a numpy restatement of the
+"""Pins of the oracle's LBVH build (CPU only; the PLOC builder: test_oracle_ploc.py): Morton keys from a numpy restatement of the
 quantiser, the order from numpy's stable argsort, the tree from an independent top-down binary
 radix tree construction, node boxes as exact unions of padded leaf boxes, the 4-wide records from
 the binary ones, and leaves that partition the slots. Also: traversal of that tree (per-ray and the
@@ -62,7 +62,8 @@ def radix_splits(keys):
 @pytest.mark.parametrize("n_tris_f", [1, 2, 3, 5, 8, 13])
 def test_build_invariants_on_icospheres(n_tris_f):
     d = scenes.icosphere(n_tris_f, 100 + n_tris_f)
-    tw = twin_from_meshes([{"pos": d["pos"], "idx": d["idx"], "nrm": d["nrm"], "uv": d["uv"]}])
+    tw = twin_from_meshes([{"pos": d["pos"], "idx": d["idx"], "nrm": d["nrm"], "uv": d["uv"]}],
+                          builder="lbvh")
     check_build(tw)
 
 
@@ -72,7 +73,7 @@ def test_build_invariants_with_duplicate_keys():
     base = rng.uniform(-1, 1, (40, 3, 3)).astype(np.float32)
     pos = np.concatenate([base, base, base[:7]]).reshape(-1, 3)
     idx = np.arange(len(pos), dtype=np.int32).reshape(-1, 3)
-    tw = twin_from_meshes([{"pos": pos, "idx": idx, "nrm": None, "uv": None}])
+    tw = twin_from_meshes([{"pos": pos, "idx": idx, "nrm": None, "uv": None}], builder="lbvh")
     check_build(tw)
     assert len(np.unique(tw.keys)) < tw.n
 

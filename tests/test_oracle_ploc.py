@@ -1,0 +1,216 @@
+"""Pins of the oracle's PLOC build (CPU only) against an independent restatement written here from
+the builder's definition (locally ordered clustering, Meister & Bittner 2018):
+
+- clusters start as the triangles in Morton order with their padded boxes;
+- each iteration, cluster i takes as nearest neighbour the position j != i in [i-16, i+16] whose
+  union box has the smallest surface area (ties to the smaller j); mutual pairs merge at the lower
+  position, in position order; an iteration without a mutual pair merges positions 0 and 1;
+- the k-th node created gets id n-2-k;
+- subtrees of <= 4 triangles under a larger parent are leaves, taking consecutive slots in the
+  order of their smallest Morton position, triangles left to right.
+
+This restatement is plain Python over exact float32 numpy arithmetic; it checks every node record,
+the slot order, the 4-wide records, the root and the bounds. Plus: traversal of the PLOC tree
+equals FP32 brute force, and the tree gives the same hits as the LBVH."""
+import numpy as np
+import pytest
+
+from helpers import twin_from_meshes, random_rays
+from rendertoy_amd import scenes
+
+LEAF_MAX = 4
+EMPTY = 0x7FFFFFFF
+R = 16
+f32 = np.float32
+
+
+def leaf_ref(first, count):
+    return int(np.int32(~np.int64(first | ((count - 1) << 29)) & 0xFFFFFFFF))
+
+
+def padded_boxes(pos):
+    v = pos.reshape(-1, 3, 3).astype(f32) + f32(0.0)
+    lo, hi = v.min(1), v.max(1)
+    pad = (np.maximum(np.abs(lo), np.abs(hi)) * f32(2.0 ** -14) + f32(2.0 ** -40)).astype(f32)
+    return (lo - pad).astype(f32), (hi + pad).astype(f32)
+
+
+def area(alo, ahi, blo, bhi):
+    d = (np.maximum(ahi, bhi) - np.minimum(alo, blo)).astype(f32)
+    return f32(f32(f32(d[0] * d[1]) + f32(d[1] * d[2])) + f32(d[2] * d[0]))
+
+
+def ploc_reference(pos, morton_order):
+    n = len(morton_order)
+    blo, bhi = padded_boxes(pos)
+    C = [("t", int(p)) for p in morton_order]
+    B = [(blo[p].copy(), bhi[p].copy()) for p in morton_order]
+    nodes = {}      # id -> (left, right, box_left, box_right, size)
+    size = {}
+    parent = {}
+    created = 0
+    while len(C) > 1:
+        m = len(C)
+        NN = []
+        for i in range(m):
+            best, bj = None, None
+            for j in range(max(0, i - R), min(m - 1, i + R) + 1):
+                if j == i:
+                    continue
+                a = area(B[i][0], B[i][1], B[j][0], B[j][1])
+                if best is None or a < best:
+                    best, bj = a, j
+            NN.append(bj)
+        if not any(NN[NN[i]] == i for i in range(m)):
+            NN[0], NN[1] = 1, 0
+        C2, B2 = [], []
+        for i in range(m):
+            j = NN[i]
+            if NN[j] == i:
+                if j < i:
+                    continue
+                nid = n - 2 - created
+                created += 1
+                sz = lambda c: 1 if c[0] == "t" else size[c[1]]
+                size[nid] = sz(C[i]) + sz(C[j])
+                nodes[nid] = (C[i], C[j], B[i], B[j])
+                parent[C[i]] = nid
+                parent[C[j]] = nid
+                C2.append(("n", nid))
+                B2.append((np.minimum(B[i][0], B[j][0]), np.maximum(B[i][1], B[j][1])))
+            else:
+                C2.append(C[i])
+                B2.append(B[i])
+        C, B = C2, B2
+    assert created == n - 1 and C[0] == ("n", 0)
+
+    def sz(c):
+        return 1 if c[0] == "t" else size[c[1]]
+
+    pos_of = {int(p): i for i, p in enumerate(morton_order)}
+
+    def tris_under(c):
+        return [c[1]] if c[0] == "t" else tris_under(nodes[c[1]][0]) + tris_under(nodes[c[1]][1])
+
+    # leaf roots by smallest Morton position, triangles left to right
+    roots = {}
+    for p in morton_order:
+        x = ("t", int(p))
+        while x in parent and size[parent[x]] <= LEAF_MAX:
+            x = ("n", parent[x])
+        roots[x] = min(pos_of[t] for t in tris_under(x))
+    order, slot_of, first = [], {}, {}
+    for x, _ in sorted(roots.items(), key=lambda kv: kv[1]):
+        def visit(c):
+            if c[0] == "t":
+                slot_of[c[1]] = len(order)
+                order.append(c[1])
+            else:
+                first[c[1]] = len(order)
+                visit(nodes[c[1]][0])
+                visit(nodes[c[1]][1])
+        visit(x)
+
+    def tref(c):
+        if c[0] == "t":
+            return leaf_ref(slot_of[c[1]], 1)
+        if size[c[1]] > LEAF_MAX:
+            return c[1]
+        return leaf_ref(first[c[1]], size[c[1]])
+
+    recs = np.zeros((n - 1, 16), f32)
+    for nid, (l, r, bl, br) in nodes.items():
+        recs[nid, [0, 2, 8]] = bl[0]
+        recs[nid, [1, 3, 9]] = bl[1]
+        recs[nid, [4, 6, 10]] = br[0]
+        recs[nid, [5, 7, 11]] = br[1]
+        recs[nid, 12:14] = np.array([tref(l), tref(r)], np.int32).view(f32)
+    return recs, np.array(order, np.uint32), (B[0][0], B[0][1])
+
+
+def wide_reference(recs):
+    out = np.zeros((len(recs), 32), f32)
+    for i, rec in enumerate(recs):
+        refs = rec[12:14].view(np.int32)
+        boxes = [(rec[[0, 2, 8]], rec[[1, 3, 9]]), (rec[[4, 6, 10]], rec[[5, 7, 11]])]
+        ent = []
+        for ref, (lo, hi) in zip(refs, boxes):
+            if ref >= 0:
+                c = recs[ref]
+                cr = c[12:14].view(np.int32)
+                ent += [(cr[0], c[[0, 2, 8]], c[[1, 3, 9]]), (cr[1], c[[4, 6, 10]], c[[5, 7, 11]])]
+            else:
+                ent.append((ref, lo, hi))
+        w = out[i]
+        wr = w[24:28].view(np.int32)
+        for k in range(4):
+            if k < len(ent):
+                wr[k] = ent[k][0]
+                w[[k, 8 + k, 16 + k]] = ent[k][1]
+                w[[4 + k, 12 + k, 20 + k]] = ent[k][2]
+            else:
+                wr[k] = EMPTY
+                w[[k, 8 + k, 16 + k]] = np.inf
+                w[[4 + k, 12 + k, 20 + k]] = -np.inf
+    return out
+
+
+def check_ploc(tw):
+    n = tw.n
+    lbvh = twin_from_meshes_arrays(tw, "lbvh")
+    np.testing.assert_array_equal(tw.keys, lbvh.keys)  # same Morton order as the LBVH
+    recs, order, (rlo, rhi) = ploc_reference(tw.pos, lbvh.order)
+    np.testing.assert_array_equal(tw.order, order)
+    np.testing.assert_array_equal(tw.tris[:, 3].view(np.int32), order.view(np.int32))
+    np.testing.assert_array_equal(tw.nodes.view(np.uint32), recs.view(np.uint32))
+    np.testing.assert_array_equal(tw.nodes4.view(np.uint32), wide_reference(recs).view(np.uint32))
+    np.testing.assert_array_equal(tw.bounds[[0, 1, 2]], rlo)
+    np.testing.assert_array_equal(tw.bounds[[4, 5, 6]], rhi)
+    assert tw.root == (0 if n > LEAF_MAX else leaf_ref(0, n))
+
+
+def twin_from_meshes_arrays(tw, builder):
+    import oracle
+    n = tw.n
+    return oracle.TwinScene(tw.pos, np.zeros((n, 9), f32), np.zeros((n, 6), f32),
+                            np.zeros(n, np.int32), [0], [0], builder=builder)
+
+
+@pytest.mark.parametrize("f", [1, 2, 3, 5, 8])
+def test_ploc_matches_restatement_on_icospheres(f):
+    d = scenes.icosphere(f, 300 + f)
+    tw = twin_from_meshes([{"pos": d["pos"], "idx": d["idx"], "nrm": None, "uv": None}], builder="ploc")
+    check_ploc(tw)
+
+
+@pytest.mark.parametrize("n", [2, 3, 4, 5, 37])
+def test_ploc_matches_restatement_on_small_random_sets(n):
+    rng = np.random.default_rng(n)
+    pos = rng.uniform(-1, 1, (n * 3, 3)).astype(f32)
+    idx = np.arange(n * 3, dtype=np.int32).reshape(-1, 3)
+    tw = twin_from_meshes([{"pos": pos, "idx": idx, "nrm": None, "uv": None}], builder="ploc")
+    check_ploc(tw)
+
+
+def test_ploc_ties_on_identical_boxes():
+    # every cluster pair has the same area: ties to the smaller position still give mutual pairs
+    base = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]], f32)
+    pos = np.tile(base, (50, 1))
+    idx = np.arange(150, dtype=np.int32).reshape(-1, 3)
+    tw = twin_from_meshes([{"pos": pos, "idx": idx, "nrm": None, "uv": None}], builder="ploc")
+    check_ploc(tw)
+
+
+def test_ploc_tree_traversal_equals_brute_force_and_lbvh():
+    d = scenes.icosphere(12, 77)
+    meshes = [{"pos": d["pos"], "idx": d["idx"], "nrm": d["nrm"], "uv": d["uv"]}]
+    ploc = twin_from_meshes(meshes, builder="ploc")
+    lbvh = twin_from_meshes(meshes, builder="lbvh")
+    o, dr = random_rays(3000, seed=5)
+    a = ploc.trace_closest(o, dr)
+    b = lbvh.trace_closest(o, dr)
+    np.testing.assert_array_equal(a["prim"], b["prim"])
+    np.testing.assert_array_equal(a["t"].view(np.uint32), b["t"].view(np.uint32))
+    _, bprim = ploc.brute_closest32(o, dr)
+    np.testing.assert_array_equal(a["prim"], bprim)
+    assert (a["prim"] >= 0).sum() > 500
