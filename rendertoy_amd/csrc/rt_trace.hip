@@ -115,7 +115,7 @@ struct TilePixel {
 // a few compact screen regions and share their upper tree and surface nodes. S = 0 keeps the
 // row-major order.
 struct TileMap {
-    uint32_t tiles_x, tiles_y;  // 16x16 blocks across / down
+    uint32_t tiles_x, tiles_y;  // blocks (16x16 pixels; 8x8 for the quad camera kernel) across / down
     uint32_t S, super_x, n_super;
 };
 
