@@ -305,3 +305,31 @@ def test_ploc_build_matches_twin_bytes_and_lbvh_hits(ren, scene):
         np.testing.assert_array_equal(_u32(a["t"]), _u32(b["t"]))
     o, d = random_rays(20000, seed=11)
     np.testing.assert_array_equal(gpu_trace(rc, o, d)["prim"], gpu_trace(lb, o, d)["prim"])
+
+
+def test_ploc_tiny_and_multi_mesh_scenes(ren):
+    """PLOC on 2..9 triangles (root a leaf up to 4), duplicated triangles (equal boxes and Morton
+    keys) and a multi-mesh scene with an empty mesh: device == twin bytes, hits == LBVH's."""
+    from rendertoy_amd import scenes
+    rng = np.random.default_rng(21)
+    cases = []
+    for n in (2, 3, 4, 5, 9):
+        pos = rng.uniform(-1, 1, (3 * n, 3)).astype(np.float32)
+        cases.append([{"pos": pos, "idx": np.arange(3 * n, dtype=np.int32).reshape(-1, 3),
+                       "nrm": None, "uv": None}])
+    base = rng.uniform(-1, 1, (30, 3, 3)).astype(np.float32)
+    dup = np.concatenate([base, base, base[:5]]).reshape(-1, 3)
+    cases.append([{"pos": dup, "idx": np.arange(len(dup), dtype=np.int32).reshape(-1, 3),
+                   "nrm": None, "uv": None}])
+    a, b = scenes.icosphere(8, 11), scenes.icosphere(4, 12)
+    b["pos"] = (b["pos"] * 0.5 + 0.3).astype(np.float32)
+    empty = {"pos": np.zeros((3, 3), np.float32), "idx": np.zeros((0, 3), np.int32), "nrm": None, "uv": None}
+    cases.append([a, empty, b])
+    o, d = random_rays(20_000, 17)
+    for meshes in cases:
+        tw = twin_from_meshes(meshes, builder="ploc")
+        with ren.tuning(builder="ploc"):
+            rc = gpu_from_meshes(meshes)
+        assert_build_equal(rc, tw)
+        assert_trace_equal(gpu_trace(rc, o, d), tw.trace_closest(o, d))
+        np.testing.assert_array_equal(gpu_trace(rc, o, d)["prim"], gpu_trace(gpu_from_meshes(meshes), o, d)["prim"])
