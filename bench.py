@@ -7,19 +7,21 @@ camera at (0,0,3) looking at the origin, 45 deg vertical FOV, 1920x1080, one pri
 
 A step = one launch of the fused ray-generation + closest-hit kernel over the whole frame
 (2,073,600 rays; hit records t,u,v,prim written per pixel) with the scene resident in HBM.
-value = rays / step time in Mrays/s. Also reported: LBVH build time, and the full render
+value = rays / step time in Mrays/s. Also reported: LBVH build time, the full render
 (primary + shadow ray + Lambert + bilinear RGBA8 texture) latency p50/p99 over the same count of
-frames.
+frames, the batched / serving / incoherent-path workloads, and ray_cast on explicit ray buffers
+(the frame's rays in camera order, and random rays in and around the sphere).
 
 roofline: algorithmic bytes per launch = per ray 16 B of hit output + 128 B per 4-wide node
 record + 48 B per triangle record its ray tests, + 48 B per hit (the winner's record re-read for
 the FP64 refinement); the per-ray visit counts come from the CPU restatement's lane-exact
 emulation of the kernel's traversal on the identical BVH (oracle/rt_twin.c), divided by the
-kernel's mean duration from HIP events on the launch stream. The packet traversal fetches a record
-once per 64-ray wave, so the bytes it actually fetches are also reported (fetched_bytes), and
+kernel's mean duration from HIP events on the launch stream. The packet traversals fetch a record
+once per packet, so the bytes they actually fetch are also reported (fetched_bytes), and
 `traffic` is the HBM traffic per launch measured by rocprofv3 (FETCH_SIZE/WRITE_SIZE, scaled by
 the same counters on a copy kernel of known size), read from profiles/k_trace_camera_c2.txt when
-that summary was taken of the same kernel on the same workload.
+that summary was taken of the same kernel on the same workload. The quad-lane kernel reads a
+record once per 16-ray packet.
 cpu_baseline: the same closest-hit traversal in the CPU restatement (C + OpenMP, all host
 threads) over full frames for about `--cpu-seconds`.
 """
@@ -58,6 +60,8 @@ def parse():
                     help="comma-separated arrival rates (requests/s)")
     ap.add_argument("--no-incoherent", action="store_true",
                     help="skip the incoherent-ray measurement (diffuse paths, 4 spp, 2 bounces)")
+    ap.add_argument("--no-ray-cast", action="store_true",
+                    help="skip the ray_cast measurement (ray buffers in camera order and random)")
     ap.add_argument("--copy-calib-mib", type=int, default=0,
                     help="first run 3 elementwise int32 adds over this many MiB (a kernel of "
                          "known read/write traffic that scales the FETCH_SIZE/WRITE_SIZE "
@@ -154,6 +158,32 @@ def measure_batched(args, ren, scenes, torch):
         "autotune_ms": {c.key(): round(ms, 3) for c, ms in table},
         "setup_s": round(gen_s, 1),
     }
+
+
+def measure_ray_cast(args, rc, cam, W, H, torch):
+    """Raycaster.ray_cast's path (rt_trace_closest / rt_trace_any) on explicit ray buffers of the
+    bench scene: the frame's primary rays in camera order, and W*H rays with origins uniform in
+    [-0.6, 0.6]^3 (around and inside the displaced sphere) and uniformly random directions."""
+    from rendertoy_amd.batch import time_launches
+    n = W * H
+    cr = rc.generate_rays(cam, W, H)
+    cam_rays = {k: cr[k] for k in ("ox", "oy", "oz", "dx", "dy", "dz")}
+    g = torch.Generator(device="cuda").manual_seed(7)
+    o = (torch.rand((n, 3), device="cuda", generator=g) * 2 - 1) * 0.6
+    d = torch.randn((n, 3), device="cuda", generator=g)
+    d = d / d.norm(dim=1, keepdim=True)
+    rnd_rays = {"ox": o[:, 0].contiguous(), "oy": o[:, 1].contiguous(), "oz": o[:, 2].contiguous(),
+                "dx": d[:, 0].contiguous(), "dy": d[:, 1].contiguous(), "dz": d[:, 2].contiguous()}
+    out = rc.trace_soa(cam_rays)
+    res = {"what": f"{n} rays per launch: the frame's primary rays as a ray buffer, and random rays "
+                   "(origins in [-0.6,0.6]^3, uniform directions)"}
+    for name, rays in (("camera_order", cam_rays), ("random", rnd_rays)):
+        ms = float(np.median(time_launches(lambda: rc.trace_soa(rays, out=out), args.batch_reps * 2)))
+        ms_any = float(np.median(time_launches(lambda: rc.occluded(rays), args.batch_reps * 2)))
+        res[name] = {"closest_ms": round(ms, 3), "closest_mrays_per_s": round(n / ms / 1e3, 1),
+                     "any_hit_ms": round(ms_any, 3), "any_hit_mrays_per_s": round(n / ms_any / 1e3, 1),
+                     "hits": int((out["prim"] >= 0).sum())}
+    return res
 
 
 def incoherent_scene(scenes):
@@ -377,6 +407,8 @@ def main():
         result["serving"] = measure_serving(args, ren, scenes, torch)
     if not args.no_incoherent:
         result["incoherent"] = measure_incoherent(args, ren, scenes, torch)
+    if not args.no_ray_cast:
+        result["ray_cast"] = measure_ray_cast(args, rc, cam, W, H, torch)
 
     # ---- roofline and CPU baseline from the CPU restatement (oracle) ----
     if not args.no_roofline or not args.no_cpu_baseline:
