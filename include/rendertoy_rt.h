@@ -193,11 +193,13 @@ int rt_set_tuning(const rt_tuning* tuning);   /* RT_EINVAL if a field is out of 
 /* Fills `out` with the scene-array and workspace byte sizes for n_tris triangles. */
 int rt_scene_sizes_for(uint32_t n_tris, rt_scene_sizes* out);
 
-/* Builds the LBVH of all meshes' triangles on the device: per-triangle AABB and centroid,
- * 30-bit Morton codes over the centroid bounds, stable LSD radix sort of (code, primitive),
- * Karras hierarchy, bottom-up AABB refit. `scene` must hold n_tris/n_meshes and the device
- * arrays sized by rt_scene_sizes_for(); on return scene->root is set. `keys_out`/`order_out`
- * (device, n_tris uint32 each, may be NULL) receive the sorted Morton codes and primitive ids. */
+/* Builds the BVH of all meshes' triangles on the device: per-triangle AABB and centroid,
+ * 30-bit Morton codes over the centroid bounds, stable LSD radix sort of (code, primitive), then
+ * the hierarchy of rt_tuning.builder: the Karras radix tree with a bottom-up AABB refit (LBVH), or
+ * PLOC clustering over the Morton order. `scene` must hold n_tris/n_meshes and the device arrays
+ * sized by rt_scene_sizes_for(); on return scene->root is set. `keys_out`/`order_out` (device,
+ * n_tris uint32 each, may be NULL) receive the sorted Morton codes and the primitive of each leaf
+ * slot (for LBVH the Morton order itself). */
 int rt_scene_build(const rt_mesh_desc* meshes, uint32_t n_meshes, rt_scene* scene,
                    void* workspace_dev, uint64_t workspace_bytes,
                    uint32_t* keys_out, uint32_t* order_out, void* stream);
