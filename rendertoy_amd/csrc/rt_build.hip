@@ -782,7 +782,8 @@ RT_DEV void ploc_flags(const int32_t* __restrict__ nn, int32_t i, bool forced, u
     merge = mutual && i < j;
 }
 
-// exclusive scan of (keep, merge) pairs packed as keep | merge << 32 over a 1024-thread... block
+// Exclusive scan of one value per thread over a block of T threads (T = blockDim.x); `total` is
+// the block's sum. The PLOC kernels scan (keep, merge) counts packed as keep | merge << 32.
 template <int T>
 RT_DEV uint64_t block_exclusive_scan(uint64_t v, uint64_t* lds, uint64_t& total) {
     const int t = threadIdx.x;
