@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define RT_ABI_VERSION 8
+#define RT_ABI_VERSION 9
 
 enum {
     RT_OK = 0,
@@ -308,6 +308,20 @@ uint64_t rt_paths_workspace_bytes(uint32_t width, uint32_t height, uint32_t spp)
 int rt_trace_paths(const rt_scene* scene, const rt_camera* cam, const rt_paths_params* params,
                    const rt_paths_out* out, void* workspace_dev, uint64_t workspace_bytes,
                    void* stream);
+
+/* ---- mesh: rendering/_modeling.py:48-49 (Mesh.compute_normals, a stub in the reference) ----
+ * Area-weighted per-vertex normals: n(v) = sum of cross(P1 - P0, P2 - P0) over the (triangle,
+ * corner) pairs that use v, added corner 0 first, then 1, then 2, ascending triangle within each,
+ * starting from +0; N(v) = n / max(sqrt((x*x + y*y) + z*z), 1e-300). All FP64 from the FP32
+ * positions, rounded once to FP32: the result is a fixed function of the mesh. Vertices no triangle
+ * uses get (0, 0, 0); out-of-range indices are clamped to n_vertices - 1 as in rt_scene_build.
+ * N of vertex v is written as three floats at (char*)normals_out + v * out_stride: pass the N
+ * field of the vertex records (e.g. vertices + 16, stride 80 for MeshVertex) to fill them in place,
+ * or a dense float3 array (stride 12). The output must not overlap the positions. A non-indexed
+ * mesh needs no workspace (NULL allowed). */
+uint64_t rt_mesh_normals_workspace_bytes(uint32_t n_vertices, uint32_t n_tris);
+int rt_mesh_compute_normals(const rt_mesh_desc* mesh, float* normals_out, uint32_t out_stride,
+                            void* workspace_dev, uint64_t workspace_bytes, void* stream);
 
 #ifdef __cplusplus
 }

@@ -72,10 +72,26 @@ def lib():
         L.tw_closest64_bvh.argtypes = [vp, i64, vp, vp, i32, i64] + [vp] * 6 + [f64, f64, f64, f64] + [vp] * 5
         L.tw_tex_sample.restype = None
         L.tw_tex_sample.argtypes = [vp, i32, i32, i64, vp, vp, i32, vp]
+        L.tw_vertex_normals.restype = None
+        L.tw_vertex_normals.argtypes = [vp, i64, vp, i64, vp]
         L.tw_num_threads.restype = ctypes.c_int
         L.tw_set_threads.argtypes = [ctypes.c_int]
         _lib = L
     return _lib
+
+
+def vertex_normals(pos, idx=None):
+    """Area-weighted per-vertex normals of the twin (tw_vertex_normals): pos [V,3] float32,
+    idx [T,3] int32 or None (vertices 3t..3t+2, T = V // 3); returns [V,3] float32."""
+    pos = np.ascontiguousarray(pos, dtype=np.float32).reshape(-1, 3)
+    if idx is None:
+        n_tris = pos.shape[0] // 3
+    else:
+        idx = np.ascontiguousarray(idx, dtype=np.int32).reshape(-1, 3)
+        n_tris = idx.shape[0]
+    out = np.empty_like(pos)
+    lib().tw_vertex_normals(_p(pos), pos.shape[0], _p(idx), n_tris, _p(out))
+    return out
 
 
 def set_threads(n):

@@ -1456,3 +1456,47 @@ TW_API void tw_paths(const float* nodes, const float* tris, int32_t root, const 
     }
     for (uint32_t k = 0; k <= bounces && k < 17; ++k) counts[k] = cnt[k];
 }
+
+/* ---- per-vertex normals: Mesh.compute_normals (rendering/_modeling.py:48-49, a stub) ----
+ * n(v) = sum of cross(P1 - P0, P2 - P0) over the (triangle t, corner k) pairs with idx[t][k] == v,
+ * added in the order k = 0, 1, 2 and ascending t within k, starting from +0; N(v) = n / d with
+ * d = sqrt((x*x + y*y) + z*z) raised to 1e-300 (a NaN d stays NaN); FP64 from the FP32 positions,
+ * rounded once to FP32. Out-of-range indices are clamped to n_vertices - 1; idx == NULL means the
+ * vertices 3t, 3t+1, 3t+2. pos: n_vertices x 3 floats; out: n_vertices x 3 floats. */
+TW_API void tw_vertex_normals(const float* pos, int64_t n_vertices, const int32_t* idx, int64_t n_tris,
+                              float* out) {
+    double* acc = (double*)calloc((size_t)(3 * (n_vertices ? n_vertices : 1)), sizeof(double));
+    for (int k = 0; k < 3; ++k) {
+        for (int64_t t = 0; t < n_tris; ++t) {
+            int64_t c[3];
+            for (int j = 0; j < 3; ++j) {
+                if (idx) {
+                    const uint32_t u = (uint32_t)idx[3 * t + j];
+                    c[j] = u < (uint64_t)n_vertices ? (int64_t)u : n_vertices - 1;
+                } else {
+                    c[j] = 3 * t + j;
+                }
+            }
+            const float* a = pos + 3 * c[0];
+            const float* b = pos + 3 * c[1];
+            const float* d = pos + 3 * c[2];
+            const double e1x = (double)b[0] - (double)a[0], e1y = (double)b[1] - (double)a[1],
+                         e1z = (double)b[2] - (double)a[2];
+            const double e2x = (double)d[0] - (double)a[0], e2y = (double)d[1] - (double)a[1],
+                         e2z = (double)d[2] - (double)a[2];
+            double* s = acc + 3 * c[k];
+            s[0] = s[0] + (e1y * e2z - e1z * e2y);
+            s[1] = s[1] + (e1z * e2x - e1x * e2z);
+            s[2] = s[2] + (e1x * e2y - e1y * e2x);
+        }
+    }
+    for (int64_t v = 0; v < n_vertices; ++v) {
+        const double x = acc[3 * v], y = acc[3 * v + 1], z = acc[3 * v + 2];
+        double d = sqrt((x * x + y * y) + z * z);
+        d = d < 1e-300 ? 1e-300 : d;
+        out[3 * v] = (float)(x / d);
+        out[3 * v + 1] = (float)(y / d);
+        out[3 * v + 2] = (float)(z / d);
+    }
+    free(acc);
+}

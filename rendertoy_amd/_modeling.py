@@ -59,7 +59,11 @@ class Mesh:
         raise Exception('Not implemented yet')
 
     def compute_normals(self):
-        raise Exception('Not implemented yet')
+        """Area-weighted per-vertex normals, computed on the device and written into the N field
+        of every vertex record (rendering/_modeling.py:48-49 is a stub in the reference); see
+        rendertoy_amd._raycaster.compute_normals for the definition."""
+        from ._raycaster import compute_normals
+        compute_normals(self)
 
     def compute_tangents(self):
         raise Exception('Not implemented yet')

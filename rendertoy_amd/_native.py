@@ -12,7 +12,7 @@ import torch  # noqa: F401  (must be imported before the HIP library; see module
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "librtx_hip.so")
 
-RT_ABI_VERSION = 8
+RT_ABI_VERSION = 9
 RT_ABSENT = 0xFFFFFFFF
 RT_EMPTY_ROOT = 0x7FFFFFFF
 RT_RENDER_SHADOWS = 1
@@ -220,6 +220,10 @@ SIGNATURES = [
     ("rt_trace_paths", ctypes.c_int,
      [ctypes.POINTER(rt_scene), ctypes.POINTER(rt_camera), ctypes.POINTER(rt_paths_params),
       ctypes.POINTER(rt_paths_out), ctypes.c_void_p, ctypes.c_uint64, ctypes.c_void_p]),
+    ("rt_mesh_normals_workspace_bytes", ctypes.c_uint64, [ctypes.c_uint32, ctypes.c_uint32]),
+    ("rt_mesh_compute_normals", ctypes.c_int,
+     [ctypes.POINTER(rt_mesh_desc), ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p,
+      ctypes.c_uint64, ctypes.c_void_p]),
 ]
 
 _lib = None

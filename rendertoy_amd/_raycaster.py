@@ -103,6 +103,74 @@ def _check_device(t, what):
         raise RuntimeError(f"{what} must live on the HIP device (got {dev})")
 
 
+def _fill_mesh_desc(d, m, keep):
+    """Fills rt_mesh_desc `d` from Mesh `m` (MeshVertex records, optional index buffer); narrowed
+    index copies are appended to `keep`, which must outlive the call that reads them."""
+    vb = m.vertices
+    _check_device(vb, "Mesh.vertices")
+    if vb.dtype != MeshVertex:
+        raise TypeError("Mesh.vertices must be a buffer of MeshVertex")
+    d.vertices = vb.data_ptr()
+    d.n_vertices = vb.shape[0]
+    d.n_tris = vb.shape[0] // 3 if m.indices is None else m.indices.size // 3  # :33
+    d.vertex_stride = MeshVertex.itemsize
+    d.pos_offset = MeshVertex.fields["P"][1]
+    d.nrm_offset = MeshVertex.fields["N"][1] if getattr(m, "has_normals", True) else N.RT_ABSENT
+    d.uv_offset = MeshVertex.fields["C"][1] if getattr(m, "has_uvs", True) else N.RT_ABSENT
+    if m.indices is not None:
+        ib = m.indices
+        _check_device(ib, "Mesh.indices")
+        if ib.dtype != np.int32:
+            # load_obj-style int64 index buffers (rendering/_loaders.py:17) are narrowed
+            t = ib.tensor(torch.int64 if ib.dtype == np.int64 else None)
+            ib32 = t.to(torch.int32).contiguous()
+            keep.append(ib32)
+            d.indices = ib32.data_ptr()
+        else:
+            d.indices = ib.data_ptr()
+    else:
+        d.indices = None
+
+
+def compute_normals(mesh: Mesh, out: torch.Tensor = None, stream=None):
+    """Area-weighted per-vertex normals of `mesh` on the device (Mesh.compute_normals,
+    rendering/_modeling.py:48-49, is a stub in the reference).
+
+    N(v) = n / |n| with n the FP64 sum of cross(P1 - P0, P2 - P0) over the triangles using v
+    (corner-0 pairs first, then corner 1, then corner 2, ascending triangle within each), rounded
+    once to float32; vertices no triangle uses get (0, 0, 0). With `out` = None the normals are
+    written into the N field of the mesh's vertex records (and the mesh is marked as having
+    normals); otherwise into `out`, a contiguous float32 device tensor [n_vertices, 3].
+    Asynchronous on `stream` (default: the current stream)."""
+    lib = N.require_gpu()
+    d = N.rt_mesh_desc()
+    keep = []
+    # every allocation below is made on the stream the kernels run on, so torch's allocator
+    # orders any reuse of the freed index copy and workspace after them
+    ts = stream if stream is not None else torch.cuda.current_stream()
+    with torch.cuda.stream(ts):
+        _fill_mesh_desc(d, mesh, keep)
+        if out is None:
+            dst, stride = mesh.vertices.data_ptr() + MeshVertex.fields["N"][1], MeshVertex.itemsize
+        else:
+            _check_device(out, "out")
+            if (out.dtype != torch.float32 or not out.is_contiguous()
+                    or tuple(out.shape) != (d.n_vertices, 3)):
+                raise TypeError(f"out must be a contiguous float32 tensor of shape ({d.n_vertices}, 3)")
+            dst, stride = out.data_ptr(), 12
+        ws, nbytes = None, 0
+        if mesh.indices is not None:
+            nbytes = lib.rt_mesh_normals_workspace_bytes(d.n_vertices, d.n_tris)
+            ws = torch.empty(nbytes, dtype=torch.uint8, device=torch.device("cuda"))
+        N.check(lib.rt_mesh_compute_normals(ctypes.byref(d), dst, stride,
+                                            None if ws is None else ws.data_ptr(), nbytes,
+                                            N.stream_handle(ts)), "rt_mesh_compute_normals")
+        del ws, keep
+    if out is None:
+        mesh.has_normals = True
+    return out
+
+
 class TextureBinding:
     """Textures and the mesh -> texture map of a scene uploaded once (device table of rt_texture
     and int32 map), for callers that render the same scene many times; pass it as `textures`."""
@@ -138,33 +206,8 @@ class Raycaster:
         keep = []
         n_tris = 0
         for k, m in enumerate(self.models):
-            vb = m.vertices
-            _check_device(vb, "Mesh.vertices")
-            if vb.dtype != MeshVertex:
-                raise TypeError("Mesh.vertices must be a buffer of MeshVertex")
-            tris = m.vertices.shape[0] // 3 if m.indices is None else m.indices.size // 3
-            d = descs[k]
-            d.vertices = vb.data_ptr()
-            d.n_vertices = vb.shape[0]
-            d.n_tris = tris
-            d.vertex_stride = MeshVertex.itemsize
-            d.pos_offset = MeshVertex.fields["P"][1]
-            d.nrm_offset = MeshVertex.fields["N"][1] if getattr(m, "has_normals", True) else N.RT_ABSENT
-            d.uv_offset = MeshVertex.fields["C"][1] if getattr(m, "has_uvs", True) else N.RT_ABSENT
-            if m.indices is not None:
-                ib = m.indices
-                _check_device(ib, "Mesh.indices")
-                if ib.dtype != np.int32:
-                    # load_obj-style int64 index buffers (rendering/_loaders.py:17) are narrowed
-                    t = ib.tensor(torch.int64 if ib.dtype == np.int64 else None)
-                    ib32 = t.to(torch.int32).contiguous()
-                    keep.append(ib32)
-                    d.indices = ib32.data_ptr()
-                else:
-                    d.indices = ib.data_ptr()
-            else:
-                d.indices = None
-            n_tris += tris
+            _fill_mesh_desc(descs[k], m, keep)
+            n_tris += descs[k].n_tris
         self.n_tris = n_tris
         self.mesh_tri_counts = [int(descs[k].n_tris) for k in range(len(self.models))]
         sizes = N.rt_scene_sizes()

@@ -25,15 +25,11 @@
 #include <utility>
 
 #include "rt_common.h"
+#include "rt_sort.h"
 
 namespace rt {
 namespace {
 
-constexpr int kSortThreads = 256;
-constexpr int kSortItems = 8;                       // items per thread per tile
-constexpr int kSortTile = kSortThreads * kSortItems;  // 2048 keys per block
-constexpr int kRadixBits = 8;
-constexpr int kRadix = 1 << kRadixBits;
 constexpr int kReduceBlocks = 512;
 
 struct Workspace {
@@ -301,114 +297,6 @@ __global__ void __launch_bounds__(256) k_morton(const float4* __restrict__ cent,
     const uint32_t qz = quantize10(c.z, cb[2], cb[6]);
     keys[i] = (expand10(qx) << 2) | (expand10(qy) << 1) | expand10(qz);
     vals[i] = i;
-}
-
-// ---- radix sort ----
-
-__global__ void __launch_bounds__(kSortThreads) k_sort_hist(const uint32_t* __restrict__ keys,
-                                                            uint32_t n, int shift,
-                                                            uint32_t* __restrict__ hist,
-                                                            uint32_t nblk) {
-    __shared__ uint32_t h[kRadix];
-    h[threadIdx.x] = 0;
-    __syncthreads();
-    const uint64_t base = (uint64_t)blockIdx.x * kSortTile;
-    for (int r = 0; r < kSortItems; ++r) {
-        const uint64_t i = base + (uint64_t)r * kSortThreads + threadIdx.x;
-        if (i < n) atomicAdd(&h[(keys[i] >> shift) & (kRadix - 1)], 1u);
-    }
-    __syncthreads();
-    hist[(uint64_t)threadIdx.x * nblk + blockIdx.x] = h[threadIdx.x];
-}
-
-// Row scan: block d turns row d of the digit-major histogram (counts of digit d in tiles
-// 0..nblk-1) into its exclusive prefix sums, in place, and writes the row total to totals[d].
-__global__ void __launch_bounds__(kSortThreads) k_sort_rowscan(uint32_t* __restrict__ hist,
-                                                               uint32_t nblk,
-                                                               uint32_t* __restrict__ totals) {
-    __shared__ uint32_t s[kSortThreads];
-    uint32_t* row = hist + (uint64_t)blockIdx.x * nblk;
-    uint32_t run = 0;
-    for (uint32_t c0 = 0; c0 < nblk; c0 += kSortThreads) {
-        const uint32_t i = c0 + threadIdx.x;
-        const uint32_t v = i < nblk ? row[i] : 0u;
-        s[threadIdx.x] = v;
-        __syncthreads();
-        for (int off = 1; off < kSortThreads; off <<= 1) {
-            const uint32_t x = threadIdx.x >= (uint32_t)off ? s[threadIdx.x - off] : 0u;
-            __syncthreads();
-            s[threadIdx.x] += x;
-            __syncthreads();
-        }
-        if (i < nblk) row[i] = run + s[threadIdx.x] - v;
-        run += s[kSortThreads - 1];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) totals[blockIdx.x] = run;
-}
-
-// Stable scatter: items of a tile are visited in rounds of 256 (item order = (round, thread)),
-// each round ranked by wave64 multi-split (8 ballots give the lanes sharing this lane's digit).
-__global__ void __launch_bounds__(kSortThreads) k_sort_scatter(
-    const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin, uint32_t n, int shift,
-    const uint32_t* __restrict__ hist, const uint32_t* __restrict__ totals, uint32_t nblk,
-    uint32_t* __restrict__ kout,
-    uint32_t* __restrict__ vout) {
-    constexpr int kWaves = kSortThreads / 64;
-    __shared__ uint32_t base[kRadix];
-    __shared__ uint32_t run[kRadix];
-    __shared__ uint32_t wcount[kWaves][kRadix];
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    // exclusive scan of the digit totals gives each digit's first output position
-    run[tid] = totals[tid];
-    __syncthreads();
-    for (int off = 1; off < kRadix; off <<= 1) {
-        const uint32_t x = tid >= off ? run[tid - off] : 0u;
-        __syncthreads();
-        run[tid] += x;
-        __syncthreads();
-    }
-    base[tid] = run[tid] - totals[tid] + hist[(uint64_t)tid * nblk + blockIdx.x];
-    __syncthreads();
-    run[tid] = 0;
-    for (int w = 0; w < kWaves; ++w) wcount[w][tid] = 0;
-    __syncthreads();
-    const uint64_t tile = (uint64_t)blockIdx.x * kSortTile;
-    for (int r = 0; r < kSortItems; ++r) {
-        const uint64_t i = tile + (uint64_t)r * kSortThreads + tid;
-        const bool valid = i < n;
-        const uint32_t key = valid ? kin[i] : 0u;
-        const uint32_t val = valid ? vin[i] : 0u;
-        const uint32_t d = (key >> shift) & (kRadix - 1);
-        uint64_t peers = __ballot(valid);
-#pragma unroll
-        for (int b = 0; b < kRadixBits; ++b) {
-            const bool bit = (d >> b) & 1u;
-            const uint64_t m = __ballot(bit);
-            peers &= bit ? m : ~m;
-        }
-        const uint32_t rank = __builtin_amdgcn_mbcnt_hi(
-            (uint32_t)(peers >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)peers, 0u));
-        const int leader = __ffsll((unsigned long long)peers) - 1;
-        if (valid && lane == leader) wcount[wave][d] = (uint32_t)__popcll(peers);
-        __syncthreads();
-        if (valid) {
-            uint32_t off = base[d] + run[d] + rank;
-            for (int w = 0; w < wave; ++w) off += wcount[w][d];
-            if (off < n) {  // always true for a consistent histogram; keeps writes in bounds
-                kout[off] = key;
-                vout[off] = val;
-            }
-        }
-        __syncthreads();
-        uint32_t add = 0;
-        for (int w = 0; w < kWaves; ++w) {
-            add += wcount[w][tid];
-            wcount[w][tid] = 0;
-        }
-        run[tid] += add;
-        __syncthreads();
-    }
 }
 
 // ---- hierarchy ----
@@ -1351,17 +1239,8 @@ extern "C" int rt_scene_build(const rt_mesh_desc* meshes, uint32_t n_meshes, rt_
     hipLaunchKernelGGL(k_morton, dim3(blocks_for(n, 256)), dim3(256), 0, s, w.cent, n, w.cbounds,
                        w.keys0, w.vals0);
     // 4. radix sort, 4 passes over the 30-bit keys
-    const uint32_t nblk = blocks_for(n, kSortTile);
-    uint32_t *ki = w.keys0, *vi = w.vals0, *ko = w.keys1, *vo = w.vals1;
-    for (int shift = 0; shift < 30; shift += kRadixBits) {
-        hipLaunchKernelGGL(k_sort_hist, dim3(nblk), dim3(kSortThreads), 0, s, ki, n, shift, w.hist, nblk);
-        hipLaunchKernelGGL(k_sort_rowscan, dim3(kRadix), dim3(kSortThreads), 0, s, w.hist, nblk,
-                           w.totals);
-        hipLaunchKernelGGL(k_sort_scatter, dim3(nblk), dim3(kSortThreads), 0, s, ki, vi, n, shift,
-                           w.hist, w.totals, nblk, ko, vo);
-        std::swap(ki, ko);
-        std::swap(vi, vo);
-    }
+    uint32_t *ki = w.keys0, *vi = w.vals0;
+    sort_pairs(ki, vi, w.keys1, w.vals1, n, 30, w.hist, w.totals, s);
     // after 4 passes the sorted arrays are back in keys0/vals0
     float4* nodes = (float4*)sc->nodes;
     if (n > 1 && current_tuning().builder == RT_BUILD_PLOC) {
