@@ -10,7 +10,8 @@ A step = one launch of the fused ray-generation + closest-hit kernel over the wh
 value = rays / step time in Mrays/s. Also reported: LBVH build time, the full render
 (primary + shadow ray + Lambert + bilinear RGBA8 texture) latency p50/p99 over the same count of
 frames, the batched / serving / incoherent-path workloads, and ray_cast on explicit ray buffers
-(the frame's rays in camera order, and random rays in and around the sphere).
+(the frame's rays in camera order, and random rays in and around the sphere), and the mesh
+entry points on the sphere mesh (Mesh.compute_normals, MeshVertex pack/unpack).
 
 roofline: algorithmic bytes per launch = per ray 16 B of hit output + 128 B per 4-wide node
 record + 48 B per triangle record its ray tests, + 48 B per hit (the winner's record re-read for
@@ -62,6 +63,8 @@ def parse():
                     help="skip the incoherent-ray measurement (diffuse paths, 4 spp, 2 bounces)")
     ap.add_argument("--no-ray-cast", action="store_true",
                     help="skip the ray_cast measurement (ray buffers in camera order and random)")
+    ap.add_argument("--no-mesh", action="store_true",
+                    help="skip the mesh entry points (vertex normals, record pack/unpack) on the sphere")
     ap.add_argument("--copy-calib-mib", type=int, default=0,
                     help="first run 3 elementwise int32 adds over this many MiB (a kernel of "
                          "known read/write traffic that scales the FETCH_SIZE/WRITE_SIZE "
@@ -183,6 +186,45 @@ def measure_ray_cast(args, rc, cam, W, H, torch):
         res[name] = {"closest_ms": round(ms, 3), "closest_mrays_per_s": round(n / ms / 1e3, 1),
                      "any_hit_ms": round(ms_any, 3), "any_hit_mrays_per_s": round(n / ms_any / 1e3, 1),
                      "hits": int((out["prim"] >= 0).sum())}
+    return res
+
+
+def measure_mesh(args, ren, sphere, torch):
+    """The mesh entry points on the bench sphere (655,362 vertices, 1,310,720 indexed triangles).
+    normals: Mesh.compute_normals (rt_mesh_compute_normals) into a dense float32 [V, 3] tensor and
+    in place into the MeshVertex records; algorithmic bytes 12 B of indices per triangle + 12 B of
+    positions read and 12 B of normals written per vertex; checked bit-equal against the scene
+    generator's numpy normals. pack / unpack: MeshVertex records from / to dense P, N, C tensors
+    (rt_mesh_pack / rt_mesh_unpack); algorithmic bytes 32 B of dense attributes + 80 B of records
+    per vertex (pack writes whole records, unpack reads the P, N, C fields: 48 B of each record)."""
+    from rendertoy_amd.batch import time_launches
+    dev = torch.device("cuda")
+    nv, nt = len(sphere["pos"]), len(sphere["idx"])
+    reps = args.batch_reps * 2
+    m = ren.mesh_from_arrays(sphere["pos"], sphere["idx"], None, sphere["uv"])
+    out = torch.empty((nv, 3), dtype=torch.float32, device=dev)
+    ren.compute_normals(m, out=out)
+    same = bool(np.array_equal(out.cpu().numpy().view(np.uint32), sphere["nrm"].view(np.uint32)))
+    ms = float(np.median(time_launches(lambda: ren.compute_normals(m, out=out), reps)))
+    ms_rec = float(np.median(time_launches(lambda: m.compute_normals(), reps)))
+    algo = 12 * nt + 24 * nv
+    res = {"normals": {
+        "what": f"area-weighted vertex normals, {nv} vertices, {nt} indexed triangles (FP64 face "
+                "normals, stable radix sort of the (vertex, pair) records, per-vertex sums)",
+        "dense_ms": round(ms, 4), "records_ms": round(ms_rec, 4), "algorithmic_bytes": algo,
+        "algorithmic_gbs": round(algo / (ms * 1e-3) / 1e9, 1), "equals_generator": same}}
+    lib = ren._native.load()
+    t = {k: torch.from_numpy(sphere[k]).to(dev) for k in ("pos", "nrm", "uv")}
+    rec = torch.empty(nv * 80, dtype=torch.uint8, device=dev)
+    s = ren._native.stream_handle()
+    pack = lambda: lib.rt_mesh_pack(t["pos"].data_ptr(), t["nrm"].data_ptr(), t["uv"].data_ptr(), nv,
+                                    rec.data_ptr(), s)
+    unpack = lambda: lib.rt_mesh_unpack(rec.data_ptr(), nv, t["pos"].data_ptr(), t["nrm"].data_ptr(),
+                                        t["uv"].data_ptr(), s)
+    for name, fn, algo in (("pack", pack, (32 + 80) * nv), ("unpack", unpack, (32 + 48) * nv)):
+        ms = float(np.median(time_launches(fn, reps)))
+        res[name] = {"ms": round(ms, 4), "algorithmic_bytes": algo,
+                     "algorithmic_gbs": round(algo / (ms * 1e-3) / 1e9, 1)}
     return res
 
 
@@ -409,6 +451,8 @@ def main():
         result["incoherent"] = measure_incoherent(args, ren, scenes, torch)
     if not args.no_ray_cast:
         result["ray_cast"] = measure_ray_cast(args, rc, cam, W, H, torch)
+    if not args.no_mesh:
+        result["mesh"] = measure_mesh(args, ren, sphere, torch)
 
     # ---- roofline and CPU baseline from the CPU restatement (oracle) ----
     if not args.no_roofline or not args.no_cpu_baseline:

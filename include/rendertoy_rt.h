@@ -323,6 +323,16 @@ uint64_t rt_mesh_normals_workspace_bytes(uint32_t n_vertices, uint32_t n_tris);
 int rt_mesh_compute_normals(const rt_mesh_desc* mesh, float* normals_out, uint32_t out_stride,
                             void* workspace_dev, uint64_t workspace_bytes, void* stream);
 
+/* MeshVertex records (rendering/_modeling.py:22-28: 80 bytes, P@0, N@16, C@32, T@48, B@64, float3
+ * fields padded to 16 bytes) from dense device arrays pos [n,3], nrm [n,3] and uv [n,2] floats;
+ * nrm/uv may be NULL (written as zero), T, B and the pad words are zero. This is the device form of
+ * filling `Mesh(vertices=...)` from arrays; `records` must be 16-byte aligned. */
+int rt_mesh_pack(const float* pos, const float* nrm, const float* uv, uint32_t n_vertices,
+                 void* records, void* stream);
+/* The inverse: dense P / N / C arrays out of MeshVertex records; any output may be NULL. */
+int rt_mesh_unpack(const void* records, uint32_t n_vertices, float* pos, float* nrm, float* uv,
+                   void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -13,7 +13,8 @@ from ._core import (kernel_struct, create_buffer_from, create_buffer, create_str
 from ._core import (float2, float3, float4, int2, int3, int4, uint2, uint3, uint4, float4x4, RGBA,
                     Texture2D, create_texture2D, texture_pointer)
 from ._modeling import Mesh, WeldMode, SubdivisionMode, MeshVertex, mesh_from_arrays
-from ._raycaster import Raycaster, BVH_AABB, BVH_Triangle, Ray, Hit, Camera, compute_normals
+from ._raycaster import Raycaster, BVH_AABB, BVH_Triangle, Ray, Hit, Camera, compute_normals, \
+    mesh_from_tensors, mesh_tensors
 from .tuning import get_tuning, set_tuning, tuning
 
 __all__ = [n for n in dir() if not n.startswith("_")]

@@ -224,6 +224,8 @@ SIGNATURES = [
     ("rt_mesh_compute_normals", ctypes.c_int,
      [ctypes.POINTER(rt_mesh_desc), ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p,
       ctypes.c_uint64, ctypes.c_void_p]),
+    ("rt_mesh_pack", ctypes.c_int, [ctypes.c_void_p] * 3 + [ctypes.c_uint32, ctypes.c_void_p, ctypes.c_void_p]),
+    ("rt_mesh_unpack", ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32] + [ctypes.c_void_p] * 4),
 ]
 
 _lib = None

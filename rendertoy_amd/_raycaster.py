@@ -171,6 +171,79 @@ def compute_normals(mesh: Mesh, out: torch.Tensor = None, stream=None):
     return out
 
 
+def _dense(t, cols, what, n=None):
+    if t is None:
+        return None
+    _check_device(t, what)
+    if t.dtype != torch.float32 or not t.is_contiguous() or t.dim() != 2 or t.shape[1] != cols:
+        raise TypeError(f"{what} must be a contiguous float32 device tensor [n, {cols}]")
+    if n is not None and t.shape[0] != n:
+        raise ValueError(f"{what} has {t.shape[0]} rows, expected {n}")
+    return t
+
+
+def mesh_from_tensors(positions: torch.Tensor, indices: torch.Tensor = None, normals: torch.Tensor = None,
+                      uvs: torch.Tensor = None, stream=None) -> Mesh:
+    """A Mesh built on the device from device tensors (positions [V,3], optional normals [V,3] and
+    uvs [V,2] float32, indices [T,3] int32/int64): the MeshVertex records (rendering/_modeling.py:
+    22-28) are written by rt_mesh_pack, with no host round trip. The device counterpart of
+    mesh_from_arrays; indices are kept as an int32 device buffer."""
+    lib = N.require_gpu()
+    pos = _dense(positions, 3, "positions")
+    n = pos.shape[0]
+    nrm = _dense(normals, 3, "normals", n)
+    uv = _dense(uvs, 2, "uvs", n)
+    ts = stream if stream is not None else torch.cuda.current_stream()
+    with torch.cuda.stream(ts):
+        vb = DeviceArray(torch.empty(max(n * MeshVertex.itemsize, 1), dtype=torch.uint8,
+                                     device=torch.device("cuda")), MeshVertex, (n,))
+        N.check(lib.rt_mesh_pack(pos.data_ptr(), None if nrm is None else nrm.data_ptr(),
+                                 None if uv is None else uv.data_ptr(), n, vb.data_ptr(),
+                                 N.stream_handle(ts)), "rt_mesh_pack")
+        ib = None
+        if indices is not None:
+            _check_device(indices, "indices")
+            if indices.dtype not in (torch.int32, torch.int64):
+                raise TypeError("indices must be an int32 or int64 device tensor")
+            if indices.dtype == torch.int64 and indices.numel() and (
+                    int(indices.max()) >= 2 ** 31 or int(indices.min()) < -2 ** 31):
+                raise ValueError("indices must fit int32")
+            i32 = indices.reshape(-1).to(torch.int32).contiguous()
+            ib = DeviceArray(i32.view(torch.uint8), np.int32, (i32.numel(),))
+    m = Mesh(vb, ib)
+    m.has_normals = nrm is not None
+    m.has_uvs = uv is not None
+    return m
+
+
+def mesh_tensors(mesh: Mesh, stream=None) -> dict:
+    """The mesh's vertex attributes as dense device tensors (positions/normals [V,3], uvs [V,2]
+    float32) read out of its MeshVertex records by rt_mesh_unpack, plus `indices` [T,3] (int32 view
+    of the index buffer, or None)."""
+    lib = N.require_gpu()
+    vb = mesh.vertices
+    _check_device(vb, "Mesh.vertices")
+    if vb.dtype != MeshVertex:
+        raise TypeError("Mesh.vertices must be a buffer of MeshVertex")
+    n = vb.shape[0]
+    ts = stream if stream is not None else torch.cuda.current_stream()
+    dev = torch.device("cuda")
+    with torch.cuda.stream(ts):
+        out = {"positions": torch.empty((n, 3), dtype=torch.float32, device=dev),
+               "normals": torch.empty((n, 3), dtype=torch.float32, device=dev),
+               "uvs": torch.empty((n, 2), dtype=torch.float32, device=dev)}
+        N.check(lib.rt_mesh_unpack(vb.data_ptr(), n, out["positions"].data_ptr(),
+                                   out["normals"].data_ptr(), out["uvs"].data_ptr(),
+                                   N.stream_handle(ts)), "rt_mesh_unpack")
+    ib = mesh.indices
+    if ib is None:
+        out["indices"] = None
+    else:
+        t = ib.tensor(torch.int64 if ib.dtype == np.int64 else torch.int32)
+        out["indices"] = t.to(torch.int32).reshape(-1, 3)
+    return out
+
+
 class TextureBinding:
     """Textures and the mesh -> texture map of a scene uploaded once (device table of rt_texture
     and int32 map), for callers that render the same scene many times; pass it as `textures`."""

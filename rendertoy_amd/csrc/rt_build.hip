@@ -7,10 +7,10 @@
 //                  AABB centroid.
 //   2. bounds      two-stage min/max reduction of the centroids.
 //   3. morton      30-bit Morton code of each centroid quantised to 1024^3 over those bounds.
-//   4. sort        stable LSD radix sort of (code, primitive) in 4 passes of 8 bits; each pass is
-//                  per-tile histogram -> per-digit row scan -> scatter; the scatter block scans
-//                  the 256 digit totals itself and ranks keys in LDS with wave64 ballot/mbcnt
-//                  multi-split so equal digits keep input order.
+//   4. sort        stable LSD radix sort of (code, primitive) in 4 passes of 8 bits (rt_sort.h):
+//                  per-tile histogram -> per-digit row scan -> scatter, the scatter ranking keys
+//                  with wave64 ballot/mbcnt multi-split so equal digits keep input order and
+//                  writing each tile digit by digit from LDS.
 //   5. hierarchy   Karras 2012 binary radix tree over the extended keys (code, slot).
 //   6. emit        permute tri/shade records into slot order.
 //   7. refit       bottom-up from every leaf; the second thread to reach a node unions its

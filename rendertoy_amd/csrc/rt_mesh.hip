@@ -1,4 +1,4 @@
-// rt_mesh.hip — per-vertex normals of a mesh on the device: Mesh.compute_normals
+// rt_mesh.hip — mesh data on the device. Per-vertex normals: Mesh.compute_normals
 // (rendering/_modeling.py:48-49, a stub in the reference that raises "Not implemented yet").
 //
 // Definition (area-weighted vertex normals):
@@ -19,6 +19,9 @@
 //   4. k_nrm_vertex per vertex: sums its run, normalises, writes float3 N at out + v * out_stride.
 // A non-indexed mesh (vertices 3t, 3t+1, 3t+2) has one pair per vertex: k_nrm_direct computes it
 // with no sort and no workspace.
+//
+// Also here: MeshVertex records <-> dense float arrays (k_mesh_pack / k_mesh_unpack), the device-side
+// ingest and export of the 80-byte vertex layout of rendering/_modeling.py:22-28.
 #include <cmath>
 
 #include "rt_common.h"
@@ -175,6 +178,52 @@ __global__ void __launch_bounds__(256) k_nrm_direct(rt_mesh_desc m, char* __rest
     store_normal(out, v, out_stride, 0.0 + f.x, 0.0 + f.y, 0.0 + f.z);
 }
 
+// MeshVertex records (80 B: P@0, N@16, C@32, T@48, B@64, float3 fields padded to 16 B) from
+// dense float arrays; absent attributes, T, B and the pad words are written as zero. One thread per
+// vertex: five 16-byte stores, coalesced across the wave.
+__global__ void __launch_bounds__(256) k_mesh_pack(const float* __restrict__ pos,
+                                                    const float* __restrict__ nrm,
+                                                    const float* __restrict__ uv, uint32_t n,
+                                                    float4* __restrict__ rec) {
+    const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= n) return;
+    const uint64_t i3 = 3 * (uint64_t)v, i2 = 2 * (uint64_t)v;
+    float4* r = rec + 5 * (uint64_t)v;
+    r[0] = make_float4(pos[i3], pos[i3 + 1], pos[i3 + 2], 0.0f);
+    r[1] = nrm ? make_float4(nrm[i3], nrm[i3 + 1], nrm[i3 + 2], 0.0f) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    r[2] = uv ? make_float4(uv[i2], uv[i2 + 1], 0.0f, 0.0f) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    r[3] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    r[4] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+}
+
+// Dense P / N / C arrays out of MeshVertex records (any output may be NULL).
+__global__ void __launch_bounds__(256) k_mesh_unpack(const float4* __restrict__ rec, uint32_t n,
+                                                      float* __restrict__ pos,
+                                                      float* __restrict__ nrm,
+                                                      float* __restrict__ uv) {
+    const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= n) return;
+    const uint64_t i3 = 3 * (uint64_t)v, i2 = 2 * (uint64_t)v;
+    const float4* r = rec + 5 * (uint64_t)v;
+    if (pos) {
+        const float4 a = r[0];
+        pos[i3] = a.x;
+        pos[i3 + 1] = a.y;
+        pos[i3 + 2] = a.z;
+    }
+    if (nrm) {
+        const float4 b = r[1];
+        nrm[i3] = b.x;
+        nrm[i3 + 1] = b.y;
+        nrm[i3 + 2] = b.z;
+    }
+    if (uv) {
+        const float4 c = r[2];
+        uv[i2] = c.x;
+        uv[i2 + 1] = c.y;
+    }
+}
+
 inline uint32_t nblocks(uint64_t n, uint32_t per) { return (uint32_t)((n + per - 1) / per); }
 
 }  // namespace
@@ -233,4 +282,28 @@ extern "C" int rt_mesh_compute_normals(const rt_mesh_desc* mesh, float* normals_
     hipLaunchKernelGGL(k_nrm_vertex, dim3(nblocks(m.n_vertices, 256)), dim3(256), 0, s, ki, vi, np,
                        m.n_tris, w.face, w.run_of, m.n_vertices, out, out_stride);
     return check_launch("rt_mesh_compute_normals");
+}
+
+extern "C" int rt_mesh_pack(const float* pos, const float* nrm, const float* uv, uint32_t n_vertices,
+                            void* records, void* stream) {
+    if (n_vertices == 0) return RT_OK;
+    if (!pos || !records) return set_error(RT_EINVAL, "rt_mesh_pack: positions and records are required");
+    if (((uintptr_t)pos | (uintptr_t)nrm | (uintptr_t)uv) & 3)
+        return set_error(RT_EINVAL, "rt_mesh_pack: arrays must be 4-byte aligned");
+    if ((uintptr_t)records & 15) return set_error(RT_EINVAL, "rt_mesh_pack: records must be 16-byte aligned");
+    hipLaunchKernelGGL(k_mesh_pack, dim3(nblocks(n_vertices, 256)), dim3(256), 0, (hipStream_t)stream,
+                       pos, nrm, uv, n_vertices, (float4*)records);
+    return check_launch("rt_mesh_pack");
+}
+
+extern "C" int rt_mesh_unpack(const void* records, uint32_t n_vertices, float* pos, float* nrm, float* uv,
+                              void* stream) {
+    if (n_vertices == 0) return RT_OK;
+    if (!records) return set_error(RT_EINVAL, "rt_mesh_unpack: records is NULL");
+    if ((uintptr_t)records & 15) return set_error(RT_EINVAL, "rt_mesh_unpack: records must be 16-byte aligned");
+    if (((uintptr_t)pos | (uintptr_t)nrm | (uintptr_t)uv) & 3)
+        return set_error(RT_EINVAL, "rt_mesh_unpack: arrays must be 4-byte aligned");
+    hipLaunchKernelGGL(k_mesh_unpack, dim3(nblocks(n_vertices, 256)), dim3(256), 0, (hipStream_t)stream,
+                       (const float4*)records, n_vertices, pos, nrm, uv);
+    return check_launch("rt_mesh_unpack");
 }

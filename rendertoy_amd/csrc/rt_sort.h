@@ -1,7 +1,8 @@
 // rt_sort.h — stable LSD radix sort of (uint32 key, uint32 value) pairs on the device, used by the
 // scene build (Morton codes) and by the per-vertex normals (vertex ids). Each pass of 8 bits is a
 // per-tile histogram -> per-digit row scan -> scatter; the scatter block scans the 256 digit totals
-// itself and ranks keys with wave64 ballot/mbcnt multi-split so equal digits keep input order.
+// itself, ranks keys with wave64 ballot/mbcnt multi-split so equal digits keep input order, groups
+// its tile by digit in LDS and writes each digit's run of the tile contiguously.
 #pragma once
 
 #include <utility>
@@ -58,34 +59,47 @@ __global__ void __launch_bounds__(kSortThreads) k_sort_rowscan(uint32_t* __restr
     }
     if (threadIdx.x == 0) totals[blockIdx.x] = run;
 }
-
-// Stable scatter: items of a tile are visited in rounds of 256 (item order = (round, thread)),
-// each round ranked by wave64 multi-split (8 ballots give the lanes sharing this lane's digit).
+// Stable scatter of one tile of kSortTile items. Items are visited in rounds of 256 (item order =
+// (round, thread)); each round is ranked by wave64 multi-split (8 ballots give the lanes sharing
+// this lane's digit) and written into LDS grouped by digit, so the tile ends up in LDS in
+// (digit, input order). The tile is then written out with consecutive threads taking consecutive
+// items: each digit's run of the tile lands in one contiguous span of the output.
 __global__ void __launch_bounds__(kSortThreads) k_sort_scatter(
     const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin, uint32_t n, int shift,
     const uint32_t* __restrict__ hist, const uint32_t* __restrict__ totals, uint32_t nblk,
     uint32_t* __restrict__ kout,
     uint32_t* __restrict__ vout) {
     constexpr int kWaves = kSortThreads / 64;
-    __shared__ uint32_t base[kRadix];
+    __shared__ uint32_t base[kRadix];    // output position of this tile's first item of digit d
+    __shared__ uint32_t lstart[kRadix];  // LDS position of this tile's first item of digit d
     __shared__ uint32_t run[kRadix];
     __shared__ uint32_t wcount[kWaves][kRadix];
+    __shared__ uint32_t sk[kSortTile];
+    __shared__ uint32_t sv[kSortTile];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    // exclusive scan of the digit totals gives each digit's first output position
+    const uint32_t blk = blockIdx.x;
+    // the row scan left exclusive per-tile prefixes: this tile's count of digit tid is the step
+    // to the next tile's prefix (to the digit total for the last tile)
+    const uint32_t h0 = hist[(uint64_t)tid * nblk + blk];
+    const uint32_t cnt = (blk + 1 < nblk ? hist[(uint64_t)tid * nblk + blk + 1] : totals[tid]) - h0;
     run[tid] = totals[tid];
+    lstart[tid] = cnt;
     __syncthreads();
-    for (int off = 1; off < kRadix; off <<= 1) {
+    for (int off = 1; off < kRadix; off <<= 1) {  // inclusive scans of the totals and tile counts
         const uint32_t x = tid >= off ? run[tid - off] : 0u;
+        const uint32_t y = tid >= off ? lstart[tid - off] : 0u;
         __syncthreads();
         run[tid] += x;
+        lstart[tid] += y;
         __syncthreads();
     }
-    base[tid] = run[tid] - totals[tid] + hist[(uint64_t)tid * nblk + blockIdx.x];
-    __syncthreads();
+    base[tid] = run[tid] - totals[tid] + h0;
+    lstart[tid] -= cnt;
     run[tid] = 0;
     for (int w = 0; w < kWaves; ++w) wcount[w][tid] = 0;
     __syncthreads();
-    const uint64_t tile = (uint64_t)blockIdx.x * kSortTile;
+    const uint64_t tile = (uint64_t)blk * kSortTile;
+    const uint32_t m = (uint32_t)(n - tile < (uint64_t)kSortTile ? n - tile : (uint64_t)kSortTile);
     for (int r = 0; r < kSortItems; ++r) {
         const uint64_t i = tile + (uint64_t)r * kSortThreads + tid;
         const bool valid = i < n;
@@ -96,8 +110,8 @@ __global__ void __launch_bounds__(kSortThreads) k_sort_scatter(
 #pragma unroll
         for (int b = 0; b < kRadixBits; ++b) {
             const bool bit = (d >> b) & 1u;
-            const uint64_t m = __ballot(bit);
-            peers &= bit ? m : ~m;
+            const uint64_t mb = __ballot(bit);
+            peers &= bit ? mb : ~mb;
         }
         const uint32_t rank = __builtin_amdgcn_mbcnt_hi(
             (uint32_t)(peers >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)peers, 0u));
@@ -105,11 +119,11 @@ __global__ void __launch_bounds__(kSortThreads) k_sort_scatter(
         if (valid && lane == leader) wcount[wave][d] = (uint32_t)__popcll(peers);
         __syncthreads();
         if (valid) {
-            uint32_t off = base[d] + run[d] + rank;
+            uint32_t off = lstart[d] + run[d] + rank;
             for (int w = 0; w < wave; ++w) off += wcount[w][d];
-            if (off < n) {  // always true for a consistent histogram; keeps writes in bounds
-                kout[off] = key;
-                vout[off] = val;
+            if (off < (uint32_t)kSortTile) {  // always true for a consistent histogram
+                sk[off] = key;
+                sv[off] = val;
             }
         }
         __syncthreads();
@@ -120,6 +134,15 @@ __global__ void __launch_bounds__(kSortThreads) k_sort_scatter(
         }
         run[tid] += add;
         __syncthreads();
+    }
+    for (uint32_t j = tid; j < m; j += kSortThreads) {
+        const uint32_t key = sk[j];
+        const uint32_t d = (key >> shift) & (kRadix - 1);
+        const uint32_t off = base[d] + (j - lstart[d]);
+        if (off < n) {  // always true for a consistent histogram; keeps writes in bounds
+            kout[off] = key;
+            vout[off] = sv[j];
+        }
     }
 }
 

@@ -1,6 +1,10 @@
-"""GPU parity of the per-vertex normals (rt_mesh_compute_normals via Mesh.compute_normals and
-rendertoy_amd.compute_normals) against the oracle's twin, bit for bit, across the sort's pass counts
-(1, 2, 3 and 4 passes of 8 bits over the vertex ids), index kinds and edge cases."""
+"""GPU parity of the mesh entry points of rt_mesh.hip.
+
+- Per-vertex normals (rt_mesh_compute_normals via Mesh.compute_normals and
+  rendertoy_amd.compute_normals) against the oracle's twin, bit for bit, across the sort's pass
+  counts (1, 2, 3 and 4 passes of 8 bits over the vertex ids), index kinds and edge cases.
+- MeshVertex records <-> dense tensors (rt_mesh_pack / rt_mesh_unpack) against the host packing
+  of mesh_from_arrays, byte for byte, and round trips."""
 import numpy as np
 import pytest
 import torch
@@ -123,3 +127,55 @@ def test_bad_arguments_raise(ren):
         ren.compute_normals(m, out=torch.empty((3, 3), dtype=torch.float32, device="cuda"))
     with pytest.raises(TypeError):
         ren.compute_normals(m, out=torch.empty((len(d["pos"]), 3), dtype=torch.float64, device="cuda"))
+
+
+def test_pack_equals_host_packing_and_round_trips(ren):
+    from rendertoy_amd import scenes
+    d = scenes.icosphere(64, 1000)
+    dev = torch.device("cuda")
+    for nrm, uv in ((d["nrm"], d["uv"]), (None, d["uv"]), (d["nrm"], None), (None, None)):
+        host = ren.mesh_from_arrays(d["pos"], d["idx"], nrm, uv)
+        m = ren.mesh_from_tensors(torch.from_numpy(d["pos"]).to(dev), torch.from_numpy(d["idx"]).to(dev),
+                                  None if nrm is None else torch.from_numpy(nrm).to(dev),
+                                  None if uv is None else torch.from_numpy(uv).to(dev))
+        assert m.has_normals == (nrm is not None) and m.has_uvs == (uv is not None)
+        np.testing.assert_array_equal(m.vertices.get().view(np.uint8), host.vertices.get().view(np.uint8))
+        np.testing.assert_array_equal(m.indices.get(), host.indices.get())
+        back = ren.mesh_tensors(m)
+        np.testing.assert_array_equal(bits(back["positions"].cpu().numpy()), bits(d["pos"]))
+        np.testing.assert_array_equal(bits(back["normals"].cpu().numpy()),
+                                      bits(np.zeros_like(d["pos"]) if nrm is None else nrm))
+        np.testing.assert_array_equal(bits(back["uvs"].cpu().numpy()),
+                                      bits(np.zeros_like(d["uv"]) if uv is None else uv))
+        np.testing.assert_array_equal(back["indices"].cpu().numpy(), d["idx"])
+
+
+def test_tensor_mesh_builds_the_same_scene(ren):
+    from rendertoy_amd import scenes
+    d = scenes.icosphere(32, 44)
+    dev = torch.device("cuda")
+    m = ren.mesh_from_tensors(torch.from_numpy(d["pos"]).to(dev),
+                              torch.from_numpy(d["idx"].astype(np.int64)).to(dev))
+    m.compute_normals()
+    host = ren.mesh_from_arrays(d["pos"], d["idx"], d["nrm"], None)
+    a, b = ren.Raycaster([m]), ren.Raycaster([host])
+    for k in ("_nodes4", "_tris", "_shade"):
+        assert torch.equal(getattr(a, k), getattr(b, k)), k
+    np.testing.assert_array_equal(bits(ren.mesh_tensors(m)["normals"].cpu().numpy()), bits(d["nrm"]))
+
+
+def test_pack_rejects_bad_arguments(ren):
+    from rendertoy_amd import _native
+    lib = _native.load()
+    buf = torch.empty(1024, dtype=torch.uint8, device="cuda")
+    p = buf.data_ptr()
+    s = _native.stream_handle()
+    assert lib.rt_mesh_pack(None, None, None, 4, p, s) == -1           # positions missing
+    assert lib.rt_mesh_pack(p + 2, None, None, 4, p + 512, s) == -1      # misaligned positions
+    assert lib.rt_mesh_pack(p, None, None, 4, p + 520, s) == -1          # records not 16-byte aligned
+    assert lib.rt_mesh_unpack(p + 8, 4, p + 512, None, None, s) == -1    # records not 16-byte aligned
+    assert lib.rt_mesh_pack(p, None, None, 0, None, s) == 0              # nothing to do
+    with pytest.raises(TypeError):
+        ren.mesh_from_tensors(torch.zeros((4, 3), dtype=torch.float64, device="cuda"))
+    with pytest.raises(ValueError):
+        ren.mesh_from_tensors(torch.zeros((4, 3), device="cuda"), normals=torch.zeros((5, 3), device="cuda"))
