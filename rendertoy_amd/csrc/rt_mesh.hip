@@ -179,21 +179,33 @@ __global__ void __launch_bounds__(256) k_nrm_direct(rt_mesh_desc m, char* __rest
 }
 
 // MeshVertex records (80 B: P@0, N@16, C@32, T@48, B@64, float3 fields padded to 16 B) from
-// dense float arrays; absent attributes, T, B and the pad words are written as zero. One thread per
-// vertex: five 16-byte stores, coalesced across the wave.
+// dense float arrays; absent attributes, T, B and the pad words are written as zero. A wave writes
+// the 64 records of its vertices as 320 consecutive 16-byte chunks, lane l taking chunks l + 64j,
+// so each store instruction covers 1 KiB of contiguous output; chunk c is field c % 5 of vertex
+// c / 5.
 __global__ void __launch_bounds__(256) k_mesh_pack(const float* __restrict__ pos,
                                                     const float* __restrict__ nrm,
                                                     const float* __restrict__ uv, uint32_t n,
                                                     float4* __restrict__ rec) {
-    const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= n) return;
-    const uint64_t i3 = 3 * (uint64_t)v, i2 = 2 * (uint64_t)v;
-    float4* r = rec + 5 * (uint64_t)v;
-    r[0] = make_float4(pos[i3], pos[i3 + 1], pos[i3 + 2], 0.0f);
-    r[1] = nrm ? make_float4(nrm[i3], nrm[i3 + 1], nrm[i3 + 2], 0.0f) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    r[2] = uv ? make_float4(uv[i2], uv[i2 + 1], 0.0f, 0.0f) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    r[3] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    r[4] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint64_t v0 = (uint64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63u);  // the wave's first vertex
+    if (v0 >= n) return;
+    const uint32_t nw = n - v0 < 64 ? (uint32_t)(n - v0) : 64u;
+#pragma unroll
+    for (uint32_t j = 0; j < 5; ++j) {
+        const uint32_t c = lane + 64u * j, vl = c / 5u, f = c - 5u * vl;
+        if (vl >= nw) break;
+        const uint64_t v = v0 + vl;
+        float4 x = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (f == 0) {
+            x = make_float4(pos[3 * v], pos[3 * v + 1], pos[3 * v + 2], 0.0f);
+        } else if (f == 1) {
+            if (nrm) x = make_float4(nrm[3 * v], nrm[3 * v + 1], nrm[3 * v + 2], 0.0f);
+        } else if (f == 2) {
+            if (uv) x = make_float4(uv[2 * v], uv[2 * v + 1], 0.0f, 0.0f);
+        }
+        rec[5 * v0 + c] = x;
+    }
 }
 
 // Dense P / N / C arrays out of MeshVertex records (any output may be NULL).
