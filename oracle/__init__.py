@@ -72,12 +72,33 @@ def lib():
         L.tw_closest64_bvh.argtypes = [vp, i64, vp, vp, i32, i64] + [vp] * 6 + [f64, f64, f64, f64] + [vp] * 5
         L.tw_tex_sample.restype = None
         L.tw_tex_sample.argtypes = [vp, i32, i32, i64, vp, vp, i32, vp]
+        L.tw_raster_ids.restype = None
+        L.tw_raster_ids.argtypes = [vp, i64, vp, ctypes.c_uint32, ctypes.c_uint32, f64, f64, vp, vp]
         L.tw_vertex_normals.restype = None
         L.tw_vertex_normals.argtypes = [vp, i64, vp, i64, vp]
         L.tw_num_threads.restype = ctypes.c_int
         L.tw_set_threads.argtypes = [ctypes.c_int]
         _lib = L
     return _lib
+
+
+RASTER_EDGE, RASTER_DEPTH_TIE = 1, 2
+
+
+def raster_ids(tri_pos, cam12, W, H, edge_eps=2e-3, depth_eps=1e-5):
+    """Primary visibility by rasterisation (tw_raster_ids): the triangle each pixel centre sees by
+    projection, near-plane clipping, screen-space edge functions and the nearest depth, in FP64.
+    tri_pos [T,9] float32, cam12 the device camera (eye, right, up, forward). Returns prim [H*W]
+    (-1 = background) and flags [H*W] (RASTER_EDGE: a triangle within edge_eps pixels of the centre
+    lies at or in front of the winner; RASTER_DEPTH_TIE: a second covering triangle within
+    depth_eps relative depth)."""
+    tri_pos = np.ascontiguousarray(tri_pos, np.float32).reshape(-1, 9)
+    cam12 = np.ascontiguousarray(cam12, np.float32)
+    prim = np.empty(W * H, np.int32)
+    flags = np.empty(W * H, np.uint8)
+    lib().tw_raster_ids(_p(tri_pos), tri_pos.shape[0], _p(cam12), W, H, float(edge_eps),
+                        float(depth_eps), _p(prim), _p(flags))
+    return {"prim": prim, "flags": flags}
 
 
 def vertex_normals(pos, idx=None):

@@ -1500,3 +1500,185 @@ TW_API void tw_vertex_normals(const float* pos, int64_t n_vertices, const int32_
     }
     free(acc);
 }
+
+/* ---- raster cross-check of primary visibility ----
+ * A second statement of which triangle each pixel centre sees, decided the way the reference's
+ * raster pipeline decides it rather than by rays: project the vertices, clip at the near plane
+ * (rendering/_raster.py:172-214), test the pixel centre (col + 0.5, row + 0.5) against screen-space
+ * edge functions (:282-311) and keep the nearest depth (:80-112). FP64 throughout, without the
+ * reference's fragment-capacity and 64x64-pixel limits. Camera: the device's 12 floats (eye,
+ * right, up, forward); a point P maps to (a, b, s) = M^-1 (P - eye) with M = [right up forward],
+ * ndc = (a/s, b/s), X = (ndc.x + 1) W/2, Y = (1 - ndc.y) H/2: the inverse of the device's ray
+ * generation, so the depth order along a pixel's ray is the order of s, and 1/s is affine in
+ * (X, Y) over a triangle. Per pixel: prim = the triangle of smallest s whose closed screen
+ * triangle contains the centre (ties to the smaller prim), -1 if none. Depths are compared as ranges: a triangle's 1/s over the disc of radius
+ * edge_eps around the centre (its screen gradient times edge_eps), widened by depth_eps relative.
+ * flags: 1 (edge) when a triangle whose edge passes within edge_eps pixels of the centre may lie at
+ * or in front of the winner; 2 (depth tie) when another covering triangle may. Those are the pixels
+ * on which a ray moved by less than edge_eps pixels could see another triangle. */
+typedef struct {
+    double X[3], Y[3], iz[3];
+    double x0, x1, y0, y1;
+    int32_t prim;
+    int internal;  /* bit j: edge j (vertex j -> j+1) is a diagonal of a clipped polygon's fan */
+    double giz;    /* |grad (1/s)| in screen space, per pixel */
+} rtri;
+
+static int clip_near(double v[][3], int n, double sn, double out[][3]) {
+    int m = 0;
+    for (int i = 0; i < n; ++i) {
+        const double* p = v[i];
+        const double* q = v[(i + 1) % n];
+        const int pin = p[2] >= sn, qin = q[2] >= sn;
+        if (pin) { out[m][0] = p[0]; out[m][1] = p[1]; out[m][2] = p[2]; ++m; }
+        if (pin != qin) {
+            const double f = (sn - p[2]) / (q[2] - p[2]);
+            out[m][0] = p[0] + f * (q[0] - p[0]);
+            out[m][1] = p[1] + f * (q[1] - p[1]);
+            out[m][2] = sn;
+            ++m;
+        }
+    }
+    return m;
+}
+
+TW_API void tw_raster_ids(const float* tri_pos, int64_t n_tris, const float* cam, uint32_t W, uint32_t H,
+                          double edge_eps, double depth_eps, int32_t* prim_out, uint8_t* flags_out) {
+    /* M^-1 by the adjugate */
+    double M[3][3];
+    for (int r = 0; r < 3; ++r) {
+        M[r][0] = cam[3 + r];
+        M[r][1] = cam[6 + r];
+        M[r][2] = cam[9 + r];
+    }
+    const double det = M[0][0] * (M[1][1] * M[2][2] - M[1][2] * M[2][1]) -
+                       M[0][1] * (M[1][0] * M[2][2] - M[1][2] * M[2][0]) +
+                       M[0][2] * (M[1][0] * M[2][1] - M[1][1] * M[2][0]);
+    double Mi[3][3];
+    Mi[0][0] = (M[1][1] * M[2][2] - M[1][2] * M[2][1]) / det;
+    Mi[0][1] = (M[0][2] * M[2][1] - M[0][1] * M[2][2]) / det;
+    Mi[0][2] = (M[0][1] * M[1][2] - M[0][2] * M[1][1]) / det;
+    Mi[1][0] = (M[1][2] * M[2][0] - M[1][0] * M[2][2]) / det;
+    Mi[1][1] = (M[0][0] * M[2][2] - M[0][2] * M[2][0]) / det;
+    Mi[1][2] = (M[0][2] * M[1][0] - M[0][0] * M[1][2]) / det;
+    Mi[2][0] = (M[1][0] * M[2][1] - M[1][1] * M[2][0]) / det;
+    Mi[2][1] = (M[0][1] * M[2][0] - M[0][0] * M[2][1]) / det;
+    Mi[2][2] = (M[0][0] * M[1][1] - M[0][1] * M[1][0]) / det;
+    const double sn = 1e-9;
+    rtri* rt = (rtri*)malloc(sizeof(rtri) * (size_t)(2 * (n_tris ? n_tris : 1)));
+    int64_t nr = 0;
+    for (int64_t t = 0; t < n_tris; ++t) {
+        double v[3][3], c[4][3];
+        for (int k = 0; k < 3; ++k) {
+            const double d0 = (double)tri_pos[9 * t + 3 * k] - cam[0];
+            const double d1 = (double)tri_pos[9 * t + 3 * k + 1] - cam[1];
+            const double d2 = (double)tri_pos[9 * t + 3 * k + 2] - cam[2];
+            for (int r = 0; r < 3; ++r) v[k][r] = Mi[r][0] * d0 + Mi[r][1] * d1 + Mi[r][2] * d2;
+        }
+        const int m = clip_near(v, 3, sn, c);
+        for (int f = 1; f + 1 < m; ++f) {  /* fan of the clipped polygon */
+            const int id[3] = {0, f, f + 1};
+            rtri* q = &rt[nr];
+            for (int k = 0; k < 3; ++k) {
+                const double* p = c[id[k]];
+                q->X[k] = (p[0] / p[2] + 1.0) * 0.5 * W;
+                q->Y[k] = (1.0 - p[1] / p[2]) * 0.5 * H;
+                q->iz[k] = 1.0 / p[2];
+            }
+            q->x0 = fmin(fmin(q->X[0], q->X[1]), q->X[2]);
+            q->x1 = fmax(fmax(q->X[0], q->X[1]), q->X[2]);
+            q->y0 = fmin(fmin(q->Y[0], q->Y[1]), q->Y[2]);
+            q->y1 = fmax(fmax(q->Y[0], q->Y[1]), q->Y[2]);
+            q->prim = (int32_t)t;
+            q->internal = (f > 1 ? 1 : 0) | (f + 2 < m ? 4 : 0);
+            {
+                const double ar = (q->X[1] - q->X[0]) * (q->Y[2] - q->Y[0]) - (q->Y[1] - q->Y[0]) * (q->X[2] - q->X[0]);
+                const double gx = ((q->iz[1] - q->iz[0]) * (q->Y[2] - q->Y[0]) - (q->iz[2] - q->iz[0]) * (q->Y[1] - q->Y[0])) / ar;
+                const double gy = ((q->X[1] - q->X[0]) * (q->iz[2] - q->iz[0]) - (q->X[2] - q->X[0]) * (q->iz[1] - q->iz[0])) / ar;
+                q->giz = hypot(gx, gy);
+            }
+            ++nr;
+        }
+    }
+    const int64_t band = 8, nbands = ((int64_t)H + band - 1) / band;
+    #pragma omp parallel for schedule(dynamic, 1)
+    for (int64_t b = 0; b < nbands; ++b) {
+        const int64_t r0 = b * band, r1 = r0 + band < (int64_t)H ? r0 + band : (int64_t)H;
+        const int64_t np = (r1 - r0) * W;
+        /* per pixel: winner's iz, its lowest and highest iz within edge_eps, the highest iz within
+           edge_eps of any other covering triangle, of any ambiguous one */
+        double* best = (double*)malloc(sizeof(double) * 5 * (size_t)np);
+        int32_t* bp = (int32_t*)malloc(sizeof(int32_t) * (size_t)np);
+        for (int64_t i = 0; i < np; ++i) {
+            for (int j = 0; j < 5; ++j) best[5 * i + j] = -INFINITY;
+            bp[i] = -1;
+        }
+        for (int64_t k = 0; k < nr; ++k) {
+            const rtri* q = &rt[k];
+            if (q->y1 + edge_eps < r0 + 0.5 || q->y0 - edge_eps > r1 - 0.5) continue;
+            const double area = (q->X[1] - q->X[0]) * (q->Y[2] - q->Y[0]) - (q->Y[1] - q->Y[0]) * (q->X[2] - q->X[0]);
+            if (area == 0.0 || !(fabs(area) < INFINITY)) continue;
+            const double sg = area > 0 ? 1.0 : -1.0;
+            double len[3];
+            for (int e = 0; e < 3; ++e) {
+                const int e2 = (e + 1) % 3;
+                len[e] = hypot(q->X[e2] - q->X[e], q->Y[e2] - q->Y[e]);
+            }
+            int64_t ca = (int64_t)ceil(q->x0 - edge_eps - 0.5), cb = (int64_t)floor(q->x1 + edge_eps - 0.5);
+            int64_t ra = (int64_t)ceil(q->y0 - edge_eps - 0.5), rb = (int64_t)floor(q->y1 + edge_eps - 0.5);
+            if (ca < 0) ca = 0;
+            if (cb > (int64_t)W - 1) cb = (int64_t)W - 1;
+            if (ra < r0) ra = r0;
+            if (rb > r1 - 1) rb = r1 - 1;
+            for (int64_t row = ra; row <= rb; ++row) {
+                const double py = row + 0.5;
+                for (int64_t col = ca; col <= cb; ++col) {
+                    const double px = col + 0.5;
+                    double e[3], dmin = INFINITY;
+                    int beyond_diagonal = 0;
+                    for (int j = 0; j < 3; ++j) {
+                        const int j2 = (j + 1) % 3;
+                        e[j] = (q->X[j2] - q->X[j]) * (py - q->Y[j]) - (q->Y[j2] - q->Y[j]) * (px - q->X[j]);
+                        const double dist = sg * e[j] / len[j];
+                        if ((q->internal >> j) & 1) {
+                            /* a fan diagonal is no silhouette: the other half of the polygon covers
+                               the centre (both halves are the same primitive) */
+                            beyond_diagonal |= dist < -1e-9;
+                        } else {
+                            dmin = dist < dmin ? dist : dmin;
+                        }
+                    }
+                    if (beyond_diagonal || dmin <= -edge_eps) continue;
+                    /* barycentric of vertex k = edge function of the opposite edge / area */
+                    const double iz = (e[1] * q->iz[0] + e[2] * q->iz[1] + e[0] * q->iz[2]) / area;
+                    const double lo = iz - edge_eps * q->giz, hi = iz + edge_eps * q->giz;
+                    const int64_t i = (row - r0) * W + col;
+                    double* st = best + 5 * i;
+                    if (dmin < edge_eps && hi > st[4]) st[4] = hi;  /* coverage ambiguous */
+                    if (dmin < 0.0) continue;
+                    if (bp[i] < 0 || iz > st[0] || (iz == st[0] && q->prim < bp[i])) {
+                        if (bp[i] >= 0 && bp[i] != q->prim && st[2] > st[3]) st[3] = st[2];
+                        st[0] = iz;
+                        st[1] = lo;
+                        st[2] = hi;
+                        bp[i] = q->prim;
+                    } else if (q->prim != bp[i] && hi > st[3]) {
+                        st[3] = hi;
+                    }
+                }
+            }
+        }
+        for (int64_t i = 0; i < np; ++i) {
+            const double* st = best + 5 * i;
+            const double floor_iz = st[1] * (1.0 - depth_eps);
+            uint8_t fl = 0;
+            if (st[4] > -INFINITY && (bp[i] < 0 || st[4] >= floor_iz)) fl |= 1;
+            if (bp[i] >= 0 && st[3] > -INFINITY && st[3] >= floor_iz) fl |= 2;
+            prim_out[r0 * W + i] = bp[i];
+            flags_out[r0 * W + i] = fl;
+        }
+        free(best);
+        free(bp);
+    }
+    free(rt);
+}
