@@ -15,7 +15,7 @@
  *   - Lambert with L = normalize(1,1,1), ambient 0.2: tutorials/lesson09_texture_mapping.py:73
  *   - repeat texture addressing, row = v: rendering/_core.py:94-96
  *
- * Two layers:
+ * Layers:
  *  (1) "twin": the FP32 algorithm of the device code, operation for operation (explicit fmaf,
  *      no contraction: build with -ffp-contract=off), so device output can be compared
  *      bit-for-bit: LBVH build (Morton codes, stable sort, Karras hierarchy, refit), ray
@@ -25,11 +25,17 @@
  *      bound of the FP32 Moller-Trumbore operations: the winner lies within max(edge_eps, bound)
  *      barycentric of an edge, a runner-up is hit within max(tie_eps relative, t bounds) of the
  *      winner (depth tie), or a triangle is missed by less than its bound in that range.
+ *  (3) "raster": primary visibility by projection, near-plane clipping, screen-space edge
+ *      functions and a depth buffer in FP64 (tw_raster_ids), the reference raster's rule
+ *      (rendering/_raster.py:80-112, 172-214, 282-311), a non-ray statement of what a pixel sees.
+ *  (4) vertex normals (tw_vertex_normals) for Mesh.compute_normals (rendering/_modeling.py:48-49).
  *
  * Pinning (tests/test_oracle_*.py): Morton vs closed-form KATs; sort vs numpy stable argsort;
  * LBVH invariants (2n-1 nodes, each leaf once, boxes are exact unions) and brute-force equality;
  * FP32 traversal vs FP32 brute force; FP64 closest hit vs closed-form geometry (sphere/plane
- * analytic intersections, axis-aligned triangles) and vs the FP32 twin within 1e-5 relative t.
+ * analytic intersections, axis-aligned triangles) and vs the FP32 twin within 1e-5 relative t;
+ * raster vs the FP64 closest hit on unflagged pixels and a projection closed form; normals vs an
+ * independent numpy statement and geometric invariants.
  */
 #include <math.h>
 #include <stdint.h>
