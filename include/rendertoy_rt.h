@@ -333,6 +333,11 @@ int rt_mesh_pack(const float* pos, const float* nrm, const float* uv, uint32_t n
 int rt_mesh_unpack(const void* records, uint32_t n_vertices, float* pos, float* nrm, float* uv,
                    void* stream);
 
+/* ---- image export: rendering/_core.py:340 (RGBA render targets are BGRA UNORM_INT8 images) ----
+ * n_pixels float RGB triples (e.g. rt_render's output) -> 4-byte B, G, R, A pixels with the OpenCL
+ * UNORM_INT8 write conversion convert_uchar_sat_rte(f * 255) (NaN -> 0) and A = 255. */
+int rt_image_to_bgra8(const float* rgb, uint32_t n_pixels, uint8_t* bgra, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -72,6 +72,8 @@ def lib():
         L.tw_closest64_bvh.argtypes = [vp, i64, vp, vp, i32, i64] + [vp] * 6 + [f64, f64, f64, f64] + [vp] * 5
         L.tw_tex_sample.restype = None
         L.tw_tex_sample.argtypes = [vp, i32, i32, i64, vp, vp, i32, vp]
+        L.tw_rgb_to_bgra8.restype = None
+        L.tw_rgb_to_bgra8.argtypes = [vp, i64, vp]
         L.tw_raster_ids.restype = None
         L.tw_raster_ids.argtypes = [vp, i64, vp, ctypes.c_uint32, ctypes.c_uint32, f64, f64, vp, vp]
         L.tw_vertex_normals.restype = None
@@ -80,6 +82,14 @@ def lib():
         L.tw_set_threads.argtypes = [ctypes.c_int]
         _lib = L
     return _lib
+
+
+def rgb_to_bgra8(rgb):
+    """The twin of rt_image_to_bgra8: float RGB [..., 3] -> uint8 B, G, R, 255 [..., 4]."""
+    rgb = np.ascontiguousarray(rgb, np.float32)
+    out = np.empty(rgb.shape[:-1] + (4,), np.uint8)
+    lib().tw_rgb_to_bgra8(_p(rgb), rgb.size // 3, _p(out))
+    return out
 
 
 RASTER_EDGE, RASTER_DEPTH_TIE = 1, 2

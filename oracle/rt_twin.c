@@ -1688,3 +1688,22 @@ TW_API void tw_raster_ids(const float* tri_pos, int64_t n_tris, const float* cam
     }
     free(rt);
 }
+
+/* ---- image export: float RGB -> BGRA8 with the OpenCL UNORM_INT8 write conversion
+ * convert_uchar_sat_rte(f * 255) (rendering/_core.py:340; OpenCL 1.2 section 8.3.1.1), NaN -> 0,
+ * alpha 255. */
+static inline uint8_t unorm8(float f) {
+    const float x = f * 255.0f;
+    if (!(x > 0.0f)) return 0;
+    if (x >= 255.0f) return 255;
+    return (uint8_t)nearbyintf(x);  /* default rounding mode: ties to even */
+}
+
+TW_API void tw_rgb_to_bgra8(const float* rgb, int64_t n, uint8_t* out) {
+    for (int64_t i = 0; i < n; ++i) {
+        out[4 * i] = unorm8(rgb[3 * i + 2]);
+        out[4 * i + 1] = unorm8(rgb[3 * i + 1]);
+        out[4 * i + 2] = unorm8(rgb[3 * i]);
+        out[4 * i + 3] = 255;
+    }
+}

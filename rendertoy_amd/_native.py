@@ -226,6 +226,7 @@ SIGNATURES = [
       ctypes.c_uint64, ctypes.c_void_p]),
     ("rt_mesh_pack", ctypes.c_int, [ctypes.c_void_p] * 3 + [ctypes.c_uint32, ctypes.c_void_p, ctypes.c_void_p]),
     ("rt_mesh_unpack", ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32] + [ctypes.c_void_p] * 4),
+    ("rt_image_to_bgra8", ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_void_p]),
 ]
 
 _lib = None

@@ -10,7 +10,8 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "librtx_hip.so")
-SOURCES = ["rt_api.hip", "rt_build.hip", "rt_trace.hip", "rt_paths.hip", "rt_mesh.hip"]
+SOURCES = ["rt_api.hip", "rt_build.hip", "rt_trace.hip", "rt_paths.hip", "rt_mesh.hip",
+           "rt_image.hip"]
 ARCH = "gfx950"
 
 # -ffp-contract=off + correctly rounded div/sqrt: every FP32 result of the kernels is a fixed
