@@ -15,6 +15,7 @@ from ._core import (float2, float3, float4, int2, int3, int4, uint2, uint3, uint
 from ._modeling import Mesh, WeldMode, SubdivisionMode, MeshVertex, mesh_from_arrays
 from ._raycaster import Raycaster, BVH_AABB, BVH_Triangle, Ray, Hit, Camera, compute_normals, \
     mesh_from_tensors, mesh_tensors
+from ._loaders import load_obj
 from .tuning import get_tuning, set_tuning, tuning
 
 __all__ = [n for n in dir() if not n.startswith("_")]

@@ -14,6 +14,8 @@ import math
 
 import numpy as np
 
+from ._loaders import normalize_unit_cube  # noqa: F401  (the reference's unit-cube rule)
+
 _PHI = (1.0 + math.sqrt(5.0)) / 2.0
 
 _ICO_V = np.array([
@@ -93,15 +95,6 @@ def geodesic_sphere(f: int):
     return pos, idx
 
 
-def normalize_unit_cube(p):
-    """rendering/_loaders.py:34-38: scalar v_min/v_max over all coordinates, then
-    (p - v_min)/size - size*0.5/size, in float32."""
-    p = p.astype(np.float32)
-    v_min = p.min()
-    v_max = p.max()
-    v_size = v_max - v_min
-    max_dim = v_size.max()
-    return ((p - v_min) / max_dim - v_size * np.float32(0.5) / max_dim).astype(np.float32)
 
 
 def vertex_normals(pos, idx):
