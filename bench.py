@@ -407,6 +407,11 @@ def main():
         b.record(stream)
     torch.cuda.synchronize()
     render_ms = [a.elapsed_time(b) for a, b in rev]
+    # export of the frame to the reference's BGRA8 render-target format (12 B read, 4 B written per px)
+    from rendertoy_amd import image as rt_image
+    from rendertoy_amd.batch import time_launches
+    bgra = rt_image.to_bgra8(img)
+    bgra_ms = float(np.median(time_launches(lambda: rt_image.to_bgra8(img, out=bgra), 20)))
     st = stats.cpu().numpy()
     render_rays = n_rays + int(st[1])
 
@@ -439,6 +444,8 @@ def main():
             "rays_per_frame": render_rays,
             "shadow_rays": int(st[1]),
             "mrays_per_s_p50": round(render_rays / (percentile(render_ms, 50) * 1e-3) / 1e6, 2),
+            "bgra8_export_ms": round(bgra_ms, 4),
+            "bgra8_export_gbs": round(16 * n_rays / (bgra_ms * 1e-3) / 1e9, 1),
         },
         "wall_s_timed": round(wall, 4),
     }
