@@ -176,7 +176,10 @@ typedef struct rt_tuning {
 /* Hierarchy builders of rt_scene_build; both start from the same Morton order and give the same
  * hits. The PLOC tree (locally ordered clustering by surface area) needs ~30% fewer traversal
  * steps per packet and ~17% fewer instructions than the LBVH radix tree, and takes ~2.6x longer
- * to build; frames whose time is set by their slowest tiles gain little from it. */
+ * to build; frames whose time is set by their slowest tiles gain little from it. A PLOC tree with
+ * a triangle more than 62 internal levels deep (the depth the traversal stacks are sized for, and
+ * the bound of the radix tree) is replaced by the LBVH; checking this makes a PLOC build wait for
+ * its stream once. */
 enum {
     RT_BUILD_LBVH = 0, /* Karras radix tree over the Morton order, leaves of <= 4 consecutive slots */
     RT_BUILD_PLOC = 1  /* PLOC (Meister & Bittner 2018), search radius 16, leaves of <= 4 triangles */

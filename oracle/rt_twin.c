@@ -131,6 +131,7 @@ static int cmp_u64(const void* a, const void* b) {
 #define TW_BUILD_LBVH 0
 #define TW_BUILD_PLOC 1
 static int ploc_hierarchy(int64_t n, const box3* leafbox, uint32_t* order, float* nodes, float* bounds);
+#define TW_MAX_TREE_DEPTH 62  /* internal levels the device's traversal stacks are sized for */
 
 TW_API int tw_build(int64_t n, const float* pos, const float* nrm, const float* uv,
                     const int32_t* mesh_of, const uint8_t* has_nrm, const uint8_t* has_uv,
@@ -230,9 +231,13 @@ TW_API int tw_build(int64_t n, const float* pos, const float* nrm, const float* 
         return 0;
     }
 
+    int lbvh = 1;
     if (builder == TW_BUILD_PLOC) {
-        if (ploc_hierarchy(n, leafbox, order, nodes, bounds) != 0) return -1;
-    } else {
+        const int rc = ploc_hierarchy(n, leafbox, order, nodes, bounds);
+        if (rc < 0) return -1;
+        lbvh = rc == 1;  /* too deep: the LBVH of the same keys, as the device does */
+    }
+    if (lbvh) {
     /* Karras 2012 binary radix tree over the 62-bit extended keys (code, slot) */
     int32_t* left = (int32_t*)malloc(sizeof(int32_t) * (n - 1));
     int32_t* right = (int32_t*)malloc(sizeof(int32_t) * (n - 1));
@@ -470,6 +475,18 @@ static int ploc_hierarchy(int64_t n, const box3* leafbox, uint32_t* order, float
         m = k;
     }
     parent[0] = -1;
+    /* the device's depth guard: a tree with a triangle under more than TW_MAX_TREE_DEPTH internal
+       nodes is not used (the caller builds the LBVH instead); `order` is still the Morton order */
+    for (int64_t p = 0; p < n; ++p) {
+        int32_t x = parent[n - 1 + p];
+        int d = 0;
+        while (x >= 0 && d <= TW_MAX_TREE_DEPTH) { ++d; x = parent[x]; }
+        if (d > TW_MAX_TREE_DEPTH) {
+            free(C); free(C2); free(NN); free(B); free(B2); free(left); free(right); free(size);
+            free(parent); free(pos_of); free(morder); free(slot_of); free(fslot);
+            return 1;
+        }
+    }
     for (int c = 0; c < 3; ++c) { bounds[c] = B[0].lo[c]; bounds[4 + c] = B[0].hi[c]; }
     bounds[3] = bounds[7] = 0.0f;
     /* leaf roots in the order of their smallest Morton position; slots left to right */

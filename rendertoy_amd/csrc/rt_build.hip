@@ -65,6 +65,7 @@ struct Workspace {
     uint32_t* scan;    // n: leaf-root sizes at their first Morton position, then their first slot
     uint32_t* part;    // 2 * chunks: per-chunk counts, then exclusive offsets
     int32_t* state;    // 8: PlocState per iteration parity
+    uint32_t* depth;   // 1: internal ancestors of the deepest triangle of the PLOC tree
     uint32_t* slot_order;  // n: primitive of each slot
     uint64_t bytes;
 };
@@ -115,6 +116,7 @@ Workspace carve(char* base, uint64_t n) {
     w.scan = (uint32_t*)take(4 * n);
     w.part = (uint32_t*)take(8 * nchunk);
     w.state = (int32_t*)take(4 * 8);
+    w.depth = (uint32_t*)take(4);
     w.slot_order = (uint32_t*)take(4 * n);
     w.bytes = off;
     return w;
@@ -552,6 +554,25 @@ __global__ void __launch_bounds__(1024) k_refit_top(const uint32_t* __restrict__
 }
 
 __global__ void k_zero_word(uint32_t* w) { *w = 0u; }
+
+// Traversals keep pending nodes on stacks sized for trees at most kMaxTreeDepth internal levels
+// deep (the Karras tree over 62-bit keys has at most 62). A PLOC tree has no such bound (a chain of
+// clusters can nest deeper), so its depth is measured: per triangle, the internal nodes above it
+// (leaves of the parent array at ni + primitive), counted up to kMaxTreeDepth + 1.
+constexpr uint32_t kMaxTreeDepth = 62;
+
+__global__ void __launch_bounds__(256) k_ploc_depth(uint32_t n, const int32_t* __restrict__ parent,
+                                                     uint32_t* __restrict__ depth) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int32_t x = parent[(uint64_t)(n - 1) + i];
+    uint32_t d = 0;
+    while (x >= 0 && d <= kMaxTreeDepth) {
+        ++d;
+        x = parent[x];
+    }
+    if (d > kMaxTreeDepth) atomicMax(depth, d);
+}
 
 // ---- PLOC builder (parallel locally-ordered clustering, Meister & Bittner 2018) ----
 // Clusters start as the triangles in Morton order (positions 0..n-1, refs ~prim, padded leaf
@@ -1243,13 +1264,14 @@ extern "C" int rt_scene_build(const rt_mesh_desc* meshes, uint32_t n_meshes, rt_
     sort_pairs(ki, vi, w.keys1, w.vals1, n, 30, w.hist, w.totals, s);
     // after 4 passes the sorted arrays are back in keys0/vals0
     float4* nodes = (float4*)sc->nodes;
-    if (n > 1 && current_tuning().builder == RT_BUILD_PLOC) {
+    bool ploc = n > 1 && current_tuning().builder == RT_BUILD_PLOC;
+    const uint32_t nchunk = blocks_for(n, kPlocChunk);
+    if (ploc) {
         // 5. PLOC hierarchy (node boxes as it merges, scene bounds at the root), 6. leaf slots and
         // traversal refs, 7. leaf records in slot order, 8. 4-wide records
         hipLaunchKernelGGL(k_ploc_init, dim3(blocks_for(n, 256)), dim3(256), 0, s, vi, n, w.leafbox,
                            w.cid[0], w.cbox[0], w.pos_of, w.parent, w.state);
         const int iters = ploc_iterations(n);
-        const uint32_t nchunk = blocks_for(n, kPlocChunk);
         const uint32_t gnn = std::min<uint32_t>(kPlocGrid, blocks_for(n, kPlocTile));
         const uint32_t gch = std::min<uint32_t>(kPlocGrid, nchunk);
         for (int it = 0; it < iters; ++it) {
@@ -1264,6 +1286,17 @@ extern "C" int rt_scene_build(const rt_mesh_desc* meshes, uint32_t n_meshes, rt_
         hipLaunchKernelGGL(k_ploc_tail, dim3(1), dim3(1024), 0, s, iters & 1, n, w.state, w.nn, w.cid[0],
                            w.cid[1], w.cbox[0], w.cbox[1], w.nleft, w.nright, w.nsize, w.parent, nodes,
                            sc->bounds);
+        // a tree deeper than the traversal stacks allow is replaced by the LBVH of the same keys;
+        // this reads one word back, so a PLOC build waits for its stream here
+        hipLaunchKernelGGL(k_zero_word, dim3(1), dim3(1), 0, s, w.depth);
+        hipLaunchKernelGGL(k_ploc_depth, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, w.parent, w.depth);
+        uint32_t deep = 0;
+        int e = check_hip(hipMemcpyAsync(&deep, w.depth, 4, hipMemcpyDeviceToHost, s), "rt_scene_build(PLOC depth)");
+        if (!e) e = check_hip(hipStreamSynchronize(s), "rt_scene_build(PLOC depth)");
+        if (e) return e;
+        ploc = deep == 0;
+    }
+    if (ploc) {
         hipLaunchKernelGGL(k_ploc_leafroots, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, vi, w.parent,
                            w.nsize, w.nleft, w.nright, w.pos_of, w.scan);
         hipLaunchKernelGGL(k_scan_chunks, dim3(nchunk), dim3(256), 0, s, n, w.scan, w.part);
@@ -1280,7 +1313,7 @@ extern "C" int rt_scene_build(const rt_mesh_desc* meshes, uint32_t n_meshes, rt_
         sc->root = n <= (uint32_t)kLeafMax ? leaf_ref(0, n) : 0;
         vi = w.slot_order;
     } else {
-    // 5. hierarchy, 6. emit, 7. refit
+    // 5. hierarchy, 6. emit, 7. refit (LBVH, or a PLOC tree found too deep)
     if (n > 1) {
         hipLaunchKernelGGL(k_zero_word, dim3(1), dim3(1), 0, s, w.top_count);
         hipLaunchKernelGGL(k_hierarchy, dim3(blocks_for(n - 1, 256)), dim3(256), 0, s, ki, n, nodes,

@@ -307,6 +307,39 @@ def test_ploc_build_matches_twin_bytes_and_lbvh_hits(ren, scene):
     np.testing.assert_array_equal(gpu_trace(rc, o, d)["prim"], gpu_trace(lb, o, d)["prim"])
 
 
+def _chain_scene(n, base=1.25):
+    """n thin triangles at exponentially spaced x: PLOC merges them as a chain (depth ~ n)."""
+    x = base ** np.arange(n)
+    x = x / x.max() * 4
+    z, s = np.zeros(n), np.full(n, 1e-3)
+    tri = np.stack([np.stack([x, z, z], 1), np.stack([x, s, z], 1), np.stack([x, z, s], 1)], 1)
+    return [{"pos": tri.astype(np.float32).reshape(-1, 3), "idx": np.arange(3 * n, dtype=np.int32).reshape(-1, 3),
+             "nrm": None, "uv": None}]
+
+
+@pytest.mark.parametrize("n", [60, 200])
+def test_ploc_depth_guard(ren, n):
+    """A PLOC tree deeper than the 62 internal levels the traversal stacks are sized for (200
+    chained triangles: 112) is replaced by the LBVH; a shallower chain (60) stays PLOC. Device ==
+    twin bytes either way, and rays along the chain hit what brute force hits."""
+    meshes = _chain_scene(n)
+    tw = twin_from_meshes(meshes, builder="ploc")
+    with ren.tuning(builder="ploc"):
+        rc = gpu_from_meshes(meshes)
+    torch.cuda.synchronize()
+    assert_build_equal(rc, tw)
+    lb = twin_from_meshes(meshes, builder="lbvh")
+    same = np.array_equal(_u32(tw.nodes4), _u32(lb.nodes4))
+    assert same == (n == 200)
+    k = 256
+    o = np.stack([np.linspace(-1, 4.5, k), np.full(k, 2e-4), np.full(k, 2e-4)], 1).astype(np.float32)
+    d = np.tile(np.float32([1, 0, 0]), (k, 1))
+    o = np.concatenate([o, o * np.float32([1, 1, 1]) + np.float32([0, 6e-4, 6e-4])])
+    d = np.concatenate([d, -d])
+    _, bp = tw.brute_closest32(o, d)
+    np.testing.assert_array_equal(gpu_trace(rc, o, d)["prim"], bp)
+
+
 def test_ploc_tiny_and_multi_mesh_scenes(ren):
     """PLOC on 2..9 triangles (root a leaf up to 4), duplicated triangles (equal boxes and Morton
     keys) and a multi-mesh scene with an empty mesh: device == twin bytes, hits == LBVH's."""

@@ -40,7 +40,12 @@ def area(alo, ahi, blo, bhi):
     return f32(f32(f32(d[0] * d[1]) + f32(d[1] * d[2])) + f32(d[2] * d[0]))
 
 
-def ploc_reference(pos, morton_order):
+def ploc_chain_depth(pos, morton_order):
+    """Internal nodes above the deepest triangle of the restatement's PLOC tree."""
+    return ploc_reference(pos, morton_order, depth_only=True)
+
+
+def ploc_reference(pos, morton_order, depth_only=False):
     n = len(morton_order)
     blo, bhi = padded_boxes(pos)
     C = [("t", int(p)) for p in morton_order]
@@ -83,6 +88,14 @@ def ploc_reference(pos, morton_order):
                 B2.append(B[i])
         C, B = C2, B2
     assert created == n - 1 and C[0] == ("n", 0)
+    if depth_only:
+        best = 0
+        for p in morton_order:
+            x, d = ("t", int(p)), 0
+            while x in parent:
+                x, d = ("n", parent[x]), d + 1
+            best = max(best, d)
+        return best
 
     def sz(c):
         return 1 if c[0] == "t" else size[c[1]]
@@ -214,3 +227,30 @@ def test_ploc_tree_traversal_equals_brute_force_and_lbvh():
     _, bprim = ploc.brute_closest32(o, dr)
     np.testing.assert_array_equal(a["prim"], bprim)
     assert (a["prim"] >= 0).sum() > 500
+
+
+def _chain(n, base=1.25):
+    x = base ** np.arange(n)
+    x = x / x.max() * 4
+    z, s = np.zeros(n), np.full(n, 1e-3)
+    tri = np.stack([np.stack([x, z, z], 1), np.stack([x, s, z], 1), np.stack([x, z, s], 1)], 1)
+    return tri.astype(f32).reshape(-1, 3), np.arange(3 * n, dtype=np.int32).reshape(-1, 3)
+
+
+@pytest.mark.parametrize("n,deep", [(60, False), (200, True)])
+def test_ploc_depth_guard_falls_back_to_lbvh(n, deep):
+    """A PLOC tree with a triangle under more than 62 internal nodes (counted on the restatement's
+    tree: 112 for 200 chained triangles) is replaced by the LBVH of the same keys, the traversal
+    stacks being sized for 62 levels; a shallower chain keeps the PLOC tree and matches the
+    restatement."""
+    pos, idx = _chain(n)
+    ploc = twin_from_meshes([{"pos": pos, "idx": idx, "nrm": None, "uv": None}], builder="ploc")
+    lbvh = twin_from_meshes([{"pos": pos, "idx": idx, "nrm": None, "uv": None}], builder="lbvh")
+    depth = ploc_chain_depth(pos, lbvh.order)
+    assert (depth > 62) == deep
+    if deep:
+        np.testing.assert_array_equal(ploc.nodes.view(np.uint32), lbvh.nodes.view(np.uint32))
+        np.testing.assert_array_equal(ploc.nodes4.view(np.uint32), lbvh.nodes4.view(np.uint32))
+        np.testing.assert_array_equal(ploc.order, lbvh.order)
+    else:
+        check_ploc(ploc)
