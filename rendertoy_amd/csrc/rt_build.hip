@@ -66,11 +66,26 @@ struct Workspace {
     uint32_t* part;    // 2 * chunks: per-chunk counts, then exclusive offsets
     int32_t* state;    // 8: PlocState per iteration parity
     uint32_t* depth;   // 1: internal ancestors of the deepest triangle of the PLOC tree
+    float4* pre;       // 2n: union of the leaf boxes from the start of the slot's refit block to it
+    float4* suf;       // 2n: union from the slot to the end of its refit block
+    float4* sparse;    // 2 * nb * levels: unions of 2^k consecutive refit blocks
     uint32_t* slot_order;  // n: primitive of each slot
     uint64_t bytes;
 };
 
 inline uint64_t align256(uint64_t x) { return (x + 255) & ~uint64_t(255); }
+
+// Refit blocks: bottom-up refit inside each range of kRefitBlock leaves runs in one workgroup
+// (k_refit_local); the nodes whose range spans several blocks ("top" nodes) are refit afterwards
+// from range unions (k_block_scan, k_block_sparse, k_refit_top).
+constexpr int kRefitBlock = 2048;
+
+// levels of the sparse table over nb blocks: level k holds unions of 2^k consecutive blocks
+inline uint32_t sparse_levels(uint64_t nb) {
+    uint32_t l = 1;
+    while ((1ull << l) <= nb) ++l;
+    return l;
+}
 
 Workspace carve(char* base, uint64_t n) {
     Workspace w{};
@@ -117,6 +132,12 @@ Workspace carve(char* base, uint64_t n) {
     w.part = (uint32_t*)take(8 * nchunk);
     w.state = (int32_t*)take(4 * 8);
     w.depth = (uint32_t*)take(4);
+    w.pre = (float4*)take(32 * n);
+    w.suf = (float4*)take(32 * n);
+    {
+        const uint64_t nb = (n + kRefitBlock - 1) / kRefitBlock;
+        w.sparse = (float4*)take(32 * nb * sparse_levels(nb));
+    }
     w.slot_order = (uint32_t*)take(4 * n);
     w.bytes = off;
     return w;
@@ -303,9 +324,6 @@ __global__ void __launch_bounds__(256) k_morton(const float4* __restrict__ cent,
 
 // ---- hierarchy ----
 
-// Refit blocks: bottom-up refit inside each range of kRefitBlock leaves runs in one workgroup
-// (k_refit_local); the nodes whose range spans several blocks ("top" nodes) are refit afterwards.
-constexpr int kRefitBlock = 2048;
 
 RT_DEV int delta(const uint32_t* __restrict__ k, int64_t n, int64_t i, int64_t j) {
     if (j < 0 || j >= n) return -1;
@@ -505,51 +523,193 @@ RT_DEV void node_box(const float4* __restrict__ leafbox, const uint32_t* __restr
     }
 }
 
-// Phase 2, in one workgroup, over the list of top nodes written by k_hierarchy: their local
-// children are complete (phase 1); a node with two local children is computed at once, one with a
-// single local child counts that child's arrival on the node's counter. The second arrival at a
-// node computes it and continues upward (every ancestor of a top node is a top node). All counters
-// are touched by the waves of this one workgroup on one CU, so workgroup-scope atomics order the
-// box stores; an agent-scope release would write the L2 back on every step of the chain, the L2s
-// of the 8 XCDs not being coherent with each other.
-__global__ void __launch_bounds__(1024) k_refit_top(const uint32_t* __restrict__ order, uint32_t n,
-                                                      const float4* __restrict__ leafbox,
-                                                      const int32_t* __restrict__ parent,
-                                                      const int32_t* __restrict__ rangej,
-                                                      const int32_t* __restrict__ split,
-                                                      uint32_t* __restrict__ flags,
-                                                      float4* __restrict__ nodes, float4* nodebox,
-                                                      float* __restrict__ bounds,
-                                                      const int32_t* __restrict__ top_list,
-                                                      const uint32_t* __restrict__ top_count) {
-    const uint32_t cnt = *top_count;
-    for (uint32_t t = threadIdx.x; t < cnt; t += blockDim.x) {
-        int32_t node = top_list[t];
-        {
-            int32_t l, r;
-            binary_children(rangej, split, node, l, r);
-            const int n_local = (int)is_local(rangej, l) + (int)is_local(rangej, r);
-            if (n_local == 0) continue;
-            if (n_local == 1) {
-                const uint32_t prev = __hip_atomic_fetch_add(&flags[node], 1u, __ATOMIC_ACQ_REL,
-                                                             __HIP_MEMORY_SCOPE_WORKGROUP);
-                if (prev == 0) continue;
+// Phase 2 prepares range unions: per refit block, the prefix and suffix unions of its leaf boxes in
+// slot order (a workgroup scan, 8 slots per thread) and the block's total, level 0 of the sparse
+// table.
+__global__ void __launch_bounds__(256) k_block_scan(const uint32_t* __restrict__ order, uint32_t n,
+                                                     const float4* __restrict__ leafbox,
+                                                     float4* __restrict__ pre, float4* __restrict__ suf,
+                                                     float4* __restrict__ sparse) {
+    constexpr int kPer = kRefitBlock / 256;
+    __shared__ float tlo[3][256], thi[3][256];
+    const uint32_t t = threadIdx.x;
+    const uint64_t s0 = (uint64_t)blockIdx.x * kRefitBlock + (uint64_t)t * kPer;
+    float lo[kPer][3], hi[kPer][3];
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+        const uint64_t sl = s0 + k;
+        if (sl < n) {
+            const uint64_t p = order[sl];
+            const float4 a = leafbox[2 * p], b = leafbox[2 * p + 1];
+            lo[k][0] = a.x; lo[k][1] = a.y; lo[k][2] = a.z;
+            hi[k][0] = b.x; hi[k][1] = b.y; hi[k][2] = b.z;
+        } else {
+            for (int c = 0; c < 3; ++c) {
+                lo[k][c] = INFINITY;
+                hi[k][c] = -INFINITY;
             }
         }
-        for (;;) {
-            int32_t l, r;
-            binary_children(rangej, split, node, l, r);
-            float l_lo[3], l_hi[3], r_lo[3], r_hi[3], u_lo[3], u_hi[3];
-            node_box(leafbox, order, nodebox, l, l_lo, l_hi);
-            node_box(leafbox, order, nodebox, r, r_lo, r_hi);
-            store_node(nodes, nodebox, bounds, node, l_lo, l_hi, r_lo, r_hi, u_lo, u_hi);
-            const int32_t p = parent[node];
-            if (p < 0) break;
-            const uint32_t prev = __hip_atomic_fetch_add(&flags[p], 1u, __ATOMIC_ACQ_REL,
-                                                         __HIP_MEMORY_SCOPE_WORKGROUP);
-            if (prev == 0) break;
-            node = p;
+    }
+    // exclusive prefix union of the thread totals, then inclusive per slot
+    float tl[3], th[3];
+    for (int c = 0; c < 3; ++c) {
+        tl[c] = INFINITY;
+        th[c] = -INFINITY;
+        for (int k = 0; k < kPer; ++k) {
+            tl[c] = fminf(tl[c], lo[k][c]);
+            th[c] = fmaxf(th[c], hi[k][c]);
         }
+        tlo[c][t] = tl[c];
+        thi[c][t] = th[c];
+    }
+    __syncthreads();
+    for (int off = 1; off < 256; off <<= 1) {  // inclusive scan of the totals
+        float xl[3], xh[3];
+        for (int c = 0; c < 3; ++c) {
+            xl[c] = t >= (uint32_t)off ? tlo[c][t - off] : INFINITY;
+            xh[c] = t >= (uint32_t)off ? thi[c][t - off] : -INFINITY;
+        }
+        __syncthreads();
+        for (int c = 0; c < 3; ++c) {
+            tlo[c][t] = fminf(tlo[c][t], xl[c]);
+            thi[c][t] = fmaxf(thi[c][t], xh[c]);
+        }
+        __syncthreads();
+    }
+    float rl[3], rh[3];
+    for (int c = 0; c < 3; ++c) {
+        rl[c] = t > 0 ? tlo[c][t - 1] : INFINITY;
+        rh[c] = t > 0 ? thi[c][t - 1] : -INFINITY;
+    }
+    if (t == 255) {
+        sparse[2 * (uint64_t)blockIdx.x] = make_float4(tlo[0][255], tlo[1][255], tlo[2][255], 0.0f);
+        sparse[2 * (uint64_t)blockIdx.x + 1] = make_float4(thi[0][255], thi[1][255], thi[2][255], 0.0f);
+    }
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+        for (int c = 0; c < 3; ++c) {
+            rl[c] = fminf(rl[c], lo[k][c]);
+            rh[c] = fmaxf(rh[c], hi[k][c]);
+        }
+        if (s0 + k < n) {
+            pre[2 * (s0 + k)] = make_float4(rl[0], rl[1], rl[2], 0.0f);
+            pre[2 * (s0 + k) + 1] = make_float4(rh[0], rh[1], rh[2], 0.0f);
+        }
+    }
+    __syncthreads();
+    // suffix: the same over the reversed thread order
+    for (int c = 0; c < 3; ++c) {
+        tlo[c][t] = tl[c];
+        thi[c][t] = th[c];
+    }
+    __syncthreads();
+    for (int off = 1; off < 256; off <<= 1) {
+        float xl[3], xh[3];
+        for (int c = 0; c < 3; ++c) {
+            xl[c] = t + off < 256u ? tlo[c][t + off] : INFINITY;
+            xh[c] = t + off < 256u ? thi[c][t + off] : -INFINITY;
+        }
+        __syncthreads();
+        for (int c = 0; c < 3; ++c) {
+            tlo[c][t] = fminf(tlo[c][t], xl[c]);
+            thi[c][t] = fmaxf(thi[c][t], xh[c]);
+        }
+        __syncthreads();
+    }
+    for (int c = 0; c < 3; ++c) {
+        rl[c] = t < 255 ? tlo[c][t + 1] : INFINITY;
+        rh[c] = t < 255 ? thi[c][t + 1] : -INFINITY;
+    }
+#pragma unroll
+    for (int k = kPer - 1; k >= 0; --k) {
+        for (int c = 0; c < 3; ++c) {
+            rl[c] = fminf(rl[c], lo[k][c]);
+            rh[c] = fmaxf(rh[c], hi[k][c]);
+        }
+        if (s0 + k < n) {
+            suf[2 * (s0 + k)] = make_float4(rl[0], rl[1], rl[2], 0.0f);
+            suf[2 * (s0 + k) + 1] = make_float4(rh[0], rh[1], rh[2], 0.0f);
+        }
+    }
+}
+
+// Sparse table levels 1.. in one workgroup: level k, block b = union of levels k-1 at b and
+// b + 2^(k-1) (the workgroup's barrier orders the levels' global stores and loads).
+__global__ void __launch_bounds__(1024) k_block_sparse(float4* __restrict__ sparse, uint32_t nb,
+                                                        uint32_t levels) {
+    for (uint32_t k = 1; k < levels; ++k) {
+        const uint32_t half = 1u << (k - 1), cnt = nb - (2u * half) + 1u;
+        const float4* src = sparse + 2 * (uint64_t)(k - 1) * nb;
+        float4* dst = sparse + 2 * (uint64_t)k * nb;
+        for (uint32_t b = threadIdx.x; b < cnt; b += blockDim.x) {
+            const float4 a0 = src[2 * (uint64_t)b], a1 = src[2 * (uint64_t)b + 1];
+            const float4 b0 = src[2 * (uint64_t)(b + half)], b1 = src[2 * (uint64_t)(b + half) + 1];
+            dst[2 * (uint64_t)b] = make_float4(fminf(a0.x, b0.x), fminf(a0.y, b0.y), fminf(a0.z, b0.z), 0.0f);
+            dst[2 * (uint64_t)b + 1] = make_float4(fmaxf(a1.x, b1.x), fmaxf(a1.y, b1.y), fmaxf(a1.z, b1.z), 0.0f);
+        }
+        __syncthreads();
+    }
+}
+
+// Union of the leaf boxes of slots [lo, hi], which span at least two refit blocks: the suffix of
+// lo's block, the prefix of hi's block and the blocks between (two overlapping sparse-table
+// entries; min/max are exact, so overlap and order do not change the result).
+RT_DEV void range_box(const float4* __restrict__ pre, const float4* __restrict__ suf,
+                      const float4* __restrict__ sparse, uint32_t nb, int32_t lo, int32_t hi,
+                      float* blo, float* bhi) {
+    const float4 s0 = suf[2 * (uint64_t)lo], s1 = suf[2 * (uint64_t)lo + 1];
+    const float4 p0 = pre[2 * (uint64_t)hi], p1 = pre[2 * (uint64_t)hi + 1];
+    blo[0] = fminf(s0.x, p0.x); blo[1] = fminf(s0.y, p0.y); blo[2] = fminf(s0.z, p0.z);
+    bhi[0] = fmaxf(s1.x, p1.x); bhi[1] = fmaxf(s1.y, p1.y); bhi[2] = fmaxf(s1.z, p1.z);
+    const int32_t b0 = lo / kRefitBlock + 1, b1 = hi / kRefitBlock - 1;
+    if (b0 <= b1) {
+        const uint32_t len = (uint32_t)(b1 - b0 + 1);
+        const uint32_t k = 31u - __builtin_clz(len);
+        const float4* lv = sparse + 2 * (uint64_t)k * nb;
+        const uint64_t x = (uint64_t)b0, y = (uint64_t)(b1 - (int32_t)(1u << k) + 1);
+        const float4 a0 = lv[2 * x], a1 = lv[2 * x + 1], c0 = lv[2 * y], c1 = lv[2 * y + 1];
+        blo[0] = fminf(blo[0], fminf(a0.x, c0.x));
+        blo[1] = fminf(blo[1], fminf(a0.y, c0.y));
+        blo[2] = fminf(blo[2], fminf(a0.z, c0.z));
+        bhi[0] = fmaxf(bhi[0], fmaxf(a1.x, c1.x));
+        bhi[1] = fmaxf(bhi[1], fmaxf(a1.y, c1.y));
+        bhi[2] = fmaxf(bhi[2], fmaxf(a1.z, c1.z));
+    }
+}
+
+// Phase 3, one thread per top node (the list written by k_hierarchy): each child's box is its
+// range union (a top child), or the box phase 1 computed (a local child or a leaf). No node waits on
+// another.
+__global__ void __launch_bounds__(256) k_refit_top(const uint32_t* __restrict__ order, uint32_t n,
+                                                    const float4* __restrict__ leafbox,
+                                                    const int32_t* __restrict__ rangej,
+                                                    const int32_t* __restrict__ split,
+                                                    float4* __restrict__ nodes, float4* nodebox,
+                                                    float* __restrict__ bounds,
+                                                    const int32_t* __restrict__ top_list,
+                                                    const uint32_t* __restrict__ top_count,
+                                                    const float4* __restrict__ pre,
+                                                    const float4* __restrict__ suf,
+                                                    const float4* __restrict__ sparse, uint32_t nb) {
+    const uint32_t cnt = *top_count;
+    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < cnt; t += gridDim.x * blockDim.x) {
+        const int32_t node = top_list[t];
+        int32_t l, r;
+        binary_children(rangej, split, node, l, r);
+        float l_lo[3], l_hi[3], r_lo[3], r_hi[3], u_lo[3], u_hi[3];
+        if (is_local(rangej, l)) {
+            node_box(leafbox, order, nodebox, l, l_lo, l_hi);
+        } else {
+            const int32_t j = rangej[l];
+            range_box(pre, suf, sparse, nb, l < j ? l : j, l < j ? j : l, l_lo, l_hi);
+        }
+        if (is_local(rangej, r)) {
+            node_box(leafbox, order, nodebox, r, r_lo, r_hi);
+        } else {
+            const int32_t j = rangej[r];
+            range_box(pre, suf, sparse, nb, r < j ? r : j, r < j ? j : r, r_lo, r_hi);
+        }
+        store_node(nodes, nodebox, bounds, node, l_lo, l_hi, r_lo, r_hi, u_lo, u_hi);
     }
 }
 
@@ -1325,9 +1485,13 @@ extern "C" int rt_scene_build(const rt_mesh_desc* meshes, uint32_t n_meshes, rt_
         hipLaunchKernelGGL(k_refit_local, dim3(blocks_for(n, kRefitBlock)), dim3(256), 0, s, vi, n,
                            w.leafbox, w.parent, w.rangej, w.split, w.flags, nodes, w.nodebox,
                            sc->bounds);
-        hipLaunchKernelGGL(k_refit_top, dim3(1), dim3(1024), 0, s, vi, n, w.leafbox, w.parent,
-                           w.rangej, w.split, w.flags, nodes, w.nodebox, sc->bounds, w.top_list,
-                           w.top_count);
+        const uint32_t nb = blocks_for(n, kRefitBlock);
+        hipLaunchKernelGGL(k_block_scan, dim3(nb), dim3(256), 0, s, vi, n, w.leafbox, w.pre, w.suf,
+                           w.sparse);
+        hipLaunchKernelGGL(k_block_sparse, dim3(1), dim3(1024), 0, s, w.sparse, nb, sparse_levels(nb));
+        hipLaunchKernelGGL(k_refit_top, dim3(std::min<uint32_t>(blocks_for(n, 256), 1024)), dim3(256), 0, s,
+                           vi, n, w.leafbox, w.rangej, w.split, nodes, w.nodebox, sc->bounds, w.top_list,
+                           w.top_count, w.pre, w.suf, w.sparse, nb);
         hipLaunchKernelGGL(k_wide, dim3(blocks_for(n - 1, 256)), dim3(256), 0, s, n,
                            (const float4*)nodes, (float4*)sc->nodes4);
         sc->root = n <= (uint32_t)kLeafMax ? leaf_ref(0, n) : 0;
