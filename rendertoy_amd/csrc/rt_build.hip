@@ -1,7 +1,7 @@
 // rt_build.hip — on-device LBVH construction for rendering/_raycaster.py:26-33
 // (Raycaster.__init__ -> _build_ads, a stub in the reference that only counts triangles).
 //
-// Pipeline (one stream, no host synchronisation):
+// Pipeline (one stream; no host synchronisation except the PLOC depth check):
 //   1. gather      per mesh: read MeshVertex P/N/C (rendering/_modeling.py:22-28) through the
 //                  optional int32 index list, write unsorted tri/shade records, padded leaf box,
 //                  AABB centroid.
@@ -13,11 +13,14 @@
 //                  writing each tile digit by digit from LDS.
 //   5. hierarchy   Karras 2012 binary radix tree over the extended keys (code, slot).
 //   6. emit        permute tri/shade records into slot order.
-//   7. refit       bottom-up from every leaf; the second thread to reach a node unions its
-//                  children's boxes and continues upward. Nodes whose leaf range lies inside one
-//                  2048-leaf block are refit by that block's workgroup through LDS counters; the
-//                  ~0.6% spanning blocks by a second pass with device-scope acq_rel counters
-//                  (each device-scope release writes back L2, so it is kept off the bulk).
+//   7. refit       nodes whose leaf range lies inside one 2048-leaf block: bottom-up in that
+//                  block's workgroup (the second thread to reach a node unions its children's
+//                  boxes and continues upward, LDS counters). The ~0.6% spanning blocks: each
+//                  child box is the union of its slot range, from per-block prefix/suffix unions
+//                  and a sparse table over blocks (min/max are exact, so this equals the
+//                  bottom-up union), one thread per node with no waiting.
+//   8. wide        4-wide records from the binary ones.
+// PLOC (rt_tuning.builder) replaces 5-7 with locally ordered clustering over the same order.
 // Every result is an integer or an exact FP32 min/max/sub, so the output is byte-identical to the
 // CPU restatement under the same inputs.
 #include <algorithm>
