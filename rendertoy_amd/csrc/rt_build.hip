@@ -13,7 +13,7 @@
 //                  writing each tile digit by digit from LDS.
 //   5. hierarchy   Karras 2012 binary radix tree over the extended keys (code, slot).
 //   6. emit        permute tri/shade records into slot order.
-//   7. refit       nodes whose leaf range lies inside one 2048-leaf block: bottom-up in that
+//   7. refit       nodes whose leaf range lies inside one 1024-leaf block: bottom-up in that
 //                  block's workgroup (the second thread to reach a node unions its children's
 //                  boxes and continues upward, LDS counters). The ~0.6% spanning blocks: each
 //                  child box is the union of its slot range, from per-block prefix/suffix unions
@@ -51,7 +51,6 @@ struct Workspace {
     int32_t* parent;   // (n-1) internal then n leaves
     int32_t* rangej;   // n-1: the other end j of internal node i's leaf range [min(i,j), max(i,j)]
     int32_t* split;    // n-1: split position g of internal node i (children ranges [lo,g], [g+1,hi])
-    uint32_t* flags;   // n-1
     float4* nodebox;   // 2(n-1)
     int32_t* top_list; // n-1: internal nodes whose leaf range spans several refit blocks
     uint32_t* top_count;
@@ -81,7 +80,7 @@ inline uint64_t align256(uint64_t x) { return (x + 255) & ~uint64_t(255); }
 // Refit blocks: bottom-up refit inside each range of kRefitBlock leaves runs in one workgroup
 // (k_refit_local); the nodes whose range spans several blocks ("top" nodes) are refit afterwards
 // from range unions (k_block_scan, k_block_sparse, k_refit_top).
-constexpr int kRefitBlock = 2048;
+constexpr int kRefitBlock = 1024;
 
 // levels of the sparse table over nb blocks: level k holds unions of 2^k consecutive blocks
 inline uint32_t sparse_levels(uint64_t nb) {
@@ -115,7 +114,6 @@ Workspace carve(char* base, uint64_t n) {
     w.parent = (int32_t*)take(4 * (ni + n));
     w.rangej = (int32_t*)take(4 * ni);
     w.split = (int32_t*)take(4 * ni);
-    w.flags = (uint32_t*)take(4 * ni);
     w.nodebox = (float4*)take(32 * ni);
     w.top_list = (int32_t*)take(4 * ni);
     w.top_count = (uint32_t*)take(4);
@@ -450,7 +448,6 @@ __global__ void __launch_bounds__(256) k_refit_local(const uint32_t* __restrict_
                                                       const int32_t* __restrict__ parent,
                                                       const int32_t* __restrict__ rangej,
                                                       const int32_t* __restrict__ split,
-                                                      uint32_t* __restrict__ flags,
                                                       float4* __restrict__ nodes,
                                                       float4* __restrict__ nodebox,
                                                       float* __restrict__ bounds) {
@@ -464,7 +461,6 @@ __global__ void __launch_bounds__(256) k_refit_local(const uint32_t* __restrict_
     for (int k = threadIdx.x; k < kRefitBlock; k += blockDim.x) {
         lflag[k] = 0;
         if (blo + k < ni) {
-            flags[blo + k] = 0;
             lj[k] = rangej[blo + k];
             lsplit[k] = split[blo + k];
             lpar[k] = parent[blo + k];
@@ -1486,7 +1482,7 @@ extern "C" int rt_scene_build(const rt_mesh_desc* meshes, uint32_t n_meshes, rt_
                        (float4*)sc->tris, (float4*)sc->shade);
     if (n > 1) {
         hipLaunchKernelGGL(k_refit_local, dim3(blocks_for(n, kRefitBlock)), dim3(256), 0, s, vi, n,
-                           w.leafbox, w.parent, w.rangej, w.split, w.flags, nodes, w.nodebox,
+                           w.leafbox, w.parent, w.rangej, w.split, nodes, w.nodebox,
                            sc->bounds);
         const uint32_t nb = blocks_for(n, kRefitBlock);
         hipLaunchKernelGGL(k_block_scan, dim3(nb), dim3(256), 0, s, vi, n, w.leafbox, w.pre, w.suf,
