@@ -307,6 +307,53 @@ def test_ploc_build_matches_twin_bytes_and_lbvh_hits(ren, scene):
     np.testing.assert_array_equal(gpu_trace(rc, o, d)["prim"], gpu_trace(lb, o, d)["prim"])
 
 
+def _edge_geometry_scene():
+    """Random triangles plus degenerate ones (points, collinear, a repeated vertex), exact duplicates
+    (equal t: the smaller primitive wins), triangles scaled to 1e30, and triangles with a NaN or an
+    infinite coordinate."""
+    rng = np.random.default_rng(3)
+    base = rng.uniform(-1, 1, (200, 3, 3)).astype(np.float32)
+    p = rng.uniform(-1, 1, (20, 3)).astype(np.float32)
+    nanp = base[:4].copy()
+    nanp[:, 1, 0] = np.nan
+    infp = base[4:8].copy()
+    infp[:, 2, 1] = np.inf
+    tris = [base, np.stack([p, p, p], 1), np.stack([p, p + 0.1, p + 0.2], 1).astype(np.float32),
+            np.stack([p, p, p + np.float32([0.3, 0, 0])], 1), base[:30].copy(), base[:5] * np.float32(1e30),
+            nanp, infp]
+    pos = np.concatenate(tris).reshape(-1, 3)
+    return [{"pos": pos, "idx": np.arange(len(pos), dtype=np.int32).reshape(-1, 3), "nrm": None, "uv": None}]
+
+
+@pytest.mark.parametrize("builder", ["lbvh", "ploc"])
+def test_edge_geometry(ren, builder):
+    """Degenerate, duplicated, huge and non-finite triangles: device == twin bytes, and every ray
+    (random, and axis-aligned with zero direction components) hits what FP32 brute force hits."""
+    meshes = _edge_geometry_scene()
+    tw = twin_from_meshes(meshes, builder=builder)
+    with ren.tuning(builder=builder):
+        rc = gpu_from_meshes(meshes)
+    torch.cuda.synchronize()
+    assert_build_equal(rc, tw)
+    o, d = random_rays(5000, seed=1)
+    rng = np.random.default_rng(4)
+    ao = rng.uniform(-1, 1, (600, 3)).astype(np.float32)
+    ad = np.zeros((600, 3), np.float32)
+    ad[np.arange(600), np.arange(600) % 3] = np.where(np.arange(600) % 2 == 0, 1.0, -1.0)
+    ao[np.arange(600), np.arange(600) % 3] = np.where(np.arange(600) % 2 == 0, -3.0, 3.0)
+    o, d = np.concatenate([o, ao]), np.concatenate([d, ad])
+    g = gpu_trace(rc, o, d)
+    t = tw.trace_closest(o, d)
+    assert_trace_equal(g, t)
+    _, bp = tw.brute_closest32(o, d)
+    np.testing.assert_array_equal(g["prim"], bp)
+    for mode in ("quad", "packet", "lane"):
+        with ren.tuning(primary=mode):
+            cam = camera(aspect=1.0)
+            a = rc.cast_camera(cam, 96, 96)["prim"].cpu().numpy()
+        np.testing.assert_array_equal(a, tw.trace_camera(cam.as_array(), 96, 96)["prim"])
+
+
 def _chain_scene(n, base=1.25):
     """n thin triangles at exponentially spaced x: PLOC merges them as a chain (depth ~ n)."""
     x = base ** np.arange(n)
