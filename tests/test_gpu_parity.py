@@ -354,6 +354,41 @@ def test_edge_geometry(ren, builder):
         np.testing.assert_array_equal(a, tw.trace_camera(cam.as_array(), 96, 96)["prim"])
 
 
+@pytest.mark.parametrize("primary", ["quad", "packet", "lane", "auto"])
+def test_render_options(ren, primary):
+    """Render flags and parameters against the twin, bit for bit: shadows off and on, point and
+    bilinear sampling, ambient/background/shadow_tmin values, odd image sizes (97x61, not a
+    multiple of any tile), 1x1 and 3x5 textures and a mesh without a texture (-1), a camera
+    whose frame is not coherent (close up and oblique)."""
+    from rendertoy_amd import scenes
+    a, b = scenes.icosphere(16, 31), scenes.icosphere(8, 32)
+    b["pos"] = (b["pos"] * 0.6 + np.float32([0.5, 0.2, 0.1])).astype(np.float32)
+    g = scenes.ground_plane()
+    meshes = [a, b, g]
+    tw = twin_from_meshes(meshes)
+    rc = gpu_from_meshes(meshes)
+    t1 = np.array([[[200, 10, 30, 255]]], np.uint8)
+    t2 = np.random.default_rng(9).integers(0, 256, (5, 3, 4), dtype=np.uint8)
+    texs = [t1, t2]
+    mt = [1, 0, -1]
+    cases = [(97, 61, (0.0, 0.0, 3.0), True, True, 0.2, (0.0, 0.0, 0.0), 1e-4),
+             (97, 61, (0.0, 0.0, 3.0), False, False, 0.35, (0.1, 0.2, 0.3), 1e-4),
+             (64, 48, (0.9, 0.4, 0.7), True, False, 0.0, (1.0, 0.5, 0.25), 1e-3)]
+    with ren.tuning(primary=primary):
+        for W, H, eye, shadows, bilinear, ambient, bg, stmin in cases:
+            cam = camera(eye=eye, aspect=W / H)
+            stats = torch.zeros(4, dtype=torch.int32, device="cuda")
+            img, hits = rc.render(cam, W, H, light=LIGHT, textures=[torch.from_numpy(x).cuda() for x in texs],
+                                  mesh_textures=mt, shadows=shadows, bilinear=bilinear, ambient=ambient,
+                                  background=bg, shadow_tmin=stmin, return_hits=True, stats=stats)
+            rgb, t, prim, st = tw.render(cam.as_array(), W, H, LIGHT, ambient=ambient, background=bg,
+                                         shadow_tmin=stmin, shadows=shadows, bilinear=bilinear,
+                                         textures=texs, mesh_tex=mt)
+            np.testing.assert_array_equal(hits["prim"].cpu().numpy(), prim)
+            np.testing.assert_array_equal(_u32(img.cpu().numpy()), _u32(rgb))
+            assert list(stats.cpu().numpy()[:3]) == list(st)
+
+
 def _chain_scene(n, base=1.25):
     """n thin triangles at exponentially spaced x: PLOC merges them as a chain (depth ~ n)."""
     x = base ** np.arange(n)
