@@ -175,7 +175,7 @@ typedef struct rt_tuning {
 
 /* Hierarchy builders of rt_scene_build; both start from the same Morton order and give the same
  * hits. The PLOC tree (locally ordered clustering by surface area) needs ~30% fewer traversal
- * steps per packet and ~17% fewer instructions than the LBVH radix tree, and takes ~2.6x longer
+ * steps per packet and ~17% fewer instructions than the LBVH radix tree, and takes ~3.7x longer
  * to build; frames whose time is set by their slowest tiles gain little from it. A PLOC tree with
  * a triangle more than 62 internal levels deep (the depth the traversal stacks are sized for, and
  * the bound of the radix tree) is replaced by the LBVH; checking this makes a PLOC build wait for
