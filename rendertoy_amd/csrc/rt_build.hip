@@ -78,8 +78,9 @@ struct Workspace {
 inline uint64_t align256(uint64_t x) { return (x + 255) & ~uint64_t(255); }
 
 // Refit blocks: bottom-up refit inside each range of kRefitBlock leaves runs in one workgroup
-// (k_refit_local); the nodes whose range spans several blocks ("top" nodes) are refit afterwards
-// from range unions (k_block_scan, k_block_sparse, k_refit_top).
+// (k_refit_local, which also scans the block's leaf boxes into prefix/suffix unions); the nodes
+// whose range spans several blocks ("top" nodes) are refit afterwards from those range unions
+// (k_block_sparse, k_refit_top).
 constexpr int kRefitBlock = 1024;
 
 // levels of the sparse table over nb blocks: level k holds unions of 2^k consecutive blocks
@@ -159,76 +160,108 @@ RT_DEV float3 load_f3(const char* base, uint64_t i, uint32_t stride, uint32_t of
     return make_float3(p[0], p[1], p[2]);
 }
 
+// Copies `per` float4s of each of the block's `rows` records from LDS (record r at
+// stage[r * pitch]) to out[0 .. rows*per), one float4 per thread and instruction.
+RT_DEV void store_staged(float4* __restrict__ out, const float4* stage, uint32_t rows, uint32_t per,
+                         uint32_t pitch) {
+    for (uint32_t q = threadIdx.x; q < rows * per; q += blockDim.x) {
+        const uint32_t r = q / per;
+        out[q] = stage[r * pitch + (q - r * per)];
+    }
+}
+
+// One thread per triangle: positions (canonical zeros), the centroid, the padded leaf box, the
+// intersection record (p0, e1, e2, primitive id) and the shading record (three normals, three
+// uvs, mesh id). The block's records go out through LDS as consecutive float4s.
 __global__ void __launch_bounds__(256) k_gather(rt_mesh_desc m, uint32_t mesh_id,
                                                  uint32_t prim_base, float4* __restrict__ cent,
                                                  float4* __restrict__ leafbox,
                                                  float4* __restrict__ tri_u,
                                                  float4* __restrict__ shade_u,
                                                  int32_t* __restrict__ mesh_offsets) {
+    constexpr uint32_t kPitch = 5;  // float4s per staged record (4 used)
+    __shared__ float4 stage[256 * kPitch];
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t == 0) mesh_offsets[mesh_id] = (int32_t)prim_base;
-    if (t >= m.n_tris) return;
-    const char* vb = (const char*)m.vertices;
-    uint32_t i0 = 3 * t, i1 = 3 * t + 1, i2 = 3 * t + 2;
-    if (m.indices) {
-        i0 = (uint32_t)m.indices[3 * (uint64_t)t];
-        i1 = (uint32_t)m.indices[3 * (uint64_t)t + 1];
-        i2 = (uint32_t)m.indices[3 * (uint64_t)t + 2];
-        // out-of-range indices are clamped so a malformed list cannot read past the buffer
-        i0 = i0 < m.n_vertices ? i0 : m.n_vertices - 1;
-        i1 = i1 < m.n_vertices ? i1 : m.n_vertices - 1;
-        i2 = i2 < m.n_vertices ? i2 : m.n_vertices - 1;
-    }
-    float3 p0 = load_f3(vb, i0, m.vertex_stride, m.pos_offset);
-    float3 p1 = load_f3(vb, i1, m.vertex_stride, m.pos_offset);
-    float3 p2 = load_f3(vb, i2, m.vertex_stride, m.pos_offset);
-    p0 = make_float3(canon(p0.x), canon(p0.y), canon(p0.z));
-    p1 = make_float3(canon(p1.x), canon(p1.y), canon(p1.z));
-    p2 = make_float3(canon(p2.x), canon(p2.y), canon(p2.z));
-    const uint32_t p = prim_base + t;
+    const uint32_t t0 = blockIdx.x * blockDim.x;
+    if (t0 >= m.n_tris) return;  // whole block (only the launch for an empty mesh)
+    const uint32_t rows = m.n_tris - t0 < blockDim.x ? m.n_tris - t0 : blockDim.x;
+    const uint64_t pb = (uint64_t)prim_base + t0;  // the block's first primitive
+    const bool mine = t < m.n_tris;
+    float4 box[2], tri[3], shd[4];
+    if (mine) {
+        const char* vb = (const char*)m.vertices;
+        uint32_t i0 = 3 * t, i1 = 3 * t + 1, i2 = 3 * t + 2;
+        if (m.indices) {
+            i0 = (uint32_t)m.indices[3 * (uint64_t)t];
+            i1 = (uint32_t)m.indices[3 * (uint64_t)t + 1];
+            i2 = (uint32_t)m.indices[3 * (uint64_t)t + 2];
+            // out-of-range indices are clamped so a malformed list cannot read past the buffer
+            i0 = i0 < m.n_vertices ? i0 : m.n_vertices - 1;
+            i1 = i1 < m.n_vertices ? i1 : m.n_vertices - 1;
+            i2 = i2 < m.n_vertices ? i2 : m.n_vertices - 1;
+        }
+        float3 p0 = load_f3(vb, i0, m.vertex_stride, m.pos_offset);
+        float3 p1 = load_f3(vb, i1, m.vertex_stride, m.pos_offset);
+        float3 p2 = load_f3(vb, i2, m.vertex_stride, m.pos_offset);
+        p0 = make_float3(canon(p0.x), canon(p0.y), canon(p0.z));
+        p1 = make_float3(canon(p1.x), canon(p1.y), canon(p1.z));
+        p2 = make_float3(canon(p2.x), canon(p2.y), canon(p2.z));
+        const uint32_t p = prim_base + t;
 
-    const float lox = fminf(fminf(p0.x, p1.x), p2.x), hix = fmaxf(fmaxf(p0.x, p1.x), p2.x);
-    const float loy = fminf(fminf(p0.y, p1.y), p2.y), hiy = fmaxf(fmaxf(p0.y, p1.y), p2.y);
-    const float loz = fminf(fminf(p0.z, p1.z), p2.z), hiz = fmaxf(fmaxf(p0.z, p1.z), p2.z);
-    cent[p] = make_float4((lox + hix) * 0.5f, (loy + hiy) * 0.5f, (loz + hiz) * 0.5f, 0.0f);
-    const float px = pad_of(lox, hix), py = pad_of(loy, hiy), pz = pad_of(loz, hiz);
-    leafbox[2 * (uint64_t)p] = make_float4(lox - px, loy - py, loz - pz, 0.0f);
-    leafbox[2 * (uint64_t)p + 1] = make_float4(hix + px, hiy + py, hiz + pz, 0.0f);
+        const float lox = fminf(fminf(p0.x, p1.x), p2.x), hix = fmaxf(fmaxf(p0.x, p1.x), p2.x);
+        const float loy = fminf(fminf(p0.y, p1.y), p2.y), hiy = fmaxf(fmaxf(p0.y, p1.y), p2.y);
+        const float loz = fminf(fminf(p0.z, p1.z), p2.z), hiz = fmaxf(fmaxf(p0.z, p1.z), p2.z);
+        cent[p] = make_float4((lox + hix) * 0.5f, (loy + hiy) * 0.5f, (loz + hiz) * 0.5f, 0.0f);
+        const float px = pad_of(lox, hix), py = pad_of(loy, hiy), pz = pad_of(loz, hiz);
+        box[0] = make_float4(lox - px, loy - py, loz - pz, 0.0f);
+        box[1] = make_float4(hix + px, hiy + py, hiz + pz, 0.0f);
 
-    const float e1x = p1.x - p0.x, e1y = p1.y - p0.y, e1z = p1.z - p0.z;
-    const float e2x = p2.x - p0.x, e2y = p2.y - p0.y, e2z = p2.z - p0.z;
-    float4* tr = tri_u + 3 * (uint64_t)p;
-    tr[0] = make_float4(p0.x, p0.y, p0.z, __int_as_float((int32_t)p));
-    tr[1] = make_float4(e1x, e1y, e1z, 0.0f);
-    tr[2] = make_float4(e2x, e2y, e2z, 0.0f);
+        const float e1x = p1.x - p0.x, e1y = p1.y - p0.y, e1z = p1.z - p0.z;
+        const float e2x = p2.x - p0.x, e2y = p2.y - p0.y, e2z = p2.z - p0.z;
+        tri[0] = make_float4(p0.x, p0.y, p0.z, __int_as_float((int32_t)p));
+        tri[1] = make_float4(e1x, e1y, e1z, 0.0f);
+        tri[2] = make_float4(e2x, e2y, e2z, 0.0f);
 
-    float3 n0, n1, n2;
-    if (m.nrm_offset != RT_ABSENT) {
-        n0 = load_f3(vb, i0, m.vertex_stride, m.nrm_offset);
-        n1 = load_f3(vb, i1, m.vertex_stride, m.nrm_offset);
-        n2 = load_f3(vb, i2, m.vertex_stride, m.nrm_offset);
-    } else {
-        const float gx = fmaf(e1y, e2z, -(e1z * e2y));
-        const float gy = fmaf(e1z, e2x, -(e1x * e2z));
-        const float gz = fmaf(e1x, e2y, -(e1y * e2x));
-        const float l2 = fmaf(gx, gx, fmaf(gy, gy, gz * gz));
-        const float il = l2 > 0.0f ? 1.0f / sqrtf(l2) : 0.0f;
-        n0 = n1 = n2 = make_float3(gx * il, gy * il, gz * il);
+        float3 n0, n1, n2;
+        if (m.nrm_offset != RT_ABSENT) {
+            n0 = load_f3(vb, i0, m.vertex_stride, m.nrm_offset);
+            n1 = load_f3(vb, i1, m.vertex_stride, m.nrm_offset);
+            n2 = load_f3(vb, i2, m.vertex_stride, m.nrm_offset);
+        } else {
+            const float gx = fmaf(e1y, e2z, -(e1z * e2y));
+            const float gy = fmaf(e1z, e2x, -(e1x * e2z));
+            const float gz = fmaf(e1x, e2y, -(e1y * e2x));
+            const float l2 = fmaf(gx, gx, fmaf(gy, gy, gz * gz));
+            const float il = l2 > 0.0f ? 1.0f / sqrtf(l2) : 0.0f;
+            n0 = n1 = n2 = make_float3(gx * il, gy * il, gz * il);
+        }
+        float2 uv0 = make_float2(0.0f, 0.0f), uv1 = uv0, uv2 = uv0;
+        if (m.uv_offset != RT_ABSENT) {
+            const float* q0 = (const float*)(vb + (uint64_t)i0 * m.vertex_stride + m.uv_offset);
+            const float* q1 = (const float*)(vb + (uint64_t)i1 * m.vertex_stride + m.uv_offset);
+            const float* q2 = (const float*)(vb + (uint64_t)i2 * m.vertex_stride + m.uv_offset);
+            uv0 = make_float2(q0[0], q0[1]);
+            uv1 = make_float2(q1[0], q1[1]);
+            uv2 = make_float2(q2[0], q2[1]);
+        }
+        shd[0] = make_float4(n0.x, n0.y, n0.z, n1.x);
+        shd[1] = make_float4(n1.y, n1.z, n2.x, n2.y);
+        shd[2] = make_float4(n2.z, uv0.x, uv0.y, uv1.x);
+        shd[3] = make_float4(uv1.y, uv2.x, uv2.y, __int_as_float((int32_t)mesh_id));
     }
-    float2 uv0 = make_float2(0.0f, 0.0f), uv1 = uv0, uv2 = uv0;
-    if (m.uv_offset != RT_ABSENT) {
-        const float* q0 = (const float*)(vb + (uint64_t)i0 * m.vertex_stride + m.uv_offset);
-        const float* q1 = (const float*)(vb + (uint64_t)i1 * m.vertex_stride + m.uv_offset);
-        const float* q2 = (const float*)(vb + (uint64_t)i2 * m.vertex_stride + m.uv_offset);
-        uv0 = make_float2(q0[0], q0[1]);
-        uv1 = make_float2(q1[0], q1[1]);
-        uv2 = make_float2(q2[0], q2[1]);
-    }
-    float4* sh = shade_u + 4 * (uint64_t)p;
-    sh[0] = make_float4(n0.x, n0.y, n0.z, n1.x);
-    sh[1] = make_float4(n1.y, n1.z, n2.x, n2.y);
-    sh[2] = make_float4(n2.z, uv0.x, uv0.y, uv1.x);
-    sh[3] = make_float4(uv1.y, uv2.x, uv2.y, __int_as_float((int32_t)mesh_id));
+    float4* st = stage + threadIdx.x * kPitch;
+    if (mine) { st[0] = shd[0]; st[1] = shd[1]; st[2] = shd[2]; st[3] = shd[3]; }
+    __syncthreads();
+    store_staged(shade_u + 4 * pb, stage, rows, 4, kPitch);
+    __syncthreads();
+    if (mine) { st[0] = tri[0]; st[1] = tri[1]; st[2] = tri[2]; }
+    __syncthreads();
+    store_staged(tri_u + 3 * pb, stage, rows, 3, kPitch);
+    __syncthreads();
+    if (mine) { st[0] = box[0]; st[1] = box[1]; }
+    __syncthreads();
+    store_staged(leafbox + 2 * pb, stage, rows, 2, kPitch);
 }
 
 __global__ void k_mesh_offsets_tail(int32_t* mesh_offsets, uint32_t n_meshes, uint32_t n) {
@@ -377,21 +410,22 @@ __global__ void __launch_bounds__(256) k_hierarchy(const uint32_t* __restrict__ 
     if (lo / kRefitBlock != hi / kRefitBlock) top_list[atomicAdd(top_count, 1u)] = (int32_t)i;
 }
 
+// One thread per float4 of the outputs: thread x copies shade float4 x (record x/4) and, for
+// x < 3n, tris float4 x (record x/3), so each store instruction writes consecutive 16 B pieces
+// and each record's source is read as consecutive pieces by neighbouring lanes.
 __global__ void __launch_bounds__(256) k_emit(const uint32_t* __restrict__ order, uint32_t n,
-                                               const float4* __restrict__ tri_u,
-                                               const float4* __restrict__ shade_u,
-                                               float4* __restrict__ tris,
-                                               float4* __restrict__ shade) {
-    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= n) return;
-    const uint64_t p = order[s];
-    tris[3 * (uint64_t)s] = tri_u[3 * p];
-    tris[3 * (uint64_t)s + 1] = tri_u[3 * p + 1];
-    tris[3 * (uint64_t)s + 2] = tri_u[3 * p + 2];
-    shade[4 * (uint64_t)s] = shade_u[4 * p];
-    shade[4 * (uint64_t)s + 1] = shade_u[4 * p + 1];
-    shade[4 * (uint64_t)s + 2] = shade_u[4 * p + 2];
-    shade[4 * (uint64_t)s + 3] = shade_u[4 * p + 3];
+                                              const float4* __restrict__ tri_u,
+                                              const float4* __restrict__ shade_u,
+                                              float4* __restrict__ tris,
+                                              float4* __restrict__ shade) {
+    const uint64_t x = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= 4 * (uint64_t)n) return;
+    const uint64_t ps = order[x >> 2];
+    shade[x] = shade_u[4 * ps + (x & 3)];
+    if (x < 3 * (uint64_t)n) {
+        const uint64_t s = x / 3;
+        tris[x] = tri_u[3 * (uint64_t)order[s] + (x - 3 * s)];
+    }
 }
 
 // ---- refit ----
@@ -443,112 +477,15 @@ RT_DEV void store_node(float4* __restrict__ nodes, float4* __restrict__ nodebox,
 // its node ids and the boxes of its leaves (gathered through `order`). The first thread to reach a
 // node stops; the second (LDS counter, workgroup acq_rel) unions the children's boxes from LDS and
 // writes the node. Also clears the counters of this block's node ids for phase 2.
-__global__ void __launch_bounds__(256) k_refit_local(const uint32_t* __restrict__ order, uint32_t n,
-                                                      const float4* __restrict__ leafbox,
-                                                      const int32_t* __restrict__ parent,
-                                                      const int32_t* __restrict__ rangej,
-                                                      const int32_t* __restrict__ split,
-                                                      float4* __restrict__ nodes,
-                                                      float4* __restrict__ nodebox,
-                                                      float* __restrict__ bounds) {
-    __shared__ uint32_t lflag[kRefitBlock];
-    __shared__ float lbox[kRefitBlock][6];   // boxes of the block's internal nodes
-    __shared__ float lleaf[kRefitBlock][6];  // boxes of the block's leaves (slots)
-    __shared__ int32_t lj[kRefitBlock], lsplit[kRefitBlock], lpar[kRefitBlock];
-    const int64_t blo = (int64_t)blockIdx.x * kRefitBlock;
-    const int64_t bhi = std::min<int64_t>(blo + kRefitBlock, n);  // exclusive
-    const int64_t ni = (int64_t)n - 1;
-    for (int k = threadIdx.x; k < kRefitBlock; k += blockDim.x) {
-        lflag[k] = 0;
-        if (blo + k < ni) {
-            lj[k] = rangej[blo + k];
-            lsplit[k] = split[blo + k];
-            lpar[k] = parent[blo + k];
-        }
-        if (blo + k < bhi) {
-            const uint64_t p = order[blo + k];
-            const float4 a = leafbox[2 * p], b = leafbox[2 * p + 1];
-            lleaf[k][0] = a.x; lleaf[k][1] = a.y; lleaf[k][2] = a.z;
-            lleaf[k][3] = b.x; lleaf[k][4] = b.y; lleaf[k][5] = b.z;
-        }
-    }
-    __syncthreads();
-    for (int64_t s = blo + threadIdx.x; s < bhi; s += blockDim.x) {
-        int32_t node = parent[ni + s];
-        while (node >= blo && node < bhi && node < ni) {
-            const int32_t li = (int32_t)(node - blo);
-            const int64_t j = lj[li];
-            const int64_t lo = std::min<int64_t>(node, j), hi = std::max<int64_t>(node, j);
-            if (lo < blo || hi >= bhi) break;  // a top node: phase 2
-            const uint32_t prev = __hip_atomic_fetch_add(&lflag[li], 1u, __ATOMIC_ACQ_REL,
-                                                         __HIP_MEMORY_SCOPE_WORKGROUP);
-            if (prev == 0) break;
-            const int32_t g = lsplit[li];
-            const int32_t l = lo == g ? ~g : g, r = hi == g + 1 ? ~(g + 1) : g + 1;
-            const float* lb = l < 0 ? lleaf[~l - blo] : lbox[l - blo];
-            const float* rb = r < 0 ? lleaf[~r - blo] : lbox[r - blo];
-            float l_lo[3], l_hi[3], r_lo[3], r_hi[3], u_lo[3], u_hi[3];
-            for (int c = 0; c < 3; ++c) {
-                l_lo[c] = lb[c];
-                l_hi[c] = lb[3 + c];
-                r_lo[c] = rb[c];
-                r_hi[c] = rb[3 + c];
-            }
-            store_node(nodes, nodebox, bounds, node, l_lo, l_hi, r_lo, r_hi, u_lo, u_hi);
-            for (int c = 0; c < 3; ++c) {
-                lbox[li][c] = u_lo[c];
-                lbox[li][3 + c] = u_hi[c];
-            }
-            node = lpar[li];
-        }
-    }
-}
-
-RT_DEV bool is_local(const int32_t* __restrict__ rangej, int32_t ref) {
-    if (ref < 0) return true;
-    const int32_t j = rangej[ref];
-    const int32_t lo = ref < j ? ref : j, hi = ref < j ? j : ref;
-    return lo / kRefitBlock == hi / kRefitBlock;
-}
-
-RT_DEV void node_box(const float4* __restrict__ leafbox, const uint32_t* __restrict__ order,
-                     const float4* nodebox, int32_t ref, float* lo, float* hi) {
-    if (ref < 0) {
-        leaf_box(leafbox, order, ref, lo, hi);
-    } else {
-        const float4 a = nodebox[2 * (uint64_t)ref], b = nodebox[2 * (uint64_t)ref + 1];
-        lo[0] = a.x; lo[1] = a.y; lo[2] = a.z;
-        hi[0] = b.x; hi[1] = b.y; hi[2] = b.z;
-    }
-}
-
-// Phase 2 prepares range unions: per refit block, the prefix and suffix unions of its leaf boxes in
-// slot order (a workgroup scan, 8 slots per thread) and the block's total, level 0 of the sparse
-// table.
-__global__ void __launch_bounds__(256) k_block_scan(const uint32_t* __restrict__ order, uint32_t n,
-                                                     const float4* __restrict__ leafbox,
-                                                     float4* __restrict__ pre, float4* __restrict__ suf,
-                                                     float4* __restrict__ sparse) {
-    constexpr int kPer = kRefitBlock / 256;
-    __shared__ float tlo[3][256], thi[3][256];
+// Range unions for the top refit, per refit block: the prefix and suffix unions of its leaf boxes
+// in slot order and the block's total, level 0 of the sparse table. A workgroup scan over the
+// block's slots, kRefitBlock / 256 per thread (thread t holds slots s0 .. s0 + kPer - 1 in lo/hi,
+// empty boxes past n); tlo/thi are 256-entry LDS rows. Called by all threads of the workgroup.
+constexpr int kPer = kRefitBlock / 256;
+RT_DEV void block_unions(const float (&lo)[kPer][3], const float (&hi)[kPer][3], uint64_t s0, uint32_t n,
+                         float4* __restrict__ pre, float4* __restrict__ suf,
+                         float4* __restrict__ sparse, float (*tlo)[256], float (*thi)[256]) {
     const uint32_t t = threadIdx.x;
-    const uint64_t s0 = (uint64_t)blockIdx.x * kRefitBlock + (uint64_t)t * kPer;
-    float lo[kPer][3], hi[kPer][3];
-#pragma unroll
-    for (int k = 0; k < kPer; ++k) {
-        const uint64_t sl = s0 + k;
-        if (sl < n) {
-            const uint64_t p = order[sl];
-            const float4 a = leafbox[2 * p], b = leafbox[2 * p + 1];
-            lo[k][0] = a.x; lo[k][1] = a.y; lo[k][2] = a.z;
-            hi[k][0] = b.x; hi[k][1] = b.y; hi[k][2] = b.z;
-        } else {
-            for (int c = 0; c < 3; ++c) {
-                lo[k][c] = INFINITY;
-                hi[k][c] = -INFINITY;
-            }
-        }
-    }
     // exclusive prefix union of the thread totals, then inclusive per slot
     float tl[3], th[3];
     for (int c = 0; c < 3; ++c) {
@@ -629,6 +566,103 @@ __global__ void __launch_bounds__(256) k_block_scan(const uint32_t* __restrict__
             suf[2 * (s0 + k)] = make_float4(rl[0], rl[1], rl[2], 0.0f);
             suf[2 * (s0 + k) + 1] = make_float4(rh[0], rh[1], rh[2], 0.0f);
         }
+    }
+}
+
+
+__global__ void __launch_bounds__(256) k_refit_local(const uint32_t* __restrict__ order, uint32_t n,
+                                                      const float4* __restrict__ leafbox,
+                                                      const int32_t* __restrict__ parent,
+                                                      const int32_t* __restrict__ rangej,
+                                                      const int32_t* __restrict__ split,
+                                                      float4* __restrict__ nodes,
+                                                      float4* __restrict__ nodebox,
+                                                      float* __restrict__ bounds,
+                                                      float4* __restrict__ pre,
+                                                      float4* __restrict__ suf,
+                                                      float4* __restrict__ sparse) {
+    __shared__ uint32_t lflag[kRefitBlock];
+    __shared__ float tlo[3][256], thi[3][256];
+    __shared__ float lbox[kRefitBlock][6];   // boxes of the block's internal nodes
+    __shared__ float lleaf[kRefitBlock][6];  // boxes of the block's leaves (slots)
+    __shared__ int32_t lj[kRefitBlock], lsplit[kRefitBlock], lpar[kRefitBlock];
+    const int64_t blo = (int64_t)blockIdx.x * kRefitBlock;
+    const int64_t bhi = std::min<int64_t>(blo + kRefitBlock, n);  // exclusive
+    const int64_t ni = (int64_t)n - 1;
+    for (int k = threadIdx.x; k < kRefitBlock; k += blockDim.x) {
+        lflag[k] = 0;
+        if (blo + k < ni) {
+            lj[k] = rangej[blo + k];
+            lsplit[k] = split[blo + k];
+            lpar[k] = parent[blo + k];
+        }
+        if (blo + k < bhi) {
+            const uint64_t p = order[blo + k];
+            const float4 a = leafbox[2 * p], b = leafbox[2 * p + 1];
+            lleaf[k][0] = a.x; lleaf[k][1] = a.y; lleaf[k][2] = a.z;
+            lleaf[k][3] = b.x; lleaf[k][4] = b.y; lleaf[k][5] = b.z;
+        }
+    }
+    __syncthreads();
+    {  // the block's range unions, from the leaf boxes already in LDS
+        float lo[kPer][3], hi[kPer][3];
+        const uint32_t l0 = threadIdx.x * kPer;
+#pragma unroll
+        for (int k = 0; k < kPer; ++k) {
+            const bool in = blo + l0 + k < bhi;
+            for (int c = 0; c < 3; ++c) {
+                lo[k][c] = in ? lleaf[l0 + k][c] : INFINITY;
+                hi[k][c] = in ? lleaf[l0 + k][3 + c] : -INFINITY;
+            }
+        }
+        block_unions(lo, hi, (uint64_t)blo + l0, n, pre, suf, sparse, tlo, thi);
+    }
+    for (int64_t s = blo + threadIdx.x; s < bhi; s += blockDim.x) {
+        int32_t node = parent[ni + s];
+        while (node >= blo && node < bhi && node < ni) {
+            const int32_t li = (int32_t)(node - blo);
+            const int64_t j = lj[li];
+            const int64_t lo = std::min<int64_t>(node, j), hi = std::max<int64_t>(node, j);
+            if (lo < blo || hi >= bhi) break;  // a top node: phase 2
+            const uint32_t prev = __hip_atomic_fetch_add(&lflag[li], 1u, __ATOMIC_ACQ_REL,
+                                                         __HIP_MEMORY_SCOPE_WORKGROUP);
+            if (prev == 0) break;
+            const int32_t g = lsplit[li];
+            const int32_t l = lo == g ? ~g : g, r = hi == g + 1 ? ~(g + 1) : g + 1;
+            const float* lb = l < 0 ? lleaf[~l - blo] : lbox[l - blo];
+            const float* rb = r < 0 ? lleaf[~r - blo] : lbox[r - blo];
+            float l_lo[3], l_hi[3], r_lo[3], r_hi[3], u_lo[3], u_hi[3];
+            for (int c = 0; c < 3; ++c) {
+                l_lo[c] = lb[c];
+                l_hi[c] = lb[3 + c];
+                r_lo[c] = rb[c];
+                r_hi[c] = rb[3 + c];
+            }
+            store_node(nodes, nodebox, bounds, node, l_lo, l_hi, r_lo, r_hi, u_lo, u_hi);
+            for (int c = 0; c < 3; ++c) {
+                lbox[li][c] = u_lo[c];
+                lbox[li][3 + c] = u_hi[c];
+            }
+            node = lpar[li];
+        }
+    }
+}
+
+RT_DEV bool is_local(const int32_t* __restrict__ rangej, int32_t ref) {
+    if (ref < 0) return true;
+    const int32_t j = rangej[ref];
+    const int32_t lo = ref < j ? ref : j, hi = ref < j ? j : ref;
+    return lo / kRefitBlock == hi / kRefitBlock;
+}
+
+RT_DEV void node_box(const float4* __restrict__ leafbox, const uint32_t* __restrict__ order,
+                     const float4* nodebox, int32_t ref, float* lo, float* hi) {
+    if (ref < 0) {
+        leaf_box(leafbox, order, ref, lo, hi);
+    } else {
+        const float4 a = nodebox[2 * (uint64_t)ref], b = nodebox[2 * (uint64_t)ref + 1];
+        lo[0] = a.x; lo[1] = a.y; lo[2] = a.z;
+        hi[0] = b.x; hi[1] = b.y; hi[2] = b.z;
     }
 }
 
@@ -1274,43 +1308,61 @@ inline int ploc_iterations(uint32_t n) {
 //   [0]=lo.x[4] [1]=hi.x[4] [2]=lo.y[4] [3]=hi.y[4] [4]=lo.z[4] [5]=hi.z[4] [6]=refs[4] [7]=0
 // Unused slots hold ref RT_EMPTY_ROOT and an empty box (lo=+inf, hi=-inf). Every binary node gets
 // a record (traversal from the root only reaches every other level), so refs stay binary ids.
+// 4-wide record of binary node i: the boxes and references of its grandchildren (a child that is a
+// leaf stands for itself), empty slots padded. The block's 256 records are staged in LDS (a
+// 144-byte stride keeps the per-lane writes off one bank) and stored as consecutive float4s, so
+// every store instruction writes 4 KB of the output contiguously. (Loading the block's binary
+// records through LDS as well measured slower: they are read once and the child reads hit L2.)
 __global__ void __launch_bounds__(256) k_wide(uint32_t n, const float4* __restrict__ nodes,
-                                               float4* __restrict__ nodes4) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= (int64_t)n - 1) return;
-    float lo[3][4], hi[3][4];
-    int32_t ref[4];
-    int k = 0;
-    const float4* nd = nodes + (uint64_t)i * 4;
-    const float4 n0 = nd[0], n1 = nd[1], n2 = nd[2], n3 = nd[3];
-    const int32_t c[2] = {__float_as_int(n3.x), __float_as_int(n3.y)};
-    const float cb[2][6] = {{n0.x, n0.z, n2.x, n0.y, n0.w, n2.y}, {n1.x, n1.z, n2.z, n1.y, n1.w, n2.w}};
-    for (int e = 0; e < 2; ++e) {
-        if (c[e] >= 0) {
-            const float4* cd = nodes + (uint64_t)c[e] * 4;
-            const float4 m0 = cd[0], m1 = cd[1], m2 = cd[2], m3 = cd[3];
-            lo[0][k] = m0.x; hi[0][k] = m0.y; lo[1][k] = m0.z; hi[1][k] = m0.w;
-            lo[2][k] = m2.x; hi[2][k] = m2.y; ref[k] = __float_as_int(m3.x); ++k;
-            lo[0][k] = m1.x; hi[0][k] = m1.y; lo[1][k] = m1.z; hi[1][k] = m1.w;
-            lo[2][k] = m2.z; hi[2][k] = m2.w; ref[k] = __float_as_int(m3.y); ++k;
-        } else {
-            lo[0][k] = cb[e][0]; lo[1][k] = cb[e][1]; lo[2][k] = cb[e][2];
-            hi[0][k] = cb[e][3]; hi[1][k] = cb[e][4]; hi[2][k] = cb[e][5];
-            ref[k] = c[e]; ++k;
+                                              float4* __restrict__ nodes4) {
+    constexpr int kPitch = 9;  // float4s per staged record
+    __shared__ float4 stage[256 * kPitch];
+    const uint64_t base = (uint64_t)blockIdx.x * blockDim.x;
+    const uint64_t i = base + threadIdx.x;
+    const uint64_t m = (uint64_t)n - 1;  // internal nodes
+    const uint64_t rows = m - base < 256 ? m - base : 256;  // records of this block
+    if (i < m) {
+        float lo[3][4], hi[3][4];
+        int32_t ref[4];
+        int k = 0;
+        const float4* nd = nodes + i * 4;
+        const float4 n0 = nd[0], n1 = nd[1], n2 = nd[2], n3 = nd[3];
+        const int32_t c[2] = {__float_as_int(n3.x), __float_as_int(n3.y)};
+        const float cb[2][6] = {{n0.x, n0.z, n2.x, n0.y, n0.w, n2.y}, {n1.x, n1.z, n2.z, n1.y, n1.w, n2.w}};
+        for (int e = 0; e < 2; ++e) {
+            if (c[e] >= 0) {
+                const float4* cd = nodes + (uint64_t)c[e] * 4;
+                const float4 m0 = cd[0], m1 = cd[1], m2 = cd[2], m3 = cd[3];
+                lo[0][k] = m0.x; hi[0][k] = m0.y; lo[1][k] = m0.z; hi[1][k] = m0.w;
+                lo[2][k] = m2.x; hi[2][k] = m2.y; ref[k] = __float_as_int(m3.x); ++k;
+                lo[0][k] = m1.x; hi[0][k] = m1.y; lo[1][k] = m1.z; hi[1][k] = m1.w;
+                lo[2][k] = m2.z; hi[2][k] = m2.w; ref[k] = __float_as_int(m3.y); ++k;
+            } else {
+                lo[0][k] = cb[e][0]; lo[1][k] = cb[e][1]; lo[2][k] = cb[e][2];
+                hi[0][k] = cb[e][3]; hi[1][k] = cb[e][4]; hi[2][k] = cb[e][5];
+                ref[k] = c[e]; ++k;
+            }
         }
+        for (; k < 4; ++k) {
+            for (int a = 0; a < 3; ++a) { lo[a][k] = INFINITY; hi[a][k] = -INFINITY; }
+            ref[k] = RT_EMPTY_ROOT;
+        }
+        float4* o = stage + threadIdx.x * kPitch;
+        for (int a = 0; a < 3; ++a) {
+            o[2 * a] = make_float4(lo[a][0], lo[a][1], lo[a][2], lo[a][3]);
+            o[2 * a + 1] = make_float4(hi[a][0], hi[a][1], hi[a][2], hi[a][3]);
+        }
+        o[6] = make_float4(__int_as_float(ref[0]), __int_as_float(ref[1]), __int_as_float(ref[2]),
+                           __int_as_float(ref[3]));
+        o[7] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     }
-    for (; k < 4; ++k) {
-        for (int a = 0; a < 3; ++a) { lo[a][k] = INFINITY; hi[a][k] = -INFINITY; }
-        ref[k] = RT_EMPTY_ROOT;
+    __syncthreads();
+    float4* out = nodes4 + base * 8;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        const uint32_t q = j * 256 + threadIdx.x;  // float4 q of the block's output
+        if (q < rows * 8) out[q] = stage[(q >> 3) * kPitch + (q & 7)];
     }
-    float4* o = nodes4 + (uint64_t)i * 8;
-    for (int a = 0; a < 3; ++a) {
-        o[2 * a] = make_float4(lo[a][0], lo[a][1], lo[a][2], lo[a][3]);
-        o[2 * a + 1] = make_float4(hi[a][0], hi[a][1], hi[a][2], hi[a][3]);
-    }
-    o[6] = make_float4(__int_as_float(ref[0]), __int_as_float(ref[1]), __int_as_float(ref[2]),
-                       __int_as_float(ref[3]));
-    o[7] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 }
 
 // Scene of one triangle: no internal node; the root is leaf slot 0.
@@ -1465,7 +1517,7 @@ extern "C" int rt_scene_build(const rt_mesh_desc* meshes, uint32_t n_meshes, rt_
                            w.nsize, w.nleft, w.nright, w.pos_of, w.scan, w.slot_order, w.slot_of, w.fslot);
         hipLaunchKernelGGL(k_ploc_refs, dim3(blocks_for(n - 1, 256)), dim3(256), 0, s, n, w.nleft,
                            w.nright, w.nsize, w.slot_of, w.fslot, nodes);
-        hipLaunchKernelGGL(k_emit, dim3(blocks_for(n, 256)), dim3(256), 0, s, w.slot_order, n, w.tri_u,
+        hipLaunchKernelGGL(k_emit, dim3(blocks_for(4 * (uint64_t)n, 256)), dim3(256), 0, s, w.slot_order, n, w.tri_u,
                            w.shade_u, (float4*)sc->tris, (float4*)sc->shade);
         hipLaunchKernelGGL(k_wide, dim3(blocks_for(n - 1, 256)), dim3(256), 0, s, n,
                            (const float4*)nodes, (float4*)sc->nodes4);
@@ -1478,15 +1530,12 @@ extern "C" int rt_scene_build(const rt_mesh_desc* meshes, uint32_t n_meshes, rt_
         hipLaunchKernelGGL(k_hierarchy, dim3(blocks_for(n - 1, 256)), dim3(256), 0, s, ki, n, nodes,
                            w.parent, w.rangej, w.split, w.top_count, w.top_list);
     }
-    hipLaunchKernelGGL(k_emit, dim3(blocks_for(n, 256)), dim3(256), 0, s, vi, n, w.tri_u, w.shade_u,
+    hipLaunchKernelGGL(k_emit, dim3(blocks_for(4 * (uint64_t)n, 256)), dim3(256), 0, s, vi, n, w.tri_u, w.shade_u,
                        (float4*)sc->tris, (float4*)sc->shade);
     if (n > 1) {
-        hipLaunchKernelGGL(k_refit_local, dim3(blocks_for(n, kRefitBlock)), dim3(256), 0, s, vi, n,
-                           w.leafbox, w.parent, w.rangej, w.split, nodes, w.nodebox,
-                           sc->bounds);
         const uint32_t nb = blocks_for(n, kRefitBlock);
-        hipLaunchKernelGGL(k_block_scan, dim3(nb), dim3(256), 0, s, vi, n, w.leafbox, w.pre, w.suf,
-                           w.sparse);
+        hipLaunchKernelGGL(k_refit_local, dim3(nb), dim3(256), 0, s, vi, n, w.leafbox, w.parent,
+                           w.rangej, w.split, nodes, w.nodebox, sc->bounds, w.pre, w.suf, w.sparse);
         hipLaunchKernelGGL(k_block_sparse, dim3(1), dim3(1024), 0, s, w.sparse, nb, sparse_levels(nb));
         hipLaunchKernelGGL(k_refit_top, dim3(std::min<uint32_t>(blocks_for(n, 256), 1024)), dim3(256), 0, s,
                            vi, n, w.leafbox, w.rangej, w.split, nodes, w.nodebox, sc->bounds, w.top_list,
