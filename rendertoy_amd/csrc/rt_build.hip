@@ -471,12 +471,6 @@ RT_DEV void store_node(float4* __restrict__ nodes, float4* __restrict__ nodebox,
     }
 }
 
-// Phase 1: block b walks up from each of its leaves through the local nodes (those whose leaf range
-// lies inside the block; a node's id is an end of its range, so local nodes have ids in the block
-// too). The block first copies what the walks read into LDS: the rangej / split / parent entries of
-// its node ids and the boxes of its leaves (gathered through `order`). The first thread to reach a
-// node stops; the second (LDS counter, workgroup acq_rel) unions the children's boxes from LDS and
-// writes the node. Also clears the counters of this block's node ids for phase 2.
 // Range unions for the top refit, per refit block: the prefix and suffix unions of its leaf boxes
 // in slot order and the block's total, level 0 of the sparse table. A workgroup scan over the
 // block's slots, kRefitBlock / 256 per thread (thread t holds slots s0 .. s0 + kPer - 1 in lo/hi,
@@ -570,6 +564,12 @@ RT_DEV void block_unions(const float (&lo)[kPer][3], const float (&hi)[kPer][3],
 }
 
 
+// Phase 1: block b walks up from each of its leaves through the local nodes (those whose leaf range
+// lies inside the block; a node's id is an end of its range, so local nodes have ids in the block
+// too). The block first copies what the walks read into LDS: the rangej / split / parent entries of
+// its node ids and the boxes of its leaves (gathered through `order`). The first thread to reach a
+// node stops; the second (LDS counter, workgroup acq_rel) unions the children's boxes from LDS and
+// writes the node. Before the walks, the block scans its leaf boxes into range unions.
 __global__ void __launch_bounds__(256) k_refit_local(const uint32_t* __restrict__ order, uint32_t n,
                                                       const float4* __restrict__ leafbox,
                                                       const int32_t* __restrict__ parent,
