@@ -36,28 +36,33 @@ __global__ void __launch_bounds__(kSortThreads) k_sort_hist(const uint32_t* __re
 
 // Row scan: block d turns row d of the digit-major histogram (counts of digit d in tiles
 // 0..nblk-1) into its exclusive prefix sums, in place, and writes the row total to totals[d].
+// Thread t takes a contiguous run of the row: its sum, one workgroup scan of the 256 sums, then
+// the run's prefixes from its offset.
 __global__ void __launch_bounds__(kSortThreads) k_sort_rowscan(uint32_t* __restrict__ hist,
                                                                uint32_t nblk,
                                                                uint32_t* __restrict__ totals) {
     __shared__ uint32_t s[kSortThreads];
     uint32_t* row = hist + (uint64_t)blockIdx.x * nblk;
-    uint32_t run = 0;
-    for (uint32_t c0 = 0; c0 < nblk; c0 += kSortThreads) {
-        const uint32_t i = c0 + threadIdx.x;
-        const uint32_t v = i < nblk ? row[i] : 0u;
-        s[threadIdx.x] = v;
+    const uint32_t per = (nblk + kSortThreads - 1) / kSortThreads;
+    const uint32_t b = threadIdx.x * per;
+    const uint32_t e = b + per < nblk ? b + per : nblk;  // run [b, e), empty past the row's end
+    uint32_t sum = 0;
+    for (uint32_t i = b; i < e; ++i) sum += row[i];
+    s[threadIdx.x] = sum;
+    __syncthreads();
+    for (int off = 1; off < kSortThreads; off <<= 1) {  // inclusive scan of the run sums
+        const uint32_t x = threadIdx.x >= (uint32_t)off ? s[threadIdx.x - off] : 0u;
         __syncthreads();
-        for (int off = 1; off < kSortThreads; off <<= 1) {
-            const uint32_t x = threadIdx.x >= (uint32_t)off ? s[threadIdx.x - off] : 0u;
-            __syncthreads();
-            s[threadIdx.x] += x;
-            __syncthreads();
-        }
-        if (i < nblk) row[i] = run + s[threadIdx.x] - v;
-        run += s[kSortThreads - 1];
+        s[threadIdx.x] += x;
         __syncthreads();
     }
-    if (threadIdx.x == 0) totals[blockIdx.x] = run;
+    uint32_t run = s[threadIdx.x] - sum;
+    for (uint32_t i = b; i < e; ++i) {
+        const uint32_t v = row[i];
+        row[i] = run;
+        run += v;
+    }
+    if (threadIdx.x == kSortThreads - 1) totals[blockIdx.x] = s[kSortThreads - 1];
 }
 // Stable scatter of one tile of kSortTile items. Items are visited in rounds of 256 (item order =
 // (round, thread)); each round is ranked by wave64 multi-split (8 ballots give the lanes sharing
