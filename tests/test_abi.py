@@ -176,3 +176,49 @@ def test_paths_entry_rejects_bad_arguments_without_gpu():
     big = N.rt_paths_params(65536, 65536, 1, 2, 1, 0, 0.7, 1.0, 1e-4)
     assert lib.rt_trace_paths(ctypes.byref(sc), ctypes.byref(cam), ctypes.byref(big), ctypes.byref(out),
                               ws, need, None) == -3
+
+
+def test_mesh_and_image_entries_reject_bad_arguments_without_gpu():
+    """Argument checks of rt_mesh_compute_normals / rt_mesh_pack / rt_mesh_unpack /
+    rt_image_to_bgra8: each case returns RT_EINVAL (or RT_OK for empty work) before any device work,
+    so the fake device addresses below are never touched."""
+    from rendertoy_amd import _native as N
+    lib = N.load()
+    EINVAL = -1
+    dev, out = 0x10000, 0x20000
+
+    def mesh(nv, nt, vertices=dev, indices=None, stride=80, pos=0):
+        return N.rt_mesh_desc(vertices, indices, nv, nt, stride, pos, N.RT_ABSENT, N.RT_ABSENT)
+
+    def normals(m, o=out, stride=12, ws=None, ws_bytes=0):
+        return lib.rt_mesh_compute_normals(None if m is None else ctypes.byref(m), o, stride, ws, ws_bytes, None)
+
+    assert normals(None) == EINVAL
+    assert normals(mesh(0, 0)) == 0                      # nothing to do
+    assert normals(mesh(0, 1)) == EINVAL                 # triangles without vertices
+    assert normals(mesh(3, 1, vertices=None)) == EINVAL
+    assert normals(mesh(3, 1, stride=8)) == EINVAL       # stride below one position
+    assert normals(mesh(3, 1, stride=16, pos=8)) == EINVAL
+    assert normals(mesh(3, 2)) == EINVAL                 # non-indexed: 3 vertices per triangle
+    assert normals(mesh(3, 1), o=None) == EINVAL
+    assert normals(mesh(3, 1), stride=8) == EINVAL
+    assert normals(mesh(3, 1), o=out + 2) == EINVAL      # misaligned output
+    need = lib.rt_mesh_normals_workspace_bytes(4, 2)
+    assert need > 0
+    assert normals(mesh(4, 2, indices=0x30000), ws=0x40000, ws_bytes=need - 1) == EINVAL
+    assert b"workspace too small" in lib.rt_last_error()
+    assert normals(mesh(4, 2, indices=0x30000), ws=0x40010, ws_bytes=need) == EINVAL
+    assert b"256-byte" in lib.rt_last_error()
+
+    assert lib.rt_mesh_pack(None, None, None, 0, None, None) == 0
+    assert lib.rt_mesh_pack(None, None, None, 4, dev, None) == EINVAL
+    assert lib.rt_mesh_pack(dev, dev + 1, None, 4, out, None) == EINVAL
+    assert lib.rt_mesh_pack(dev, None, None, 4, out + 8, None) == EINVAL
+    assert lib.rt_mesh_unpack(None, 0, None, None, None, None) == 0
+    assert lib.rt_mesh_unpack(None, 4, dev, None, None, None) == EINVAL
+    assert lib.rt_mesh_unpack(out + 8, 4, dev, None, None, None) == EINVAL
+    assert lib.rt_mesh_unpack(out, 4, dev + 2, None, None, None) == EINVAL
+
+    assert lib.rt_image_to_bgra8(None, 0, None, None) == 0
+    assert lib.rt_image_to_bgra8(None, 4, out, None) == EINVAL
+    assert lib.rt_image_to_bgra8(dev, 4, out + 1, None) == EINVAL
