@@ -288,16 +288,20 @@ class Raycaster:
         dev = torch.device("cuda")
         # allocate on the stream the build runs on: torch's allocator orders reuse of the freed
         # workspace after the build kernels on that stream
+        # the scene's arrays are 256-byte-aligned slices of one allocation (one allocator call)
+        parts = [("_nodes", sizes.nodes_bytes, torch.uint8), ("_nodes4", sizes.nodes4_bytes, torch.uint8),
+                 ("_tris", sizes.tris_bytes, torch.uint8), ("_shade", sizes.shade_bytes, torch.uint8),
+                 ("_bounds", 32, torch.float32), ("_mesh_offsets", 4 * (len(self.models) + 1), torch.int32),
+                 ("keys", 4 * max(1, n_tris), torch.int32), ("order", 4 * max(1, n_tris), torch.int32)]
+        offsets, total = [], 0
+        for _, nbytes, _ in parts:
+            offsets.append(total)
+            total += (int(nbytes) + 255) // 256 * 256
         with torch.cuda.stream(self._torch_stream()):
-            self._nodes = torch.empty(sizes.nodes_bytes, dtype=torch.uint8, device=dev)
-            self._nodes4 = torch.empty(sizes.nodes4_bytes, dtype=torch.uint8, device=dev)
-            self._tris = torch.empty(sizes.tris_bytes, dtype=torch.uint8, device=dev)
-            self._shade = torch.empty(sizes.shade_bytes, dtype=torch.uint8, device=dev)
-            self._bounds = torch.empty(8, dtype=torch.float32, device=dev)
-            self._mesh_offsets = torch.empty(len(self.models) + 1, dtype=torch.int32, device=dev)
-            self.keys = torch.empty(max(1, n_tris), dtype=torch.int32, device=dev)
-            self.order = torch.empty(max(1, n_tris), dtype=torch.int32, device=dev)
+            arena = torch.empty(total, dtype=torch.uint8, device=dev)
             ws = torch.empty(sizes.workspace_bytes, dtype=torch.uint8, device=dev)
+        for (name, nbytes, dtype), off in zip(parts, offsets):
+            setattr(self, name, arena[off:off + int(nbytes)].view(dtype))
         sc = N.rt_scene()
         sc.n_tris = n_tris
         sc.n_meshes = len(self.models)
