@@ -67,7 +67,7 @@ def lib():
         L.tw_paths.restype = None
         L.tw_paths.argtypes = [vp, vp, i32, vp] + [ctypes.c_uint32] * 6 + [f32, f32, f32, vp, vp, vp]
         L.tw_closest64_brute.restype = None
-        L.tw_closest64_brute.argtypes = [vp, i64, i64] + [vp] * 6 + [f64, f64, f64] + [vp] * 5
+        L.tw_closest64_brute.argtypes = [vp, i64, i64] + [vp] * 6 + [f64, f64, f64, f64] + [vp] * 5
         L.tw_closest64_bvh.restype = None
         L.tw_closest64_bvh.argtypes = [vp, i64, vp, vp, i32, i64] + [vp] * 6 + [f64, f64, f64, f64] + [vp] * 5
         L.tw_tex_sample.restype = None
@@ -366,8 +366,8 @@ EDGE, DEPTH_TIE, NEAR_MISS = 1, 2, 4
 
 
 def closest64(tri_pos, o, d, tmin=0.0, edge_eps=1e-6, tie_eps=1e-5, margin=1e-4, scene=None):
-    """FP64 closest hit (brute force, or culled by `scene`'s hierarchy with FP64 boxes inflated
-    by `margin`): dict t (float64, inf on miss), prim (-1 on miss), u, v, and `edge`, a bit set of
+    """FP64 closest hit over the triangles whose FP64 box, inflated by `margin`, the ray crosses
+    (brute force over all boxes, or found through `scene`'s hierarchy: the same set): dict t (float64, inf on miss), prim (-1 on miss), u, v, and `edge`, a bit set of
     EDGE (winner within max(edge_eps, FP32 error bound) barycentric of an edge), DEPTH_TIE
     (another hit within max(tie_eps relative, FP32 t error bounds) of the winner), NEAR_MISS (a
     triangle missed by less than its FP32 barycentric error bound within that range): the rays on
@@ -385,7 +385,7 @@ def closest64(tri_pos, o, d, tmin=0.0, edge_eps=1e-6, tie_eps=1e-5, margin=1e-4,
     edge = np.empty(n, np.uint8)
     if scene is None:
         lib().tw_closest64_brute(_p(tri_pos), tri_pos.shape[0], n, *[_p(c) for c in cols], tmin,
-                                 edge_eps, tie_eps, _p(t), _p(prim), _p(u), _p(v), _p(edge))
+                                 edge_eps, tie_eps, margin, _p(t), _p(prim), _p(u), _p(v), _p(edge))
     else:
         order = np.ascontiguousarray(scene.order if scene.n else np.zeros(1, np.uint32), np.uint32)
         lib().tw_closest64_bvh(_p(tri_pos), tri_pos.shape[0], _p(scene.nodes_full), _p(order),

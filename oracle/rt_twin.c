@@ -1190,19 +1190,6 @@ static void closest64_ray(const float* tri_pos, int64_t n_tris, const double* o,
     *edge = e;
 }
 
-TW_API void tw_closest64_brute(const float* tri_pos, int64_t n_tris, int64_t n, const float* ox,
-                               const float* oy, const float* oz, const float* dx,
-                               const float* dy, const float* dz, double tmin, double edge_eps,
-                               double tie_eps, double* out_t, int32_t* out_prim, double* out_u, double* out_v,
-                               uint8_t* out_edge) {
-    #pragma omp parallel for schedule(dynamic, 16)
-    for (int64_t i = 0; i < n; ++i) {
-        double o[3] = {ox[i], oy[i], oz[i]}, d[3] = {dx[i], dy[i], dz[i]};
-        closest64_ray(tri_pos, n_tris, o, d, tmin, edge_eps, tie_eps, NULL, 0, &out_t[i], &out_prim[i],
-                      &out_u[i], &out_v[i], &out_edge[i]);
-    }
-}
-
 /* FP64 box of a traversal ref: internal node box, or the union of a leaf's slot boxes. */
 static void ref_box64(const double* nb, const double* lb, int32_t ref, double* b) {
     if (ref >= 0) {
@@ -1231,6 +1218,42 @@ static int box_hit64(const double* b, const double* o, const double* d, double t
         tn = fmax(tn, t0); tf = fmin(tf, t1);
     }
     return !(tn > tf * (1.0 + 1e-12) + 1e-300 || tf < tmin - margin);
+}
+
+/* FP64 closest hit over every triangle whose box, inflated by `margin`, the ray crosses at
+ * t >= tmin - margin (box_hit64). This is the candidate set the hierarchy version below reaches
+ * through the tree: a hit lies inside its triangle's box, so no winner or depth tie is ever left
+ * out, and a traversal only tests triangles of boxes its ray enters, so near misses are counted
+ * among those triangles only (a nearly parallel triangle far off the ray has unbounded FP32 error
+ * bounds but is never tested). */
+TW_API void tw_closest64_brute(const float* tri_pos, int64_t n_tris, int64_t n, const float* ox,
+                               const float* oy, const float* oz, const float* dx,
+                               const float* dy, const float* dz, double tmin, double edge_eps,
+                               double tie_eps, double margin, double* out_t, int32_t* out_prim,
+                               double* out_u, double* out_v, uint8_t* out_edge) {
+    double* tb = (double*)malloc(sizeof(double) * 6 * (n_tris > 0 ? n_tris : 1));
+    for (int64_t p = 0; p < n_tris; ++p) {
+        const float* f = tri_pos + p * 9;
+        for (int c = 0; c < 3; ++c) {
+            tb[p * 6 + c] = fmin(fmin((double)f[c], (double)f[3 + c]), (double)f[6 + c]) - margin;
+            tb[p * 6 + 3 + c] = fmax(fmax((double)f[c], (double)f[3 + c]), (double)f[6 + c]) + margin;
+        }
+    }
+    #pragma omp parallel
+    {
+        int64_t* cand = (int64_t*)malloc(sizeof(int64_t) * (n_tris > 0 ? n_tris : 1));
+        #pragma omp for schedule(dynamic, 16)
+        for (int64_t i = 0; i < n; ++i) {
+            double o[3] = {ox[i], oy[i], oz[i]}, d[3] = {dx[i], dy[i], dz[i]};
+            int64_t nc = 0;
+            for (int64_t p = 0; p < n_tris; ++p)
+                if (box_hit64(tb + p * 6, o, d, tmin, margin)) cand[nc++] = p;
+            closest64_ray(tri_pos, n_tris, o, d, tmin, edge_eps, tie_eps, cand, nc, &out_t[i],
+                          &out_prim[i], &out_u[i], &out_v[i], &out_edge[i]);
+        }
+        free(cand);
+    }
+    free(tb);
 }
 
 /* FP64 closest hit culled by an FP64 BVH over the same leaves: boxes are recomputed in FP64 from
