@@ -23,8 +23,14 @@ once per packet, so the bytes they actually fetch are also reported (fetched_byt
 the same counters on a copy kernel of known size), read from profiles/k_trace_camera_c2.txt when
 that summary was taken of the same kernel on the same workload. The quad-lane kernel reads a
 record once per 16-ray packet.
+`bound` is "latency": the kernel is set by the dependent node fetches of its slowest packets,
+not by HBM; `hbm_frac` = measured HBM traffic / kernel time / 8 TB/s.
 cpu_baseline: the same closest-hit traversal in the CPU restatement (C + OpenMP, all host
 threads) over full frames for about `--cpu-seconds`.
+c1: the CPU-runnable workload (81,920 triangles, 512^2, closest hit + Lambert) on the GPU and on
+the host (all threads and 1 thread, median of 5).
+accuracy: the rendered frame against the FP64 oracle (IDs, flagged rays by kind, t error, RGB
+error on unflagged pixels).
 """
 import argparse
 import json
@@ -56,13 +62,17 @@ def parse():
     ap.add_argument("--batch-reps", type=int, default=5)
     ap.add_argument("--no-serving", action="store_true",
                     help="skip the serving measurement (64 resident scenes, Poisson arrivals)")
-    ap.add_argument("--serving-requests", type=int, default=300, help="requests per arrival rate")
-    ap.add_argument("--serving-rates", default="100,500,2000",
+    ap.add_argument("--serving-requests", type=int, default=1000, help="requests per arrival rate")
+    ap.add_argument("--serving-rates", default="50,100,200,500,1000,2000",
                     help="comma-separated arrival rates (requests/s)")
     ap.add_argument("--no-incoherent", action="store_true",
                     help="skip the incoherent-ray measurement (diffuse paths, 4 spp, 2 bounces)")
     ap.add_argument("--no-ray-cast", action="store_true",
                     help="skip the ray_cast measurement (ray buffers in camera order and random)")
+    ap.add_argument("--no-c1", action="store_true",
+                    help="skip the C1 leg (81,920 triangles, 512^2, GPU and CPU at all / 1 thread)")
+    ap.add_argument("--no-accuracy", action="store_true",
+                    help="skip the FP64 accuracy report of the rendered frame")
     ap.add_argument("--no-mesh", action="store_true",
                     help="skip the mesh entry points (vertex normals, record pack/unpack) on the sphere")
     ap.add_argument("--copy-calib-mib", type=int, default=0,
@@ -95,23 +105,14 @@ def measured_traffic(workload, kernel):
 
 
 def batched_workload(n_scenes, ren, scenes, torch):
-    """The batched workload: scene i has T_i = exp(U[ln 1e3, ln 5e5]) triangles (icosphere
-    frequency round(sqrt(T_i / 20))), a random rotation about a uniform axis and a scale in
-    [0.8, 1.2], and an image of 128^2, 256^2, 512^2 or 1024^2 (uniform), all from the seed
-    3000 + i; camera (0,0,3) -> origin, 45 deg; Lambert + one shadow ray per lit hit, albedo 1."""
+    """The batched workload (scenes.batch_scene: scene i from the seed 3000 + i, 1k-500k
+    triangles, random transform, image 128^2..1024^2); camera (0,0,3) -> origin, 45 deg; Lambert
+    + one shadow ray per lit hit, albedo 1."""
     from rendertoy_amd.batch import RenderRequest
     reqs, tris = [], 0
     for i in range(n_scenes):
-        rng = np.random.default_rng(3000 + i)
-        T = math.exp(rng.uniform(math.log(1e3), math.log(5e5)))
-        f = max(1, int(round(math.sqrt(T / 20.0))))
-        res = int(rng.choice([128, 256, 512, 1024]))
-        M = scenes.random_transform(rng)
-        d = scenes.icosphere(f, 3000 + i)
-        pos = (d["pos"].astype(np.float64) @ M).astype(np.float32)
-        nrm = d["nrm"].astype(np.float64) @ M
-        nrm = (nrm / np.linalg.norm(nrm, axis=1, keepdims=True)).astype(np.float32)
-        rc = ren.Raycaster([ren.mesh_from_arrays(pos, d["idx"], nrm, d["uv"])])
+        d, res = scenes.batch_scene(i)
+        rc = ren.Raycaster([ren.mesh_from_arrays(d["pos"], d["idx"], d["nrm"], d["uv"])])
         cam = ren.Camera((0.0, 0.0, 3.0), (0.0, 0.0, 0.0), (0.0, 1.0, 0.0), math.pi / 4, 1.0)
         reqs.append(RenderRequest(rc, cam, res, res))
         tris += rc.n_tris
@@ -136,6 +137,13 @@ def measure_batched(args, ren, scenes, torch):
         for k, r in enumerate(reqs):
             r.raycaster.render(r.camera, r.width, r.height, out=plain_imgs[k], stats=plain_stats[k])
     plain_ms = time_launches(plain, args.batch_reps)
+    # the same 256 plain launches replayed from one captured graph: no host launch overhead
+    plain()
+    torch.cuda.synchronize()
+    pg = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(pg):
+        plain()
+    plain_graph_ms = time_launches(pg.replay, args.batch_reps)
     b = BatchRender(reqs)
     best, table = autotune(b, reps=2)
     b.capture()
@@ -151,6 +159,8 @@ def measure_batched(args, ren, scenes, torch):
         "scenes": len(reqs), "triangles": int(tris), "primary_rays": int(b.rays_primary),
         "rays": rays,
         "plain_ms_p50": round(p_med, 3), "plain_ms_p99": round(percentile(plain_ms, 99), 3),
+        "plain_graph_ms_p50": round(float(np.median(plain_graph_ms)), 3),
+        "speedup_vs_plain_graph": round(float(np.median(plain_graph_ms)) / b_med, 3),
         "batched_ms_p50": round(b_med, 3), "batched_ms_p99": round(percentile(batch_ms, 99), 3),
         "plain_mrays_per_s": round(rays / (p_med * 1e-3) / 1e6, 2),
         "batched_mrays_per_s": round(rays / (b_med * 1e-3) / 1e6, 2),
@@ -228,65 +238,68 @@ def measure_mesh(args, ren, sphere, torch):
     return res
 
 
-def incoherent_scene(scenes):
-    """8 displaced icospheres of frequency 112 (250,880 triangles each), sphere k scaled by 0.35
-    and centred at a point uniform in [-1, 1]^3, from the seed 5000 + k."""
-    meshes = []
-    for k in range(8):
-        rng = np.random.default_rng(5000 + k)
-        c = rng.uniform(-1.0, 1.0, 3)
-        d = dict(scenes.icosphere(112, 5000 + k))
-        d["pos"] = (d["pos"].astype(np.float64) * 0.35 + c).astype(np.float32)
-        meshes.append(d)
-    return meshes
-
-
 def measure_incoherent(args, ren, scenes, torch):
     """Diffuse paths at 1920x1080, 4 samples per pixel, primary + 2 cosine-weighted bounces
-    (Philox4x32-10 key (0x5EED, frame)); rays/s counts the rays actually traced (a path stops
-    when it escapes)."""
+    (Philox4x32-10 key (0x5EED, frame)) on the 8-sphere cluster (scenes.cluster_meshes); rays/s
+    counts the rays actually traced (a path stops when it escapes). Per depth: frames with 0, 1 and
+    2 bounces are timed alike, and the rays and time a depth adds give its rate. Reported for the
+    workload's view (0,0,3) and for scenes.cluster_view(), which frames one sphere so nearly every
+    camera ray hits and bounces."""
     from rendertoy_amd.batch import time_launches
-    meshes = incoherent_scene(scenes)
+    meshes = scenes.cluster_meshes()
     rc = ren.Raycaster([ren.mesh_from_arrays(m["pos"], m["idx"], m["nrm"], m["uv"]) for m in meshes])
     W, H, spp, bounces = 1920, 1080, 4, 2
-    cam = ren.Camera((0.0, 0.0, 3.0), (0.0, 0.0, 0.0), (0.0, 1.0, 0.0), math.pi / 4, W / H)
     out = torch.empty((H, W), dtype=torch.float32, device="cuda")
-    counts = torch.zeros(bounces + 1, dtype=torch.int32, device="cuda")
-    frame = [0]
+    res = {"what": "8 spheres x 250,880 triangles, 1920x1080 x 4 spp, primary + 2 diffuse bounces",
+           "triangles": int(rc.n_tris), "nominal_rays_per_frame": W * H * spp * (bounces + 1)}
+    views = {"workload": ((0.0, 0.0, 3.0), math.pi / 4, (0.0, 0.0, 0.0)), "framed": scenes.cluster_view()}
+    for name, (eye, fov, target) in views.items():
+        cam = ren.Camera(eye, target, (0.0, 1.0, 0.0), fov, W / H)
+        per_b = {}
+        for b in range(bounces + 1):
+            counts = torch.zeros(b + 1, dtype=torch.int32, device="cuda")
+            frame = [0]
 
-    def step():
-        frame[0] += 1
-        rc.trace_paths(cam, W, H, spp=spp, bounces=bounces, frame=frame[0], out=out, counts=counts)
-    ms = time_launches(step, max(5, args.steps // 5), warmup=2)
-    cnt = counts.cpu().numpy().astype(np.int64)
-    rays = int(cnt.sum())
-    med = float(np.median(ms))
-    return {"what": "8 spheres x 250,880 triangles, 1920x1080 x 4 spp, primary + 2 diffuse bounces",
-            "triangles": int(rc.n_tris), "ms_per_frame_p50": round(med, 3),
-            "ms_per_frame_p99": round(percentile(ms, 99), 3),
-            "rays_per_frame": rays, "rays_by_depth": [int(x) for x in cnt],
-            "nominal_rays_per_frame": W * H * spp * (bounces + 1),
-            "mrays_per_s": round(rays / (med * 1e-3) / 1e6, 2)}
+            def step():
+                frame[0] += 1
+                rc.trace_paths(cam, W, H, spp=spp, bounces=b, frame=frame[0], out=out, counts=counts)
+            ms = time_launches(step, max(5, args.steps // 5), warmup=2)
+            per_b[b] = (float(np.median(ms)), percentile(ms, 99), counts.cpu().numpy().astype(np.int64))
+        med, p99, cnt = per_b[bounces]
+        depth_rate = []
+        for b in range(bounces + 1):
+            dt = per_b[b][0] - (per_b[b - 1][0] if b else 0.0)
+            depth_rate.append(round(int(cnt[b]) / (dt * 1e-3) / 1e6, 1) if dt > 0 else None)
+        res[name] = {"eye": [round(x, 4) for x in eye], "fov": round(fov, 4),
+                     "ms_per_frame_p50": round(med, 3), "ms_per_frame_p99": round(p99, 3),
+                     "rays_per_frame": int(cnt.sum()), "rays_by_depth": [int(x) for x in cnt],
+                     "mrays_per_s": round(int(cnt.sum()) / (med * 1e-3) / 1e6, 2),
+                     "ms_by_bounces": [round(per_b[b][0], 3) for b in range(bounces + 1)],
+                     "mrays_per_s_by_depth": depth_rate}
+    # the workload's figures at top level as before
+    for k in ("ms_per_frame_p50", "ms_per_frame_p99", "rays_per_frame", "rays_by_depth", "mrays_per_s"):
+        res[k] = res["workload"][k]
+    return res
 
 
 def serving_pool(n_scenes, scenes):
-    """The serving pool: scene k has T = exp(U[ln 1e4, ln 2e6]) triangles (icosphere frequency
-    round(sqrt(T / 20))) and a 512^2 RGBA8 value-noise texture, from the seeds 4000 + k / 4100 + k.
-    Host data is generated when the scene is first made resident."""
+    """The serving pool (scenes.serving_scene: scene k from the seeds 4000 + k / 4100 + k, 10k-2M
+    triangles, 512^2 RGBA8 texture); host data is generated when the scene is first made resident."""
     from rendertoy_amd.serving import SceneSpec
 
     def make(k):
-        rng = np.random.default_rng(4000 + k)
-        T = math.exp(rng.uniform(math.log(1e4), math.log(2e6)))
-        f = max(1, int(round(math.sqrt(T / 20.0))))
-        return SceneSpec([scenes.icosphere(f, 4000 + k)], scenes.value_noise_rgba8(512, 4100 + k))
+        d, tex = scenes.serving_scene(k)
+        return SceneSpec([d], tex)
     return {k: (lambda k=k: make(k)) for k in range(n_scenes)}
 
 
 def measure_serving(args, ren, scenes, torch):
     """Requests (scene uniform over 64 resident scenes, eye 3d with d uniform on the sphere, image
-    256^2 / 512^2 / 1024^2) arriving as a Poisson process at each rate, served by the batching
-    loop; latency = GPU completion of the request's batch - arrival."""
+    256^2 / 512^2 / 1024^2) arriving as a Poisson process at each rate, served by the continuous-
+    batching loop (RenderServer: batches on 4 streams, at most 4 in flight) and by the per-request
+    loop (PlainServer: one rt_render per request as it arrives) on the same traces; latency = GPU
+    completion - arrival. `saturation`: every request present at t = 0 (throughput = requests /
+    makespan, the capacity of each loop)."""
     from rendertoy_amd import serving
     from rendertoy_amd.batch import time_launches
     g0 = time.time()
@@ -296,7 +309,6 @@ def measure_serving(args, ren, scenes, torch):
     torch.cuda.synchronize()
     setup_s = time.time() - g0
     tris = sum(r.raycaster.n_tris for r in store.resident.values())
-    # one request alone (no queueing) per image size, scene 0
     res = store.acquire(0)
     alone = {}
     for size in (256, 512, 1024):
@@ -305,19 +317,121 @@ def measure_serving(args, ren, scenes, torch):
         alone[str(size)] = round(float(np.median(ms)), 3)
     store.release(res)
     srv = serving.RenderServer(store)
+    plain = serving.PlainServer(store)
     rates = {}
     for rate in [float(x) for x in args.serving_rates.split(",") if x]:
-        reqs = serving.make_requests(args.serving_requests, 64, rate, 4000 + int(rate))
-        srv.batches = []
-        srv.run(reqs)
-        sm = serving.latency_summary(reqs)
-        sm["mean_batch"] = round(float(np.mean(srv.batches)), 2)
-        rates[str(int(rate))] = sm
-    return {"what": "64 resident scenes, 512^2 RGBA8 textures, Poisson arrivals, images 256^2..1024^2",
+        row = {}
+        for name, server in (("batched", srv), ("plain", plain)):
+            reqs = serving.make_requests(args.serving_requests, 64, rate, 4000 + int(rate))
+            if name == "batched":
+                srv.batches = []
+            server.run(reqs)
+            sm = serving.latency_summary(reqs)
+            if name == "batched":
+                sm["mean_batch"] = round(float(np.mean(srv.batches)), 2)
+            row[name] = sm
+        rates[str(int(rate))] = row
+    sat = {}
+    for name, server in (("batched", srv), ("plain", plain)):
+        reqs = serving.make_requests(args.serving_requests, 64, 1e9, 4999)
+        if name == "batched":
+            srv.batches = []
+        server.run(reqs, realtime=False)
+        sm = serving.latency_summary(reqs, realtime=False)
+        if name == "batched":
+            sm["mean_batch"] = round(float(np.mean(srv.batches)), 2)
+        sat[name] = sm
+    sat["throughput_ratio"] = round(sat["batched"]["throughput_rps"] / sat["plain"]["throughput_rps"], 3)
+    return {"what": "64 resident scenes, 512^2 RGBA8 textures, Poisson arrivals, images 256^2..1024^2; "
+                    "continuous batching on 4 streams vs one rt_render per request",
             "triangles_resident": int(tris), "resident_bytes": int(store.bytes),
-            "alone_render_ms": alone, "rates_rps": rates,
+            "alone_render_ms": alone, "rates_rps": rates, "saturation": sat,
             "store": {"hits": store.hits, "misses": store.misses, "evictions": store.evictions},
             "setup_s": round(setup_s, 1)}
+
+
+def measure_c1(args, ren, scenes, torch):
+    """The CPU-runnable workload: icosphere f=64 (81,920 triangles, seed 1000, positions only),
+    512x512, 1 ray per pixel from (0,0,3), 45 deg; closest hit + Lambert (L = normalize(1,1,1),
+    ambient 0.2), no shadow, no texture. GPU: rt_render; CPU: the twin's per-ray closest hit
+    (same BVH, binary nodes) + the same Lambert in numpy, on all host threads and on 1 thread,
+    median of 5 runs each."""
+    import oracle
+    from rendertoy_amd.batch import time_launches
+    d = scenes.icosphere(64, 1000)
+    rc = ren.Raycaster([ren.mesh_from_arrays(d["pos"], d["idx"])])
+    W = H = 512
+    cam = ren.Camera((0.0, 0.0, 3.0), (0.0, 0.0, 0.0), (0.0, 1.0, 0.0), math.pi / 4, 1.0)
+    light = (np.array([1, 1, 1], np.float32) / np.float32(np.sqrt(3.0))).astype(np.float32)
+    img = torch.empty((H, W, 3), dtype=torch.float32, device="cuda")
+    ms = time_launches(lambda: rc.render(cam, W, H, light=light, shadows=False, out=img), 50, warmup=5)
+    gpu_med = float(np.median(ms))
+    pos, nrm, uv, mo = scenes.triangle_arrays([{"pos": d["pos"], "idx": d["idx"], "nrm": None, "uv": None}])
+    tw = oracle.TwinScene(pos, nrm, uv, mo, [0], [0])
+    o, dd = oracle.twin_generate_rays(cam.as_array(), W, H)
+    geo = np.cross(pos[:, 3:6] - pos[:, 0:3], pos[:, 6:9] - pos[:, 0:3]).astype(np.float32)
+    geo /= np.linalg.norm(geo, axis=1, keepdims=True)
+
+    def cpu_frame():
+        h = tw.trace_closest(o, dd)
+        hit = h["prim"] >= 0
+        ndl = np.maximum(geo[h["prim"][hit]] @ light, 0)
+        rgb = np.zeros((W * H, 3), np.float32)
+        rgb[hit] = (0.2 + ndl)[:, None]
+        return rgb
+    threads = host_threads()
+    out = {}
+    for nt in (threads, 1):
+        oracle.set_threads(nt)
+        runs = []
+        for _ in range(5):
+            c0 = time.perf_counter()
+            cpu_frame()
+            runs.append(time.perf_counter() - c0)
+        out[nt] = float(np.median(runs))
+    oracle.set_threads(threads)
+    n = W * H
+    return {"what": "icosphere f=64 (81,920 triangles), 512x512, closest hit + Lambert, no shadow",
+            "gpu_ms_p50": round(gpu_med, 4), "gpu_mrays_per_s": round(n / gpu_med / 1e3, 1),
+            "cpu_threads": threads, "cpu_ms_median_of_5": round(out[threads] * 1e3, 2),
+            "cpu_mrays_per_s": round(n / out[threads] / 1e6, 2),
+            "cpu_1thread_ms_median_of_5": round(out[1] * 1e3, 2),
+            "cpu_1thread_mrays_per_s": round(n / out[1] / 1e6, 3),
+            "gpu_over_cpu": round(out[threads] * 1e3 / gpu_med, 1)}
+
+
+def host_threads():
+    threads = len(os.sched_getaffinity(0))
+    if os.environ.get("OMP_NUM_THREADS", "").isdigit():
+        threads = min(threads, int(os.environ["OMP_NUM_THREADS"]))
+    return threads
+
+
+def accuracy_report(tw, meshes, scenes, oracle, cam, W, H, prim, t, img, light, tex):
+    """The FP64 accuracy bar on the rendered C2 frame: primary IDs against the FP64 closest hit
+    (counted mismatches, unflagged mismatches, flags by kind), t relative error where the IDs
+    agree (max, mean), and the RGB error on unflagged pixels against FP64 shading."""
+    pos, nrm, uv, mo = scenes.triangle_arrays(meshes)
+    o, d = oracle.twin_generate_rays(cam.as_array(), W, H)
+    r64 = tw.closest64(o, d)
+    flag = r64["edge"]
+    mism = prim != r64["prim"]
+    agree = (prim >= 0) & ~mism
+    rel = np.abs(t[agree].astype(np.float64) - r64["t"][agree]) / r64["t"][agree]
+    rgb64, sflags = oracle.shade64(pos, nrm, uv, mo, [True, True], o, d, light, textures=[tex],
+                                   mesh_tex=[0, 0], scene=tw, primary=r64)
+    ok = (sflags == 0) & ~mism
+    err = np.abs(img.reshape(-1, 3).astype(np.float64) - rgb64)[ok]
+    return {"what": f"{W}x{H} render of the workload vs the FP64 oracle",
+            "rays": int(len(prim)), "hits": int((prim >= 0).sum()),
+            "id_mismatch": int(mism.sum()), "unflagged_mismatch": int((mism & (flag == 0)).sum()),
+            "flag_edge": int(((flag & oracle.EDGE) != 0).sum()),
+            "flag_depth_tie": int(((flag & oracle.DEPTH_TIE) != 0).sum()),
+            "flag_near_miss": int(((flag & oracle.NEAR_MISS) != 0).sum()),
+            "t_rel_max": float(rel.max()) if rel.size else 0.0,
+            "t_rel_mean": float(rel.mean()) if rel.size else 0.0,
+            "rgb_max_err_unflagged": float(err.max()) if err.size else 0.0,
+            "rgb_pixels_checked": int(ok.sum()), "shadow_flagged": int(((sflags & 8) != 0).sum())}
 
 
 def percentile(a, q):
@@ -391,22 +505,26 @@ def main():
     prim = out["prim"].cpu().numpy()
     hits = int((prim >= 0).sum())
 
-    # ---- full render latency (primary + shadow + shade) ----
+    # ---- full render latency (primary + shadow + shade), texture bound once ----
     texg = torch.from_numpy(tex).to(dev)
+    binding = ren._raycaster.TextureBinding([texg], [0, 0])
     light = (np.array([1, 1, 1], np.float32) / np.float32(np.sqrt(3.0))).astype(np.float32)
     img = torch.empty((H, W, 3), dtype=torch.float32, device=dev)
     stats = torch.zeros(4, dtype=torch.int32, device=dev)
     for _ in range(args.warmup):
-        rc.render(cam, W, H, light=light, textures=[texg], mesh_textures=[0, 0], out=img, stats=stats)
+        rc.render(cam, W, H, light=light, textures=binding, out=img, stats=stats)
     rev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
            for _ in range(args.steps)]
     torch.cuda.synchronize()
     for a, b in rev:
         a.record(stream)
-        rc.render(cam, W, H, light=light, textures=[texg], mesh_textures=[0, 0], out=img, stats=stats)
+        rc.render(cam, W, H, light=light, textures=binding, out=img, stats=stats)
         b.record(stream)
     torch.cuda.synchronize()
     render_ms = [a.elapsed_time(b) for a, b in rev]
+    _, rhits = rc.render(cam, W, H, light=light, textures=binding, out=img, return_hits=True)
+    r_prim, r_t = rhits["prim"].cpu().numpy(), rhits["t"].cpu().numpy()
+    r_img = img.cpu().numpy()
     # export of the frame to the reference's BGRA8 render-target format (12 B read, 4 B written per px)
     from rendertoy_amd import image as rt_image
     from rendertoy_amd.batch import time_launches
@@ -460,9 +578,11 @@ def main():
         result["ray_cast"] = measure_ray_cast(args, rc, cam, W, H, torch)
     if not args.no_mesh:
         result["mesh"] = measure_mesh(args, ren, sphere, torch)
+    if not args.no_c1:
+        result["c1"] = measure_c1(args, ren, scenes, torch)
 
     # ---- roofline and CPU baseline from the CPU restatement (oracle) ----
-    if not args.no_roofline or not args.no_cpu_baseline:
+    if not args.no_roofline or not args.no_cpu_baseline or not args.no_accuracy:
         import oracle
         tw_t0 = time.time()
         pos, nrm, uv, mo = scenes.triangle_arrays(meshes)
@@ -478,11 +598,16 @@ def main():
             fetched = 16 * n_rays + 128 * int(wv["wave_nodes"].sum()) + 48 * int(wv["wave_tris"].sum()) \
                 + 48 * twin_hits
             achieved = algo_bytes / (ms_per_step * 1e-3) / 1e9
+            traffic = measured_traffic(result["config"]["workload"],
+                                       "k_trace_camera_quad" if quad else "k_trace_camera")
             result["roofline"] = {
-                "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                # the kernel is bound by the dependent node fetches of its slowest packets (SQ
+                # counters: ~40% issue, ~39% memory wait), not by HBM: hbm_frac is what reaches HBM
+                "bound": "latency", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 5),
-                "traffic": measured_traffic(result["config"]["workload"],
-                                            "k_trace_camera_quad" if quad else "k_trace_camera"),
+                "traffic": traffic,
+                "hbm_gbs": None if traffic is None else round(traffic / (ms_per_step * 1e-3) / 1e9, 1),
+                "hbm_frac": None if traffic is None else round(traffic / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 5),
                 "algorithmic_bytes_per_launch": int(algo_bytes),
                 "bytes_model": "per ray: 16 B hit + 128 B per 4-wide node record + 48 B per "
                                "triangle record its ray tests; + 48 B per hit (FP64 refinement)",
@@ -492,11 +617,12 @@ def main():
                 "kernel": "k_trace_camera_quad" if quad else "k_trace_camera",
             }
             result["parity_primary_ids_equal_twin"] = bool(np.array_equal(prim, wv["prim"]))
+        if not args.no_accuracy:
+            result["accuracy"] = accuracy_report(tw, meshes, scenes, oracle, cam, W, H, r_prim, r_t,
+                                                 r_img, light, tex)
         if not args.no_cpu_baseline:
             o, d = oracle.twin_generate_rays(cam.as_array(), W, H)
-            threads = len(os.sched_getaffinity(0))
-            if os.environ.get("OMP_NUM_THREADS", "").isdigit():
-                threads = min(threads, int(os.environ["OMP_NUM_THREADS"]))
+            threads = host_threads()
             oracle.set_threads(threads)
             reps, c0 = 0, time.perf_counter()
             while True:

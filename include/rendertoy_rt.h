@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define RT_ABI_VERSION 9
+#define RT_ABI_VERSION 10
 
 enum {
     RT_OK = 0,
@@ -71,6 +71,7 @@ typedef struct rt_scene {
     void* shade;              /* device, n_tris * 64 bytes (N0,N1,N2,UV0,UV1,UV2,mesh) */
     int32_t* mesh_offsets;    /* device, n_meshes+1 prefix sums of triangle counts */
     float* bounds;            /* device, 8 floats: scene AABB lo.xyz, 0, hi.xyz, 0 */
+    int32_t* prim_slot;       /* device, n_tris: the triangle-record slot of each global primitive id */
 } rt_scene;
 
 #define RT_EMPTY_ROOT ((int32_t)0x7FFFFFFF)
@@ -78,6 +79,7 @@ typedef struct rt_scene {
 /* Per-call byte sizes of the scene arrays and of the build workspace for n_tris triangles. */
 typedef struct rt_scene_sizes {
     uint64_t nodes_bytes, nodes4_bytes, tris_bytes, shade_bytes, bounds_bytes, workspace_bytes;
+    uint64_t prim_slot_bytes;
 } rt_scene_sizes;
 
 /* Rays as field pointers plus one byte stride shared by all fields: SoA arrays use stride 4,
@@ -154,13 +156,18 @@ enum {
     RT_TRAVERSAL_AUTO = 3,   /* camera rays only: PACKET when an 8x8 pixel tile spans at most 0.009
                                 of the image plane at unit distance (16 tan(fov/2) / height, e.g.
                                 45 deg at >= 768 rows), else LANE; decided per image */
-    RT_TRAVERSAL_QUAD = 4    /* camera rays: packets of 16 rays (4x4 pixels), 4 lanes per ray, each
+    RT_TRAVERSAL_QUAD = 4,   /* camera rays: packets of 16 rays (4x4 pixels), 4 lanes per ray, each
                                 lane testing one child box / one leaf triangle (the default) */
+    RT_TRAVERSAL_BINARY = 5  /* ray buffers (rt_trace_closest / rt_trace_any): each lane walks its
+                                own ray over the binary nodes, nearer child first, per-lane stack in
+                                LDS (the default for ray buffers) */
 };
 #define RT_LANE_STACK_MAX 32
 
-/* Launch choices of rt_trace_camera / rt_render (process-wide, read at every launch; no
- * counterpart in the reference, whose kernels take a fixed 32-thread group, _core.py:286). */
+/* Launch choices of the build, trace and render entry points (no counterpart in the reference,
+ * whose kernels take a fixed 32-thread group, _core.py:286). Every entry point that uses them takes
+ * a `const rt_tuning*`: the caller's choices for that call, or NULL for the process default set by
+ * rt_set_tuning (so two callers on two streams never share mutable state). */
 typedef struct rt_tuning {
     int32_t primary;          /* RT_TRAVERSAL_* for camera rays (default QUAD) */
     int32_t shadow;           /* RT_TRAVERSAL_* for rt_render's shadow rays (default LANE); not
@@ -171,6 +178,15 @@ typedef struct rt_tuning {
                                  needing more is traced again by its wave with the packet traversal */
     uint32_t packet_budget;   /* RT_TRAVERSAL_HYBRID: packet steps before the per-lane walk (256) */
     uint32_t builder;         /* RT_BUILD_* hierarchy of rt_scene_build (default LBVH) */
+    int32_t query;            /* RT_TRAVERSAL_BINARY (default) or RT_TRAVERSAL_LANE (per-lane walk of
+                                 the 4-wide nodes) for rt_trace_closest / rt_trace_any; both keep
+                                 their stacks in LDS (lane_stack entries per lane) and trace a ray
+                                 that needs more again with the packet traversal */
+    uint32_t handoff_steps;   /* rt_trace_camera with a workspace (QUAD): node steps a 16-ray packet
+                                 takes before it hands the rest of its walk to other waves (0 = never,
+                                 the default) */
+    uint32_t item_steps;      /* node steps a handed-off subtree is walked before what is left of it
+                                 is handed on again in pieces (>= 1; default 32) */
 } rt_tuning;
 
 /* Hierarchy builders of rt_scene_build; both start from the same Morton order and give the same
@@ -187,9 +203,13 @@ enum {
 
 /* ---- library ---- */
 int rt_abi_version(void);
+/* Identity of the sources the library was built from (a hash of its source files), so a loader can
+ * refuse a library older than the sources next to it. */
+const char* rt_build_id(void);
 const char* rt_last_error(void);
-int rt_get_tuning(rt_tuning* out);
-int rt_set_tuning(const rt_tuning* tuning);   /* RT_EINVAL if a field is out of range */
+int rt_get_tuning(rt_tuning* out);            /* the process default */
+int rt_set_tuning(const rt_tuning* tuning);   /* sets the process default; RT_EINVAL if a field is
+                                                 out of range (as for a per-call rt_tuning) */
 
 /* ---- scene build: rendering/_raycaster.py:26-33 (Raycaster.__init__ / _build_ads) ---- */
 
@@ -205,18 +225,33 @@ int rt_scene_sizes_for(uint32_t n_tris, rt_scene_sizes* out);
  * slot (for LBVH the Morton order itself). */
 int rt_scene_build(const rt_mesh_desc* meshes, uint32_t n_meshes, rt_scene* scene,
                    void* workspace_dev, uint64_t workspace_bytes,
-                   uint32_t* keys_out, uint32_t* order_out, void* stream);
+                   uint32_t* keys_out, uint32_t* order_out, const rt_tuning* tuning, void* stream);
+
+/* Export of a built scene in the reference's record types (rendering/_raycaster.py:8-22):
+ *  BVH_AABB     {float3 min @0; float3 max @16; int count @32; int elements @36} (48 B),
+ *  BVH_Triangle {float3 v0 @0; float3 v1 @16; float3 v2 @32; int index @48; int mesh @52} (64 B).
+ * `aabb_out` receives 2*n_tris - 1 nodes (1 for an empty scene): node 0 is the root with the scene
+ * bounds; an internal node has count 0 and its two children at `elements` and `elements` + 1; a
+ * leaf has count 1..4 and its triangles at `elements` .. `elements` + count - 1 of `tris_out`
+ * (n_tris records in leaf order; index = triangle within its mesh, mesh = position in the mesh
+ * list, as the hit identity). Boxes are the hierarchy's conservative (padded) child boxes, so every
+ * triangle lies inside each box on its path; v0, v1, v2 are read from the meshes the scene was
+ * built from (the same descriptors). An empty scene exports one node with count 0, elements -1, min = +inf,
+ * max = -inf. Both outputs must be 16-byte aligned. `n_nodes_out` (device, one uint32, may be
+ * NULL) receives the number of node records written, 2 L - 1 for L leaves. */
+int rt_scene_export(const rt_scene* scene, const rt_mesh_desc* meshes, uint32_t n_meshes,
+                    void* aabb_out, void* tris_out, uint32_t* n_nodes_out, void* stream);
 
 /* ---- queries: rendering/_raycaster.py:35-36 (Raycaster.ray_cast) ---- */
 
 /* Closest hit of n rays (two-sided Moller-Trumbore, hit interval (tmin, tmax), ties in t go to
  * the smaller global primitive id). */
 int rt_trace_closest(const rt_scene* scene, const rt_rays* rays, uint32_t n,
-                     const rt_hits* hits, void* stream);
+                     const rt_hits* hits, const rt_tuning* tuning, void* stream);
 
 /* Any hit (occlusion) of n rays in (tmin, tmax); occluded[i] = 1 if some triangle is hit. */
 int rt_trace_any(const rt_scene* scene, const rt_rays* rays, uint32_t n,
-                 uint8_t* occluded_dev, void* stream);
+                 uint8_t* occluded_dev, const rt_tuning* tuning, void* stream);
 
 /* Camera rays of a W x H image written as rays (SoA, stride 4), for inspection and tests. */
 int rt_generate_rays(const rt_camera* cam, uint32_t width, uint32_t height,
@@ -224,9 +259,15 @@ int rt_generate_rays(const rt_camera* cam, uint32_t width, uint32_t height,
                      void* stream);
 
 /* Closest hit of the primary rays of a W x H image (ray generation fused into the traversal);
- * hit record of pixel (col,row) at index row*W + col. */
+ * hit record of pixel (col,row) at index row*W + col. `workspace_dev` (rt_trace_camera_workspace_bytes
+ * bytes, 256-byte aligned, any contents) lets the slowest 16-ray packets of the QUAD traversal hand
+ * their remaining subtrees to other waves (rt_tuning.handoff_steps, item_steps); NULL traces every
+ * packet to its end in its own wave. Results are identical either way. Calls sharing a workspace
+ * must not overlap (one workspace per stream). */
+uint64_t rt_trace_camera_workspace_bytes(uint32_t width, uint32_t height);
 int rt_trace_camera(const rt_scene* scene, const rt_camera* cam, uint32_t width, uint32_t height,
-                    const rt_hits* hits, void* stream);
+                    const rt_hits* hits, void* workspace_dev, uint64_t workspace_bytes,
+                    const rt_tuning* tuning, void* stream);
 
 /* Full render in one kernel: per 8x8 pixel tile, ray generation, closest-hit trace, FP64 hit
  * refinement, Lambert with interpolated normals, one shadow ray toward the light per lit hit,
@@ -236,7 +277,7 @@ int rt_trace_camera(const rt_scene* scene, const rt_camera* cam, uint32_t width,
 uint64_t rt_render_workspace_bytes(uint32_t n_pixels);
 int rt_render(const rt_scene* scene, const rt_camera* cam, const rt_render_params* params,
               const rt_render_out* out, void* workspace_dev, uint64_t workspace_bytes,
-              void* stream);
+              const rt_tuning* tuning, void* stream);
 
 /* ---- batched render: many independent (scene, camera, image) jobs in one persistent launch ----
  * No reference counterpart (the reference renders one image per call). One job = one rt_render
@@ -310,7 +351,7 @@ typedef struct rt_paths_out {
 uint64_t rt_paths_workspace_bytes(uint32_t width, uint32_t height, uint32_t spp);
 int rt_trace_paths(const rt_scene* scene, const rt_camera* cam, const rt_paths_params* params,
                    const rt_paths_out* out, void* workspace_dev, uint64_t workspace_bytes,
-                   void* stream);
+                   const rt_tuning* tuning, void* stream);
 
 /* ---- mesh: rendering/_modeling.py:48-49 (Mesh.compute_normals, a stub in the reference) ----
  * Area-weighted per-vertex normals: n(v) = sum of cross(P1 - P0, P2 - P0) over the (triangle,
@@ -340,6 +381,12 @@ int rt_mesh_unpack(const void* records, uint32_t n_vertices, float* pos, float* 
  * n_pixels float RGB triples (e.g. rt_render's output) -> 4-byte B, G, R, A pixels with the OpenCL
  * UNORM_INT8 write conversion convert_uchar_sat_rte(f * 255) (NaN -> 0) and A = 255. */
 int rt_image_to_bgra8(const float* rgb, uint32_t n_pixels, uint8_t* bgra, void* stream);
+
+/* ---- textures: rendering/_core.py:551-578 (create_texture2D's default float4 texels) ----
+ * n_texels float4 RGBA texels (16 B, the reference's texture format) -> RGBA8 texels (R, G, B, A
+ * bytes) with the same UNORM_INT8 conversion, so a texture made by create_texture2D(w, h) can be
+ * bound to rt_render once converted; `rgba` must be 16-byte aligned. */
+int rt_texels_to_rgba8(const float* rgba, uint64_t n_texels, uint8_t* out, void* stream);
 
 #ifdef __cplusplus
 }

@@ -396,15 +396,16 @@ def closest64(tri_pos, o, d, tmin=0.0, edge_eps=1e-6, tie_eps=1e-5, margin=1e-4,
 
 def shade64(tri_pos, tri_nrm, tri_uv, mesh_of, has_nrm, o, d, light, ambient=0.2,
             background=(0.0, 0.0, 0.0), shadow_tmin=1e-4, shadows=True, textures=(),
-            mesh_tex=None, scene=None):
+            mesh_tex=None, scene=None, primary=None):
     """FP64 shading of rays o + t d (the device's shading formula evaluated in FP64 on the FP64
     closest hit): rgb [n, 3] float64 and flags [n] (bit set of the primary ray's FP64 flags,
     plus 8 when the shadow ray is ambiguous). rgb = albedo * (ambient + V * max(0, N.L)), N the
     normalised barycentric interpolation (w0, u, v) of the vertex normals (geometric normal for
     meshes without normals), albedo the OpenCL linear filter of the mesh's RGBA8 texture (1 when
-    none), V = 0 when the shadow ray from the hit point toward L hits anything beyond shadow_tmin."""
+    none), V = 0 when the shadow ray from the hit point toward L hits anything beyond shadow_tmin.
+    `primary`: the closest64 result of the same rays, if already computed."""
     tri_pos = np.ascontiguousarray(tri_pos, np.float32)
-    prim = closest64(tri_pos, o, d, scene=scene)
+    prim = primary if primary is not None else closest64(tri_pos, o, d, scene=scene)
     n = len(o)
     rgb = np.tile(np.asarray(background, np.float64), (n, 1))
     flags = prim["edge"].astype(np.int32)

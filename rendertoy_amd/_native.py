@@ -12,7 +12,7 @@ import torch  # noqa: F401  (must be imported before the HIP library; see module
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "librtx_hip.so")
 
-RT_ABI_VERSION = 9
+RT_ABI_VERSION = 10
 RT_ABSENT = 0xFFFFFFFF
 RT_EMPTY_ROOT = 0x7FFFFFFF
 RT_RENDER_SHADOWS = 1
@@ -22,6 +22,7 @@ RT_TRAVERSAL_LANE = 1
 RT_TRAVERSAL_HYBRID = 2
 RT_TRAVERSAL_AUTO = 3
 RT_TRAVERSAL_QUAD = 4
+RT_TRAVERSAL_BINARY = 5
 RT_LANE_STACK_MAX = 32
 RT_BUILD_LBVH = 0
 RT_BUILD_PLOC = 1
@@ -59,6 +60,7 @@ class rt_scene(ctypes.Structure):
         ("shade", ctypes.c_void_p),
         ("mesh_offsets", ctypes.c_void_p),
         ("bounds", ctypes.c_void_p),
+        ("prim_slot", ctypes.c_void_p),
     ]
 
 
@@ -70,6 +72,7 @@ class rt_scene_sizes(ctypes.Structure):
         ("shade_bytes", ctypes.c_uint64),
         ("bounds_bytes", ctypes.c_uint64),
         ("workspace_bytes", ctypes.c_uint64),
+        ("prim_slot_bytes", ctypes.c_uint64),
     ]
 
 
@@ -141,6 +144,9 @@ class rt_tuning(ctypes.Structure):
         ("lane_stack", ctypes.c_uint32),
         ("packet_budget", ctypes.c_uint32),
         ("builder", ctypes.c_uint32),
+        ("query", ctypes.c_int32),
+        ("handoff_steps", ctypes.c_uint32),
+        ("item_steps", ctypes.c_uint32),
     ]
 
 
@@ -188,29 +194,35 @@ class rt_paths_out(ctypes.Structure):
 # (name, restype, argtypes) of every exported entry point
 SIGNATURES = [
     ("rt_abi_version", ctypes.c_int, []),
+    ("rt_build_id", ctypes.c_char_p, []),
     ("rt_last_error", ctypes.c_char_p, []),
     ("rt_get_tuning", ctypes.c_int, [ctypes.POINTER(rt_tuning)]),
     ("rt_set_tuning", ctypes.c_int, [ctypes.POINTER(rt_tuning)]),
     ("rt_scene_sizes_for", ctypes.c_int, [ctypes.c_uint32, ctypes.POINTER(rt_scene_sizes)]),
     ("rt_scene_build", ctypes.c_int,
      [ctypes.POINTER(rt_mesh_desc), ctypes.c_uint32, ctypes.POINTER(rt_scene), ctypes.c_void_p,
-      ctypes.c_uint64, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+      ctypes.c_uint64, ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(rt_tuning), ctypes.c_void_p]),
+    ("rt_scene_export", ctypes.c_int,
+     [ctypes.POINTER(rt_scene), ctypes.POINTER(rt_mesh_desc), ctypes.c_uint32, ctypes.c_void_p,
+      ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     ("rt_trace_closest", ctypes.c_int,
      [ctypes.POINTER(rt_scene), ctypes.POINTER(rt_rays), ctypes.c_uint32, ctypes.POINTER(rt_hits),
-      ctypes.c_void_p]),
+      ctypes.POINTER(rt_tuning), ctypes.c_void_p]),
     ("rt_trace_any", ctypes.c_int,
      [ctypes.POINTER(rt_scene), ctypes.POINTER(rt_rays), ctypes.c_uint32, ctypes.c_void_p,
-      ctypes.c_void_p]),
+      ctypes.POINTER(rt_tuning), ctypes.c_void_p]),
     ("rt_generate_rays", ctypes.c_int,
      [ctypes.POINTER(rt_camera), ctypes.c_uint32, ctypes.c_uint32] + [ctypes.c_void_p] * 6
      + [ctypes.c_void_p]),
+    ("rt_trace_camera_workspace_bytes", ctypes.c_uint64, [ctypes.c_uint32, ctypes.c_uint32]),
     ("rt_trace_camera", ctypes.c_int,
      [ctypes.POINTER(rt_scene), ctypes.POINTER(rt_camera), ctypes.c_uint32, ctypes.c_uint32,
-      ctypes.POINTER(rt_hits), ctypes.c_void_p]),
+      ctypes.POINTER(rt_hits), ctypes.c_void_p, ctypes.c_uint64, ctypes.POINTER(rt_tuning),
+      ctypes.c_void_p]),
     ("rt_render_workspace_bytes", ctypes.c_uint64, [ctypes.c_uint32]),
     ("rt_render", ctypes.c_int,
      [ctypes.POINTER(rt_scene), ctypes.POINTER(rt_camera), ctypes.POINTER(rt_render_params),
-      ctypes.POINTER(rt_render_out), ctypes.c_void_p, ctypes.c_uint64, ctypes.c_void_p]),
+      ctypes.POINTER(rt_render_out), ctypes.c_void_p, ctypes.c_uint64, ctypes.POINTER(rt_tuning), ctypes.c_void_p]),
     ("rt_render_batch_workspace_bytes", ctypes.c_uint64, [ctypes.c_uint32]),
     ("rt_render_batch", ctypes.c_int,
      [ctypes.c_void_p, ctypes.c_uint32, ctypes.POINTER(rt_batch_config), ctypes.c_void_p,
@@ -219,7 +231,7 @@ SIGNATURES = [
     ("rt_paths_workspace_bytes", ctypes.c_uint64, [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32]),
     ("rt_trace_paths", ctypes.c_int,
      [ctypes.POINTER(rt_scene), ctypes.POINTER(rt_camera), ctypes.POINTER(rt_paths_params),
-      ctypes.POINTER(rt_paths_out), ctypes.c_void_p, ctypes.c_uint64, ctypes.c_void_p]),
+      ctypes.POINTER(rt_paths_out), ctypes.c_void_p, ctypes.c_uint64, ctypes.POINTER(rt_tuning), ctypes.c_void_p]),
     ("rt_mesh_normals_workspace_bytes", ctypes.c_uint64, [ctypes.c_uint32, ctypes.c_uint32]),
     ("rt_mesh_compute_normals", ctypes.c_int,
      [ctypes.POINTER(rt_mesh_desc), ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p,
@@ -227,6 +239,7 @@ SIGNATURES = [
     ("rt_mesh_pack", ctypes.c_int, [ctypes.c_void_p] * 3 + [ctypes.c_uint32, ctypes.c_void_p, ctypes.c_void_p]),
     ("rt_mesh_unpack", ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32] + [ctypes.c_void_p] * 4),
     ("rt_image_to_bgra8", ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_void_p]),
+    ("rt_texels_to_rgba8", ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_void_p, ctypes.c_void_p]),
 ]
 
 _lib = None
@@ -249,6 +262,11 @@ def load(path=LIB_PATH):
     v = lib.rt_abi_version()
     if v != RT_ABI_VERSION:
         raise RuntimeError(f"{path}: ABI version {v}, expected {RT_ABI_VERSION}; rebuild it")
+    from .build import source_id
+    built, have = lib.rt_build_id().decode(), source_id()
+    if built != have:
+        raise RuntimeError(f"{path} was built from other sources (build id {built}, sources {have}): "
+                           "rebuild it with `python -m rendertoy_amd.build`")
     if path == LIB_PATH:
         _lib = lib
     return lib

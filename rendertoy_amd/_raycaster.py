@@ -3,10 +3,16 @@
 Reference API kept: `Raycaster(models: List[Mesh])` builds the acceleration structure
 (`_build_ads`, rendering/_raycaster.py:26-33) and `ray_cast(rays) -> array` answers closest-hit
 queries (:35-36); the `BVH_AABB` and `BVH_Triangle` structs (:8-22) are registered with the same
-layout. Additions: `Ray`/`Hit` structs, `occluded(rays)`, `render(...)` and `generate_rays(...)`.
-Triangle identity of a hit is (mesh, index) as BVH_Triangle defines it (:16-22).
+layout, and `export_ads()` fills them from the built hierarchy. Additions: `Ray`/`Hit` structs,
+`occluded(rays)`, `render(...)` and `generate_rays(...)`. Triangle identity of a hit is
+(mesh, index) as BVH_Triangle defines it (:16-22).
+
+Streams: a Raycaster made with `stream=s` runs its build and every query and render on `s`, after
+the work already queued on the caller's current stream (inputs made there are ready); its outputs
+are ready on `s` (wait for `rc.stream` before reading them on another stream).
 Every call runs on the GPU; there is no CPU path.
 """
+import contextlib
 import ctypes
 import math
 import typing
@@ -15,7 +21,8 @@ import numpy as np
 import torch
 
 from . import _native as N
-from ._core import (kernel_struct, float3, float4, DeviceArray, create_buffer, Dispatcher,
+from .tuning import make_tuning, as_dict as tuning_dict, tuning_arg
+from ._core import (kernel_struct, float3, float4, float4x4, DeviceArray, create_buffer, Dispatcher,
                     to_array, texture_pointer, RGBA)
 from ._modeling import Mesh, MeshVertex
 
@@ -78,6 +85,30 @@ class Camera:
         self.fov = float(fov)
         self.aspect = float(aspect)
 
+    @classmethod
+    def from_matrices(cls, view, proj):
+        """The camera of a view matrix from look_at(eye, target, up) and a projection from
+        perspective(fov, aspect, znear, zfar) (rendering/_core.py:528-548; the matrices the
+        reference's shaders take, tutorials/lesson08_rasterization.py:94-99): the basis is read
+        from the view matrix's columns (x, y, z axes), the eye from its translation row, and
+        tan(fov/2), aspect from the projection's diagonal. The ray of pixel (col, row) is then the
+        inverse of that projection through the pixel centre."""
+        V = np.asarray(to_array(view) if np.asarray(view).dtype == float4x4 else view, np.float64)
+        P = np.asarray(to_array(proj) if np.asarray(proj).dtype == float4x4 else proj, np.float64)
+        V, P = V.reshape(4, 4), P.reshape(4, 4)
+        if P[2, 3] != 1.0 or P[3, 3] != 0.0 or P[0, 0] <= 0 or P[1, 1] <= 0:
+            raise ValueError("proj must be a perspective() matrix (w = view z, positive scales)")
+        M = V[:3, :3]  # columns: x, y, z axes
+        cam = cls.__new__(cls)
+        cam.eye = (-(V[3, :3] @ M.T)).astype(np.float32)
+        ws, hs = np.float32(P[0, 0]), np.float32(P[1, 1])
+        cam.right = (V[:3, 0].astype(np.float32) / ws).astype(np.float32)
+        cam.up = (V[:3, 1].astype(np.float32) / hs).astype(np.float32)
+        cam.forward = V[:3, 2].astype(np.float32)
+        cam.fov = float(2.0 * math.atan(1.0 / float(hs)))
+        cam.aspect = float(hs / ws)
+        return cam
+
     def as_array(self):
         """12 float32: eye, right, up, forward."""
         return np.concatenate([self.eye, self.right, self.up, self.forward]).astype(np.float32)
@@ -103,9 +134,37 @@ def _check_device(t, what):
         raise RuntimeError(f"{what} must live on the HIP device (got {dev})")
 
 
+def _check_out(t, what, dtype, n):
+    """An output tensor the kernels write n elements of: device, dtype, contiguous, large enough."""
+    _check_device(t, what)
+    if t.dtype != dtype or not t.is_contiguous():
+        raise TypeError(f"{what} must be a contiguous {dtype} device tensor")
+    if t.numel() < n:
+        raise ValueError(f"{what} holds {t.numel()} elements, {n} are written")
+
+
+def _narrow_indices(ib, n_vertices):
+    """int32 device copy of an index buffer of another integer type. Values outside
+    [0, n_vertices) become n_vertices - 1, the rule the build applies to int32 indices, so no
+    value can wrap into range."""
+    kinds = {np.dtype(np.int64): torch.int64, np.dtype(np.int16): torch.int16,
+             np.dtype(np.int8): torch.int8, np.dtype(np.uint8): torch.uint8}
+    dt = np.dtype(ib.dtype)
+    if dt in kinds:
+        t = ib.tensor(kinds[dt]).to(torch.int64)
+    elif dt == np.uint16:
+        t = ib.tensor(torch.int16).to(torch.int64) & 0xFFFF
+    else:
+        raise TypeError(f"index buffers must be int32, uint32, int64, int16, uint16, int8 or uint8 "
+                        f"(got {dt})")
+    last = max(int(n_vertices) - 1, 0)
+    return torch.where((t < 0) | (t >= n_vertices), torch.full_like(t, last), t).to(torch.int32)
+
+
 def _fill_mesh_desc(d, m, keep):
     """Fills rt_mesh_desc `d` from Mesh `m` (MeshVertex records, optional index buffer); narrowed
-    index copies are appended to `keep`, which must outlive the call that reads them."""
+    index copies are appended to `keep`, which must outlive the call that reads them (they are made
+    on the current stream)."""
     vb = m.vertices
     _check_device(vb, "Mesh.vertices")
     if vb.dtype != MeshVertex:
@@ -120,14 +179,14 @@ def _fill_mesh_desc(d, m, keep):
     if m.indices is not None:
         ib = m.indices
         _check_device(ib, "Mesh.indices")
-        if ib.dtype != np.int32:
-            # load_obj-style int64 index buffers (rendering/_loaders.py:17) are narrowed
-            t = ib.tensor(torch.int64 if ib.dtype == np.int64 else None)
-            ib32 = t.to(torch.int32).contiguous()
+        if np.dtype(ib.dtype) in (np.dtype(np.int32), np.dtype(np.uint32)):
+            # uint32 indices are read as their bits: values >= 2^31 are out of range either way
+            d.indices = ib.data_ptr()
+        else:
+            # e.g. the int64 index buffers of load_obj (rendering/_loaders.py:17)
+            ib32 = _narrow_indices(ib, d.n_vertices)
             keep.append(ib32)
             d.indices = ib32.data_ptr()
-        else:
-            d.indices = ib.data_ptr()
     else:
         d.indices = None
 
@@ -244,33 +303,100 @@ def mesh_tensors(mesh: Mesh, stream=None) -> dict:
     return out
 
 
+def _texture_rgba8(tx, keep, stream):
+    """(device pointer, width, height) of RGBA8 texels for one texture argument: a uint8 tensor
+    [H, W, 4], a float32 tensor [H, W, 4], or a (memory, Texture2D) pair from create_texture2D with
+    RGBA or float4 texels (the reference's default, rendering/_core.py:557-578). Float texels are
+    converted once, here, with the UNORM_INT8 rule (rt_texels_to_rgba8); the converted copy is
+    appended to `keep`."""
+    if isinstance(tx, tuple):
+        mem, desc = tx
+        h, w = mem.shape[0], mem.shape[1]
+        if mem.dtype == RGBA:
+            return texture_pointer(desc), w, h
+        if mem.dtype != float4:
+            raise TypeError("texture memory must hold RGBA (uchar4) or float4 texels")
+        src, n = texture_pointer(desc), w * h
+    else:
+        _check_device(tx, "texture")
+        if tx.dim() != 3 or tx.shape[2] != 4 or not tx.is_contiguous():
+            raise TypeError("texture tensors must be contiguous [H, W, 4]")
+        h, w = tx.shape[0], tx.shape[1]
+        if tx.dtype == torch.uint8:
+            return tx.data_ptr(), w, h
+        if tx.dtype != torch.float32:
+            raise TypeError("texture tensors must be uint8 (RGBA8) or float32 (RGBA) [H, W, 4]")
+        src, n = tx.data_ptr(), w * h
+        keep.append(tx)
+    lib = N.load()
+    dst = torch.empty((h, w, 4), dtype=torch.uint8, device=torch.device("cuda"))
+    N.check(lib.rt_texels_to_rgba8(src, n, dst.data_ptr(), N.stream_handle(stream)), "rt_texels_to_rgba8")
+    keep.append(dst)
+    return dst.data_ptr(), w, h
+
+
 class TextureBinding:
     """Textures and the mesh -> texture map of a scene uploaded once (device table of rt_texture
-    and int32 map), for callers that render the same scene many times; pass it as `textures`."""
+    and int32 map), for callers that render the same scene many times; pass it as `textures`.
+    Float texels are converted to RGBA8 once, here. `mesh_textures[m]` is the texture of mesh m
+    (-1: albedo 1); it must have one entry per mesh of the scenes it is used with."""
 
-    def __init__(self, textures, mesh_textures):
+    def __init__(self, textures, mesh_textures, stream=None):
         dev = torch.device("cuda")
+        ts = stream if stream is not None else torch.cuda.current_stream()
         arr = (N.rt_texture * max(1, len(textures)))()
-        for i, tx in enumerate(textures):
-            _check_device(tx, "texture")
-            if tx.dtype != torch.uint8 or tx.dim() != 3 or tx.shape[2] != 4 or not tx.is_contiguous():
-                raise TypeError("texture tensors must be contiguous uint8 [H, W, 4]")
-            arr[i].texels = tx.data_ptr()
-            arr[i].width, arr[i].height = tx.shape[1], tx.shape[0]
+        self._keep = []
+        with torch.cuda.stream(ts):
+            for i, tx in enumerate(textures):
+                arr[i].texels, arr[i].width, arr[i].height = _texture_rgba8(tx, self._keep, ts)
+            self.table = torch.frombuffer(bytearray(bytes(arr)), dtype=torch.uint8).to(dev)
+            self.mesh_map = torch.tensor(list(mesh_textures), dtype=torch.int32).to(dev)
         self.textures = list(textures)
         self.n_textures = len(textures)
-        self.table = torch.frombuffer(bytearray(bytes(arr)), dtype=torch.uint8).to(dev)
-        self.mesh_map = torch.tensor(list(mesh_textures), dtype=torch.int32).to(dev)
+        self.n_meshes = len(list(mesh_textures))
 
 
 class Raycaster:
-    """Closest-hit ray caster over a list of meshes (rendering/_raycaster.py:25-36)."""
+    """Closest-hit ray caster over a list of meshes (rendering/_raycaster.py:25-36).
 
-    def __init__(self, models: typing.List[Mesh], stream=None):
+    `stream`: the torch stream its kernels run on (default: the caller's current stream at each
+    call). `tuning`: launch choices for this scene (a dict of rendertoy_amd.tuning fields over the
+    process default at construction, e.g. {"builder": "ploc", "query": "lane"}); every method also
+    takes a `tuning=` dict for one call. Without either, each call uses the process default."""
+
+    def __init__(self, models: typing.List[Mesh], stream=None, tuning=None):
         self.models = models
         self._stream = stream
+        self._tuning = None if tuning is None else make_tuning(None, **tuning)
         self._render_ws = None
         self._build_ads()
+
+    # ------------------------------------------------------------------ helpers
+    @property
+    def stream(self):
+        """The stream the kernels of this scene run on."""
+        return self._torch_stream()
+
+    def _torch_stream(self):
+        return self._stream if self._stream is not None else torch.cuda.current_stream()
+
+    @contextlib.contextmanager
+    def _on_stream(self):
+        """Runs the block with this scene's stream current (allocations, copies and kernels all on
+        it), after the work already queued on the caller's stream."""
+        ts = self._torch_stream()
+        cur = torch.cuda.current_stream()
+        if ts != cur:
+            ts.wait_stream(cur)
+        with torch.cuda.stream(ts):
+            yield ts
+
+    def _tu(self, tuning):
+        """The rt_tuning of one call (None: this scene's, or the process default)."""
+        if tuning is None:
+            return self._tuning
+        base = None if self._tuning is None else tuning_dict(self._tuning)
+        return make_tuning(base, **tuning)
 
     # ------------------------------------------------------------------ build
     def _build_ads(self):
@@ -278,62 +404,62 @@ class Raycaster:
         descs = (N.rt_mesh_desc * max(1, len(self.models)))()
         keep = []
         n_tris = 0
-        for k, m in enumerate(self.models):
-            _fill_mesh_desc(descs[k], m, keep)
-            n_tris += descs[k].n_tris
-        self.n_tris = n_tris
-        self.mesh_tri_counts = [int(descs[k].n_tris) for k in range(len(self.models))]
-        sizes = N.rt_scene_sizes()
-        N.check(lib.rt_scene_sizes_for(n_tris, ctypes.byref(sizes)), "rt_scene_sizes_for")
-        dev = torch.device("cuda")
-        # allocate on the stream the build runs on: torch's allocator orders reuse of the freed
-        # workspace after the build kernels on that stream
-        # the scene's arrays are 256-byte-aligned slices of one allocation (one allocator call)
-        parts = [("_nodes", sizes.nodes_bytes, torch.uint8), ("_nodes4", sizes.nodes4_bytes, torch.uint8),
-                 ("_tris", sizes.tris_bytes, torch.uint8), ("_shade", sizes.shade_bytes, torch.uint8),
-                 ("_bounds", 32, torch.float32), ("_mesh_offsets", 4 * (len(self.models) + 1), torch.int32),
-                 ("keys", 4 * max(1, n_tris), torch.int32), ("order", 4 * max(1, n_tris), torch.int32)]
-        offsets, total = [], 0
-        for _, nbytes, _ in parts:
-            offsets.append(total)
-            total += (int(nbytes) + 255) // 256 * 256
-        with torch.cuda.stream(self._torch_stream()):
+        with self._on_stream() as ts:
+            for k, m in enumerate(self.models):
+                _fill_mesh_desc(descs[k], m, keep)
+                n_tris += descs[k].n_tris
+            self.n_tris = n_tris
+            self.mesh_tri_counts = [int(descs[k].n_tris) for k in range(len(self.models))]
+            sizes = N.rt_scene_sizes()
+            N.check(lib.rt_scene_sizes_for(n_tris, ctypes.byref(sizes)), "rt_scene_sizes_for")
+            dev = torch.device("cuda")
+            # the scene's arrays are 256-byte-aligned slices of one allocation (one allocator
+            # call); allocated on the build's stream, so torch orders reuse of the freed workspace
+            # after the build kernels
+            parts = [("_nodes", sizes.nodes_bytes, torch.uint8), ("_nodes4", sizes.nodes4_bytes, torch.uint8),
+                     ("_tris", sizes.tris_bytes, torch.uint8), ("_shade", sizes.shade_bytes, torch.uint8),
+                     ("_bounds", 32, torch.float32), ("_mesh_offsets", 4 * (len(self.models) + 1), torch.int32),
+                     ("_prim_slot", sizes.prim_slot_bytes, torch.int32),
+                     ("keys", 4 * max(1, n_tris), torch.int32), ("order", 4 * max(1, n_tris), torch.int32)]
+            offsets, total = [], 0
+            for _, nbytes, _ in parts:
+                offsets.append(total)
+                total += (int(nbytes) + 255) // 256 * 256
             arena = torch.empty(total, dtype=torch.uint8, device=dev)
             ws = torch.empty(sizes.workspace_bytes, dtype=torch.uint8, device=dev)
-        for (name, nbytes, dtype), off in zip(parts, offsets):
-            setattr(self, name, arena[off:off + int(nbytes)].view(dtype))
-        sc = N.rt_scene()
-        sc.n_tris = n_tris
-        sc.n_meshes = len(self.models)
-        sc.nodes = self._nodes.data_ptr()
-        sc.nodes4 = self._nodes4.data_ptr()
-        sc.tris = self._tris.data_ptr()
-        sc.shade = self._shade.data_ptr()
-        sc.mesh_offsets = self._mesh_offsets.data_ptr()
-        sc.bounds = self._bounds.data_ptr()
-        code = lib.rt_scene_build(descs, len(self.models), ctypes.byref(sc), ws.data_ptr(),
-                                  sizes.workspace_bytes, self.keys.data_ptr(),
-                                  self.order.data_ptr(), N.stream_handle(self._stream))
-        N.check(code, "rt_scene_build")
+            for (name, nbytes, dtype), off in zip(parts, offsets):
+                setattr(self, name, arena[off:off + int(nbytes)].view(dtype))
+            sc = N.rt_scene()
+            sc.n_tris = n_tris
+            sc.n_meshes = len(self.models)
+            sc.nodes = self._nodes.data_ptr()
+            sc.nodes4 = self._nodes4.data_ptr()
+            sc.tris = self._tris.data_ptr()
+            sc.shade = self._shade.data_ptr()
+            sc.mesh_offsets = self._mesh_offsets.data_ptr()
+            sc.bounds = self._bounds.data_ptr()
+            sc.prim_slot = self._prim_slot.data_ptr()
+            code = lib.rt_scene_build(descs, len(self.models), ctypes.byref(sc), ws.data_ptr(),
+                                      sizes.workspace_bytes, self.keys.data_ptr(),
+                                      self.order.data_ptr(), tuning_arg(self._tuning),
+                                      N.stream_handle(ts))
+            N.check(code, "rt_scene_build")
+            del ws
         self._scene = sc
         self._keep = keep
-        del ws
 
     def device_bytes(self) -> int:
         """Device memory held by the built scene (BVH, triangle and shading records, keys)."""
         ts = [self._nodes, self._nodes4, self._tris, self._shade, self._bounds, self._mesh_offsets,
-              self.keys, self.order]
+              self._prim_slot, self.keys, self.order]
         return int(sum(t.numel() * t.element_size() for t in ts))
 
     def release_inputs(self):
         """Drops the references to the input meshes' device buffers; queries and renders only read
-        the built scene, so a resident scene need not keep its vertices."""
+        the built scene, so a resident scene need not keep its vertices (export_ads needs them)."""
         self.n_models = len(self.models)
         self.models = [None] * self.n_models
         self._keep = []
-
-    def _torch_stream(self):
-        return self._stream if self._stream is not None else torch.cuda.current_stream()
 
     @property
     def root(self) -> int:
@@ -352,87 +478,147 @@ class Raycaster:
             "bounds": self._bounds,
             "keys": self.keys[:n],
             "order": self.order[:n],
+            "prim_slot": self._prim_slot[:n],
             "mesh_offsets": self._mesh_offsets,
         }
 
+    def export_ads(self):
+        """The built hierarchy in the reference's record types (rendering/_raycaster.py:8-22):
+        (nodes, triangles) as device buffers of BVH_AABB and BVH_Triangle. nodes[0] is the root
+        (box = scene bounds); an internal node has count 0 and its children at `elements` and
+        `elements` + 1; a leaf has count 1..4 triangles at triangles[elements : elements + count].
+        Triangles are in leaf order with the hit identity (index within the mesh, mesh position),
+        and their vertices are the input meshes' (so the meshes must not have been released).
+        Boxes are the hierarchy's conservative child boxes: every triangle lies inside each box on
+        its path. Synchronises once (the node count)."""
+        lib = N.require_gpu()
+        if any(m is None for m in self.models):
+            raise RuntimeError("export_ads needs the input meshes; release_inputs() dropped them")
+        descs = (N.rt_mesh_desc * max(1, len(self.models)))()
+        keep = []
+        n = self.n_tris
+        with self._on_stream() as ts:
+            for k, m in enumerate(self.models):
+                _fill_mesh_desc(descs[k], m, keep)
+            nodes = create_buffer(max(2 * n - 1, 1), BVH_AABB)
+            tris = create_buffer(max(n, 1), BVH_Triangle)
+            count = torch.zeros(1, dtype=torch.int32, device=torch.device("cuda"))
+            N.check(lib.rt_scene_export(ctypes.byref(self._scene), descs, len(self.models),
+                                        nodes.data_ptr(), tris.data_ptr(), count.data_ptr(),
+                                        N.stream_handle(ts)), "rt_scene_export")
+            c = int(count.item())
+        return nodes[:c], tris[:n]
+
     # ------------------------------------------------------------------ queries
-    def _rays_desc(self, rays, tmin, tmax):
+    def _rays_desc(self, rays, tmin, tmax, n=None):
+        """rt_rays of a Ray buffer or a dict of SoA float32 tensors (ox, oy, oz, dx, dy, dz and
+        optional per-ray tmin, tmax); `n`: trace only the first n rays (default: all)."""
         d = N.rt_rays()
         if isinstance(rays, DeviceArray):
             if rays.dtype != Ray:
                 raise TypeError("rays must be a buffer of Ray")
             _check_device(rays, "rays")
             base = rays.data_ptr()
-            n = rays.size
+            avail = rays.size
             d.ox, d.oy, d.oz = base, base + 4, base + 8
             d.dx, d.dy, d.dz = base + 16, base + 20, base + 24
             d.stride = Ray.itemsize
         elif isinstance(rays, dict):
-            for k in ("ox", "oy", "oz", "dx", "dy", "dz"):
-                _check_device(rays[k], "rays")
-                if rays[k].dtype != torch.float32 or not rays[k].is_contiguous():
+            missing = [k for k in ("ox", "oy", "oz", "dx", "dy", "dz") if k not in rays]
+            if missing:
+                raise KeyError(f"SoA rays lack the fields {missing}")
+            avail = rays["ox"].numel()
+            for k in ("ox", "oy", "oz", "dx", "dy", "dz", "tmin", "tmax"):
+                if k not in rays or rays[k] is None:
+                    continue
+                t = rays[k]
+                _check_device(t, f"rays[{k!r}]")
+                if t.dtype != torch.float32 or not t.is_contiguous():
                     raise TypeError("SoA ray fields must be contiguous float32 tensors")
-            n = rays["ox"].numel()
+                if t.numel() != avail:
+                    raise ValueError(f"rays[{k!r}] has {t.numel()} elements, rays['ox'] {avail}")
             d.ox, d.oy, d.oz = (rays[k].data_ptr() for k in ("ox", "oy", "oz"))
             d.dx, d.dy, d.dz = (rays[k].data_ptr() for k in ("dx", "dy", "dz"))
             d.stride = 4
-            if "tmin" in rays:
+            if rays.get("tmin") is not None:
                 d.tmin = rays["tmin"].data_ptr()
-            if "tmax" in rays:
+            if rays.get("tmax") is not None:
                 d.tmax = rays["tmax"].data_ptr()
         else:
             raise TypeError("rays must be a Ray buffer or a dict of SoA float32 tensors")
+        if n is None:
+            n = avail
+        if not 0 <= int(n) <= avail:
+            raise ValueError(f"{n} rays requested, the buffer holds {avail}")
+        if n >= 2 ** 32:
+            raise ValueError("at most 2^32 - 1 rays per call")
         d.tmin_default = float(tmin)
         d.tmax_default = float(tmax)
-        return d, n
+        return d, int(n)
 
-    def ray_cast(self, rays, tmin: float = 0.0, tmax: float = math.inf) -> DeviceArray:
+    def ray_cast(self, rays, tmin: float = 0.0, tmax: float = math.inf, tuning=None) -> DeviceArray:
         """Closest hit of every ray: a buffer of Hit (rendering/_raycaster.py:35-36)."""
         lib = N.require_gpu()
         d, n = self._rays_desc(rays, tmin, tmax)
-        out = create_buffer(n, Hit)
-        base = out.data_ptr()
-        h = N.rt_hits()
-        h.t, h.u, h.v = base, base + 4, base + 8
-        h.index, h.mesh = base + 12, base + 16
-        h.prim = None
-        h.stride = Hit.itemsize
-        N.check(lib.rt_trace_closest(ctypes.byref(self._scene), ctypes.byref(d), n, ctypes.byref(h),
-                                     N.stream_handle(self._stream)), "rt_trace_closest")
+        with self._on_stream() as ts:
+            out = create_buffer(n, Hit)
+            base = out.data_ptr()
+            h = N.rt_hits()
+            h.t, h.u, h.v = base, base + 4, base + 8
+            h.index, h.mesh = base + 12, base + 16
+            h.prim = None
+            h.stride = Hit.itemsize
+            N.check(lib.rt_trace_closest(ctypes.byref(self._scene), ctypes.byref(d), n, ctypes.byref(h),
+                                         tuning_arg(self._tu(tuning)), N.stream_handle(ts)),
+                    "rt_trace_closest")
         return out
 
-    def trace_soa(self, rays: dict, tmin=0.0, tmax=math.inf, out=None):
-        """Closest hit of SoA rays into SoA torch tensors t, u, v, prim, index, mesh."""
+    _HIT_FIELDS = (("t", torch.float32), ("u", torch.float32), ("v", torch.float32),
+                   ("prim", torch.int32), ("index", torch.int32), ("mesh", torch.int32))
+
+    def trace_soa(self, rays: dict, tmin=0.0, tmax=math.inf, out=None, n=None, tuning=None):
+        """Closest hit of SoA rays into SoA torch tensors t, u, v, prim, index, mesh (`out`: a dict
+        of any of them, each at least n elements; default: all six, allocated). `n`: trace only
+        the first n rays."""
         lib = N.require_gpu()
-        d, n = self._rays_desc(rays, tmin, tmax)
-        dev = torch.device("cuda")
-        if out is None:
-            out = {k: torch.empty(n, dtype=torch.float32, device=dev) for k in ("t", "u", "v")}
-            out.update({k: torch.empty(n, dtype=torch.int32, device=dev)
-                        for k in ("prim", "index", "mesh")})
-        h = N.rt_hits()
-        for k in ("t", "u", "v", "prim", "index", "mesh"):
-            setattr(h, k, out[k].data_ptr() if out.get(k) is not None else None)
-        h.stride = 4
-        N.check(lib.rt_trace_closest(ctypes.byref(self._scene), ctypes.byref(d), n, ctypes.byref(h),
-                                     N.stream_handle(self._stream)), "rt_trace_closest")
+        d, n = self._rays_desc(rays, tmin, tmax, n)
+        with self._on_stream() as ts:
+            if out is None:
+                out = {k: torch.empty(n, dtype=dt, device=torch.device("cuda")) for k, dt in self._HIT_FIELDS}
+            h = N.rt_hits()
+            for k, dt in self._HIT_FIELDS:
+                if out.get(k) is not None:
+                    _check_out(out[k], f"out[{k!r}]", dt, n)
+                setattr(h, k, out[k].data_ptr() if out.get(k) is not None else None)
+            h.stride = 4
+            N.check(lib.rt_trace_closest(ctypes.byref(self._scene), ctypes.byref(d), n, ctypes.byref(h),
+                                         tuning_arg(self._tu(tuning)), N.stream_handle(ts)),
+                    "rt_trace_closest")
         return out
 
-    def occluded(self, rays, tmin: float = 0.0, tmax: float = math.inf) -> torch.Tensor:
+    def occluded(self, rays, tmin: float = 0.0, tmax: float = math.inf, out=None, n=None,
+                 tuning=None) -> torch.Tensor:
         """Any-hit: uint8 tensor, 1 where some triangle lies in (tmin, tmax) along the ray."""
         lib = N.require_gpu()
-        d, n = self._rays_desc(rays, tmin, tmax)
-        out = torch.empty(max(n, 1), dtype=torch.uint8, device="cuda")
-        N.check(lib.rt_trace_any(ctypes.byref(self._scene), ctypes.byref(d), n, out.data_ptr(),
-                                 N.stream_handle(self._stream)), "rt_trace_any")
+        d, n = self._rays_desc(rays, tmin, tmax, n)
+        with self._on_stream() as ts:
+            if out is None:
+                out = torch.empty(max(n, 1), dtype=torch.uint8, device="cuda")
+            else:
+                _check_out(out, "out", torch.uint8, n)
+            N.check(lib.rt_trace_any(ctypes.byref(self._scene), ctypes.byref(d), n, out.data_ptr(),
+                                     tuning_arg(self._tu(tuning)), N.stream_handle(ts)),
+                    "rt_trace_any")
         return out[:n]
 
     @property
     def trace(self):
-        """`rc.trace[n](rays, hits)` launch form of the reference's Dispatcher (rendering/_core.py:261)."""
-        def _fn(n, rays, hits_out):
-            res = self.trace_soa(rays, out=hits_out)
-            return res
+        """`rc.trace[n](rays, hits)`: the launch form of the reference's Dispatcher
+        (rendering/_core.py:252-253, 261-290): exactly the first n rays are traced (the reference
+        launches n work-items and every id >= n returns); `hits` is a dict of SoA output tensors
+        of at least n elements (None: allocated)."""
+        def _fn(n, rays, hits_out=None):
+            return self.trace_soa(rays, out=hits_out, n=n)
         return Dispatcher(_fn, "trace")
 
     # ------------------------------------------------------------------ camera rays / render
@@ -442,58 +628,81 @@ class Raycaster:
         lib = N.require_gpu()
         n = width * height
         dev = torch.device("cuda")
-        out = {k: torch.empty(max(n, 1), dtype=torch.float32, device=dev)
-               for k in ("ox", "oy", "oz", "dx", "dy", "dz")}
-        cam = camera.to_c()
-        N.check(lib.rt_generate_rays(ctypes.byref(cam), width, height,
-                                     *[out[k].data_ptr() for k in ("ox", "oy", "oz", "dx", "dy", "dz")],
-                                     N.stream_handle(stream)), "rt_generate_rays")
+        ts = stream if stream is not None else torch.cuda.current_stream()
+        with torch.cuda.stream(ts):
+            out = {k: torch.empty(max(n, 1), dtype=torch.float32, device=dev)
+                   for k in ("ox", "oy", "oz", "dx", "dy", "dz")}
+            cam = camera.to_c()
+            N.check(lib.rt_generate_rays(ctypes.byref(cam), width, height,
+                                         *[out[k].data_ptr() for k in ("ox", "oy", "oz", "dx", "dy", "dz")],
+                                         N.stream_handle(ts)), "rt_generate_rays")
         return {k: v[:n] for k, v in out.items()}
 
-    def cast_camera(self, camera: Camera, width: int, height: int, out=None, fields=("t", "u", "v", "prim")):
+    def cast_camera(self, camera: Camera, width: int, height: int, out=None,
+                    fields=("t", "u", "v", "prim"), tuning=None):
         """Closest hit of the primary rays of a width x height image (ray generation fused):
-        SoA tensors over pixels (row-major), misses t=+inf, prim/index/mesh=-1."""
+        SoA tensors over pixels (row-major), misses t=+inf, prim/index/mesh=-1. `out`: a dict of
+        any of t, u, v, prim, index, mesh tensors of at least width*height elements."""
         lib = N.require_gpu()
         n = width * height
-        dev = torch.device("cuda")
-        if out is None:
-            out = {}
-            for k in fields:
-                out[k] = torch.empty(max(n, 1), dtype=torch.float32 if k in ("t", "u", "v") else torch.int32,
-                                     device=dev)
-        h = N.rt_hits()
-        for k in ("t", "u", "v", "prim", "index", "mesh"):
-            setattr(h, k, out[k].data_ptr() if out.get(k) is not None else None)
-        h.stride = 4
-        cam = camera.to_c()
-        N.check(lib.rt_trace_camera(ctypes.byref(self._scene), ctypes.byref(cam), width, height,
-                                    ctypes.byref(h), N.stream_handle(self._stream)), "rt_trace_camera")
+        with self._on_stream() as ts:
+            if out is None:
+                dts = dict(self._HIT_FIELDS)
+                out = {k: torch.empty(max(n, 1), dtype=dts[k], device=torch.device("cuda")) for k in fields}
+            h = N.rt_hits()
+            for k, dt in self._HIT_FIELDS:
+                if out.get(k) is not None:
+                    _check_out(out[k], f"out[{k!r}]", dt, n)
+                setattr(h, k, out[k].data_ptr() if out.get(k) is not None else None)
+            h.stride = 4
+            cam = camera.to_c()
+            ws = self._camera_workspace(width, height)
+            N.check(lib.rt_trace_camera(ctypes.byref(self._scene), ctypes.byref(cam), width, height,
+                                        ctypes.byref(h), None if ws is None else ws.data_ptr(),
+                                        0 if ws is None else ws.numel(), tuning_arg(self._tu(tuning)),
+                                        N.stream_handle(ts)), "rt_trace_camera")
         return out
 
     def trace_paths(self, camera: Camera, width: int, height: int, spp: int = 4, bounces: int = 2,
                     seed: int = 0x5EED, frame: int = 0, albedo: float = 0.7, sky: float = 1.0,
-                    ray_tmin: float = 1e-4, out=None, counts=None):
+                    ray_tmin: float = 1e-4, out=None, counts=None, tuning=None):
         """Diffuse path tracing under a uniform sky (rt_trace_paths): `spp` jittered camera rays
         per pixel, each followed by up to `bounces` cosine-weighted bounces (Philox4x32-10 with key
         (seed, frame)); returns the radiance image [height, width] float32. `counts` (optional
         int32 tensor of bounces+1) receives the rays traced at each depth."""
         lib = N.require_gpu()
-        dev = torch.device("cuda")
         p = N.rt_paths_params(width, height, spp, bounces, seed & 0xFFFFFFFF, frame & 0xFFFFFFFF,
                               float(albedo), float(sky), float(ray_tmin))
-        if out is None:
-            out = torch.empty((height, width), dtype=torch.float32, device=dev)
-        elif out.dtype != torch.float32 or out.numel() < width * height or not out.is_contiguous():
-            raise TypeError("out must be a contiguous float32 tensor of height*width elements")
-        need = lib.rt_paths_workspace_bytes(width, height, spp)
-        if getattr(self, "_paths_ws", None) is None or self._paths_ws.numel() < need:
-            self._paths_ws = torch.empty(need, dtype=torch.uint8, device=dev)
-        o = N.rt_paths_out(out.data_ptr(), counts.data_ptr() if counts is not None else None)
-        cam = camera.to_c()
-        N.check(lib.rt_trace_paths(ctypes.byref(self._scene), ctypes.byref(cam), ctypes.byref(p),
-                                   ctypes.byref(o), self._paths_ws.data_ptr(), self._paths_ws.numel(),
-                                   N.stream_handle(self._stream)), "rt_trace_paths")
+        with self._on_stream() as ts:
+            if out is None:
+                out = torch.empty((height, width), dtype=torch.float32, device=torch.device("cuda"))
+            else:
+                _check_out(out, "out", torch.float32, width * height)
+            if counts is not None:
+                _check_out(counts, "counts", torch.int32, bounces + 1)
+            need = lib.rt_paths_workspace_bytes(width, height, spp)
+            if getattr(self, "_paths_ws", None) is None or self._paths_ws.numel() < need:
+                self._paths_ws = torch.empty(need, dtype=torch.uint8, device=torch.device("cuda"))
+            o = N.rt_paths_out(out.data_ptr(), counts.data_ptr() if counts is not None else None)
+            cam = camera.to_c()
+            N.check(lib.rt_trace_paths(ctypes.byref(self._scene), ctypes.byref(cam), ctypes.byref(p),
+                                       ctypes.byref(o), self._paths_ws.data_ptr(), self._paths_ws.numel(),
+                                       tuning_arg(self._tu(tuning)), N.stream_handle(ts)),
+                    "rt_trace_paths")
         return out
+
+    def _camera_workspace(self, width, height):
+        """The camera kernel's workspace on the current stream, or None when it needs none. One
+        per stream: calls on one stream run in order, calls on two streams must not share one."""
+        need = N.load().rt_trace_camera_workspace_bytes(width, height)
+        if need == 0:
+            return None
+        pool = self.__dict__.setdefault("_cam_ws", {})
+        key = torch.cuda.current_stream().cuda_stream
+        ws = pool.get(key)
+        if ws is None or ws.numel() < need:
+            ws = pool[key] = torch.empty(need, dtype=torch.uint8, device="cuda")
+        return ws
 
     def _render_workspace(self, n_pixels):
         lib = N.load()
@@ -502,9 +711,14 @@ class Raycaster:
             self._render_ws = torch.empty(need, dtype=torch.uint8, device="cuda")
         return self._render_ws
 
+    @property
+    def n_meshes(self):
+        return len(self.models)
+
     def _render_args(self, width, height, light, textures, mesh_textures, shadows, bilinear,
                      ambient, background, shadow_tmin, out, return_hits, stats):
-        """rt_render_params / rt_render_out of one render plus the device tensors they point to."""
+        """rt_render_params / rt_render_out of one render plus the device tensors they point to
+        (allocated on the current stream)."""
         n = width * height
         dev = torch.device("cuda")
         if light is None:
@@ -519,6 +733,9 @@ class Raycaster:
         p.shadow_tmin = float(shadow_tmin)
         tex_keep = []
         if isinstance(textures, TextureBinding):
+            if textures.n_meshes != self.n_meshes:
+                raise ValueError(f"the texture binding maps {textures.n_meshes} meshes, the scene has "
+                                 f"{self.n_meshes}")
             p.textures_dev = textures.table.data_ptr()
             p.n_textures = textures.n_textures
             p.mesh_texture_dev = textures.mesh_map.data_ptr()
@@ -526,33 +743,27 @@ class Raycaster:
             textures, mesh_textures = None, None
         if textures:
             arr = (N.rt_texture * len(textures))()
+            ts = torch.cuda.current_stream()
             for i, tx in enumerate(textures):
-                if isinstance(tx, tuple):
-                    mem, desc = tx
-                    if mem.dtype != RGBA:
-                        raise TypeError("render samples RGBA (uchar4) textures")
-                    arr[i].texels = texture_pointer(desc)
-                    arr[i].width, arr[i].height = mem.shape[1], mem.shape[0]
-                else:
-                    _check_device(tx, "texture")
-                    if tx.dtype != torch.uint8 or tx.dim() != 3 or tx.shape[2] != 4 or not tx.is_contiguous():
-                        raise TypeError("texture tensors must be contiguous uint8 [H, W, 4]")
-                    arr[i].texels = tx.data_ptr()
-                    arr[i].width, arr[i].height = tx.shape[1], tx.shape[0]
+                arr[i].texels, arr[i].width, arr[i].height = _texture_rgba8(tx, tex_keep, ts)
             tdev = torch.frombuffer(bytearray(bytes(arr)), dtype=torch.uint8).to(dev)
             tex_keep.append(tdev)
             p.textures_dev = tdev.data_ptr()
             p.n_textures = len(textures)
             if mesh_textures is None:
-                mesh_textures = [0] * len(self.models)
+                mesh_textures = [0] * self.n_meshes
         if mesh_textures is not None:
-            mt = torch.tensor(list(mesh_textures), dtype=torch.int32).to(dev)
+            mesh_textures = list(mesh_textures)
+            if len(mesh_textures) != self.n_meshes:
+                raise ValueError(f"mesh_textures has {len(mesh_textures)} entries, the scene has "
+                                 f"{self.n_meshes} meshes")
+            mt = torch.tensor(mesh_textures, dtype=torch.int32).to(dev)
             tex_keep.append(mt)
             p.mesh_texture_dev = mt.data_ptr()
         if out is None:
             out = torch.empty((height, width, 3), dtype=torch.float32, device=dev)
-        elif out.dtype != torch.float32 or out.numel() < 3 * n or not out.is_contiguous():
-            raise TypeError("out must be a contiguous float32 tensor of height*width*3 elements")
+        else:
+            _check_out(out, "out", torch.float32, 3 * n)
         o = N.rt_render_out()
         o.rgb = out.data_ptr()
         hits = None
@@ -561,6 +772,7 @@ class Raycaster:
                     "prim": torch.empty(max(n, 1), dtype=torch.int32, device=dev)}
             o.t, o.prim = hits["t"].data_ptr(), hits["prim"].data_ptr()
         if stats is not None:
+            _check_out(stats, "stats", torch.int32, 4)
             o.stats = stats.data_ptr()
         return p, o, out, hits, tex_keep
 
@@ -570,29 +782,31 @@ class Raycaster:
         Returns the list of RGB images; see rendertoy_amd.batch for batches over many scenes."""
         from .batch import RenderRequest, BatchRender
         reqs = [RenderRequest(self, **r) for r in requests]
-        b = BatchRender(reqs, config=config)
+        b = BatchRender(reqs, config=config, stream=self._stream)
         b.launch()
         return b.images
 
     def render(self, camera: Camera, width: int, height: int, light=None, textures=None,
                mesh_textures=None, shadows=True, bilinear=True, ambient=0.2,
                background=(0.0, 0.0, 0.0), shadow_tmin=1e-4, out=None, return_hits=False,
-               stats=None):
+               stats=None, tuning=None):
         """RGB float32 image [height, width, 3] of the scene: closest hit, Lambert with
         L = normalize(1,1,1) and ambient 0.2 by default (tutorials/lesson09_texture_mapping.py:73),
-        shadow ray toward L, albedo from RGBA8 textures (texels as uint8 tensors [H, W, 4] or
-        (memory, Texture2D) pairs from create_texture2D(w, h, RGBA)); `mesh_textures[m]` picks
-        the texture of mesh m (-1: albedo 1)."""
+        shadow ray toward L, albedo from the textures: uint8 RGBA8 or float32 RGBA tensors
+        [H, W, 4], (memory, Texture2D) pairs from create_texture2D(w, h) (float4 texels, the
+        reference's default, converted to RGBA8 with the UNORM rule) or create_texture2D(w, h,
+        RGBA), or a TextureBinding; `mesh_textures[m]` picks the texture of mesh m (-1: albedo 1)."""
         lib = N.require_gpu()
         n = width * height
-        p, o, out, hits, tex_keep = self._render_args(width, height, light, textures, mesh_textures,
-                                                      shadows, bilinear, ambient, background,
-                                                      shadow_tmin, out, return_hits, stats)
-        ws = self._render_workspace(n)
-        cam = camera.to_c()
-        N.check(lib.rt_render(ctypes.byref(self._scene), ctypes.byref(cam), ctypes.byref(p),
-                              ctypes.byref(o), ws.data_ptr(), ws.numel(),
-                              N.stream_handle(self._stream)), "rt_render")
+        with self._on_stream() as ts:
+            p, o, out, hits, tex_keep = self._render_args(width, height, light, textures, mesh_textures,
+                                                          shadows, bilinear, ambient, background,
+                                                          shadow_tmin, out, return_hits, stats)
+            ws = self._render_workspace(n)
+            cam = camera.to_c()
+            N.check(lib.rt_render(ctypes.byref(self._scene), ctypes.byref(cam), ctypes.byref(p),
+                                  ctypes.byref(o), ws.data_ptr(), ws.numel(),
+                                  tuning_arg(self._tu(tuning)), N.stream_handle(ts)), "rt_render")
         self._tex_keep = tex_keep  # alive until the next render on this stream
         if return_hits:
             return out, hits

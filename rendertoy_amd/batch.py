@@ -88,33 +88,47 @@ class BatchRender:
         self._stream = stream
         dev = torch.device("cuda")
         n = len(requests)
-        self.stats = torch.zeros((n, 4), dtype=torch.int32, device=dev) if stats else None
-        jobs = (N.rt_render_job * n)()
-        self.images, self.hits, self._keep = [], [], []
-        self.rays_primary = 0
-        for i, r in enumerate(requests):
-            if r.width * r.height > N.RT_BATCH_MAX_JOB_PIXELS:
-                raise ValueError(f"request {i}: more than {N.RT_BATCH_MAX_JOB_PIXELS} pixels")
-            p, o, out, hits, keep = r.raycaster._render_args(
-                r.width, r.height, r.light, r.textures, r.mesh_textures, r.shadows, r.bilinear,
-                r.ambient, r.background, r.shadow_tmin, r.out, r.return_hits,
-                self.stats[i] if stats else None)
-            jobs[i].scene = r.raycaster._scene
-            jobs[i].cam = r.camera.to_c()
-            jobs[i].params = p
-            jobs[i].out = o
-            self.images.append(out)
-            self.hits.append(hits)
-            self._keep += keep + [r.raycaster]
-            self.rays_primary += r.width * r.height
-        self.n_jobs = n
-        # pinned staging + asynchronous copy: a pageable copy would wait for every batch already
-        # queued on the stream, and a serving loop enqueues the next batch behind a running one
-        self._jobs_host = torch.frombuffer(bytearray(bytes(jobs)), dtype=torch.uint8).pin_memory()
-        self.jobs_dev = self._jobs_host.to(dev, non_blocking=True)
-        wsb = lib.rt_render_batch_workspace_bytes(n)
-        self.workspace = torch.zeros(wsb, dtype=torch.uint8, device=dev)
+        ts = self._torch_stream()
+        cur = torch.cuda.current_stream()
+        if ts != cur:
+            ts.wait_stream(cur)  # inputs queued on the caller's stream are ready
+        # every buffer and copy below is made on the stream the launches run on
+        with torch.cuda.stream(ts):
+            self.stats = torch.zeros((n, 4), dtype=torch.int32, device=dev) if stats else None
+            jobs = (N.rt_render_job * n)()
+            self.images, self.hits, self._keep = [], [], []
+            self.rays_primary = 0
+            for i, r in enumerate(requests):
+                if r.width * r.height > N.RT_BATCH_MAX_JOB_PIXELS:
+                    raise ValueError(f"request {i}: more than {N.RT_BATCH_MAX_JOB_PIXELS} pixels")
+                p, o, out, hits, keep = r.raycaster._render_args(
+                    r.width, r.height, r.light, r.textures, r.mesh_textures, r.shadows, r.bilinear,
+                    r.ambient, r.background, r.shadow_tmin, r.out, r.return_hits,
+                    self.stats[i] if stats else None)
+                jobs[i].scene = r.raycaster._scene
+                jobs[i].cam = r.camera.to_c()
+                jobs[i].params = p
+                jobs[i].out = o
+                self.images.append(out)
+                self.hits.append(hits)
+                self._keep += keep + [r.raycaster]
+                self.rays_primary += r.width * r.height
+            self.n_jobs = n
+            # pinned staging + asynchronous copy: a pageable copy would wait for every batch already
+            # queued on the stream, and a serving loop enqueues the next batch behind a running one
+            self._jobs_host = torch.frombuffer(bytearray(bytes(jobs)), dtype=torch.uint8).pin_memory()
+            self.jobs_dev = self._jobs_host.to(dev, non_blocking=True)
+            wsb = lib.rt_render_batch_workspace_bytes(n)
+            self.workspace = torch.zeros(wsb, dtype=torch.uint8, device=dev)
         self._graph = None
+
+    def _torch_stream(self):
+        return self._stream if self._stream is not None else torch.cuda.current_stream()
+
+    @property
+    def stream(self):
+        """The stream the batch's launches run on (its outputs are ready there)."""
+        return self._torch_stream()
 
     def launch(self, config: BatchConfig = None):
         """Renders every job (asynchronous on the stream)."""
@@ -129,7 +143,7 @@ class BatchRender:
         self.launch()  # warm up outside the capture
         torch.cuda.synchronize()
         g = torch.cuda.CUDAGraph()
-        with torch.cuda.graph(g):
+        with torch.cuda.graph(g, stream=self._stream):
             self.launch()
         self._graph = g
         return g

@@ -1322,38 +1322,43 @@ __global__ void __launch_bounds__(256) k_wide(uint32_t n, const float4* __restri
     const uint64_t m = (uint64_t)n - 1;  // internal nodes
     const uint64_t rows = m - base < 256 ? m - base : 256;  // records of this block
     if (i < m) {
-        float lo[3][4], hi[3][4];
-        int32_t ref[4];
-        int k = 0;
+        // slot order: the two children of the left child (or the left child itself if it is a
+        // leaf), then those of the right child; every index below is a compile-time constant, so
+        // the record stays in registers
         const float4* nd = nodes + i * 4;
         const float4 n0 = nd[0], n1 = nd[1], n2 = nd[2], n3 = nd[3];
-        const int32_t c[2] = {__float_as_int(n3.x), __float_as_int(n3.y)};
-        const float cb[2][6] = {{n0.x, n0.z, n2.x, n0.y, n0.w, n2.y}, {n1.x, n1.z, n2.z, n1.y, n1.w, n2.w}};
-        for (int e = 0; e < 2; ++e) {
-            if (c[e] >= 0) {
-                const float4* cd = nodes + (uint64_t)c[e] * 4;
+        const int32_t c0 = __float_as_int(n3.x), c1 = __float_as_int(n3.y);
+        struct Slot {
+            float lx, ly, lz, hx, hy, hz;
+            int32_t ref;
+        };
+        // the one or two entries a child contributes: its grandchildren, or itself if a leaf
+        auto entries = [&](int32_t c, float4 b, float bz0, float bz1, Slot& e0, Slot& e1) {
+            if (c >= 0) {
+                const float4* cd = nodes + (uint64_t)c * 4;
                 const float4 m0 = cd[0], m1 = cd[1], m2 = cd[2], m3 = cd[3];
-                lo[0][k] = m0.x; hi[0][k] = m0.y; lo[1][k] = m0.z; hi[1][k] = m0.w;
-                lo[2][k] = m2.x; hi[2][k] = m2.y; ref[k] = __float_as_int(m3.x); ++k;
-                lo[0][k] = m1.x; hi[0][k] = m1.y; lo[1][k] = m1.z; hi[1][k] = m1.w;
-                lo[2][k] = m2.z; hi[2][k] = m2.w; ref[k] = __float_as_int(m3.y); ++k;
+                e0 = {m0.x, m0.z, m2.x, m0.y, m0.w, m2.y, __float_as_int(m3.x)};
+                e1 = {m1.x, m1.z, m2.z, m1.y, m1.w, m2.w, __float_as_int(m3.y)};
             } else {
-                lo[0][k] = cb[e][0]; lo[1][k] = cb[e][1]; lo[2][k] = cb[e][2];
-                hi[0][k] = cb[e][3]; hi[1][k] = cb[e][4]; hi[2][k] = cb[e][5];
-                ref[k] = c[e]; ++k;
+                e0 = {b.x, b.z, bz0, b.y, b.w, bz1, c};
+                e1 = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY, RT_EMPTY_ROOT};
             }
-        }
-        for (; k < 4; ++k) {
-            for (int a = 0; a < 3; ++a) { lo[a][k] = INFINITY; hi[a][k] = -INFINITY; }
-            ref[k] = RT_EMPTY_ROOT;
-        }
+        };
+        Slot a0, a1, b0, b1;
+        entries(c0, n0, n2.x, n2.y, a0, a1);
+        entries(c1, n1, n2.z, n2.w, b0, b1);
+        const bool two = c0 >= 0;  // the left child contributes two entries
+        const Slot e[4] = {a0, two ? a1 : b0, two ? b0 : b1, two ? b1 : a1};
+        // (with a leaf left child, slot 3 takes a1: the empty entry)
         float4* o = stage + threadIdx.x * kPitch;
-        for (int a = 0; a < 3; ++a) {
-            o[2 * a] = make_float4(lo[a][0], lo[a][1], lo[a][2], lo[a][3]);
-            o[2 * a + 1] = make_float4(hi[a][0], hi[a][1], hi[a][2], hi[a][3]);
-        }
-        o[6] = make_float4(__int_as_float(ref[0]), __int_as_float(ref[1]), __int_as_float(ref[2]),
-                           __int_as_float(ref[3]));
+        o[0] = make_float4(e[0].lx, e[1].lx, e[2].lx, e[3].lx);
+        o[1] = make_float4(e[0].hx, e[1].hx, e[2].hx, e[3].hx);
+        o[2] = make_float4(e[0].ly, e[1].ly, e[2].ly, e[3].ly);
+        o[3] = make_float4(e[0].hy, e[1].hy, e[2].hy, e[3].hy);
+        o[4] = make_float4(e[0].lz, e[1].lz, e[2].lz, e[3].lz);
+        o[5] = make_float4(e[0].hz, e[1].hz, e[2].hz, e[3].hz);
+        o[6] = make_float4(__int_as_float(e[0].ref), __int_as_float(e[1].ref), __int_as_float(e[2].ref),
+                           __int_as_float(e[3].ref));
         o[7] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     }
     __syncthreads();
@@ -1388,6 +1393,98 @@ __global__ void __launch_bounds__(256) k_copy_u32(const uint32_t* __restrict__ a
 
 inline uint32_t blocks_for(uint64_t n, uint32_t per) { return (uint32_t)((n + per - 1) / per); }
 
+// ---- export in the reference's record types (rendering/_raycaster.py:8-22) ----
+// BVH_AABB (48 B) as 3 float4: (min.xyz, 0), (max.xyz, 0), (count, elements, 0, 0) as int bits.
+RT_DEV void put_aabb(float4* out, uint32_t j, float lx, float ly, float lz, float hx, float hy,
+                     float hz, int32_t count, int32_t elements) {
+    out[3 * (uint64_t)j] = make_float4(lx, ly, lz, 0.0f);
+    out[3 * (uint64_t)j + 1] = make_float4(hx, hy, hz, 0.0f);
+    out[3 * (uint64_t)j + 2] = make_float4(__int_as_float(count), __int_as_float(elements), 0.0f, 0.0f);
+}
+
+// Breadth-first numbering of the traversal tree in one workgroup: node 0 is the root; the
+// children of the internal nodes of one level are appended as pairs in the order of their parents
+// (an exclusive scan over each chunk of 1024 nodes of the level), so the numbering is a fixed
+// function of the tree. A node's record is written by its parent (box, and its traversal ref
+// parked in `elements` with count -1) and completed when its own level is processed (count and
+// elements). 2 L - 1 records for L leaves.
+__global__ void __launch_bounds__(1024) k_export_nodes(const float4* __restrict__ nodes, int32_t root,
+                                                       const float* __restrict__ bounds,
+                                                       float4* __restrict__ out,
+                                                       uint32_t* __restrict__ n_out) {
+    __shared__ uint32_t scan[1024];
+    const uint32_t tid = threadIdx.x;
+    if (tid == 0)
+        put_aabb(out, 0, bounds[0], bounds[1], bounds[2], bounds[4], bounds[5], bounds[6], -1, root);
+    __syncthreads();
+    uint32_t lo = 0, hi = 1, next = 1;
+    while (lo < hi) {
+        for (uint32_t base = lo; base < hi; base += 1024) {
+            const uint32_t j = base + tid;
+            const bool in = j < hi;
+            const int32_t ref = in ? __float_as_int(out[3 * (uint64_t)j + 2].y) : 0;
+            const uint32_t internal = (in && ref >= 0) ? 1u : 0u;
+            scan[tid] = internal;
+            __syncthreads();
+            for (uint32_t off = 1; off < 1024; off <<= 1) {  // inclusive Hillis-Steele scan
+                const uint32_t v = tid >= off ? scan[tid - off] : 0u;
+                __syncthreads();
+                scan[tid] += v;
+                __syncthreads();
+            }
+            const uint32_t before = scan[tid] - internal, total = scan[1023];
+            if (internal) {
+                const uint32_t cb = next + 2u * before;
+                const float4* nd = nodes + (uint64_t)ref * 4;
+                const float4 n0 = nd[0], n1 = nd[1], n2 = nd[2], n3 = nd[3];
+                put_aabb(out, cb, n0.x, n0.z, n2.x, n0.y, n0.w, n2.y, -1, __float_as_int(n3.x));
+                put_aabb(out, cb + 1, n1.x, n1.z, n2.z, n1.y, n1.w, n2.w, -1, __float_as_int(n3.y));
+                out[3 * (uint64_t)j + 2] = make_float4(__int_as_float(0), __int_as_float((int32_t)cb), 0.0f, 0.0f);
+            } else if (in) {
+                out[3 * (uint64_t)j + 2] = make_float4(__int_as_float((int32_t)leaf_count(ref)),
+                                                       __int_as_float((int32_t)leaf_first(ref)), 0.0f, 0.0f);
+            }
+            next += 2u * total;
+            __syncthreads();
+        }
+        lo = hi;
+        hi = next;
+    }
+    if (tid == 0 && n_out) *n_out = next;
+}
+
+__global__ void k_export_empty(float4* __restrict__ out, uint32_t* __restrict__ n_out) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        put_aabb(out, 0, INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY, 0, -1);
+        if (n_out) *n_out = 1;
+    }
+}
+
+// prim_slot[order[s]] = s: the triangle-record slot of each global primitive id
+__global__ void __launch_bounds__(256) k_prim_slot(const uint32_t* __restrict__ order, uint32_t n,
+                                                        uint32_t* __restrict__ inv) {
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < n) inv[order[s]] = s;
+}
+
+// BVH_Triangle records (64 B as 4 float4: v0, v1, v2, (index, mesh, 0, 0)) of mesh m, written at
+// the slot of each triangle; vertex indices are clamped to n_vertices - 1 as the build does.
+__global__ void __launch_bounds__(256) k_export_tris(rt_mesh_desc d, uint32_t m, uint32_t base,
+                                                     const uint32_t* __restrict__ inv,
+                                                     float4* __restrict__ out) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= d.n_tris) return;
+    const uint64_t s = inv[base + t];
+    float4* o = out + 4 * s;
+    for (int k = 0; k < 3; ++k) {
+        uint32_t vi = d.indices ? (uint32_t)d.indices[3 * (uint64_t)t + k] : 3 * t + k;
+        vi = vi < d.n_vertices ? vi : d.n_vertices - 1;
+        const float* P = (const float*)((const char*)d.vertices + (uint64_t)vi * d.vertex_stride + d.pos_offset);
+        o[k] = make_float4(P[0], P[1], P[2], 0.0f);
+    }
+    o[3] = make_float4(__int_as_float((int32_t)t), __int_as_float((int32_t)m), 0.0f, 0.0f);
+}
+
 }  // namespace
 }  // namespace rt
 
@@ -1403,13 +1500,16 @@ extern "C" int rt_scene_sizes_for(uint32_t n_tris, rt_scene_sizes* out) {
     out->shade_bytes = 64 * (n ? n : 1);
     out->bounds_bytes = 32;
     out->workspace_bytes = carve(nullptr, n ? n : 1).bytes;
+    out->prim_slot_bytes = 4 * (n ? n : 1);
     return RT_OK;
 }
 
 extern "C" int rt_scene_build(const rt_mesh_desc* meshes, uint32_t n_meshes, rt_scene* sc,
                               void* workspace, uint64_t ws_bytes, uint32_t* keys_out,
-                              uint32_t* order_out, void* stream) {
+                              uint32_t* order_out, const rt_tuning* tuning, void* stream) {
     if (!sc) return set_error(RT_EINVAL, "rt_scene_build: scene is NULL");
+    rt_tuning tu;
+    if (int e = resolve_tuning(tuning, tu, "rt_scene_build")) return e;
     if (n_meshes && !meshes) return set_error(RT_EINVAL, "rt_scene_build: meshes is NULL");
     uint64_t n64 = 0;
     for (uint32_t m = 0; m < n_meshes; ++m) {
@@ -1431,7 +1531,8 @@ extern "C" int rt_scene_build(const rt_mesh_desc* meshes, uint32_t n_meshes, rt_
     const uint32_t n = (uint32_t)n64;
     if (sc->n_tris != n || sc->n_meshes != n_meshes)
         return set_error(RT_EINVAL, "rt_scene_build: scene n_tris/n_meshes do not match the meshes");
-    if (!sc->nodes || !sc->nodes4 || !sc->tris || !sc->shade || !sc->mesh_offsets || !sc->bounds)
+    if (!sc->nodes || !sc->nodes4 || !sc->tris || !sc->shade || !sc->mesh_offsets || !sc->bounds ||
+        !sc->prim_slot)
         return set_error(RT_EINVAL, "rt_scene_build: scene arrays must be allocated");
     if (((uintptr_t)sc->nodes | (uintptr_t)sc->nodes4 | (uintptr_t)sc->tris | (uintptr_t)sc->shade) & 15)
         return set_error(RT_EINVAL, "rt_scene_build: scene arrays must be 16-byte aligned");
@@ -1475,7 +1576,7 @@ extern "C" int rt_scene_build(const rt_mesh_desc* meshes, uint32_t n_meshes, rt_
     sort_pairs(ki, vi, w.keys1, w.vals1, n, 30, w.hist, w.totals, s);
     // after 4 passes the sorted arrays are back in keys0/vals0
     float4* nodes = (float4*)sc->nodes;
-    bool ploc = n > 1 && current_tuning().builder == RT_BUILD_PLOC;
+    bool ploc = n > 1 && tu.builder == RT_BUILD_PLOC;
     const uint32_t nchunk = blocks_for(n, kPlocChunk);
     if (ploc) {
         // 5. PLOC hierarchy (node boxes as it merges, scene bounds at the root), 6. leaf slots and
@@ -1548,9 +1649,47 @@ extern "C" int rt_scene_build(const rt_mesh_desc* meshes, uint32_t n_meshes, rt_
         sc->root = ~0;
     }
     }
+    hipLaunchKernelGGL(k_prim_slot, dim3(blocks_for(n, 256)), dim3(256), 0, s, vi, n,
+                       (uint32_t*)sc->prim_slot);
     if (keys_out)
         hipLaunchKernelGGL(k_copy_u32, dim3(blocks_for(n, 256)), dim3(256), 0, s, ki, keys_out, n);
     if (order_out)
         hipLaunchKernelGGL(k_copy_u32, dim3(blocks_for(n, 256)), dim3(256), 0, s, vi, order_out, n);
     return check_launch("rt_scene_build");
+}
+
+extern "C" int rt_scene_export(const rt_scene* sc, const rt_mesh_desc* meshes, uint32_t n_meshes,
+                               void* aabb_out, void* tris_out, uint32_t* n_nodes_out, void* stream) {
+    if (!sc || !aabb_out) return set_error(RT_EINVAL, "rt_scene_export: NULL argument");
+    if (((uintptr_t)aabb_out | (uintptr_t)tris_out) & 15)
+        return set_error(RT_EINVAL, "rt_scene_export: outputs must be 16-byte aligned");
+    hipStream_t s = (hipStream_t)stream;
+    float4* aabb = (float4*)aabb_out;
+    const uint32_t n = sc->n_tris;
+    if (sc->root == RT_EMPTY_ROOT || n == 0) {
+        hipLaunchKernelGGL(k_export_empty, dim3(1), dim3(1), 0, s, aabb, n_nodes_out);
+        return check_launch("rt_scene_export");
+    }
+    if (!tris_out || !sc->prim_slot || !sc->bounds || (sc->root >= 0 && !sc->nodes))
+        return set_error(RT_EINVAL, "rt_scene_export: NULL argument");
+    if (n_meshes != sc->n_meshes || (n_meshes && !meshes))
+        return set_error(RT_EINVAL, "rt_scene_export: the meshes must be those the scene was built from");
+    uint64_t total = 0;
+    for (uint32_t m = 0; m < n_meshes; ++m) {
+        if (meshes[m].n_tris && (!meshes[m].vertices || !meshes[m].n_vertices))
+            return set_error(RT_EINVAL, "rt_scene_export: mesh vertices NULL");
+        total += meshes[m].n_tris;
+    }
+    if (total != n) return set_error(RT_EINVAL, "rt_scene_export: mesh triangle counts do not match the scene");
+    const uint32_t* inv = (const uint32_t*)sc->prim_slot;
+    uint32_t base = 0;
+    for (uint32_t m = 0; m < n_meshes; ++m) {
+        if (meshes[m].n_tris)
+            hipLaunchKernelGGL(k_export_tris, dim3(blocks_for(meshes[m].n_tris, 256)), dim3(256), 0, s,
+                               meshes[m], m, base, inv, (float4*)tris_out);
+        base += meshes[m].n_tris;
+    }
+    hipLaunchKernelGGL(k_export_nodes, dim3(1), dim3(1024), 0, s, (const float4*)sc->nodes, sc->root,
+                       sc->bounds, aabb, n_nodes_out);
+    return check_launch("rt_scene_export");
 }

@@ -26,7 +26,6 @@
 
 namespace rt {
 
-constexpr int kStackDepth = 64;        // >= 62: depth bound of a radix tree over 62-bit keys
 constexpr int kWideStack = 128;        // >= 3 * 31: pushes per level x levels of the 4-wide tree
 constexpr int kPacketLds = 48;         // float4 per wave: triangles of 4 leaves staged by a node step
 constexpr float kTfarScale = 1.0000004f;  // 1 + 2*gamma(3) rounded up (Ize 2013, robust slabs)
@@ -146,21 +145,25 @@ RT_DEV bool test_leaf(const float4* __restrict__ tris, int32_t ref, const Ray& r
     return changed;
 }
 
-// Closest-hit (ANY=false) or any-hit (ANY=true) traversal of the two-child node layout, nearer
-// child first. `skip_slot` is never tested (the shadow ray's own triangle). On return h.prim < 0
-// means no hit in (tmin, tmax).
+// Closest-hit (ANY=false) or any-hit (ANY=true) per-lane traversal of the two-child node layout,
+// nearer child first, pending children on a per-lane stack in LDS: entry k of this lane lives at
+// stk[k * stride] (stride = threads per block, so the lanes of a wave touch consecutive words at
+// equal depth and every access is one LDS instruction with no bank conflict). `skip_slot` is never
+// tested (a shadow ray's own triangle). Returns false if the ray needed more than `limit` pending
+// entries: the hit is then invalid and the caller traces the ray again with a traversal whose stack
+// always suffices. On a true return h.prim < 0 means no hit in (tmin, tmax).
 template <bool ANY>
-RT_DEV void traverse(const float4* __restrict__ nodes, const float4* __restrict__ tris,
-                     int32_t root, const Ray& r, int32_t skip_slot, Hit& h) {
+RT_DEV bool traverse_bin(const float4* __restrict__ nodes, const float4* __restrict__ tris,
+                         int32_t root, const Ray& r, int32_t skip_slot, Hit& h, int32_t* stk,
+                         int stride, int limit) {
     h.t = r.tmax;
     h.u = 0.0f;
     h.v = 0.0f;
     h.prim = -1;
     h.slot = -1;
-    if (root == RT_EMPTY_ROOT) return;
+    if (root == RT_EMPTY_ROOT) return true;
     const float idx = safe_inv(r.dx), idy = safe_inv(r.dy), idz = safe_inv(r.dz);
     const float oxx = r.ox * idx, oyy = r.oy * idy, ozz = r.oz * idz;
-    int32_t stack[kStackDepth];
     int sp = 0;
     int32_t cur = root;
     for (;;) {
@@ -181,28 +184,21 @@ RT_DEV void traverse(const float4* __restrict__ nodes, const float4* __restrict_
             const bool hit1 = tn1 <= fminf(tf1, h.t);
             const int32_t r0 = __float_as_int(n3.x), r1 = __float_as_int(n3.y);
             if (hit0 && hit1) {
-                int32_t nearc = r0, farc = r1;
-                if (tn1 < tn0) {
-                    nearc = r1;
-                    farc = r0;
-                }
-                if (sp < kStackDepth) stack[sp++] = farc;
-                cur = nearc;
+                if (sp >= limit) return false;
+                const bool swap = tn1 < tn0;
+                stk[(sp++) * stride] = swap ? r0 : r1;
+                cur = swap ? r1 : r0;
                 continue;
             }
-            if (hit0) {
-                cur = r0;
-                continue;
-            }
-            if (hit1) {
-                cur = r1;
+            if (hit0 | hit1) {
+                cur = hit0 ? r0 : r1;
                 continue;
             }
         } else {
-            if (test_leaf(tris, cur, r, true, skip_slot, h) && ANY) return;
+            if (test_leaf(tris, cur, r, true, skip_slot, h) && ANY) return true;
         }
-        if (sp == 0) break;
-        cur = stack[--sp];
+        if (sp == 0) return true;
+        cur = stk[(--sp) * stride];
     }
 }
 
@@ -437,22 +433,38 @@ RT_DEV float quad_min(float t) {
 // bit 4q set if any lane of quad q is set in the ballot b
 RT_DEV uint64_t quad_any(uint64_t b) { return (b | (b >> 1) | (b >> 2) | (b >> 3)) & kQuadBits; }
 
+// Stack entries as the quad walk keeps them: a node and its 16 ray bits packed in 32 bits (ray q
+// at bit 4q for q < 8, at bit 4(q-8)+1 for q >= 8), entry k in lane k (k < 64) or lane k - 64.
+RT_DEV uint32_t quad_mask_pack(uint64_t mask) { return (uint32_t)mask | ((uint32_t)(mask >> 32) << 1); }
+RT_DEV uint64_t quad_mask_unpack(uint32_t cm) {
+    return (uint64_t)(cm & 0x11111111u) | ((uint64_t)((cm >> 1) & 0x11111111u) << 32);
+}
+
+// What a budgeted quad walk leaves when it stops: `sp` stack entries held as described above, plus
+// the node it would visit next with its rays. The callback gets these and the lanes' candidates.
+struct QuadRest {
+    int32_t sp;
+    int32_t node0, node1;   // lane k: entries k and 64 + k
+    uint32_t m0, m1;
+    int32_t cur;            // next node (wave-uniform)
+    uint32_t cur_m;         // its packed ray bits
+};
+
 // The traversal proper: on return each lane's h is its own closest candidate (see traverse_quad).
-template <bool ANY>
-RT_DEV void traverse_quad_lanes(const float4* __restrict__ nodes4, const float4* __restrict__ tris,
-                                int32_t root, const Ray& r, bool want, int32_t skip_slot, Hit& h) {
-    h.t = r.tmax;
-    h.u = 0.0f;
-    h.v = 0.0f;
-    h.prim = -1;
-    h.slot = -1;
+// The walk starts at node `start` with the rays of `m` (bit 4q = ray q) and the candidates already
+// in h (all four lanes of a quad hold the same one: a bound found elsewhere, or none). With
+// BUDGET, after `budget` node steps it stops, calls handoff(rest) with what is left, and returns
+// false; handoff may return false to refuse (the walk then continues unbudgeted). Called by all 64
+// lanes with uniform control flow.
+template <bool ANY, bool BUDGET, typename Handoff>
+RT_DEV bool traverse_quad_core(const float4* __restrict__ nodes4, const float4* __restrict__ tris,
+                               int32_t start, uint64_t m, const Ray& r, int32_t skip_slot, Hit& h,
+                               int budget, Handoff&& handoff) {
     const uint32_t lane = __lane_id(), c = lane & 3u, qb = lane & ~3u;
-    uint64_t m = __ballot(want) & kQuadBits;
-    if (root == RT_EMPTY_ROOT || m == 0) return;
     uint64_t alive = m;
     const float idx = safe_inv(r.dx), idy = safe_inv(r.dy), idz = safe_inv(r.dz);
     const float oxx = r.ox * idx, oyy = r.oy * idy, ozz = r.oz * idz;
-    float qt = h.t;  // the quad's best t so far (culling); h holds each lane's own best until the end
+    float qt = quad_min(h.t);  // the quad's best t so far (culling); h holds each lane's own best
     // tests triangle c of each leaf in `lref` (ray in the leaf's mask), keeps this lane's closest
     // candidate in h and refreshes the quad's best t
     auto leaves = [&](uint32_t lset, const int32_t* lref, const uint64_t* lmsk) {
@@ -495,11 +507,11 @@ RT_DEV void traverse_quad_lanes(const float4* __restrict__ nodes4, const float4*
         }
         qt = quad_min(h.t);
     };
-    if (root < 0) {  // the whole tree is one leaf
-        const int32_t lref[4] = {root, 0, 0, 0};
+    if (start < 0) {  // the whole tree is one leaf
+        const int32_t lref[4] = {start, 0, 0, 0};
         const uint64_t lmsk[4] = {m, 0, 0, 0};
         leaves(1u, lref, lmsk);
-        return;
+        return true;
     }
     int32_t stk_node0 = 0, stk_node1 = 0;
     uint32_t stk_m0 = 0, stk_m1 = 0;  // ray masks, bit 4q -> bit 4q (q < 8) / 4(q-8)+1
@@ -512,26 +524,34 @@ RT_DEV void traverse_quad_lanes(const float4* __restrict__ nodes4, const float4*
             const int32_t lsp = hi ? sp - 64 : sp;
             node = __builtin_amdgcn_readlane(hi ? stk_node1 : stk_node0, lsp);
             const uint32_t cm = (uint32_t)__builtin_amdgcn_readlane((int)(hi ? stk_m1 : stk_m0), lsp);
-            mask = (uint64_t)(cm & 0x11111111u) | ((uint64_t)((cm >> 1) & 0x11111111u) << 32);
+            mask = quad_mask_unpack(cm);
             if (ANY) mask &= alive;
             if (mask != 0) return true;
         }
     };
     // lane c's child of node `cur`: its box and ref (fields c of the SoA record)
-    const float* nf = (const float*)(nodes4 + (size_t)root * 8);
+    int32_t cur = start;
+    const float* nf = (const float*)(nodes4 + (size_t)start * 8);
     float clx = nf[c], chx = nf[4 + c], cly = nf[8 + c], chy = nf[12 + c], clz = nf[16 + c],
           chz = nf[20 + c];
     int32_t cref = __float_as_int(nf[24 + c]);
+    int steps = 0;
     for (;;) {
         if (ANY) {
             m &= alive;
             if (m == 0) {
                 int32_t node;
-                if (!pop(node, m)) return;
+                if (!pop(node, m)) return true;
+                cur = node;
                 nf = (const float*)(nodes4 + (size_t)node * 8);
                 clx = nf[c]; chx = nf[4 + c]; cly = nf[8 + c]; chy = nf[12 + c]; clz = nf[16 + c];
                 chz = nf[20 + c]; cref = __float_as_int(nf[24 + c]);
             }
+        }
+        if (BUDGET && ++steps > budget) {
+            QuadRest rest{sp, stk_node0, stk_node1, stk_m0, stk_m1, cur, quad_mask_pack(m)};
+            if (handoff(rest)) return false;
+            budget = 0x7FFFFFFF;  // refused: walk on to the end
         }
         const bool mine = (m >> qb) & 1u;
         const float x0 = fmaf(clx, idx, -oxx), x1 = fmaf(chx, idx, -oxx);
@@ -563,7 +583,7 @@ RT_DEV void traverse_quad_lanes(const float4* __restrict__ nodes4, const float4*
         auto push = [&](int32_t node, uint64_t mask) {
             if (sp < kWideStack) {
                 const bool at0 = (int32_t)lane == sp, at1 = (int32_t)lane + 64 == sp;
-                const uint32_t cm = (uint32_t)mask | ((uint32_t)(mask >> 32) << 1);
+                const uint32_t cm = quad_mask_pack(mask);
                 stk_node0 = at0 ? node : stk_node0;
                 stk_m0 = at0 ? cm : stk_m0;
                 stk_node1 = at1 ? node : stk_node1;
@@ -629,6 +649,7 @@ RT_DEV void traverse_quad_lanes(const float4* __restrict__ nodes4, const float4*
         bool more = next_m != 0;
         if (!more) more = pop(next, next_m);
         if (more) {  // the next node's fetch overlaps the leaf tests
+            cur = next;
             nf = (const float*)(nodes4 + (size_t)next * 8);
             clx = nf[c]; chx = nf[4 + c]; cly = nf[8 + c]; chy = nf[12 + c]; clz = nf[16 + c];
             chz = nf[20 + c]; cref = __float_as_int(nf[24 + c]);
@@ -637,12 +658,28 @@ RT_DEV void traverse_quad_lanes(const float4* __restrict__ nodes4, const float4*
             leaves(leafs, lref, lmsk);
             if (ANY) {
                 alive &= ~quad_any(__ballot(h.prim >= 0));
-                if (alive == 0) return;
+                if (alive == 0) return true;
             }
         }
-        if (!more) return;
+        if (!more) return true;
         m = next_m;
     }
+}
+
+// The unbudgeted walk of the 16 rays of a wave from the root (lane 4q+c holds ray q; `want` is the
+// same in the 4 lanes of a quad).
+template <bool ANY>
+RT_DEV void traverse_quad_lanes(const float4* __restrict__ nodes4, const float4* __restrict__ tris,
+                                int32_t root, const Ray& r, bool want, int32_t skip_slot, Hit& h) {
+    h.t = r.tmax;
+    h.u = 0.0f;
+    h.v = 0.0f;
+    h.prim = -1;
+    h.slot = -1;
+    const uint64_t m = __ballot(want) & kQuadBits;
+    if (root == RT_EMPTY_ROOT || m == 0) return;
+    traverse_quad_core<ANY, false>(nodes4, tris, root, m, r, skip_slot, h, 0,
+                                   [](const QuadRest&) { return true; });
 }
 
 // Quad-lane traversal of the 16 rays of a wave (lane 4q+c holds ray q); h = the ray's closest hit
@@ -771,4 +808,6 @@ int check_hip(hipError_t e, const char* what);
 int check_launch(const char* what);
 int device_cu_count();
 rt_tuning current_tuning();
+// the call's tuning: *in validated, or the process default when in is NULL
+int resolve_tuning(const rt_tuning* in, rt_tuning& out, const char* what);
 }  // namespace rt

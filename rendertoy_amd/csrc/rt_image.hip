@@ -24,10 +24,30 @@ __global__ void __launch_bounds__(256) k_rgb_to_bgra8(const float* __restrict__ 
     bgra[i] = b | (g << 8) | (r << 16) | (255u << 24);
 }
 
+// One texel per thread: float4 RGBA (16 B) -> 4 UNORM8 bytes in R, G, B, A order.
+__global__ void __launch_bounds__(256) k_texels_to_rgba8(const float4* __restrict__ in, uint64_t n,
+                                                          uint32_t* __restrict__ out) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float4 c = in[i];
+    out[i] = unorm8(c.x) | (unorm8(c.y) << 8) | (unorm8(c.z) << 16) | (unorm8(c.w) << 24);
+}
+
 }  // namespace
 }  // namespace rt
 
 using namespace rt;
+
+extern "C" int rt_texels_to_rgba8(const float* rgba, uint64_t n_texels, uint8_t* out, void* stream) {
+    if (n_texels == 0) return RT_OK;
+    if (!rgba || !out) return set_error(RT_EINVAL, "rt_texels_to_rgba8: input and output are required");
+    if (((uintptr_t)rgba & 15) || ((uintptr_t)out & 3))
+        return set_error(RT_EINVAL, "rt_texels_to_rgba8: input must be 16-byte, output 4-byte aligned");
+    if (n_texels > (1ull << 36)) return set_error(RT_ECAPACITY, "rt_texels_to_rgba8: too many texels");
+    hipLaunchKernelGGL(k_texels_to_rgba8, dim3((uint32_t)((n_texels + 255) / 256)), dim3(256), 0,
+                       (hipStream_t)stream, (const float4*)rgba, n_texels, (uint32_t*)out);
+    return check_launch("rt_texels_to_rgba8");
+}
 
 extern "C" int rt_image_to_bgra8(const float* rgb, uint32_t n_pixels, uint8_t* bgra, void* stream) {
     if (n_pixels == 0) return RT_OK;

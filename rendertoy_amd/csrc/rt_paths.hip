@@ -254,7 +254,7 @@ extern "C" uint64_t rt_paths_workspace_bytes(uint32_t width, uint32_t height, ui
 
 extern "C" int rt_trace_paths(const rt_scene* scene, const rt_camera* cam, const rt_paths_params* p,
                               const rt_paths_out* out, void* workspace, uint64_t ws_bytes,
-                              void* stream) {
+                              const rt_tuning* tuning, void* stream) {
     int e = check_scene(scene, "rt_trace_paths");
     if (e) return e;
     if (!cam || !p || !out) return set_error(RT_EINVAL, "rt_trace_paths: NULL argument");
@@ -265,6 +265,8 @@ extern "C" int rt_trace_paths(const rt_scene* scene, const rt_camera* cam, const
     if (n == 0) return RT_OK;
     if (n > 0x7FFFFFFFull) return set_error(RT_ECAPACITY, "rt_trace_paths: width*height*spp >= 2^31");
     if (!out->radiance) return set_error(RT_EINVAL, "rt_trace_paths: radiance output is NULL");
+    rt_tuning tu;
+    if ((e = resolve_tuning(tuning, tu, "rt_trace_paths"))) return e;
     if (!workspace || ws_bytes < rt_paths_workspace_bytes(p->width, p->height, p->spp))
         return set_error(RT_EINVAL, "rt_trace_paths: workspace too small");
     if ((uintptr_t)workspace & 255)
@@ -286,7 +288,6 @@ extern "C" int rt_trace_paths(const rt_scene* scene, const rt_camera* cam, const
     a.q[1] = a.q[0] + 3 * n;
     a.cap = (uint32_t)n;
     hipStream_t s = (hipStream_t)stream;
-    const rt_tuning tu = current_tuning();
     hipLaunchKernelGGL(k_paths_init, dim3(1), dim3(64), 0, s, a);
     const SceneArgs sc = scene_args(scene);
     const uint32_t tiles_x = (p->width + 15) / 16, tiles_y = (p->height + 15) / 16;

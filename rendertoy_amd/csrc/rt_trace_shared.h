@@ -13,16 +13,18 @@ struct SceneArgs {
     const float4* tris;
     const float4* shade;
     const int32_t* mesh_offsets;
+    const int32_t* prim_slot;
     int32_t root;
     uint32_t n_meshes;
 };
 
-// Traversal of one ray per lane by one of three algorithms (MODE = RT_TRAVERSAL_*); `pk` = the
+// Traversal of one ray per lane by one of four algorithms (MODE = RT_TRAVERSAL_*); `pk` = the
 // wave's kPacketLds-float4 LDS area of the packet traversal:
 //  PACKET  the wave's rays walk one node sequence together (traverse_packet);
-//  LANE    each lane walks its own ray with a per-lane stack of `lane_stack` entries in LDS at `stk`
-//          (entries `stk_stride` words apart); a ray that overflows it is traced again by its wave
-//          with the packet traversal (128-entry stack, enough for any tree);
+//  LANE    each lane walks its own ray over the 4-wide nodes with a per-lane stack of `lane_stack`
+//          entries in LDS at `stk` (entries `stk_stride` words apart); a ray that overflows it is
+//          traced again by its wave with the packet traversal (128-entry stack, enough for any tree);
+//  BINARY  as LANE over the binary nodes (traverse_bin), same stack and fallback;
 //  HYBRID  packet traversal for at most `budget` steps; if the wave is not done by then, each lane
 //          continues alone from the root with its best hit so far as the bound (as LANE).
 // All three return the same hit: the closest hit does not depend on the visiting order, and a
@@ -31,7 +33,8 @@ struct SceneArgs {
 template <bool ANY, int MODE>
 RT_DEV void trace_block(const SceneArgs& sc, const Ray& r, bool want, int32_t skip_slot, Hit& h,
                         float4* pk, int32_t* stk, int stk_stride, int lane_stack, int budget) {
-    static_assert(MODE >= RT_TRAVERSAL_PACKET && MODE <= RT_TRAVERSAL_HYBRID, "resolved mode");
+    static_assert((MODE >= RT_TRAVERSAL_PACKET && MODE <= RT_TRAVERSAL_HYBRID) ||
+                      MODE == RT_TRAVERSAL_BINARY, "resolved mode");
     if (MODE == RT_TRAVERSAL_PACKET) {
         traverse_packet<ANY>(sc.nodes4, sc.tris, sc.root, r, want, skip_slot, h, pk);
         return;
@@ -49,9 +52,12 @@ RT_DEV void trace_block(const SceneArgs& sc, const Ray& r, bool want, int32_t sk
         h.slot = -1;
     }
     bool ok = true;
-    if (walk)
+    if (MODE == RT_TRAVERSAL_BINARY) {
+        if (walk) ok = traverse_bin<ANY>(sc.nodes, sc.tris, sc.root, r, skip_slot, h, stk, stk_stride, lane_stack);
+    } else if (walk) {
         ok = traverse_lane4<ANY>(sc.nodes4, sc.tris, sc.root, r, skip_slot, h, stk, stk_stride,
                                  lane_stack, fresh);
+    }
     const bool redo = !ok;
     if (__ballot(redo)) {
         Hit h2;
@@ -97,6 +103,7 @@ inline SceneArgs scene_args(const rt_scene* s) {
     a.tris = (const float4*)s->tris;
     a.shade = (const float4*)s->shade;
     a.mesh_offsets = s->mesh_offsets;
+    a.prim_slot = s->prim_slot;
     a.root = s->root;
     a.n_meshes = s->n_meshes;
     return a;

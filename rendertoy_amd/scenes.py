@@ -9,6 +9,14 @@ asset, models/dragon.obj, is absent).
   value_noise_rgba8()  RGBA8 value noise: octaves on 8/16/32/64 periodic lattices, smoothstep.
   random_transform()   rotation about a uniform axis + uniform scale.
 RNG: numpy PCG64 seeded per item (seed S(cfg, k) = 1000*cfg + k in the workloads).
+
+The three multi-scene workloads, shared by the benchmark and the tests that run them at size:
+  batch_scene(i)       scene i of the batched workload (256 scenes: 1k-500k triangles, random
+                       transform, image 128^2..1024^2), seed 3000 + i.
+  serving_scene(k)     scene k of the serving pool (64 scenes: 10k-2M triangles, 512^2 RGBA8
+                       texture), seeds 4000 + k / 4100 + k.
+  cluster_meshes()     the incoherent-ray cluster: 8 spheres of frequency 112 (250,880 triangles
+                       each), scale 0.35, centres uniform in [-1, 1]^3, seeds 5000 + k.
 """
 import math
 
@@ -190,3 +198,67 @@ def triangle_arrays(meshes):
     cat = lambda a, w: np.ascontiguousarray(np.concatenate(a).astype(np.float32)) if a else np.zeros((0, w), np.float32)
     return (cat(pos, 9), cat(nrm, 9), cat(uv, 6),
             np.concatenate(mo).astype(np.int32) if mo else np.zeros(0, np.int32))
+
+
+def batch_scene(i: int):
+    """Scene i of the batched workload: T = exp(U[ln 1e3, ln 5e5]) triangles (icosphere frequency
+    round(sqrt(T / 20))), a random rotation about a uniform axis and scale in [0.8, 1.2], and an
+    image side uniform over {128, 256, 512, 1024}, all from the seed 3000 + i. Returns (mesh dict
+    with pos / idx / nrm / uv, image side)."""
+    rng = np.random.default_rng(3000 + i)
+    T = math.exp(rng.uniform(math.log(1e3), math.log(5e5)))
+    f = max(1, int(round(math.sqrt(T / 20.0))))
+    res = int(rng.choice([128, 256, 512, 1024]))
+    M = random_transform(rng)
+    d = dict(icosphere(f, 3000 + i))
+    d["pos"] = (d["pos"].astype(np.float64) @ M).astype(np.float32)
+    nrm = d["nrm"].astype(np.float64) @ M
+    d["nrm"] = (nrm / np.linalg.norm(nrm, axis=1, keepdims=True)).astype(np.float32)
+    return d, res
+
+
+def serving_scene(k: int):
+    """Scene k of the serving pool: T = exp(U[ln 1e4, ln 2e6]) triangles (icosphere frequency
+    round(sqrt(T / 20))) from the seed 4000 + k, and a 512^2 RGBA8 value-noise texture from the
+    seed 4100 + k. Returns (mesh dict, texture [512, 512, 4] uint8)."""
+    rng = np.random.default_rng(4000 + k)
+    T = math.exp(rng.uniform(math.log(1e4), math.log(2e6)))
+    f = max(1, int(round(math.sqrt(T / 20.0))))
+    return icosphere(f, 4000 + k), value_noise_rgba8(512, 4100 + k)
+
+
+def cluster_meshes():
+    """The incoherent-ray workload's 8 displaced icospheres of frequency 112 (250,880 triangles
+    each), sphere k scaled by 0.35 and centred at a point uniform in [-1, 1]^3 from the seed
+    5000 + k."""
+    meshes = []
+    for k in range(8):
+        rng = np.random.default_rng(5000 + k)
+        c = rng.uniform(-1.0, 1.0, 3)
+        d = dict(icosphere(112, 5000 + k))
+        d["pos"] = (d["pos"].astype(np.float64) * 0.35 + c).astype(np.float32)
+        meshes.append(d)
+    return meshes
+
+
+def cluster_view(distance=0.5):
+    """A view of the incoherent-ray cluster in which nearly every camera ray hits: the camera looks
+    at the centre c of one sphere from `distance` away, placed between that sphere and the
+    cluster's centroid (so the lit hemisphere faces the other spheres and bounces reach them), with
+    a vertical fov such that the corners of a 16:9 frame stay inside the sphere's silhouette.
+    Returns (eye, vertical fov in radians, target). The first sphere (seed order) whose camera
+    position keeps clear of every other sphere is taken."""
+    meshes = cluster_meshes()
+    centres = [m["pos"].astype(np.float64).mean(0) for m in meshes]
+    radii = [np.linalg.norm(m["pos"] - c, axis=1).min() for m, c in zip(meshes, centres)]
+    centroid = np.mean(centres, axis=0)
+    for k, c in enumerate(centres):
+        toward = centroid - c
+        eye = c + distance * toward / np.linalg.norm(toward)
+        clear = all(np.linalg.norm(eye - o) > 2.0 * r for j, (o, r) in enumerate(zip(centres, radii)) if j != k)
+        if not clear:
+            continue
+        ang = math.asin(min(1.0, 0.95 * radii[k] / distance))   # silhouette half-angle, with margin
+        half = math.atan(math.tan(ang) / math.sqrt(1.0 + (16.0 / 9.0) ** 2))
+        return tuple(float(x) for x in eye), 2.0 * half, tuple(float(x) for x in c)
+    raise RuntimeError("no clear view of the cluster")

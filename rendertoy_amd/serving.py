@@ -1,13 +1,20 @@
-"""Render serving: resident scenes under a device-memory budget (LRU) and a request loop that
-batches every request admitted since the previous launch into one rt_render_batch launch.
+"""Render serving: resident scenes under a device-memory budget (LRU) and request loops.
 
-The reference has no serving layer (it renders one frame per call from a tutorial script); this
-module is the caller the batched kernel is for. Scheduling is continuous batching at launch
-granularity: while one batch runs on the GPU the loop admits arrivals, makes their scenes
-resident (building the BVH on a miss, evicting least-recently-used scenes that no queued batch
-uses) and enqueues the next batch behind it on the same stream, so the GPU goes from one batch
-to the next without waiting for the host. A request's latency is the GPU completion time of its
-batch (HIP event) minus its arrival time.
+The reference has no serving layer: its only request loop is the interactive frame loop of the
+tutorials (tutorials/lesson09_texture_mapping.py:112-143), one frame per iteration. Two loops run
+the same request traces here:
+
+- `RenderServer`: continuous batching. While batches run on the GPU the loop admits arrivals,
+  makes their scenes resident (building the BVH on a miss, evicting least-recently-used scenes
+  that no in-flight batch uses) and launches every request admitted since the previous launch as
+  one rt_render_batch launch, on the least busy of `n_streams` equal-priority streams: batches on
+  different streams overlap on the device (one batch's tail tiles share the CUs with the next
+  batch's bulk), and a new batch starts as soon as a slot (at most `max_inflight` batches in
+  flight) frees up.
+- `PlainServer`: the per-request baseline: one rt_render launch per request as it arrives, in
+  arrival order on one stream (what a caller of the plain entry point does).
+
+A request's latency is the GPU completion time of its work (HIP event) minus its arrival time.
 """
 import collections
 import dataclasses
@@ -150,20 +157,26 @@ def camera_for(eye):
 
 
 class RenderServer:
-    """Serves Requests in arrival order (see the module docstring)."""
+    """Serves Requests in arrival order by continuous batching (see the module docstring)."""
 
     def __init__(self, store: SceneStore, config: BatchConfig = None, max_batch: int = 512,
-                 max_inflight: int = 2):
+                 max_inflight: int = 4, n_streams: int = 4):
         self.store = store
         self.config = config or BatchConfig(waves_per_cu=16, tiles_per_grab=4, shadow="lane")
         self.max_batch = max_batch
         self.max_inflight = max_inflight
+        # equal (default) priority streams: batches on different streams overlap on the device
+        self.streams = [torch.cuda.Stream() for _ in range(max(1, n_streams))]
         self.batches = []      # requests per launched batch
         self.batch_log = []    # (requests, GPU start s, GPU end s) per batch, from the run's start
 
-    def _launch(self, reqs, t0_event):
+    def _pick_stream(self, inflight):
+        busy = collections.Counter(id(f["stream"]) for f in inflight)
+        return min(self.streams, key=lambda st: busy[id(st)])
+
+    def _launch(self, reqs, t0_event, stream):
         """Launches the longest prefix of `reqs` whose scenes fit the budget next to the pinned
-        ones; returns (record or None, requests left over)."""
+        ones on `stream`; returns (record or None, requests left over)."""
         residents = []
         for k, r in enumerate(reqs):
             try:
@@ -177,14 +190,15 @@ class RenderServer:
             rest = []
         rr = [RenderRequest(res.raycaster, camera_for(r.eye), r.size, r.size, textures=res.binding)
               for r, res in zip(reqs, residents)]
-        b = BatchRender(rr, config=self.config, stats=False)
+        # scenes built on the current stream are ready before the batch starts (BatchRender waits)
+        b = BatchRender(rr, config=self.config, stats=False, stream=stream)
         start = torch.cuda.Event(enable_timing=True)
-        start.record()
+        start.record(stream)
         b.launch()
         ev = torch.cuda.Event(enable_timing=True)
-        ev.record()
+        ev.record(stream)
         return {"reqs": reqs, "batch": b, "event": ev, "start": start, "residents": residents,
-                "t0": t0_event}, rest
+                "t0": t0_event, "stream": stream}, rest
 
     def _finish(self, f):
         done_s = f["t0"].elapsed_time(f["event"]) * 1e-3
@@ -207,7 +221,7 @@ class RenderServer:
         self._realtime = realtime  # else every request is present at t = 0: latency from there
         pending = collections.deque(sorted(requests, key=lambda r: r.arrival))
         queue = collections.deque()
-        inflight = collections.deque()
+        inflight = []
         torch.cuda.synchronize()
         t0_event = torch.cuda.Event(enable_timing=True)
         t0_event.record()
@@ -216,27 +230,75 @@ class RenderServer:
             now = time.perf_counter() - t0
             while pending and (not realtime or pending[0].arrival <= now):
                 queue.append(pending.popleft())
-            while inflight and inflight[0]["event"].query():
-                self._finish(inflight.popleft())
+            for f in [f for f in inflight if f["event"].query()]:
+                inflight.remove(f)
+                self._finish(f)
             if queue and len(inflight) < self.max_inflight:
                 n = min(len(queue), self.max_batch)
-                rec, rest = self._launch([queue.popleft() for _ in range(n)], t0_event)
+                rec, rest = self._launch([queue.popleft() for _ in range(n)], t0_event,
+                                         self._pick_stream(inflight))
                 queue.extendleft(reversed(rest))
                 if rec is not None:
                     inflight.append(rec)
                     continue
                 if not inflight:  # a single scene larger than the whole budget
                     raise MemoryError(f"scene {rest[0].scene} does not fit the budget")
-                inflight[0]["event"].synchronize()  # the oldest batch unpins its scenes
-                self._finish(inflight.popleft())
+                oldest = inflight[0]  # launch order
+                oldest["event"].synchronize()  # an in-flight batch unpins its scenes
+                inflight.remove(oldest)
+                self._finish(oldest)
                 continue
             if inflight and (len(inflight) >= self.max_inflight or not pending):
-                inflight[0]["event"].synchronize()  # no slot, or nothing left to admit
+                # no slot, or nothing left to admit: wait briefly for a batch to complete
+                if not any(f["event"].query() for f in inflight):
+                    time.sleep(2e-5)
                 continue
             if pending:  # a slot is free: wait for the next arrival
                 dt = pending[0].arrival - (time.perf_counter() - t0)
                 if dt > 0:
                     time.sleep(min(dt, 2e-4 if inflight else 1e-3))
+        torch.cuda.synchronize()
+        return requests
+
+
+class PlainServer:
+    """The per-request baseline: each request is rendered by its own rt_render launch as it
+    arrives, in arrival order on one stream (scenes resident in the same store)."""
+
+    def __init__(self, store: SceneStore, stream=None):
+        self.store = store
+        self.stream = stream or torch.cuda.Stream()
+
+    def run(self, requests: typing.List[Request], realtime: bool = True):
+        pending = sorted(requests, key=lambda r: r.arrival)
+        torch.cuda.synchronize()
+        t0_event = torch.cuda.Event(enable_timing=True)
+        t0_event.record()
+        t0 = time.perf_counter()
+        done = []
+        for r in pending:
+            if realtime:
+                dt = r.arrival - (time.perf_counter() - t0)
+                while dt > 0:
+                    time.sleep(min(dt, 2e-4))
+                    dt = r.arrival - (time.perf_counter() - t0)
+            res = self.store.acquire(r.scene)
+            self.stream.wait_stream(torch.cuda.current_stream())
+            with torch.cuda.stream(self.stream):
+                start = torch.cuda.Event(enable_timing=True)
+                start.record(self.stream)
+                img = res.raycaster.render(camera_for(r.eye), r.size, r.size, textures=res.binding)
+                ev = torch.cuda.Event(enable_timing=True)
+                ev.record(self.stream)
+            self.store.release(res)
+            done.append((r, start, ev, img if r.keep_image else None))
+        torch.cuda.synchronize()
+        for r, start, ev, img in done:
+            done_s = t0_event.elapsed_time(ev) * 1e-3
+            r.latency = done_s - (r.arrival if realtime else 0.0)
+            r.service = done_s - t0_event.elapsed_time(start) * 1e-3
+            if r.keep_image:
+                r.image = img
         return requests
 
 
@@ -260,10 +322,13 @@ def make_requests(n: int, n_scenes: int, rate: float, seed: int, sizes=(256, 512
     return out
 
 
-def latency_summary(reqs):
+def latency_summary(reqs, realtime=True):
     lat = np.array([r.latency for r in reqs]) * 1e3
     svc = np.array([r.service for r in reqs]) * 1e3
-    span = max(r.arrival + r.latency for r in reqs) - min(r.arrival for r in reqs)
+    if realtime:
+        span = max(r.arrival + r.latency for r in reqs) - min(r.arrival for r in reqs)
+    else:  # every request present at t = 0: the makespan
+        span = max(r.latency for r in reqs)
     return {"requests": len(reqs), "p50_ms": round(float(np.percentile(lat, 50)), 3),
             "p99_ms": round(float(np.percentile(lat, 99)), 3),
             "max_ms": round(float(lat.max()), 3),

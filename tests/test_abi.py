@@ -50,7 +50,7 @@ def test_sizes_query_and_error_path_without_gpu():
     assert lib.rt_scene_sizes_for(1 << 30, ctypes.byref(s)) == -3
     assert b"too many" in lib.rt_last_error()
     # NULL scene pointer is rejected before any launch
-    assert lib.rt_trace_closest(None, None, 1, None, None) == -1
+    assert lib.rt_trace_closest(None, None, 1, None, None, None) == -1
 
 
 PROBE = r"""
@@ -64,8 +64,9 @@ int main(void) {
     O(rt_mesh_desc, n_tris) O(rt_mesh_desc, vertex_stride) O(rt_mesh_desc, pos_offset)
     O(rt_mesh_desc, nrm_offset) O(rt_mesh_desc, uv_offset)
     S(rt_scene) O(rt_scene, root) O(rt_scene, nodes) O(rt_scene, nodes4) O(rt_scene, tris)
-    O(rt_scene, shade) O(rt_scene, mesh_offsets) O(rt_scene, bounds)
+    O(rt_scene, shade) O(rt_scene, mesh_offsets) O(rt_scene, bounds) O(rt_scene, prim_slot)
     S(rt_scene_sizes) O(rt_scene_sizes, nodes4_bytes) O(rt_scene_sizes, workspace_bytes)
+    O(rt_scene_sizes, prim_slot_bytes)
     S(rt_rays) O(rt_rays, tmin) O(rt_rays, stride) O(rt_rays, tmin_default) O(rt_rays, tmax_default)
     S(rt_hits) O(rt_hits, mesh) O(rt_hits, stride)
     S(rt_camera) O(rt_camera, up) O(rt_camera, forward)
@@ -78,7 +79,8 @@ int main(void) {
     S(rt_render_job) O(rt_render_job, cam) O(rt_render_job, params) O(rt_render_job, out)
     S(rt_batch_config) O(rt_batch_config, tiles_per_grab) O(rt_batch_config, primary)
     O(rt_batch_config, shadow) O(rt_batch_config, lane_stack) O(rt_batch_config, packet_budget)
-    O(rt_tuning, packet_budget) O(rt_tuning, builder)
+    O(rt_tuning, packet_budget) O(rt_tuning, builder) O(rt_tuning, query)
+    O(rt_tuning, handoff_steps) O(rt_tuning, item_steps)
     S(rt_paths_params) O(rt_paths_params, spp) O(rt_paths_params, seed) O(rt_paths_params, albedo)
     O(rt_paths_params, ray_tmin) S(rt_paths_out) O(rt_paths_out, counts)
     return 0;
@@ -116,18 +118,71 @@ def test_tuning_get_set_and_range_checks_without_gpu():
     d = ren.get_tuning()
     assert d == {"primary": _native.RT_TRAVERSAL_QUAD, "shadow": _native.RT_TRAVERSAL_LANE,
                  "super_tile": 0, "lane_stack": _native.RT_LANE_STACK_MAX, "packet_budget": 256,
-                 "builder": _native.RT_BUILD_LBVH}
-    with ren.tuning(primary="hybrid", lane_stack=3, super_tile=2, packet_budget=7, builder="ploc"):
+                 "builder": _native.RT_BUILD_LBVH, "query": _native.RT_TRAVERSAL_BINARY,
+                 "handoff_steps": 0, "item_steps": 32}
+    with ren.tuning(primary="hybrid", lane_stack=3, super_tile=2, packet_budget=7, builder="ploc",
+                    query="lane", handoff_steps=9, item_steps=5):
         assert ren.get_tuning() == {"primary": 2, "shadow": 1, "super_tile": 2, "lane_stack": 3,
-                                    "packet_budget": 7, "builder": _native.RT_BUILD_PLOC}
+                                    "packet_budget": 7, "builder": _native.RT_BUILD_PLOC,
+                                    "query": _native.RT_TRAVERSAL_LANE, "handoff_steps": 9,
+                                    "item_steps": 5}
     assert ren.get_tuning() == d
     for bad in ({"lane_stack": 0}, {"lane_stack": 33}, {"primary": 5}, {"shadow": 3},
-                {"super_tile": 65}, {"packet_budget": 0}, {"builder": 2}):
+                {"super_tile": 65}, {"packet_budget": 0}, {"builder": 2}, {"query": "packet"},
+                {"query": "quad"}, {"item_steps": 0}, {"handoff_steps": (1 << 20) + 1}):
         with pytest.raises(RuntimeError, match="rt_set_tuning"):
             ren.set_tuning(**bad)
     assert ren.get_tuning() == d
     with pytest.raises(TypeError):
         ren.set_tuning(tile=4)
+
+
+def test_per_call_tuning_is_checked_before_any_launch_without_gpu():
+    """A per-call rt_tuning is range-checked like rt_set_tuning, before any device work (the
+    fake addresses below are never touched), and NULL means the process default."""
+    from rendertoy_amd import _native as N
+    from rendertoy_amd.tuning import make_tuning
+    lib = N.load()
+    sc = N.rt_scene()
+    sc.root = N.RT_EMPTY_ROOT
+    cam = N.rt_camera()
+    hits = N.rt_hits(0x1000, None, None, None, None, None, 4)
+    rays = N.rt_rays(0x1000, 0x1000, 0x1000, 0x1000, 0x1000, 0x1000, None, None, 4, 0.0, 1.0)
+    for bad in (dict(lane_stack=0), dict(primary=7), dict(query=N.RT_TRAVERSAL_PACKET), dict(builder=3)):
+        t = make_tuning(None, **bad)
+        assert lib.rt_trace_camera(ctypes.byref(sc), ctypes.byref(cam), 8, 8, ctypes.byref(hits),
+                                   None, 0, ctypes.byref(t), None) == -1
+        assert b"rt_trace_camera: tuning" in lib.rt_last_error()
+        assert lib.rt_trace_closest(ctypes.byref(sc), ctypes.byref(rays), 8, ctypes.byref(hits),
+                                    ctypes.byref(t), None) == -1
+        assert lib.rt_trace_any(ctypes.byref(sc), ctypes.byref(rays), 8, 0x1000, ctypes.byref(t), None) == -1
+        assert b"rt_trace_any: tuning" in lib.rt_last_error()
+    # nothing to trace: no launch, whatever the tuning
+    assert lib.rt_trace_camera(ctypes.byref(sc), ctypes.byref(cam), 0, 8, ctypes.byref(hits), None, 0,
+                               None, None) == 0
+
+
+def test_export_and_texel_entries_reject_bad_arguments_without_gpu():
+    from rendertoy_amd import _native as N
+    lib = N.load()
+    sc = N.rt_scene()
+    sc.n_tris, sc.n_meshes, sc.root = 10, 1, 0
+    sc.nodes, sc.bounds, sc.prim_slot = 0x1000, 0x2000, 0x6000
+    m = N.rt_mesh_desc(0x3000, None, 30, 10, 80, 0, N.RT_ABSENT, N.RT_ABSENT)
+    assert lib.rt_scene_export(None, None, 0, 0x4000, 0x5000, None, None) == -1
+    assert lib.rt_scene_export(ctypes.byref(sc), ctypes.byref(m), 1, 0x4008, 0x5000, None, None) == -1
+    assert b"16-byte" in lib.rt_last_error()
+    assert lib.rt_scene_export(ctypes.byref(sc), ctypes.byref(m), 2, 0x4000, 0x5000, None, None) == -1
+    m.n_tris = 9
+    assert lib.rt_scene_export(ctypes.byref(sc), ctypes.byref(m), 1, 0x4000, 0x5000, None, None) == -1
+    assert b"do not match" in lib.rt_last_error()
+    m.n_tris = 10
+    sc.prim_slot = None
+    assert lib.rt_scene_export(ctypes.byref(sc), ctypes.byref(m), 1, 0x4000, 0x5000, None, None) == -1
+    assert lib.rt_texels_to_rgba8(None, 0, None, None) == 0
+    assert lib.rt_texels_to_rgba8(None, 4, 0x1000, None) == -1
+    assert lib.rt_texels_to_rgba8(0x1008, 4, 0x2000, None) == -1
+    assert lib.rt_texels_to_rgba8(0x1000, 4, 0x2002, None) == -1
 
 
 def test_batch_entry_rejects_bad_arguments_without_gpu():
@@ -163,19 +218,19 @@ def test_paths_entry_rejects_bad_arguments_without_gpu():
     ok = N.rt_paths_params(8, 8, 4, 2, 1, 0, 0.7, 1.0, 1e-4)
     need = lib.rt_paths_workspace_bytes(8, 8, 4)
     assert need == 256 + 4 * 256 + 2 * 48 * 256  # counters, samples, two queues
-    assert lib.rt_trace_paths(None, ctypes.byref(cam), ctypes.byref(ok), ctypes.byref(out), ws, need, None) == -1
+    assert lib.rt_trace_paths(None, ctypes.byref(cam), ctypes.byref(ok), ctypes.byref(out), ws, need, None, None) == -1
     for bad in (N.rt_paths_params(8, 8, 0, 2, 1, 0, 0.7, 1.0, 1e-4),
                 N.rt_paths_params(8, 8, 4, 17, 1, 0, 0.7, 1.0, 1e-4)):
         assert lib.rt_trace_paths(ctypes.byref(sc), ctypes.byref(cam), ctypes.byref(bad),
-                                  ctypes.byref(out), ws, need, None) == -1
+                                  ctypes.byref(out), ws, need, None, None) == -1
     assert lib.rt_trace_paths(ctypes.byref(sc), ctypes.byref(cam), ctypes.byref(ok), ctypes.byref(out),
-                              ws, need - 1, None) == -1
+                              ws, need - 1, None, None) == -1
     assert lib.rt_trace_paths(ctypes.byref(sc), ctypes.byref(cam), ctypes.byref(ok), ctypes.byref(out),
-                              ws + 16, need, None) == -1
+                              ws + 16, need, None, None) == -1
     assert b"256-byte" in lib.rt_last_error()
     big = N.rt_paths_params(65536, 65536, 1, 2, 1, 0, 0.7, 1.0, 1e-4)
     assert lib.rt_trace_paths(ctypes.byref(sc), ctypes.byref(cam), ctypes.byref(big), ctypes.byref(out),
-                              ws, need, None) == -3
+                              ws, need, None, None) == -3
 
 
 def test_mesh_and_image_entries_reject_bad_arguments_without_gpu():
