@@ -331,6 +331,8 @@ def measure_serving(args, ren, scenes, torch):
                 sm["mean_batch"] = round(float(np.mean(srv.batches)), 2)
             row[name] = sm
         rates[str(int(rate))] = row
+        note(f"serving at {rate:.0f} rps: p99 batched {row['batched'].get('p99_ms')} ms, "
+             f"plain {row['plain'].get('p99_ms')} ms")
     sat = {}
     for name, server in (("batched", srv), ("plain", plain)):
         reqs = serving.make_requests(args.serving_requests, 64, 1e9, 4999)
@@ -434,6 +436,14 @@ def accuracy_report(tw, meshes, scenes, oracle, cam, W, H, prim, t, img, light, 
             "rgb_pixels_checked": int(ok.sum()), "shadow_flagged": int(((sflags & 8) != 0).sum())}
 
 
+_T0 = time.time()
+
+
+def note(msg):
+    """A progress line on stderr (stdout carries only the result line)."""
+    print(f"[bench {time.time() - _T0:6.1f} s] {msg}", file=sys.stderr, flush=True)
+
+
 def percentile(a, q):
     return float(np.percentile(np.asarray(a, dtype=np.float64), q))
 
@@ -485,6 +495,7 @@ def main():
            "v": torch.empty(n_rays, dtype=torch.float32, device=dev),
            "prim": torch.empty(n_rays, dtype=torch.int32, device=dev)}
 
+    note(f"scene built: {rc.n_tris} triangles, build {np.median(builds):.2f} ms")
     # ---- closest-hit steps ----
     for _ in range(args.warmup):
         rc.cast_camera(cam, W, H, out=out)
@@ -505,6 +516,7 @@ def main():
     prim = out["prim"].cpu().numpy()
     hits = int((prim >= 0).sum())
 
+    note(f"closest hit: {mrays:.1f} Mrays/s, {ms_per_step:.4f} ms per step")
     # ---- full render latency (primary + shadow + shade), texture bound once ----
     texg = torch.from_numpy(tex).to(dev)
     binding = ren._raycaster.TextureBinding([texg], [0, 0])
@@ -568,22 +580,30 @@ def main():
         "wall_s_timed": round(wall, 4),
     }
 
+    note(f"render: p50 {percentile(render_ms, 50):.4f} ms")
     if not args.no_batched:
+        note("batched")
         result["batched"] = measure_batched(args, ren, scenes, torch)
     if not args.no_serving:
+        note("serving")
         result["serving"] = measure_serving(args, ren, scenes, torch)
     if not args.no_incoherent:
+        note("incoherent")
         result["incoherent"] = measure_incoherent(args, ren, scenes, torch)
     if not args.no_ray_cast:
+        note("ray_cast")
         result["ray_cast"] = measure_ray_cast(args, rc, cam, W, H, torch)
     if not args.no_mesh:
+        note("mesh")
         result["mesh"] = measure_mesh(args, ren, sphere, torch)
     if not args.no_c1:
+        note("c1")
         result["c1"] = measure_c1(args, ren, scenes, torch)
 
     # ---- roofline and CPU baseline from the CPU restatement (oracle) ----
     if not args.no_roofline or not args.no_cpu_baseline or not args.no_accuracy:
         import oracle
+        note("oracle: roofline, accuracy, cpu baseline")
         tw_t0 = time.time()
         pos, nrm, uv, mo = scenes.triangle_arrays(meshes)
         tw = oracle.TwinScene(pos, nrm, uv, mo, [1, 1], [1, 1])
@@ -618,9 +638,11 @@ def main():
             }
             result["parity_primary_ids_equal_twin"] = bool(np.array_equal(prim, wv["prim"]))
         if not args.no_accuracy:
+            note("accuracy")
             result["accuracy"] = accuracy_report(tw, meshes, scenes, oracle, cam, W, H, r_prim, r_t,
                                                  r_img, light, tex)
         if not args.no_cpu_baseline:
+            note("cpu baseline")
             o, d = oracle.twin_generate_rays(cam.as_array(), W, H)
             threads = host_threads()
             oracle.set_threads(threads)

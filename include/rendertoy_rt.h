@@ -182,11 +182,9 @@ typedef struct rt_tuning {
                                  the 4-wide nodes) for rt_trace_closest / rt_trace_any; both keep
                                  their stacks in LDS (lane_stack entries per lane) and trace a ray
                                  that needs more again with the packet traversal */
-    uint32_t handoff_steps;   /* rt_trace_camera with a workspace (QUAD): node steps a 16-ray packet
-                                 takes before it hands the rest of its walk to other waves (0 = never,
-                                 the default) */
-    uint32_t item_steps;      /* node steps a handed-off subtree is walked before what is left of it
-                                 is handed on again in pieces (>= 1; default 32) */
+    uint32_t handoff_steps;   /* must be 0: reserved for a hand-off of the slowest camera packets' walks
+                                 to other waves, which is switched off (its first runs did not finish) */
+    uint32_t item_steps;      /* reserved with handoff_steps: 1..2^20, default 32 */
 } rt_tuning;
 
 /* Hierarchy builders of rt_scene_build; both start from the same Morton order and give the same
@@ -259,11 +257,8 @@ int rt_generate_rays(const rt_camera* cam, uint32_t width, uint32_t height,
                      void* stream);
 
 /* Closest hit of the primary rays of a W x H image (ray generation fused into the traversal);
- * hit record of pixel (col,row) at index row*W + col. `workspace_dev` (rt_trace_camera_workspace_bytes
- * bytes, 256-byte aligned, any contents) lets the slowest 16-ray packets of the QUAD traversal hand
- * their remaining subtrees to other waves (rt_tuning.handoff_steps, item_steps); NULL traces every
- * packet to its end in its own wave. Results are identical either way. Calls sharing a workspace
- * must not overlap (one workspace per stream). */
+ * hit record of pixel (col,row) at index row*W + col. `workspace_dev` needs
+ * rt_trace_camera_workspace_bytes(W, H) bytes (currently 0: NULL is accepted). */
 uint64_t rt_trace_camera_workspace_bytes(uint32_t width, uint32_t height);
 int rt_trace_camera(const rt_scene* scene, const rt_camera* cam, uint32_t width, uint32_t height,
                     const rt_hits* hits, void* workspace_dev, uint64_t workspace_bytes,

@@ -159,6 +159,66 @@ class BatchRender:
         return int(self.workspace[4:8].view(torch.int32).item())
 
 
+class BatchSlot:
+    """Reusable buffers for launching batches back to back on one stream without per-batch
+    allocation of the job table: a pinned host table, its device copy and the batch workspace,
+    sized for `max_jobs`. `launch(requests)` fills the tables from the requests' render
+    arguments, copies them asynchronously and launches rt_render_batch; the output images are
+    allocated per request (torch's caching allocator) and returned. Before the host table is
+    refilled, the previous launch's copy out of it is waited for (an event after the copy)."""
+
+    def __init__(self, max_jobs: int, stream=None, config: BatchConfig = None):
+        lib = N.require_gpu()
+        self.max_jobs = int(max_jobs)
+        self.stream = stream if stream is not None else torch.cuda.current_stream()
+        self.config = config or BatchConfig()
+        nbytes = ctypes.sizeof(N.rt_render_job) * self.max_jobs
+        self._jobs = (N.rt_render_job * self.max_jobs)()
+        self.host = torch.empty(nbytes, dtype=torch.uint8).pin_memory()
+        with torch.cuda.stream(self.stream):
+            self.dev = torch.empty(nbytes, dtype=torch.uint8, device="cuda")
+            self.workspace = torch.empty(lib.rt_render_batch_workspace_bytes(self.max_jobs),
+                                         dtype=torch.uint8, device="cuda")
+        self._copied = None
+        self._keep = []
+
+    def launch(self, requests: typing.List[RenderRequest], config: BatchConfig = None):
+        """Renders the requests (at most max_jobs) in one launch on the slot's stream; returns
+        their images [h, w, 3] float32 (ready when the stream reaches the launch's end)."""
+        lib = N.load()
+        n = len(requests)
+        if not 0 < n <= self.max_jobs:
+            raise ValueError(f"1..{self.max_jobs} requests per launch")
+        if self._copied is not None:
+            self._copied.synchronize()  # the previous table left the host buffer
+        images, keep = [], []
+        with torch.cuda.stream(self.stream):
+            for i, r in enumerate(requests):
+                if r.width * r.height > N.RT_BATCH_MAX_JOB_PIXELS:
+                    raise ValueError(f"request {i}: more than {N.RT_BATCH_MAX_JOB_PIXELS} pixels")
+                p, o, out, hits, k = r.raycaster._render_args(
+                    r.width, r.height, r.light, r.textures, r.mesh_textures, r.shadows, r.bilinear,
+                    r.ambient, r.background, r.shadow_tmin, r.out, r.return_hits, None)
+                j = self._jobs[i]
+                j.scene = r.raycaster._scene
+                j.cam = r.camera.to_c()
+                j.params = p
+                j.out = o
+                images.append(out)
+                keep += k
+            nbytes = ctypes.sizeof(N.rt_render_job) * n
+            ctypes.memmove(self.host.data_ptr(), ctypes.addressof(self._jobs), nbytes)
+            self.dev[:nbytes].copy_(self.host[:nbytes], non_blocking=True)
+            self._copied = torch.cuda.Event()
+            self._copied.record(self.stream)
+            cfg = (config or self.config).to_c()
+            N.check(lib.rt_render_batch(self.dev.data_ptr(), n, ctypes.byref(cfg), self.workspace.data_ptr(),
+                                        self.workspace.numel(), N.stream_handle(self.stream)),
+                    "rt_render_batch")
+        self._keep = keep  # alive until the next launch on this slot's stream
+        return images
+
+
 def time_launches(fn, reps, warmup=1):
     """Per-call durations (ms) of `fn` from HIP events on the current stream."""
     for _ in range(warmup):

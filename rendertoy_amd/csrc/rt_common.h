@@ -454,8 +454,8 @@ struct QuadRest {
 // The walk starts at node `start` with the rays of `m` (bit 4q = ray q) and the candidates already
 // in h (all four lanes of a quad hold the same one: a bound found elsewhere, or none). With
 // BUDGET, after `budget` node steps it stops, calls handoff(rest) with what is left, and returns
-// false; handoff may return false to refuse (the walk then continues unbudgeted). Called by all 64
-// lanes with uniform control flow.
+// false; handoff may return false to refuse (the walk then continues unbudgeted); no kernel takes
+// the budgeted form at present. Called by all 64 lanes with uniform control flow.
 template <bool ANY, bool BUDGET, typename Handoff>
 RT_DEV bool traverse_quad_core(const float4* __restrict__ nodes4, const float4* __restrict__ tris,
                                int32_t start, uint64_t m, const Ray& r, int32_t skip_slot, Hit& h,

@@ -213,140 +213,10 @@ __global__ void __launch_bounds__(256) k_trace_camera(SceneArgs sc, rt_camera ca
     if (p.valid) store_hit(sc, hits, (uint64_t)p.row * W + p.col, h);
 }
 
-// ---- camera rays: hand-off of the slowest packets to a queue of subtrees ----
-// A frame's time is set by a few 16-ray packets whose rays graze the geometry: their walks are long
-// dependent chains of node fetches while the rest of the GPU idles. With a workspace, a packet that
-// has taken `handoff` node steps stops and hands what it has left -- its stack entries and the
-// node it would visit next, each with its ray bits -- to a queue; the best hit of each of its rays
-// so far goes to a 64-bit key per ray, (float bits of t) << 32 | primitive (positive floats order
-// as their bits, so the integer minimum is the closest hit with ties to the smaller primitive, the
-// walk's own rule). A persistent kernel then walks the queued subtrees on every wave of the GPU,
-// each item bounded by its rays' current keys and re-queued in pieces after `item_steps` steps,
-// merging its hits by atomicMin; a last kernel turns the keys of the handed-off packets into hit
-// records (the winner's FP32 Moller-Trumbore and FP64 refinement, as the packet would have).
-// The closest hit does not depend on the order boxes are visited in, so the hits are the same.
-//
-// Workspace: uint32 header[64], deferred[P] packet ids, keys[16 P] uint64, items[Q] of 4 uint32
-// {deferred slot, node, packed ray bits, 0}; P = 4x4-pixel packets of the image, Q = 8 P. The
-// header's counters are cleared at the start of each call; nothing else needs a start value.
-// Queue protocol: a producer reserves n consecutive items (tail), counts them as pending, writes
-// them, and publishes them by moving `published` from its first item to past its last -- in
-// reservation order, so `published` never passes an unwritten item; a consumer claims the item at
-// `head` while head < published. A wave that walks an item keeps it pending until it is done
-// (including what it re-queues), so an empty queue with nothing pending stays empty.
-// Every cross-wave word (counters, items, keys) is read and written with agent-scope atomics, which
-// the device keeps coherent across the XCDs' L2s; waits poll with relaxed loads (an acquire load
-// would invalidate the L2 on every poll, starving the waves doing the work) and end at a
-// wall-clock deadline, so no wave waits forever whatever happens.
-enum { kQStatus = 0, kQDeferred = 1, kQTail = 2, kQHead = 3, kQPending = 4, kQPublished = 5,
-       kQHeaderWords = 64 };
-constexpr uint32_t kQNone = 0xFFFFFFFFu;
-constexpr uint64_t kWaitTicks = 200000000ull;  // 2 s of the 100 MHz real-time counter
-
-RT_DEV uint64_t now_ticks() { return __builtin_amdgcn_s_memrealtime(); }
-
-struct CamQueue {
-    uint32_t* hdr;
-    uint32_t* deferred;
-    unsigned long long* keys;
-    uint32_t* items;
-    uint32_t tiles_x4;  // 4x4-pixel packets per image row
-    uint32_t cap;       // items
-};
-
-struct CamQueueLayout {
-    uint64_t deferred, keys, items, bytes;
-    uint32_t packets, cap;
-};
-
-inline CamQueueLayout cam_queue_layout(uint32_t W, uint32_t H) {
-    auto al = [](uint64_t x) { return (x + 255) & ~255ull; };
-    CamQueueLayout l;
-    l.packets = ((W + 3) / 4) * ((H + 3) / 4);
-    l.cap = 8u * l.packets > 4096u ? 8u * l.packets : 4096u;
-    l.deferred = al(4ull * kQHeaderWords);
-    l.keys = l.deferred + al(4ull * l.packets);
-    l.items = l.keys + al(8ull * 16 * l.packets);
-    l.bytes = l.items + al(16ull * l.cap);
-    return l;
-}
-
-RT_DEV uint32_t ld_relaxed(uint32_t* p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
-RT_DEV void st_relaxed(uint32_t* p, uint32_t v) { __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
-
-RT_DEV unsigned long long hit_key(float t, int32_t prim) {
-    return ((unsigned long long)__float_as_uint(t) << 32) | (uint32_t)prim;
-}
-
-// Pixel of ray q (0..15) of 4x4 packet `pkt`.
-RT_DEV TilePixel packet_pixel(uint32_t pkt, uint32_t q, uint32_t tiles_x4, uint32_t W, uint32_t H) {
-    TilePixel p;
-    p.col = (pkt % tiles_x4) * 4 + (q & 3);
-    p.row = (pkt / tiles_x4) * 4 + (q >> 2);
-    p.valid = p.col < W && p.row < H;
-    return p;
-}
-
-// Reserves n consecutive items (lane 0 reserves, every lane gets the first index), or kQNone if
-// the queue is full; on success the pending count grows by n before any of them is published.
-RT_DEV uint32_t q_reserve(const CamQueue& q, uint32_t n) {
-    uint32_t base = kQNone;
-    if (__lane_id() == 0) {
-        uint32_t t = ld_relaxed(&q.hdr[kQTail]);
-        for (int tries = 0; tries < 1024 && t + n <= q.cap; ++tries) {
-            const uint32_t seen = atomicCAS(&q.hdr[kQTail], t, t + n);
-            if (seen == t) {
-                base = t;
-                atomicAdd(&q.hdr[kQPending], n);
-                break;
-            }
-            t = seen;
-        }
-    }
-    return (uint32_t)__builtin_amdgcn_readfirstlane((int)base);
-}
-
-RT_DEV void q_put(const CamQueue& q, uint32_t idx, uint32_t d, int32_t node, uint32_t cm) {
-    uint32_t* it = q.items + 4ull * idx;
-    st_relaxed(it, d);
-    st_relaxed(it + 1, (uint32_t)node);
-    st_relaxed(it + 2, cm);
-}
-
-// Writes what a budgeted walk left (the `rest` of traverse_quad_core) as the sp + 1 reserved
-// items from `base`, all of deferred packet d, and publishes them after every earlier
-// reservation (bounded wait). Called by all 64 lanes.
-RT_DEV bool q_publish(const CamQueue& q, const QuadRest& rest, uint32_t d, uint32_t base) {
-    const uint32_t lane = __lane_id(), n = (uint32_t)rest.sp + 1;
-    if ((int32_t)lane < rest.sp) q_put(q, base + lane, d, rest.node0, rest.m0);
-    if ((int32_t)lane + 64 < rest.sp) q_put(q, base + 64 + lane, d, rest.node1, rest.m1);
-    if (lane == 0) q_put(q, base + (uint32_t)rest.sp, d, rest.cur, rest.cur_m);
-    // the wave's item stores complete before `published` moves (both are agent-scope atomics)
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-    __builtin_amdgcn_wave_barrier();
-    if (lane == 0) {
-        const uint64_t t0 = now_ticks();
-        bool late = false;
-        while (ld_relaxed(&q.hdr[kQPublished]) != base) {
-            if (now_ticks() - t0 > kWaitTicks) {
-                late = true;
-                break;
-            }
-            __builtin_amdgcn_s_sleep(1);
-        }
-        if (late) atomicOr(&q.hdr[kQStatus], 2u);  // never expected
-        else st_relaxed(&q.hdr[kQPublished], base + n);
-    }
-    return true;
-}
-
 // Camera closest hit with quad-lane packets: a 256-thread block covers 8x8 pixels, wave w the 4x4
-// sub-tile ((w & 1) * 4, (w >> 1) * 4), lanes 4q..4q+3 the pixel (q & 3, q >> 2) of it. HANDOFF:
-// a packet past `budget` node steps is handed to the queue (see above).
-template <bool HANDOFF>
+// sub-tile ((w & 1) * 4, (w >> 1) * 4), lanes 4q..4q+3 the pixel (q & 3, q >> 2) of it.
 __global__ void __launch_bounds__(256) k_trace_camera_quad(SceneArgs sc, rt_camera cam, uint32_t W,
-                                                            uint32_t H, TileMap tm, rt_hits hits,
-                                                            CamQueue q, int budget) {
+                                                           uint32_t H, TileMap tm, rt_hits hits) {
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6, qr = lane >> 2;
     uint32_t tx, ty;
     if (!block_tile(tm, tx, ty)) return;
@@ -356,149 +226,9 @@ __global__ void __launch_bounds__(256) k_trace_camera_quad(SceneArgs sc, rt_came
     p.valid = p.col < W && p.row < H;
     const Ray r = camera_ray(cam, W, H, p);
     Hit h;
-    if constexpr (HANDOFF) {
-        h.t = r.tmax;
-        h.u = 0.0f;
-        h.v = 0.0f;
-        h.prim = -1;
-        h.slot = -1;
-        const uint64_t m = __ballot(p.valid) & kQuadBits;
-        bool done = true;
-        if (sc.root != RT_EMPTY_ROOT && m != 0) {
-            const uint32_t pkt = (ty * 2 + (wave >> 1)) * q.tiles_x4 + tx * 2 + (wave & 1);
-            done = traverse_quad_core<false, true>(
-                sc.nodes4, sc.tris, sc.root, m, r, -1, h, budget, [&](const QuadRest& rest) -> bool {
-                    const uint32_t base = q_reserve(q, (uint32_t)rest.sp + 1);
-                    if (base == kQNone) return false;  // queue full: this wave walks on
-                    uint32_t d = 0;
-                    if (lane == 0) {
-                        d = atomicAdd(&q.hdr[kQDeferred], 1u);
-                        q.deferred[d] = pkt;
-                    }
-                    d = (uint32_t)__builtin_amdgcn_readfirstlane((int)d);
-                    quad_closest(h.t, h.u, h.v, h.prim, h.slot);
-                    if ((lane & 3) == 0) q.keys[16ull * d + qr] = hit_key(h.t, h.prim);
-                    return q_publish(q, rest, d, base);
-                });
-        }
-        if (!done) return;  // the packet's rays are finished by the queue (wave-uniform)
-        quad_closest(h.t, h.u, h.v, h.prim, h.slot);
-    } else {
-        traverse_quad<false>(sc.nodes4, sc.tris, sc.root, r, p.valid, -1, h);
-    }
+    traverse_quad<false>(sc.nodes4, sc.tris, sc.root, r, p.valid, -1, h);
     if (p.valid && (lane & 3) == 0) {
         if (h.prim >= 0) refine_hit(sc.tris, r, h);
-        store_hit(sc, hits, (uint64_t)p.row * W + p.col, h);
-    }
-}
-
-__global__ void k_camq_prepare(uint32_t* __restrict__ hdr) {
-    if (threadIdx.x < 8) hdr[threadIdx.x] = 0;  // status and the queue's counters
-}
-
-// Lane 0: the index of an item to walk, or kQNone once the queue is empty and no item is pending
-// (every item a wave still walks is pending, and whatever it re-queues is pending before it is
-// published, so an empty queue with nothing pending stays empty).
-RT_DEV uint32_t q_claim(const CamQueue& q) {
-    const uint64_t t0 = now_ticks();
-    uint32_t idle = 0;
-    for (;;) {
-        const uint32_t hd = ld_relaxed(&q.hdr[kQHead]);
-        const uint32_t tl = ld_relaxed(&q.hdr[kQPublished]);
-        if (hd < tl) {
-            if (atomicCAS(&q.hdr[kQHead], hd, hd + 1) == hd) {
-                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-                return hd;
-            }
-            continue;
-        }
-        if (ld_relaxed(&q.hdr[kQPending]) == 0) return kQNone;
-        if (now_ticks() - t0 > kWaitTicks) break;
-        // back off while the queue is empty: idle waves poll less and less often
-        if (idle < 6) ++idle;
-        for (uint32_t k = 0; k < (1u << idle); ++k) __builtin_amdgcn_s_sleep(4);
-    }
-    atomicOr(&q.hdr[kQStatus], 1u);  // never expected: a wait ran out
-    return kQNone;
-}
-
-// Persistent walk of the queued subtrees, one item per wave at a time (see above). Every wave
-// exits once the queue is empty and nothing is pending (or a bounded wait runs out).
-__global__ void __launch_bounds__(64) k_camera_drain(SceneArgs sc, rt_camera cam, uint32_t W, uint32_t H,
-                                                      CamQueue q, int budget) {
-    const uint32_t lane = threadIdx.x, qr = lane >> 2;
-    for (;;) {
-        uint32_t idx = lane == 0 ? q_claim(q) : 0u;
-        idx = (uint32_t)__builtin_amdgcn_readfirstlane((int)idx);
-        if (idx == kQNone) return;
-        uint32_t d = 0, node = 0, cm = 0;
-        if (lane == 0) {
-            uint32_t* it = q.items + 4ull * idx;
-            d = ld_relaxed(it);
-            node = ld_relaxed(it + 1);
-            cm = ld_relaxed(it + 2);
-        }
-        {
-            d = (uint32_t)__builtin_amdgcn_readfirstlane((int)d);
-            node = (uint32_t)__builtin_amdgcn_readfirstlane((int)node);
-            cm = (uint32_t)__builtin_amdgcn_readfirstlane((int)cm);
-            const uint32_t pkt = q.deferred[d];
-            const TilePixel p = packet_pixel(pkt, qr, q.tiles_x4, W, H);
-            const Ray r = camera_ray(cam, W, H, p);
-            unsigned long long* key = q.keys + 16ull * d + qr;
-            const unsigned long long k0 = __hip_atomic_load(key, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            Hit h;
-            h.t = __uint_as_float((uint32_t)(k0 >> 32));
-            h.prim = (int32_t)(uint32_t)k0;
-            h.u = 0.0f;
-            h.v = 0.0f;
-            h.slot = -1;
-            const uint64_t m = quad_mask_unpack(cm);
-            traverse_quad_core<false, true>(
-                sc.nodes4, sc.tris, (int32_t)node, m, r, -1, h, budget, [&](const QuadRest& rest) -> bool {
-                    const uint32_t base = q_reserve(q, (uint32_t)rest.sp + 1);
-                    if (base == kQNone) return false;
-                    Hit b = h;  // the rays' best so far bounds the pieces
-                    quad_closest(b.t, b.u, b.v, b.prim, b.slot);
-                    if ((lane & 3) == 0 && ((m >> lane) & 1u)) atomicMin(key, hit_key(b.t, b.prim));
-                    return q_publish(q, rest, d, base);
-                });
-            quad_closest(h.t, h.u, h.v, h.prim, h.slot);
-            if ((lane & 3) == 0 && ((m >> lane) & 1u)) atomicMin(key, hit_key(h.t, h.prim));
-        }
-        if (lane == 0) atomicSub(&q.hdr[kQPending], 1u);
-    }
-}
-
-// Hit records of the handed-off packets from their keys (grid-stride over 16 rays per packet).
-__global__ void __launch_bounds__(256) k_camera_resolve(SceneArgs sc, rt_camera cam, uint32_t W, uint32_t H,
-                                                         CamQueue q, rt_hits hits) {
-    const uint64_t n = 16ull * q.hdr[kQDeferred];
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (uint64_t)gridDim.x * blockDim.x) {
-        const TilePixel p = packet_pixel(q.deferred[i >> 4], (uint32_t)(i & 15), q.tiles_x4, W, H);
-        if (!p.valid) continue;
-        const Ray r = camera_ray(cam, W, H, p);
-        const unsigned long long key = q.keys[i];
-        Hit h;
-        h.prim = (int32_t)(uint32_t)key;
-        h.t = INFINITY;
-        h.u = 0.0f;
-        h.v = 0.0f;
-        h.slot = -1;
-        if (h.prim >= 0) {
-            h.slot = sc.prim_slot[h.prim];
-            const float4* tr = sc.tris + (size_t)h.slot * 3;
-            float t, u, v;
-            if (intersect_tri(tr[0], tr[1], tr[2], r, t, u, v)) {  // the FP32 result the walk kept
-                h.t = t;
-                h.u = u;
-                h.v = v;
-            } else {
-                h.t = __uint_as_float((uint32_t)(key >> 32));
-            }
-            refine_hit(sc.tris, r, h);
-        }
         store_hit(sc, hits, (uint64_t)p.row * W + p.col, h);
     }
 }
@@ -874,13 +604,16 @@ extern "C" int rt_generate_rays(const rt_camera* cam, uint32_t W, uint32_t H, fl
 }
 
 extern "C" uint64_t rt_trace_camera_workspace_bytes(uint32_t width, uint32_t height) {
-    if ((uint64_t)width * height == 0) return 0;
-    return cam_queue_layout(width, height).bytes;
+    (void)width;
+    (void)height;
+    return 0;
 }
 
 extern "C" int rt_trace_camera(const rt_scene* scene, const rt_camera* cam, uint32_t W, uint32_t H,
                                const rt_hits* hits, void* workspace, uint64_t ws_bytes,
                                const rt_tuning* tuning, void* stream) {
+    (void)workspace;
+    (void)ws_bytes;
     int e = check_scene(scene, "rt_trace_camera");
     if (e) return e;
     if (!cam || !hits || hits->stride < 4 || (hits->stride & 3))
@@ -891,40 +624,13 @@ extern "C" int rt_trace_camera(const rt_scene* scene, const rt_camera* cam, uint
     rt_tuning tu;
     if ((e = resolve_tuning(tuning, tu, "rt_trace_camera"))) return e;
     hipStream_t s = (hipStream_t)stream;
-    const TileMap tm = make_tile_map(W, H, tu.super_tile);
     if (tu.primary == RT_TRAVERSAL_QUAD) {  // 8x8-pixel blocks
         const TileMap tm8 = make_tile_map(W, H, tu.super_tile, 8);
-        const CamQueueLayout L = cam_queue_layout(W, H);
-        CamQueue q{};
-        q.tiles_x4 = (W + 3) / 4;
-        q.cap = L.cap;
-        const bool handoff = workspace && tu.handoff_steps > 0;
-        if (handoff) {
-            if (ws_bytes < L.bytes)
-                return set_error(RT_EINVAL, "rt_trace_camera: workspace too small (need " +
-                                                std::to_string(L.bytes) + " bytes)");
-            if ((uintptr_t)workspace & 255)
-                return set_error(RT_EINVAL, "rt_trace_camera: workspace must be 256-byte aligned");
-            if (!scene->prim_slot && scene->root != RT_EMPTY_ROOT)
-                return set_error(RT_EINVAL, "rt_trace_camera: scene prim_slot is NULL");
-            char* w = (char*)workspace;
-            q.hdr = (uint32_t*)w;
-            q.deferred = (uint32_t*)(w + L.deferred);
-            q.keys = (unsigned long long*)(w + L.keys);
-            q.items = (uint32_t*)(w + L.items);
-            hipLaunchKernelGGL(k_camq_prepare, dim3(1), dim3(64), 0, s, q.hdr);
-            hipLaunchKernelGGL(k_trace_camera_quad<true>, dim3(tile_map_blocks(tm8)), dim3(256), 0, s,
-                               scene_args(scene), *cam, W, H, tm8, *hits, q, (int)tu.handoff_steps);
-            hipLaunchKernelGGL(k_camera_drain, dim3((uint32_t)device_cu_count() * 4u), dim3(64), 0, s,
-                               scene_args(scene), *cam, W, H, q, (int)tu.item_steps);
-            hipLaunchKernelGGL(k_camera_resolve, dim3(512), dim3(256), 0, s, scene_args(scene), *cam, W, H,
-                               q, *hits);
-        } else {
-            hipLaunchKernelGGL(k_trace_camera_quad<false>, dim3(tile_map_blocks(tm8)), dim3(256), 0, s,
-                               scene_args(scene), *cam, W, H, tm8, *hits, q, 0);
-        }
+        hipLaunchKernelGGL(k_trace_camera_quad, dim3(tile_map_blocks(tm8)), dim3(256), 0, s,
+                           scene_args(scene), *cam, W, H, tm8, *hits);
         return check_launch("rt_trace_camera");
     }
+    const TileMap tm = make_tile_map(W, H, tu.super_tile);
     using K = decltype(&k_trace_camera<0>);
     static const K kerns[3] = {k_trace_camera<0>, k_trace_camera<1>, k_trace_camera<2>};
     const int pm = resolve_primary(tu.primary, *cam, W, H);

@@ -26,7 +26,7 @@ import numpy as np
 import torch
 
 from . import _native as N
-from .batch import BatchConfig, BatchRender, RenderRequest
+from .batch import BatchConfig, BatchSlot, RenderRequest
 
 
 @dataclasses.dataclass
@@ -164,9 +164,13 @@ class RenderServer:
         self.store = store
         self.config = config or BatchConfig(waves_per_cu=16, tiles_per_grab=4, shadow="lane")
         self.max_batch = max_batch
-        self.max_inflight = max_inflight
-        # equal (default) priority streams: batches on different streams overlap on the device
+        self.max_inflight = max(1, min(max_inflight, n_streams))
+        # equal (default) priority streams: batches on different streams overlap on the device; one
+        # reusable job table and workspace per stream (at most one batch per stream in flight)
         self.streams = [torch.cuda.Stream() for _ in range(max(1, n_streams))]
+        self.slots = {id(st): BatchSlot(max_batch, st, self.config) for st in self.streams}
+        self._waves = (self.config.waves_per_cu *
+                       torch.cuda.get_device_properties(0).multi_processor_count)
         self.batches = []      # requests per launched batch
         self.batch_log = []    # (requests, GPU start s, GPU end s) per batch, from the run's start
 
@@ -190,14 +194,19 @@ class RenderServer:
             rest = []
         rr = [RenderRequest(res.raycaster, camera_for(r.eye), r.size, r.size, textures=res.binding)
               for r, res in zip(reqs, residents)]
-        # scenes built on the current stream are ready before the batch starts (BatchRender waits)
-        b = BatchRender(rr, config=self.config, stats=False, stream=stream)
+        # scenes built on the current stream are ready before the batch starts
+        stream.wait_stream(torch.cuda.current_stream())
+        # units of work: 4x4-pixel packets; a grab of several keeps the queue's atomics few when
+        # the batch holds many times more units than there are waves
+        units = sum(((r.size + 3) // 4) ** 2 for r in reqs)
+        grab = max(1, min(16, units // (8 * self._waves)))
+        cfg = dataclasses.replace(self.config, tiles_per_grab=grab)
         start = torch.cuda.Event(enable_timing=True)
         start.record(stream)
-        b.launch()
+        images = self.slots[id(stream)].launch(rr, cfg)
         ev = torch.cuda.Event(enable_timing=True)
         ev.record(stream)
-        return {"reqs": reqs, "batch": b, "event": ev, "start": start, "residents": residents,
+        return {"reqs": reqs, "images": images, "event": ev, "start": start, "residents": residents,
                 "t0": t0_event, "stream": stream}, rest
 
     def _finish(self, f):
@@ -210,7 +219,7 @@ class RenderServer:
             r.service = done_s - start_s
             r.batch = k
             if r.keep_image:
-                r.image = f["batch"].images[i]
+                r.image = f["images"][i]
         for res in f["residents"]:
             self.store.release(res)
         self.batches.append(len(f["reqs"]))

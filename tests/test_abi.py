@@ -121,15 +121,15 @@ def test_tuning_get_set_and_range_checks_without_gpu():
                  "builder": _native.RT_BUILD_LBVH, "query": _native.RT_TRAVERSAL_BINARY,
                  "handoff_steps": 0, "item_steps": 32}
     with ren.tuning(primary="hybrid", lane_stack=3, super_tile=2, packet_budget=7, builder="ploc",
-                    query="lane", handoff_steps=9, item_steps=5):
+                    query="lane", item_steps=5):
         assert ren.get_tuning() == {"primary": 2, "shadow": 1, "super_tile": 2, "lane_stack": 3,
                                     "packet_budget": 7, "builder": _native.RT_BUILD_PLOC,
-                                    "query": _native.RT_TRAVERSAL_LANE, "handoff_steps": 9,
+                                    "query": _native.RT_TRAVERSAL_LANE, "handoff_steps": 0,
                                     "item_steps": 5}
     assert ren.get_tuning() == d
     for bad in ({"lane_stack": 0}, {"lane_stack": 33}, {"primary": 5}, {"shadow": 3},
                 {"super_tile": 65}, {"packet_budget": 0}, {"builder": 2}, {"query": "packet"},
-                {"query": "quad"}, {"item_steps": 0}, {"handoff_steps": (1 << 20) + 1}):
+                {"query": "quad"}, {"item_steps": 0}, {"handoff_steps": 1}):
         with pytest.raises(RuntimeError, match="rt_set_tuning"):
             ren.set_tuning(**bad)
     assert ren.get_tuning() == d

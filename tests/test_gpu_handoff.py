@@ -12,7 +12,12 @@ import torch
 from conftest import c2_scene
 from helpers import camera, gpu_from_meshes, twin_from_meshes
 
-pytestmark = pytest.mark.gpu
+# The hand-off is switched off (rt_tuning.handoff_steps must be 0): the first frame with it at
+# 1920x1080 (handoff 48, item 32) did not finish, both before and after its queue's waits were made
+# relaxed and bounded by a deadline, and no cause has been shown. These tests stay for when it is.
+pytestmark = [pytest.mark.gpu,
+              pytest.mark.skip(reason="camera hand-off switched off: its 1920x1080 run did not finish, "
+                                      "cause not shown")]
 
 
 def _u32(a):
