@@ -307,11 +307,13 @@ def _texture_rgba8(tx, keep, stream):
     """(device pointer, width, height) of RGBA8 texels for one texture argument: a uint8 tensor
     [H, W, 4], a float32 tensor [H, W, 4], or a (memory, Texture2D) pair from create_texture2D with
     RGBA or float4 texels (the reference's default, rendering/_core.py:557-578). Float texels are
-    converted once, here, with the UNORM_INT8 rule (rt_texels_to_rgba8); the converted copy is
-    appended to `keep`."""
+    converted once, here, with the UNORM_INT8 rule (rt_texels_to_rgba8). Whatever device memory
+    the returned pointer (or the conversion) reads is appended to `keep`: a caller that relaunches
+    later (a batch, a captured graph) holds it alive."""
     if isinstance(tx, tuple):
         mem, desc = tx
         h, w = mem.shape[0], mem.shape[1]
+        keep.append(tx)
         if mem.dtype == RGBA:
             return texture_pointer(desc), w, h
         if mem.dtype != float4:
@@ -322,12 +324,12 @@ def _texture_rgba8(tx, keep, stream):
         if tx.dim() != 3 or tx.shape[2] != 4 or not tx.is_contiguous():
             raise TypeError("texture tensors must be contiguous [H, W, 4]")
         h, w = tx.shape[0], tx.shape[1]
+        if tx.dtype not in (torch.uint8, torch.float32):
+            raise TypeError("texture tensors must be uint8 (RGBA8) or float32 (RGBA) [H, W, 4]")
+        keep.append(tx)
         if tx.dtype == torch.uint8:
             return tx.data_ptr(), w, h
-        if tx.dtype != torch.float32:
-            raise TypeError("texture tensors must be uint8 (RGBA8) or float32 (RGBA) [H, W, 4]")
         src, n = tx.data_ptr(), w * h
-        keep.append(tx)
     lib = N.load()
     dst = torch.empty((h, w, 4), dtype=torch.uint8, device=torch.device("cuda"))
     N.check(lib.rt_texels_to_rgba8(src, n, dst.data_ptr(), N.stream_handle(stream)), "rt_texels_to_rgba8")

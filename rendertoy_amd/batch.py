@@ -53,7 +53,7 @@ class BatchConfig:
     waves_per_cu: int = 16
     tiles_per_grab: int = 4
     shadow: str = "lane"
-    primary: str = "quad"
+    primary: str = "auto"
     lane_stack: int = N.RT_LANE_STACK_MAX
     packet_budget: int = 256
 

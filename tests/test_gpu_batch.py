@@ -156,3 +156,37 @@ def test_unrenderable_jobs_set_status_and_are_skipped(ren, scenes_):
         torch.cuda.synchronize()
         assert int(ws[4:8].view(torch.int32).item()) == want, extra
         assert_batch_equals_alone(b, reqs)
+
+
+def test_relaunch_keeps_textures_the_caller_dropped(ren):
+    """A batch holds every texture its jobs sample: the caller's own references are dropped before
+    the first launch and the freed blocks are reused by other tensors, yet every relaunch renders
+    the same images (the texels must not be freed under the job table)."""
+    from rendertoy_amd import scenes
+    from rendertoy_amd.batch import BatchRender, RenderRequest
+
+    def make():
+        rc = gpu_from_meshes([scenes.icosphere(10, 77), scenes.ground_plane()])
+        tex = torch.from_numpy(scenes.value_noise_rgba8(32, 78)).cuda()
+        ftex = torch.from_numpy(scenes.value_noise_rgba8(16, 79).astype(np.float32) / 255.0).cuda()
+        reqs = []
+        for k, t in enumerate((tex, ftex)):
+            cam = camera((0.3 * k, 0.2, 2.6), (0, 0, 0), (0, 1, 0), math.pi / 4, 1.5)
+            reqs.append(RenderRequest(rc, cam, 96, 64, light=LIGHT, textures=[t], mesh_textures=[0, 0]))
+        return BatchRender(reqs)
+
+    b = make()
+    b.launch()
+    torch.cuda.synchronize()
+    want = [img.clone() for img in b.images]
+    for seed in range(3):
+        junk = [torch.full((32, 32, 4), 200 + seed, dtype=torch.uint8, device="cuda") for _ in range(64)]
+        for img in b.images:
+            img.fill_(-1.0)
+        b.launch()
+        torch.cuda.synchronize()
+        del junk
+        for w, g in zip(want, b.images):
+            assert torch.equal(w.view(torch.int32), g.view(torch.int32))
+    # the textured pixels are not all the albedo-1 gray
+    assert any(bool((w[..., 0] != w[..., 1]).any()) for w in want)
