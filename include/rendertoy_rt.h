@@ -316,6 +316,17 @@ int rt_render_batch(const rt_render_job* jobs_dev, uint32_t n_jobs, const rt_bat
 /* Device address of the uint32 status word inside a batch workspace (0 = every job rendered). */
 const uint32_t* rt_render_batch_status(const void* workspace_dev);
 
+/* rt_render_batch without a workspace argument and without any allocation: no counterpart in the
+ * reference; for callers that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc). The batch workspace is a file-scope device array
+ * of the library sized for RT_BATCH_MAX_JOBS jobs (16,644 bytes, one copy per device, held until
+ * the process ends); more jobs return RT_ECAPACITY. Output is bit-identical to rt_render_batch.
+ * Every call on one device shares that array: calls whose launches overlap on one device (e.g. on
+ * two streams) race; issue them on one stream or wait between them. The status word is not
+ * readable (a skipped job leaves its outputs unwritten). */
+int rt_render_batch_static(const rt_render_job* jobs_dev, uint32_t n_jobs, const rt_batch_config* cfg,
+                           void* stream);
+
 /* ---- incoherent rays: diffuse paths (no reference counterpart) ----
  * spp jittered camera rays per pixel, each followed by up to `bounces` cosine-weighted diffuse
  * bounces off the geometric normal (facing the incoming ray), lit by a uniform sky: a sample's

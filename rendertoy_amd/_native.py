@@ -228,6 +228,8 @@ SIGNATURES = [
      [ctypes.c_void_p, ctypes.c_uint32, ctypes.POINTER(rt_batch_config), ctypes.c_void_p,
       ctypes.c_uint64, ctypes.c_void_p]),
     ("rt_render_batch_status", ctypes.c_void_p, [ctypes.c_void_p]),
+    ("rt_render_batch_static", ctypes.c_int,
+     [ctypes.c_void_p, ctypes.c_uint32, ctypes.POINTER(rt_batch_config), ctypes.c_void_p]),
     ("rt_paths_workspace_bytes", ctypes.c_uint64, [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32]),
     ("rt_trace_paths", ctypes.c_int,
      [ctypes.POINTER(rt_scene), ctypes.POINTER(rt_camera), ctypes.POINTER(rt_paths_params),

@@ -12,6 +12,7 @@
 //   ray toward the light per lit hit (wave-coherent any-hit over the tile's hit points, skipping
 //   each hit's own triangle), RGBA8 texture fetch, RGB float32 store.
 #include <algorithm>
+#include <mutex>
 
 #include "rt_trace_shared.h"
 
@@ -743,4 +744,43 @@ extern "C" int rt_render_batch(const rt_render_job* jobs_dev, uint32_t n_jobs, c
         return set_error(RT_EINVAL, "rt_render_batch: workspace too small");
     if ((uintptr_t)workspace & 15) return set_error(RT_EINVAL, "rt_render_batch: workspace must be 16-byte aligned");
     return launch_batch(jobs_dev, n_jobs, cfg, (uint32_t*)workspace, (hipStream_t)stream, "rt_render_batch");
+}
+
+// ---- allocation-free batched render ----
+// The batch workspace of RT_BATCH_MAX_JOBS jobs as a file-scope device array: every device the
+// library runs on holds its own copy until the process ends. In host code `scratch` only names the
+// symbol; its device address comes from hipGetSymbolAddress, looked up once per device (with that
+// device current) and kept in g_scratch.
+constexpr uint32_t kStaticBatchWords = kBatchPrefixWord + RT_BATCH_MAX_JOBS + 1;  // 16,644 bytes
+static __device__ __align__(256) uint32_t scratch[kStaticBatchWords];
+
+namespace rt {
+namespace {
+std::mutex g_scratch_mutex;
+uint32_t* g_scratch[64] = {nullptr};
+
+int scratch_address(uint32_t** out, const char* what) {
+    int dev = 0;
+    if (int e = check_hip(hipGetDevice(&dev), what)) return e;
+    if (dev < 0 || dev >= 64) return set_error(RT_ECAPACITY, std::string(what) + ": device id above 63");
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch[dev]) {
+        void* p = nullptr;
+        if (int e = check_hip(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), what)) return e;
+        g_scratch[dev] = (uint32_t*)p;
+    }
+    *out = g_scratch[dev];
+    return RT_OK;
+}
+}  // namespace
+}  // namespace rt
+
+extern "C" int rt_render_batch_static(const rt_render_job* jobs_dev, uint32_t n_jobs,
+                                      const rt_batch_config* cfg, void* stream) {
+    int e = check_batch_args(jobs_dev, n_jobs, cfg, "rt_render_batch_static");
+    if (e) return e;
+    if (n_jobs == 0) return RT_OK;
+    uint32_t* ws = nullptr;
+    if ((e = scratch_address(&ws, "rt_render_batch_static"))) return e;
+    return launch_batch(jobs_dev, n_jobs, cfg, ws, (hipStream_t)stream, "rt_render_batch_static");
 }
