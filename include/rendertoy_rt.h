@@ -178,10 +178,11 @@ typedef struct rt_tuning {
                                  needing more is traced again by its wave with the packet traversal */
     uint32_t packet_budget;   /* RT_TRAVERSAL_HYBRID: packet steps before the per-lane walk (256) */
     uint32_t builder;         /* RT_BUILD_* hierarchy of rt_scene_build (default LBVH) */
-    int32_t query;            /* RT_TRAVERSAL_BINARY (default) or RT_TRAVERSAL_LANE (per-lane walk of
-                                 the 4-wide nodes) for rt_trace_closest / rt_trace_any; both keep
-                                 their stacks in LDS (lane_stack entries per lane) and trace a ray
-                                 that needs more again with the packet traversal */
+    int32_t query;            /* RT_TRAVERSAL_LANE (per-lane walk of the 4-wide nodes, default) or
+                                 RT_TRAVERSAL_BINARY (of the binary nodes) for rt_trace_closest /
+                                 rt_trace_any; both keep their stacks in LDS (lane_stack entries per
+                                 lane) and trace a ray that needs more again with the packet
+                                 traversal */
     uint32_t handoff_steps;   /* must be 0: reserved for a hand-off of the slowest camera packets' walks
                                  to other waves, which is switched off (its first runs did not finish) */
     uint32_t item_steps;      /* reserved with handoff_steps: 1..2^20, default 32 */

@@ -363,7 +363,7 @@ class Raycaster:
 
     `stream`: the torch stream its kernels run on (default: the caller's current stream at each
     call). `tuning`: launch choices for this scene (a dict of rendertoy_amd.tuning fields over the
-    process default at construction, e.g. {"builder": "ploc", "query": "lane"}); every method also
+    process default at construction, e.g. {"builder": "ploc", "query": "binary"}); every method also
     takes a `tuning=` dict for one call. Without either, each call uses the process default."""
 
     def __init__(self, models: typing.List[Mesh], stream=None, tuning=None):

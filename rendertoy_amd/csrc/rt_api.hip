@@ -12,7 +12,7 @@ std::mutex g_cu_mutex;
 int g_cu_count[64] = {0};
 std::mutex g_tuning_mutex;
 rt_tuning g_tuning = {RT_TRAVERSAL_QUAD, RT_TRAVERSAL_LANE, 0u, (uint32_t)RT_LANE_STACK_MAX, 256u,
-                      RT_BUILD_LBVH, RT_TRAVERSAL_BINARY, 0u, 32u};
+                      RT_BUILD_LBVH, RT_TRAVERSAL_LANE, 0u, 32u};
 }  // namespace
 
 rt_tuning current_tuning() {

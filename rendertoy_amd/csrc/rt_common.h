@@ -145,6 +145,72 @@ RT_DEV bool test_leaf(const float4* __restrict__ tris, int32_t ref, const Ray& r
     return changed;
 }
 
+// Resumable state of a per-lane walk (traverse_bin / traverse_lane4): the ray's slab terms, the
+// node it visits next and its stack depth (the entries live in the caller's LDS stack).
+struct LaneWalk {
+    float idx, idy, idz, oxx, oyy, ozz;
+    int32_t cur;
+    int sp;
+};
+
+RT_DEV void walk_start(const Ray& r, int32_t root, LaneWalk& w) {
+    w.idx = safe_inv(r.dx);
+    w.idy = safe_inv(r.dy);
+    w.idz = safe_inv(r.dz);
+    w.oxx = r.ox * w.idx;
+    w.oyy = r.oy * w.idy;
+    w.ozz = r.oz * w.idz;
+    w.cur = root;
+    w.sp = 0;
+}
+
+// What a bounded run of a per-lane walk ended with.
+enum { kWalkOn = 0, kWalkDone = 1, kWalkOverflow = 2 };
+
+// At most `steps` iterations (one node or one leaf each) of the binary walk from state w:
+// kWalkDone when the walk has ended (h final), kWalkOverflow when a push would exceed `limit`
+// entries (h invalid), else kWalkOn (w and h hold the walk so far).
+template <bool ANY>
+RT_DEV int bin_steps(const float4* __restrict__ nodes, const float4* __restrict__ tris, const Ray& r,
+                     int32_t skip_slot, Hit& h, LaneWalk& w, int32_t* stk, int stride, int limit,
+                     int steps) {
+    for (int s = 0; s < steps; ++s) {
+        if (w.cur >= 0) {
+            const float4* nd = nodes + (size_t)w.cur * 4;
+            const float4 n0 = nd[0], n1 = nd[1], n2 = nd[2], n3 = nd[3];
+            const float a0 = fmaf(n0.x, w.idx, -w.oxx), a1 = fmaf(n0.y, w.idx, -w.oxx);
+            const float a2 = fmaf(n0.z, w.idy, -w.oyy), a3 = fmaf(n0.w, w.idy, -w.oyy);
+            const float a4 = fmaf(n2.x, w.idz, -w.ozz), a5 = fmaf(n2.y, w.idz, -w.ozz);
+            const float b0 = fmaf(n1.x, w.idx, -w.oxx), b1 = fmaf(n1.y, w.idx, -w.oxx);
+            const float b2 = fmaf(n1.z, w.idy, -w.oyy), b3 = fmaf(n1.w, w.idy, -w.oyy);
+            const float b4 = fmaf(n2.z, w.idz, -w.ozz), b5 = fmaf(n2.w, w.idz, -w.ozz);
+            const float tn0 = fmaxf(fmaxf(fminf(a0, a1), fminf(a2, a3)), fmaxf(fminf(a4, a5), r.tmin));
+            const float tf0 = fminf(fminf(fmaxf(a0, a1), fmaxf(a2, a3)), fmaxf(a4, a5)) * kTfarScale;
+            const float tn1 = fmaxf(fmaxf(fminf(b0, b1), fminf(b2, b3)), fmaxf(fminf(b4, b5), r.tmin));
+            const float tf1 = fminf(fminf(fmaxf(b0, b1), fmaxf(b2, b3)), fmaxf(b4, b5)) * kTfarScale;
+            const bool hit0 = tn0 <= fminf(tf0, h.t);
+            const bool hit1 = tn1 <= fminf(tf1, h.t);
+            const int32_t r0 = __float_as_int(n3.x), r1 = __float_as_int(n3.y);
+            if (hit0 && hit1) {
+                if (w.sp >= limit) return kWalkOverflow;
+                const bool swap = tn1 < tn0;
+                stk[(w.sp++) * stride] = swap ? r0 : r1;
+                w.cur = swap ? r1 : r0;
+                continue;
+            }
+            if (hit0 | hit1) {
+                w.cur = hit0 ? r0 : r1;
+                continue;
+            }
+        } else {
+            if (test_leaf(tris, w.cur, r, true, skip_slot, h) && ANY) return kWalkDone;
+        }
+        if (w.sp == 0) return kWalkDone;
+        w.cur = stk[(--w.sp) * stride];
+    }
+    return kWalkOn;
+}
+
 // Closest-hit (ANY=false) or any-hit (ANY=true) per-lane traversal of the two-child node layout,
 // nearer child first, pending children on a per-lane stack in LDS: entry k of this lane lives at
 // stk[k * stride] (stride = threads per block, so the lanes of a wave touch consecutive words at
@@ -162,44 +228,9 @@ RT_DEV bool traverse_bin(const float4* __restrict__ nodes, const float4* __restr
     h.prim = -1;
     h.slot = -1;
     if (root == RT_EMPTY_ROOT) return true;
-    const float idx = safe_inv(r.dx), idy = safe_inv(r.dy), idz = safe_inv(r.dz);
-    const float oxx = r.ox * idx, oyy = r.oy * idy, ozz = r.oz * idz;
-    int sp = 0;
-    int32_t cur = root;
-    for (;;) {
-        if (cur >= 0) {
-            const float4* nd = nodes + (size_t)cur * 4;
-            const float4 n0 = nd[0], n1 = nd[1], n2 = nd[2], n3 = nd[3];
-            const float a0 = fmaf(n0.x, idx, -oxx), a1 = fmaf(n0.y, idx, -oxx);
-            const float a2 = fmaf(n0.z, idy, -oyy), a3 = fmaf(n0.w, idy, -oyy);
-            const float a4 = fmaf(n2.x, idz, -ozz), a5 = fmaf(n2.y, idz, -ozz);
-            const float b0 = fmaf(n1.x, idx, -oxx), b1 = fmaf(n1.y, idx, -oxx);
-            const float b2 = fmaf(n1.z, idy, -oyy), b3 = fmaf(n1.w, idy, -oyy);
-            const float b4 = fmaf(n2.z, idz, -ozz), b5 = fmaf(n2.w, idz, -ozz);
-            const float tn0 = fmaxf(fmaxf(fminf(a0, a1), fminf(a2, a3)), fmaxf(fminf(a4, a5), r.tmin));
-            const float tf0 = fminf(fminf(fmaxf(a0, a1), fmaxf(a2, a3)), fmaxf(a4, a5)) * kTfarScale;
-            const float tn1 = fmaxf(fmaxf(fminf(b0, b1), fminf(b2, b3)), fmaxf(fminf(b4, b5), r.tmin));
-            const float tf1 = fminf(fminf(fmaxf(b0, b1), fmaxf(b2, b3)), fmaxf(b4, b5)) * kTfarScale;
-            const bool hit0 = tn0 <= fminf(tf0, h.t);
-            const bool hit1 = tn1 <= fminf(tf1, h.t);
-            const int32_t r0 = __float_as_int(n3.x), r1 = __float_as_int(n3.y);
-            if (hit0 && hit1) {
-                if (sp >= limit) return false;
-                const bool swap = tn1 < tn0;
-                stk[(sp++) * stride] = swap ? r0 : r1;
-                cur = swap ? r1 : r0;
-                continue;
-            }
-            if (hit0 | hit1) {
-                cur = hit0 ? r0 : r1;
-                continue;
-            }
-        } else {
-            if (test_leaf(tris, cur, r, true, skip_slot, h) && ANY) return true;
-        }
-        if (sp == 0) return true;
-        cur = stk[(--sp) * stride];
-    }
+    LaneWalk w;
+    walk_start(r, root, w);
+    return bin_steps<ANY>(nodes, tris, r, skip_slot, h, w, stk, stride, limit, 0x7FFFFFFF) != kWalkOverflow;
 }
 
 // Wave-coherent ("packet") traversal of the 4-wide nodes for coherent rays (camera tiles, shadow
@@ -691,34 +722,16 @@ RT_DEV void traverse_quad(const float4* __restrict__ nodes4, const float4* __res
     quad_closest(h.t, h.u, h.v, h.prim, h.slot);
 }
 
-// Per-lane traversal of the 4-wide nodes: every lane walks its own ray (no lane waits on another's
-// subtrees, so the longest lane bounds the wave), nearer children first by the lane's own entry
-// distances, pending children on a per-lane stack in LDS. `stk` points at this thread's first
-// entry; entry k lives at stk[k * stride] (stride = threads per block, so the lanes of a wave touch
-// consecutive words at equal depth). Returns false if the ray needed more than `limit` (at most
-// kLaneStack) pending entries (3 per level of the 4-wide tree: up to 93): the hit is then invalid
-// and the caller traces the ray again with a traversal whose stack always suffices.
 constexpr int kLaneStack = RT_LANE_STACK_MAX;
 
+// At most `steps` iterations of the 4-wide walk from state w (returns as bin_steps).
 template <bool ANY>
-RT_DEV bool traverse_lane4(const float4* __restrict__ nodes4, const float4* __restrict__ tris,
-                           int32_t root, const Ray& r, int32_t skip_slot, Hit& h, int32_t* stk,
-                           int stride, int limit, bool fresh = true) {
-    if (fresh) {  // else h holds a hit of this ray found earlier: it bounds the walk
-        h.t = r.tmax;
-        h.u = 0.0f;
-        h.v = 0.0f;
-        h.prim = -1;
-        h.slot = -1;
-    }
-    if (root == RT_EMPTY_ROOT) return true;
-    const float idx = safe_inv(r.dx), idy = safe_inv(r.dy), idz = safe_inv(r.dz);
-    const float oxx = r.ox * idx, oyy = r.oy * idy, ozz = r.oz * idz;
-    int sp = 0;
-    int32_t cur = root;
-    for (;;) {
-        if (cur >= 0) {
-            const float4* nd = nodes4 + (size_t)cur * 8;
+RT_DEV int lane4_steps(const float4* __restrict__ nodes4, const float4* __restrict__ tris, const Ray& r,
+                       int32_t skip_slot, Hit& h, LaneWalk& w, int32_t* stk, int stride, int limit,
+                       int steps) {
+    for (int s = 0; s < steps; ++s) {
+        if (w.cur >= 0) {
+            const float4* nd = nodes4 + (size_t)w.cur * 8;
             const float4 lx = nd[0], hx = nd[1], ly = nd[2], hy = nd[3], lz = nd[4], hz = nd[5];
             const float4 rf = nd[6];
             const float clx[4] = {lx.x, lx.y, lx.z, lx.w}, chx[4] = {hx.x, hx.y, hx.z, hx.w};
@@ -729,9 +742,9 @@ RT_DEV bool traverse_lane4(const float4* __restrict__ nodes4, const float4* __re
             float key[4];
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
-                const float x0 = fmaf(clx[k], idx, -oxx), x1 = fmaf(chx[k], idx, -oxx);
-                const float y0 = fmaf(cly[k], idy, -oyy), y1 = fmaf(chy[k], idy, -oyy);
-                const float z0 = fmaf(clz[k], idz, -ozz), z1 = fmaf(chz[k], idz, -ozz);
+                const float x0 = fmaf(clx[k], w.idx, -w.oxx), x1 = fmaf(chx[k], w.idx, -w.oxx);
+                const float y0 = fmaf(cly[k], w.idy, -w.oyy), y1 = fmaf(chy[k], w.idy, -w.oyy);
+                const float z0 = fmaf(clz[k], w.idz, -w.ozz), z1 = fmaf(chz[k], w.idz, -w.ozz);
                 const float tn = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), r.tmin));
                 const float tf = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fmaxf(z0, z1)) * kTfarScale;
                 const bool hk = (ref[k] != RT_EMPTY_ROOT) & (tn <= fminf(tf, h.t));
@@ -755,20 +768,45 @@ RT_DEV bool traverse_lane4(const float4* __restrict__ nodes4, const float4* __re
 #undef RT_LSWAP
             const int nhit = (k0 < INFINITY) + (k1 < INFINITY) + (k2 < INFINITY) + (k3 < INFINITY);
             if (nhit > 0) {
-                if (sp + nhit - 1 > limit) return false;
+                if (w.sp + nhit - 1 > limit) return kWalkOverflow;
                 // push the farther hit children (farthest first), continue with the nearest
-                if (nhit >= 4) stk[(sp++) * stride] = r3;
-                if (nhit >= 3) stk[(sp++) * stride] = r2;
-                if (nhit >= 2) stk[(sp++) * stride] = r1;
-                cur = r0;
+                if (nhit >= 4) stk[(w.sp++) * stride] = r3;
+                if (nhit >= 3) stk[(w.sp++) * stride] = r2;
+                if (nhit >= 2) stk[(w.sp++) * stride] = r1;
+                w.cur = r0;
                 continue;
             }
         } else {
-            if (test_leaf(tris, cur, r, true, skip_slot, h) && ANY) return true;
+            if (test_leaf(tris, w.cur, r, true, skip_slot, h) && ANY) return kWalkDone;
         }
-        if (sp == 0) return true;
-        cur = stk[(--sp) * stride];
+        if (w.sp == 0) return kWalkDone;
+        w.cur = stk[(--w.sp) * stride];
     }
+    return kWalkOn;
+}
+
+// Per-lane traversal of the 4-wide nodes: every lane walks its own ray (no lane waits on another's
+// subtrees, so the longest lane bounds the wave), nearer children first by the lane's own entry
+// distances, pending children on a per-lane stack in LDS. `stk` points at this thread's first
+// entry; entry k lives at stk[k * stride] (stride = threads per block, so the lanes of a wave touch
+// consecutive words at equal depth). Returns false if the ray needed more than `limit` (at most
+// kLaneStack) pending entries (3 per level of the 4-wide tree: up to 93): the hit is then invalid
+// and the caller traces the ray again with a traversal whose stack always suffices.
+template <bool ANY>
+RT_DEV bool traverse_lane4(const float4* __restrict__ nodes4, const float4* __restrict__ tris,
+                           int32_t root, const Ray& r, int32_t skip_slot, Hit& h, int32_t* stk,
+                           int stride, int limit, bool fresh = true) {
+    if (fresh) {  // else h holds a hit of this ray found earlier: it bounds the walk
+        h.t = r.tmax;
+        h.u = 0.0f;
+        h.v = 0.0f;
+        h.prim = -1;
+        h.slot = -1;
+    }
+    if (root == RT_EMPTY_ROOT) return true;
+    LaneWalk w;
+    walk_start(r, root, w);
+    return lane4_steps<ANY>(nodes4, tris, r, skip_slot, h, w, stk, stride, limit, 0x7FFFFFFF) != kWalkOverflow;
 }
 
 // Primary ray direction of pixel (col,row) (rendering/_core.py:528-548 look_at/perspective,

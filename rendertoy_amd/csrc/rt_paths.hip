@@ -9,8 +9,8 @@
 //   k_paths_init     counters
 //   k_paths_primary  camera rays (8x8 tile per wave, AUTO coherence rule), hits append a bounce ray
 //                    to queue 0 (wave ballot + one atomic per wave = compaction)
-//   k_paths_bounce   per depth: every wave walks 64-ray chunks of the queue (count read on the
-//                    device), per-lane traversal, hits append to the other queue
+//   k_paths_bounce   per depth: the waves share the queue's 64-ray groups (count read on the
+//                    device), per-lane walks with refill, hits append to the other queue
 //   k_paths_resolve  per-pixel mean, ray counts out
 // Random numbers: Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11), key = (seed, frame), counter =
 // (pixel, sample, depth, 0); 32-bit outputs -> [0,1) as (x >> 8) * 2^-24.
@@ -186,49 +186,43 @@ __global__ void __launch_bounds__(256) k_paths_primary(SceneArgs sc, rt_camera c
     }
 }
 
-// Rays of depth k: waves take 64-ray chunks of queue (k-1)&1 until its count (written by the
-// previous kernel on the stream) is exhausted; every wave reaches the end of the loop.
+// Rays of depth k, queue (k-1)&1 (its count written by the previous kernel on the stream): per-lane
+// walks of the 4-wide nodes with refill (refill_walks), the grid's waves sharing the queue's 64-ray
+// groups; each finished ray writes its sample (a miss) or appends its bounce to the other queue.
 __global__ void __launch_bounds__(64) k_paths_bounce(SceneArgs sc, PathArgs a, uint32_t k,
                                                       int lane_stack) {
     __shared__ int32_t stk[kLaneStack * 64];
     __shared__ float4 pk[kPacketLds];
+    __shared__ uint32_t ovf[kRefillList];
     const uint32_t n = a.counts[k];
     const float4* qi = a.q[(k - 1) & 1];
     float4* qo = a.q[k & 1];
-    const uint32_t lane = threadIdx.x;
-    for (uint32_t base = blockIdx.x * 64u; base < n; base += gridDim.x * 64u) {
-        const uint32_t i = base + lane;
-        const bool valid = i < n;
-        Ray r;
-        float thr = 0.0f;
-        uint32_t id = 0;
-        int32_t skip = -1;
-        if (valid) {
+    refill_walks<false, RT_TRAVERSAL_LANE>(
+        sc, n, gridDim.x, blockIdx.x, stk + threadIdx.x, 64, lane_stack, ovf, pk,
+        [&](uint32_t i, Ray& r) -> int32_t {
             const float4 e0 = qi[(size_t)i * 3], e1 = qi[(size_t)i * 3 + 1], e2 = qi[(size_t)i * 3 + 2];
             r.ox = e0.x;
             r.oy = e0.y;
             r.oz = e0.z;
-            thr = e0.w;
             r.dx = e1.x;
             r.dy = e1.y;
             r.dz = e1.z;
-            id = (uint32_t)__float_as_int(e1.w);
-            skip = __float_as_int(e2.x);
-        } else {
-            r.ox = r.oy = r.oz = 0.0f;
-            r.dx = 0.0f;
-            r.dy = 0.0f;
-            r.dz = 1.0f;
-        }
-        r.tmin = a.tmin;
-        r.tmax = INFINITY;
-        Hit h;
-        trace_block<false, RT_TRAVERSAL_LANE>(sc, r, valid, skip, h, pk, stk + lane, 64, lane_stack, 0);
-        const bool hit = valid && h.prim >= 0;
-        if (valid && !hit) a.samples[id] = thr * a.sky;
-        if (hit) refine_hit(sc.tris, r, h);
-        if (k < a.bounces) emit_bounce(sc, a, r, h, hit, id, k + 1, thr, qo, &a.counts[k + 1]);
-    }
+            r.tmin = a.tmin;
+            r.tmax = INFINITY;
+            return __float_as_int(e2.x);
+        },
+        [&](bool fin, uint32_t i, const Ray& r, Hit& h) {
+            float thr = 0.0f;
+            uint32_t id = 0;
+            if (fin) {
+                thr = qi[(size_t)i * 3].w;
+                id = (uint32_t)__float_as_int(qi[(size_t)i * 3 + 1].w);
+            }
+            const bool hit = fin && h.prim >= 0;
+            if (fin && !hit) a.samples[id] = thr * a.sky;
+            if (hit) refine_hit(sc.tris, r, h);
+            if (k < a.bounces) emit_bounce(sc, a, r, h, hit, id, k + 1, thr, qo, &a.counts[k + 1]);
+        });
 }
 
 __global__ void __launch_bounds__(256) k_paths_resolve(PathArgs a, float* __restrict__ radiance,

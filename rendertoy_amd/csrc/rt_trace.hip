@@ -1,9 +1,10 @@
 // rt_trace.hip — closest-hit / any-hit queries and the render path.
 //
 // Queries (rendering/_raycaster.py:35-36, Raycaster.ray_cast — a stub in the reference):
-//   k_trace_closest / k_trace_any: one lane per ray with its own stack (rays may be incoherent);
-//   rays and hits are read/written through (pointer, byte stride) fields so SoA (stride 4) and
-//   the AoS Ray/Hit records both coalesce.
+//   k_query (closest or any hit): each lane walks its own ray with its own LDS stack (rays may be
+//   incoherent) and takes the next ray of its wave's chunk when its walk ends; rays and hits are
+//   read/written through (pointer, byte stride) fields, so SoA (stride 4) and the AoS Ray/Hit
+//   records are both accepted.
 // Camera rays:
 //   each wave64 owns an 8x8 pixel tile (a 256-thread block = 16x16 pixels); rays are generated in
 //   registers and traced with the wave-coherent traversal (scalar node fetches, one VGPR stack).
@@ -79,36 +80,35 @@ RT_DEV Ray query_ray(const rt_rays& rays, uint32_t i, bool valid) {
     return r;
 }
 
-// Closest hit of a ray buffer, one lane per ray (MODE: RT_TRAVERSAL_BINARY or LANE); the per-lane
-// stacks (lane_stack entries of 4 bytes per thread) are the block's dynamic LDS.
-template <int MODE>
-__global__ void __launch_bounds__(256) k_trace_closest(SceneArgs sc, rt_rays rays, uint32_t n,
-                                                        rt_hits hits, int lane_stack) {
-    extern __shared__ int32_t qstk[];
-    __shared__ float4 pk[4][kPacketLds];
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool valid = i < n;
-    const Ray r = query_ray(rays, i, valid);
-    Hit h;
-    trace_block<false, MODE>(sc, r, valid, -1, h, pk[threadIdx.x >> 6], qstk + threadIdx.x, 256,
-                             lane_stack, 0);
-    if (!valid) return;
-    if (h.prim >= 0) refine_hit(sc.tris, r, h);
-    store_hit(sc, hits, i, h);
+// Ray queries (closest hit / any hit of a ray buffer): per-lane walks with refill (refill_walks)
+// over a grid of as many waves as the device holds at once.
+template <bool ANY>
+RT_DEV void finish_query(const SceneArgs& sc, const rt_hits& hits, uint8_t* occluded, uint32_t i,
+                         const Ray& r, Hit& h) {
+    if (ANY) {
+        occluded[i] = h.prim >= 0 ? 1 : 0;
+    } else {
+        if (h.prim >= 0) refine_hit(sc.tris, r, h);
+        store_hit(sc, hits, i, h);
+    }
 }
 
-template <int MODE>
-__global__ void __launch_bounds__(256) k_trace_any(SceneArgs sc, rt_rays rays, uint32_t n,
-                                                    uint8_t* __restrict__ occluded, int lane_stack) {
-    extern __shared__ int32_t qstk[];
+template <bool ANY, int MODE>
+__global__ void __launch_bounds__(256) k_query(SceneArgs sc, rt_rays rays, uint32_t n, rt_hits hits,
+                                                uint8_t* occluded, int lane_stack, uint32_t n_waves) {
+    extern __shared__ int32_t qstk[];  // lane_stack entries per thread, entry k at [k * 256]
     __shared__ float4 pk[4][kPacketLds];
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool valid = i < n;
-    const Ray r = query_ray(rays, i, valid);
-    Hit h;
-    trace_block<true, MODE>(sc, r, valid, -1, h, pk[threadIdx.x >> 6], qstk + threadIdx.x, 256,
-                            lane_stack, 0);
-    if (valid) occluded[i] = h.prim >= 0 ? 1 : 0;
+    __shared__ uint32_t ovf[4][kRefillList];
+    const uint32_t wave = threadIdx.x >> 6;
+    refill_walks<ANY, MODE>(
+        sc, n, n_waves, blockIdx.x * 4 + wave, qstk + threadIdx.x, 256, lane_stack, ovf[wave], pk[wave],
+        [&](uint32_t i, Ray& r) -> int32_t {
+            r = load_ray(rays, i);
+            return -1;
+        },
+        [&](bool fin, uint32_t i, const Ray& r, Hit& h) {
+            if (fin) finish_query<ANY>(sc, hits, occluded, i, r, h);
+        });
 }
 
 __global__ void __launch_bounds__(256) k_generate_rays(rt_camera cam, uint32_t W, uint32_t H,
@@ -551,6 +551,31 @@ int check_rays(const rt_rays* r, const char* what) {
 }
 
 inline uint32_t blocks_for(uint64_t n, uint32_t per) { return (uint32_t)((n + per - 1) / per); }
+// Blocks of a k_query launch: as many as the device holds at once (the CU count times the
+// kernel's residency per CU at its LDS size, cached per device, kernel variant and stack size), or
+// fewer when the buffer has fewer 64-ray groups.
+uint32_t query_blocks(const void* kern, int variant, uint32_t lane_stack, uint64_t n) {
+    static std::mutex mu;
+    static int per_cu_cache[64][4][RT_LANE_STACK_MAX + 1] = {};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) dev = 0;
+    int per_cu = 0;
+    {
+        std::lock_guard<std::mutex> lock(mu);
+        int& slot = per_cu_cache[dev][variant][lane_stack];
+        if (slot == 0) {
+            int nb = 0;
+            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kern, 256, 4u * 256u * lane_stack) !=
+                    hipSuccess || nb <= 0)
+                nb = 1;
+            slot = nb;
+        }
+        per_cu = slot;
+    }
+    const uint64_t groups = (n + 63) / 64, waves = (uint64_t)per_cu * device_cu_count() * 4u;
+    const uint64_t w = groups < waves ? groups : waves;
+    return (uint32_t)((w + 3) / 4);
+}
 
 }  // namespace
 }  // namespace rt
@@ -568,10 +593,11 @@ extern "C" int rt_trace_closest(const rt_scene* scene, const rt_rays* rays, uint
     if ((e = resolve_tuning(tuning, tu, "rt_trace_closest"))) return e;
     if (n == 0) return RT_OK;
     const size_t lds = 4u * 256u * tu.lane_stack;
-    auto k = tu.query == RT_TRAVERSAL_LANE ? k_trace_closest<RT_TRAVERSAL_LANE>
-                                           : k_trace_closest<RT_TRAVERSAL_BINARY>;
-    hipLaunchKernelGGL(k, dim3(blocks_for(n, 256)), dim3(256), lds, (hipStream_t)stream,
-                       scene_args(scene), *rays, n, *hits, (int)tu.lane_stack);
+    auto k = tu.query == RT_TRAVERSAL_LANE ? k_query<false, RT_TRAVERSAL_LANE>
+                                           : k_query<false, RT_TRAVERSAL_BINARY>;
+    const uint32_t blocks = query_blocks((const void*)k, tu.query == RT_TRAVERSAL_LANE, tu.lane_stack, n);
+    hipLaunchKernelGGL(k, dim3(blocks), dim3(256), lds, (hipStream_t)stream, scene_args(scene), *rays, n,
+                       *hits, (uint8_t*)nullptr, (int)tu.lane_stack, 4u * blocks);
     return check_launch("rt_trace_closest");
 }
 
@@ -585,10 +611,12 @@ extern "C" int rt_trace_any(const rt_scene* scene, const rt_rays* rays, uint32_t
     if ((e = resolve_tuning(tuning, tu, "rt_trace_any"))) return e;
     if (n == 0) return RT_OK;
     const size_t lds = 4u * 256u * tu.lane_stack;
-    auto k = tu.query == RT_TRAVERSAL_LANE ? k_trace_any<RT_TRAVERSAL_LANE>
-                                           : k_trace_any<RT_TRAVERSAL_BINARY>;
-    hipLaunchKernelGGL(k, dim3(blocks_for(n, 256)), dim3(256), lds, (hipStream_t)stream,
-                       scene_args(scene), *rays, n, occluded, (int)tu.lane_stack);
+    auto k = tu.query == RT_TRAVERSAL_LANE ? k_query<true, RT_TRAVERSAL_LANE>
+                                           : k_query<true, RT_TRAVERSAL_BINARY>;
+    const uint32_t blocks =
+        query_blocks((const void*)k, 2 + (tu.query == RT_TRAVERSAL_LANE), tu.lane_stack, n);
+    hipLaunchKernelGGL(k, dim3(blocks), dim3(256), lds, (hipStream_t)stream, scene_args(scene), *rays, n,
+                       rt_hits{}, occluded, (int)tu.lane_stack, 4u * blocks);
     return check_launch("rt_trace_any");
 }
 

@@ -68,6 +68,117 @@ RT_DEV void trace_block(const SceneArgs& sc, const Ray& r, bool want, int32_t sk
 
 constexpr bool uses_lds(int mode) { return mode != RT_TRAVERSAL_PACKET; }
 
+// ---- per-lane walks with refill ----
+// Walk lengths of incoherent rays are heavy-tailed (on random rays through a 1.3M-triangle scene
+// the median ray visits ~18 nodes, the 99th percentile ~480), so a wave holding one ray per lane
+// runs as long as its longest lane while most lanes idle. refill_walks keeps the lanes fed instead:
+// - rays 0..n-1 are taken in groups of 64 consecutive ones; of G waves, wave `wid` owns the groups
+//   wid, wid + G, wid + 2G, ... (consecutive rays stay together, so coherent buffers keep their
+//   coherence, and each wave samples the whole buffer, which evens out the waves' loads);
+// - the wave walks in rounds of at most kRefillSteps iterations (one node or one leaf) per lane,
+//   and whenever kRefillIdle or more lanes are idle, the idle lanes take the wave's next rays in
+//   lane order (one ballot);
+// - a lane's walk (bin_steps or lane4_steps, MODE) is the one-ray walk resumed across rounds, so
+//   every ray gets the hit it gets alone;
+// - a ray whose LDS stack would overflow is listed in LDS (`list`, kRefillList entries); when the
+//   list could fill, the wave stops taking rays, finishes those it holds and traces the listed ones
+//   again with the packet traversal, 64 at a time, then goes on.
+// `load(i, r)` loads ray i into r and returns the triangle slot it must skip (-1: none);
+// `finish(fin, i, r, h)` is called by all 64 lanes after every round, with fin set on the lanes
+// whose ray i has its final hit h (it may use wave-wide operations). Called by all 64 lanes.
+#ifndef RT_REFILL_STEPS
+#define RT_REFILL_STEPS 16
+#endif
+#ifndef RT_REFILL_IDLE
+#define RT_REFILL_IDLE 8
+#endif
+constexpr int kRefillSteps = RT_REFILL_STEPS;       // walk iterations per round
+constexpr uint32_t kRefillIdle = RT_REFILL_IDLE;    // idle lanes that trigger a refill
+constexpr uint32_t kRefillList = 128;               // overflow list entries per wave
+
+RT_DEV uint32_t lane_rank(uint64_t mask) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+
+template <bool ANY, int MODE, typename Load, typename Finish>
+RT_DEV void refill_walks(const SceneArgs& sc, uint64_t n, uint32_t n_waves, uint32_t wid, int32_t* stk,
+                         int stk_stride, int lane_stack, uint32_t* list, float4* pk, Load&& load,
+                         Finish&& finish) {
+    static_assert(MODE == RT_TRAVERSAL_LANE || MODE == RT_TRAVERSAL_BINARY, "per-lane walk");
+    // position p of the wave's sequence -> ray index (group p / 64 of the wave, offset p % 64)
+    auto ray_at = [&](uint64_t p) -> uint64_t { return (((p >> 6) * n_waves + wid) << 6) | (p & 63u); };
+    uint64_t cursor = 0;             // wave-uniform
+    bool more = ray_at(0) < n;       // wave-uniform: rays left in the wave's sequence
+    const uint32_t lane = __lane_id();
+    while (more) {
+        uint32_t listed = 0;  // wave-uniform
+        bool busy = false;
+        uint32_t ri = 0;      // the lane's ray while busy
+        int32_t skip = -1;
+        Ray r;
+        r.ox = r.oy = r.oz = 0.0f;
+        r.dx = 1.0f;
+        r.dy = r.dz = 0.0f;
+        r.tmin = 0.0f;
+        r.tmax = INFINITY;
+        Hit h;
+        h.t = INFINITY;
+        h.u = h.v = 0.0f;
+        h.prim = h.slot = -1;
+        LaneWalk w;
+        for (;;) {
+            const uint64_t idle = __ballot(!busy);
+            const uint32_t n_idle = (uint32_t)__popcll(idle);
+            if (more && listed + 64u <= kRefillList && (n_idle >= kRefillIdle || n_idle == 64u)) {
+                if (!busy) {
+                    const uint64_t i = ray_at(cursor + lane_rank(idle));
+                    if (i < n) {
+                        ri = (uint32_t)i;
+                        skip = load(ri, r);
+                        h.t = r.tmax;
+                        h.u = h.v = 0.0f;
+                        h.prim = h.slot = -1;
+                        walk_start(r, sc.root, w);
+                        busy = true;
+                    }
+                }
+                cursor += n_idle;
+                more = ray_at(cursor) < n;  // later positions map to larger indices
+            }
+            if (__ballot(busy) == 0) break;
+            int st = kWalkOn;
+            if (busy) {
+                if (sc.root == RT_EMPTY_ROOT)
+                    st = kWalkDone;
+                else if (MODE == RT_TRAVERSAL_BINARY)
+                    st = bin_steps<ANY>(sc.nodes, sc.tris, r, skip, h, w, stk, stk_stride, lane_stack, kRefillSteps);
+                else
+                    st = lane4_steps<ANY>(sc.nodes4, sc.tris, r, skip, h, w, stk, stk_stride, lane_stack,
+                                          kRefillSteps);
+            }
+            finish(st == kWalkDone, ri, r, h);
+            if (st != kWalkOn) busy = false;
+            const uint64_t over = __ballot(st == kWalkOverflow);
+            if (over) {  // refills stop while listed > kRefillList - 64: listed stays <= kRefillList
+                if (st == kWalkOverflow) list[listed + lane_rank(over)] = ri;
+                listed += (uint32_t)__popcll(over);
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");  // the list's stores before its loads
+        for (uint32_t k = 0; k < listed; k += 64u) {
+            const bool want = k + lane < listed;
+            const uint32_t i = want ? list[k + lane] : 0u;
+            Ray rr = r;  // the placeholder's fields for lanes without a listed ray
+            int32_t sk = -1;
+            if (want) sk = load(i, rr);
+            Hit h2;
+            traverse_packet<ANY>(sc.nodes4, sc.tris, sc.root, rr, want, sk, h2, pk);
+            finish(want, i, rr, h2);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");  // its loads before the next stores
+    }
+}
+
 // RT_TRAVERSAL_AUTO for camera rays: the packet traversal when the 8x8 tile of a wave spans at most
 // kCoherentTile in the image plane at unit distance (8 pixels x 2 tan(fov/2) / height, or the
 // width's counterpart), where its rays share most of their nodes; per-lane beyond.

@@ -7,7 +7,8 @@ the same request traces here:
 - `RenderServer`: continuous batching. While batches run on the GPU the loop admits arrivals,
   makes their scenes resident (building the BVH on a miss, evicting least-recently-used scenes
   that no in-flight batch uses) and launches every request admitted since the previous launch as
-  one rt_render_batch launch, on the least busy of `n_streams` equal-priority streams: batches on
+  one rt_render_batch launch (a lone request as one rt_render), on the least busy of `n_streams`
+  equal-priority streams: batches on
   different streams overlap on the device (one batch's tail tiles share the CUs with the next
   batch's bulk), and a new batch starts as soon as a slot (at most `max_inflight` batches in
   flight) frees up.
@@ -196,14 +197,23 @@ class RenderServer:
               for r, res in zip(reqs, residents)]
         # scenes built on the current stream are ready before the batch starts
         stream.wait_stream(torch.cuda.current_stream())
-        # units of work: 4x4-pixel packets; a grab of several keeps the queue's atomics few when
-        # the batch holds many times more units than there are waves
-        units = sum(((r.size + 3) // 4) ** 2 for r in reqs)
-        grab = max(1, min(16, units // (8 * self._waves)))
-        cfg = dataclasses.replace(self.config, tiles_per_grab=grab)
         start = torch.cuda.Event(enable_timing=True)
         start.record(stream)
-        images = self.slots[id(stream)].launch(rr, cfg)
+        if len(rr) == 1:
+            # a batch of one is one rt_render: the persistent batch kernel would only add its
+            # prepare pass and job lookup to the same work
+            r = rr[0]
+            with torch.cuda.stream(stream):
+                images = [r.raycaster.render(r.camera, r.width, r.height, textures=r.textures)]
+        else:
+            # units of work: the batch kernel's tiles (4x4 pixels for QUAD, 8x8 otherwise); a grab
+            # of several keeps the work counter's atomics few when the batch holds many times more
+            # tiles than there are waves
+            unit = 4 if self.config.primary == "quad" else 8
+            units = sum(((q.size + unit - 1) // unit) ** 2 for q in reqs)
+            grab = max(1, min(16, units // (8 * self._waves)))
+            cfg = dataclasses.replace(self.config, tiles_per_grab=grab)
+            images = self.slots[id(stream)].launch(rr, cfg)
         ev = torch.cuda.Event(enable_timing=True)
         ev.record(stream)
         return {"reqs": reqs, "images": images, "event": ev, "start": start, "residents": residents,

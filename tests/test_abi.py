@@ -118,13 +118,13 @@ def test_tuning_get_set_and_range_checks_without_gpu():
     d = ren.get_tuning()
     assert d == {"primary": _native.RT_TRAVERSAL_QUAD, "shadow": _native.RT_TRAVERSAL_LANE,
                  "super_tile": 0, "lane_stack": _native.RT_LANE_STACK_MAX, "packet_budget": 256,
-                 "builder": _native.RT_BUILD_LBVH, "query": _native.RT_TRAVERSAL_BINARY,
+                 "builder": _native.RT_BUILD_LBVH, "query": _native.RT_TRAVERSAL_LANE,
                  "handoff_steps": 0, "item_steps": 32}
     with ren.tuning(primary="hybrid", lane_stack=3, super_tile=2, packet_budget=7, builder="ploc",
-                    query="lane", item_steps=5):
+                    query="binary", item_steps=5):
         assert ren.get_tuning() == {"primary": 2, "shadow": 1, "super_tile": 2, "lane_stack": 3,
                                     "packet_budget": 7, "builder": _native.RT_BUILD_PLOC,
-                                    "query": _native.RT_TRAVERSAL_LANE, "handoff_steps": 0,
+                                    "query": _native.RT_TRAVERSAL_BINARY, "handoff_steps": 0,
                                     "item_steps": 5}
     assert ren.get_tuning() == d
     for bad in ({"lane_stack": 0}, {"lane_stack": 33}, {"primary": 5}, {"shadow": 3},

@@ -172,6 +172,32 @@ def test_ray_buffer_traversals_and_stack_overflow(ren, query, lane_stack):
     np.testing.assert_array_equal(g2["prim"].cpu().numpy(), r2["prim"])
 
 
+@pytest.mark.parametrize("query,lane_stack", [("lane", 1), ("binary", 2), ("lane", 32)])
+def test_long_ray_buffers_refill_lanes_and_retrace_overflows(ren, query, lane_stack):
+    """A buffer of 1,500,001 rays: more 64-ray groups than the GPU holds waves, so every wave refills
+    its lanes from several groups; with 1- and 2-entry stacks most rays overflow, so the waves' LDS
+    lists fill, refills pause and the listed rays are traced again with the packet traversal
+    between phases. Every hit equals the twin's."""
+    from rendertoy_amd import scenes
+    meshes = [scenes.icosphere(24, 91), scenes.ground_plane()]
+    tw = twin_from_meshes(meshes)
+    rc = gpu_from_meshes(meshes)
+    n = 1_500_001
+    o, d = random_rays(n, 97)
+    rng = np.random.default_rng(98)
+    k = n // 3  # a third start inside the sphere
+    o[:k] = rng.uniform(-0.3, 0.3, (k, 3)).astype(np.float32)
+    tu = {"query": query, "lane_stack": lane_stack}
+    g = rc.trace_soa(soa_tensors(o, d), tuning=tu)
+    ref = tw.trace_closest(o, d)
+    np.testing.assert_array_equal(g["prim"].cpu().numpy(), ref["prim"])
+    hit = ref["prim"] >= 0
+    np.testing.assert_array_equal(_u32(g["t"].cpu().numpy()[hit]), _u32(ref["t"][hit]))
+    occ = rc.occluded(soa_tensors(o, d), tuning=tu).cpu().numpy()
+    np.testing.assert_array_equal(occ, tw.trace_any(o, d))
+    assert hit.mean() > 0.3
+
+
 # ---------------------------------------------------------------------------------------------
 # per-call tuning, concurrently on two streams
 # ---------------------------------------------------------------------------------------------
