@@ -10,7 +10,9 @@
 //   k_paths_primary  camera rays (8x8 tile per wave, AUTO coherence rule), hits append a bounce ray
 //                    to queue 0 (wave ballot + one atomic per wave = compaction)
 //   k_paths_bounce   per depth: the waves share the queue's 64-ray groups (count read on the
-//                    device), per-lane walks with refill, hits append to the other queue
+//                    device), per-lane walks with refill; each ray records the slot it hit
+//   k_paths_emit     per depth, in queue order: misses write their sample, hits append their
+//                    bounce to the other queue
 //   k_paths_resolve  per-pixel mean, ray counts out
 // Random numbers: Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11), key = (seed, frame), counter =
 // (pixel, sample, depth, 0); 32-bit outputs -> [0,1) as (x >> 8) * 2^-24.
@@ -102,7 +104,8 @@ struct PathArgs {
     uint32_t W, H, spp, bounces, key0, key1;
     float albedo, sky, tmin;
     float* samples;        // W*H*spp, sample id = pixel * spp + s
-    float4* q[2];          // ray queues: (o.xyz, throughput), (d.xyz, id bits), (skip bits, 0, 0, 0)
+    float4* q[2];          // ray queues: (o.xyz, throughput), (d.xyz, id bits), (skip bits, hit slot
+                           // bits, hit t, 0); the hit fields are written by k_paths_bounce
     uint32_t* counts;      // [0] camera rays, [k] rays of depth k (k = 1..bounces)
     uint32_t cap;          // queue capacity (W*H*spp)
 };
@@ -188,15 +191,16 @@ __global__ void __launch_bounds__(256) k_paths_primary(SceneArgs sc, rt_camera c
 
 // Rays of depth k, queue (k-1)&1 (its count written by the previous kernel on the stream): per-lane
 // walks of the 4-wide nodes with refill (refill_walks), the grid's waves sharing the queue's 64-ray
-// groups; each finished ray writes its sample (a miss) or appends its bounce to the other queue.
+// groups. Each finished ray records the triangle slot it hit (-1: none) and its FP32 t in its own
+// queue entry (words 1 and 2 of its third float4); k_paths_emit then shades the queue in order. Keeping the sampling
+// and emission out of the walk loop keeps the walk's registers few.
 __global__ void __launch_bounds__(64) k_paths_bounce(SceneArgs sc, PathArgs a, uint32_t k,
                                                       int lane_stack) {
     __shared__ int32_t stk[kLaneStack * 64];
     __shared__ float4 pk[kPacketLds];
     __shared__ uint32_t ovf[kRefillList];
     const uint32_t n = a.counts[k];
-    const float4* qi = a.q[(k - 1) & 1];
-    float4* qo = a.q[k & 1];
+    float4* qi = a.q[(k - 1) & 1];
     refill_walks<false, RT_TRAVERSAL_LANE>(
         sc, n, gridDim.x, blockIdx.x, stk + threadIdx.x, 64, lane_stack, ovf, pk,
         [&](uint32_t i, Ray& r) -> int32_t {
@@ -211,18 +215,56 @@ __global__ void __launch_bounds__(64) k_paths_bounce(SceneArgs sc, PathArgs a, u
             r.tmax = INFINITY;
             return __float_as_int(e2.x);
         },
-        [&](bool fin, uint32_t i, const Ray& r, Hit& h) {
-            float thr = 0.0f;
-            uint32_t id = 0;
+        [&](bool fin, uint32_t i, const Ray&, Hit& h) {
             if (fin) {
-                thr = qi[(size_t)i * 3].w;
-                id = (uint32_t)__float_as_int(qi[(size_t)i * 3 + 1].w);
+                float* e2 = (float*)(qi + (size_t)i * 3 + 2);
+                e2[1] = __int_as_float(h.slot);
+                e2[2] = h.t;  // the walk's FP32 t, kept where the FP64 refinement declines
             }
-            const bool hit = fin && h.prim >= 0;
-            if (fin && !hit) a.samples[id] = thr * a.sky;
-            if (hit) refine_hit(sc.tris, r, h);
-            if (k < a.bounces) emit_bounce(sc, a, r, h, hit, id, k + 1, thr, qo, &a.counts[k + 1]);
         });
+}
+
+// The rays of depth k after k_paths_bounce, in queue order: a miss writes its sample; a hit is
+// refined in FP64 and appends its bounce to queue k&1 (if k < bounces). All lanes of a wave run
+// the same iterations (the emission is wave-wide).
+__global__ void __launch_bounds__(256) k_paths_emit(SceneArgs sc, PathArgs a, uint32_t k) {
+    const uint32_t n = a.counts[k];
+    const float4* qi = a.q[(k - 1) & 1];
+    float4* qo = a.q[k & 1];
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += stride) {
+        const uint32_t i = base + threadIdx.x;
+        const bool valid = i < n;
+        Ray r;
+        r.ox = r.oy = r.oz = 0.0f;
+        r.dx = 1.0f;
+        r.dy = r.dz = 0.0f;
+        Hit h;
+        h.t = INFINITY;
+        h.u = h.v = 0.0f;
+        h.prim = h.slot = -1;
+        float thr = 0.0f;
+        uint32_t id = 0;
+        if (valid) {
+            const float4 e0 = qi[(size_t)i * 3], e1 = qi[(size_t)i * 3 + 1], e2 = qi[(size_t)i * 3 + 2];
+            r.ox = e0.x;
+            r.oy = e0.y;
+            r.oz = e0.z;
+            thr = e0.w;
+            r.dx = e1.x;
+            r.dy = e1.y;
+            r.dz = e1.z;
+            id = (uint32_t)__float_as_int(e1.w);
+            h.slot = __float_as_int(e2.y);
+            h.t = e2.z;
+        }
+        r.tmin = a.tmin;
+        r.tmax = INFINITY;
+        const bool hit = valid && h.slot >= 0;
+        if (valid && !hit) a.samples[id] = thr * a.sky;
+        if (hit) refine_hit(sc.tris, r, h);
+        if (k < a.bounces) emit_bounce(sc, a, r, h, hit, id, k + 1, thr, qo, &a.counts[k + 1]);
+    }
 }
 
 __global__ void __launch_bounds__(256) k_paths_resolve(PathArgs a, float* __restrict__ radiance,
@@ -298,8 +340,10 @@ extern "C" int rt_trace_paths(const rt_scene* scene, const rt_camera* cam, const
                            tiles_x, (int)tu.lane_stack, (int)tu.packet_budget, coherent);
     }
     const uint32_t grid = (uint32_t)device_cu_count() * 16u;
-    for (uint32_t k = 1; k <= p->bounces; ++k)
+    for (uint32_t k = 1; k <= p->bounces; ++k) {
         hipLaunchKernelGGL(k_paths_bounce, dim3(grid), dim3(64), 0, s, sc, a, k, (int)tu.lane_stack);
+        hipLaunchKernelGGL(k_paths_emit, dim3(grid), dim3(256), 0, s, sc, a, k);
+    }
     const uint64_t px = (uint64_t)p->width * p->height;
     const uint32_t rb = (uint32_t)((std::max<uint64_t>(px, RT_PATHS_MAX_BOUNCES + 1) + 255) / 256);
     hipLaunchKernelGGL(k_paths_resolve, dim3(rb), dim3(256), 0, s, a, out->radiance, out->counts);

@@ -161,12 +161,13 @@ class RenderServer:
     """Serves Requests in arrival order by continuous batching (see the module docstring)."""
 
     def __init__(self, store: SceneStore, config: BatchConfig = None, max_batch: int = 512,
-                 max_inflight: int = 4, n_streams: int = 4):
+                 max_inflight: int = 4, n_streams: int = 4, split_backlog: int = 256):
         self.store = store
         # the best launch shape on the saturated serving workload (500 requests over the 64-scene
         # pool): 16 waves per CU, 4 tiles per grab, per-image AUTO primary walk, per-lane shadows
         self.config = config or BatchConfig(waves_per_cu=16, tiles_per_grab=4, shadow="lane", primary="auto")
         self.max_batch = max_batch
+        self.split_backlog = split_backlog
         self.max_inflight = max(1, min(max_inflight, n_streams))
         # equal (default) priority streams: batches on different streams overlap on the device; one
         # reusable job table and workspace per stream (at most one batch per stream in flight)
@@ -193,6 +194,11 @@ class RenderServer:
                 break
         else:
             rest = []
+        # the batch's jobs grouped by scene (larger images first): tiles of one scene follow each
+        # other in the batch's tile order, so its hierarchy stays in the caches
+        order = sorted(range(len(reqs)), key=lambda i: (reqs[i].scene, -reqs[i].size))
+        reqs = [reqs[i] for i in order]
+        residents = [residents[i] for i in order]
         rr = [RenderRequest(res.raycaster, camera_for(r.eye), r.size, r.size, textures=res.binding)
               for r, res in zip(reqs, residents)]
         # scenes built on the current stream are ready before the batch starts
@@ -246,9 +252,13 @@ class RenderServer:
                 inflight.remove(f)
                 self._finish(f)
             if queue and len(inflight) < self.max_inflight:
-                # a backlog is split over the free slots, so that its batches overlap on the device
+                # a large backlog is split over the free slots, so that its batches overlap on the
+                # device; a short queue goes as one batch (renders sharing the device lengthen each
+                # other, which only pays when every batch is large)
                 free = self.max_inflight - len(inflight)
-                n = min(len(queue), self.max_batch, max(1, -(-len(queue) // free)))
+                n = min(len(queue), self.max_batch)
+                if len(queue) >= self.split_backlog:
+                    n = min(n, -(-len(queue) // free))
                 rec, rest = self._launch([queue.popleft() for _ in range(n)], t0_event,
                                          self._pick_stream(inflight))
                 queue.extendleft(reversed(rest))
