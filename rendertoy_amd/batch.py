@@ -53,7 +53,7 @@ class BatchConfig:
     waves_per_cu: int = 16
     tiles_per_grab: int = 4
     shadow: str = "lane"
-    primary: str = "auto"
+    primary: str = "lane"
     lane_stack: int = N.RT_LANE_STACK_MAX
     packet_budget: int = 256
 
@@ -235,7 +235,8 @@ def time_launches(fn, reps, warmup=1):
 
 DEFAULT_CANDIDATES = ([BatchConfig(w, g, "lane", "quad") for w in (8, 16, 32) for g in (1, 4, 16)] +
                       [BatchConfig(w, g, s, p) for p in ("auto", "packet") for s in ("lane", "packet")
-                       for w in (16, 32) for g in (1, 4)])
+                       for w in (16, 32) for g in (1, 4)] +
+                      [BatchConfig(w, g, "lane", "lane") for w in (16, 32) for g in (1, 4, 16)])
 
 
 def autotune(batch: BatchRender, candidates=None, reps=3):

@@ -724,11 +724,12 @@ RT_DEV void traverse_quad(const float4* __restrict__ nodes4, const float4* __res
 
 constexpr int kLaneStack = RT_LANE_STACK_MAX;
 
-// At most `steps` iterations of the 4-wide walk from state w (returns as bin_steps).
+// At most `steps` iterations of the 4-wide walk from state w (returns as bin_steps). `any_rt`: the
+// walk ends at the first hit, as with ANY (a per-lane choice).
 template <bool ANY>
 RT_DEV int lane4_steps(const float4* __restrict__ nodes4, const float4* __restrict__ tris, const Ray& r,
                        int32_t skip_slot, Hit& h, LaneWalk& w, int32_t* stk, int stride, int limit,
-                       int steps) {
+                       int steps, bool any_rt = false) {
     for (int s = 0; s < steps; ++s) {
         if (w.cur >= 0) {
             const float4* nd = nodes4 + (size_t)w.cur * 8;
@@ -777,7 +778,7 @@ RT_DEV int lane4_steps(const float4* __restrict__ nodes4, const float4* __restri
                 continue;
             }
         } else {
-            if (test_leaf(tris, w.cur, r, true, skip_slot, h) && ANY) return kWalkDone;
+            if (test_leaf(tris, w.cur, r, true, skip_slot, h) && (ANY || any_rt)) return kWalkDone;
         }
         if (w.sp == 0) return kWalkDone;
         w.cur = stk[(--w.sp) * stride];

@@ -278,6 +278,47 @@ RT_DEV void tex_sample(const rt_texture& tx, float u, float v, bool bilinear, fl
     }
 }
 
+// Shading terms of a refined hit h: Lambert factor max(0, N.L) with the normal interpolated from the
+// vertex normals and normalised, the interpolated texture coordinates and the mesh id.
+RT_DEV void shade_terms(const SceneArgs& sc, const rt_render_params& p, const Hit& h, float& ndl, float& tu,
+                        float& tv, int32_t& mesh) {
+    const float4* s = sc.shade + (uint64_t)h.slot * 4;
+    const float4 s0 = s[0], s1 = s[1], s2 = s[2], s3 = s[3];
+    const float w0 = 1.0f - h.u - h.v;
+    const float nx = fmaf(w0, s0.x, fmaf(h.u, s0.w, h.v * s1.z));
+    const float ny = fmaf(w0, s0.y, fmaf(h.u, s1.x, h.v * s1.w));
+    const float nz = fmaf(w0, s0.z, fmaf(h.u, s1.y, h.v * s2.x));
+    const float l2 = fmaf(nx, nx, fmaf(ny, ny, nz * nz));
+    const float il = l2 > 0.0f ? 1.0f / sqrtf(l2) : 0.0f;
+    ndl = fmaf(nx * il, p.light[0], fmaf(ny * il, p.light[1], (nz * il) * p.light[2]));
+    ndl = fmaxf(ndl, 0.0f);
+    tu = fmaf(w0, s2.y, fmaf(h.u, s2.w, h.v * s3.y));
+    tv = fmaf(w0, s2.z, fmaf(h.u, s3.x, h.v * s3.z));
+    mesh = __float_as_int(s3.w);
+}
+
+// The shadow ray of hit h of primary ray r: from the hit point toward the light.
+RT_DEV Ray shadow_ray(const rt_render_params& p, const Ray& r, const Hit& h) {
+    Ray sr;
+    sr.ox = fmaf(h.t, r.dx, r.ox);
+    sr.oy = fmaf(h.t, r.dy, r.oy);
+    sr.oz = fmaf(h.t, r.dz, r.oz);
+    sr.dx = p.light[0];
+    sr.dy = p.light[1];
+    sr.dz = p.light[2];
+    sr.tmin = p.shadow_tmin;
+    sr.tmax = INFINITY;
+    return sr;
+}
+
+// Albedo of a hit on mesh `mesh` at (tu, tv): its texture's sample, or 1 without one.
+RT_DEV void albedo(const rt_render_params& p, int32_t mesh, float tu, float tv, float* alb) {
+    if (!p.mesh_texture_dev) return;
+    const int32_t ti = p.mesh_texture_dev[mesh];
+    if (ti >= 0 && (uint32_t)ti < p.n_textures)
+        tex_sample(p.textures_dev[ti], tu, tv, (p.flags & RT_RENDER_BILINEAR) != 0, alb);
+}
+
 // Shading of one pixel per lane: primary closest hit, FP64 refinement, Lambert with interpolated
 // normals, one shadow ray toward the light per lit hit (skipping the hit's own triangle), RGBA8
 // albedo, RGB store; per-wave counts into stats. PM / SM: traversal (RT_TRAVERSAL_*) of the primary
@@ -304,33 +345,11 @@ RT_DEV void render_pixel(const SceneArgs& sc, const rt_camera& cam, const rt_ren
 
     float ndl = 0.0f, tu = 0.0f, tv = 0.0f;
     int32_t mesh = 0;
-    if (hit) {
-        const float4* s = sc.shade + (uint64_t)h.slot * 4;
-        const float4 s0 = s[0], s1 = s[1], s2 = s[2], s3 = s[3];
-        const float w0 = 1.0f - h.u - h.v;
-        const float nx = fmaf(w0, s0.x, fmaf(h.u, s0.w, h.v * s1.z));
-        const float ny = fmaf(w0, s0.y, fmaf(h.u, s1.x, h.v * s1.w));
-        const float nz = fmaf(w0, s0.z, fmaf(h.u, s1.y, h.v * s2.x));
-        const float l2 = fmaf(nx, nx, fmaf(ny, ny, nz * nz));
-        const float il = l2 > 0.0f ? 1.0f / sqrtf(l2) : 0.0f;
-        ndl = fmaf(nx * il, p.light[0], fmaf(ny * il, p.light[1], (nz * il) * p.light[2]));
-        ndl = fmaxf(ndl, 0.0f);
-        tu = fmaf(w0, s2.y, fmaf(h.u, s2.w, h.v * s3.y));
-        tv = fmaf(w0, s2.z, fmaf(h.u, s3.x, h.v * s3.z));
-        mesh = __float_as_int(s3.w);
-    }
+    if (hit) shade_terms(sc, p, h, ndl, tu, tv, mesh);
 
     // shadow ray toward the light from every lit hit of the tile
     const bool want_shadow = hit && (p.flags & RT_RENDER_SHADOWS) && ndl > 0.0f;
-    Ray sr;
-    sr.ox = fmaf(h.t, r.dx, r.ox);
-    sr.oy = fmaf(h.t, r.dy, r.oy);
-    sr.oz = fmaf(h.t, r.dz, r.oz);
-    sr.dx = p.light[0];
-    sr.dy = p.light[1];
-    sr.dz = p.light[2];
-    sr.tmin = p.shadow_tmin;
-    sr.tmax = INFINITY;
+    const Ray sr = shadow_ray(p, r, h);
     Hit sh;
     if constexpr (QL)
         traverse_quad<true>(sc.nodes4, sc.tris, sc.root, sr, want_shadow, hit ? h.slot : -1, sh);
@@ -340,11 +359,7 @@ RT_DEV void render_pixel(const SceneArgs& sc, const rt_camera& cam, const rt_ren
     const bool occluded = want_shadow && sh.prim >= 0;
 
     float alb[3] = {1.0f, 1.0f, 1.0f};
-    if (hit && p.mesh_texture_dev) {
-        const int32_t ti = p.mesh_texture_dev[mesh];
-        if (ti >= 0 && (uint32_t)ti < p.n_textures)
-            tex_sample(p.textures_dev[ti], tu, tv, (p.flags & RT_RENDER_BILINEAR) != 0, alb);
-    }
+    if (hit) albedo(p, mesh, tu, tv, alb);
     if (px.valid && (!QL || (threadIdx.x & 3) == 0)) {
         const uint64_t pix = (uint64_t)px.row * W + px.col;
         if (hit) {
@@ -536,6 +551,224 @@ __global__ void __launch_bounds__(64) k_render_batch(const rt_render_job* __rest
                                  coherent_tiles(job.cam, p.width, p.height));
         }
     }
+}
+
+// Batched render with per-lane refill (rt_batch_config primary = shadow = RT_TRAVERSAL_LANE). The
+// wave takes `grab` tiles at a time from the work counter, as k_render_batch does, but its lanes
+// are not tied to one tile: each lane carries one pixel through its primary walk, its shading and
+// its shadow walk (both walks resumable, in rounds of kRefillSteps iterations), and whenever
+// kRefillIdle or more lanes are idle they take the next pixels of the wave's tiles (one ballot).
+// A wave holds pixels of one job at a time (it drains before it moves to another job's tiles), so
+// the job's scene and parameters stay wave-uniform. A walk that would overflow its LDS stack is
+// traced again at once by the whole wave with the packet traversal. Every pixel gets exactly the
+// values render_pixel<LANE, LANE> gives it (same walks, refinement, shading and store).
+__global__ void __launch_bounds__(64) k_render_batch_refill(const rt_render_job* __restrict__ jobs,
+                                                             uint32_t n_jobs, uint32_t* __restrict__ ws,
+                                                             uint32_t grab, int lane_stack) {
+    extern __shared__ uint32_t smem[];
+    __shared__ float4 pk[kPacketLds];
+    uint32_t* pref = smem;
+    int32_t* stk = (int32_t*)(smem + ((n_jobs + 1 + 3) & ~3u)) + threadIdx.x;
+    const uint32_t lane = threadIdx.x;
+    for (uint32_t j = lane; j <= n_jobs; j += 64) pref[j] = ws[kBatchPrefixWord + j];
+    __syncthreads();
+    const uint32_t total = pref[n_jobs];
+
+    // wave-uniform: the tile being handed out, its next pixel, the end of the grabbed tiles, and
+    // the job the lanes work on (with its scene, camera, parameters and counts)
+    uint32_t tile = 0, tile_px = 64, pool_end = 0, job = 0xFFFFFFFFu, tile_job = 0;
+    bool exhausted = false;
+    SceneArgs sc{};
+    rt_camera cam{};
+    rt_render_params p{};
+    rt_render_out out{};
+    uint32_t tiles_x = 1, job_first = 0, n_hit = 0, n_sh = 0, n_oc = 0;
+    auto flush = [&]() {
+        if (job != 0xFFFFFFFFu && out.stats && lane == 0) {
+            if (n_hit) atomicAdd(&out.stats[0], n_hit);
+            if (n_sh) atomicAdd(&out.stats[1], n_sh);
+            if (n_oc) atomicAdd(&out.stats[2], n_oc);
+        }
+        n_hit = n_sh = n_oc = 0;
+    };
+
+    // per lane: the pixel, the phase (0: primary walk, 1: shadow walk), the walk and the shading
+    bool busy = false;
+    uint32_t phase = 0, col = 0, row = 0;
+    Ray r;
+    r.ox = r.oy = r.oz = 0.0f;
+    r.dx = 1.0f;
+    r.dy = r.dz = 0.0f;
+    r.tmin = 0.0f;
+    r.tmax = INFINITY;
+    Hit h;
+    h.t = INFINITY;
+    h.u = h.v = 0.0f;
+    h.prim = h.slot = -1;
+    LaneWalk w;
+    walk_start(r, RT_EMPTY_ROOT, w);
+    int32_t skip = -1;
+    float ndl = 0.0f, alb0 = 1.0f, alb1 = 1.0f, alb2 = 1.0f;
+
+    for (;;) {
+        // ---- refill from the current tile (moving on to the next tile / pool / job as needed)
+        const uint64_t idle = __ballot(!busy);
+        const uint32_t n_idle = (uint32_t)__popcll(idle);
+        if (!exhausted && (n_idle >= kRefillIdle || n_idle == 64u)) {
+            if (tile_px == 64u) {
+                if (++tile >= pool_end) {
+                    uint32_t u0 = 0;
+                    if (lane == 0) u0 = atomicAdd(&ws[0], grab);
+                    u0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)u0);
+                    if (u0 >= total) {
+                        exhausted = true;
+                    } else {
+                        tile = u0;
+                        pool_end = u0 + grab < total ? u0 + grab : total;
+                    }
+                }
+                if (!exhausted) {
+                    tile_px = 0;
+                    uint32_t lo = 0, hi = n_jobs;  // largest j with pref[j] <= tile
+                    while (hi - lo > 1) {
+                        const uint32_t mid = (lo + hi) >> 1;
+                        if (pref[mid] <= tile) lo = mid; else hi = mid;
+                    }
+                    tile_job = (uint32_t)__builtin_amdgcn_readfirstlane((int)lo);
+                }
+            }
+            if (!exhausted && tile_job != job && n_idle == 64u) {  // drained: move to the tile's job
+                flush();
+                job = tile_job;
+                const rt_render_job& jb = jobs[job];
+                sc.nodes = (const float4*)jb.scene.nodes;
+                sc.nodes4 = (const float4*)jb.scene.nodes4;
+                sc.tris = (const float4*)jb.scene.tris;
+                sc.shade = (const float4*)jb.scene.shade;
+                sc.mesh_offsets = jb.scene.mesh_offsets;
+                sc.prim_slot = jb.scene.prim_slot;
+                sc.root = jb.scene.root;
+                sc.n_meshes = jb.scene.n_meshes;
+                cam = jb.cam;
+                p = jb.params;
+                out = jb.out;
+                tiles_x = (p.width + 7u) / 8u;
+                job_first = pref[job];
+            }
+            if (!exhausted && tile_job == job) {
+                const uint32_t take = 64u - tile_px < n_idle ? 64u - tile_px : n_idle;
+                if (!busy) {
+                    const uint32_t k = lane_rank(idle);
+                    if (k < take) {
+                        const uint32_t q = tile_px + k, local = tile - job_first;
+                        TilePixel px;
+                        px.col = (local % tiles_x) * 8u + (q & 7u);
+                        px.row = (local / tiles_x) * 8u + (q >> 3);
+                        px.valid = px.col < p.width && px.row < p.height;
+                        if (px.valid) {
+                            col = px.col;
+                            row = px.row;
+                            r = camera_ray(cam, p.width, p.height, px);
+                            h.t = r.tmax;
+                            h.u = h.v = 0.0f;
+                            h.prim = h.slot = -1;
+                            walk_start(r, sc.root, w);
+                            skip = -1;
+                            phase = 0;
+                            busy = true;
+                        }
+                    }
+                }
+                tile_px += take;
+            }
+        }
+        if (__ballot(busy) == 0) {
+            if (exhausted) break;
+            continue;
+        }
+
+        // ---- walk
+        int st = kWalkOn;
+        if (busy) {
+            if (sc.root == RT_EMPTY_ROOT)
+                st = kWalkDone;
+            else
+                st = lane4_steps<false>(sc.nodes4, sc.tris, r, skip, h, w, stk, 64, lane_stack, kRefillSteps,
+                                        phase == 1u);
+        }
+        if (__ballot(st == kWalkOverflow)) {  // rare: the whole wave traces those rays as a packet
+            const bool redo0 = st == kWalkOverflow && phase == 0u, redo1 = st == kWalkOverflow && phase == 1u;
+            if (__ballot(redo0)) {
+                Hit h2;
+                traverse_packet<false>(sc.nodes4, sc.tris, sc.root, r, redo0, -1, h2, pk);
+                if (redo0) h = h2;
+            }
+            if (__ballot(redo1)) {
+                Hit h2;
+                traverse_packet<true>(sc.nodes4, sc.tris, sc.root, r, redo1, skip, h2, pk);
+                if (redo1) h = h2;
+            }
+            if (st == kWalkOverflow) st = kWalkDone;
+        }
+
+        // ---- finish: shade after the primary walk (and start the shadow walk), store after the last
+        const bool fin0 = st == kWalkDone && phase == 0u, fin1 = st == kWalkDone && phase == 1u;
+        bool hit = false, go_shadow = false, occluded = false, store = false;
+        if (fin0) {
+            hit = h.prim >= 0;
+            ndl = 0.0f;
+            alb0 = alb1 = alb2 = 1.0f;
+            if (hit) {
+                refine_hit(sc.tris, r, h);
+                float tu = 0.0f, tv = 0.0f;
+                int32_t mesh = 0;
+                shade_terms(sc, p, h, ndl, tu, tv, mesh);
+                float alb[3] = {1.0f, 1.0f, 1.0f};
+                albedo(p, mesh, tu, tv, alb);
+                alb0 = alb[0];
+                alb1 = alb[1];
+                alb2 = alb[2];
+            }
+            const uint64_t pix = (uint64_t)row * p.width + col;
+            if (out.t) out.t[pix] = hit ? h.t : INFINITY;
+            if (out.prim) out.prim[pix] = hit ? h.prim : -1;
+            go_shadow = hit && (p.flags & RT_RENDER_SHADOWS) && ndl > 0.0f;
+            if (go_shadow) {
+                r = shadow_ray(p, r, h);
+                skip = h.slot;
+                h.t = r.tmax;
+                h.u = h.v = 0.0f;
+                h.prim = h.slot = -1;
+                walk_start(r, sc.root, w);
+                phase = 1;
+            } else {
+                store = true;
+            }
+        }
+        if (fin1) {
+            occluded = h.prim >= 0;
+            hit = true;
+            store = true;
+        }
+        if (store) {
+            const uint64_t pix = (uint64_t)row * p.width + col;
+            if (hit) {
+                const float kk = fmaf(occluded ? 0.0f : 1.0f, ndl, p.ambient);
+                out.rgb[3 * pix] = alb0 * kk;
+                out.rgb[3 * pix + 1] = alb1 * kk;
+                out.rgb[3 * pix + 2] = alb2 * kk;
+            } else {
+                out.rgb[3 * pix] = p.background[0];
+                out.rgb[3 * pix + 1] = p.background[1];
+                out.rgb[3 * pix + 2] = p.background[2];
+            }
+            busy = false;
+        }
+        n_hit += (uint32_t)__popcll(__ballot(fin0 && hit));
+        n_sh += (uint32_t)__popcll(__ballot(go_shadow));
+        n_oc += (uint32_t)__popcll(__ballot(occluded));
+    }
+    flush();
 }
 
 __global__ void k_zero_u32(uint32_t* c, uint32_t n) {
@@ -754,6 +987,11 @@ int launch_batch(const rt_render_job* jobs_dev, uint32_t n_jobs, const rt_batch_
         {k_render_batch<1, 0>, k_render_batch<1, 1>, k_render_batch<1, 2>},
         {k_render_batch<2, 0>, k_render_batch<2, 1>, k_render_batch<2, 2>},
         {k_render_batch<3, 0>, k_render_batch<3, 1>, k_render_batch<3, 2>}};
+    if (cfg->primary == RT_TRAVERSAL_LANE && cfg->shadow == RT_TRAVERSAL_LANE) {
+        hipLaunchKernelGGL(k_render_batch_refill, dim3(grid), dim3(64), lds, s, jobs_dev, n_jobs, ws,
+                           cfg->tiles_per_grab, (int)cfg->lane_stack);
+        return check_launch(what);
+    }
     const K kern = quad ? k_render_batch<RT_TRAVERSAL_QUAD, RT_TRAVERSAL_QUAD> : kerns[cfg->primary][cfg->shadow];
     hipLaunchKernelGGL(kern, dim3(grid), dim3(64), lds, s, jobs_dev, n_jobs, ws, cfg->tiles_per_grab,
                        (int)cfg->lane_stack, (int)cfg->packet_budget);

@@ -164,8 +164,8 @@ class RenderServer:
                  max_inflight: int = 4, n_streams: int = 4, split_backlog: int = 256):
         self.store = store
         # the best launch shape on the saturated serving workload (500 requests over the 64-scene
-        # pool): 16 waves per CU, 4 tiles per grab, per-image AUTO primary walk, per-lane shadows
-        self.config = config or BatchConfig(waves_per_cu=16, tiles_per_grab=4, shadow="lane", primary="auto")
+        # pool): 16 waves per CU, 4 tiles per grab, per-lane primary and shadow walks with refill
+        self.config = config or BatchConfig(waves_per_cu=16, tiles_per_grab=4, shadow="lane", primary="lane")
         self.max_batch = max_batch
         self.split_backlog = split_backlog
         self.max_inflight = max(1, min(max_inflight, n_streams))

@@ -79,7 +79,11 @@ def assert_batch_equals_alone(b, reqs):
                                  dict(tiles_per_grab=3, waves_per_cu=1, primary="lane"),
                                  dict(tiles_per_grab=64, waves_per_cu=32, lane_stack=2),
                                  dict(primary="hybrid", shadow="hybrid", packet_budget=2),
-                                 dict(primary="packet", shadow="lane"), dict(primary="auto")])
+                                 dict(primary="packet", shadow="lane"), dict(primary="auto"),
+                                 # per-lane refill kernel (LANE + LANE): stacks of 1 entry force the
+                                 # packet retrace of both walks; grabs of 64 tiles cross jobs
+                                 dict(primary="lane", shadow="lane", lane_stack=1, tiles_per_grab=2),
+                                 dict(primary="lane", shadow="lane", tiles_per_grab=64, waves_per_cu=32)])
 def test_batch_equals_individual_renders(ren, scenes_, cfg):
     from rendertoy_amd.batch import BatchRender, BatchConfig
     reqs = requests(ren, scenes_)
