@@ -120,6 +120,10 @@ class Camera:
         return c
 
 
+# the default light direction, normalize(1, 1, 1) in float32 (tutorials/lesson09_texture_mapping.py:73)
+_DEFAULT_LIGHT = [float(v) for v in np.array([1.0, 1.0, 1.0], dtype=np.float32) / np.float32(np.sqrt(3.0))]
+
+
 def _ptr(t) -> int:
     if t is None:
         return None
@@ -723,13 +727,10 @@ class Raycaster:
         (allocated on the current stream)."""
         n = width * height
         dev = torch.device("cuda")
-        if light is None:
-            light = np.array([1.0, 1.0, 1.0], dtype=np.float32) / np.float32(np.sqrt(3.0))
-        light = np.asarray(light, dtype=np.float32)
         p = N.rt_render_params()
         p.width, p.height = width, height
         p.flags = (N.RT_RENDER_SHADOWS if shadows else 0) | (N.RT_RENDER_BILINEAR if bilinear else 0)
-        p.light[:] = [float(v) for v in light]
+        p.light[:] = _DEFAULT_LIGHT if light is None else [float(v) for v in np.asarray(light, dtype=np.float32)]
         p.ambient = float(ambient)
         p.background[:] = [float(v) for v in background]
         p.shadow_tmin = float(shadow_tmin)

@@ -6,12 +6,12 @@ the same request traces here:
 
 - `RenderServer`: continuous batching. While batches run on the GPU the loop admits arrivals,
   makes their scenes resident (building the BVH on a miss, evicting least-recently-used scenes
-  that no in-flight batch uses) and launches every request admitted since the previous launch as
-  one rt_render_batch launch (a lone request as one rt_render), on the least busy of `n_streams`
-  equal-priority streams: batches on
-  different streams overlap on the device (one batch's tail tiles share the CUs with the next
-  batch's bulk), and a new batch starts as soon as a slot (at most `max_inflight` batches in
-  flight) frees up.
+  that no in-flight batch uses) and launches the requests admitted since the previous launch (at
+  most `max_batch`, grouped by scene) as one rt_render_batch launch (a lone request as one
+  rt_render, in arrival order on the first stream), on the least busy of `n_streams`
+  equal-priority streams: batches on different streams overlap on the device (one batch's tail
+  tiles share the CUs with the next batch's bulk), and a new batch starts as soon as a slot (at
+  most `max_inflight` batches in flight) frees up.
 - `PlainServer`: the per-request baseline: one rt_render launch per request as it arrives, in
   arrival order on one stream (what a caller of the plain entry point does).
 
@@ -149,25 +149,43 @@ class Request:
 
 
 def camera_for(eye):
-    """Camera at `eye` looking at the origin, 45 deg, up +y (+z when the view is near-vertical)."""
+    """Camera at `eye` looking at the origin, 45 deg, aspect 1, up +y (+z when the view is
+    near-vertical): the basis of Camera(eye, origin, up) computed on Python floats in double
+    precision and stored as float32 (the serving loop makes one camera per request, and numpy's
+    per-call overhead on 3-vectors was ~60 us of host time per camera)."""
     from ._raycaster import Camera
-    e = np.asarray(eye, np.float64)
-    d = -e / np.linalg.norm(e)
-    up = (0.0, 0.0, 1.0) if abs(d[1]) > 0.99 else (0.0, 1.0, 0.0)
-    return Camera(tuple(float(x) for x in e), (0.0, 0.0, 0.0), up, math.pi / 4, 1.0)
+    ex, ey, ez = (float(v) for v in eye)
+    n = math.sqrt(ex * ex + ey * ey + ez * ez)
+    zx, zy, zz = -ex / n, -ey / n, -ez / n
+    ux, uy, uz = (0.0, 0.0, 1.0) if abs(zy) > 0.99 else (0.0, 1.0, 0.0)
+    xx, xy, xz = uy * zz - uz * zy, uz * zx - ux * zz, ux * zy - uy * zx  # up x z, normalised
+    m = math.sqrt(xx * xx + xy * xy + xz * xz)
+    xx, xy, xz = xx / m, xy / m, xz / m
+    yx, yy, yz = zy * xz - zz * xy, zz * xx - zx * xz, zx * xy - zy * xx  # z x x
+    t = math.tan(math.pi / 8)
+    cam = Camera.__new__(Camera)
+    cam.eye = np.array((ex, ey, ez), np.float32)
+    cam.right = np.array((xx * t, xy * t, xz * t), np.float32)
+    cam.up = np.array((yx * t, yy * t, yz * t), np.float32)
+    cam.forward = np.array((zx, zy, zz), np.float32)
+    cam.fov = math.pi / 4
+    cam.aspect = 1.0
+    return cam
 
 
 class RenderServer:
     """Serves Requests in arrival order by continuous batching (see the module docstring)."""
 
-    def __init__(self, store: SceneStore, config: BatchConfig = None, max_batch: int = 512,
-                 max_inflight: int = 4, n_streams: int = 4, split_backlog: int = 256):
+    def __init__(self, store: SceneStore, config: BatchConfig = None, max_batch: int = 128,
+                 max_inflight: int = 4, n_streams: int = 4):
         self.store = store
         # the best launch shape on the saturated serving workload (500 requests over the 64-scene
         # pool): 16 waves per CU, 4 tiles per grab, per-lane primary and shadow walks with refill
         self.config = config or BatchConfig(waves_per_cu=16, tiles_per_grab=4, shadow="lane", primary="lane")
+        # at most 128 requests per batch: a backlog goes out as several batches that overlap on
+        # the device (saturated serving workload: 12.9k requests/s at 128, 12.3k at 256, 6.8k as
+        # one batch of 1000, whose job table the host must build before anything starts)
         self.max_batch = max_batch
-        self.split_backlog = split_backlog
         self.max_inflight = max(1, min(max_inflight, n_streams))
         # equal (default) priority streams: batches on different streams overlap on the device; one
         # reusable job table and workspace per stream (at most one batch per stream in flight)
@@ -252,15 +270,12 @@ class RenderServer:
                 inflight.remove(f)
                 self._finish(f)
             if queue and len(inflight) < self.max_inflight:
-                # a large backlog is split over the free slots, so that its batches overlap on the
-                # device; a short queue goes as one batch (renders sharing the device lengthen each
-                # other, which only pays when every batch is large)
-                free = self.max_inflight - len(inflight)
                 n = min(len(queue), self.max_batch)
-                if len(queue) >= self.split_backlog:
-                    n = min(n, -(-len(queue) // free))
-                rec, rest = self._launch([queue.popleft() for _ in range(n)], t0_event,
-                                         self._pick_stream(inflight))
+                # lone requests keep to the first stream, in arrival order (as the plain loop runs
+                # them), so they do not share the device with each other; batches take the least
+                # busy stream
+                stream = self.streams[0] if n == 1 else self._pick_stream(inflight)
+                rec, rest = self._launch([queue.popleft() for _ in range(n)], t0_event, stream)
                 queue.extendleft(reversed(rest))
                 if rec is not None:
                     inflight.append(rec)
