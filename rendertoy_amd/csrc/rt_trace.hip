@@ -101,7 +101,7 @@ __global__ void __launch_bounds__(256) k_query(SceneArgs sc, rt_rays rays, uint3
     __shared__ uint32_t ovf[4][kRefillList];
     const uint32_t wave = threadIdx.x >> 6;
     refill_walks<ANY, MODE>(
-        sc, n, n_waves, blockIdx.x * 4 + wave, qstk + threadIdx.x, 256, lane_stack, ovf[wave], pk[wave],
+        sc, n, n_waves, refill_slot(blockIdx.x, wave, 4, gridDim.x), qstk + threadIdx.x, 256, lane_stack, ovf[wave], pk[wave],
         [&](uint32_t i, Ray& r) -> int32_t {
             r = load_ray(rays, i);
             return -1;

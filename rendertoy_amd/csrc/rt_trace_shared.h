@@ -100,6 +100,15 @@ RT_DEV uint32_t lane_rank(uint64_t mask) {
     return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
 }
 
+// The wave's place in refill_walks' order: blocks are handed to the 8 XCDs round robin (block b runs
+// on XCD b % 8), so waves w = b * waves_per_block + i are renumbered XCD-major; the G waves work on
+// G consecutive 64-ray groups at a time, and each XCD then takes a contiguous run of them, which
+// keeps a coherent buffer's neighbouring rays in one L2. (Identity unless the grid is a multiple of 8.)
+RT_DEV uint32_t refill_slot(uint32_t block, uint32_t wave_in_block, uint32_t waves_per_block, uint32_t n_blocks) {
+    if (n_blocks & 7u) return block * waves_per_block + wave_in_block;
+    return ((block & 7u) * (n_blocks >> 3) + (block >> 3)) * waves_per_block + wave_in_block;
+}
+
 template <bool ANY, int MODE, typename Load, typename Finish>
 RT_DEV void refill_walks(const SceneArgs& sc, uint64_t n, uint32_t n_waves, uint32_t wid, int32_t* stk,
                          int stk_stride, int lane_stack, uint32_t* list, float4* pk, Load&& load,
