@@ -7,8 +7,8 @@ the same request traces here:
 - `RenderServer`: continuous batching. While batches run on the GPU the loop admits arrivals,
   makes their scenes resident (building the BVH on a miss, evicting least-recently-used scenes
   that no in-flight batch uses) and launches the requests admitted since the previous launch (at
-  most `max_batch`, grouped by scene) as one rt_render_batch launch (a lone request as one
-  rt_render, in arrival order on the first stream), on the least busy of `n_streams`
+  most `max_batch`, grouped by scene) as one rt_render_batch launch (up to `plain_max` requests as
+  rt_render calls, in arrival order on the first stream), on the least busy of `n_streams`
   equal-priority streams: batches on different streams overlap on the device (one batch's tail
   tiles share the CUs with the next batch's bulk), and a new batch starts as soon as a slot (at
   most `max_inflight` batches in flight) frees up.
@@ -177,7 +177,7 @@ class RenderServer:
     """Serves Requests in arrival order by continuous batching (see the module docstring)."""
 
     def __init__(self, store: SceneStore, config: BatchConfig = None, max_batch: int = 128,
-                 max_inflight: int = 4, n_streams: int = 4):
+                 max_inflight: int = 4, n_streams: int = 4, plain_max: int = 4):
         self.store = store
         # the best launch shape on the saturated serving workload (500 requests over the 64-scene
         # pool): 16 waves per CU, 4 tiles per grab, per-lane primary and shadow walks with refill
@@ -186,6 +186,10 @@ class RenderServer:
         # the device (saturated serving workload: 12.9k requests/s at 128, 12.3k at 256, 6.8k as
         # one batch of 1000, whose job table the host must build before anything starts)
         self.max_batch = max_batch
+        # launches of at most plain_max requests run as rt_render calls in arrival order on the
+        # first stream, as the plain loop runs them: at 500-1000 requests/s, batch-kernel launches
+        # of 2-4 requests raised the p99 by ~1 ms over plain, these keep it equal
+        self.plain_max = plain_max
         self.max_inflight = max(1, min(max_inflight, n_streams))
         # equal (default) priority streams: batches on different streams overlap on the device; one
         # reusable job table and workspace per stream (at most one batch per stream in flight)
@@ -212,29 +216,36 @@ class RenderServer:
                 break
         else:
             rest = []
-        # the batch's jobs grouped by scene (larger images first): tiles of one scene follow each
-        # other in the batch's tile order, so its hierarchy stays in the caches
-        order = sorted(range(len(reqs)), key=lambda i: (reqs[i].scene, -reqs[i].size))
-        reqs = [reqs[i] for i in order]
-        residents = [residents[i] for i in order]
+        small = len(reqs) <= self.plain_max
+        if not small:
+            # the batch's jobs grouped by scene (larger images first): tiles of one scene follow
+            # each other in the batch's tile order, so its hierarchy stays in the caches
+            order = sorted(range(len(reqs)), key=lambda i: (reqs[i].scene, -reqs[i].size))
+            reqs = [reqs[i] for i in order]
+            residents = [residents[i] for i in order]
         rr = [RenderRequest(res.raycaster, camera_for(r.eye), r.size, r.size, textures=res.binding)
               for r, res in zip(reqs, residents)]
         # scenes built on the current stream are ready before the batch starts
         stream.wait_stream(torch.cuda.current_stream())
         start = torch.cuda.Event(enable_timing=True)
         start.record(stream)
-        if len(rr) == 1:
-            # a batch of one is one rt_render: the persistent batch kernel would only add its
-            # prepare pass and job lookup to the same work
-            r = rr[0]
+        events = None
+        if small:
+            # a few requests: one rt_render each, in arrival order (the batch kernel pays off only
+            # for larger batches); each request completes at its own event
+            images, events = [], []
             with torch.cuda.stream(stream):
-                images = [r.raycaster.render(r.camera, r.width, r.height, textures=r.textures)]
+                for r in rr:
+                    images.append(r.raycaster.render(r.camera, r.width, r.height, textures=r.textures))
+                    events.append(torch.cuda.Event(enable_timing=True))
+                    events[-1].record(stream)
+            ev = events[-1]
         else:
             images = self.slots[id(stream)].launch(rr, self.config)
-        ev = torch.cuda.Event(enable_timing=True)
-        ev.record(stream)
-        return {"reqs": reqs, "images": images, "event": ev, "start": start, "residents": residents,
-                "t0": t0_event, "stream": stream}, rest
+            ev = torch.cuda.Event(enable_timing=True)
+            ev.record(stream)
+        return {"reqs": reqs, "images": images, "event": ev, "events": events, "start": start,
+                "residents": residents, "t0": t0_event, "stream": stream}, rest
 
     def _finish(self, f):
         done_s = f["t0"].elapsed_time(f["event"]) * 1e-3
@@ -242,6 +253,8 @@ class RenderServer:
         k = len(self.batches)
         self.batch_log.append((len(f["reqs"]), start_s, done_s))
         for i, r in enumerate(f["reqs"]):
+            if f["events"] is not None:
+                done_s = f["t0"].elapsed_time(f["events"][i]) * 1e-3
             r.latency = done_s - (r.arrival if self._realtime else 0.0)
             r.service = done_s - start_s
             r.batch = k
@@ -271,10 +284,10 @@ class RenderServer:
                 self._finish(f)
             if queue and len(inflight) < self.max_inflight:
                 n = min(len(queue), self.max_batch)
-                # lone requests keep to the first stream, in arrival order (as the plain loop runs
+                # small launches keep to the first stream, in arrival order (as the plain loop runs
                 # them), so they do not share the device with each other; batches take the least
                 # busy stream
-                stream = self.streams[0] if n == 1 else self._pick_stream(inflight)
+                stream = self.streams[0] if n <= self.plain_max else self._pick_stream(inflight)
                 rec, rest = self._launch([queue.popleft() for _ in range(n)], t0_event, stream)
                 queue.extendleft(reversed(rest))
                 if rec is not None:
