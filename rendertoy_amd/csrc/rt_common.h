@@ -50,6 +50,16 @@ struct Hit {
     int32_t prim, slot;
 };
 
+// A pointer read from device memory (a job table, a texture table) is a generic pointer, and its
+// loads and stores compile to flat instructions, which the compiler must wait for as both vector-
+// memory and LDS/scalar operations. Every such pointer here addresses global memory: rebuilding it
+// as an address-space-1 pointer from its integer value lets the loads and stores through it be
+// global ones (a cast from generic and back would fold away).
+template <typename T>
+RT_DEV T* as_global(T* p) {
+    return (T*)(__attribute__((address_space(1))) T*)(uintptr_t)p;
+}
+
 RT_DEV float safe_inv(float d) {
     float dd = fabsf(d) < 0x1p-80f ? copysignf(0x1p-80f, d) : d;
     return 1.0f / dd;

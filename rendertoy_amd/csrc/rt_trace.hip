@@ -240,7 +240,7 @@ RT_DEV int wrap_i(int i, int n) {
 }
 
 RT_DEV float4 texel(const uint8_t* __restrict__ tex, int W, int x, int y) {
-    const uint32_t p = *(const uint32_t*)(tex + ((uint64_t)y * W + x) * 4);
+    const uint32_t p = *as_global((const uint32_t*)(tex + ((uint64_t)y * W + x) * 4));
     return make_float4((float)(p & 0xFF), (float)((p >> 8) & 0xFF), (float)((p >> 16) & 0xFF),
                        (float)(p >> 24));
 }
@@ -314,9 +314,9 @@ RT_DEV Ray shadow_ray(const rt_render_params& p, const Ray& r, const Hit& h) {
 // Albedo of a hit on mesh `mesh` at (tu, tv): its texture's sample, or 1 without one.
 RT_DEV void albedo(const rt_render_params& p, int32_t mesh, float tu, float tv, float* alb) {
     if (!p.mesh_texture_dev) return;
-    const int32_t ti = p.mesh_texture_dev[mesh];
+    const int32_t ti = as_global(p.mesh_texture_dev)[mesh];
     if (ti >= 0 && (uint32_t)ti < p.n_textures)
-        tex_sample(p.textures_dev[ti], tu, tv, (p.flags & RT_RENDER_BILINEAR) != 0, alb);
+        tex_sample(as_global(p.textures_dev)[ti], tu, tv, (p.flags & RT_RENDER_BILINEAR) != 0, alb);
 }
 
 // Shading of one pixel per lane: primary closest hit, FP64 refinement, Lambert with interpolated
@@ -461,7 +461,7 @@ __global__ void __launch_bounds__(1024) k_batch_prepare(const rt_render_job* __r
         cnt[k] = 0;
         if (j < n_jobs) {
             cnt[k] = job_tiles(jobs[j], unit, bad);
-            uint32_t* st = jobs[j].out.stats;
+            uint32_t* st = as_global(jobs[j].out.stats);
             if (st) {
                 st[0] = 0;
                 st[1] = 0;
@@ -495,6 +495,36 @@ __global__ void __launch_bounds__(1024) k_batch_prepare(const rt_render_job* __r
         ws[0] = 0;
         ws[1] = status;
     }
+}
+
+// A job's scene arrays, parameters and outputs with their pointers known to be global (as_global).
+RT_DEV SceneArgs job_scene(const rt_render_job& j) {
+    SceneArgs sc;
+    sc.nodes = as_global((const float4*)j.scene.nodes);
+    sc.nodes4 = as_global((const float4*)j.scene.nodes4);
+    sc.tris = as_global((const float4*)j.scene.tris);
+    sc.shade = as_global((const float4*)j.scene.shade);
+    sc.mesh_offsets = as_global(j.scene.mesh_offsets);
+    sc.prim_slot = as_global(j.scene.prim_slot);
+    sc.root = j.scene.root;
+    sc.n_meshes = j.scene.n_meshes;
+    return sc;
+}
+
+RT_DEV rt_render_params job_params(const rt_render_job& j) {
+    rt_render_params p = j.params;
+    p.textures_dev = as_global(p.textures_dev);
+    p.mesh_texture_dev = as_global(p.mesh_texture_dev);
+    return p;
+}
+
+RT_DEV rt_render_out job_out(const rt_render_job& j) {
+    rt_render_out o = j.out;
+    o.rgb = as_global(o.rgb);
+    o.t = as_global(o.t);
+    o.prim = as_global(o.prim);
+    o.stats = as_global(o.stats);
+    return o;
 }
 
 // Persistent batched render: one wave per block; each wave pops `grab` consecutive 8x8 tiles of the
@@ -537,18 +567,10 @@ __global__ void __launch_bounds__(64) k_render_batch(const rt_render_job* __rest
             px.col = (local % tiles_x) * unit + dx;
             px.row = (local / tiles_x) * unit + dy;
             px.valid = px.col < p.width && px.row < p.height;
-            SceneArgs sc;
-            sc.nodes = (const float4*)job.scene.nodes;
-            sc.nodes4 = (const float4*)job.scene.nodes4;
-            sc.tris = (const float4*)job.scene.tris;
-            sc.shade = (const float4*)job.scene.shade;
-            sc.mesh_offsets = job.scene.mesh_offsets;
-            sc.prim_slot = job.scene.prim_slot;
-            sc.root = job.scene.root;
-            sc.n_meshes = job.scene.n_meshes;
-            render_pixel<PM, SM>(sc, job.cam, p, px, job.out.rgb, job.out.t, job.out.prim,
-                                 job.out.stats, pk, stk + lane, 64, lane_stack, budget,
-                                 coherent_tiles(job.cam, p.width, p.height));
+            const SceneArgs sc = job_scene(job);
+            const rt_render_out o = job_out(job);
+            render_pixel<PM, SM>(sc, job.cam, job_params(job), px, o.rgb, o.t, o.prim, o.stats, pk,
+                                 stk + lane, 64, lane_stack, budget, coherent_tiles(job.cam, p.width, p.height));
         }
     }
 }
@@ -641,17 +663,10 @@ __global__ void __launch_bounds__(64) k_render_batch_refill(const rt_render_job*
                 flush();
                 job = tile_job;
                 const rt_render_job& jb = jobs[job];
-                sc.nodes = (const float4*)jb.scene.nodes;
-                sc.nodes4 = (const float4*)jb.scene.nodes4;
-                sc.tris = (const float4*)jb.scene.tris;
-                sc.shade = (const float4*)jb.scene.shade;
-                sc.mesh_offsets = jb.scene.mesh_offsets;
-                sc.prim_slot = jb.scene.prim_slot;
-                sc.root = jb.scene.root;
-                sc.n_meshes = jb.scene.n_meshes;
+                sc = job_scene(jb);
                 cam = jb.cam;
-                p = jb.params;
-                out = jb.out;
+                p = job_params(jb);
+                out = job_out(jb);
                 tiles_x = (p.width + 7u) / 8u;
                 job_first = pref[job];
             }
