@@ -156,11 +156,14 @@ RT_DEV bool test_leaf(const float4* __restrict__ tris, int32_t ref, const Ray& r
 }
 
 // Resumable state of a per-lane walk (traverse_bin / traverse_lane4): the ray's slab terms, the
-// node it visits next and its stack depth (the entries live in the caller's LDS stack).
+// node it visits next (RT_EMPTY_ROOT: none left), its stack depth (the entries live in the caller's
+// LDS stack) and the leaf it has parked for the wave's next leaf round (lane4_steps; RT_EMPTY_ROOT:
+// none).
 struct LaneWalk {
     float idx, idy, idz, oxx, oyy, ozz;
     int32_t cur;
     int sp;
+    int32_t leaf;
 };
 
 RT_DEV void walk_start(const Ray& r, int32_t root, LaneWalk& w) {
@@ -172,6 +175,7 @@ RT_DEV void walk_start(const Ray& r, int32_t root, LaneWalk& w) {
     w.ozz = r.oz * w.idz;
     w.cur = root;
     w.sp = 0;
+    w.leaf = RT_EMPTY_ROOT;
 }
 
 // What a bounded run of a per-lane walk ended with.
@@ -734,12 +738,97 @@ RT_DEV void traverse_quad(const float4* __restrict__ nodes4, const float4* __res
 
 constexpr int kLaneStack = RT_LANE_STACK_MAX;
 
+// Leaf rounds of the per-lane walk (lane4_steps' PARK): the lanes of a wave test their parked
+// leaves together once at least PARK quarters of the walking lanes hold one, or when no lane has a
+// node left to visit. PARK = 0: no parking (each iteration visits one node or tests one leaf per
+// lane). The defaults per caller:
+#ifndef RT_PARK_RENDER
+#define RT_PARK_RENDER 2  // render and batch kernels (primary and shadow walks)
+#endif
+#ifndef RT_PARK_QUERY
+#define RT_PARK_QUERY 2   // rt_trace_closest
+#endif
+#ifndef RT_PARK_QUERY_ANY
+#define RT_PARK_QUERY_ANY 2  // rt_trace_any
+#endif
+
 // At most `steps` iterations of the 4-wide walk from state w (returns as bin_steps). `any_rt`: the
 // walk ends at the first hit, as with ANY (a per-lane choice).
-template <bool ANY>
+// PARK > 0: lanes of a wave sit at inner nodes and at leaves in different iterations, and a wave
+// issues the node code and the leaf code (its costlier part) whenever any lane needs either. So a
+// lane that reaches a leaf parks it and goes on with its stack, and the leaf code runs once per leaf
+// round for all parked leaves (speculative traversal, Aila & Laine 2009). The closest hit does not
+// depend on the visiting order (a node is skipped only past the best hit so far, which a later leaf
+// can only lower), so the hits are those of the one-node-or-leaf walk. Parking costs ~3 VGPRs.
+template <bool ANY, int PARK = RT_PARK_RENDER>
 RT_DEV int lane4_steps(const float4* __restrict__ nodes4, const float4* __restrict__ tris, const Ray& r,
                        int32_t skip_slot, Hit& h, LaneWalk& w, int32_t* stk, int stride, int limit,
                        int steps, bool any_rt = false) {
+    static_assert(PARK >= 0 && PARK <= 4, "quarters of the walking lanes");
+    if constexpr (PARK > 0) {
+    for (int s = 0; s < steps; ++s) {
+        if (w.cur >= 0 && w.cur != RT_EMPTY_ROOT) {
+            const float4* nd = nodes4 + (size_t)w.cur * 8;
+            const float4 lx = nd[0], hx = nd[1], ly = nd[2], hy = nd[3], lz = nd[4], hz = nd[5];
+            const float4 rf = nd[6];
+            const float clx[4] = {lx.x, lx.y, lx.z, lx.w}, chx[4] = {hx.x, hx.y, hx.z, hx.w};
+            const float cly[4] = {ly.x, ly.y, ly.z, ly.w}, chy[4] = {hy.x, hy.y, hy.z, hy.w};
+            const float clz[4] = {lz.x, lz.y, lz.z, lz.w}, chz[4] = {hz.x, hz.y, hz.z, hz.w};
+            const int32_t ref[4] = {__float_as_int(rf.x), __float_as_int(rf.y), __float_as_int(rf.z),
+                                    __float_as_int(rf.w)};
+            float key[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const float x0 = fmaf(clx[k], w.idx, -w.oxx), x1 = fmaf(chx[k], w.idx, -w.oxx);
+                const float y0 = fmaf(cly[k], w.idy, -w.oyy), y1 = fmaf(chy[k], w.idy, -w.oyy);
+                const float z0 = fmaf(clz[k], w.idz, -w.ozz), z1 = fmaf(chz[k], w.idz, -w.ozz);
+                const float tn = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), r.tmin));
+                const float tf = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fmaxf(z0, z1)) * kTfarScale;
+                const bool hk = (ref[k] != RT_EMPTY_ROOT) & (tn <= fminf(tf, h.t));
+                key[k] = hk ? tn : INFINITY;
+            }
+            float k0 = key[0], k1 = key[1], k2 = key[2], k3 = key[3];
+            int32_t r0 = ref[0], r1 = ref[1], r2 = ref[2], r3 = ref[3];
+            int i0 = 0, i1 = 1, i2 = 2, i3 = 3;
+#define RT_LSWAP(ka, ra, ia, kb, rb, ib)                                     \
+    if (kb < ka || (kb == ka && ib < ia)) {                                  \
+        const float tk = ka; ka = kb; kb = tk;                               \
+        const int32_t tr = ra; ra = rb; rb = tr;                             \
+        const int ti = ia; ia = ib; ib = ti;                                 \
+    }
+            RT_LSWAP(k0, r0, i0, k1, r1, i1)
+            RT_LSWAP(k2, r2, i2, k3, r3, i3)
+            RT_LSWAP(k0, r0, i0, k2, r2, i2)
+            RT_LSWAP(k1, r1, i1, k3, r3, i3)
+            RT_LSWAP(k1, r1, i1, k2, r2, i2)
+#undef RT_LSWAP
+            const int nhit = (k0 < INFINITY) + (k1 < INFINITY) + (k2 < INFINITY) + (k3 < INFINITY);
+            if (nhit > 0) {
+                if (w.sp + nhit - 1 > limit) return kWalkOverflow;
+                if (nhit >= 4) stk[(w.sp++) * stride] = r3;
+                if (nhit >= 3) stk[(w.sp++) * stride] = r2;
+                if (nhit >= 2) stk[(w.sp++) * stride] = r1;
+                w.cur = r0;
+            } else {
+                w.cur = w.sp ? stk[(--w.sp) * stride] : RT_EMPTY_ROOT;
+            }
+        } else if (w.cur < 0 && w.leaf == RT_EMPTY_ROOT) {  // park the leaf, go on with the stack
+            w.leaf = w.cur;
+            w.cur = w.sp ? stk[(--w.sp) * stride] : RT_EMPTY_ROOT;
+        }
+        const uint64_t parked = __ballot(w.leaf != RT_EMPTY_ROOT);
+        const uint64_t nodes = __ballot(w.cur >= 0 && w.cur != RT_EMPTY_ROOT);
+        if (parked != 0 && (nodes == 0 || 4 * __popcll(parked) >= PARK * __popcll(__ballot(true)))) {
+            if (w.leaf != RT_EMPTY_ROOT) {
+                const bool changed = test_leaf(tris, w.leaf, r, true, skip_slot, h);
+                w.leaf = RT_EMPTY_ROOT;
+                if (changed && (ANY || any_rt)) return kWalkDone;
+            }
+        }
+        if (w.cur == RT_EMPTY_ROOT && w.leaf == RT_EMPTY_ROOT) return kWalkDone;
+    }
+    return kWalkOn;
+    } else {
     for (int s = 0; s < steps; ++s) {
         if (w.cur >= 0) {
             const float4* nd = nodes4 + (size_t)w.cur * 8;
@@ -794,6 +883,7 @@ RT_DEV int lane4_steps(const float4* __restrict__ nodes4, const float4* __restri
         w.cur = stk[(--w.sp) * stride];
     }
     return kWalkOn;
+    }
 }
 
 // Per-lane traversal of the 4-wide nodes: every lane walks its own ray (no lane waits on another's
@@ -817,7 +907,8 @@ RT_DEV bool traverse_lane4(const float4* __restrict__ nodes4, const float4* __re
     if (root == RT_EMPTY_ROOT) return true;
     LaneWalk w;
     walk_start(r, root, w);
-    return lane4_steps<ANY>(nodes4, tris, r, skip_slot, h, w, stk, stride, limit, 0x7FFFFFFF) != kWalkOverflow;
+    return lane4_steps<ANY, RT_PARK_RENDER>(nodes4, tris, r, skip_slot, h, w, stk, stride, limit, 0x7FFFFFFF) !=
+           kWalkOverflow;
 }
 
 // Primary ray direction of pixel (col,row) (rendering/_core.py:528-548 look_at/perspective,

@@ -201,7 +201,8 @@ __global__ void __launch_bounds__(64) k_paths_bounce(SceneArgs sc, PathArgs a, u
     __shared__ uint32_t ovf[kRefillList];
     const uint32_t n = a.counts[k];
     float4* qi = a.q[(k - 1) & 1];
-    refill_walks<false, RT_TRAVERSAL_LANE>(
+    // no parked leaves: their registers take this kernel past 128 VGPRs (3 waves per SIMD, not 4)
+    refill_walks<false, RT_TRAVERSAL_LANE, 0>(
         sc, n, gridDim.x, refill_slot(blockIdx.x, 0, 1, gridDim.x), stk + threadIdx.x, 64, lane_stack, ovf, pk,
         [&](uint32_t i, Ray& r) -> int32_t {
             const float4 e0 = qi[(size_t)i * 3], e1 = qi[(size_t)i * 3 + 1], e2 = qi[(size_t)i * 3 + 2];

@@ -100,7 +100,7 @@ __global__ void __launch_bounds__(256) k_query(SceneArgs sc, rt_rays rays, uint3
     __shared__ float4 pk[4][kPacketLds];
     __shared__ uint32_t ovf[4][kRefillList];
     const uint32_t wave = threadIdx.x >> 6;
-    refill_walks<ANY, MODE>(
+    refill_walks<ANY, MODE, ANY ? RT_PARK_QUERY_ANY : RT_PARK_QUERY>(
         sc, n, n_waves, refill_slot(blockIdx.x, wave, 4, gridDim.x), qstk + threadIdx.x, 256, lane_stack, ovf[wave], pk[wave],
         [&](uint32_t i, Ray& r) -> int32_t {
             r = load_ray(rays, i);
@@ -708,7 +708,7 @@ __global__ void __launch_bounds__(64) k_render_batch_refill(const rt_render_job*
             if (sc.root == RT_EMPTY_ROOT)
                 st = kWalkDone;
             else
-                st = lane4_steps<false>(sc.nodes4, sc.tris, r, skip, h, w, stk, 64, lane_stack, kRefillSteps,
+                st = lane4_steps<false, RT_PARK_RENDER>(sc.nodes4, sc.tris, r, skip, h, w, stk, 64, lane_stack, kRefillSteps,
                                         phase == 1u);
         }
         if (__ballot(st == kWalkOverflow)) {  // rare: the whole wave traces those rays as a packet

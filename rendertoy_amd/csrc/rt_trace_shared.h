@@ -78,8 +78,8 @@ constexpr bool uses_lds(int mode) { return mode != RT_TRAVERSAL_PACKET; }
 // - the wave walks in rounds of at most kRefillSteps iterations (one node or one leaf) per lane,
 //   and whenever kRefillIdle or more lanes are idle, the idle lanes take the wave's next rays in
 //   lane order (one ballot);
-// - a lane's walk (bin_steps or lane4_steps, MODE) is the one-ray walk resumed across rounds, so
-//   every ray gets the hit it gets alone;
+// - a lane's walk (bin_steps or lane4_steps, MODE; PARK: lane4_steps' leaf rounds) is the one-ray
+//   walk resumed across rounds, so every ray gets the hit it gets alone;
 // - a ray whose LDS stack would overflow is listed in LDS (`list`, kRefillList entries); when the
 //   list could fill, the wave stops taking rays, finishes those it holds and traces the listed ones
 //   again with the packet traversal, 64 at a time, then goes on.
@@ -109,7 +109,7 @@ RT_DEV uint32_t refill_slot(uint32_t block, uint32_t wave_in_block, uint32_t wav
     return ((block & 7u) * (n_blocks >> 3) + (block >> 3)) * waves_per_block + wave_in_block;
 }
 
-template <bool ANY, int MODE, typename Load, typename Finish>
+template <bool ANY, int MODE, int PARK, typename Load, typename Finish>
 RT_DEV void refill_walks(const SceneArgs& sc, uint64_t n, uint32_t n_waves, uint32_t wid, int32_t* stk,
                          int stk_stride, int lane_stack, uint32_t* list, float4* pk, Load&& load,
                          Finish&& finish) {
@@ -162,7 +162,7 @@ RT_DEV void refill_walks(const SceneArgs& sc, uint64_t n, uint32_t n_waves, uint
                 else if (MODE == RT_TRAVERSAL_BINARY)
                     st = bin_steps<ANY>(sc.nodes, sc.tris, r, skip, h, w, stk, stk_stride, lane_stack, kRefillSteps);
                 else
-                    st = lane4_steps<ANY>(sc.nodes4, sc.tris, r, skip, h, w, stk, stk_stride, lane_stack,
+                    st = lane4_steps<ANY, PARK>(sc.nodes4, sc.tris, r, skip, h, w, stk, stk_stride, lane_stack,
                                           kRefillSteps);
             }
             finish(st == kWalkDone, ri, r, h);
