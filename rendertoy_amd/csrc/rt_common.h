@@ -752,6 +752,51 @@ constexpr int kLaneStack = RT_LANE_STACK_MAX;
 #define RT_PARK_QUERY_ANY 2  // rt_trace_any
 #endif
 
+// The children of 4-wide node `cur` whose boxes ray r (slab terms in w) enters no later than `best`:
+// returns how many, with their refs in c[0..n-1] nearest entry first (equal entry distances in any
+// order: the visiting order decides only how soon the best hit shrinks, never which hit is found).
+RT_DEV int lane4_children(const float4* __restrict__ nodes4, int32_t cur, const Ray& r, const LaneWalk& w,
+                          float best, int32_t c[4]) {
+    const float4* nd = nodes4 + (size_t)cur * 8;
+    const float4 lx = nd[0], hx = nd[1], ly = nd[2], hy = nd[3], lz = nd[4], hz = nd[5];
+    const float4 rf = nd[6];
+    const float clx[4] = {lx.x, lx.y, lx.z, lx.w}, chx[4] = {hx.x, hx.y, hx.z, hx.w};
+    const float cly[4] = {ly.x, ly.y, ly.z, ly.w}, chy[4] = {hy.x, hy.y, hy.z, hy.w};
+    const float clz[4] = {lz.x, lz.y, lz.z, lz.w}, chz[4] = {hz.x, hz.y, hz.z, hz.w};
+    int32_t ref[4] = {__float_as_int(rf.x), __float_as_int(rf.y), __float_as_int(rf.z), __float_as_int(rf.w)};
+    float key[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const float x0 = fmaf(clx[k], w.idx, -w.oxx), x1 = fmaf(chx[k], w.idx, -w.oxx);
+        const float y0 = fmaf(cly[k], w.idy, -w.oyy), y1 = fmaf(chy[k], w.idy, -w.oyy);
+        const float z0 = fmaf(clz[k], w.idz, -w.ozz), z1 = fmaf(chz[k], w.idz, -w.ozz);
+        const float tn = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), r.tmin));
+        const float tf = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fmaxf(z0, z1)) * kTfarScale;
+        const bool hk = (ref[k] != RT_EMPTY_ROOT) & (tn <= fminf(tf, best));
+        key[k] = hk ? tn : INFINITY;
+    }
+    // 4-element sorting network over (key, ref): missed children (key +inf) sort last
+#define RT_CSWAP(a, b)                                              \
+    {                                                               \
+        const bool sw = key[b] < key[a];                            \
+        const float ka = key[a], kb = key[b];                       \
+        const int32_t ra = ref[a], rb = ref[b];                     \
+        key[a] = sw ? kb : ka;                                      \
+        key[b] = sw ? ka : kb;                                      \
+        ref[a] = sw ? rb : ra;                                      \
+        ref[b] = sw ? ra : rb;                                      \
+    }
+    RT_CSWAP(0, 1)
+    RT_CSWAP(2, 3)
+    RT_CSWAP(0, 2)
+    RT_CSWAP(1, 3)
+    RT_CSWAP(1, 2)
+#undef RT_CSWAP
+#pragma unroll
+    for (int k = 0; k < 4; ++k) c[k] = ref[k];
+    return (key[0] < INFINITY) + (key[1] < INFINITY) + (key[2] < INFINITY) + (key[3] < INFINITY);
+}
+
 // At most `steps` iterations of the 4-wide walk from state w (returns as bin_steps). `any_rt`: the
 // walk ends at the first hit, as with ANY (a per-lane choice).
 // PARK > 0: lanes of a wave sit at inner nodes and at leaves in different iterations, and a wave
@@ -765,125 +810,55 @@ RT_DEV int lane4_steps(const float4* __restrict__ nodes4, const float4* __restri
                        int32_t skip_slot, Hit& h, LaneWalk& w, int32_t* stk, int stride, int limit,
                        int steps, bool any_rt = false) {
     static_assert(PARK >= 0 && PARK <= 4, "quarters of the walking lanes");
-    if constexpr (PARK > 0) {
     for (int s = 0; s < steps; ++s) {
-        if (w.cur >= 0 && w.cur != RT_EMPTY_ROOT) {
-            const float4* nd = nodes4 + (size_t)w.cur * 8;
-            const float4 lx = nd[0], hx = nd[1], ly = nd[2], hy = nd[3], lz = nd[4], hz = nd[5];
-            const float4 rf = nd[6];
-            const float clx[4] = {lx.x, lx.y, lx.z, lx.w}, chx[4] = {hx.x, hx.y, hx.z, hx.w};
-            const float cly[4] = {ly.x, ly.y, ly.z, ly.w}, chy[4] = {hy.x, hy.y, hy.z, hy.w};
-            const float clz[4] = {lz.x, lz.y, lz.z, lz.w}, chz[4] = {hz.x, hz.y, hz.z, hz.w};
-            const int32_t ref[4] = {__float_as_int(rf.x), __float_as_int(rf.y), __float_as_int(rf.z),
-                                    __float_as_int(rf.w)};
-            float key[4];
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                const float x0 = fmaf(clx[k], w.idx, -w.oxx), x1 = fmaf(chx[k], w.idx, -w.oxx);
-                const float y0 = fmaf(cly[k], w.idy, -w.oyy), y1 = fmaf(chy[k], w.idy, -w.oyy);
-                const float z0 = fmaf(clz[k], w.idz, -w.ozz), z1 = fmaf(chz[k], w.idz, -w.ozz);
-                const float tn = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), r.tmin));
-                const float tf = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fmaxf(z0, z1)) * kTfarScale;
-                const bool hk = (ref[k] != RT_EMPTY_ROOT) & (tn <= fminf(tf, h.t));
-                key[k] = hk ? tn : INFINITY;
-            }
-            float k0 = key[0], k1 = key[1], k2 = key[2], k3 = key[3];
-            int32_t r0 = ref[0], r1 = ref[1], r2 = ref[2], r3 = ref[3];
-            int i0 = 0, i1 = 1, i2 = 2, i3 = 3;
-#define RT_LSWAP(ka, ra, ia, kb, rb, ib)                                     \
-    if (kb < ka || (kb == ka && ib < ia)) {                                  \
-        const float tk = ka; ka = kb; kb = tk;                               \
-        const int32_t tr = ra; ra = rb; rb = tr;                             \
-        const int ti = ia; ia = ib; ib = ti;                                 \
-    }
-            RT_LSWAP(k0, r0, i0, k1, r1, i1)
-            RT_LSWAP(k2, r2, i2, k3, r3, i3)
-            RT_LSWAP(k0, r0, i0, k2, r2, i2)
-            RT_LSWAP(k1, r1, i1, k3, r3, i3)
-            RT_LSWAP(k1, r1, i1, k2, r2, i2)
-#undef RT_LSWAP
-            const int nhit = (k0 < INFINITY) + (k1 < INFINITY) + (k2 < INFINITY) + (k3 < INFINITY);
-            if (nhit > 0) {
-                if (w.sp + nhit - 1 > limit) return kWalkOverflow;
-                if (nhit >= 4) stk[(w.sp++) * stride] = r3;
-                if (nhit >= 3) stk[(w.sp++) * stride] = r2;
-                if (nhit >= 2) stk[(w.sp++) * stride] = r1;
-                w.cur = r0;
-            } else {
+        if constexpr (PARK > 0) {
+            if (w.cur >= 0 && w.cur != RT_EMPTY_ROOT) {
+                int32_t c[4];
+                const int nhit = lane4_children(nodes4, w.cur, r, w, h.t, c);
+                if (nhit > 0) {
+                    if (w.sp + nhit - 1 > limit) return kWalkOverflow;
+                    // push the farther children (farthest first), go on with the nearest
+                    if (nhit >= 4) stk[(w.sp++) * stride] = c[3];
+                    if (nhit >= 3) stk[(w.sp++) * stride] = c[2];
+                    if (nhit >= 2) stk[(w.sp++) * stride] = c[1];
+                    w.cur = c[0];
+                } else {
+                    w.cur = w.sp ? stk[(--w.sp) * stride] : RT_EMPTY_ROOT;
+                }
+            } else if (w.cur < 0 && w.leaf == RT_EMPTY_ROOT) {  // park the leaf, go on with the stack
+                w.leaf = w.cur;
                 w.cur = w.sp ? stk[(--w.sp) * stride] : RT_EMPTY_ROOT;
             }
-        } else if (w.cur < 0 && w.leaf == RT_EMPTY_ROOT) {  // park the leaf, go on with the stack
-            w.leaf = w.cur;
-            w.cur = w.sp ? stk[(--w.sp) * stride] : RT_EMPTY_ROOT;
-        }
-        const uint64_t parked = __ballot(w.leaf != RT_EMPTY_ROOT);
-        const uint64_t nodes = __ballot(w.cur >= 0 && w.cur != RT_EMPTY_ROOT);
-        if (parked != 0 && (nodes == 0 || 4 * __popcll(parked) >= PARK * __popcll(__ballot(true)))) {
-            if (w.leaf != RT_EMPTY_ROOT) {
-                const bool changed = test_leaf(tris, w.leaf, r, true, skip_slot, h);
-                w.leaf = RT_EMPTY_ROOT;
-                if (changed && (ANY || any_rt)) return kWalkDone;
+            const uint64_t parked = __ballot(w.leaf != RT_EMPTY_ROOT);
+            const uint64_t nodes = __ballot(w.cur >= 0 && w.cur != RT_EMPTY_ROOT);
+            if (parked != 0 && (nodes == 0 || 4 * __popcll(parked) >= PARK * __popcll(__ballot(true)))) {
+                if (w.leaf != RT_EMPTY_ROOT) {
+                    const bool changed = test_leaf(tris, w.leaf, r, true, skip_slot, h);
+                    w.leaf = RT_EMPTY_ROOT;
+                    if (changed && (ANY || any_rt)) return kWalkDone;
+                }
             }
-        }
-        if (w.cur == RT_EMPTY_ROOT && w.leaf == RT_EMPTY_ROOT) return kWalkDone;
-    }
-    return kWalkOn;
-    } else {
-    for (int s = 0; s < steps; ++s) {
-        if (w.cur >= 0) {
-            const float4* nd = nodes4 + (size_t)w.cur * 8;
-            const float4 lx = nd[0], hx = nd[1], ly = nd[2], hy = nd[3], lz = nd[4], hz = nd[5];
-            const float4 rf = nd[6];
-            const float clx[4] = {lx.x, lx.y, lx.z, lx.w}, chx[4] = {hx.x, hx.y, hx.z, hx.w};
-            const float cly[4] = {ly.x, ly.y, ly.z, ly.w}, chy[4] = {hy.x, hy.y, hy.z, hy.w};
-            const float clz[4] = {lz.x, lz.y, lz.z, lz.w}, chz[4] = {hz.x, hz.y, hz.z, hz.w};
-            const int32_t ref[4] = {__float_as_int(rf.x), __float_as_int(rf.y), __float_as_int(rf.z),
-                                    __float_as_int(rf.w)};
-            float key[4];
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                const float x0 = fmaf(clx[k], w.idx, -w.oxx), x1 = fmaf(chx[k], w.idx, -w.oxx);
-                const float y0 = fmaf(cly[k], w.idy, -w.oyy), y1 = fmaf(chy[k], w.idy, -w.oyy);
-                const float z0 = fmaf(clz[k], w.idz, -w.ozz), z1 = fmaf(chz[k], w.idz, -w.ozz);
-                const float tn = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), r.tmin));
-                const float tf = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fmaxf(z0, z1)) * kTfarScale;
-                const bool hk = (ref[k] != RT_EMPTY_ROOT) & (tn <= fminf(tf, h.t));
-                key[k] = hk ? tn : INFINITY;
-            }
-            // (key, k) order by a 4-element sorting network over (key, ref) pairs
-            float k0 = key[0], k1 = key[1], k2 = key[2], k3 = key[3];
-            int32_t r0 = ref[0], r1 = ref[1], r2 = ref[2], r3 = ref[3];
-            int i0 = 0, i1 = 1, i2 = 2, i3 = 3;
-#define RT_LSWAP(ka, ra, ia, kb, rb, ib)                                     \
-    if (kb < ka || (kb == ka && ib < ia)) {                                  \
-        const float tk = ka; ka = kb; kb = tk;                               \
-        const int32_t tr = ra; ra = rb; rb = tr;                             \
-        const int ti = ia; ia = ib; ib = ti;                                 \
-    }
-            RT_LSWAP(k0, r0, i0, k1, r1, i1)
-            RT_LSWAP(k2, r2, i2, k3, r3, i3)
-            RT_LSWAP(k0, r0, i0, k2, r2, i2)
-            RT_LSWAP(k1, r1, i1, k3, r3, i3)
-            RT_LSWAP(k1, r1, i1, k2, r2, i2)
-#undef RT_LSWAP
-            const int nhit = (k0 < INFINITY) + (k1 < INFINITY) + (k2 < INFINITY) + (k3 < INFINITY);
-            if (nhit > 0) {
-                if (w.sp + nhit - 1 > limit) return kWalkOverflow;
-                // push the farther hit children (farthest first), continue with the nearest
-                if (nhit >= 4) stk[(w.sp++) * stride] = r3;
-                if (nhit >= 3) stk[(w.sp++) * stride] = r2;
-                if (nhit >= 2) stk[(w.sp++) * stride] = r1;
-                w.cur = r0;
-                continue;
-            }
+            if (w.cur == RT_EMPTY_ROOT && w.leaf == RT_EMPTY_ROOT) return kWalkDone;
         } else {
-            if (test_leaf(tris, w.cur, r, true, skip_slot, h) && (ANY || any_rt)) return kWalkDone;
+            if (w.cur >= 0) {
+                int32_t c[4];
+                const int nhit = lane4_children(nodes4, w.cur, r, w, h.t, c);
+                if (nhit > 0) {
+                    if (w.sp + nhit - 1 > limit) return kWalkOverflow;
+                    if (nhit >= 4) stk[(w.sp++) * stride] = c[3];
+                    if (nhit >= 3) stk[(w.sp++) * stride] = c[2];
+                    if (nhit >= 2) stk[(w.sp++) * stride] = c[1];
+                    w.cur = c[0];
+                    continue;
+                }
+            } else {
+                if (test_leaf(tris, w.cur, r, true, skip_slot, h) && (ANY || any_rt)) return kWalkDone;
+            }
+            if (w.sp == 0) return kWalkDone;
+            w.cur = stk[(--w.sp) * stride];
         }
-        if (w.sp == 0) return kWalkDone;
-        w.cur = stk[(--w.sp) * stride];
     }
     return kWalkOn;
-    }
 }
 
 // Per-lane traversal of the 4-wide nodes: every lane walks its own ray (no lane waits on another's
