@@ -751,6 +751,9 @@ constexpr int kLaneStack = RT_LANE_STACK_MAX;
 #ifndef RT_PARK_QUERY_ANY
 #define RT_PARK_QUERY_ANY 2  // rt_trace_any
 #endif
+#ifndef RT_PARK_PATHS
+#define RT_PARK_PATHS 2   // rt_trace_paths' bounce rays
+#endif
 
 // The children of 4-wide node `cur` whose boxes ray r (slab terms in w) enters no later than `best`:
 // returns how many, with their refs in c[0..n-1] nearest entry first (equal entry distances in any

@@ -194,15 +194,16 @@ __global__ void __launch_bounds__(256) k_paths_primary(SceneArgs sc, rt_camera c
 // groups. Each finished ray records the triangle slot it hit (-1: none) and its FP32 t in its own
 // queue entry (words 1 and 2 of its third float4); k_paths_emit then shades the queue in order. Keeping the sampling
 // and emission out of the walk loop keeps the walk's registers few.
-__global__ void __launch_bounds__(64) k_paths_bounce(SceneArgs sc, PathArgs a, uint32_t k,
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_paths_bounce(SceneArgs sc, PathArgs a, uint32_t k,
                                                       int lane_stack) {
     __shared__ int32_t stk[kLaneStack * 64];
     __shared__ float4 pk[kPacketLds];
     __shared__ uint32_t ovf[kRefillList];
     const uint32_t n = a.counts[k];
     float4* qi = a.q[(k - 1) & 1];
-    // no parked leaves: their registers take this kernel past 128 VGPRs (3 waves per SIMD, not 4)
-    refill_walks<false, RT_TRAVERSAL_LANE, 0>(
+    // parked leaves (RT_PARK_PATHS) take ~3 more VGPRs: the waves-per-EU bound keeps the kernel at
+    // 128 (4 waves per SIMD) without scratch
+    refill_walks<false, RT_TRAVERSAL_LANE, RT_PARK_PATHS>(
         sc, n, gridDim.x, refill_slot(blockIdx.x, 0, 1, gridDim.x), stk + threadIdx.x, 64, lane_stack, ovf, pk,
         [&](uint32_t i, Ray& r) -> int32_t {
             const float4 e0 = qi[(size_t)i * 3], e1 = qi[(size_t)i * 3 + 1], e2 = qi[(size_t)i * 3 + 2];
