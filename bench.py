@@ -439,6 +439,21 @@ def accuracy_report(tw, meshes, scenes, oracle, cam, W, H, prim, t, img, light, 
 _T0 = time.time()
 
 
+def time_launches_stream(fn, steps, warmup, stream):
+    """Per-call milliseconds of fn() by HIP events recorded on `stream`, after `warmup` calls."""
+    import torch
+    for _ in range(warmup):
+        fn()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
+    torch.cuda.synchronize()
+    for a, b in ev:
+        a.record(stream)
+        fn()
+        b.record(stream)
+    torch.cuda.synchronize()
+    return [a.elapsed_time(b) for a, b in ev]
+
+
 def note(msg):
     """A progress line on stderr (stdout carries only the result line)."""
     print(f"[bench {time.time() - _T0:6.1f} s] {msg}", file=sys.stderr, flush=True)
@@ -537,6 +552,13 @@ def main():
     _, rhits = rc.render(cam, W, H, light=light, textures=binding, out=img, return_hits=True)
     r_prim, r_t = rhits["prim"].cpu().numpy(), rhits["t"].cpu().numpy()
     r_img = img.cpu().numpy()
+    # the same frame with per-lane primary walks (tuning primary="lane"): on this dense scene they
+    # beat the default packet primary; the image must be the same bits
+    lane_tu = {"primary": "lane", "shadow": "lane"}
+    img_lane = torch.empty_like(img)
+    lane_ms = time_launches_stream(lambda: rc.render(cam, W, H, light=light, textures=binding, out=img_lane,
+                                                     tuning=lane_tu), args.steps, args.warmup, stream)
+    lane_same = bool(torch.equal(img_lane.view(torch.int32), img.view(torch.int32)))
     # export of the frame to the reference's BGRA8 render-target format (12 B read, 4 B written per px)
     from rendertoy_amd import image as rt_image
     from rendertoy_amd.batch import time_launches
@@ -576,11 +598,13 @@ def main():
             "mrays_per_s_p50": round(render_rays / (percentile(render_ms, 50) * 1e-3) / 1e6, 2),
             "bgra8_export_ms": round(bgra_ms, 4),
             "bgra8_export_gbs": round(16 * n_rays / (bgra_ms * 1e-3) / 1e9, 1),
+            "lane_primary": {"tuning": lane_tu, "latency_ms_p50": round(percentile(lane_ms, 50), 4),
+                             "latency_ms_p99": round(percentile(lane_ms, 99), 4), "image_equal": lane_same},
         },
         "wall_s_timed": round(wall, 4),
     }
 
-    note(f"render: p50 {percentile(render_ms, 50):.4f} ms")
+    note(f"render: p50 {percentile(render_ms, 50):.4f} ms (lane primary {percentile(lane_ms, 50):.4f} ms)")
     if not args.no_batched:
         note("batched")
         result["batched"] = measure_batched(args, ren, scenes, torch)
