@@ -172,6 +172,42 @@ def test_ray_buffer_traversals_and_stack_overflow(ren, query, lane_stack):
     np.testing.assert_array_equal(g2["prim"].cpu().numpy(), r2["prim"])
 
 
+@pytest.mark.parametrize("n_tris", [1, 2, 4])
+def test_leaf_root_scenes_through_every_lane_walk(ren, n_tris):
+    """At most 4 triangles make one leaf at the root: a per-lane walk parks it at its first step
+    with no node left to visit. The queries (LANE and BINARY), the render under every primary
+    traversal and the batch refill kernel give the twin's hits and the same image."""
+    from rendertoy_amd.batch import BatchRender, BatchConfig, RenderRequest
+    rng = np.random.default_rng(700 + n_tris)
+    mesh = {"pos": rng.uniform(-0.6, 0.6, (3 * n_tris, 3)).astype(np.float32),
+            "idx": np.arange(3 * n_tris, dtype=np.int32).reshape(-1, 3), "nrm": None, "uv": None}
+    tw = twin_from_meshes([mesh])
+    rc = gpu_from_meshes([mesh])
+    assert rc.root < 0
+    o, d = random_rays(4000, 701 + n_tris)
+    ref = tw.trace_closest(o, d)
+    hit = ref["prim"] >= 0
+    assert hit.any() and not hit.all()
+    for q in ("lane", "binary"):
+        g = rc.trace_soa(soa_tensors(o, d), tuning={"query": q})
+        np.testing.assert_array_equal(g["prim"].cpu().numpy(), ref["prim"])
+        np.testing.assert_array_equal(_u32(g["t"].cpu().numpy()[hit]), _u32(ref["t"][hit]))
+        occ = rc.occluded(soa_tensors(o, d), tuning={"query": q}).cpu().numpy()
+        np.testing.assert_array_equal(occ, tw.trace_any(o, d))
+    W, H = 96, 64
+    cam = camera((0.2, 0.3, 2.5), aspect=W / H)
+    rgb, t, prim, st = tw.render(cam.as_array(), W, H, LIGHT)
+    assert (prim >= 0).any()
+    for primary in ("packet", "lane", "auto", "hybrid", "quad"):
+        img = rc.render(cam, W, H, light=LIGHT, tuning={"primary": primary, "shadow": "lane"})
+        np.testing.assert_array_equal(_u32(img.cpu().numpy()), _u32(rgb), err_msg=primary)
+    b = BatchRender([RenderRequest(rc, cam, W, H, light=LIGHT)] * 2, config=BatchConfig(16, 1, "lane", "lane"))
+    b.launch()
+    torch.cuda.synchronize()
+    for im in b.images:
+        np.testing.assert_array_equal(_u32(im.cpu().numpy()), _u32(rgb))
+
+
 @pytest.mark.parametrize("query,lane_stack", [("lane", 1), ("binary", 2), ("lane", 32)])
 def test_long_ray_buffers_refill_lanes_and_retrace_overflows(ren, query, lane_stack):
     """A buffer of 1,500,001 rays: more 64-ray groups than the GPU holds waves, so every wave refills
