@@ -485,26 +485,13 @@ RT_DEV uint64_t quad_mask_unpack(uint32_t cm) {
     return (uint64_t)(cm & 0x11111111u) | ((uint64_t)((cm >> 1) & 0x11111111u) << 32);
 }
 
-// What a budgeted quad walk leaves when it stops: `sp` stack entries held as described above, plus
-// the node it would visit next with its rays. The callback gets these and the lanes' candidates.
-struct QuadRest {
-    int32_t sp;
-    int32_t node0, node1;   // lane k: entries k and 64 + k
-    uint32_t m0, m1;
-    int32_t cur;            // next node (wave-uniform)
-    uint32_t cur_m;         // its packed ray bits
-};
-
 // The traversal proper: on return each lane's h is its own closest candidate (see traverse_quad).
 // The walk starts at node `start` with the rays of `m` (bit 4q = ray q) and the candidates already
-// in h (all four lanes of a quad hold the same one: a bound found elsewhere, or none). With
-// BUDGET, after `budget` node steps it stops, calls handoff(rest) with what is left, and returns
-// false; handoff may return false to refuse (the walk then continues unbudgeted); no kernel takes
-// the budgeted form at present. Called by all 64 lanes with uniform control flow.
-template <bool ANY, bool BUDGET, typename Handoff>
-RT_DEV bool traverse_quad_core(const float4* __restrict__ nodes4, const float4* __restrict__ tris,
-                               int32_t start, uint64_t m, const Ray& r, int32_t skip_slot, Hit& h,
-                               int budget, Handoff&& handoff) {
+// in h (all four lanes of a quad hold the same one: a bound found elsewhere, or none). Called by
+// all 64 lanes with uniform control flow.
+template <bool ANY>
+RT_DEV void traverse_quad_core(const float4* __restrict__ nodes4, const float4* __restrict__ tris,
+                               int32_t start, uint64_t m, const Ray& r, int32_t skip_slot, Hit& h) {
     const uint32_t lane = __lane_id(), c = lane & 3u, qb = lane & ~3u;
     uint64_t alive = m;
     const float idx = safe_inv(r.dx), idy = safe_inv(r.dy), idz = safe_inv(r.dz);
@@ -556,7 +543,7 @@ RT_DEV bool traverse_quad_core(const float4* __restrict__ nodes4, const float4* 
         const int32_t lref[4] = {start, 0, 0, 0};
         const uint64_t lmsk[4] = {m, 0, 0, 0};
         leaves(1u, lref, lmsk);
-        return true;
+        return;
     }
     int32_t stk_node0 = 0, stk_node1 = 0;
     uint32_t stk_m0 = 0, stk_m1 = 0;  // ray masks, bit 4q -> bit 4q (q < 8) / 4(q-8)+1
@@ -574,29 +561,21 @@ RT_DEV bool traverse_quad_core(const float4* __restrict__ nodes4, const float4* 
             if (mask != 0) return true;
         }
     };
-    // lane c's child of node `cur`: its box and ref (fields c of the SoA record)
-    int32_t cur = start;
+    // lane c's child of the node being visited: its box and ref (fields c of the SoA record)
     const float* nf = (const float*)(nodes4 + (size_t)start * 8);
     float clx = nf[c], chx = nf[4 + c], cly = nf[8 + c], chy = nf[12 + c], clz = nf[16 + c],
           chz = nf[20 + c];
     int32_t cref = __float_as_int(nf[24 + c]);
-    int steps = 0;
     for (;;) {
         if (ANY) {
             m &= alive;
             if (m == 0) {
                 int32_t node;
-                if (!pop(node, m)) return true;
-                cur = node;
+                if (!pop(node, m)) return;
                 nf = (const float*)(nodes4 + (size_t)node * 8);
                 clx = nf[c]; chx = nf[4 + c]; cly = nf[8 + c]; chy = nf[12 + c]; clz = nf[16 + c];
                 chz = nf[20 + c]; cref = __float_as_int(nf[24 + c]);
             }
-        }
-        if (BUDGET && ++steps > budget) {
-            QuadRest rest{sp, stk_node0, stk_node1, stk_m0, stk_m1, cur, quad_mask_pack(m)};
-            if (handoff(rest)) return false;
-            budget = 0x7FFFFFFF;  // refused: walk on to the end
         }
         const bool mine = (m >> qb) & 1u;
         const float x0 = fmaf(clx, idx, -oxx), x1 = fmaf(chx, idx, -oxx);
@@ -694,7 +673,6 @@ RT_DEV bool traverse_quad_core(const float4* __restrict__ nodes4, const float4* 
         bool more = next_m != 0;
         if (!more) more = pop(next, next_m);
         if (more) {  // the next node's fetch overlaps the leaf tests
-            cur = next;
             nf = (const float*)(nodes4 + (size_t)next * 8);
             clx = nf[c]; chx = nf[4 + c]; cly = nf[8 + c]; chy = nf[12 + c]; clz = nf[16 + c];
             chz = nf[20 + c]; cref = __float_as_int(nf[24 + c]);
@@ -703,10 +681,10 @@ RT_DEV bool traverse_quad_core(const float4* __restrict__ nodes4, const float4* 
             leaves(leafs, lref, lmsk);
             if (ANY) {
                 alive &= ~quad_any(__ballot(h.prim >= 0));
-                if (alive == 0) return true;
+                if (alive == 0) return;
             }
         }
-        if (!more) return true;
+        if (!more) return;
         m = next_m;
     }
 }
@@ -723,8 +701,7 @@ RT_DEV void traverse_quad_lanes(const float4* __restrict__ nodes4, const float4*
     h.slot = -1;
     const uint64_t m = __ballot(want) & kQuadBits;
     if (root == RT_EMPTY_ROOT || m == 0) return;
-    traverse_quad_core<ANY, false>(nodes4, tris, root, m, r, skip_slot, h, 0,
-                                   [](const QuadRest&) { return true; });
+    traverse_quad_core<ANY>(nodes4, tris, root, m, r, skip_slot, h);
 }
 
 // Quad-lane traversal of the 16 rays of a wave (lane 4q+c holds ray q); h = the ray's closest hit
