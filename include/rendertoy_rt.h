@@ -170,8 +170,8 @@ enum {
  * rt_set_tuning (so two callers on two streams never share mutable state). */
 typedef struct rt_tuning {
     int32_t primary;          /* RT_TRAVERSAL_* for camera rays (default QUAD) */
-    int32_t shadow;           /* RT_TRAVERSAL_* for rt_render's shadow rays (default LANE); not
-                                 QUAD, which is for camera rays */
+    int32_t shadow;           /* RT_TRAVERSAL_* for rt_render's shadow rays (default LANE; not used
+                                 with a QUAD primary); not QUAD, which is for camera rays */
     uint32_t super_tile;      /* blocks are mapped to XCDs in super-tiles of this many 16x16-pixel
                                  blocks per side; 0 = row-major block order (default) */
     uint32_t lane_stack;      /* per-lane stack entries, 1..RT_LANE_STACK_MAX (default 32); a ray
@@ -268,8 +268,10 @@ int rt_trace_camera(const rt_scene* scene, const rt_camera* cam, uint32_t width,
 /* Full render in one kernel: per 8x8 pixel tile, ray generation, closest-hit trace, FP64 hit
  * refinement, Lambert with interpolated normals, one shadow ray toward the light per lit hit,
  * RGBA8 texture fetch, RGB store. `workspace_dev` needs rt_render_workspace_bytes(width*height)
- * bytes (currently 0: NULL is accepted). The QUAD camera-ray traversal walks the primary rays with
- * the packet traversal here (faster inside this kernel). */
+ * bytes (currently 0: NULL is accepted). With the QUAD camera-ray traversal (the default) the
+ * primary and the shadow rays both take the quad-lane traversal (16-ray packets over 4x4 pixels, 4
+ * lanes per ray) and `tuning->shadow` is not used; with the others the primary rays take the
+ * chosen traversal and the shadow rays `tuning->shadow`. */
 uint64_t rt_render_workspace_bytes(uint32_t n_pixels);
 int rt_render(const rt_scene* scene, const rt_camera* cam, const rt_render_params* params,
               const rt_render_out* out, void* workspace_dev, uint64_t workspace_bytes,
