@@ -558,12 +558,21 @@ __global__ void __launch_bounds__(64) k_render_batch(const rt_render_job* __rest
                                                       uint32_t grab, int lane_stack, int budget) {
     extern __shared__ uint32_t smem[];
     __shared__ float4 pk[kPacketLds];
+    __shared__ uint32_t bstat[3];  // the wave's counts for job `sjob` since its last flush
     uint32_t* pref = smem;
     int32_t* stk = (int32_t*)(smem + ((n_jobs + 1 + 3) & ~3u));
     const uint32_t lane = threadIdx.x;
     for (uint32_t j = lane; j <= n_jobs; j += 64) pref[j] = ws[kBatchPrefixWord + j];
+    if (lane < 3) bstat[lane] = 0;
     __syncthreads();
     const uint32_t total = pref[n_jobs];
+    uint32_t* sjob = nullptr;  // wave-uniform: the stats words the counts belong to
+    auto flush = [&]() {  // one atomic per counter per job the wave worked on, not one per tile
+        __syncthreads();
+        if (sjob && lane < 3 && bstat[lane]) atomicAdd(&sjob[lane], bstat[lane]);
+        if (lane < 3) bstat[lane] = 0;
+        __syncthreads();
+    };
     for (;;) {
         uint32_t u0 = 0;
         if (lane == 0) u0 = atomicAdd(&ws[0], grab);
@@ -591,10 +600,15 @@ __global__ void __launch_bounds__(64) k_render_batch(const rt_render_job* __rest
             px.valid = px.col < p.width && px.row < p.height;
             const SceneArgs sc = job_scene(job);
             const rt_render_out o = job_out(job);
-            render_pixel<PM, SM>(sc, job.cam, job_params(job), px, o.rgb, o.t, o.prim, o.stats, pk,
+            if (o.stats != sjob) {
+                flush();
+                sjob = o.stats;
+            }
+            render_pixel<PM, SM>(sc, job.cam, job_params(job), px, o.rgb, o.t, o.prim, sjob ? bstat : nullptr, pk,
                                  stk + lane, 64, lane_stack, budget, coherent_tiles(job.cam, p.width, p.height));
         }
     }
+    flush();
 }
 
 // Batched render with per-lane refill (rt_batch_config primary = shadow = RT_TRAVERSAL_LANE). The

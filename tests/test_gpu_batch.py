@@ -83,7 +83,10 @@ def assert_batch_equals_alone(b, reqs):
                                  # per-lane refill kernel (LANE + LANE): stacks of 1 entry force the
                                  # packet retrace of both walks; grabs of 64 tiles cross jobs
                                  dict(primary="lane", shadow="lane", lane_stack=1, tiles_per_grab=2),
-                                 dict(primary="lane", shadow="lane", tiles_per_grab=64, waves_per_cu=32)])
+                                 dict(primary="lane", shadow="lane", tiles_per_grab=64, waves_per_cu=32),
+                                 # quad-lane tiles (4x4 units) for both rays; grabs of 5 units cross jobs
+                                 dict(primary="quad", tiles_per_grab=5, waves_per_cu=8),
+                                 dict(primary="quad", tiles_per_grab=64, waves_per_cu=32)])
 def test_batch_equals_individual_renders(ren, scenes_, cfg):
     from rendertoy_amd.batch import BatchRender, BatchConfig
     reqs = requests(ren, scenes_)
