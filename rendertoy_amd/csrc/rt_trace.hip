@@ -620,6 +620,11 @@ __global__ void __launch_bounds__(64) k_render_batch(const rt_render_job* __rest
 // the job's scene and parameters stay wave-uniform. A walk that would overflow its LDS stack is
 // traced again at once by the whole wave with the packet traversal. Every pixel gets exactly the
 // values render_pixel<LANE, LANE> gives it (same walks, refinement, shading and store).
+#ifndef RT_REFILL_PASSES
+#define RT_REFILL_PASSES 8
+#endif
+constexpr int kRefillPasses = RT_REFILL_PASSES;  // refills per round while culled rays free lanes
+
 __global__ void __launch_bounds__(64) k_render_batch_refill(const rt_render_job* __restrict__ jobs,
                                                              uint32_t n_jobs, uint32_t* __restrict__ ws,
                                                              uint32_t grab, int lane_stack) {
@@ -641,6 +646,8 @@ __global__ void __launch_bounds__(64) k_render_batch_refill(const rt_render_job*
     rt_render_params p{};
     rt_render_out out{};
     uint32_t tiles_x = 1, job_first = 0, n_hit = 0, n_sh = 0, n_oc = 0;
+    bool cull = false;  // the job's scene box culls new primary rays
+    float blo0 = 0.0f, blo1 = 0.0f, blo2 = 0.0f, bhi0 = 0.0f, bhi1 = 0.0f, bhi2 = 0.0f;
     auto flush = [&]() {
         if (job != 0xFFFFFFFFu && out.stats && lane == 0) {
             if (n_hit) atomicAdd(&out.stats[0], n_hit);
@@ -669,10 +676,14 @@ __global__ void __launch_bounds__(64) k_render_batch_refill(const rt_render_job*
     float ndl = 0.0f, alb0 = 1.0f, alb1 = 1.0f, alb2 = 1.0f;
 
     for (;;) {
-        // ---- refill from the current tile (moving on to the next tile / pool / job as needed)
-        const uint64_t idle = __ballot(!busy);
-        const uint32_t n_idle = (uint32_t)__popcll(idle);
-        if (!exhausted && (n_idle >= kRefillIdle || n_idle == 64u)) {
+        // ---- refill from the current tile (moving on to the next tile / pool / job as needed). A new
+        // primary ray that misses the scene's box (`cull`) is finished at once with the values a
+        // walk would give it (background, t = inf, prim = -1, no counts) and its lane takes the next
+        // pixel in another pass, so the lanes walk rays that can hit something.
+        for (int pass = 0; pass < kRefillPasses; ++pass) {
+            const uint64_t idle = __ballot(!busy);
+            const uint32_t n_idle = (uint32_t)__popcll(idle);
+            if (exhausted || !(n_idle >= kRefillIdle || n_idle == 64u)) break;
             if (tile_px == 64u) {
                 if (++tile >= pool_end) {
                     uint32_t u0 = 0;
@@ -705,7 +716,16 @@ __global__ void __launch_bounds__(64) k_render_batch_refill(const rt_render_job*
                 out = job_out(jb);
                 tiles_x = (p.width + 7u) / 8u;
                 job_first = pref[job];
+                // the scene box (written with the hierarchy's root node, so only when the root is
+                // an internal node) contains every child box of the root
+                cull = sc.root >= 0 && sc.root != RT_EMPTY_ROOT && jb.scene.bounds != nullptr;
+                if (cull) {
+                    const float* bb = as_global((const float*)jb.scene.bounds);
+                    blo0 = bb[0]; blo1 = bb[1]; blo2 = bb[2];
+                    bhi0 = bb[4]; bhi1 = bb[5]; bhi2 = bb[6];
+                }
             }
+            bool culled = false;
             if (!exhausted && tile_job == job) {
                 const uint32_t take = 64u - tile_px < n_idle ? 64u - tile_px : n_idle;
                 if (!busy) {
@@ -726,12 +746,30 @@ __global__ void __launch_bounds__(64) k_render_batch_refill(const rt_render_job*
                             walk_start(r, sc.root, w);
                             skip = -1;
                             phase = 0;
-                            busy = true;
+                            if (cull) {  // the node test of the root's children, on their union
+                                const float x0 = fmaf(blo0, w.idx, -w.oxx), x1 = fmaf(bhi0, w.idx, -w.oxx);
+                                const float y0 = fmaf(blo1, w.idy, -w.oyy), y1 = fmaf(bhi1, w.idy, -w.oyy);
+                                const float z0 = fmaf(blo2, w.idz, -w.ozz), z1 = fmaf(bhi2, w.idz, -w.ozz);
+                                const float tn = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), r.tmin));
+                                const float tf = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fmaxf(z0, z1)) * kTfarScale;
+                                culled = !(tn <= fminf(tf, h.t));
+                            }
+                            if (culled) {
+                                const uint64_t pix = (uint64_t)row * p.width + col;
+                                if (out.t) out.t[pix] = INFINITY;
+                                if (out.prim) out.prim[pix] = -1;
+                                out.rgb[3 * pix] = p.background[0];
+                                out.rgb[3 * pix + 1] = p.background[1];
+                                out.rgb[3 * pix + 2] = p.background[2];
+                            } else {
+                                busy = true;
+                            }
                         }
                     }
                 }
                 tile_px += take;
             }
+            if (__ballot(culled) == 0) break;  // no lane freed: as full as this refill makes them
         }
         if (__ballot(busy) == 0) {
             if (exhausted) break;
