@@ -299,7 +299,7 @@ def measure_serving(args, ren, scenes, torch):
     batching loop (RenderServer: batches on 4 streams, at most 4 in flight) and by the per-request
     loop (PlainServer: one rt_render per request as it arrives) on the same traces; latency = GPU
     completion - arrival. `saturation`: every request present at t = 0 (throughput = requests /
-    makespan, the capacity of each loop)."""
+    makespan, the capacity of each loop), after one untimed run of each loop; the median of 3."""
     from rendertoy_amd import serving
     from rendertoy_amd.batch import time_launches
     g0 = time.time()
@@ -333,16 +333,24 @@ def measure_serving(args, ren, scenes, torch):
         rates[str(int(rate))] = row
         note(f"serving at {rate:.0f} rps: p99 batched {row['batched'].get('p99_ms')} ms, "
              f"plain {row['plain'].get('p99_ms')} ms")
+    # saturation: one untimed run per loop first (the batched loop meets its largest batches here
+    # for the first time), then 3 runs; the run with the median throughput is reported
     sat = {}
     for name, server in (("batched", srv), ("plain", plain)):
-        reqs = serving.make_requests(args.serving_requests, 64, 1e9, 4999)
-        if name == "batched":
-            srv.batches = []
-        server.run(reqs, realtime=False)
-        sm = serving.latency_summary(reqs, realtime=False)
-        if name == "batched":
-            sm["mean_batch"] = round(float(np.mean(srv.batches)), 2)
-        sat[name] = sm
+        runs = []
+        for k in range(4):
+            reqs = serving.make_requests(args.serving_requests, 64, 1e9, 4999)
+            if name == "batched":
+                srv.batches = []
+            server.run(reqs, realtime=False)
+            sm = serving.latency_summary(reqs, realtime=False)
+            if name == "batched":
+                sm["mean_batch"] = round(float(np.mean(srv.batches)), 2)
+            if k:
+                runs.append(sm)
+        runs.sort(key=lambda x: x["throughput_rps"])
+        sat[name] = dict(runs[len(runs) // 2])
+        sat[name]["throughput_rps_runs"] = [x["throughput_rps"] for x in runs]
     sat["throughput_ratio"] = round(sat["batched"]["throughput_rps"] / sat["plain"]["throughput_rps"], 3)
     return {"what": "64 resident scenes, 512^2 RGBA8 textures, Poisson arrivals, images 256^2..1024^2; "
                     "continuous batching on 4 streams vs one rt_render per request",
