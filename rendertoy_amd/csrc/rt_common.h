@@ -735,23 +735,53 @@ constexpr int kLaneStack = RT_LANE_STACK_MAX;
 // The children of 4-wide node `cur` whose boxes ray r (slab terms in w) enters no later than `best`:
 // returns how many, with their refs in c[0..n-1] nearest entry first (equal entry distances in any
 // order: the visiting order decides only how soon the best hit shrinks, never which hit is found).
+// NF: near and far planes chosen by the ray's direction signs instead of min/max. Both planes go
+// through the same fma and rounding is monotone in the plane, so min(fma(lo), fma(hi)) is fma(lo)
+// when the inverse direction is positive and fma(hi) when it is negative: loading the planes in that
+// order gives the same entry and exit values without the min/max, at the cost of one address per
+// plane. The bounce walks of rt_trace_paths use it (7% faster there); the other per-lane walks keep
+// the min/max form (the batch refill kernel is 5-8% slower with it).
+template <bool NF = false>
 RT_DEV int lane4_children(const float4* __restrict__ nodes4, int32_t cur, const Ray& r, const LaneWalk& w,
                           float best, int32_t c[4]) {
     const float4* nd = nodes4 + (size_t)cur * 8;
-    const float4 lx = nd[0], hx = nd[1], ly = nd[2], hy = nd[3], lz = nd[4], hz = nd[5];
+    float cnx[4], cfx[4], cny[4], cfy[4], cnz[4], cfz[4];
+    if constexpr (NF) {
+        const uint32_t sx = __float_as_uint(w.idx) >> 31, sy = __float_as_uint(w.idy) >> 31,
+                       sz = __float_as_uint(w.idz) >> 31;
+        const float4 nx = nd[sx], fx = nd[1u - sx], ny = nd[2u + sy], fy = nd[3u - sy], nz = nd[4u + sz],
+                     fz = nd[5u - sz];
+        cnx[0] = nx.x; cnx[1] = nx.y; cnx[2] = nx.z; cnx[3] = nx.w;
+        cfx[0] = fx.x; cfx[1] = fx.y; cfx[2] = fx.z; cfx[3] = fx.w;
+        cny[0] = ny.x; cny[1] = ny.y; cny[2] = ny.z; cny[3] = ny.w;
+        cfy[0] = fy.x; cfy[1] = fy.y; cfy[2] = fy.z; cfy[3] = fy.w;
+        cnz[0] = nz.x; cnz[1] = nz.y; cnz[2] = nz.z; cnz[3] = nz.w;
+        cfz[0] = fz.x; cfz[1] = fz.y; cfz[2] = fz.z; cfz[3] = fz.w;
+    } else {
+        const float4 lx = nd[0], hx = nd[1], ly = nd[2], hy = nd[3], lz = nd[4], hz = nd[5];
+        cnx[0] = lx.x; cnx[1] = lx.y; cnx[2] = lx.z; cnx[3] = lx.w;
+        cfx[0] = hx.x; cfx[1] = hx.y; cfx[2] = hx.z; cfx[3] = hx.w;
+        cny[0] = ly.x; cny[1] = ly.y; cny[2] = ly.z; cny[3] = ly.w;
+        cfy[0] = hy.x; cfy[1] = hy.y; cfy[2] = hy.z; cfy[3] = hy.w;
+        cnz[0] = lz.x; cnz[1] = lz.y; cnz[2] = lz.z; cnz[3] = lz.w;
+        cfz[0] = hz.x; cfz[1] = hz.y; cfz[2] = hz.z; cfz[3] = hz.w;
+    }
     const float4 rf = nd[6];
-    const float clx[4] = {lx.x, lx.y, lx.z, lx.w}, chx[4] = {hx.x, hx.y, hx.z, hx.w};
-    const float cly[4] = {ly.x, ly.y, ly.z, ly.w}, chy[4] = {hy.x, hy.y, hy.z, hy.w};
-    const float clz[4] = {lz.x, lz.y, lz.z, lz.w}, chz[4] = {hz.x, hz.y, hz.z, hz.w};
     int32_t ref[4] = {__float_as_int(rf.x), __float_as_int(rf.y), __float_as_int(rf.z), __float_as_int(rf.w)};
     float key[4];
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-        const float x0 = fmaf(clx[k], w.idx, -w.oxx), x1 = fmaf(chx[k], w.idx, -w.oxx);
-        const float y0 = fmaf(cly[k], w.idy, -w.oyy), y1 = fmaf(chy[k], w.idy, -w.oyy);
-        const float z0 = fmaf(clz[k], w.idz, -w.ozz), z1 = fmaf(chz[k], w.idz, -w.ozz);
-        const float tn = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), r.tmin));
-        const float tf = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fmaxf(z0, z1)) * kTfarScale;
+        const float x0 = fmaf(cnx[k], w.idx, -w.oxx), x1 = fmaf(cfx[k], w.idx, -w.oxx);
+        const float y0 = fmaf(cny[k], w.idy, -w.oyy), y1 = fmaf(cfy[k], w.idy, -w.oyy);
+        const float z0 = fmaf(cnz[k], w.idz, -w.ozz), z1 = fmaf(cfz[k], w.idz, -w.ozz);
+        float tn, tf;
+        if constexpr (NF) {
+            tn = fmaxf(fmaxf(x0, y0), fmaxf(z0, r.tmin));
+            tf = fminf(fminf(x1, y1), z1) * kTfarScale;
+        } else {
+            tn = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), r.tmin));
+            tf = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fmaxf(z0, z1)) * kTfarScale;
+        }
         const bool hk = (ref[k] != RT_EMPTY_ROOT) & (tn <= fminf(tf, best));
         key[k] = hk ? tn : INFINITY;
     }
@@ -785,7 +815,7 @@ RT_DEV int lane4_children(const float4* __restrict__ nodes4, int32_t cur, const 
 // round for all parked leaves (speculative traversal, Aila & Laine 2009). The closest hit does not
 // depend on the visiting order (a node is skipped only past the best hit so far, which a later leaf
 // can only lower), so the hits are those of the one-node-or-leaf walk. Parking costs ~3 VGPRs.
-template <bool ANY, int PARK = RT_PARK_RENDER>
+template <bool ANY, int PARK = RT_PARK_RENDER, bool NF = false>
 RT_DEV int lane4_steps(const float4* __restrict__ nodes4, const float4* __restrict__ tris, const Ray& r,
                        int32_t skip_slot, Hit& h, LaneWalk& w, int32_t* stk, int stride, int limit,
                        int steps, bool any_rt = false) {
@@ -794,7 +824,7 @@ RT_DEV int lane4_steps(const float4* __restrict__ nodes4, const float4* __restri
         if constexpr (PARK > 0) {
             if (w.cur >= 0 && w.cur != RT_EMPTY_ROOT) {
                 int32_t c[4];
-                const int nhit = lane4_children(nodes4, w.cur, r, w, h.t, c);
+                const int nhit = lane4_children<NF>(nodes4, w.cur, r, w, h.t, c);
                 if (nhit > 0) {
                     if (w.sp + nhit - 1 > limit) return kWalkOverflow;
                     // push the farther children (farthest first), go on with the nearest
@@ -822,7 +852,7 @@ RT_DEV int lane4_steps(const float4* __restrict__ nodes4, const float4* __restri
         } else {
             if (w.cur >= 0) {
                 int32_t c[4];
-                const int nhit = lane4_children(nodes4, w.cur, r, w, h.t, c);
+                const int nhit = lane4_children<NF>(nodes4, w.cur, r, w, h.t, c);
                 if (nhit > 0) {
                     if (w.sp + nhit - 1 > limit) return kWalkOverflow;
                     if (nhit >= 4) stk[(w.sp++) * stride] = c[3];

@@ -109,7 +109,7 @@ RT_DEV uint32_t refill_slot(uint32_t block, uint32_t wave_in_block, uint32_t wav
     return ((block & 7u) * (n_blocks >> 3) + (block >> 3)) * waves_per_block + wave_in_block;
 }
 
-template <bool ANY, int MODE, int PARK, typename Load, typename Finish>
+template <bool ANY, int MODE, int PARK, bool NF = false, typename Load, typename Finish>
 RT_DEV void refill_walks(const SceneArgs& sc, uint64_t n, uint32_t n_waves, uint32_t wid, int32_t* stk,
                          int stk_stride, int lane_stack, uint32_t* list, float4* pk, Load&& load,
                          Finish&& finish) {
@@ -162,7 +162,7 @@ RT_DEV void refill_walks(const SceneArgs& sc, uint64_t n, uint32_t n_waves, uint
                 else if (MODE == RT_TRAVERSAL_BINARY)
                     st = bin_steps<ANY>(sc.nodes, sc.tris, r, skip, h, w, stk, stk_stride, lane_stack, kRefillSteps);
                 else
-                    st = lane4_steps<ANY, PARK>(sc.nodes4, sc.tris, r, skip, h, w, stk, stk_stride, lane_stack,
+                    st = lane4_steps<ANY, PARK, NF>(sc.nodes4, sc.tris, r, skip, h, w, stk, stk_stride, lane_stack,
                                           kRefillSteps);
             }
             finish(st == kWalkDone, ri, r, h);
