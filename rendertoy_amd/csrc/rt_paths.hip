@@ -7,7 +7,8 @@
 //   surface after the last bounce; pixel = mean of its samples (fixed order).
 // Kernels (all on the caller's stream, no host synchronisation, no memset: graph-capturable):
 //   k_paths_init     counters
-//   k_paths_primary  camera rays (8x8 tile per wave, AUTO coherence rule), hits append a bounce ray
+//   k_paths_primary  camera rays (QUAD: a 4x4 tile per one-wave workgroup; else an 8x8 tile per
+//                    wave, AUTO coherence rule), hits append a bounce ray
 //                    to queue 0 (wave ballot + one atomic per wave = compaction)
 //   k_paths_bounce   per depth: the waves share the queue's 64-ray groups (count read on the
 //                    device), per-lane walks with refill; each ray records the slot it hit
@@ -145,16 +146,27 @@ template <int PM>
 __global__ void __launch_bounds__(256) k_paths_primary(SceneArgs sc, rt_camera cam, PathArgs a,
                                                         uint32_t tiles_x, int lane_stack, int budget,
                                                         bool coherent) {
-    // PM == RT_TRAVERSAL_QUAD: quad layout, a block covers 8x8 pixels (wave w the 4x4 sub-tile
-    // ((w & 1) * 4, (w >> 1) * 4), lanes 4q..4q+3 its pixel (q & 3, q >> 2)); else 16x16 pixels
+    // PM == RT_TRAVERSAL_QUAD: quad layout, one wave per workgroup (wave j of XCD x takes the 4x4
+    // quarter w = j & 3 of 8x8 block (j >> 2) * 8 + x, row-major, at ((w & 1) * 4, (w >> 1) * 4),
+    // lanes 4q..4q+3 its pixel (q & 3, q >> 2)); else a 256-thread block covers 16x16 pixels
     constexpr bool QL = PM == RT_TRAVERSAL_QUAD;
     __shared__ int32_t stk[(uses_lds(PM) && !QL) ? kLaneStack * 256 : 1];
-    __shared__ float4 pk[4][kPacketLds];
-    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const uint32_t col = QL ? (blockIdx.x % tiles_x) * 8 + (wave & 1) * 4 + ((lane >> 2) & 3)
-                            : (blockIdx.x % tiles_x) * 16 + (wave & 1) * 8 + (lane & 7);
-    const uint32_t row = QL ? (blockIdx.x / tiles_x) * 8 + (wave >> 1) * 4 + (lane >> 4)
-                            : (blockIdx.x / tiles_x) * 16 + (wave >> 1) * 8 + (lane >> 3);
+    __shared__ float4 pk[QL ? 1 : 4][kPacketLds];
+    const uint32_t lane = threadIdx.x & 63;
+    uint32_t wave, blk;
+    if constexpr (QL) {
+        const uint32_t xcd = blockIdx.x & 7u, j = blockIdx.x >> 3;
+        wave = j & 3u;
+        blk = (j >> 2) * 8u + xcd;
+        if (blk >= tiles_x * ((a.H + 7u) / 8u)) return;
+    } else {
+        wave = threadIdx.x >> 6;
+        blk = blockIdx.x;
+    }
+    const uint32_t col = QL ? (blk % tiles_x) * 8 + (wave & 1) * 4 + ((lane >> 2) & 3)
+                            : (blk % tiles_x) * 16 + (wave & 1) * 8 + (lane & 7);
+    const uint32_t row = QL ? (blk / tiles_x) * 8 + (wave >> 1) * 4 + (lane >> 4)
+                            : (blk / tiles_x) * 16 + (wave >> 1) * 8 + (lane >> 3);
     const bool lead = !QL || (lane & 3) == 0;  // the lane that writes for its pixel
     const bool valid = col < a.W && row < a.H;
     const uint32_t pixel = valid ? row * a.W + col : 0u;
@@ -335,8 +347,8 @@ extern "C" int rt_trace_paths(const rt_scene* scene, const rt_camera* cam, const
                               k_paths_primary<3>};
     if (tu.primary == RT_TRAVERSAL_QUAD) {
         const uint32_t tx8 = (p->width + 7) / 8, ty8 = (p->height + 7) / 8;
-        hipLaunchKernelGGL(k_paths_primary<RT_TRAVERSAL_QUAD>, dim3(tx8 * ty8), dim3(256), 0, s, sc,
-                           *cam, a, tx8, (int)tu.lane_stack, (int)tu.packet_budget, coherent);
+        hipLaunchKernelGGL(k_paths_primary<RT_TRAVERSAL_QUAD>, dim3(32u * ((tx8 * ty8 + 7u) / 8u)), dim3(64), 0, s,
+                           sc, *cam, a, tx8, (int)tu.lane_stack, (int)tu.packet_budget, coherent);
     } else {
         hipLaunchKernelGGL(prim[tu.primary], dim3(tiles_x * tiles_y), dim3(256), 0, s, sc, *cam, a,
                            tiles_x, (int)tu.lane_stack, (int)tu.packet_budget, coherent);

@@ -159,13 +159,12 @@ uint32_t tile_map_blocks(const TileMap& m) {
     return 8u * m.S * m.S * ((m.n_super + 7u) / 8u);
 }
 
-// Tile (tx, ty) of this block; false if the block maps outside the image (it then does nothing).
-RT_DEV bool block_tile(const TileMap& m, uint32_t& tx, uint32_t& ty) {
-    const uint32_t b = blockIdx.x;
+// Tile (tx, ty) of block b; false if the block maps outside the image (it then does nothing).
+RT_DEV bool block_tile_of(const TileMap& m, uint32_t b, uint32_t& tx, uint32_t& ty) {
     if (!m.S) {
         tx = b % m.tiles_x;
         ty = b / m.tiles_x;
-        return true;
+        return ty < m.tiles_y;
     }
     const uint32_t x = b & 7u, j = b >> 3, SS = m.S * m.S;
     const uint32_t s = (j / SS) * 8u + x, w = j % SS;
@@ -174,6 +173,8 @@ RT_DEV bool block_tile(const TileMap& m, uint32_t& tx, uint32_t& ty) {
     ty = (s / m.super_x) * m.S + w / m.S;
     return tx < m.tiles_x && ty < m.tiles_y;
 }
+
+RT_DEV bool block_tile(const TileMap& m, uint32_t& tx, uint32_t& ty) { return block_tile_of(m, blockIdx.x, tx, ty); }
 
 RT_DEV TilePixel tile_pixel(uint32_t W, uint32_t H, uint32_t tx, uint32_t ty) {
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -214,13 +215,18 @@ __global__ void __launch_bounds__(256) k_trace_camera(SceneArgs sc, rt_camera ca
     if (p.valid) store_hit(sc, hits, (uint64_t)p.row * W + p.col, h);
 }
 
-// Camera closest hit with quad-lane packets: a 256-thread block covers 8x8 pixels, wave w the 4x4
-// sub-tile ((w & 1) * 4, (w >> 1) * 4), lanes 4q..4q+3 the pixel (q & 3, q >> 2) of it.
-__global__ void __launch_bounds__(256) k_trace_camera_quad(SceneArgs sc, rt_camera cam, uint32_t W,
-                                                           uint32_t H, TileMap tm, rt_hits hits) {
-    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6, qr = lane >> 2;
+// Camera closest hit with quad-lane packets, one wave per workgroup: wave j of XCD x (workgroup
+// 8j + x, as workgroups are handed to the 8 XCDs round robin) takes the 4x4 quarter w = j & 3 of
+// the 8x8-pixel block (j >> 2) * 8 + x of the tile map, at ((w & 1) * 4, (w >> 1) * 4), and lanes
+// 4r..4r+3 the pixel (r & 3, r >> 2) of the quarter. The four quarters of a block are consecutive waves of one XCD
+// (they share its L2), and a wave that ends frees its slot for the next one at once (a CU starts a
+// 4-wave workgroup only once it has 4 free slots: 3-4% slower frames).
+__global__ void __launch_bounds__(64) k_trace_camera_quad(SceneArgs sc, rt_camera cam, uint32_t W,
+                                                          uint32_t H, TileMap tm, rt_hits hits) {
+    const uint32_t lane = threadIdx.x & 63, qr = lane >> 2;
+    const uint32_t xcd = blockIdx.x & 7u, j = blockIdx.x >> 3, wave = j & 3u;
     uint32_t tx, ty;
-    if (!block_tile(tm, tx, ty)) return;
+    if (!block_tile_of(tm, (j >> 2) * 8u + xcd, tx, ty)) return;
     TilePixel p;
     p.col = tx * 8 + (wave & 1) * 4 + (qr & 3);
     p.row = ty * 8 + (wave >> 1) * 4 + (qr >> 2);
@@ -425,21 +431,40 @@ __global__ void __launch_bounds__(256) k_render(SceneArgs sc, rt_camera cam, rt_
 
 // Fused render with quad-lane packets for the primary and the shadow rays: one 8x8-pixel tile per
 // block, wave w = 4x4 quarter (w & 1, w >> 1), lanes 4q..4q+3 = pixel (q & 3, q >> 2).
-__global__ void __launch_bounds__(256) k_render_quad(SceneArgs sc, rt_camera cam, rt_render_params p,
+// W1: one wave per workgroup (the camera kernel's mapping: wave j of XCD x renders quarter j & 3 of
+// 8x8 block (j >> 2) * 8 + x, row-major), for frames without counts; else 4-wave workgroups whose
+// counts are summed in LDS (one wave per workgroup would take one global atomic per wave).
+template <bool W1>
+__global__ void __launch_bounds__(W1 ? 64 : 256) k_render_quad(SceneArgs sc, rt_camera cam, rt_render_params p,
                                                       uint32_t tiles_x8, float* __restrict__ rgb,
                                                       float* __restrict__ out_t,
                                                       int32_t* __restrict__ out_prim,
                                                       uint32_t* __restrict__ stats) {
-    const uint32_t wave = threadIdx.x >> 6, q = (threadIdx.x & 63) >> 2;
+    const uint32_t q = (threadIdx.x & 63) >> 2;
+    uint32_t wave, blk;
+    if (W1) {
+        const uint32_t xcd = blockIdx.x & 7u, j = blockIdx.x >> 3;
+        wave = j & 3u;
+        blk = (j >> 2) * 8u + xcd;
+        if (blk >= tiles_x8 * ((p.height + 7u) / 8u)) return;
+    } else {
+        wave = threadIdx.x >> 6;
+        blk = blockIdx.x;
+    }
     TilePixel px;
-    px.col = (blockIdx.x % tiles_x8) * 8 + (wave & 1) * 4 + (q & 3);
-    px.row = (blockIdx.x / tiles_x8) * 8 + (wave >> 1) * 4 + (q >> 2);
+    px.col = (blk % tiles_x8) * 8 + (wave & 1) * 4 + (q & 3);
+    px.row = (blk / tiles_x8) * 8 + (wave >> 1) * 4 + (q >> 2);
     px.valid = px.col < p.width && px.row < p.height;
-    __shared__ uint32_t bstat[3];
-    block_stats_begin(stats, bstat);
-    render_pixel<RT_TRAVERSAL_QUAD, RT_TRAVERSAL_QUAD>(sc, cam, p, px, rgb, out_t, out_prim,
-                                                       stats ? bstat : nullptr, nullptr, nullptr, 0, 0, 0, true);
-    block_stats_end(stats, bstat);
+    if (W1) {
+        render_pixel<RT_TRAVERSAL_QUAD, RT_TRAVERSAL_QUAD>(sc, cam, p, px, rgb, out_t, out_prim, nullptr, nullptr,
+                                                           nullptr, 0, 0, 0, true);
+    } else {
+        __shared__ uint32_t bstat[3];
+        block_stats_begin(stats, bstat);
+        render_pixel<RT_TRAVERSAL_QUAD, RT_TRAVERSAL_QUAD>(sc, cam, p, px, rgb, out_t, out_prim,
+                                                           stats ? bstat : nullptr, nullptr, nullptr, 0, 0, 0, true);
+        block_stats_end(stats, bstat);
+    }
 }
 
 // ---- batched render ----
@@ -977,7 +1002,7 @@ extern "C" int rt_trace_camera(const rt_scene* scene, const rt_camera* cam, uint
     hipStream_t s = (hipStream_t)stream;
     if (tu.primary == RT_TRAVERSAL_QUAD) {  // 8x8-pixel blocks
         const TileMap tm8 = make_tile_map(W, H, tu.super_tile, 8);
-        hipLaunchKernelGGL(k_trace_camera_quad, dim3(tile_map_blocks(tm8)), dim3(256), 0, s,
+        hipLaunchKernelGGL(k_trace_camera_quad, dim3(32u * ((tile_map_blocks(tm8) + 7u) / 8u)), dim3(64), 0, s,
                            scene_args(scene), *cam, W, H, tm8, *hits);
         return check_launch("rt_trace_camera");
     }
@@ -1023,8 +1048,12 @@ extern "C" int rt_render(const rt_scene* scene, const rt_camera* cam, const rt_r
     // lane shadows and 1.45 ms for lane + lane, and at least as fast on smaller scenes
     if (tu.primary == RT_TRAVERSAL_QUAD) {
         const uint32_t tx8 = (p->width + 7) / 8, ty8 = (p->height + 7) / 8;
-        hipLaunchKernelGGL(k_render_quad, dim3(tx8 * ty8), dim3(256), 0, s, scene_args(scene), *cam, *p,
-                           tx8, out->rgb, out->t, out->prim, out->stats);
+        if (out->stats)
+            hipLaunchKernelGGL(k_render_quad<false>, dim3(tx8 * ty8), dim3(256), 0, s, scene_args(scene), *cam, *p,
+                               tx8, out->rgb, out->t, out->prim, out->stats);
+        else
+            hipLaunchKernelGGL(k_render_quad<true>, dim3(32u * ((tx8 * ty8 + 7u) / 8u)), dim3(64), 0, s,
+                               scene_args(scene), *cam, *p, tx8, out->rgb, out->t, out->prim, nullptr);
         return check_launch("rt_render");
     }
     const int pm = resolve_primary(tu.primary, *cam, p->width, p->height);
