@@ -150,7 +150,8 @@ def test_render_c1_matches_twin(c1, twin):
 @pytest.mark.parametrize("primary,shadow,lane_stack,super_tile,budget", [
     ("lane", "lane", 32, 0, 256), ("lane", "packet", 2, 3, 256), ("packet", "lane", 1, 1, 256),
     ("lane", "lane", 4, 0, 256), ("hybrid", "hybrid", 32, 0, 3), ("hybrid", "lane", 2, 0, 1),
-    ("hybrid", "hybrid", 32, 0, 40), ("auto", "lane", 32, 0, 256), ("quad", "lane", 32, 0, 256)])
+    ("hybrid", "hybrid", 32, 0, 40), ("auto", "lane", 32, 0, 256), ("quad", "lane", 32, 0, 256),
+    ("quad", "lane", 32, 2, 256), ("quad", "lane", 32, 5, 256)])
 def test_tuning_choices_give_identical_results(c1, twin, primary, shadow, lane_stack, super_tile,
                                                budget):
     """Every traversal / stack / block-mapping / budget choice returns the twin's hits and RGB.
@@ -169,6 +170,30 @@ def test_tuning_choices_give_identical_results(c1, twin, primary, shadow, lane_s
         gc = {k: v.cpu().numpy() for k, v in rc.cast_camera(cam, 512, 512).items()}
     np.testing.assert_array_equal(hits["prim"].cpu().numpy(), prim)
     np.testing.assert_array_equal(_u32(img.cpu().numpy()), _u32(rgb))
+    assert list(stats.cpu().numpy()[:3]) == list(st)
+    assert_trace_equal(gc, ref)
+
+
+@pytest.mark.parametrize("W,H", [(512, 512), (333, 257), (9, 17), (1, 1), (8, 72)])
+def test_render_with_and_without_counts(c1, W, H):
+    """rt_render's quad-lane kernel runs one wave per workgroup for a frame without counts and as
+    4-wave workgroups that sum the counts for a frame with them: both give the twin's image and hits
+    at square, odd and tiny frame sizes, and the camera kernel (one wave per workgroup) its hits."""
+    import rendertoy_amd as ren
+    meshes, tw, rc = c1
+    cam = camera(aspect=W / H)
+    rgb, t, prim, st = tw.render(cam.as_array(), W, H, LIGHT, shadows=True)
+    ref = tw.trace_camera(cam.as_array(), W, H)
+    with ren.tuning(primary="quad"):
+        img0, hits0 = rc.render(cam, W, H, light=LIGHT, shadows=True, bilinear=True, return_hits=True)
+        stats = torch.zeros(4, dtype=torch.int32, device="cuda")
+        img1, hits1 = rc.render(cam, W, H, light=LIGHT, shadows=True, bilinear=True, return_hits=True,
+                                stats=stats)
+        gc = {k: v.cpu().numpy() for k, v in rc.cast_camera(cam, W, H).items()}
+    for img, hits in ((img0, hits0), (img1, hits1)):
+        np.testing.assert_array_equal(hits["prim"].cpu().numpy(), prim)
+        np.testing.assert_array_equal(_u32(hits["t"].cpu().numpy()), _u32(np.where(prim >= 0, t, np.float32(np.inf))))
+        np.testing.assert_array_equal(_u32(img.cpu().numpy()), _u32(rgb))
     assert list(stats.cpu().numpy()[:3]) == list(st)
     assert_trace_equal(gc, ref)
 
