@@ -113,8 +113,10 @@ struct PathArgs {
 
 // The bounce ray leaving the closest hit h of ray r (sample `id` at depth k+1): appended to queue
 // `qo` by the lanes with `go` set (compaction: ballot + one atomic per wave).
-RT_DEV void emit_bounce(const SceneArgs& sc, const PathArgs& a, const Ray& r, const Hit& h, bool go,
-                        uint32_t id, uint32_t depth, float thr, float4* qo, uint32_t* counter) {
+// The queue entry of the bounce of a hit (go): origin and throughput, direction and sample id, the
+// hit's slot (skipped by the bounce's walk).
+RT_DEV void bounce_entry(const SceneArgs& sc, const PathArgs& a, const Ray& r, const Hit& h, bool go,
+                         uint32_t id, uint32_t depth, float thr, float4& e0, float4& e1, float4& e2) {
     float ox = 0.0f, oy = 0.0f, oz = 0.0f, dx = 0.0f, dy = 0.0f, dz = 0.0f;
     if (go) {
         const float4* tr = sc.tris + (size_t)h.slot * 3;
@@ -127,21 +129,36 @@ RT_DEV void emit_bounce(const SceneArgs& sc, const PathArgs& a, const Ray& r, co
         oy = fmaf(h.t, r.dy, r.oy);
         oz = fmaf(h.t, r.dz, r.oz);
     }
+    e0 = make_float4(ox, oy, oz, thr * a.albedo);
+    e1 = make_float4(dx, dy, dz, __int_as_float((int)id));
+    e2 = make_float4(__int_as_float(h.slot), 0.0f, 0.0f, 0.0f);
+}
+
+// Appends the bounce of each lane with `go` to queue qo (one atomic per wave, on `counter`).
+RT_DEV void emit_bounce(const SceneArgs& sc, const PathArgs& a, const Ray& r, const Hit& h, bool go,
+                        uint32_t id, uint32_t depth, float thr, float4* qo, uint32_t* counter) {
+    float4 e0, e1, e2;
+    bounce_entry(sc, a, r, h, go, id, depth, thr, e0, e1, e2);
     const uint32_t slot = wave_append(counter, go);
     if (go) {
         float4* e = qo + (size_t)slot * 3;
-        e[0] = make_float4(ox, oy, oz, thr * a.albedo);
-        e[1] = make_float4(dx, dy, dz, __int_as_float((int)id));
-        e[2] = make_float4(__int_as_float(h.slot), 0.0f, 0.0f, 0.0f);
+        e[0] = e0;
+        e[1] = e1;
+        e[2] = e2;
     }
+}
+
+// Lane ranks among the set bits of a wave mask (the lane's position in an append).
+RT_DEV uint32_t mask_rank(uint64_t mask) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
 }
 
 __global__ void k_paths_init(PathArgs a) {
     if (threadIdx.x <= a.bounces) a.counts[threadIdx.x] = threadIdx.x == 0 ? a.W * a.H * a.spp : 0u;
 }
 
-// Camera rays: a 256-thread block covers 16x16 pixels, a wave an 8x8 tile, for every sample in
-// turn (sample s of pixel p jittered by the first two Philox words of counter (p, s, 0, 0)).
+// Camera rays, for every sample in turn (sample s of pixel p jittered by the first two Philox
+// words of counter (p, s, 0, 0)); the pixel layout per traversal is below.
 template <int PM>
 __global__ void __launch_bounds__(256) k_paths_primary(SceneArgs sc, rt_camera cam, PathArgs a,
                                                         uint32_t tiles_x, int lane_stack, int budget,
@@ -152,6 +169,10 @@ __global__ void __launch_bounds__(256) k_paths_primary(SceneArgs sc, rt_camera c
     constexpr bool QL = PM == RT_TRAVERSAL_QUAD;
     __shared__ int32_t stk[(uses_lds(PM) && !QL) ? kLaneStack * 256 : 1];
     __shared__ float4 pk[QL ? 1 : 4][kPacketLds];
+    // QUAD: the bounce entries of up to 4 samples of the wave's 16 pixels, appended with one queue
+    // reservation (one same-address atomic per wave and sample took ~2 ms of a framed 4-spp frame)
+    __shared__ float4 stage[QL ? 64 * 3 : 1];
+    uint32_t staged = 0;
     const uint32_t lane = threadIdx.x & 63;
     uint32_t wave, blk;
     if constexpr (QL) {
@@ -197,7 +218,36 @@ __global__ void __launch_bounds__(256) k_paths_primary(SceneArgs sc, rt_camera c
         if (hit) refine_hit(sc.tris, r, h);
         const uint32_t id = pixel * a.spp + s;
         if (valid && lead) a.samples[id] = hit ? 0.0f : a.sky;
-        if (a.bounces > 0) emit_bounce(sc, a, r, h, hit && lead, id, 1u, 1.0f, a.q[0], &a.counts[1]);
+        if (a.bounces > 0) {
+            if constexpr (QL) {
+                const bool go = hit && lead;
+                float4 e0, e1, e2;
+                bounce_entry(sc, a, r, h, go, id, 1u, 1.0f, e0, e1, e2);
+                const uint64_t mask = __ballot(go);
+                if (go) {
+                    const uint32_t j = (staged + mask_rank(mask)) * 3u;
+                    stage[j] = e0;
+                    stage[j + 1] = e1;
+                    stage[j + 2] = e2;
+                }
+                staged += (uint32_t)__popcll(mask);
+                if ((s & 3u) == 3u || s + 1u == a.spp) {
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                    __builtin_amdgcn_wave_barrier();
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                    uint32_t b = 0;
+                    if (staged && lane == 0) b = atomicAdd(&a.counts[1], staged);
+                    b = (uint32_t)__shfl((int)b, 0);
+                    for (uint32_t x = lane; x < staged * 3u; x += 64u) a.q[0][(size_t)b * 3 + x] = stage[x];
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                    __builtin_amdgcn_wave_barrier();
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                    staged = 0;
+                }
+            } else {
+                emit_bounce(sc, a, r, h, hit && lead, id, 1u, 1.0f, a.q[0], &a.counts[1]);
+            }
+        }
     }
 }
 
@@ -246,6 +296,22 @@ __global__ void __launch_bounds__(256) k_paths_emit(SceneArgs sc, PathArgs a, ui
     const float4* qi = a.q[(k - 1) & 1];
     float4* qo = a.q[k & 1];
     const uint32_t stride = gridDim.x * blockDim.x;
+    // One queue reservation per wave: the wave first counts the hits among all the entries it will
+    // shade (their slot words), reserves that many entries of the next queue with one atomic, and
+    // then appends its bounces there in order. Appending per iteration took one same-address
+    // atomic per 64 entries, which set the kernel's time; the queue order does not enter any result.
+    const bool emits = k < a.bounces;
+    uint32_t wbase = 0;
+    if (emits) {
+        uint32_t cnt = 0;
+        for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += stride) {
+            const uint32_t i = base + threadIdx.x;
+            const bool hit = i < n && __float_as_int(qi[(size_t)i * 3 + 2].y) >= 0;
+            cnt += (uint32_t)__popcll(__ballot(hit));
+        }
+        if (cnt && (threadIdx.x & 63u) == 0u) wbase = atomicAdd(&a.counts[k + 1], cnt);
+        wbase = (uint32_t)__shfl((int)wbase, 0);
+    }
     for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += stride) {
         const uint32_t i = base + threadIdx.x;
         const bool valid = i < n;
@@ -277,7 +343,18 @@ __global__ void __launch_bounds__(256) k_paths_emit(SceneArgs sc, PathArgs a, ui
         const bool hit = valid && h.slot >= 0;
         if (valid && !hit) a.samples[id] = thr * a.sky;
         if (hit) refine_hit(sc.tris, r, h);
-        if (k < a.bounces) emit_bounce(sc, a, r, h, hit, id, k + 1, thr, qo, &a.counts[k + 1]);
+        if (emits) {
+            float4 e0, e1, e2;
+            bounce_entry(sc, a, r, h, hit, id, k + 1, thr, e0, e1, e2);
+            const uint64_t mask = __ballot(hit);
+            if (hit) {
+                float4* e = qo + (size_t)(wbase + mask_rank(mask)) * 3;
+                e[0] = e0;
+                e[1] = e1;
+                e[2] = e2;
+            }
+            wbase += (uint32_t)__popcll(mask);
+        }
     }
 }
 
