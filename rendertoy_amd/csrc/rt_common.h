@@ -489,7 +489,7 @@ RT_DEV uint64_t quad_mask_unpack(uint32_t cm) {
 // The walk starts at node `start` with the rays of `m` (bit 4q = ray q) and the candidates already
 // in h (all four lanes of a quad hold the same one: a bound found elsewhere, or none). Called by
 // all 64 lanes with uniform control flow.
-template <bool ANY>
+template <bool ANY, int LG = 4>
 RT_DEV void traverse_quad_core(const float4* __restrict__ nodes4, const float4* __restrict__ tris,
                                int32_t start, uint64_t m, const Ray& r, int32_t skip_slot, Hit& h) {
     const uint32_t lane = __lane_id(), c = lane & 3u, qb = lane & ~3u;
@@ -498,13 +498,17 @@ RT_DEV void traverse_quad_core(const float4* __restrict__ nodes4, const float4* 
     const float oxx = r.ox * idx, oyy = r.oy * idy, ozz = r.oz * idz;
     float qt = quad_min(h.t);  // the quad's best t so far (culling); h holds each lane's own best
     // tests triangle c of each leaf in `lref` (ray in the leaf's mask), keeps this lane's closest
-    // candidate in h and refreshes the quad's best t
+    // candidate in h and refreshes the quad's best t. The records of LG leaves are fetched before
+    // their tests: LG = 4 takes one memory round trip per step (48 VGPRs of records; the camera
+    // kernel at 90 VGPRs, 5 waves per SIMD), LG = 2 at most two (74 VGPRs, 6 waves per SIMD).
     auto leaves = [&](uint32_t lset, const int32_t* lref, const uint64_t* lmsk) {
         float bt = INFINITY, bu = 0.0f, bv = 0.0f;
         int32_t bp = -1, bs = -1;
         float4 ta[4], tb[4], tc[4];
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
+        for (int g = 0; g < 4; g += LG) {
+#pragma unroll
+        for (int k = g; k < g + LG; ++k) {
             if (!((lset >> k) & 1u)) continue;
             const uint32_t first = leaf_first(lref[k]), cnt = leaf_count(lref[k]);
             const float4* tr = tris + (size_t)(first + (c < cnt ? c : 0u)) * 3;
@@ -512,8 +516,9 @@ RT_DEV void traverse_quad_core(const float4* __restrict__ nodes4, const float4* 
             tb[k] = tr[1];
             tc[k] = tr[2];
         }
+        if (LG < 4) __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
+        for (int k = g; k < g + LG; ++k) {
             if (!((lset >> k) & 1u)) continue;
             const uint32_t first = leaf_first(lref[k]), cnt = leaf_count(lref[k]);
             const int32_t slot = (int32_t)(first + c);
@@ -529,6 +534,7 @@ RT_DEV void traverse_quad_core(const float4* __restrict__ nodes4, const float4* 
                     bs = slot;
                 }
             }
+        }
         }
         if (bp >= 0 && (bt < h.t || (bt == h.t && bp < h.prim))) {
             h.t = bt;
@@ -691,7 +697,7 @@ RT_DEV void traverse_quad_core(const float4* __restrict__ nodes4, const float4* 
 
 // The unbudgeted walk of the 16 rays of a wave from the root (lane 4q+c holds ray q; `want` is the
 // same in the 4 lanes of a quad).
-template <bool ANY>
+template <bool ANY, int LG = 4>
 RT_DEV void traverse_quad_lanes(const float4* __restrict__ nodes4, const float4* __restrict__ tris,
                                 int32_t root, const Ray& r, bool want, int32_t skip_slot, Hit& h) {
     h.t = r.tmax;
@@ -701,15 +707,15 @@ RT_DEV void traverse_quad_lanes(const float4* __restrict__ nodes4, const float4*
     h.slot = -1;
     const uint64_t m = __ballot(want) & kQuadBits;
     if (root == RT_EMPTY_ROOT || m == 0) return;
-    traverse_quad_core<ANY>(nodes4, tris, root, m, r, skip_slot, h);
+    traverse_quad_core<ANY, LG>(nodes4, tris, root, m, r, skip_slot, h);
 }
 
 // Quad-lane traversal of the 16 rays of a wave (lane 4q+c holds ray q); h = the ray's closest hit
 // (ANY: a hit) in all 4 lanes of its quad. Called by all 64 lanes.
-template <bool ANY>
+template <bool ANY, int LG = 4>
 RT_DEV void traverse_quad(const float4* __restrict__ nodes4, const float4* __restrict__ tris,
                           int32_t root, const Ray& r, bool want, int32_t skip_slot, Hit& h) {
-    traverse_quad_lanes<ANY>(nodes4, tris, root, r, want, skip_slot, h);
+    traverse_quad_lanes<ANY, LG>(nodes4, tris, root, r, want, skip_slot, h);
     quad_closest(h.t, h.u, h.v, h.prim, h.slot);
 }
 
