@@ -33,7 +33,7 @@ def run(rc, cam, W, H, **kw):
 
 
 @pytest.mark.parametrize("primary", ["quad", "auto", "packet", "lane"])
-@pytest.mark.parametrize("bounces,spp", [(2, 4), (0, 1), (3, 2)])
+@pytest.mark.parametrize("bounces,spp", [(2, 4), (0, 1), (3, 2), (2, 7)])
 def test_paths_equal_twin(ren, cluster, primary, bounces, spp):
     meshes, tw, rc = cluster
     W, H = 96, 64
