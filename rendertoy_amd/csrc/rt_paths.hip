@@ -209,14 +209,15 @@ __global__ void __launch_bounds__(256) k_paths_primary(SceneArgs sc, rt_camera c
         r.tmin = 0.0f;
         r.tmax = INFINITY;
         Hit h;
-        if constexpr (QL)
-            traverse_quad<false, 2>(sc.nodes4, sc.tris, sc.root, r, valid, -1, h);  // leaves in pairs:
-            // 94 VGPRs and 5 waves per SIMD instead of 111 and 4; the 4-spp frames of the
-            // 2M-triangle cluster take 2.5% less time this way (the camera and render kernels gain
-            // nothing from pairs on the C2 frame and lose 4% on C1, so they keep LG = 4)
-        else
+        if constexpr (QL) {
+            // leaf records in pairs: 94 VGPRs and 5 waves per SIMD instead of 111 and 4; the 4-spp
+            // frames of the 2M-triangle cluster take 2.5% less time this way (the camera and render
+            // kernels gain nothing from pairs on the C2 frame and lose 4% on C1, so they keep 4)
+            traverse_quad<false, 2>(sc.nodes4, sc.tris, sc.root, r, valid, -1, h);
+        } else {
             trace_primary<PM>(sc, r, valid, h, pk[wave], stk + threadIdx.x, 256, lane_stack, budget,
                               coherent);
+        }
         const bool hit = valid && h.prim >= 0;
         if (hit) refine_hit(sc.tris, r, h);
         const uint32_t id = pixel * a.spp + s;
