@@ -267,8 +267,10 @@ int rt_trace_camera(const rt_scene* scene, const rt_camera* cam, uint32_t width,
 
 /* Full render in one kernel: per 8x8 pixel tile, ray generation, closest-hit trace, FP64 hit
  * refinement, Lambert with interpolated normals, one shadow ray toward the light per lit hit,
- * RGBA8 texture fetch, RGB store. `workspace_dev` needs rt_render_workspace_bytes(width*height)
- * bytes (currently 0: NULL is accepted). With the QUAD camera-ray traversal (the default) the
+ * RGBA8 texture fetch, RGB store. `workspace_dev` takes rt_render_workspace_bytes(width*height)
+ * bytes (32 KiB): with the QUAD traversal a render that returns counts keeps per-slot partial
+ * counts there; NULL (or fewer bytes) is accepted, the counts are then summed per 4-wave workgroup
+ * (slower). Two calls in flight must not share a workspace. With the QUAD camera-ray traversal (the default) the
  * primary and the shadow rays both take the quad-lane traversal (16-ray packets over 4x4 pixels, 4
  * lanes per ray) and `tuning->shadow` is not used; with the others the primary rays take the
  * chosen traversal and the shadow rays `tuning->shadow`. */

@@ -374,7 +374,6 @@ class Raycaster:
         self.models = models
         self._stream = stream
         self._tuning = None if tuning is None else make_tuning(None, **tuning)
-        self._render_ws = None
         self._build_ads()
 
     # ------------------------------------------------------------------ helpers
@@ -711,11 +710,16 @@ class Raycaster:
         return ws
 
     def _render_workspace(self, n_pixels):
-        lib = N.load()
-        need = lib.rt_render_workspace_bytes(n_pixels)
-        if self._render_ws is None or self._render_ws.numel() < need:
-            self._render_ws = torch.empty(need, dtype=torch.uint8, device="cuda")
-        return self._render_ws
+        """rt_render's workspace on the current stream (the partial counts of a render that returns
+        counts). One per stream: calls on one stream run in order, calls on two streams must not
+        share one."""
+        need = N.load().rt_render_workspace_bytes(n_pixels)
+        pool = self.__dict__.setdefault("_render_ws_pool", {})
+        key = torch.cuda.current_stream().cuda_stream
+        ws = pool.get(key)
+        if ws is None or ws.numel() < need:
+            ws = pool[key] = torch.empty(need, dtype=torch.uint8, device="cuda")
+        return ws
 
     @property
     def n_meshes(self):

@@ -432,8 +432,12 @@ __global__ void __launch_bounds__(256) k_render(SceneArgs sc, rt_camera cam, rt_
 // Fused render with quad-lane packets for the primary and the shadow rays: one 8x8-pixel tile per
 // block, wave w = 4x4 quarter (w & 1, w >> 1), lanes 4q..4q+3 = pixel (q & 3, q >> 2).
 // W1: one wave per workgroup (the camera kernel's mapping: wave j of XCD x renders quarter j & 3 of
-// 8x8 block (j >> 2) * 8 + x, row-major), for frames without counts; else 4-wave workgroups whose
-// counts are summed in LDS (one wave per workgroup would take one global atomic per wave).
+// 8x8 block (j >> 2) * 8 + x, row-major); its counts, if any, go to partial slot blockIdx % 256 of
+// `stats` (kCountSlots slots kCountStride words apart, summed by k_sum_counts), so that no counter
+// takes more than one atomic in 256 of the frame's waves. Else 4-wave workgroups whose counts are
+// summed in LDS and added to `stats` itself once per workgroup (for a render without a workspace).
+constexpr uint32_t kCountSlots = 256, kCountStride = 32;  // 128 B per slot
+
 template <bool W1>
 __global__ void __launch_bounds__(W1 ? 64 : 256) k_render_quad(SceneArgs sc, rt_camera cam, rt_render_params p,
                                                       uint32_t tiles_x8, float* __restrict__ rgb,
@@ -456,7 +460,8 @@ __global__ void __launch_bounds__(W1 ? 64 : 256) k_render_quad(SceneArgs sc, rt_
     px.row = (blk / tiles_x8) * 8 + (wave >> 1) * 4 + (q >> 2);
     px.valid = px.col < p.width && px.row < p.height;
     if (W1) {
-        render_pixel<RT_TRAVERSAL_QUAD, RT_TRAVERSAL_QUAD>(sc, cam, p, px, rgb, out_t, out_prim, nullptr, nullptr,
+        uint32_t* slot = stats ? stats + (blockIdx.x % kCountSlots) * kCountStride : nullptr;
+        render_pixel<RT_TRAVERSAL_QUAD, RT_TRAVERSAL_QUAD>(sc, cam, p, px, rgb, out_t, out_prim, slot, nullptr,
                                                            nullptr, 0, 0, 0, true);
     } else {
         __shared__ uint32_t bstat[3];
@@ -889,6 +894,22 @@ __global__ void k_zero_u32(uint32_t* c, uint32_t n) {
     if (threadIdx.x < n) c[threadIdx.x] = 0;
 }
 
+__global__ void k_zero_words(uint32_t* w, uint32_t n) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) w[i] = 0;
+}
+
+// stats[0..2] = the sums of the kCountSlots partial counts (stats zeroed before)
+__global__ void __launch_bounds__(256) k_sum_counts(const uint32_t* __restrict__ part, uint32_t* __restrict__ stats) {
+    __shared__ uint32_t sum[3];
+    if (threadIdx.x < 3) sum[threadIdx.x] = 0;
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < kCountSlots; i += blockDim.x)
+        for (int c = 0; c < 3; ++c)
+            if (part[i * kCountStride + c]) atomicAdd(&sum[c], part[i * kCountStride + c]);
+    __syncthreads();
+    if (threadIdx.x < 3) stats[threadIdx.x] = sum[threadIdx.x];
+}
+
 int check_rays(const rt_rays* r, const char* what) {
     if (!r || !r->ox || !r->oy || !r->oz || !r->dx || !r->dy || !r->dz)
         return set_error(RT_EINVAL, std::string(what) + ": ray field pointer is NULL");
@@ -1017,14 +1038,12 @@ extern "C" int rt_trace_camera(const rt_scene* scene, const rt_camera* cam, uint
 
 extern "C" uint64_t rt_render_workspace_bytes(uint32_t n_pixels) {
     (void)n_pixels;
-    return 0;
+    return (uint64_t)kCountSlots * kCountStride * sizeof(uint32_t);
 }
 
 extern "C" int rt_render(const rt_scene* scene, const rt_camera* cam, const rt_render_params* p,
                          const rt_render_out* out, void* workspace, uint64_t ws_bytes,
                          const rt_tuning* tuning, void* stream) {
-    (void)workspace;
-    (void)ws_bytes;
     int e = check_scene(scene, "rt_render");
     if (e) return e;
     if (!cam || !p || !out) return set_error(RT_EINVAL, "rt_render: NULL argument");
@@ -1048,12 +1067,17 @@ extern "C" int rt_render(const rt_scene* scene, const rt_camera* cam, const rt_r
     // lane shadows and 1.45 ms for lane + lane, and at least as fast on smaller scenes
     if (tu.primary == RT_TRAVERSAL_QUAD) {
         const uint32_t tx8 = (p->width + 7) / 8, ty8 = (p->height + 7) / 8;
-        if (out->stats)
+        const bool partial = out->stats && workspace && ws_bytes >= rt_render_workspace_bytes((uint32_t)n);
+        if (out->stats && !partial) {
             hipLaunchKernelGGL(k_render_quad<false>, dim3(tx8 * ty8), dim3(256), 0, s, scene_args(scene), *cam, *p,
                                tx8, out->rgb, out->t, out->prim, out->stats);
-        else
+        } else {
+            uint32_t* part = partial ? (uint32_t*)workspace : nullptr;
+            if (part) hipLaunchKernelGGL(k_zero_words, dim3(8), dim3(256), 0, s, part, kCountSlots * kCountStride);
             hipLaunchKernelGGL(k_render_quad<true>, dim3(32u * ((tx8 * ty8 + 7u) / 8u)), dim3(64), 0, s,
-                               scene_args(scene), *cam, *p, tx8, out->rgb, out->t, out->prim, nullptr);
+                               scene_args(scene), *cam, *p, tx8, out->rgb, out->t, out->prim, part);
+            if (part) hipLaunchKernelGGL(k_sum_counts, dim3(1), dim3(256), 0, s, part, out->stats);
+        }
         return check_launch("rt_render");
     }
     const int pm = resolve_primary(tu.primary, *cam, p->width, p->height);

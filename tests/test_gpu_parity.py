@@ -175,10 +175,12 @@ def test_tuning_choices_give_identical_results(c1, twin, primary, shadow, lane_s
 
 
 @pytest.mark.parametrize("W,H", [(512, 512), (333, 257), (9, 17), (1, 1), (8, 72)])
-def test_render_with_and_without_counts(c1, W, H):
-    """rt_render's quad-lane kernel runs one wave per workgroup for a frame without counts and as
-    4-wave workgroups that sum the counts for a frame with them: both give the twin's image and hits
-    at square, odd and tiny frame sizes, and the camera kernel (one wave per workgroup) its hits."""
+def test_render_with_and_without_counts(c1, W, H, monkeypatch):
+    """rt_render's quad-lane kernel runs one wave per workgroup for a frame without counts and for a
+    frame with counts and a workspace (partial counts per slot, summed after), and as 4-wave
+    workgroups that sum the counts in LDS for a frame with counts and no workspace: all three give
+    the twin's image, hits and counts at square, odd and tiny frame sizes, and the camera kernel (one
+    wave per workgroup) its hits."""
     import rendertoy_amd as ren
     meshes, tw, rc = c1
     cam = camera(aspect=W / H)
@@ -186,15 +188,20 @@ def test_render_with_and_without_counts(c1, W, H):
     ref = tw.trace_camera(cam.as_array(), W, H)
     with ren.tuning(primary="quad"):
         img0, hits0 = rc.render(cam, W, H, light=LIGHT, shadows=True, bilinear=True, return_hits=True)
-        stats = torch.zeros(4, dtype=torch.int32, device="cuda")
+        stats1 = torch.zeros(4, dtype=torch.int32, device="cuda")
         img1, hits1 = rc.render(cam, W, H, light=LIGHT, shadows=True, bilinear=True, return_hits=True,
-                                stats=stats)
+                                stats=stats1)
         gc = {k: v.cpu().numpy() for k, v in rc.cast_camera(cam, W, H).items()}
-    for img, hits in ((img0, hits0), (img1, hits1)):
+        monkeypatch.setattr(rc, "_render_workspace", lambda n: torch.empty(0, dtype=torch.uint8, device="cuda"))
+        stats2 = torch.zeros(4, dtype=torch.int32, device="cuda")
+        img2, hits2 = rc.render(cam, W, H, light=LIGHT, shadows=True, bilinear=True, return_hits=True,
+                                stats=stats2)
+    for img, hits in ((img0, hits0), (img1, hits1), (img2, hits2)):
         np.testing.assert_array_equal(hits["prim"].cpu().numpy(), prim)
         np.testing.assert_array_equal(_u32(hits["t"].cpu().numpy()), _u32(np.where(prim >= 0, t, np.float32(np.inf))))
         np.testing.assert_array_equal(_u32(img.cpu().numpy()), _u32(rgb))
-    assert list(stats.cpu().numpy()[:3]) == list(st)
+    for stats in (stats1, stats2):
+        assert list(stats.cpu().numpy()) == [*st, 0]
     assert_trace_equal(gc, ref)
 
 
