@@ -269,7 +269,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_
     float4* qi = a.q[(k - 1) & 1];
     // parked leaves (RT_PARK_PATHS) take ~3 more VGPRs: the waves-per-EU bound keeps the kernel at
     // 128 (4 waves per SIMD) without scratch
-    refill_walks<false, RT_TRAVERSAL_LANE, RT_PARK_PATHS, true>(
+    refill_walks<false, RT_TRAVERSAL_LANE, RT_PARK_PATHS, true, false>(
         sc, n, gridDim.x, refill_slot(blockIdx.x, 0, 1, gridDim.x), stk + threadIdx.x, 64, lane_stack, ovf, pk,
         [&](uint32_t i, Ray& r) -> int32_t {
             const float4 e0 = qi[(size_t)i * 3], e1 = qi[(size_t)i * 3 + 1], e2 = qi[(size_t)i * 3 + 2];

@@ -109,7 +109,118 @@ RT_DEV uint32_t refill_slot(uint32_t block, uint32_t wave_in_block, uint32_t wav
     return ((block & 7u) * (n_blocks >> 3) + (block >> 3)) * waves_per_block + wave_in_block;
 }
 
-template <bool ANY, int MODE, int PARK, bool NF = false, typename Load, typename Finish>
+// ---- one ray walked by a whole wave (the last rays of a refill_walks sequence) ----
+// The 64 lanes walk the ray r (the same in every lane) together: each step takes up to 16 entries
+// from the top of a stack shared by the wave (entry e at at(e), at most `cap`), lane j tests child
+// j & 3 of entry j >> 2 against the wave's best t, the entered internal children are pushed
+// (ballot compaction) and each entered leaf is tested by its lane; the wave's closest candidate
+// (t, then the smaller primitive id) is reduced after each step. The entries are taken in stack
+// order, not nearest first: the closest hit does not depend on the visiting order, so the result
+// is the one-lane walk's. h: the ray's best candidate so far on entry (the same in every lane),
+// its closest hit on return (ANY: a hit). Returns false, h still a valid bound, if a push would
+// pass `cap` entries; the caller then finishes the ray another way.
+// busy lanes at the end of a sequence that the wave walks this way (measured on 2.07 M random rays
+// through the 1.31M-triangle scene: 4 for closest hits, 8 for any hits)
+constexpr uint32_t kCoopLanesClosest = 4, kCoopLanesAny = 8;
+
+template <bool ANY, typename At>
+RT_DEV bool wave_walk(const float4* __restrict__ nodes4, const float4* __restrict__ tris, int32_t root,
+                      const Ray& r, int32_t skip_slot, Hit& h, At at, uint32_t cap) {
+    const uint32_t lane = __lane_id(), c = lane & 3u, entry = lane >> 2;
+    if (root == RT_EMPTY_ROOT) return true;
+    // merges the lanes' candidates (each no worse than h, some equal to it) into h
+    auto reduce = [&](const Hit& cand) {
+        float t = cand.t;
+        int32_t p = cand.prim;
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) {
+            const float ot = __shfl_xor(t, off);
+            const int32_t op = __shfl_xor(p, off);
+            const bool take = op >= 0 && (p < 0 || ot < t || (ot == t && op < p));
+            t = take ? ot : t;
+            p = take ? op : p;
+        }
+        if (p < 0 || !(t < h.t || (t == h.t && p < h.prim))) return;
+        const uint64_t win = __ballot(cand.prim == p && cand.t == t);
+        const int wl = __ffsll((unsigned long long)win) - 1;
+        h.t = t;
+        h.prim = p;
+        h.u = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(cand.u), wl));
+        h.v = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(cand.v), wl));
+        h.slot = __builtin_amdgcn_readlane(cand.slot, wl);
+    };
+    // this lane's leaf, one triangle at a time (few registers: this runs inside the refill loop)
+    auto leaf_one = [&](int32_t ref, Hit& cand) {
+        const uint32_t first = leaf_first(ref), count = leaf_count(ref);
+        for (uint32_t k = 0; k < count; ++k) {
+            const int32_t slot = (int32_t)(first + k);
+            const float4* tr = tris + (size_t)slot * 3;
+            float tt, uu, vv;
+            const float4 ta = tr[0];
+            if (slot != skip_slot && intersect_tri(ta, tr[1], tr[2], r, tt, uu, vv) && tt > r.tmin) {
+                const int32_t prim = __float_as_int(ta.w);
+                if (tt < cand.t || (tt == cand.t && prim < cand.prim)) {
+                    cand.t = tt;
+                    cand.u = uu;
+                    cand.v = vv;
+                    cand.prim = prim;
+                    cand.slot = slot;
+                }
+            }
+        }
+    };
+    if (root < 0) {  // the whole tree is one leaf
+        Hit cand = h;
+        if (lane == 0) leaf_one(root, cand);
+        reduce(cand);
+        return true;
+    }
+    const float idx = safe_inv(r.dx), idy = safe_inv(r.dy), idz = safe_inv(r.dz);
+    const float oxx = r.ox * idx, oyy = r.oy * idy, ozz = r.oz * idz;
+    if (lane == 0) *at(0u) = root;
+    uint32_t sp = 1;
+    while (sp > 0) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        const uint32_t k = sp < 16u ? sp : 16u, base = sp - k;
+        const int32_t node = entry < k ? *at(base + entry) : RT_EMPTY_ROOT;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        sp = base;
+        bool push = false, leaf = false;
+        int32_t cref = RT_EMPTY_ROOT;
+        if (node != RT_EMPTY_ROOT) {
+            const float* nf = (const float*)(nodes4 + (size_t)node * 8);
+            const float clx = nf[c], chx = nf[4 + c], cly = nf[8 + c], chy = nf[12 + c], clz = nf[16 + c],
+                        chz = nf[20 + c];
+            cref = __float_as_int(nf[24 + c]);
+            const float x0 = fmaf(clx, idx, -oxx), x1 = fmaf(chx, idx, -oxx);
+            const float y0 = fmaf(cly, idy, -oyy), y1 = fmaf(chy, idy, -oyy);
+            const float z0 = fmaf(clz, idz, -ozz), z1 = fmaf(chz, idz, -ozz);
+            const float tn = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), r.tmin));
+            const float tf = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fmaxf(z0, z1)) * kTfarScale;
+            const bool hk = (cref != RT_EMPTY_ROOT) & (tn <= fminf(tf, h.t));
+            push = hk & (cref >= 0);
+            leaf = hk & (cref < 0);
+        }
+        const uint64_t pm = __ballot(push);
+        const uint32_t np = (uint32_t)__popcll(pm);
+        if (sp + np > cap) return false;
+        if (push) *at(sp + lane_rank(pm)) = cref;
+        sp += np;
+        if (__ballot(leaf)) {
+            Hit cand = h;
+            if (leaf) leaf_one(cref, cand);
+            reduce(cand);
+            if (ANY && h.prim >= 0) return true;
+        }
+    }
+    return true;
+}
+
+template <bool ANY, int MODE, int PARK, bool NF = false, bool COOP = true, typename Load, typename Finish>
 RT_DEV void refill_walks(const SceneArgs& sc, uint64_t n, uint32_t n_waves, uint32_t wid, int32_t* stk,
                          int stk_stride, int lane_stack, uint32_t* list, float4* pk, Load&& load,
                          Finish&& finish) {
@@ -171,6 +282,53 @@ RT_DEV void refill_walks(const SceneArgs& sc, uint64_t n, uint32_t n_waves, uint
             if (over) {  // refills stop while listed > kRefillList - 64: listed stays <= kRefillList
                 if (st == kWalkOverflow) list[listed + lane_rank(over)] = ri;
                 listed += (uint32_t)__popcll(over);
+            }
+            // The sequence's last rays: once nothing is left to take and at most kCoop lanes walk,
+            // the whole wave walks each of those rays in turn (wave_walk), on the wave's lane stacks
+            // (given up by their walks, restarted from the root with their best candidates). A ray
+            // whose shared stack would overflow goes to the packet retrace like an overflow. A
+            // sequence's slowest rays set the launch's floor: 32 K random rays took 1.13 ms (any
+            // hit 0.92) one lane each, 0.79 (0.38) this way.
+            constexpr uint32_t kCoop = ANY ? kCoopLanesAny : kCoopLanesClosest;
+            if (COOP && MODE == RT_TRAVERSAL_LANE && !more) {
+                uint64_t bm = __ballot(busy);
+                if (bm != 0 && (uint32_t)__popcll(bm) <= kCoop && listed + kCoop <= kRefillList) {
+                    int32_t* stk0 = stk - lane;  // lane 0's column of the wave's stacks
+                    auto at = [&](uint32_t e) -> int32_t* { return stk0 + (size_t)(e >> 6) * stk_stride + (e & 63u); };
+                    while (bm) {
+                        const int b = __ffsll((unsigned long long)bm) - 1;
+                        bm &= bm - 1;
+                        Ray rb;
+                        rb.ox = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(r.ox), b));
+                        rb.oy = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(r.oy), b));
+                        rb.oz = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(r.oz), b));
+                        rb.dx = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(r.dx), b));
+                        rb.dy = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(r.dy), b));
+                        rb.dz = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(r.dz), b));
+                        rb.tmin = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(r.tmin), b));
+                        rb.tmax = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(r.tmax), b));
+                        Hit hb;
+                        hb.t = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(h.t), b));
+                        hb.u = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(h.u), b));
+                        hb.v = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(h.v), b));
+                        hb.prim = __builtin_amdgcn_readlane(h.prim, b);
+                        hb.slot = __builtin_amdgcn_readlane(h.slot, b);
+                        const int32_t skb = __builtin_amdgcn_readlane(skip, b);
+                        const bool ok = wave_walk<ANY>(sc.nodes4, sc.tris, sc.root, rb, skb, hb, at,
+                                                       64u * (uint32_t)lane_stack);
+                        if (ok) {
+                            finish((int)lane == b, ri, r, hb);
+                        } else {
+                            if ((int)lane == b) list[listed] = ri;
+                            listed += 1u;
+                        }
+                        if ((int)lane == b) busy = false;
+                    }
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                    __builtin_amdgcn_wave_barrier();
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                    break;  // every lane is idle and nothing is left to take
+                }
             }
         }
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");  // the list's stores before its loads
