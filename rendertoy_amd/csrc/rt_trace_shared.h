@@ -116,16 +116,18 @@ RT_DEV uint32_t refill_slot(uint32_t block, uint32_t wave_in_block, uint32_t wav
 // (ballot compaction) and each entered leaf is tested by its lane; the wave's closest candidate
 // (t, then the smaller primitive id) is reduced after each step. The entries are taken in stack
 // order, not nearest first: the closest hit does not depend on the visiting order, so the result
-// is the one-lane walk's. h: the ray's best candidate so far on entry (the same in every lane),
-// its closest hit on return (ANY: a hit). Returns false, h still a valid bound, if a push would
-// pass `cap` entries; the caller then finishes the ray another way.
+// is the one-lane walk's. The stack starts with the root, or with `seeded` entries the caller
+// wrote (a one-lane walk's pending nodes and leaves; a leaf entry is tested by one lane). h: the
+// ray's best candidate so far on entry (the same in every lane), its closest hit on return (ANY:
+// a hit). Returns false, h still a valid bound, if a push would pass `cap` entries; the caller
+// then finishes the ray another way.
 // busy lanes at the end of a sequence that the wave walks this way (measured on 2.07 M random rays
 // through the 1.31M-triangle scene: 4 for closest hits, 8 for any hits)
 constexpr uint32_t kCoopLanesClosest = 4, kCoopLanesAny = 8;
 
 template <bool ANY, typename At>
 RT_DEV bool wave_walk(const float4* __restrict__ nodes4, const float4* __restrict__ tris, int32_t root,
-                      const Ray& r, int32_t skip_slot, Hit& h, At at, uint32_t cap) {
+                      const Ray& r, int32_t skip_slot, Hit& h, At at, uint32_t cap, uint32_t seeded = 0) {
     const uint32_t lane = __lane_id(), c = lane & 3u, entry = lane >> 2;
     if (root == RT_EMPTY_ROOT) return true;
     // merges the lanes' candidates (each no worse than h, some equal to it) into h
@@ -177,8 +179,11 @@ RT_DEV bool wave_walk(const float4* __restrict__ nodes4, const float4* __restric
     }
     const float idx = safe_inv(r.dx), idy = safe_inv(r.dy), idz = safe_inv(r.dz);
     const float oxx = r.ox * idx, oyy = r.oy * idy, ozz = r.oz * idz;
-    if (lane == 0) *at(0u) = root;
-    uint32_t sp = 1;
+    uint32_t sp = seeded;  // entries 0..seeded-1: the ray's pending nodes and leaves, pushed by the caller
+    if (!seeded) {
+        if (lane == 0) *at(0u) = root;
+        sp = 1;
+    }
     while (sp > 0) {
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
@@ -191,7 +196,10 @@ RT_DEV bool wave_walk(const float4* __restrict__ nodes4, const float4* __restric
         sp = base;
         bool push = false, leaf = false;
         int32_t cref = RT_EMPTY_ROOT;
-        if (node != RT_EMPTY_ROOT) {
+        if (node < 0 && node != RT_EMPTY_ROOT) {  // a pending leaf (from a seeded stack): lane c = 0 tests it
+            leaf = c == 0u;
+            cref = node;
+        } else if (node != RT_EMPTY_ROOT) {
             const float* nf = (const float*)(nodes4 + (size_t)node * 8);
             const float clx = nf[c], chx = nf[4 + c], cly = nf[8 + c], chy = nf[12 + c], clz = nf[16 + c],
                         chz = nf[20 + c];
@@ -284,20 +292,37 @@ RT_DEV void refill_walks(const SceneArgs& sc, uint64_t n, uint32_t n_waves, uint
                 listed += (uint32_t)__popcll(over);
             }
             // The sequence's last rays: once nothing is left to take and at most kCoop lanes walk,
-            // the whole wave walks each of those rays in turn (wave_walk), on the wave's lane stacks
-            // (given up by their walks, restarted from the root with their best candidates). A ray
-            // whose shared stack would overflow goes to the packet retrace like an overflow. A
-            // sequence's slowest rays set the launch's floor: 32 K random rays took 1.13 ms (any
-            // hit 0.92) one lane each, 0.79 (0.38) this way.
+            // the whole wave walks each of those rays in turn (wave_walk) on the wave's lane stacks,
+            // from where its one-lane walk stopped (its stack entries, current node and parked leaf,
+            // read from all the lanes before the first wave walk takes the stacks). A ray whose
+            // shared stack would overflow goes to the packet retrace like an overflow. A sequence's
+            // slowest rays set the launch's floor: 32 K random rays took 1.13 ms (any hit 0.92) one
+            // lane each, 0.67 (0.37) this way (0.79 / 0.38 restarted from the root).
             constexpr uint32_t kCoop = ANY ? kCoopLanesAny : kCoopLanesClosest;
             if (COOP && MODE == RT_TRAVERSAL_LANE && !more) {
                 uint64_t bm = __ballot(busy);
                 if (bm != 0 && (uint32_t)__popcll(bm) <= kCoop && listed + kCoop <= kRefillList) {
                     int32_t* stk0 = stk - lane;  // lane 0's column of the wave's stacks
                     auto at = [&](uint32_t e) -> int32_t* { return stk0 + (size_t)(e >> 6) * stk_stride + (e & 63u); };
-                    while (bm) {
-                        const int b = __ffsll((unsigned long long)bm) - 1;
-                        bm &= bm - 1;
+                    // the rays' pending work (stack entries, current node, parked leaf), read before
+                    // the first wave walk takes the stacks: entry j of ray k in lane j's pe[k]
+                    int32_t pe[kCoop], pcur[kCoop], pleaf[kCoop];
+                    uint32_t psp[kCoop];
+                    int pb[kCoop];
+#pragma unroll
+                    for (uint32_t k = 0; k < kCoop; ++k) {
+                        pb[k] = bm ? __ffsll((unsigned long long)bm) - 1 : -1;
+                        bm &= bm ? bm - 1 : 0ull;
+                        const int bk = pb[k] < 0 ? 0 : pb[k];
+                        psp[k] = pb[k] < 0 ? 0u : (uint32_t)__builtin_amdgcn_readlane(w.sp, bk);
+                        pcur[k] = pb[k] < 0 ? RT_EMPTY_ROOT : __builtin_amdgcn_readlane(w.cur, bk);
+                        pleaf[k] = pb[k] < 0 ? RT_EMPTY_ROOT : __builtin_amdgcn_readlane(w.leaf, bk);
+                        pe[k] = lane < psp[k] ? stk0[(size_t)lane * stk_stride + (uint32_t)bk] : RT_EMPTY_ROOT;
+                    }
+#pragma unroll
+                    for (uint32_t k = 0; k < kCoop; ++k) {
+                        const int b = pb[k];
+                        if (b < 0) break;
                         Ray rb;
                         rb.ox = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(r.ox), b));
                         rb.oy = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(r.oy), b));
@@ -314,8 +339,23 @@ RT_DEV void refill_walks(const SceneArgs& sc, uint64_t n, uint32_t n_waves, uint
                         hb.prim = __builtin_amdgcn_readlane(h.prim, b);
                         hb.slot = __builtin_amdgcn_readlane(h.slot, b);
                         const int32_t skb = __builtin_amdgcn_readlane(skip, b);
-                        const bool ok = wave_walk<ANY>(sc.nodes4, sc.tris, sc.root, rb, skb, hb, at,
-                                                       64u * (uint32_t)lane_stack);
+                        // the shared stack starts with the ray's pending work, so its walk goes on
+                        // where the lane's stopped (a ray with none left is done)
+                        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                        __builtin_amdgcn_wave_barrier();
+                        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                        if (lane < psp[k]) *at(lane) = pe[k];
+                        uint32_t seeded = psp[k];
+                        if (pcur[k] != RT_EMPTY_ROOT) {
+                            if (lane == 0) *at(seeded) = pcur[k];
+                            ++seeded;
+                        }
+                        if (pleaf[k] != RT_EMPTY_ROOT) {
+                            if (lane == 0) *at(seeded) = pleaf[k];
+                            ++seeded;
+                        }
+                        const bool ok = seeded == 0u || wave_walk<ANY>(sc.nodes4, sc.tris, sc.root, rb, skb, hb, at,
+                                                                       64u * (uint32_t)lane_stack, seeded);
                         if (ok) {
                             finish((int)lane == b, ri, r, hb);
                         } else {
