@@ -121,9 +121,10 @@ RT_DEV uint32_t refill_slot(uint32_t block, uint32_t wave_in_block, uint32_t wav
 // ray's best candidate so far on entry (the same in every lane), its closest hit on return (ANY:
 // a hit). Returns false, h still a valid bound, if a push would pass `cap` entries; the caller
 // then finishes the ray another way.
-// busy lanes at the end of a sequence that the wave walks this way (measured on 2.07 M random rays
-// through the 1.31M-triangle scene: 4 for closest hits, 8 for any hits)
-constexpr uint32_t kCoopLanesClosest = 4, kCoopLanesAny = 8;
+// busy lanes at the end of a sequence that the wave walks this way (measured on 2.07 M camera-order
+// and random rays through the 1.31M-triangle scene: 8 for both; 16 takes 168-202 VGPRs and a wave
+// or two per SIMD)
+constexpr uint32_t kCoopLanesClosest = 8, kCoopLanesAny = 8;
 
 template <bool ANY, typename At>
 RT_DEV bool wave_walk(const float4* __restrict__ nodes4, const float4* __restrict__ tris, int32_t root,
@@ -297,7 +298,8 @@ RT_DEV void refill_walks(const SceneArgs& sc, uint64_t n, uint32_t n_waves, uint
             // read from all the lanes before the first wave walk takes the stacks). A ray whose
             // shared stack would overflow goes to the packet retrace like an overflow. A sequence's
             // slowest rays set the launch's floor: 32 K random rays took 1.13 ms (any hit 0.92) one
-            // lane each, 0.67 (0.37) this way (0.79 / 0.38 restarted from the root).
+            // lane each, 0.61 (0.38) this way (0.79 / 0.38 restarted from the root, 0.67 closest
+            // with the wave walking a sequence's last 4 rays instead of 8).
             constexpr uint32_t kCoop = ANY ? kCoopLanesAny : kCoopLanesClosest;
             if (COOP && MODE == RT_TRAVERSAL_LANE && !more) {
                 uint64_t bm = __ballot(busy);
